@@ -1,0 +1,260 @@
+"""ORACLE (test infrastructure only).
+
+ctypes front end of the CPU restatement (liboracle.so, built by
+oracle/Makefile) and of the reference's own BVH compiled from its sources
+(oracle/_ref/libsprout_ref.so, built by oracle/ref.mk when the reference is
+present).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg may import this module; the product never does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+RCP_PINNED = 0
+RCP_NATIVE = 1
+
+NODE_DTYPE = np.dtype(
+    [
+        ("lo", "<f4", 3),
+        ("data", "<u4"),
+        ("hi", "<f4", 3),
+        ("axis", "u1"),
+        ("num", "u1"),
+        ("pad", "u1", 2),
+    ]
+)
+assert NODE_DTYPE.itemsize == 32
+
+
+class _OracleBvh(ctypes.Structure):
+    _fields_ = [
+        ("nodes", ctypes.c_void_p),
+        ("slots", ctypes.c_void_p),
+        ("num_nodes", ctypes.c_uint32),
+        ("num_slots", ctypes.c_uint32),
+        ("spatial_split_threshold", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def build_oracle() -> str:
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile")], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build_oracle()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.oracle_sbvh_build.restype = ctypes.c_int32
+        _lib.oracle_sbvh_build.argtypes = [
+            ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+            ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_OracleBvh),
+        ]
+        _lib.oracle_bvh_free.argtypes = [ctypes.POINTER(_OracleBvh)]
+        _lib.oracle_trace.restype = ctypes.c_int32
+        _lib.oracle_trace.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_uint32,
+        ]
+        _lib.oracle_brute_force.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
+            ctypes.c_int32, ctypes.c_void_p,
+        ]
+        _lib.oracle_reciprocal.restype = ctypes.c_float
+        _lib.oracle_reciprocal.argtypes = [ctypes.c_float, ctypes.c_int32]
+        _lib.oracle_reciprocal3.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
+        _lib.oracle_offset_ray.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.oracle_surface_area.restype = ctypes.c_float
+        _lib.oracle_surface_area.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class Tree:
+    """An SBVH in sprout's serialized layout: 32-byte nodes + slot -> input triangle id."""
+
+    def __init__(self, nodes: np.ndarray, slots: np.ndarray, spatial_split_threshold: int = 0):
+        self.nodes = nodes
+        self.slots = slots
+        self.spatial_split_threshold = spatial_split_threshold
+
+    @property
+    def num_nodes(self) -> int:
+        return len(self.nodes)
+
+    def depth(self) -> int:
+        """Maximum node depth (root = 0)."""
+        nodes = self.nodes
+        depth = np.zeros(len(nodes), np.int64)
+        inner = np.nonzero(nodes["num"] == 0)[0]
+        # children always follow their parent in the depth-first layout
+        for i in inner:
+            c = nodes["data"][i]
+            depth[c] = depth[c + 1] = depth[i] + 1
+        return int(depth.max())
+
+
+def build(positions: np.ndarray, indices: np.ndarray, num_slices=16, sweep_threshold=64,
+          max_primitives=4) -> Tree:
+    positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+    out = _OracleBvh()
+    st = lib().oracle_sbvh_build(_ptr(positions), len(positions), _ptr(indices), len(indices),
+                                 num_slices, sweep_threshold, max_primitives, ctypes.byref(out))
+    if st != 0:
+        raise RuntimeError(f"oracle_sbvh_build failed: {st}")
+    nodes = np.ctypeslib.as_array(
+        ctypes.cast(out.nodes, ctypes.POINTER(ctypes.c_uint8)), (out.num_nodes * 32,)
+    ).view(NODE_DTYPE).copy()
+    slots = np.ctypeslib.as_array(
+        ctypes.cast(out.slots, ctypes.POINTER(ctypes.c_uint32)), (out.num_slots,)
+    ).copy()
+    sst = out.spatial_split_threshold
+    lib().oracle_bvh_free(ctypes.byref(out))
+    return Tree(nodes, slots, sst)
+
+
+def trace(tree: Tree, positions, indices, rays, any_hit=False, rcp_mode=RCP_PINNED,
+          counts=False, threads=None):
+    """Closest hit (hits: (n,4) uint32 words t,u,v,id) or any hit (n,) uint32."""
+    positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = len(rays)
+    hits = np.zeros((n,) if any_hit else (n, 4), np.uint32)
+    cnt = np.zeros((n, 2), np.uint32) if counts else None
+    if threads is None:
+        threads = os.cpu_count() or 1
+    st = lib().oracle_trace(_ptr(tree.nodes), _ptr(tree.slots), _ptr(positions), _ptr(indices),
+                            _ptr(rays), n, int(any_hit), rcp_mode, _ptr(hits),
+                            _ptr(cnt) if counts else None, threads)
+    if st != 0:
+        raise RuntimeError(f"oracle_trace failed: {st}")
+    return (hits, cnt) if counts else hits
+
+
+def brute_force(positions, indices, rays, rcp_mode=RCP_PINNED):
+    positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    hits = np.zeros((len(rays), 4), np.uint32)
+    lib().oracle_brute_force(_ptr(positions), _ptr(indices), len(indices), _ptr(rays), len(rays),
+                             rcp_mode, _ptr(hits))
+    return hits
+
+
+def reciprocal(x: float, rcp_mode=RCP_PINNED) -> float:
+    return lib().oracle_reciprocal(float(x), rcp_mode)
+
+
+def reciprocal3(d, rcp_mode=RCP_PINNED) -> np.ndarray:
+    d = np.ascontiguousarray(d, np.float32).reshape(3)
+    out = np.zeros(3, np.float32)
+    lib().oracle_reciprocal3(_ptr(d), rcp_mode, _ptr(out))
+    return out
+
+
+def offset_ray(p, n) -> np.ndarray:
+    p = np.ascontiguousarray(p, np.float32).reshape(3)
+    n = np.ascontiguousarray(n, np.float32).reshape(3)
+    out = np.zeros(3, np.float32)
+    lib().oracle_offset_ray(_ptr(p), _ptr(n), _ptr(out))
+    return out
+
+
+def surface_area(lo, hi) -> float:
+    lo = np.ascontiguousarray(lo, np.float32).reshape(3)
+    hi = np.ascontiguousarray(hi, np.float32).reshape(3)
+    return lib().oracle_surface_area(_ptr(lo), _ptr(hi))
+
+
+# ---------------------------------------------------------------------------
+# The reference's own BVH, compiled from its sources (oracle/ref.mk).
+
+
+def ref_lib_path(arch: str = "native") -> str:
+    name = "libsprout_ref.so" if arch == "native" else f"libsprout_ref_{arch.replace('-', '_')}.so"
+    return os.path.join(REF_DIR, name)
+
+
+class RefTree:
+    """sprout's own Builder_SAH + bvh::Tree, loaded from oracle/_ref."""
+
+    def __init__(self, positions, indices, num_slices=16, sweep_threshold=64, max_primitives=4,
+                 threads=None, arch="native"):
+        path = ref_lib_path(arch)
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self._lib = l = ctypes.CDLL(path)
+        l.sref_build.restype = ctypes.c_void_p
+        l.sref_build.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        l.sref_free.argtypes = [ctypes.c_void_p]
+        l.sref_build_seconds.restype = ctypes.c_double
+        l.sref_build_seconds.argtypes = [ctypes.c_void_p]
+        l.sref_num_nodes.restype = ctypes.c_uint32
+        l.sref_num_nodes.argtypes = [ctypes.c_void_p]
+        l.sref_num_slots.restype = ctypes.c_uint32
+        l.sref_num_slots.argtypes = [ctypes.c_void_p]
+        l.sref_nodes.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        l.sref_slots.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        l.sref_trace.restype = ctypes.c_double
+        l.sref_trace.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32,
+                                 ctypes.c_void_p, ctypes.c_uint32]
+        self.positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        self.threads = threads or os.cpu_count() or 1
+        self._h = l.sref_build(_ptr(self.positions), len(self.positions), _ptr(self.indices),
+                               len(self.indices), num_slices, sweep_threshold, max_primitives,
+                               self.threads)
+        self.build_seconds = l.sref_build_seconds(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.sref_free(self._h)
+            self._h = None
+
+    def tree(self) -> Tree:
+        nodes = np.zeros(self._lib.sref_num_nodes(self._h), NODE_DTYPE)
+        self._lib.sref_nodes(self._h, _ptr(nodes))
+        slots = np.zeros(self._lib.sref_num_slots(self._h), np.uint32)
+        self._lib.sref_slots(self._h, _ptr(slots))
+        return Tree(nodes, slots)
+
+    def trace(self, rays, any_hit=False, threads=None):
+        """Returns (hits, seconds) in the same layout as oracle.trace."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        hits = np.zeros((n,) if any_hit else (n, 4), np.uint32)
+        secs = self._lib.sref_trace(self._h, _ptr(rays), n, int(any_hit), _ptr(hits),
+                                    threads or self.threads)
+        return hits, secs
+
+
+def nodes_equal(a: np.ndarray, b: np.ndarray) -> bool:
+    """Node arrays equal; bounds compared as float values (so -0 == +0)."""
+    if a.shape != b.shape:
+        return False
+    return bool(
+        np.array_equal(a["lo"], b["lo"]) and np.array_equal(a["hi"], b["hi"])
+        and np.array_equal(a["data"], b["data"]) and np.array_equal(a["num"], b["num"])
+        and np.array_equal(np.where(a["num"] == 0, a["axis"], 0), np.where(b["num"] == 0, b["axis"], 0))
+    )

@@ -1,0 +1,44 @@
+# Builds the reference's own triangle SBVH builder and traversal, from the
+# reference sources where they lie, into oracle/_ref/ (never committed).
+# Flags follow the reference's CMake configuration for a Release build on
+# Linux (source/CMakeLists.txt: C++20, -O3, -march=native, no RTTI, no
+# exceptions, SU_ACESCG / SU_IBL_MIS_COMPENSATION / SU_ZSTD definitions).
+#
+#   make -f oracle/ref.mk                 # native build (golden vectors)
+#   make -f oracle/ref.mk ARCH=x86-64-v3  # portable build (CPU baseline on other hosts)
+#
+# Only this container has the reference; the built .so travels with the tree.
+
+REF   ?= /root/reference
+ARCH  ?= native
+HERE  := $(dir $(lastword $(MAKEFILE_LIST)))
+OUT   := $(HERE)_ref
+SRC   := $(REF)/source
+
+ifeq ($(ARCH),native)
+LIB   := $(OUT)/libsprout_ref.so
+else
+LIB   := $(OUT)/libsprout_ref_$(subst -,_,$(ARCH)).so
+endif
+
+CXXFLAGS := -std=c++20 -O3 -DNDEBUG -march=$(ARCH) -fno-rtti -fno-exceptions -fPIC \
+            -DSU_ACESCG -DSU_IBL_MIS_COMPENSATION -DSU_ZSTD -w \
+            -I$(SRC)/core -I$(SRC) -I$(SRC)/base -I$(REF)/thirdparty/include
+
+REF_SOURCES := \
+  $(SRC)/core/scene/bvh/scene_bvh_builder_base.cpp \
+  $(SRC)/core/scene/shape/triangle/bvh/triangle_bvh_builder_sah.cpp \
+  $(SRC)/core/scene/shape/triangle/bvh/triangle_bvh_helper.cpp \
+  $(SRC)/core/scene/shape/triangle/bvh/triangle_bvh_tree.cpp \
+  $(SRC)/core/scene/shape/shape_vertex.cpp \
+  $(SRC)/base/thread/thread_pool.cpp \
+  $(SRC)/core/logging/logging.cpp \
+  $(SRC)/core/logging/log.cpp
+
+all: $(LIB)
+
+$(LIB): $(REF_SOURCES) $(HERE)ref_driver.cpp
+	@mkdir -p $(OUT)
+	g++ $(CXXFLAGS) -shared -o $@ $(HERE)ref_driver.cpp $(REF_SOURCES) -lpthread
+
+.PHONY: all

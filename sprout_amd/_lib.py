@@ -1,0 +1,110 @@
+"""ctypes binding of libsprout_gpu.so (the C ABI declared in include/sprout_gpu.h).
+
+Loading fails loudly when the library has not been built: there is no
+fallback path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsprout_gpu.so")
+
+MI_NULL = 0xFFFFFFFF
+
+STATUS = {
+    0: "MI_OK",
+    -1: "MI_ERR_NO_CONTEXT",
+    -2: "MI_ERR_ARGS",
+    -3: "MI_ERR_HIP",
+    -4: "MI_ERR_HANDLE",
+    -5: "MI_ERR_UNSPLITTABLE",
+    -6: "MI_ERR_STACK",
+}
+
+
+class MiError(RuntimeError):
+    def __init__(self, status: int, where: str, message: str):
+        self.status = status
+        super().__init__(f"{where}: {STATUS.get(status, status)}: {message}")
+
+
+class MiRay(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("min_t", ctypes.c_float),
+                ("direction", ctypes.c_float * 3), ("max_t", ctypes.c_float)]
+
+
+class MiHit(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_float), ("u", ctypes.c_float), ("v", ctypes.c_float),
+                ("triangle", ctypes.c_uint32)]
+
+
+class MiBvhSettings(ctypes.Structure):
+    _fields_ = [("num_slices", ctypes.c_uint32), ("sweep_threshold", ctypes.c_uint32),
+                ("max_primitives", ctypes.c_uint32)]
+
+
+class MiBvhInfo(ctypes.Structure):
+    _fields_ = [("num_nodes", ctypes.c_uint32), ("num_slots", ctypes.c_uint32),
+                ("num_triangles", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("aabb_min", ctypes.c_float * 3), ("aabb_max", ctypes.c_float * 3),
+                ("build_ms", ctypes.c_double)]
+
+
+class MiTraceConfig(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32)]
+
+
+_u32 = ctypes.c_uint32
+_i32 = ctypes.c_int32
+_u64 = ctypes.c_uint64
+_f32 = ctypes.c_float
+_vp = ctypes.c_void_p
+_pu32 = ctypes.POINTER(ctypes.c_uint32)
+
+# name -> (restype, argtypes); every function of include/sprout_gpu.h
+SIGNATURES = {
+    "mi_init": (_i32, [_i32]),
+    "mi_release": (_i32, []),
+    "mi_last_error": (ctypes.c_char_p, []),
+    "mi_bvh_build": (_i32, [_vp, _u32, _u32, _vp, _u32, ctypes.POINTER(MiBvhSettings), _vp, _pu32]),
+    "mi_bvh_upload": (_i32, [_vp, _u32, _vp, _u32, _vp, _u32, _u32, _vp, _u32, _pu32]),
+    "mi_bvh_release": (_i32, [_u32]),
+    "mi_bvh_get_info": (_i32, [_u32, ctypes.POINTER(MiBvhInfo)]),
+    "mi_bvh_download": (_i32, [_u32, _vp, _vp]),
+    "mi_trace_closest": (_i32, [_u32, _vp, _u32, _vp, _vp]),
+    "mi_trace_any": (_i32, [_u32, _vp, _u32, _vp, _vp]),
+    "mi_trace_counts": (_i32, [_u32, _vp, _u32, _i32, _vp, _vp, _vp]),
+    "mi_set_trace_config": (_i32, [ctypes.POINTER(MiTraceConfig)]),
+    "mi_sync": (_i32, [_vp]),
+    "mi_gen_displaced_icosphere": (_i32, [_u32, _u32, _f32, _u64, _u32, _f32, _f32, _u64, _vp,
+                                          _pu32, _vp, _pu32]),
+    "mi_gen_camera_rays": (_i32, [_u32, _u32, _vp, _vp, _vp, _f32, _vp]),
+    "mi_gen_bounce_rays": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp, _u32]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m sprout_amd.build` "
+                "(there is no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(status: int, where: str) -> None:
+    if status != 0:
+        msg = load().mi_last_error()
+        raise MiError(status, where, msg.decode() if msg else "")

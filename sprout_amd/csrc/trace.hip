@@ -1,0 +1,358 @@
+// Closest-hit and any-hit traversal of sprout's binary SBVH on gfx950.
+//
+// Replaces scene::shape::triangle::bvh::Tree::intersect / intersect_p
+// (core/scene/shape/triangle/bvh/triangle_bvh_tree.cpp:41-90,145-185).
+//
+// Tree layout in HBM (built by bvh_build.hip or uploaded):
+//   nodes: sprout's 32-byte scene::bvh::Node, two float4 per node
+//          {min.xyz, children_or_data}{max.xyz, axis | num_indices << 8}.
+//          The array starts 32 bytes into a 64-byte aligned allocation, so
+//          every sibling pair (children a, a + 1 with a odd) is one aligned
+//          64-byte line segment.
+//   tris:  48 bytes per tree slot {a.xyz, input triangle id}{b - a}{c - a}.
+//          The edge vectors are the same rounded differences the reference
+//          computes per test, so the intersection arithmetic is unchanged.
+//
+// Two loops, both bit-identical to the reference in result:
+//   * trace_pairs: one thread per ray over a persistent grid.  At an inner
+//     node both children are tested from one 64-byte fetch; the far child
+//     (by the sign of inv_dir on the node's axis, as the reference) is pushed
+//     only when its box is hit, and re-tested with the current max_t when
+//     popped.  The box test only gets stricter as max_t shrinks, so the
+//     triangles are tested in exactly the reference's order.
+//   * trace_reference: the reference's loop verbatim (fetch, test, push far,
+//     descend near), counting node fetches and triangle tests per ray; this
+//     gives the algorithmic bytes per ray.
+// The traversal stack lives in LDS (Lds_entries per lane, laid out
+// entry-major so a wave's 64 lanes hit 64 different banks) and spills to a
+// per-thread global area beyond that, sized from the tree's depth.
+
+#include "common.hpp"
+#include "device_math.hpp"
+#include "trace.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace mi {
+
+namespace {
+
+using dev::F3;
+using dev::Null;
+
+constexpr uint32_t Lds_entries = 32;
+
+struct Stack {
+    uint32_t* lds;        // Lds_entries * block_size words, entry-major
+    uint32_t* spill;      // spill_entries * total_threads words, entry-major
+    uint32_t  lane;       // threadIdx.x
+    uint32_t  block_size;
+    uint32_t  gid;
+    uint32_t  total_threads;
+    uint32_t  sp;
+
+    __device__ void push(uint32_t v) {
+        if (sp < Lds_entries) {
+            lds[sp * block_size + lane] = v;
+        } else {
+            spill[size_t(sp - Lds_entries) * total_threads + gid] = v;
+        }
+        ++sp;
+    }
+
+    __device__ uint32_t pop() {
+        --sp;
+        if (sp < Lds_entries) {
+            return lds[sp * block_size + lane];
+        }
+        return spill[size_t(sp - Lds_entries) * total_threads + gid];
+    }
+};
+
+struct Ray {
+    F3    o;
+    F3    d;
+    F3    inv;
+    float min_t;
+    float max_t;
+    uint32_t sign_bits;  // bit a set when inv_dir[a] < 0 (base/math/simd.inl:230-236)
+};
+
+__device__ __forceinline__ Ray load_ray(mi_ray const* rays, uint32_t r) {
+    float4 const* p  = reinterpret_cast<float4 const*>(rays) + 2 * size_t(r);
+    float4 const  ro = p[0];
+    float4 const  rd = p[1];
+
+    Ray ray;
+    ray.o     = {ro.x, ro.y, ro.z};
+    ray.min_t = ro.w;
+    ray.d     = {rd.x, rd.y, rd.z};
+    ray.max_t = rd.w;
+    ray.inv   = {dev::reciprocal_lane(rd.x), dev::reciprocal_lane(rd.y), dev::reciprocal_lane(rd.z)};
+    ray.sign_bits = (ray.inv.x < 0.f ? 1u : 0u) | (ray.inv.y < 0.f ? 2u : 0u) |
+                    (ray.inv.z < 0.f ? 4u : 0u);
+    return ray;
+}
+
+__device__ __forceinline__ uint32_t node_num(uint32_t meta) {
+    return (meta >> 8) & 0xFFu;
+}
+
+__device__ __forceinline__ uint32_t node_axis(uint32_t meta) {
+    return meta & 0xFFu;
+}
+
+// Tests the triangles of a leaf in slot order.  Closest: every accepted hit
+// shrinks max_t (ties accepted, so the later equal-t triangle wins).
+// Any: returns true on the first hit.
+template <bool Any>
+__device__ __forceinline__ bool leaf(float4 const* __restrict__ tris, uint32_t start, uint32_t num,
+                                     Ray& ray, uint32_t& index, float& u, float& v,
+                                     uint32_t& tests) {
+    for (uint32_t i = start, end = start + num; i < end; ++i) {
+        ++tests;
+
+        float4 const ta = tris[3 * size_t(i) + 0];
+        float4 const t1 = tris[3 * size_t(i) + 1];
+        float4 const t2 = tris[3 * size_t(i) + 2];
+
+        float t, tu, tv;
+        if (dev::triangle({ta.x, ta.y, ta.z}, {t1.x, t1.y, t1.z}, {t2.x, t2.y, t2.z}, ray.o, ray.d,
+                          ray.min_t, ray.max_t, t, tu, tv)) {
+            if (Any) {
+                return true;
+            }
+            ray.max_t = t;
+            u         = tu;
+            v         = tv;
+            index     = __float_as_uint(ta.w);
+        }
+    }
+    return false;
+}
+
+template <bool Any>
+__global__ void trace_pairs(float4 const* __restrict__ nodes, float4 const* __restrict__ tris,
+                            mi_ray const* __restrict__ rays, uint32_t num_rays, void* out,
+                            uint32_t* spill, uint32_t total_threads) {
+    extern __shared__ uint32_t lds_stack[];
+
+    Stack stack;
+    stack.lds           = lds_stack;
+    stack.spill         = spill;
+    stack.lane          = threadIdx.x;
+    stack.block_size    = blockDim.x;
+    stack.gid           = blockIdx.x * blockDim.x + threadIdx.x;
+    stack.total_threads = total_threads;
+
+    for (uint32_t r = stack.gid; r < num_rays; r += total_threads) {
+        Ray ray = load_ray(rays, r);
+
+        uint32_t index = Null;
+        float    u     = 0.f;
+        float    v     = 0.f;
+        bool     hit_p = false;
+
+        stack.sp = 0;
+
+        float4 const rlo = nodes[0];
+        float4 const rhi = nodes[1];
+
+        uint32_t data = __float_as_uint(rlo.w);
+        uint32_t meta = __float_as_uint(rhi.w);
+
+        bool live = dev::slab(rlo, rhi, ray.o, ray.inv, ray.min_t, ray.max_t);
+
+        while (live) {
+            uint32_t const num = node_num(meta);
+
+            if (0 == num) {
+                uint32_t const a = data;
+
+                float4 const alo = nodes[2 * size_t(a) + 0];
+                float4 const ahi = nodes[2 * size_t(a) + 1];
+                float4 const blo = nodes[2 * size_t(a) + 2];
+                float4 const bhi = nodes[2 * size_t(a) + 3];
+
+                bool const ha = dev::slab(alo, ahi, ray.o, ray.inv, ray.min_t, ray.max_t);
+                bool const hb = dev::slab(blo, bhi, ray.o, ray.inv, ray.min_t, ray.max_t);
+
+                bool const flip = (ray.sign_bits >> node_axis(meta)) & 1u;
+
+                bool const hn = flip ? hb : ha;
+                bool const hf = flip ? ha : hb;
+
+                if (hn | hf) {
+                    if (hn & hf) {
+                        stack.push(flip ? a : a + 1);
+                    }
+                    bool const take_b = hn ? flip : !flip;
+                    data = __float_as_uint(take_b ? blo.w : alo.w);
+                    meta = __float_as_uint(take_b ? bhi.w : ahi.w);
+                    continue;
+                }
+            } else {
+                uint32_t tests = 0;
+                if (leaf<Any>(tris, data, num, ray, index, u, v, tests)) {
+                    hit_p = true;
+                    break;
+                }
+            }
+
+            // pop until a pending far child still passes the box test
+            live = false;
+            while (stack.sp > 0) {
+                uint32_t const m   = stack.pop();
+                float4 const   mlo = nodes[2 * size_t(m) + 0];
+                float4 const   mhi = nodes[2 * size_t(m) + 1];
+                if (dev::slab(mlo, mhi, ray.o, ray.inv, ray.min_t, ray.max_t)) {
+                    data = __float_as_uint(mlo.w);
+                    meta = __float_as_uint(mhi.w);
+                    live = true;
+                    break;
+                }
+            }
+        }
+
+        if (Any) {
+            static_cast<uint32_t*>(out)[r] = hit_p ? 1u : 0u;
+        } else {
+            float4 h;
+            h.x = ray.max_t;
+            h.y = Null != index ? u : 0.f;
+            h.z = Null != index ? v : 0.f;
+            h.w = __uint_as_float(index);
+            reinterpret_cast<float4*>(out)[r] = h;
+        }
+    }
+}
+
+template <bool Any>
+__global__ void trace_reference(float4 const* __restrict__ nodes, float4 const* __restrict__ tris,
+                                mi_ray const* __restrict__ rays, uint32_t num_rays, void* out,
+                                uint32_t* counts, uint32_t* spill, uint32_t total_threads) {
+    extern __shared__ uint32_t lds_stack[];
+
+    Stack stack;
+    stack.lds           = lds_stack;
+    stack.spill         = spill;
+    stack.lane          = threadIdx.x;
+    stack.block_size    = blockDim.x;
+    stack.gid           = blockIdx.x * blockDim.x + threadIdx.x;
+    stack.total_threads = total_threads;
+
+    for (uint32_t r = stack.gid; r < num_rays; r += total_threads) {
+        Ray ray = load_ray(rays, r);
+
+        uint32_t index = Null;
+        float    u     = 0.f;
+        float    v     = 0.f;
+        bool     hit_p = false;
+
+        uint32_t num_nodes = 0;
+        uint32_t num_tris  = 0;
+
+        stack.sp = 0;
+        stack.push(Null);
+
+        uint32_t n = 0;
+        while (Null != n) {
+            float4 const lo = nodes[2 * size_t(n) + 0];
+            float4 const hi = nodes[2 * size_t(n) + 1];
+            ++num_nodes;
+
+            if (dev::slab(lo, hi, ray.o, ray.inv, ray.min_t, ray.max_t)) {
+                uint32_t const data = __float_as_uint(lo.w);
+                uint32_t const meta = __float_as_uint(hi.w);
+                uint32_t const num  = node_num(meta);
+
+                if (0 == num) {
+                    if (0 == ((ray.sign_bits >> node_axis(meta)) & 1u)) {
+                        stack.push(data + 1);
+                        n = data;
+                    } else {
+                        stack.push(data);
+                        n = data + 1;
+                    }
+                    continue;
+                }
+
+                // intersect_p stops at the first hit: only the tests made count
+                if (leaf<Any>(tris, data, num, ray, index, u, v, num_tris)) {
+                    hit_p = true;
+                    break;
+                }
+            }
+
+            n = stack.pop();
+        }
+
+        if (Any) {
+            static_cast<uint32_t*>(out)[r] = hit_p ? 1u : 0u;
+        } else {
+            float4 h;
+            h.x = ray.max_t;
+            h.y = Null != index ? u : 0.f;
+            h.z = Null != index ? v : 0.f;
+            h.w = __uint_as_float(index);
+            reinterpret_cast<float4*>(out)[r] = h;
+        }
+
+        counts[2 * size_t(r) + 0] = num_nodes;
+        counts[2 * size_t(r) + 1] = num_tris;
+    }
+}
+
+}  // namespace
+
+int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays,
+                     uint32_t num_rays, void* out, uint32_t* counts, bool any_hit,
+                     hipStream_t stream) {
+    if (0 == num_rays) {
+        return MI_OK;
+    }
+
+    uint32_t const entries = stack_entries(bvh.max_depth);
+    if (entries > Reference_stack_size) {
+        return fail(MI_ERR_STACK,
+                    "tree depth %u needs %u stack entries, more than the reference's %u",
+                    bvh.max_depth, entries, Reference_stack_size);
+    }
+
+    uint32_t const block   = res.block_size;
+    uint32_t const blocks  = res.grid_blocks(num_rays);
+    uint32_t const threads = blocks * block;
+
+    uint32_t const spill_entries = entries > Lds_entries ? entries - Lds_entries : 0;
+    if (int32_t const st = res.reserve_spill(size_t(spill_entries) * threads); st) {
+        return st;
+    }
+
+    size_t const lds_bytes = size_t(Lds_entries) * block * sizeof(uint32_t);
+
+    float4 const* nodes = bvh.nodes;
+    float4 const* tris  = bvh.tris;
+
+    if (counts) {
+        if (any_hit) {
+            hipLaunchKernelGGL(trace_reference<true>, dim3(blocks), dim3(block), lds_bytes, stream,
+                               nodes, tris, rays, num_rays, out, counts, res.spill, threads);
+        } else {
+            hipLaunchKernelGGL(trace_reference<false>, dim3(blocks), dim3(block), lds_bytes, stream,
+                               nodes, tris, rays, num_rays, out, counts, res.spill, threads);
+        }
+    } else if (any_hit) {
+        hipLaunchKernelGGL(trace_pairs<true>, dim3(blocks), dim3(block), lds_bytes, stream, nodes,
+                           tris, rays, num_rays, out, res.spill, threads);
+    } else {
+        hipLaunchKernelGGL(trace_pairs<false>, dim3(blocks), dim3(block), lds_bytes, stream, nodes,
+                           tris, rays, num_rays, out, res.spill, threads);
+    }
+
+    if (hipError_t const e = hipGetLastError(); e != hipSuccess) {
+        return fail(MI_ERR_HIP, "trace launch: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+}  // namespace mi

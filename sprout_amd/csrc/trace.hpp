@@ -1,0 +1,68 @@
+// Device-side tree and traversal launch interface (host code).
+#pragma once
+
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace mi {
+
+// A tree resident in HBM; see trace.hip for the layout.
+struct DeviceBvh {
+    void*     nodes_alloc = nullptr;  // 64-byte aligned; nodes start 32 bytes in
+    float4*   nodes       = nullptr;  // 2 float4 per node
+    float4*   tris        = nullptr;  // 3 float4 per tree slot
+    uint32_t* slots       = nullptr;  // input triangle id per tree slot
+
+    uint32_t num_nodes     = 0;
+    uint32_t num_slots     = 0;
+    uint32_t num_triangles = 0;
+    uint32_t max_depth     = 0;
+
+    float  aabb_min[3] = {0.f, 0.f, 0.f};
+    float  aabb_max[3] = {0.f, 0.f, 0.f};
+    double build_ms    = 0.0;
+
+    void release();
+};
+
+// Persistent-grid launch state shared by all traversals of a context.
+struct TraceResources {
+    uint32_t  block_size    = 256;
+    uint32_t  blocks_per_cu = 4;
+    uint32_t  num_cus       = 256;
+    uint32_t* spill         = nullptr;
+    size_t    spill_words   = 0;
+
+    uint32_t grid_blocks(uint32_t num_rays) const {
+        uint64_t const wanted   = (uint64_t(num_rays) + block_size - 1) / block_size;
+        uint64_t const resident = uint64_t(num_cus) * blocks_per_cu;
+        uint64_t const b        = wanted < resident ? wanted : resident;
+        return uint32_t(b > 0 ? b : 1);
+    }
+
+    int32_t reserve_spill(size_t words);
+
+    void release();
+};
+
+// Allocates the device tree from host arrays in sprout's serialized layout
+// (nodes: 32-byte scene::bvh::Node; slots: input triangle per tree slot) and
+// a host mesh; computes depth and bounds.
+int32_t upload_bvh(void const* nodes, uint32_t num_nodes, uint32_t const* slots,
+                   uint32_t num_slots, float const* positions, uint32_t num_vertices,
+                   uint32_t position_stride, uint32_t const* indices, uint32_t num_triangles,
+                   DeviceBvh& out);
+
+// Fills bvh.tris from device-resident slots and mesh.
+int32_t fill_triangles(DeviceBvh& bvh, float const* positions, uint32_t position_stride,
+                       uint32_t const* indices, hipStream_t stream);
+
+int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays,
+                     uint32_t num_rays, void* out, uint32_t* counts, bool any_hit,
+                     hipStream_t stream);
+
+}  // namespace mi

@@ -1,0 +1,63 @@
+"""Synthetic inputs of the BVH micro-benchmark (host-side generators in the C ABI).
+
+Mesh: an icosphere subdivided `subdivisions` times (20 * 4^s triangles),
+radially displaced by `octaves`-octave value noise of amplitude `amplitude`,
+plus `num_random` small triangles in the unit cube whose vertices sit at
+log-normal(mu, sigma) distances from a random centre.  At subdivisions=8,
+octaves=4, amplitude=0.1, num_random=100000, mu=-5, sigma=0.5 this is the
+1.41 M-triangle micro-benchmark mesh.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+FLT_MAX = np.float32(3.4028234663852886e38)
+
+
+def displaced_icosphere(subdivisions=8, octaves=4, amplitude=0.1, noise_seed=1, num_random=100000,
+                        log_mu=-5.0, log_sigma=0.5, random_seed=2):
+    lib = _lib.load()
+    nv = ctypes.c_uint32(0)
+    nt = ctypes.c_uint32(0)
+    check(lib.mi_gen_displaced_icosphere(subdivisions, octaves, amplitude, noise_seed, num_random,
+                                         log_mu, log_sigma, random_seed, None, ctypes.byref(nv),
+                                         None, ctypes.byref(nt)), "mi_gen_displaced_icosphere")
+    pos = np.zeros((nv.value, 3), np.float32)
+    idx = np.zeros((nt.value, 3), np.uint32)
+    check(lib.mi_gen_displaced_icosphere(subdivisions, octaves, amplitude, noise_seed, num_random,
+                                         log_mu, log_sigma, random_seed, pos.ctypes.data,
+                                         ctypes.byref(nv), idx.ctypes.data, ctypes.byref(nt)),
+          "mi_gen_displaced_icosphere")
+    return pos, idx
+
+
+def camera_rays(width=2048, height=2048, eye=(0.6, 0.9, 3.2), target=(0.0, 0.0, 0.0),
+                up=(0.0, 1.0, 0.0), fov_deg=45.0) -> np.ndarray:
+    """(width*height, 8) float32 pinhole rays, min_t 0, max_t FLT_MAX."""
+    rays = np.zeros((width * height, 8), np.float32)
+    e = np.asarray(eye, np.float32)
+    t = np.asarray(target, np.float32)
+    u = np.asarray(up, np.float32)
+    check(_lib.load().mi_gen_camera_rays(width, height, e.ctypes.data, t.ctypes.data, u.ctypes.data,
+                                         fov_deg, rays.ctypes.data), "mi_gen_camera_rays")
+    return rays
+
+
+def bounce_rays(rays: np.ndarray, hits: np.ndarray, positions: np.ndarray, indices: np.ndarray,
+                num_out: int, seed: int = 3) -> np.ndarray:
+    """Cosine bounce rays from the closest hits (hits as (n, 4) 32-bit words)."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    hits = np.ascontiguousarray(hits).view(np.uint32)
+    positions = np.ascontiguousarray(positions, np.float32)
+    indices = np.ascontiguousarray(indices, np.uint32)
+    out = np.zeros((num_out, 8), np.float32)
+    check(_lib.load().mi_gen_bounce_rays(rays.ctypes.data, hits.ctypes.data, len(rays),
+                                         positions.ctypes.data, 3, indices.ctypes.data, seed,
+                                         out.ctypes.data, num_out), "mi_gen_bounce_rays")
+    return out
