@@ -1,0 +1,232 @@
+"""BVH traversal micro-benchmark on one MI355X: Grays/s closest-hit.
+
+Workload (the reference's BVH micro-benchmark configuration):
+  mesh    icosphere subdivided 8x (1 310 720 triangles) radially displaced by
+          4-octave value noise (amplitude 0.1, seed 1) + 100 000 triangles in
+          the unit cube with log-normal(-5, 0.5) edge lengths (seed 2)
+  tree    sprout's SBVH, 16 slices / sweep 64 / 4 per leaf, built on the GPU
+  rays    2048 x 2048 pinhole camera rays; 4 194 304 cosine bounce rays from
+          their closest hits (offset_ray), min_t 0, max_t FLT_MAX
+  step    one closest-hit query of all 4 194 304 bounce rays (rays resident
+          in HBM before the timed region)
+
+Prints one JSON line.  Secondary figures (coherent closest-hit, incoherent
+any-hit, build time) go into the same line.  The roofline object uses the
+reference loop's algorithmic bytes per ray (32 B per node fetched + 64 B per
+triangle test, counted on the GPU by the reference-order kernel on the same
+tree and rays) over the average kernel time measured with HIP events.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "trace_closest_c1_traffic.json")
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed_launches(fn, steps: int, warmup: int):
+    """Runs fn() warmup + steps times; returns (wall seconds of the timed
+    steps, mean per-launch device time in ms from HIP events)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for start, end in events:
+        start.record()
+        fn()
+        end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = float(np.mean([s.elapsed_time(e) for s, e in events]))
+    return wall, ms
+
+
+def cpu_baseline(pos, idx, rays_np, threads: int):
+    """sprout's own Tree::intersect (oracle/_ref, compiled from the reference
+    sources) on a bounded sample of the same rays; the oracle restatement when
+    that library is absent."""
+    from oracle import oracle as O
+
+    sample = rays_np[: min(len(rays_np), 1 << 21)]
+    for arch in ("x86-64-v3", "native"):
+        if os.path.exists(O.ref_lib_path(arch)):
+            ref = O.RefTree(pos, idx, threads=threads, arch=arch)
+            ref.trace(sample[:65536])
+            _, secs = ref.trace(sample)
+            return {
+                "value": round(len(sample) / secs / 1e9, 5),
+                "unit": "Grays/s",
+                "cores": threads,
+                "kind": "reference",
+                "sample": f"{len(sample)} of the incoherent rays, sprout bvh::Tree::intersect "
+                          f"(built -O3 -march={arch}) on {threads} host threads; tree built by "
+                          f"sprout's Builder_SAH in {ref.build_seconds:.2f} s",
+            }
+    tree = O.build(pos, idx)
+    t0 = time.perf_counter()
+    O.trace(tree, pos, idx, sample, threads=threads)
+    secs = time.perf_counter() - t0
+    return {
+        "value": round(len(sample) / secs / 1e9, 5),
+        "unit": "Grays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(sample)} of the incoherent rays, oracle restatement on {threads} threads",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--width", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the ray batch "
+              "shards across GPUs with a replicated tree, which this run does not measure)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from sprout_amd import bvh, scenes
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    bvh.init(0)
+
+    log("generating mesh")
+    pos, idx = scenes.displaced_icosphere()
+    d_pos = torch.from_numpy(pos).cuda()
+    d_idx = torch.from_numpy(idx.view(np.int32)).cuda()
+
+    log("building SBVH on the GPU")
+    builder = bvh.Builder_SAH(16, 64, 4)
+    tree = builder.build(d_pos, d_idx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tree = builder.build(d_pos, d_idx)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    info = tree.info()
+    log(f"  {info.num_nodes} nodes, {info.num_slots} slots, depth {info.max_depth}, "
+        f"{build_s * 1e3:.1f} ms")
+
+    w = args.width
+    log(f"camera rays {w}x{w}")
+    cam = scenes.camera_rays(w, w)
+    d_cam = torch.from_numpy(cam).cuda()
+    cam_hits = tree.intersect(d_cam)
+    torch.cuda.synchronize()
+    cam_hits_np = cam_hits.cpu().numpy()
+    log("bounce rays")
+    bounce = scenes.bounce_rays(cam, cam_hits_np, pos, idx, w * w, seed=3)
+    d_bounce = torch.from_numpy(bounce).cuda()
+    n = len(bounce)
+
+    hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    occ = torch.empty((n,), dtype=torch.int32, device="cuda")
+    hits_cam = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+
+    log(f"timing closest-hit, {args.steps} steps of {n} incoherent rays")
+    wall, ms_incoh = timed_launches(lambda: tree.intersect(d_bounce, hits), args.steps, args.warmup)
+    value = n * args.steps / wall / 1e9
+
+    _, ms_coh = timed_launches(lambda: tree.intersect(d_cam, hits_cam), max(10, args.steps // 5), 2)
+    _, ms_any = timed_launches(lambda: tree.intersect_p(d_bounce, occ), max(10, args.steps // 5), 2)
+
+    log("counting reference-loop work per ray")
+    _, cnt = tree.counts(d_bounce)
+    torch.cuda.synchronize()
+    cnt = cnt.cpu().numpy().astype(np.int64)
+    alg_bytes = int(32 * cnt[:, 0].sum() + 64 * cnt[:, 1].sum())
+    achieved = alg_bytes / (ms_incoh * 1e-3) / 1e9
+
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        with open(PROFILE_TRAFFIC) as f:
+            prof = json.load(f)
+        if prof.get("num_rays") == n and prof.get("num_nodes") == info.num_nodes:
+            traffic = prof.get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"CPU baseline on {threads} threads")
+        cpu = cpu_baseline(pos, idx, bounce, threads)
+
+    line = {
+        "metric": "Grays/s closest-hit",
+        "value": round(value, 4),
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "BVH micro-benchmark: SBVH(16,64,4) over a 1.41M-triangle displaced "
+                        "icosphere + random triangles, closest hit of 4M incoherent cosine bounce rays",
+            "triangles": int(len(idx)),
+            "rays": int(n),
+            "tree_nodes": int(info.num_nodes),
+            "tree_depth": int(info.max_depth),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_ray": round(alg_bytes / n, 1),
+            "kernel_ms": round(ms_incoh, 4),
+        },
+        "cpu_baseline": cpu,
+        "extra": {
+            "coherent_closest_grays": round(n / (ms_coh * 1e-3) / 1e9, 4),
+            "incoherent_any_grays": round(n / (ms_any * 1e-3) / 1e9, 4),
+            "incoherent_closest_kernel_grays": round(n / (ms_incoh * 1e-3) / 1e9, 4),
+            "gpu_build_ms": round(build_s * 1e3, 2),
+            "nodes_per_ray": round(float(cnt[:, 0].mean()), 2),
+            "tri_tests_per_ray": round(float(cnt[:, 1].mean()), 2),
+            "bounce_hit_fraction": None,
+        },
+    }
+    hit_frac = float((hits.cpu().numpy().view(np.uint32)[:, 3] != 0xFFFFFFFF).mean())
+    line["extra"]["bounce_hit_fraction"] = round(hit_frac, 4)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

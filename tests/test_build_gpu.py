@@ -1,0 +1,183 @@
+"""GPU SBVH builder (mi_bvh_build) against the reference's own trees.
+
+The built tree must be node-for-node and slot-for-slot the tree sprout's
+Builder_SAH produces: checked against the golden fixture (written by the
+reference library), against the oracle restatement, and, for the full
+1.41 M-triangle micro-benchmark mesh, against the SHA-256 of the
+reference's tree.
+"""
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+
+    from sprout_amd import bvh
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    bvh.init(0)
+    yield bvh
+    torch.cuda.synchronize()
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def gpu_build(gpu, pos, idx, **settings):
+    import torch
+
+    p = torch.from_numpy(np.ascontiguousarray(pos, np.float32)).cuda()
+    i = torch.from_numpy(np.ascontiguousarray(idx, np.uint32).view(np.int32)).cuda()
+    tree = gpu.Builder_SAH(**settings).build(p, i)
+    torch.cuda.synchronize()
+    return tree
+
+
+def canonical(nodes):
+    n = np.ascontiguousarray(nodes).view(O.NODE_DTYPE).copy()
+    n["lo"] = n["lo"] + np.float32(0)
+    n["hi"] = n["hi"] + np.float32(0)
+    n["axis"] = np.where(n["num"] == 0, n["axis"], 0)
+    n["pad"] = 0
+    return n
+
+
+def assert_same_tree(nodes, slots, ref_nodes, ref_slots):
+    a = canonical(nodes)
+    b = canonical(ref_nodes)
+    assert len(a) == len(b)
+    if not O.nodes_equal(a, b):
+        diff = np.nonzero(a.view(np.uint8).reshape(-1, 32) != b.view(np.uint8).reshape(-1, 32))[0]
+        raise AssertionError(f"{len(np.unique(diff))} nodes differ, first {diff[:4]}: "
+                             f"{a[diff[0]]} vs {b[diff[0]]}")
+    assert np.array_equal(slots, ref_slots)
+
+
+def test_builds_the_reference_tree_golden(gpu, golden_small):
+    g = golden_small
+    log("build golden small mesh")
+    tree = gpu_build(gpu, g["positions"], g["indices"])
+    nodes, slots = tree.download()
+    ref_nodes = np.ascontiguousarray(g["nodes"]).view(O.NODE_DTYPE).reshape(-1)
+    assert_same_tree(nodes, slots, ref_nodes, g["slots"])
+    info = tree.info()
+    assert info.max_depth == O.Tree(ref_nodes, g["slots"]).depth()
+
+
+def test_built_tree_traces_like_the_oracle(gpu, golden_small):
+    import torch
+
+    g = golden_small
+    log("build + trace golden")
+    tree = gpu_build(gpu, g["positions"], g["indices"])
+    hits = tree.intersect(torch.from_numpy(g["rays"]).cuda())
+    torch.cuda.synchronize()
+    ref_nodes = np.ascontiguousarray(g["nodes"]).view(O.NODE_DTYPE).reshape(-1)
+    want = O.trace(O.Tree(ref_nodes, g["slots"]), g["positions"], g["indices"], g["rays"])
+    assert np.array_equal(hits.cpu().numpy().view(np.uint32), want)
+
+
+def random_soup(n, seed, log_mu=-3.0):
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 3)).astype(np.float32)
+    e = (np.exp(rng.normal(log_mu, 0.7, (n, 3, 3))) * rng.choice([-1, 1], (n, 3, 3))).astype(np.float32)
+    pos = (c[:, None, :] + e).reshape(-1, 3).astype(np.float32)
+    return pos, np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+
+
+def flat_grid(k):
+    """k x k quads in the z = 0 plane: flat nodes (zero extent in z)."""
+    ys, xs = np.mgrid[0:k + 1, 0:k + 1].astype(np.float32)
+    pos = np.stack([xs.ravel() / k, ys.ravel() / k, np.zeros((k + 1) ** 2, np.float32)], 1)
+    v = lambda i, j: i * (k + 1) + j  # noqa: E731
+    tris = []
+    for i in range(k):
+        for j in range(k):
+            tris.append([v(i, j), v(i, j + 1), v(i + 1, j + 1)])
+            tris.append([v(i, j), v(i + 1, j + 1), v(i + 1, j)])
+    return pos.astype(np.float32), np.array(tris, np.uint32)
+
+
+@pytest.mark.parametrize("case", ["soup-5", "soup-300", "soup-20000", "sphere-5", "flat-60",
+                                  "sweep-8", "slices-4"])
+def test_builds_the_oracle_tree(gpu, case):
+    from sprout_amd import scenes
+
+    settings = {}
+    if case.startswith("soup"):
+        pos, idx = random_soup(int(case.split("-")[1]), seed=len(case))
+    elif case == "sphere-5":
+        pos, idx = scenes.displaced_icosphere(subdivisions=5, num_random=20000, log_mu=-4.0)
+    elif case == "flat-60":
+        pos, idx = flat_grid(60)
+    elif case == "sweep-8":
+        pos, idx = random_soup(5000, seed=3)
+        settings = dict(sweep_threshold=8, max_primitives=2)
+    else:
+        pos, idx = random_soup(5000, seed=4)
+        settings = dict(num_slices=4)
+    log(f"build {case}: {len(idx)} triangles {settings}")
+    otree = O.build(pos, idx, **settings)
+    tree = gpu_build(gpu, pos, idx, **settings)
+    nodes, slots = tree.download()
+    assert_same_tree(nodes, slots, otree.nodes, otree.slots)
+
+
+def test_full_microbenchmark_mesh_matches_reference_digest(gpu):
+    from sprout_amd import scenes
+
+    stats = json.load(open(os.path.join(ROOT, "tests", "golden", "bvh_c1_stats.json")))
+    pos, idx = scenes.displaced_icosphere()
+    log(f"build the {len(idx)}-triangle mesh")
+    tree = gpu_build(gpu, pos, idx)
+    info = tree.info()
+    log(f"  {info.num_nodes} nodes, depth {info.max_depth}, {info.build_ms:.1f} ms")
+    nodes, slots = tree.download()
+    assert info.num_nodes == stats["num_nodes"]
+    assert info.num_slots == stats["num_slots"]
+    assert info.max_depth == stats["max_depth"]
+    assert hashlib.sha256(canonical(nodes).tobytes()).hexdigest() == stats["nodes_sha256"]
+    assert hashlib.sha256(slots.astype("<u4").tobytes()).hexdigest() == stats["slots_sha256"]
+
+
+def test_duplicates_up_to_255_become_one_leaf_and_more_are_rejected(gpu):
+    from sprout_amd._lib import MiError
+
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    pos = np.tile(tri, (200, 1))
+    idx = np.arange(600, dtype=np.uint32).reshape(-1, 3)
+    tree = gpu_build(gpu, pos, idx)
+    nodes, slots = tree.download()
+    otree = O.build(pos, idx)
+    assert_same_tree(nodes, slots, otree.nodes, otree.slots)
+
+    pos = np.tile(tri, (300, 1))
+    idx = np.arange(900, dtype=np.uint32).reshape(-1, 3)
+    with pytest.raises(MiError) as e:
+        gpu_build(gpu, pos, idx)
+    assert e.value.status == -5
+
+
+def test_out_of_range_indices_are_rejected(gpu):
+    from sprout_amd._lib import MiError
+
+    pos, idx = random_soup(100, seed=1)
+    idx = idx.copy()
+    idx[17, 1] = len(pos) + 5
+    with pytest.raises(MiError) as e:
+        gpu_build(gpu, pos, idx)
+    assert e.value.status == -2
