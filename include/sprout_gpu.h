@@ -138,14 +138,21 @@ int32_t mi_trace_counts(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, int
 
 /* Traversal kernel launch configuration (0 = library default). */
 typedef struct mi_trace_config {
-    uint32_t block_size;       /* threads per workgroup, multiple of 64 */
-    uint32_t blocks_per_cu;    /* resident workgroups per CU for the persistent grid */
+    uint32_t block_size;       /* threads per workgroup, multiple of 64 (default 256) */
+    uint32_t blocks_per_cu;    /* workgroups per CU of the persistent grid (default 8) */
+    uint32_t kernel;           /* 0: per-lane ray refill, 1: static grid-stride,
+                                  2: refill with inner/leaf phases (while-while, default) */
+    uint32_t refill_lanes;     /* kernels 0, 2: hand out rays once this many lanes are idle (24) */
+    uint32_t claim_batch;      /* kernels 0, 2: rays a wave claims per atomic (64) */
+    uint32_t lds_entries;      /* stack entries per lane kept in LDS, <= 32 (default 16;
+                                  0: up to 32); deeper entries spill to global memory */
 } mi_trace_config;
 
 int32_t mi_set_trace_config(mi_trace_config const* config);
 
-/* Waits for the stream and reports errors raised by earlier kernels on it
- * (e.g. MI_ERR_STACK). */
+/* Waits for the stream and reports errors of earlier calls on it.
+ * Traversals share the context's ray counter and stack spill area: calls on
+ * different streams must not run at the same time. */
 int32_t mi_sync(void* stream);
 
 /* ---- synthetic scenes (host-side, deterministic; used by tests and bench) ---- */

@@ -155,8 +155,11 @@ def upload(nodes: np.ndarray, slots: np.ndarray, positions: np.ndarray, indices:
     return Tree(handle.value)
 
 
-def set_trace_config(block_size: int = 0, blocks_per_cu: int = 0) -> None:
-    cfg = _lib.MiTraceConfig(block_size, blocks_per_cu)
+def set_trace_config(block_size: int = 0, blocks_per_cu: int = 0, kernel: int = 0,
+                     refill_lanes: int = 0, claim_batch: int = 0, lds_entries: int = 0) -> None:
+    """Traversal launch shape; 0 keeps the library default of each field."""
+    cfg = _lib.MiTraceConfig(block_size, blocks_per_cu, kernel, refill_lanes, claim_batch,
+                             lds_entries)
     check(_lib.load().mi_set_trace_config(ctypes.byref(cfg)), "mi_set_trace_config")
 
 

@@ -73,7 +73,9 @@ int32_t TraceResources::reserve_spill(size_t words) {
 
 void TraceResources::release() {
     (void)hipFree(spill);
+    (void)hipFree(counter);
     spill       = nullptr;
+    counter     = nullptr;
     spill_words = 0;
 }
 

@@ -90,9 +90,15 @@ int32_t mi_init(int32_t device) {
         return fail(MI_ERR_HIP, "mi_init: %s", hipGetErrorString(e));
     }
 
+    uint32_t* counter = nullptr;
+    if (hipError_t const e = hipMalloc(&counter, sizeof(uint32_t)); e != hipSuccess) {
+        return fail(MI_ERR_HIP, "mi_init: %s", hipGetErrorString(e));
+    }
+
     context                = new Context;
     context->device        = device;
     context->trace.num_cus = uint32_t(prop.multiProcessorCount);
+    context->trace.counter = counter;
     return MI_OK;
 }
 
@@ -274,13 +280,24 @@ int32_t mi_set_trace_config(mi_trace_config const* config) {
     if (!config) {
         return fail(MI_ERR_ARGS, "mi_set_trace_config: null config");
     }
-    uint32_t const block = config->block_size ? config->block_size : 256;
-    uint32_t const bpc   = config->blocks_per_cu ? config->blocks_per_cu : 4;
-    if (block % 64 || block > 1024 || bpc > 64) {
-        return fail(MI_ERR_ARGS, "mi_set_trace_config: block %u / blocks per CU %u", block, bpc);
+    TraceResources const defaults;
+    uint32_t const block = config->block_size ? config->block_size : defaults.block_size;
+    uint32_t const bpc   = config->blocks_per_cu ? config->blocks_per_cu : defaults.blocks_per_cu;
+    uint32_t const lanes = config->refill_lanes ? config->refill_lanes : defaults.refill_lanes;
+    uint32_t const batch = config->claim_batch ? config->claim_batch : defaults.claim_batch;
+    if (block % 64 || block > 1024 || bpc > 64 || lanes > 64 || config->kernel > 2 ||
+        batch > 65536 || config->lds_entries > 32) {
+        return fail(MI_ERR_ARGS,
+                    "mi_set_trace_config: block %u, blocks per CU %u, kernel %u, refill lanes %u, "
+                    "claim batch %u, lds entries %u",
+                    block, bpc, config->kernel, lanes, batch, config->lds_entries);
     }
     context->trace.block_size    = block;
     context->trace.blocks_per_cu = bpc;
+    context->trace.kernel        = config->kernel;
+    context->trace.refill_lanes  = lanes;
+    context->trace.claim_batch   = batch;
+    context->trace.lds_entries   = config->lds_entries ? config->lds_entries : defaults.lds_entries;
     return MI_OK;
 }
 

@@ -40,33 +40,61 @@ namespace {
 using dev::F3;
 using dev::Null;
 
-constexpr uint32_t Lds_entries = 32;
+constexpr uint32_t Max_lds_entries = 32;
 
 struct Stack {
-    uint32_t* lds;        // Lds_entries * block_size words, entry-major
-    uint32_t* spill;      // spill_entries * total_threads words, entry-major
-    uint32_t  lane;       // threadIdx.x
+    uint32_t* lds;          // lds_entries * block_size words, entry-major
+    uint32_t* spill;        // spill entries * total_threads words, entry-major
+    uint32_t  lds_entries;  // <= Max_lds_entries
+    uint32_t  lane;         // threadIdx.x
     uint32_t  block_size;
     uint32_t  gid;
     uint32_t  total_threads;
     uint32_t  sp;
 
+    __device__ void init(uint32_t* lds_base, uint32_t* spill_base, uint32_t entries,
+                         uint32_t threads) {
+        lds           = lds_base;
+        spill         = spill_base;
+        lds_entries   = entries;
+        lane          = threadIdx.x;
+        block_size    = blockDim.x;
+        gid           = blockIdx.x * blockDim.x + threadIdx.x;
+        total_threads = threads;
+        sp            = 0;
+    }
+
     __device__ void push(uint32_t v) {
-        if (sp < Lds_entries) {
+        if (sp < lds_entries) {
             lds[sp * block_size + lane] = v;
         } else {
-            spill[size_t(sp - Lds_entries) * total_threads + gid] = v;
+            spill[size_t(sp - lds_entries) * total_threads + gid] = v;
         }
         ++sp;
     }
 
     __device__ uint32_t pop() {
         --sp;
-        if (sp < Lds_entries) {
+        if (sp < lds_entries) {
             return lds[sp * block_size + lane];
         }
-        return spill[size_t(sp - Lds_entries) * total_threads + gid];
+        return spill[size_t(sp - lds_entries) * total_threads + gid];
     }
+};
+
+struct Launch {
+    float4 const* nodes;
+    float4 const* tris;
+    mi_ray const* rays;
+    uint32_t      num_rays;
+    void*         out;
+    uint32_t*     counts;
+    uint32_t*     spill;
+    uint32_t      total_threads;
+    uint32_t      lds_entries;
+    uint32_t*     counter;     // next unclaimed ray (trace_refill)
+    uint32_t      refill_lanes;
+    uint32_t      claim_batch;
 };
 
 struct Ray {
@@ -132,18 +160,18 @@ __device__ __forceinline__ bool leaf(float4 const* __restrict__ tris, uint32_t s
 }
 
 template <bool Any>
-__global__ void trace_pairs(float4 const* __restrict__ nodes, float4 const* __restrict__ tris,
-                            mi_ray const* __restrict__ rays, uint32_t num_rays, void* out,
-                            uint32_t* spill, uint32_t total_threads) {
+__global__ void trace_pairs(Launch p) {
     extern __shared__ uint32_t lds_stack[];
 
+    float4 const* __restrict__ nodes = p.nodes;
+    float4 const* __restrict__ tris  = p.tris;
+    mi_ray const* __restrict__ rays  = p.rays;
+    uint32_t const num_rays          = p.num_rays;
+    uint32_t const total_threads     = p.total_threads;
+    void* const    out               = p.out;
+
     Stack stack;
-    stack.lds           = lds_stack;
-    stack.spill         = spill;
-    stack.lane          = threadIdx.x;
-    stack.block_size    = blockDim.x;
-    stack.gid           = blockIdx.x * blockDim.x + threadIdx.x;
-    stack.total_threads = total_threads;
+    stack.init(lds_stack, p.spill, p.lds_entries, total_threads);
 
     for (uint32_t r = stack.gid; r < num_rays; r += total_threads) {
         Ray ray = load_ray(rays, r);
@@ -228,18 +256,19 @@ __global__ void trace_pairs(float4 const* __restrict__ nodes, float4 const* __re
 }
 
 template <bool Any>
-__global__ void trace_reference(float4 const* __restrict__ nodes, float4 const* __restrict__ tris,
-                                mi_ray const* __restrict__ rays, uint32_t num_rays, void* out,
-                                uint32_t* counts, uint32_t* spill, uint32_t total_threads) {
+__global__ void trace_reference(Launch p) {
     extern __shared__ uint32_t lds_stack[];
 
+    float4 const* __restrict__ nodes = p.nodes;
+    float4 const* __restrict__ tris  = p.tris;
+    mi_ray const* __restrict__ rays  = p.rays;
+    uint32_t const num_rays          = p.num_rays;
+    uint32_t const total_threads     = p.total_threads;
+    void* const    out               = p.out;
+    uint32_t* const counts           = p.counts;
+
     Stack stack;
-    stack.lds           = lds_stack;
-    stack.spill         = spill;
-    stack.lane          = threadIdx.x;
-    stack.block_size    = blockDim.x;
-    stack.gid           = blockIdx.x * blockDim.x + threadIdx.x;
-    stack.total_threads = total_threads;
+    stack.init(lds_stack, p.spill, p.lds_entries, total_threads);
 
     for (uint32_t r = stack.gid; r < num_rays; r += total_threads) {
         Ray ray = load_ray(rays, r);
@@ -303,6 +332,198 @@ __global__ void trace_reference(float4 const* __restrict__ nodes, float4 const* 
     }
 }
 
+
+__device__ __forceinline__ void write_result(Launch const& p, uint32_t r, bool any, bool hit_p,
+                                             float max_t, uint32_t index, float u, float v) {
+    if (any) {
+        static_cast<uint32_t*>(p.out)[r] = hit_p ? 1u : 0u;
+    } else {
+        float4 h;
+        h.x = max_t;
+        h.y = Null != index ? u : 0.f;
+        h.z = Null != index ? v : 0.f;
+        h.w = __uint_as_float(index);
+        reinterpret_cast<float4*>(p.out)[r] = h;
+    }
+}
+
+// Pops pending far children until one still passes the box test with the
+// current max_t; returns false when the stack runs empty (the ray is done).
+__device__ __forceinline__ bool pop_next(float4 const* __restrict__ nodes, Stack& stack,
+                                         Ray const& ray, uint32_t& data, uint32_t& meta) {
+    while (stack.sp > 0) {
+        uint32_t const m   = stack.pop();
+        float4 const   mlo = nodes[2 * size_t(m) + 0];
+        float4 const   mhi = nodes[2 * size_t(m) + 1];
+        if (dev::slab(mlo, mhi, ray.o, ray.inv, ray.min_t, ray.max_t)) {
+            data = __float_as_uint(mlo.w);
+            meta = __float_as_uint(mhi.w);
+            return true;
+        }
+    }
+    return false;
+}
+
+// One inner node: both children from one 64-byte fetch, near/far by the
+// sign of inv_dir on the node's axis (triangle_bvh_tree.cpp:64-70).
+// Returns false when the ray is done.
+__device__ __forceinline__ bool inner_step(float4 const* __restrict__ nodes, Stack& stack,
+                                           Ray const& ray, uint32_t& data, uint32_t& meta) {
+    uint32_t const a = data;
+
+    float4 const alo = nodes[2 * size_t(a) + 0];
+    float4 const ahi = nodes[2 * size_t(a) + 1];
+    float4 const blo = nodes[2 * size_t(a) + 2];
+    float4 const bhi = nodes[2 * size_t(a) + 3];
+
+    bool const ha = dev::slab(alo, ahi, ray.o, ray.inv, ray.min_t, ray.max_t);
+    bool const hb = dev::slab(blo, bhi, ray.o, ray.inv, ray.min_t, ray.max_t);
+
+    bool const flip = (ray.sign_bits >> node_axis(meta)) & 1u;
+    bool const hn   = flip ? hb : ha;
+    bool const hf   = flip ? ha : hb;
+
+    if (hn | hf) {
+        if (hn & hf) {
+            stack.push(flip ? a : a + 1);
+        }
+        bool const take_b = hn ? flip : !flip;
+        data              = __float_as_uint(take_b ? blo.w : alo.w);
+        meta              = __float_as_uint(take_b ? bhi.w : ahi.w);
+        return true;
+    }
+    return pop_next(nodes, stack, ray, data, meta);
+}
+
+// Persistent traversal with per-lane ray refill: a lane whose ray is done
+// takes the next unclaimed ray, so the 64 lanes of a wave stay busy while
+// rays of very different lengths finish.  Each wave claims rays in batches
+// of claim_batch from the global counter (one atomic per batch) and hands
+// them out from a wave-uniform pool once at least refill_lanes lanes are
+// idle (or all are).  WhileWhile separates inner-node steps from leaf steps
+// (all lanes descend until every active lane sits at a leaf); the visiting
+// order of each ray is the same either way.
+template <bool Any, bool WhileWhile>
+__global__ void trace_refill(Launch p) {
+    extern __shared__ uint32_t lds_stack[];
+
+    float4 const* __restrict__ nodes = p.nodes;
+    float4 const* __restrict__ tris  = p.tris;
+
+    Stack stack;
+    stack.init(lds_stack, p.spill, p.lds_entries, p.total_threads);
+
+    uint32_t const lane    = threadIdx.x & 63;
+    uint64_t const lt_mask = (uint64_t(1) << lane) - 1;
+
+    bool     active    = false;
+    bool     exhausted = false;  // the global counter is past the last ray (wave-uniform)
+    uint32_t pool_next = 0;      // wave-uniform pool of claimed rays [pool_next, pool_end)
+    uint32_t pool_end  = 0;
+    uint32_t r         = 0;
+    Ray      ray;
+    uint32_t data  = 0;
+    uint32_t meta  = 0;
+    uint32_t index = Null;
+    float    u     = 0.f;
+    float    v     = 0.f;
+
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        uint32_t const n_idle = uint32_t(__popcll(idle));
+
+        if ((n_idle >= p.refill_lanes || 64 == n_idle) && !(exhausted && pool_next >= pool_end)) {
+            while (idle) {
+                if (pool_next >= pool_end) {
+                    if (exhausted) {
+                        break;
+                    }
+                    uint32_t const leader =
+                        uint32_t(__ffsll(static_cast<unsigned long long>(idle))) - 1;
+                    uint32_t base = 0;
+                    if (lane == leader) {
+                        base = atomicAdd(p.counter, p.claim_batch);
+                    }
+                    base = __shfl(base, leader);
+                    if (base >= p.num_rays) {
+                        exhausted = true;
+                        break;
+                    }
+                    pool_next = base;
+                    pool_end  = p.num_rays - base <= p.claim_batch ? p.num_rays : base + p.claim_batch;
+                    if (pool_end == p.num_rays) {
+                        exhausted = true;
+                    }
+                }
+                uint32_t const avail = pool_end - pool_next;
+                uint32_t const want  = uint32_t(__popcll(idle));
+                uint32_t const take  = avail < want ? avail : want;
+                bool const     mine  = (idle >> lane) & 1u;
+                uint32_t const rank  = uint32_t(__popcll(idle & lt_mask));
+                if (mine && rank < take) {
+                    r        = pool_next + rank;
+                    ray      = load_ray(p.rays, r);
+                    index    = Null;
+                    u        = 0.f;
+                    v        = 0.f;
+                    stack.sp = 0;
+
+                    float4 const rlo = nodes[0];
+                    float4 const rhi = nodes[1];
+                    if (dev::slab(rlo, rhi, ray.o, ray.inv, ray.min_t, ray.max_t)) {
+                        data   = __float_as_uint(rlo.w);
+                        meta   = __float_as_uint(rhi.w);
+                        active = true;
+                    } else {
+                        write_result(p, r, Any, false, ray.max_t, Null, 0.f, 0.f);
+                    }
+                }
+                pool_next += take;
+                idle = __ballot(mine && rank >= take);
+            }
+        }
+
+        if (0 == __ballot(active)) {
+            if (exhausted && pool_next >= pool_end) {
+                break;
+            }
+            continue;
+        }
+
+        if (WhileWhile) {
+            while (__ballot(active && 0 == node_num(meta))) {
+                if (active && 0 == node_num(meta)) {
+                    if (!inner_step(nodes, stack, ray, data, meta)) {
+                        write_result(p, r, Any, false, ray.max_t, index, u, v);
+                        active = false;
+                    }
+                }
+            }
+        }
+
+        if (active) {
+            uint32_t const num = node_num(meta);
+            bool           live;
+            bool           hit_p = false;
+            if (0 == num) {
+                live = inner_step(nodes, stack, ray, data, meta);
+            } else {
+                uint32_t tests = 0;
+                hit_p          = leaf<Any>(tris, data, num, ray, index, u, v, tests);
+                live           = !hit_p && pop_next(nodes, stack, ray, data, meta);
+            }
+            if (!live) {
+                write_result(p, r, Any, hit_p, ray.max_t, index, u, v);
+                active = false;
+            }
+        }
+    }
+}
+
+__global__ void reset_counter(uint32_t* counter) {
+    *counter = 0;
+}
+
 }  // namespace
 
 int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays,
@@ -323,30 +544,46 @@ int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* ra
     uint32_t const blocks  = res.grid_blocks(num_rays);
     uint32_t const threads = blocks * block;
 
-    uint32_t const spill_entries = entries > Lds_entries ? entries - Lds_entries : 0;
+    uint32_t const lds_cap     = res.lds_entries ? res.lds_entries : Max_lds_entries;
+    uint32_t const lds_entries = entries < lds_cap ? entries : lds_cap;
+    uint32_t const spill_entries = entries - lds_entries;
     if (int32_t const st = res.reserve_spill(size_t(spill_entries) * threads); st) {
         return st;
     }
 
-    size_t const lds_bytes = size_t(Lds_entries) * block * sizeof(uint32_t);
+    size_t const lds_bytes = size_t(lds_entries) * block * sizeof(uint32_t);
 
-    float4 const* nodes = bvh.nodes;
-    float4 const* tris  = bvh.tris;
+    Launch p;
+    p.nodes         = bvh.nodes;
+    p.tris          = bvh.tris;
+    p.rays          = rays;
+    p.num_rays      = num_rays;
+    p.out           = out;
+    p.counts        = counts;
+    p.spill         = res.spill;
+    p.total_threads = threads;
+    p.lds_entries   = lds_entries;
+    p.counter       = res.counter;
+    p.refill_lanes  = res.refill_lanes;
+    p.claim_batch   = res.claim_batch;
 
     if (counts) {
         if (any_hit) {
-            hipLaunchKernelGGL(trace_reference<true>, dim3(blocks), dim3(block), lds_bytes, stream,
-                               nodes, tris, rays, num_rays, out, counts, res.spill, threads);
+            hipLaunchKernelGGL(trace_reference<true>, dim3(blocks), dim3(block), lds_bytes, stream, p);
         } else {
-            hipLaunchKernelGGL(trace_reference<false>, dim3(blocks), dim3(block), lds_bytes, stream,
-                               nodes, tris, rays, num_rays, out, counts, res.spill, threads);
+            hipLaunchKernelGGL(trace_reference<false>, dim3(blocks), dim3(block), lds_bytes, stream, p);
         }
+    } else if (res.kernel != 1) {
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, res.counter);
+        using Kernel    = void (*)(Launch);
+        bool const ww   = 2 == res.kernel;
+        Kernel     kern = any_hit ? (ww ? trace_refill<true, true> : trace_refill<true, false>)
+                                  : (ww ? trace_refill<false, true> : trace_refill<false, false>);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(block), lds_bytes, stream, p);
     } else if (any_hit) {
-        hipLaunchKernelGGL(trace_pairs<true>, dim3(blocks), dim3(block), lds_bytes, stream, nodes,
-                           tris, rays, num_rays, out, res.spill, threads);
+        hipLaunchKernelGGL(trace_pairs<true>, dim3(blocks), dim3(block), lds_bytes, stream, p);
     } else {
-        hipLaunchKernelGGL(trace_pairs<false>, dim3(blocks), dim3(block), lds_bytes, stream, nodes,
-                           tris, rays, num_rays, out, res.spill, threads);
+        hipLaunchKernelGGL(trace_pairs<false>, dim3(blocks), dim3(block), lds_bytes, stream, p);
     }
 
     if (hipError_t const e = hipGetLastError(); e != hipSuccess) {

@@ -29,7 +29,7 @@ import glob
 import os
 import statistics
 
-KERNEL = "trace_pairs<false>"
+KERNEL = "trace_refill<false, true>"
 COPY_BYTES = 256 << 20
 
 
@@ -103,7 +103,8 @@ def kernel_table(kt, top=12):
     rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
     lines = ["| kernel | calls | avg us | total % |", "|---|---|---|---|"]
     for r in rows[:top]:
-        name = r["Name"].split("(")[0].replace("mi::(anonymous namespace)::", "")
+        name = r["Name"].replace("mi::(anonymous namespace)::", "").replace("void ", "")
+        name = name.split("(")[0]
         lines.append(f"| `{name[:70]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                      f"{float(r['Percentage']):.1f} |")
     return "\n".join(lines)

@@ -122,15 +122,22 @@ def test_medium_mesh_closest_and_any(gpu, medium):
     assert np.array_equal(run_any(gpu, tree, rays), O.trace(otree, pos, idx, rays, any_hit=True))
 
 
-@pytest.mark.parametrize("block,per_cu", [(64, 1), (128, 3), (256, 8), (512, 2), (1024, 1)])
-def test_launch_shapes_do_not_change_results(gpu, medium, block, per_cu):
+@pytest.mark.parametrize("kernel,block,per_cu,lanes,batch,lds", [
+    (0, 64, 1, 1, 1, 32), (0, 128, 3, 8, 7, 4), (0, 256, 8, 16, 256, 16), (0, 512, 2, 64, 64, 1),
+    (0, 1024, 1, 32, 1000, 8), (1, 64, 1, 0, 0, 32), (1, 256, 8, 0, 0, 2), (1, 1024, 1, 0, 0, 16),
+    (2, 64, 2, 24, 64, 3), (2, 256, 8, 24, 64, 16), (2, 512, 4, 1, 5, 32)])
+def test_launch_shapes_do_not_change_results(gpu, medium, kernel, block, per_cu, lanes, batch, lds):
+    """Every kernel variant and launch shape gives the same bits; small LDS
+    stacks force the global spill path on this depth-20 tree."""
     pos, idx, otree, tree, rays = medium
-    log(f"launch shape block={block} blocks_per_cu={per_cu}")
-    gpu.set_trace_config(block, per_cu)
+    log(f"kernel {kernel} block={block} blocks_per_cu={per_cu} refill_lanes={lanes} "
+        f"claim_batch={batch} lds_entries={lds}")
+    gpu.set_trace_config(block, per_cu, kernel, lanes, batch, lds)
     try:
         assert np.array_equal(run_closest(gpu, tree, rays), O.trace(otree, pos, idx, rays))
+        assert np.array_equal(run_any(gpu, tree, rays), O.trace(otree, pos, idx, rays, any_hit=True))
     finally:
-        gpu.set_trace_config(0, 0)
+        gpu.set_trace_config()
 
 
 def test_empty_and_single_ray_batches(gpu, golden_small):
