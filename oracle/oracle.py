@@ -81,6 +81,12 @@ def lib():
         _lib.oracle_offset_ray.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.oracle_surface_area.restype = ctypes.c_float
         _lib.oracle_surface_area.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        for name in ("oracle_substitute_evaluate", "oracle_substitute_sample"):
+            getattr(_lib, name).argtypes = [ctypes.c_uint32] + [ctypes.c_void_p] * 5
+        _lib.oracle_ggx_e_table.argtypes = [ctypes.c_void_p]
+        _lib.oracle_schlick_f0.restype = ctypes.c_float
+        _lib.oracle_schlick_f0.argtypes = [ctypes.c_float, ctypes.c_float]
+        _lib.oracle_orthonormal_basis.argtypes = [ctypes.c_void_p] * 3
     return _lib
 
 
@@ -184,6 +190,59 @@ def surface_area(lo, hi) -> float:
     lo = np.ascontiguousarray(lo, np.float32).reshape(3)
     hi = np.ascontiguousarray(hi, np.float32).reshape(3)
     return lib().oracle_surface_area(_ptr(lo), _ptr(hi))
+
+
+# ---------------------------------------------------------------------------
+# Substitute BSDF
+
+
+def _f32(a, cols):
+    return np.ascontiguousarray(a, np.float32).reshape(-1, cols)
+
+
+def substitute_evaluate(params, frames, wo, wi, lib_=None, prefix="oracle"):
+    """(n, 4): reflection.rgb (times n.wi), pdf.  params (n, 6) = color, roughness,
+    metallic, ior; frames (n, 12) = t, b, n, geo_n."""
+    params, frames, wo, wi = _f32(params, 6), _f32(frames, 12), _f32(wo, 3), _f32(wi, 3)
+    out = np.zeros((len(params), 4), np.float32)
+    getattr(lib_ or lib(), f"{prefix}_substitute_evaluate")(
+        len(params), _ptr(params), _ptr(frames), _ptr(wo), _ptr(wi), _ptr(out))
+    return out
+
+
+def substitute_sample(params, frames, wo, xi, lib_=None, prefix="oracle"):
+    """(n, 8): wi.xyz, reflection.rgb, pdf, type bits (as float32 view)."""
+    params, frames, wo, xi = _f32(params, 6), _f32(frames, 12), _f32(wo, 3), _f32(xi, 3)
+    out = np.zeros((len(params), 8), np.float32)
+    getattr(lib_ or lib(), f"{prefix}_substitute_sample")(
+        len(params), _ptr(params), _ptr(frames), _ptr(wo), _ptr(xi), _ptr(out))
+    return out
+
+
+def ggx_e_table(lib_=None, prefix="oracle") -> np.ndarray:
+    out = np.zeros(1024, np.float32)
+    getattr(lib_ or lib(), f"{prefix}_ggx_e_table")(_ptr(out))
+    return out.reshape(32, 32)
+
+
+def orthonormal_basis(n):
+    n = np.ascontiguousarray(n, np.float32).reshape(3)
+    t = np.zeros(3, np.float32)
+    b = np.zeros(3, np.float32)
+    lib().oracle_orthonormal_basis(_ptr(n), _ptr(t), _ptr(b))
+    return t, b
+
+
+def ref_bsdf_lib(arch: str = "native"):
+    """The reference's Substitute sample (oracle/_ref), or None when not built."""
+    path = ref_lib_path(arch)
+    if not os.path.exists(path):
+        return None
+    l = ctypes.CDLL(path)
+    for name in ("sref_substitute_evaluate", "sref_substitute_sample"):
+        getattr(l, name).argtypes = [ctypes.c_uint32] + [ctypes.c_void_p] * 5
+    l.sref_ggx_e_table.argtypes = [ctypes.c_void_p]
+    return l
 
 
 # ---------------------------------------------------------------------------

@@ -1,5 +1,6 @@
-# Builds the reference's own triangle SBVH builder and traversal, from the
-# reference sources where they lie, into oracle/_ref/ (never committed).
+# Builds the reference's own triangle SBVH builder and traversal and its
+# Substitute (Disney + GGX) material sample, from the reference sources where
+# they lie, into oracle/_ref/ (never committed).
 # Flags follow the reference's CMake configuration for a Release build on
 # Linux (source/CMakeLists.txt: C++20, -O3, -march=native, no RTTI, no
 # exceptions, SU_ACESCG / SU_IBL_MIS_COMPENSATION / SU_ZSTD definitions).
@@ -33,12 +34,16 @@ REF_SOURCES := \
   $(SRC)/core/scene/shape/shape_vertex.cpp \
   $(SRC)/base/thread/thread_pool.cpp \
   $(SRC)/core/logging/logging.cpp \
-  $(SRC)/core/logging/log.cpp
+  $(SRC)/core/logging/log.cpp \
+  $(SRC)/core/scene/material/substitute/substitute_sample.cpp \
+  $(SRC)/core/sampler/sampler.cpp
 
 all: $(LIB)
 
-$(LIB): $(REF_SOURCES) $(HERE)ref_driver.cpp
+DRIVERS := $(HERE)ref_driver.cpp $(HERE)ref_bsdf_driver.cpp
+
+$(LIB): $(REF_SOURCES) $(DRIVERS)
 	@mkdir -p $(OUT)
-	g++ $(CXXFLAGS) -shared -o $@ $(HERE)ref_driver.cpp $(REF_SOURCES) -lpthread
+	g++ $(CXXFLAGS) -shared -o $@ $(DRIVERS) $(REF_SOURCES) -lpthread
 
 .PHONY: all

@@ -14,6 +14,10 @@ inputs made by this repository's generators:
   bvh_c1_stats.json the 1.41 M-triangle micro-benchmark mesh: the
                     reference tree's node/slot counts, depth and SHA-256 of
                     its nodes (bounds by value) and slots.
+  bsdf_small.npz    the reference's Substitute material sample (Disney +
+                    GGX): 4096 evaluate cases and 4096 sample cases over random
+                    frames, directions, colours, roughness, metallic, and the
+                    32 x 32 GGX energy table it interpolates.
 
     python tests/golden/make_golden.py
 """
@@ -103,6 +107,34 @@ def small_rays(pos: np.ndarray, idx: np.ndarray, tree: O.Tree, rng) -> np.ndarra
         np.concatenate([cam, bounce, rand, aligned, c, aimed]).astype(np.float32))
 
 
+def bsdf_cases(rng, n):
+    def unit(v):
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    normal = unit(rng.normal(size=(n, 3))).astype(np.float32)
+    frames = np.zeros((n, 12), np.float32)
+    for i in range(n):
+        t, b = O.orthonormal_basis(normal[i])
+        frames[i, 0:3] = t
+        frames[i, 3:6] = b
+    frames[:, 6:9] = normal
+    frames[:, 9:12] = normal
+    wo = unit(rng.normal(size=(n, 3)))
+    wo = np.where((wo * normal).sum(1, keepdims=True) < 0, -wo, wo).astype(np.float32)
+    wi = unit(rng.normal(size=(n, 3)))
+    wi = np.where((wi * normal).sum(1, keepdims=True) < 0, -wi, wi).astype(np.float32)
+    params = np.zeros((n, 6), np.float32)
+    params[:, 0:3] = rng.uniform(0.05, 0.95, (n, 3))
+    params[:, 3] = rng.uniform(0.0, 1.0, n)
+    params[:, 4] = (rng.random(n) < 0.3).astype(np.float32)
+    params[: n // 4, 4] = rng.uniform(0, 1, n // 4)
+    params[:, 5] = 1.46
+    # a few back-facing wo (no contribution in the reference)
+    wo[-16:] = -wo[-16:]
+    xi = rng.random((n, 3)).astype(np.float32)
+    return params, frames, wo, wi, xi
+
+
 def main() -> None:
     subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "ref.mk")], check=True)
 
@@ -130,6 +162,16 @@ def main() -> None:
         slots=tree.slots, rays=rays, hits=hits, occluded=occluded, rcp_probe=probe,
         rcp_native=native_rcp,
         host=np.array(platform.processor() or platform.machine()),
+    )
+
+    ref_bsdf = O.ref_bsdf_lib()
+    params, frames, wo, wi, xi = bsdf_cases(np.random.default_rng(7), 4096)
+    np.savez_compressed(
+        os.path.join(HERE, "bsdf_small.npz"),
+        params=params, frames=frames, wo=wo, wi=wi, xi=xi,
+        evaluate=O.substitute_evaluate(params, frames, wo, wi, ref_bsdf, "sref"),
+        sample=O.substitute_sample(params, frames, wo, xi, ref_bsdf, "sref"),
+        e_table=O.ggx_e_table(ref_bsdf, "sref"),
     )
 
     pos1, idx1 = scenes.displaced_icosphere()
