@@ -101,6 +101,50 @@ def cpu_baseline(pos, idx, rays_np, threads: int):
     }
 
 
+def room_render(frames: int, cpu: bool):
+    """The box room (256 x 256, 16 spp, max depth 8, Russian roulette from
+    depth 4, seed 1): one frame = one mi_render call of 16 samples of every
+    pixel; returns figures for the JSON line."""
+    import torch
+
+    from sprout_amd import render, scenes
+
+    pos, idx, tmat, mats = scenes.room()
+    sc = render.Scene(pos, idx, tmat, mats)
+    cam = render.camera(256, 256, **scenes.ROOM_CAMERA)
+    acc = torch.zeros((256, 256, 3), dtype=torch.float64, device="cuda")
+    render.render(sc, cam, 16, samples_per_wave=16, seed=1, accum=acc)
+    torch.cuda.synchronize()
+    paths = rays = 0.0
+    t0 = time.perf_counter()
+    for f in range(frames):
+        _, st = render.render(sc, cam, 16, samples_per_wave=16, seed=1, first_sample=16 * (f + 1),
+                              accum=acc, stats=True)
+        paths += st.paths
+        rays += st.extend_rays + st.shadow_rays
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {
+        "workload": "box room 256x256, 16 spp, max depth 8, RR from depth 4, MIS, Lambert",
+        "ms_per_frame": round(wall / frames * 1e3, 3),
+        "samples_per_s": round(paths / wall, 1),
+        "grays_all_rays": round(rays / wall / 1e9, 4),
+        "rays_per_sample": round(rays / paths, 3),
+    }
+    sc.release()
+    if cpu:
+        from oracle import oracle as O
+
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        O.render(pos, idx, tmat, mats, 256, 256, 4, scenes.ROOM_CAMERA, seed=1, threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_port_samples_per_s"] = round(256 * 256 * 4 / dt, 1)
+        out["cpu_port_threads"] = threads
+        out["cpu_port_sample"] = "256x256 x 4 spp of the same room, oracle/pt.c"
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +282,8 @@ def main() -> int:
     }
     hit_frac = float((hits.cpu().numpy().view(np.uint32)[:, 3] != 0xFFFFFFFF).mean())
     line["extra"]["bounce_hit_fraction"] = round(hit_frac, 4)
+    log("box room path tracing")
+    line["extra"]["room_render"] = room_render(5, not args.no_cpu_baseline)
     print(json.dumps(line), flush=True)
     return 0
 

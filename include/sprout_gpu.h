@@ -1,5 +1,6 @@
 /*
- * sprout_gpu: MI355X (gfx950) SBVH build and ray traversal for sprout.
+ * sprout_gpu: MI355X (gfx950) SBVH build, ray traversal and wavefront MIS
+ * path tracing for sprout.
  *
  * C ABI.  Conventions follow sprout's own C API (source/capi/sprout.h:39-122,
  * source/capi/capi.cpp:112-131): int32_t status (0 ok, negative error),
@@ -155,7 +156,154 @@ int32_t mi_set_trace_config(mi_trace_config const* config);
  * different streams must not run at the same time. */
 int32_t mi_sync(void* stream);
 
+/* ---- wavefront MIS path tracer ---- */
+
+enum mi_material_kind {
+    MI_MATERIAL_SUBSTITUTE = 0, /* Disney diffuse + GGX with energy compensation
+                                   (substitute_material.cpp:19-30, substitute_sample.cpp:10-45) */
+    MI_MATERIAL_LAMBERT    = 1, /* lambert::Iso (lambert/lambert.inl:15-45) */
+    MI_MATERIAL_LIGHT      = 2, /* one-sided constant emitter, pure emissive
+                                   (light/light_constant.cpp:13-35) */
+};
+
+/* A material.  emission is the radiance of a LIGHT material; ior is the
+ * Substitute's index of refraction (f0 = schlick_f0(ior, 1)). */
+typedef struct mi_material {
+    float    color[3];
+    float    emission[3];
+    float    roughness;
+    float    metallic;
+    float    ior;
+    uint32_t kind;
+} mi_material;
+
+/* Pinhole camera: vertical field of view in degrees; pixel (x, y) row-major
+ * from the top-left; each sample jitters uniformly inside its pixel (box
+ * filter). */
+typedef struct mi_camera {
+    uint32_t width;
+    uint32_t height;
+    float    eye[3];
+    float    target[3];
+    float    up[3];
+    float    fov_deg;
+} mi_camera;
+
+/* Pathtracer_MIS settings (pathtracer_mis.hpp Settings; take_loader.cpp:617-739).
+ * A render runs samples_per_pixel samples as waves of samples_per_wave
+ * samples of every pixel (samples_per_pixel must be a multiple of it).
+ * Sample s of pixel (x, y) draws its random numbers from sprout's PCG32
+ * (base/random/generator.inl:16-49) started as start(seed, s * W * H +
+ * y * W + x), like the per-pixel start of rendering_worker.cpp:104 with the
+ * sample index in place of the iteration; samples first_sample ..
+ * first_sample + samples_per_pixel - 1 are rendered, so successive calls
+ * continue a progressive render. */
+typedef struct mi_render_settings {
+    uint32_t samples_per_pixel;
+    uint32_t samples_per_wave;
+    uint32_t first_sample;
+    uint32_t min_bounces; /* Russian roulette from this depth on (sprout default 4) */
+    uint32_t max_bounces; /* <= 64 */
+    uint32_t flags;       /* MI_RENDER_* */
+    uint64_t seed;
+} mi_render_settings;
+
+enum mi_render_flags {
+    MI_RENDER_NO_GRAPH = 1, /* launch the bounce loop kernel by kernel instead of as a hipGraph */
+};
+
+typedef struct mi_render_stats {
+    double seconds;      /* host wall time of the call, launch to completion */
+    double paths;        /* camera samples */
+    double extend_rays;  /* closest-hit rays traced */
+    double shadow_rays;  /* any-hit rays traced */
+} mi_render_stats;
+
+/* Creates a scene from HOST arrays: triangles with one material each; the
+ * emissive triangles (kind LIGHT) form the scene's light, sampled uniformly
+ * by area.  The tree is built on the device with *bvh_settings (NULL:
+ * {16, 64, 4}).  Mirrors su_create_triangle_mesh_async + material creation +
+ * Scene::compile (capi.cpp:378-406, scene.cpp:216-267). */
+int32_t mi_scene_create(float const* positions, uint32_t num_vertices, uint32_t const* indices,
+                        uint32_t num_triangles, uint32_t const* triangle_materials,
+                        mi_material const* materials, uint32_t num_materials,
+                        mi_bvh_settings const* bvh_settings, uint32_t* scene);
+
+int32_t mi_scene_release(uint32_t scene);
+
+/* Renders and ADDS each pixel's radiance sum over the samples into accum,
+ * a DEVICE buffer of width * height * 3 doubles (rgb); the image is accum
+ * divided by the total sample count.  Enqueued on stream (void*
+ * hipStream_t, NULL = null stream).  stats (HOST, may be NULL) makes the
+ * call wait for the render and fill it.  Replaces the per-sample
+ * Pathtracer_MIS::li (pathtracer_mis.cpp:59-93) behind the tile loop of
+ * Worker::render (rendering_worker.cpp:73-127) for a whole frame, the
+ * boundary of Driver::render_frame_forward (rendering_driver.cpp:295-360). */
+int32_t mi_render(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
+                  double* accum, void* stream, mi_render_stats* stats);
+
+/* The 32 x 32 GGX energy table E(n.wo, alpha) (rows alpha) regenerated on the
+ * device by sprout's procedure (ggx_integrate.cpp:17-45, 202-240; the data of
+ * ggx_integral.inl).  DEVICE pointer of 1024 floats. */
+int32_t mi_ggx_e_table(float* table, void* stream);
+
+/* Batch Substitute material queries on DEVICE arrays, one case per index:
+ * params[6] (color rgb, roughness, metallic, ior), frames[12] (tangent,
+ * bitangent, shading normal, geometric normal), wo[3].
+ * evaluate: wi[3] -> out[4] (reflection * n.wi rgb, pdf)   (substitute_sample.cpp:10-26)
+ * sample:   xi[3] (lobe choice, 2-D sample) -> out[8] (wi xyz, reflection rgb,
+ *           pdf, bxdf type bits as uint32; pdf 0 = no sample) (substitute_sample.cpp:28-45) */
+int32_t mi_substitute_evaluate(uint32_t n, float const* params, float const* frames,
+                               float const* wo, float const* wi, float* out, void* stream);
+int32_t mi_substitute_sample(uint32_t n, float const* params, float const* frames,
+                             float const* wo, float const* xi, float* out, void* stream);
+
+/* ---- path-guiding network (no counterpart in the reference) ----
+ *
+ * A small MLP maps a shading point (position in the scene box, normal and
+ * outgoing direction, each mapped to [0,1]; 9 inputs) through a
+ * 12-frequency sin/cos encoding (216 features, padded to 224) and
+ * 224-64-64-64-32 layers (ReLU) to 8 von Mises-Fisher lobes x (a, b, c, d):
+ * mean direction (pi sigmoid(a), 2 pi sigmoid(b)) in spherical angles,
+ * kappa = exp(clamp(c, -4, 8)), lobe weight softmax(d).  BF16 weights, FP32
+ * accumulation and FP32 master parameters trained with Adam on the loss
+ * -sum_i (t_i / sum t) log q(w_i).  Parameters (24800 floats): per layer
+ * W [in][out] row-major then b [out].  The nearest reference code are the
+ * particle-importance maps (core/rendering/integrator/particle/
+ * particle_importance.hpp); nothing in sprout learns. */
+
+#define MI_GUIDE_NUM_PARAMS 24800u
+
+int32_t mi_guide_create(uint64_t seed, uint32_t* guide);
+int32_t mi_guide_release(uint32_t guide);
+/* HOST arrays of MI_GUIDE_NUM_PARAMS floats; set also resets Adam's moments */
+int32_t mi_guide_set_params(uint32_t guide, float const* params);
+int32_t mi_guide_get_params(uint32_t guide, float* params);
+/* DEVICE arrays: x n x 9 inputs, out n x 32 raw outputs */
+int32_t mi_guide_forward(uint32_t guide, float const* x, uint32_t n, float* out, void* stream);
+/* DEVICE arrays: x n x 9, dirs n x 3 unit directions, weights n (>= 0);
+ * grad receives MI_GUIDE_NUM_PARAMS floats of the loss gradient */
+int32_t mi_guide_gradients(uint32_t guide, float const* x, float const* dirs,
+                           float const* weights, uint32_t n, float* grad, void* stream);
+/* one Adam step (beta 0.9 / 0.99, eps 1e-8) on the batch */
+int32_t mi_guide_train_step(uint32_t guide, float const* x, float const* dirs,
+                            float const* weights, uint32_t n, float learning_rate,
+                            void* stream);
+
 /* ---- synthetic scenes (host-side, deterministic; used by tests and bench) ---- */
+
+/* The procedural box room: five inward-facing Lambertian quads of the box
+ * [-1,1] x [0,2] x [-1,1] (floor, ceiling, back wall grey 0.75; left wall
+ * red 0.63/0.06/0.05; right wall green 0.14/0.45/0.09; the +z side open),
+ * two axis-aligned grey boxes, and a downward-facing ceiling quad light of
+ * radiance 17/12/4 (x, z in [-0.25, 0.25], y = 1.98).  With slit != 0 the
+ * light is enclosed by a box of occluding walls, open only through a slit
+ * of that width on its -z side.  Material 0 grey, 1 red, 2 green, 3 light.
+ * Pass NULL buffers to query the counts.  HOST pointers. */
+int32_t mi_gen_room(float slit, float* positions, uint32_t* num_vertices, uint32_t* indices,
+                    uint32_t* num_triangles, uint32_t* triangle_materials,
+                    mi_material* materials, uint32_t* num_materials);
+
 
 /* Icosphere of num_subdivisions subdivisions of an icosahedron (20 * 4^s
  * triangles, 10 * 4^s + 2 vertices), radius 1, radially displaced by

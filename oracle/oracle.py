@@ -45,6 +45,25 @@ class _OracleBvh(ctypes.Structure):
     ]
 
 
+class _OracleCamera(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("eye", ctypes.c_float * 3), ("target", ctypes.c_float * 3),
+                ("up", ctypes.c_float * 3), ("fov_deg", ctypes.c_float)]
+
+
+class _OraclePtSettings(ctypes.Structure):
+    _fields_ = [("samples_per_pixel", ctypes.c_uint32), ("first_sample", ctypes.c_uint32),
+                ("min_bounces", ctypes.c_uint32), ("max_bounces", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64), ("mode", ctypes.c_int32),
+                ("num_threads", ctypes.c_uint32)]
+
+
+PT_MIS, PT_BSDF, PT_NEE = 0, 1, 2
+
+# same layout as mi_material / oracle_material (40 bytes)
+MATERIAL_DTYPE = np.dtype([("color", "<f4", 3), ("emission", "<f4", 3), ("roughness", "<f4"),
+                           ("metallic", "<f4"), ("ior", "<f4"), ("kind", "<u4")])
+
 _lib = None
 
 
@@ -87,6 +106,11 @@ def lib():
         _lib.oracle_schlick_f0.restype = ctypes.c_float
         _lib.oracle_schlick_f0.argtypes = [ctypes.c_float, ctypes.c_float]
         _lib.oracle_orthonormal_basis.argtypes = [ctypes.c_void_p] * 3
+        _lib.oracle_render.restype = ctypes.c_int32
+        _lib.oracle_render.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                       ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_uint32, ctypes.POINTER(_OracleCamera),
+                                       ctypes.POINTER(_OraclePtSettings), ctypes.c_void_p]
     return _lib
 
 
@@ -317,3 +341,26 @@ def nodes_equal(a: np.ndarray, b: np.ndarray) -> bool:
         and np.array_equal(a["data"], b["data"]) and np.array_equal(a["num"], b["num"])
         and np.array_equal(np.where(a["num"] == 0, a["axis"], 0), np.where(b["num"] == 0, b["axis"], 0))
     )
+
+
+def render(positions, indices, triangle_materials, materials, width, height, spp, camera,
+           first_sample=0, min_bounces=4, max_bounces=8, seed=1, mode=PT_MIS, threads=None):
+    """Radiance sums per pixel, (height, width, 3) float64, of the CPU path
+    tracer (pt.c).  camera: dict with eye, target, up, fov_deg."""
+    pos = np.ascontiguousarray(positions, np.float32)
+    idx = np.ascontiguousarray(indices, np.uint32)
+    tmat = np.ascontiguousarray(triangle_materials, np.uint32)
+    mats = np.ascontiguousarray(materials).view(MATERIAL_DTYPE)
+    cam = _OracleCamera(width, height, (ctypes.c_float * 3)(*camera["eye"]),
+                        (ctypes.c_float * 3)(*camera["target"]),
+                        (ctypes.c_float * 3)(*camera["up"]), camera["fov_deg"])
+    n = threads or min(os.cpu_count() or 1, 16)
+    st = _OraclePtSettings(spp, first_sample, min_bounces, max_bounces, seed, mode, n)
+    accum = np.zeros((height, width, 3), np.float64)
+    status = lib().oracle_render(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(tmat),
+                                 _ptr(mats), len(mats), ctypes.byref(cam), ctypes.byref(st),
+                                 _ptr(accum))
+    if status:
+        raise RuntimeError(f"oracle_render: status {status}")
+    return accum
+

@@ -1,0 +1,548 @@
+/*
+ * ORACLE (test infrastructure only; never linked into the product).
+ * CPU restatement of sprout's MIS path tracer for triangle scenes with
+ * Lambert, Substitute and one-sided constant-light materials:
+ *
+ *   Pathtracer_MIS::integrate         core/rendering/integrator/surface/pathtracer_mis.cpp:95-279
+ *   sample_lights / evaluate_light    pathtracer_mis.cpp:281-347
+ *   connect_light                     pathtracer_mis.cpp:349-380
+ *   power_heuristic, predivided_power_heuristic, russian_roulette
+ *                                     core/rendering/integrator/integrator_helper.hpp:32-52
+ *   Mesh::sample / Mesh::pdf          core/scene/shape/triangle/triangle_mesh.cpp:462-562
+ *   Intersection::offset_p            core/scene/prop/prop_intersection.inl:86-121
+ *   lambert::Iso                      core/scene/material/lambert/lambert.inl:15-45
+ *   rnd::Generator (PCG32)            base/random/generator.inl:16-49
+ *   sample_triangle_uniform           base/math/sampling.inl:39-43
+ *
+ * Paths draw their random numbers from PCG32 started as start(seed,
+ * sample * W * H + y * W + x), in this order: 2 for the pixel jitter; per
+ * vertex 1 for Russian roulette when it applies, 3 for the light sample when
+ * the scene has lights, 3 for the BSDF sample.  Emitters are chosen by area
+ * among all emissive triangles (the reference's single-light tree pick times
+ * its per-part area choice); the pinhole camera jitters uniformly inside the
+ * pixel.  Closest and any hits come from the oracle's own SBVH (sbvh.c,
+ * trace.c), the Substitute material from bsdf.c.
+ *
+ * Besides the reference's estimator (ORACLE_PT_MIS) it has two more that
+ * converge to the same image and so check its weights: ORACLE_PT_BSDF (no
+ * light sampling, every emitter hit counts fully) and ORACLE_PT_NEE (light
+ * sampling weighted 1 / light_pdf, emitters hit by BSDF samples count only at
+ * the first vertex).
+ */
+
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SENTINEL 0xFFFFFFFFu
+#define PI_INV_F 0.318309886183790671538f
+#define DOT_MIN 0.00001f
+#define RAY_MAX_T 3.40282347e+38f
+
+enum { KIND_SUBSTITUTE = 0, KIND_LAMBERT = 1, KIND_LIGHT = 2 };
+enum { TYPE_DIFFUSE = 5, TYPE_SPECULAR = 17 };
+
+typedef struct {
+    uint64_t state;
+    uint64_t inc;
+} pcg;
+
+static uint32_t pcg_next(pcg* g) {
+    uint64_t const old = g->state;
+    g->state           = old * 6364136223846793005ull + (g->inc | 1);
+    uint32_t const xrs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t const rot = (uint32_t)(old >> 59u);
+    return (xrs >> rot) | (xrs << ((0u - rot) & 31u));
+}
+
+static void pcg_start(pcg* g, uint64_t state, uint64_t sequence) {
+    g->state = 0;
+    g->inc   = (sequence << 1u) | 1u;
+    pcg_next(g);
+    g->state += state;
+    pcg_next(g);
+}
+
+static float pcg_float(pcg* g) {
+    uint32_t bits = (pcg_next(g) & 0x007FFFFFu) | 0x3F800000u;
+    float    f;
+    memcpy(&f, &bits, 4);
+    return f - 1.f;
+}
+
+typedef struct {
+    float x, y, z;
+} v3;
+
+static v3 mk(float x, float y, float z) {
+    v3 r = {x, y, z};
+    return r;
+}
+static v3 add(v3 a, v3 b) {
+    return mk(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+static v3 sub(v3 a, v3 b) {
+    return mk(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+static v3 scale(float s, v3 a) {
+    return mk(s * a.x, s * a.y, s * a.z);
+}
+static float dot(v3 a, v3 b) {
+    return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+static v3 cross(v3 a, v3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static v3 normalize(v3 v) {
+    float const s = 1.f / sqrtf(dot(v, v));
+    return scale(s, v);
+}
+static float clampf(float x, float lo, float hi) {
+    return x < lo ? lo : (hi < x ? hi : x);
+}
+
+static v3 offset_ray(v3 p, v3 n) {
+    float const pa[3] = {p.x, p.y, p.z};
+    float const na[3] = {n.x, n.y, n.z};
+    float       o[3];
+    oracle_offset_ray(pa, na, o);
+    return mk(o[0], o[1], o[2]);
+}
+
+typedef struct {
+    float const*            positions;
+    uint32_t const*         indices;
+    uint32_t const*         tri_material;
+    oracle_material const*  materials;
+    oracle_bvh              bvh;
+    uint32_t*               lights;
+    float*                  light_cdf;
+    uint32_t                num_lights;
+    float                   light_area;
+    oracle_camera const*    cam;
+    oracle_pt_settings const* set;
+    v3                      fwd, right, up;
+    float                   tan_half, aspect;
+    double*                 accum;
+    uint32_t                next_row;
+    pthread_mutex_t         lock;
+    int32_t                 status;
+} scene_t;
+
+static v3 vertex(scene_t const* s, uint32_t i) {
+    float const* p = s->positions + 3 * (size_t)i;
+    return mk(p[0], p[1], p[2]);
+}
+
+static void tri_vertices(scene_t const* s, uint32_t t, v3* a, v3* b, v3* c) {
+    *a = vertex(s, s->indices[3 * (size_t)t + 0]);
+    *b = vertex(s, s->indices[3 * (size_t)t + 1]);
+    *c = vertex(s, s->indices[3 * (size_t)t + 2]);
+}
+
+static int32_t closest(scene_t const* s, v3 o, v3 d, float max_t, uint32_t hit[4]) {
+    float const ray[8] = {o.x, o.y, o.z, 0.f, d.x, d.y, d.z, max_t};
+    return oracle_trace(s->bvh.nodes, s->bvh.slots, s->positions, s->indices, ray, 1, 0,
+                        ORACLE_RCP_PINNED, hit, NULL, 1);
+}
+
+static int32_t occluded(scene_t const* s, v3 o, v3 d, float max_t, uint32_t* occ) {
+    float const ray[8] = {o.x, o.y, o.z, 0.f, d.x, d.y, d.z, max_t};
+    return oracle_trace(s->bvh.nodes, s->bvh.slots, s->positions, s->indices, ray, 1, 1,
+                        ORACLE_RCP_PINNED, occ, NULL, 1);
+}
+
+typedef struct {
+    int32_t kind;
+    float   params[6];  /* color, roughness, metallic, ior */
+    float   frame[12];  /* t, b, n, geo_n */
+    v3      n;
+    v3      t, b;
+    v3      wo;
+    float   albedo[3];
+} surface;
+
+static void surface_set(surface* sf, oracle_material const* m, v3 geo_n, v3 wo) {
+    float const na[3] = {geo_n.x, geo_n.y, geo_n.z};
+    float       t[3], b[3];
+    oracle_orthonormal_basis(na, t, b);
+    sf->kind = (int32_t)m->kind;
+    for (int c = 0; c < 3; ++c) {
+        sf->params[c] = m->color[c];
+        sf->albedo[c] = (1.f - m->metallic) * m->color[c];
+    }
+    sf->params[3] = m->roughness;
+    sf->params[4] = m->metallic;
+    sf->params[5] = m->ior;
+    for (int k = 0; k < 3; ++k) {
+        sf->frame[k]     = t[k];
+        sf->frame[3 + k] = b[k];
+        sf->frame[6 + k] = na[k];
+        sf->frame[9 + k] = na[k];
+    }
+    sf->n  = geo_n;
+    sf->t  = mk(t[0], t[1], t[2]);
+    sf->b  = mk(b[0], b[1], b[2]);
+    sf->wo = wo;
+}
+
+/* out: reflection * n.wi (rgb), pdf */
+static void surface_evaluate(surface const* sf, v3 wi, float out[4]) {
+    if (KIND_LAMBERT == sf->kind) {
+        float const n_dot_wi = clampf(dot(sf->n, wi), DOT_MIN, 1.f);
+        for (int c = 0; c < 3; ++c) {
+            out[c] = n_dot_wi * (PI_INV_F * sf->albedo[c]);
+        }
+        out[3] = n_dot_wi * PI_INV_F;
+        return;
+    }
+    float const wo[3] = {sf->wo.x, sf->wo.y, sf->wo.z};
+    float const w[3]  = {wi.x, wi.y, wi.z};
+    oracle_substitute_evaluate(1, sf->params, sf->frame, wo, w, out);
+}
+
+/* out: wi, reflection (rgb), pdf, type */
+static void surface_sample(surface const* sf, float xl, float x0, float x1, float out[8]) {
+    if (KIND_LAMBERT == sf->kind) {
+        float xy[2];
+        oracle_disk_concentric(x0, x1, xy);
+        float const z  = sqrtf(fmaxf(0.f, 1.f - xy[0] * xy[0] - xy[1] * xy[1]));
+        v3 const    tw = mk(xy[0] * sf->t.x + xy[1] * sf->b.x + z * sf->n.x,
+                            xy[0] * sf->t.y + xy[1] * sf->b.y + z * sf->n.y,
+                            xy[0] * sf->t.z + xy[1] * sf->b.z + z * sf->n.z);
+        v3 const    wi = normalize(tw);
+        float const n_dot_wi = clampf(dot(sf->n, wi), DOT_MIN, 1.f);
+        out[0] = wi.x;
+        out[1] = wi.y;
+        out[2] = wi.z;
+        for (int c = 0; c < 3; ++c) {
+            out[3 + c] = n_dot_wi * (PI_INV_F * sf->albedo[c]);
+        }
+        out[6]               = n_dot_wi * PI_INV_F;
+        uint32_t const type = TYPE_DIFFUSE;
+        memcpy(&out[7], &type, 4);
+        return;
+    }
+    float const wo[3] = {sf->wo.x, sf->wo.y, sf->wo.z};
+    float const xi[3] = {xl, x0, x1};
+    oracle_substitute_sample(1, sf->params, sf->frame, wo, xi, out);
+}
+
+static float power_heuristic(float f, float g) {
+    float const f2 = f * f;
+    return f2 / (f2 + g * g);
+}
+
+static float predivided_power_heuristic(float f, float g) {
+    return f / (f * f + g * g);
+}
+
+static uint32_t pick_light(scene_t const* s, float u) {
+    uint32_t lo = 0;
+    uint32_t hi = s->num_lights - 1;
+    while (lo < hi) {
+        uint32_t const mid = (lo + hi) >> 1;
+        if (u < s->light_cdf[mid]) {
+            hi = mid;
+        } else {
+            lo = mid + 1;
+        }
+    }
+    return s->lights[lo];
+}
+
+/* One camera sample; adds its radiance to rgb. */
+static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, double rgb[3]) {
+    oracle_pt_settings const* set = s->set;
+    uint32_t const W = s->cam->width;
+    uint32_t const H = s->cam->height;
+    int32_t const  mode = set->mode;
+
+    pcg rng;
+    pcg_start(&rng, set->seed, (uint64_t)sample * ((uint64_t)W * H) + (uint64_t)y * W + x);
+    float const jx = pcg_float(&rng);
+    float const jy = pcg_float(&rng);
+
+    float const sx = ((2.f * ((float)x + jx)) / (float)W - 1.f) * s->tan_half * s->aspect;
+    float const sy = (1.f - (2.f * ((float)y + jy)) / (float)H) * s->tan_half;
+    v3          o  = mk(s->cam->eye[0], s->cam->eye[1], s->cam->eye[2]);
+    v3          d  = normalize(mk((s->fwd.x + sx * s->right.x) + sy * s->up.x,
+                                  (s->fwd.y + sx * s->right.y) + sy * s->up.y,
+                                  (s->fwd.z + sx * s->right.z) + sy * s->up.z));
+
+    float    throughput[3] = {1.f, 1.f, 1.f};
+    float    radiance[3]   = {0.f, 0.f, 0.f};
+    float    bsdf_pdf      = 0.f;
+    int      singular      = 1;
+
+    for (uint32_t depth = 0;; ++depth) {
+        uint32_t hit[4];
+        if (closest(s, o, d, RAY_MAX_T, hit)) {
+            return ORACLE_ERR_STACK;
+        }
+        if (SENTINEL == hit[3]) {
+            break;
+        }
+        float t, hu, hv;
+        memcpy(&t, &hit[0], 4);
+        memcpy(&hu, &hit[1], 4);
+        memcpy(&hv, &hit[2], 4);
+        uint32_t const tri = hit[3];
+
+        v3 a, b, c;
+        tri_vertices(s, tri, &a, &b, &c);
+        float const bw    = (1.f - hu) - hv;
+        v3 const    hit_p = add(add(scale(bw, a), scale(hu, b)), scale(hv, c));
+        v3 const    geo_n = normalize(cross(sub(b, a), sub(c, a)));
+        v3 const    wo    = mk(-d.x, -d.y, -d.z);
+
+        oracle_material const* m = &s->materials[s->tri_material[tri]];
+
+        if (KIND_LIGHT == m->kind) {
+            float const cos_l = dot(geo_n, wo);
+            if (cos_l > 0.f) {
+                float weight = 1.f;
+                if (0 != depth && !singular) {
+                    if (ORACLE_PT_MIS == mode) {
+                        float const ls_pdf = (t * t) / (cos_l * s->light_area);
+                        weight             = power_heuristic(bsdf_pdf, ls_pdf);
+                    } else if (ORACLE_PT_NEE == mode) {
+                        weight = 0.f;
+                    }
+                }
+                for (int k = 0; k < 3; ++k) {
+                    radiance[k] += throughput[k] * (weight * m->emission[k]);
+                }
+            }
+            break;
+        }
+
+        if (0 != depth) {
+            if (depth >= set->max_bounces) {
+                break;
+            }
+            if (depth >= set->min_bounces) {
+                float const q = fmaxf(throughput[0], fmaxf(throughput[1], throughput[2]));
+                if (pcg_float(&rng) > q) {
+                    break;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    throughput[k] /= q;
+                }
+            }
+        }
+
+        if (!(dot(geo_n, wo) > 0.f)) {
+            break;
+        }
+
+        surface sf;
+        surface_set(&sf, m, geo_n, wo);
+
+        v3 const p_o = offset_ray(hit_p, geo_n);
+        if (s->num_lights) {
+            float const    u0 = pcg_float(&rng);
+            float const    u1 = pcg_float(&rng);
+            float const    u2 = pcg_float(&rng);
+            uint32_t const lt = pick_light(s, u0);
+            if (ORACLE_PT_BSDF != mode) {
+                v3 la, lb, lc;
+                tri_vertices(s, lt, &la, &lb, &lc);
+                float const su = sqrtf(u1);
+                float const lu = 1.f - su;
+                float const lv = u2 * su;
+                float const lw = (1.f - lu) - lv;
+                v3 const    lp = add(add(scale(lw, la), scale(lu, lb)), scale(lv, lc));
+                v3 const    ln = normalize(cross(sub(lb, la), sub(lc, la)));
+
+                v3 const    axis = sub(offset_ray(lp, ln), p_o);
+                float const sl   = dot(axis, axis);
+                float const dist = sqrtf(sl);
+                v3 const    wi   = mk(axis.x / dist, axis.y / dist, axis.z / dist);
+                float const cl   = -dot(ln, wi);
+                if (cl >= DOT_MIN) {
+                    float f[4];
+                    surface_evaluate(&sf, wi, f);
+                    float const light_pdf = sl / (cl * s->light_area);
+                    float const weight    = ORACLE_PT_MIS == mode
+                                                ? predivided_power_heuristic(light_pdf, f[3])
+                                                : 1.f / light_pdf;
+                    oracle_material const* lm = &s->materials[s->tri_material[lt]];
+                    float                  contrib[3];
+                    for (int k = 0; k < 3; ++k) {
+                        contrib[k] = throughput[k] * (weight * (lm->emission[k] * f[k]));
+                    }
+                    if (contrib[0] > 0.f || contrib[1] > 0.f || contrib[2] > 0.f) {
+                        uint32_t occ = 0;
+                        if (occluded(s, p_o, wi, dist, &occ)) {
+                            return ORACLE_ERR_STACK;
+                        }
+                        if (!occ) {
+                            for (int k = 0; k < 3; ++k) {
+                                radiance[k] += contrib[k];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+
+        float const xl = pcg_float(&rng);
+        float const x0 = pcg_float(&rng);
+        float const x1 = pcg_float(&rng);
+        float       bs[8];
+        surface_sample(&sf, xl, x0, x1, bs);
+        if (!(bs[6] > 0.f)) {
+            break;
+        }
+        uint32_t type;
+        memcpy(&type, &bs[7], 4);
+        singular = TYPE_SPECULAR == type;
+        bsdf_pdf = bs[6];
+        for (int k = 0; k < 3; ++k) {
+            throughput[k] *= bs[3 + k] / bs[6];
+        }
+        v3 const wi = mk(bs[0], bs[1], bs[2]);
+        o           = offset_ray(hit_p, dot(geo_n, wi) > 0.f ? geo_n : mk(-geo_n.x, -geo_n.y, -geo_n.z));
+        d           = wi;
+    }
+
+    for (int k = 0; k < 3; ++k) {
+        rgb[k] += (double)radiance[k];
+    }
+    return 0;
+}
+
+static void* render_rows(void* arg) {
+    scene_t* s = (scene_t*)arg;
+    uint32_t const W = s->cam->width;
+    uint32_t const H = s->cam->height;
+    for (;;) {
+        pthread_mutex_lock(&s->lock);
+        uint32_t const y = s->next_row++;
+        pthread_mutex_unlock(&s->lock);
+        if (y >= H) {
+            break;
+        }
+        for (uint32_t x = 0; x < W; ++x) {
+            double rgb[3] = {0.0, 0.0, 0.0};
+            for (uint32_t k = 0; k < s->set->samples_per_pixel; ++k) {
+                if (path(s, x, y, s->set->first_sample + k, rgb)) {
+                    s->status = ORACLE_ERR_STACK;
+                }
+            }
+            double* a = s->accum + 3 * ((size_t)y * W + x);
+            for (int c = 0; c < 3; ++c) {
+                a[c] += rgb[c];
+            }
+        }
+    }
+    return NULL;
+}
+
+int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t const* indices,
+                      uint32_t num_triangles, uint32_t const* tri_material,
+                      oracle_material const* materials, uint32_t num_materials,
+                      oracle_camera const* cam, oracle_pt_settings const* set, double* accum) {
+    if (!positions || !indices || !tri_material || !materials || !cam || !set || !accum ||
+        0 == cam->width || 0 == cam->height || 0 == num_triangles) {
+        return ORACLE_ERR_ARGS;
+    }
+    for (uint32_t t = 0; t < num_triangles; ++t) {
+        if (tri_material[t] >= num_materials) {
+            return ORACLE_ERR_ARGS;
+        }
+    }
+
+    scene_t s;
+    memset(&s, 0, sizeof(s));
+    s.positions    = positions;
+    s.indices      = indices;
+    s.tri_material = tri_material;
+    s.materials    = materials;
+    s.cam          = cam;
+    s.set          = set;
+    s.accum        = accum;
+
+    int32_t const st = oracle_sbvh_build(positions, num_vertices, indices, num_triangles, 16, 64, 4,
+                                         &s.bvh);
+    if (st) {
+        return st;
+    }
+
+    /* emissive triangles and their area CDF (areas in double) */
+    s.lights    = (uint32_t*)malloc(sizeof(uint32_t) * num_triangles);
+    s.light_cdf = (float*)malloc(sizeof(float) * num_triangles);
+    double* areas = (double*)malloc(sizeof(double) * num_triangles);
+    double  total = 0.0;
+    for (uint32_t t = 0; t < num_triangles; ++t) {
+        if (KIND_LIGHT != materials[tri_material[t]].kind) {
+            continue;
+        }
+        float const* a = positions + 3 * (size_t)indices[3 * (size_t)t + 0];
+        float const* b = positions + 3 * (size_t)indices[3 * (size_t)t + 1];
+        float const* c = positions + 3 * (size_t)indices[3 * (size_t)t + 2];
+        double const e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
+        double const e2[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
+        double const x = e1[1] * e2[2] - e1[2] * e2[1];
+        double const y = e1[2] * e2[0] - e1[0] * e2[2];
+        double const z = e1[0] * e2[1] - e1[1] * e2[0];
+        double const area = 0.5 * sqrt(x * x + y * y + z * z);
+        if (area > 0.0) {
+            s.lights[s.num_lights] = t;
+            areas[s.num_lights++]  = area;
+            total += area;
+        }
+    }
+    double run = 0.0;
+    for (uint32_t i = 0; i < s.num_lights; ++i) {
+        run += areas[i];
+        s.light_cdf[i] = i + 1 == s.num_lights ? 1.f : (float)(run / total);
+    }
+    free(areas);
+    s.light_area = (float)total;
+
+    /* camera frame in double, rounded to float: right = fwd x up, up' = right x fwd */
+    double f[3] = {(double)cam->target[0] - cam->eye[0], (double)cam->target[1] - cam->eye[1],
+                   (double)cam->target[2] - cam->eye[2]};
+    double const fl = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (int k = 0; k < 3; ++k) {
+        f[k] /= fl;
+    }
+    double const u0[3] = {cam->up[0], cam->up[1], cam->up[2]};
+    double r[3] = {f[1] * u0[2] - f[2] * u0[1], f[2] * u0[0] - f[0] * u0[2],
+                   f[0] * u0[1] - f[1] * u0[0]};
+    double const rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    for (int k = 0; k < 3; ++k) {
+        r[k] /= rl;
+    }
+    double const up[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2],
+                          r[0] * f[1] - r[1] * f[0]};
+    s.fwd      = mk((float)f[0], (float)f[1], (float)f[2]);
+    s.right    = mk((float)r[0], (float)r[1], (float)r[2]);
+    s.up       = mk((float)up[0], (float)up[1], (float)up[2]);
+    s.tan_half = (float)tan(0.5 * (double)cam->fov_deg * 3.14159265358979323846 / 180.0);
+    s.aspect   = (float)((double)cam->width / (double)cam->height);
+
+    pthread_mutex_init(&s.lock, NULL);
+    uint32_t nt = set->num_threads ? set->num_threads : 1;
+    if (nt > 256) {
+        nt = 256;
+    }
+    pthread_t threads[256];
+    for (uint32_t i = 0; i < nt; ++i) {
+        pthread_create(&threads[i], NULL, render_rows, &s);
+    }
+    for (uint32_t i = 0; i < nt; ++i) {
+        pthread_join(threads[i], NULL);
+    }
+    pthread_mutex_destroy(&s.lock);
+
+    free(s.lights);
+    free(s.light_cdf);
+    oracle_bvh_free(&s.bvh);
+    return s.status;
+}

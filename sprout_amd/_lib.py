@@ -59,6 +59,35 @@ class MiTraceConfig(ctypes.Structure):
                 ("claim_batch", ctypes.c_uint32), ("lds_entries", ctypes.c_uint32)]
 
 
+class MiMaterial(ctypes.Structure):
+    _fields_ = [("color", ctypes.c_float * 3), ("emission", ctypes.c_float * 3),
+                ("roughness", ctypes.c_float), ("metallic", ctypes.c_float),
+                ("ior", ctypes.c_float), ("kind", ctypes.c_uint32)]
+
+
+class MiCamera(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("eye", ctypes.c_float * 3), ("target", ctypes.c_float * 3),
+                ("up", ctypes.c_float * 3), ("fov_deg", ctypes.c_float)]
+
+
+class MiRenderSettings(ctypes.Structure):
+    _fields_ = [("samples_per_pixel", ctypes.c_uint32), ("samples_per_wave", ctypes.c_uint32),
+                ("first_sample", ctypes.c_uint32), ("min_bounces", ctypes.c_uint32),
+                ("max_bounces", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64)]
+
+
+class MiRenderStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("paths", ctypes.c_double),
+                ("extend_rays", ctypes.c_double), ("shadow_rays", ctypes.c_double)]
+
+
+MI_MATERIAL_SUBSTITUTE = 0
+MI_MATERIAL_LAMBERT = 1
+MI_MATERIAL_LIGHT = 2
+MI_RENDER_NO_GRAPH = 1
+
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int32
 _u64 = ctypes.c_uint64
@@ -85,6 +114,22 @@ SIGNATURES = {
                                           _pu32, _vp, _pu32]),
     "mi_gen_camera_rays": (_i32, [_u32, _u32, _vp, _vp, _vp, _f32, _vp]),
     "mi_gen_bounce_rays": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp, _u32]),
+    "mi_scene_create": (_i32, [_vp, _u32, _vp, _u32, _vp, _vp, _u32,
+                               ctypes.POINTER(MiBvhSettings), _pu32]),
+    "mi_scene_release": (_i32, [_u32]),
+    "mi_render": (_i32, [_u32, ctypes.POINTER(MiCamera), ctypes.POINTER(MiRenderSettings), _vp,
+                         _vp, ctypes.POINTER(MiRenderStats)]),
+    "mi_ggx_e_table": (_i32, [_vp, _vp]),
+    "mi_substitute_evaluate": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mi_substitute_sample": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mi_gen_room": (_i32, [_f32, _vp, _pu32, _vp, _pu32, _vp, _vp, _pu32]),
+    "mi_guide_create": (_i32, [_u64, _pu32]),
+    "mi_guide_release": (_i32, [_u32]),
+    "mi_guide_set_params": (_i32, [_u32, _vp]),
+    "mi_guide_get_params": (_i32, [_u32, _vp]),
+    "mi_guide_forward": (_i32, [_u32, _vp, _u32, _vp, _vp]),
+    "mi_guide_gradients": (_i32, [_u32, _vp, _vp, _vp, _u32, _vp, _vp]),
+    "mi_guide_train_step": (_i32, [_u32, _vp, _vp, _vp, _u32, _f32, _vp]),
 }
 
 _lib = None

@@ -2,6 +2,8 @@
 
 #include "bvh_build.hpp"
 #include "common.hpp"
+#include "guide.hpp"
+#include "render.hpp"
 #include "trace.hpp"
 
 #include <hip/hip_runtime.h>
@@ -18,9 +20,14 @@ namespace {
 thread_local char last_error[512] = "";
 
 struct Context {
-    int32_t                 device = -1;
-    TraceResources          trace;
-    std::vector<DeviceBvh*> bvhs;  // handle = index
+    int32_t                   device = -1;
+    TraceResources            trace;
+    std::vector<DeviceBvh*>   bvhs;    // handle = index
+    std::vector<RenderScene*> scenes;  // handle = index
+    std::vector<GuideNet*>    guides;  // handle = index
+    TraceResources            render_trace;  // the renderer's own ray counter and spill area
+    RenderState               render_state;
+    float*                    e_table = nullptr;  // GGX energy table, made on first use
 };
 
 Context* context = nullptr;
@@ -46,6 +53,30 @@ uint32_t store(DeviceBvh const& bvh) {
     }
     context->bvhs.push_back(b);
     return uint32_t(context->bvhs.size() - 1);
+}
+
+RenderScene* lookup_scene(uint32_t handle) {
+    if (!context || handle >= context->scenes.size()) {
+        return nullptr;
+    }
+    return context->scenes[handle];
+}
+
+GuideNet* lookup_guide(uint32_t handle) {
+    if (!context || handle >= context->guides.size()) {
+        return nullptr;
+    }
+    return context->guides[handle];
+}
+
+int32_t energy_table(float const** table) {
+    if (!context->e_table) {
+        if (int32_t const st = make_energy_table(&context->e_table); st) {
+            return st;
+        }
+    }
+    *table = context->e_table;
+    return MI_OK;
 }
 
 }  // namespace
@@ -114,6 +145,21 @@ int32_t mi_release(void) {
             delete b;
         }
     }
+    for (RenderScene* sc : context->scenes) {
+        if (sc) {
+            sc->release();
+            delete sc;
+        }
+    }
+    for (GuideNet* g : context->guides) {
+        if (g) {
+            g->release();
+            delete g;
+        }
+    }
+    context->render_state.release();
+    context->render_trace.release();
+    (void)hipFree(context->e_table);
     context->trace.release();
     delete context;
     context = nullptr;
@@ -310,6 +356,236 @@ int32_t mi_sync(void* stream) {
         return fail(MI_ERR_HIP, "mi_sync: %s", hipGetErrorString(e));
     }
     return MI_OK;
+}
+
+int32_t mi_scene_create(float const* positions, uint32_t num_vertices, uint32_t const* indices,
+                        uint32_t num_triangles, uint32_t const* triangle_materials,
+                        mi_material const* materials, uint32_t num_materials,
+                        mi_bvh_settings const* bvh_settings, uint32_t* scene) {
+    clear_error();
+    if (!context) {
+        return fail(MI_ERR_NO_CONTEXT, "mi_scene_create: call mi_init first");
+    }
+    if (!positions || !indices || !triangle_materials || !materials || !scene ||
+        0 == num_vertices || 0 == num_triangles || 0 == num_materials) {
+        return fail(MI_ERR_ARGS, "mi_scene_create: bad arguments");
+    }
+    mi_bvh_settings const st = bvh_settings ? *bvh_settings : mi_bvh_settings{16, 64, 4};
+    if (0 == st.num_slices || 0 == st.max_primitives || st.max_primitives > 0xFF) {
+        return fail(MI_ERR_ARGS, "mi_scene_create: bad tree settings");
+    }
+    RenderScene out;
+    if (int32_t const r = create_scene(positions, num_vertices, indices, num_triangles,
+                                       triangle_materials, materials, num_materials, st, out);
+        r) {
+        return r;
+    }
+    auto* sc = new RenderScene(out);
+    for (uint32_t i = 0; i < context->scenes.size(); ++i) {
+        if (!context->scenes[i]) {
+            context->scenes[i] = sc;
+            *scene             = i;
+            return MI_OK;
+        }
+    }
+    context->scenes.push_back(sc);
+    *scene = uint32_t(context->scenes.size() - 1);
+    return MI_OK;
+}
+
+int32_t mi_scene_release(uint32_t scene) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT,
+                    "mi_scene_release: bad handle %u", scene);
+    }
+    (void)hipDeviceSynchronize();
+    sc->release();
+    delete sc;
+    context->scenes[scene] = nullptr;
+    return MI_OK;
+}
+
+int32_t mi_render(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
+                  double* accum, void* stream, mi_render_stats* stats) {
+    clear_error();
+    if (!context) {
+        return fail(MI_ERR_NO_CONTEXT, "mi_render: call mi_init first");
+    }
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc) {
+        return fail(MI_ERR_HANDLE, "mi_render: bad scene handle %u", scene);
+    }
+    if (!camera || !settings || !accum) {
+        return fail(MI_ERR_ARGS, "mi_render: bad arguments");
+    }
+    float const* table = nullptr;
+    if (int32_t const st = energy_table(&table); st) {
+        return st;
+    }
+    TraceResources& rt = context->render_trace;
+    if (!rt.counter) {
+        if (hipError_t const e = hipMalloc(&rt.counter, sizeof(uint32_t)); e) {
+            return fail(MI_ERR_HIP, "mi_render: %s", hipGetErrorString(e));
+        }
+    }
+    // launch shape of the traversal as configured by mi_set_trace_config
+    rt.block_size    = context->trace.block_size;
+    rt.blocks_per_cu = context->trace.blocks_per_cu;
+    rt.num_cus       = context->trace.num_cus;
+    rt.kernel        = context->trace.kernel;
+    rt.refill_lanes  = context->trace.refill_lanes;
+    rt.claim_batch   = context->trace.claim_batch;
+    rt.lds_entries   = context->trace.lds_entries;
+    return render(*sc, rt, table, context->render_state, *camera, *settings, accum,
+                  as_stream(stream), stats);
+}
+
+int32_t mi_ggx_e_table(float* table, void* stream) {
+    clear_error();
+    if (!context) {
+        return fail(MI_ERR_NO_CONTEXT, "mi_ggx_e_table: call mi_init first");
+    }
+    if (!table) {
+        return fail(MI_ERR_ARGS, "mi_ggx_e_table: bad arguments");
+    }
+    float const* t = nullptr;
+    if (int32_t const st = energy_table(&t); st) {
+        return st;
+    }
+    if (hipError_t const e = hipMemcpyAsync(table, t, 32 * 32 * sizeof(float),
+                                            hipMemcpyDeviceToDevice, as_stream(stream));
+        e) {
+        return fail(MI_ERR_HIP, "mi_ggx_e_table: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+namespace {
+
+int32_t substitute_batch(bool sample, uint32_t n, float const* params, float const* frames,
+                         float const* wo, float const* w_or_xi, float* out, void* stream) {
+    clear_error();
+    if (!context) {
+        return fail(MI_ERR_NO_CONTEXT, "mi_substitute_*: call mi_init first");
+    }
+    if (n && (!params || !frames || !wo || !w_or_xi || !out)) {
+        return fail(MI_ERR_ARGS, "mi_substitute_*: bad arguments");
+    }
+    float const* t = nullptr;
+    if (int32_t const st = energy_table(&t); st) {
+        return st;
+    }
+    return bsdf_batch(sample, n, params, frames, wo, w_or_xi, t, out, as_stream(stream));
+}
+
+}  // namespace
+
+int32_t mi_substitute_evaluate(uint32_t n, float const* params, float const* frames,
+                               float const* wo, float const* wi, float* out, void* stream) {
+    return substitute_batch(false, n, params, frames, wo, wi, out, stream);
+}
+
+int32_t mi_substitute_sample(uint32_t n, float const* params, float const* frames,
+                             float const* wo, float const* xi, float* out, void* stream) {
+    return substitute_batch(true, n, params, frames, wo, xi, out, stream);
+}
+
+int32_t mi_guide_create(uint64_t seed, uint32_t* guide) {
+    clear_error();
+    if (!context) {
+        return fail(MI_ERR_NO_CONTEXT, "mi_guide_create: call mi_init first");
+    }
+    if (!guide) {
+        return fail(MI_ERR_ARGS, "mi_guide_create: bad arguments");
+    }
+    GuideNet n;
+    if (int32_t const st = guide_create(seed, context->trace.num_cus, n); st) {
+        return st;
+    }
+    auto* g = new GuideNet(n);
+    for (uint32_t i = 0; i < context->guides.size(); ++i) {
+        if (!context->guides[i]) {
+            context->guides[i] = g;
+            *guide             = i;
+            return MI_OK;
+        }
+    }
+    context->guides.push_back(g);
+    *guide = uint32_t(context->guides.size() - 1);
+    return MI_OK;
+}
+
+int32_t mi_guide_release(uint32_t guide) {
+    clear_error();
+    GuideNet* g = lookup_guide(guide);
+    if (!g) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT, "mi_guide_release: bad handle %u",
+                    guide);
+    }
+    (void)hipDeviceSynchronize();
+    g->release();
+    delete g;
+    context->guides[guide] = nullptr;
+    return MI_OK;
+}
+
+int32_t mi_guide_set_params(uint32_t guide, float const* params) {
+    clear_error();
+    GuideNet* g = lookup_guide(guide);
+    if (!g || !params) {
+        return fail(g ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_guide_set_params: bad arguments");
+    }
+    return guide_set_params(*g, params);
+}
+
+int32_t mi_guide_get_params(uint32_t guide, float* params) {
+    clear_error();
+    GuideNet* g = lookup_guide(guide);
+    if (!g || !params) {
+        return fail(g ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_guide_get_params: bad arguments");
+    }
+    (void)hipDeviceSynchronize();
+    return guide_get_params(*g, params);
+}
+
+int32_t mi_guide_forward(uint32_t guide, float const* x, uint32_t n, float* out, void* stream) {
+    clear_error();
+    GuideNet* g = lookup_guide(guide);
+    if (!g || (n && (!x || !out))) {
+        return fail(g ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_guide_forward: bad arguments");
+    }
+    return guide_forward(*g, x, n, out, as_stream(stream));
+}
+
+int32_t mi_guide_gradients(uint32_t guide, float const* x, float const* dirs,
+                           float const* weights, uint32_t n, float* grad, void* stream) {
+    clear_error();
+    GuideNet* g = lookup_guide(guide);
+    if (!g || 0 == n || !x || !dirs || !weights || !grad) {
+        return fail(g ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_guide_gradients: bad arguments");
+    }
+    if (int32_t const st = guide_gradients(*g, x, dirs, weights, n, as_stream(stream)); st) {
+        return st;
+    }
+    if (hipError_t const e = hipMemcpyAsync(grad, g->grad, guide::Num_params * sizeof(float),
+                                            hipMemcpyDeviceToDevice, as_stream(stream));
+        e) {
+        return fail(MI_ERR_HIP, "mi_guide_gradients: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+int32_t mi_guide_train_step(uint32_t guide, float const* x, float const* dirs,
+                            float const* weights, uint32_t n, float learning_rate,
+                            void* stream) {
+    clear_error();
+    GuideNet* g = lookup_guide(guide);
+    if (!g || 0 == n || !x || !dirs || !weights || !(learning_rate > 0.f)) {
+        return fail(g ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_guide_train_step: bad arguments");
+    }
+    return guide_adam_step(*g, x, dirs, weights, n, learning_rate, as_stream(stream));
 }
 
 }  // extern "C"

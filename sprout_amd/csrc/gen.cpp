@@ -371,3 +371,108 @@ extern "C" int32_t mi_gen_bounce_rays(mi_ray const* in_rays, mi_hit const* hits,
     }
     return MI_OK;
 }
+
+namespace {
+
+struct Room {
+    std::vector<float>    positions;
+    std::vector<uint32_t> indices;
+    std::vector<uint32_t> materials;
+
+    // Axis-aligned quad with corners p0..p3 (in order around it), wound so
+    // that cross(b - a, c - a) points along n.
+    void quad(D3 p0, D3 p1, D3 p2, D3 p3, D3 n, uint32_t material) {
+        D3 const e1 = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+        D3 const e2 = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+        D3 const c  = {e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z,
+                       e1.x * e2.y - e1.y * e2.x};
+        bool const flip = c.x * n.x + c.y * n.y + c.z * n.z < 0.0;
+        uint32_t const base = uint32_t(positions.size() / 3);
+        for (D3 const& p : {p0, p1, p2, p3}) {
+            positions.push_back(float(p.x));
+            positions.push_back(float(p.y));
+            positions.push_back(float(p.z));
+        }
+        uint32_t const t[6] = {0, 1, 2, 0, 2, 3};
+        uint32_t const f[6] = {0, 2, 1, 0, 3, 2};
+        for (int k = 0; k < 6; ++k) {
+            indices.push_back(base + (flip ? f[k] : t[k]));
+        }
+        materials.push_back(material);
+        materials.push_back(material);
+    }
+
+    // Box [lo, hi] with outward faces.
+    void box(D3 lo, D3 hi, uint32_t material) {
+        quad({lo.x, lo.y, lo.z}, {hi.x, lo.y, lo.z}, {hi.x, lo.y, hi.z}, {lo.x, lo.y, hi.z}, {0, -1, 0}, material);
+        quad({lo.x, hi.y, lo.z}, {hi.x, hi.y, lo.z}, {hi.x, hi.y, hi.z}, {lo.x, hi.y, hi.z}, {0, 1, 0}, material);
+        quad({lo.x, lo.y, lo.z}, {lo.x, hi.y, lo.z}, {lo.x, hi.y, hi.z}, {lo.x, lo.y, hi.z}, {-1, 0, 0}, material);
+        quad({hi.x, lo.y, lo.z}, {hi.x, hi.y, lo.z}, {hi.x, hi.y, hi.z}, {hi.x, lo.y, hi.z}, {1, 0, 0}, material);
+        quad({lo.x, lo.y, lo.z}, {hi.x, lo.y, lo.z}, {hi.x, hi.y, lo.z}, {lo.x, hi.y, lo.z}, {0, 0, -1}, material);
+        quad({lo.x, lo.y, hi.z}, {hi.x, lo.y, hi.z}, {hi.x, hi.y, hi.z}, {lo.x, hi.y, hi.z}, {0, 0, 1}, material);
+    }
+};
+
+Room make_room(float slit) {
+    Room r;
+    // floor, ceiling, back wall (grey), left (red), right (green); +z open
+    r.quad({-1, 0, -1}, {1, 0, -1}, {1, 0, 1}, {-1, 0, 1}, {0, 1, 0}, 0);
+    r.quad({-1, 2, -1}, {1, 2, -1}, {1, 2, 1}, {-1, 2, 1}, {0, -1, 0}, 0);
+    r.quad({-1, 0, -1}, {1, 0, -1}, {1, 2, -1}, {-1, 2, -1}, {0, 0, 1}, 0);
+    r.quad({-1, 0, -1}, {-1, 2, -1}, {-1, 2, 1}, {-1, 0, 1}, {1, 0, 0}, 1);
+    r.quad({1, 0, -1}, {1, 2, -1}, {1, 2, 1}, {1, 0, 1}, {-1, 0, 0}, 2);
+    // a tall and a short box
+    r.box({-0.65, 0.0, -0.65}, {-0.15, 1.2, -0.15}, 0);
+    r.box({0.15, 0.0, -0.2}, {0.65, 0.6, 0.3}, 0);
+    // ceiling light, facing down
+    r.quad({-0.25, 1.98, -0.25}, {0.25, 1.98, -0.25}, {0.25, 1.98, 0.25}, {-0.25, 1.98, 0.25},
+           {0, -1, 0}, 3);
+    if (slit > 0.f) {
+        // enclosure below the light: bottom plate and side walls facing out,
+        // the -z wall open for y in [1.7, 1.7 + slit]
+        double const lo = 1.7;
+        double const s  = 0.3;
+        r.quad({-s, lo, -s}, {s, lo, -s}, {s, lo, s}, {-s, lo, s}, {0, -1, 0}, 0);
+        r.quad({-s, lo, -s}, {-s, 2, -s}, {-s, 2, s}, {-s, lo, s}, {-1, 0, 0}, 0);
+        r.quad({s, lo, -s}, {s, 2, -s}, {s, 2, s}, {s, lo, s}, {1, 0, 0}, 0);
+        r.quad({-s, lo, s}, {s, lo, s}, {s, 2, s}, {-s, 2, s}, {0, 0, 1}, 0);
+        double const top = lo + double(slit);
+        r.quad({-s, top, -s}, {s, top, -s}, {s, 2, -s}, {-s, 2, -s}, {0, 0, -1}, 0);
+    }
+    return r;
+}
+
+}  // namespace
+
+extern "C" int32_t mi_gen_room(float slit, float* positions, uint32_t* num_vertices,
+                               uint32_t* indices, uint32_t* num_triangles,
+                               uint32_t* triangle_materials, mi_material* materials,
+                               uint32_t* num_materials) {
+    if (!num_vertices || !num_triangles || !num_materials || !(slit >= 0.f) || !(slit < 0.3f)) {
+        return mi::fail(MI_ERR_ARGS, "mi_gen_room: bad arguments");
+    }
+    Room const r = make_room(slit);
+    *num_vertices  = uint32_t(r.positions.size() / 3);
+    *num_triangles = uint32_t(r.indices.size() / 3);
+    *num_materials = 4;
+    if (positions) {
+        std::memcpy(positions, r.positions.data(), r.positions.size() * sizeof(float));
+    }
+    if (indices) {
+        std::memcpy(indices, r.indices.data(), r.indices.size() * sizeof(uint32_t));
+    }
+    if (triangle_materials) {
+        std::memcpy(triangle_materials, r.materials.data(), r.materials.size() * sizeof(uint32_t));
+    }
+    if (materials) {
+        mi_material const grey  = {{0.75f, 0.75f, 0.75f}, {0, 0, 0}, 1.f, 0.f, 1.46f, MI_MATERIAL_LAMBERT};
+        mi_material const red   = {{0.63f, 0.06f, 0.05f}, {0, 0, 0}, 1.f, 0.f, 1.46f, MI_MATERIAL_LAMBERT};
+        mi_material const green = {{0.14f, 0.45f, 0.09f}, {0, 0, 0}, 1.f, 0.f, 1.46f, MI_MATERIAL_LAMBERT};
+        mi_material const light = {{0, 0, 0}, {17.f, 12.f, 4.f}, 1.f, 0.f, 1.f, MI_MATERIAL_LIGHT};
+        materials[0] = grey;
+        materials[1] = red;
+        materials[2] = green;
+        materials[3] = light;
+    }
+    return MI_OK;
+}

@@ -1,0 +1,1063 @@
+// Wavefront MIS path tracer (sprout's Pathtracer_MIS) on gfx950.
+//
+//   Pathtracer_MIS::integrate        core/rendering/integrator/surface/pathtracer_mis.cpp:95-279
+//   sample_lights / evaluate_light   pathtracer_mis.cpp:281-347
+//   connect_light                    pathtracer_mis.cpp:349-380
+//   power_heuristic, predivided_power_heuristic, russian_roulette
+//                                    core/rendering/integrator/integrator_helper.hpp:32-52
+//   Mesh::sample / Mesh::pdf (area light, solid angle pdf d^2 / (cos * area))
+//                                    core/scene/shape/triangle/triangle_mesh.cpp:462-562
+//   Intersection::offset_p, evaluate_radiance
+//                                    core/scene/prop/prop_intersection.inl:67-121
+//   offset_ray                       core/scene/ray_offset.inl:18-29
+//   rnd::Generator (PCG32)           base/random/generator.inl:16-49
+//
+// A wave is samples_per_wave samples of every pixel, one path per (sample
+// plane, pixel).  Per bounce:
+//   trace   closest hit of the live rays (the BVH kernel; the ray count is
+//           read on the device)
+//   shade   one thread per live path: emission at the hit (MIS-weighted past
+//           the first vertex), the depth limit, Russian roulette, next-event
+//           estimation (one light sample; the shadow ray is queued with its
+//           contribution), BSDF sample, next ray queued.  Queues are
+//           compacted with a wave ballot and one atomic per wave.
+//   shadow  any hit of the queued shadow rays; unoccluded contributions are
+//           added to their path's radiance.
+// Every path owns its radiance, so no two threads add into one word, and at
+// the end of a wave the planes are summed per pixel in plane order into the
+// caller's FP64 accumulator: results do not depend on scheduling.
+//
+// Random numbers of a path, in this order (shared with the CPU restatement
+// that checks this file): 2 for the pixel jitter; per vertex 1 for Russian
+// roulette when it applies, 3 for the light sample (light choice, point on
+// the triangle) when the scene has lights, 3 for the BSDF sample (lobe
+// choice, 2-D direction sample).
+//
+// The bounce loop has a fixed shape (max_bounces + 1 iterations whose kernels
+// read their counts on the device and whose sample base lives in device
+// memory), so one wave is captured into a hipGraph and replayed per wave.
+
+#include "bsdf.hpp"
+#include "bvh_build.hpp"
+#include "common.hpp"
+#include "device_math.hpp"
+#include "render.hpp"
+#include "trace.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace mi {
+
+namespace {
+
+using dev::V3;
+using dev::v3;
+
+constexpr float Ray_max_t = 3.40282347e+38f;
+
+// Counter words of RenderState::counters
+constexpr uint32_t C_queue0     = 0;  // live rays, two ping-pong queues
+constexpr uint32_t C_shadow     = 2;  // shadow rays
+constexpr uint32_t C_base       = 4;  // first sample index of the current wave
+constexpr uint32_t C_totals     = 8;  // two uint64: extension and shadow rays traced
+constexpr uint32_t Num_counters = 16;
+
+// rnd::Generator, base/random/generator.inl:16-49
+struct Rng {
+    uint64_t state;
+    uint64_t inc;
+
+    __device__ void start(uint64_t s, uint64_t sequence) {
+        state = 0;
+        inc   = (sequence << 1u) | 1u;
+        next();
+        state += s;
+        next();
+    }
+
+    __device__ uint32_t next() {
+        uint64_t const old = state;
+        state              = old * 6364136223846793005ull + (inc | 1);
+        uint32_t const xrs = uint32_t(((old >> 18u) ^ old) >> 27u);
+        uint32_t const rot = uint32_t(old >> 59u);
+        return (xrs >> rot) | (xrs << ((0u - rot) & 31u));
+    }
+
+    __device__ float uniform() {
+        uint32_t const bits = (next() & 0x007FFFFFu) | 0x3F800000u;
+        return __uint_as_float(bits) - 1.f;
+    }
+};
+
+constexpr uint32_t Flag_singular = 1;  // State::Treat_as_singular
+
+struct Path {
+    float    throughput[3];
+    float    radiance[3];
+    float    bsdf_pdf;  // pdf of the BSDF sample that spawned the current ray
+    uint32_t flags;
+    Rng      rng;
+};
+
+struct Shadow {
+    float    contrib[3];
+    uint32_t path;
+};
+
+struct Wave {
+    // scene
+    float const*           positions;
+    uint32_t const*        indices;
+    uint32_t const*        tri_material;
+    Render_material const* materials;
+    uint32_t const*        light_tris;
+    float const*           light_cdf;
+    uint32_t               num_lights;
+    float                  light_area;
+    float const*           e_table;
+    // camera
+    float    eye[3];
+    float    fwd[3];
+    float    right[3];
+    float    up[3];
+    float    tan_half;
+    float    aspect;
+    uint32_t width;
+    uint32_t height;
+    // settings
+    uint32_t min_bounces;
+    uint32_t max_bounces;
+    uint64_t seed;
+    uint32_t num_paths;
+    uint32_t num_pixels;
+    // state
+    Path*     paths;
+    mi_ray*   rays[2];
+    uint32_t* ray_path[2];
+    mi_hit*   hits;
+    mi_ray*   shadow_rays;
+    Shadow*   shadows;
+    uint32_t* occluded;
+    uint32_t* counters;
+};
+
+__device__ __forceinline__ float offset_bits(float p, int32_t o) {
+    return __int_as_float(__float_as_int(p) + (p < 0.f ? -o : o));
+}
+
+// offset_ray, core/scene/ray_offset.inl:18-29
+__device__ __forceinline__ V3 offset_ray(V3 p, V3 n) {
+    float const   origin      = 1.f / 32.f;
+    float const   float_scale = 1.f / 65536.f;
+    float const   int_scale   = 256.f;
+    int32_t const ox          = int32_t(int_scale * n.x);
+    int32_t const oy          = int32_t(int_scale * n.y);
+    int32_t const oz          = int32_t(int_scale * n.z);
+    return {fabsf(p.x) < origin ? __builtin_fmaf(float_scale, n.x, p.x) : offset_bits(p.x, ox),
+            fabsf(p.y) < origin ? __builtin_fmaf(float_scale, n.y, p.y) : offset_bits(p.y, oy),
+            fabsf(p.z) < origin ? __builtin_fmaf(float_scale, n.z, p.z) : offset_bits(p.z, oz)};
+}
+
+__device__ __forceinline__ V3 vertex(Wave const& w, uint32_t i) {
+    float const* p = w.positions + 3 * size_t(i);
+    return {p[0], p[1], p[2]};
+}
+
+__device__ __forceinline__ float max_component(float const t[3]) {
+    return fmaxf(t[0], fmaxf(t[1], t[2]));
+}
+
+// power_heuristic, integrator_helper.hpp:32-35
+__device__ __forceinline__ float power_heuristic(float f, float g) {
+    float const f2 = f * f;
+    return f2 / (f2 + g * g);
+}
+
+// predivided_power_heuristic, integrator_helper.hpp:38-40
+__device__ __forceinline__ float predivided_power_heuristic(float f, float g) {
+    return f / (f * f + g * g);
+}
+
+// Appends one entry per requesting lane to a queue: one atomic per wave.
+// Every lane of the wave must call it.
+__device__ __forceinline__ uint32_t queue_slot(bool want, uint32_t* counter) {
+    uint64_t const mask = __ballot(want);
+    uint32_t const lane = threadIdx.x & 63;
+    if (0 == mask) {
+        return 0;
+    }
+    uint32_t const leader = uint32_t(__ffsll(static_cast<unsigned long long>(mask))) - 1;
+    uint32_t       base   = 0;
+    if (lane == leader) {
+        base = atomicAdd(counter, uint32_t(__popcll(mask)));
+    }
+    base = __shfl(base, leader);
+    return base + uint32_t(__popcll(mask & ((uint64_t(1) << lane) - 1)));
+}
+
+// The material of a hit, Lambert or Substitute (both one-sided).
+struct Surface {
+    dev::Material_closure sub;
+    uint32_t              kind;
+
+    __device__ void evaluate(V3 wi, float out[4]) const {
+        if (MI_MATERIAL_LAMBERT == kind) {
+            // lambert::Iso::reflection times n.wi, lambert.inl:15-26
+            float const n_dot_wi = dev::clamp_dot(sub.layer.n, wi);
+            for (int c = 0; c < 3; ++c) {
+                out[c] = n_dot_wi * (dev::Pi_inv * sub.albedo[c]);
+            }
+            out[3] = n_dot_wi * dev::Pi_inv;
+            return;
+        }
+        sub.evaluate(wi, out);
+    }
+
+    __device__ void sample(float xl, float x0, float x1, dev::Bxdf_sample& s) const {
+        if (MI_MATERIAL_LAMBERT == kind) {
+            // lambert::Iso::reflect, lambert.inl:28-45
+            V3 const    is       = dev::hemisphere_cosine(x0, x1);
+            V3 const    wi       = dev::normalize(sub.layer.tangent_to_world(is));
+            float const n_dot_wi = dev::clamp_dot(sub.layer.n, wi);
+            for (int c = 0; c < 3; ++c) {
+                s.refl[c] = n_dot_wi * (dev::Pi_inv * sub.albedo[c]);
+            }
+            s.wi   = wi;
+            s.pdf  = n_dot_wi * dev::Pi_inv;
+            s.type = dev::Type_diffuse;
+            return;
+        }
+        sub.sample(xl, x0, x1, s);
+    }
+};
+
+__global__ void set_base(uint32_t* counters, uint32_t first_sample) {
+    counters[C_base] = first_sample;
+    for (uint32_t k = 0; k < 4; ++k) {
+        counters[C_totals + k] = 0;
+    }
+}
+
+__global__ void advance_base(uint32_t* counters, uint32_t samples) {
+    counters[C_base] += samples;
+}
+
+__global__ void __launch_bounds__(256) raygen(Wave w) {
+    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= w.num_paths) {
+        return;
+    }
+    uint32_t const pixel  = p % w.num_pixels;
+    uint32_t const plane  = p / w.num_pixels;
+    uint32_t const sample = w.counters[C_base] + plane;
+    uint32_t const x      = pixel % w.width;
+    uint32_t const y      = pixel / w.width;
+
+    Path path;
+    path.rng.start(w.seed, uint64_t(sample) * w.num_pixels + pixel);
+    float const jx = path.rng.uniform();
+    float const jy = path.rng.uniform();
+
+    float const sx = ((2.f * (float(x) + jx)) / float(w.width) - 1.f) * w.tan_half * w.aspect;
+    float const sy = (1.f - (2.f * (float(y) + jy)) / float(w.height)) * w.tan_half;
+    V3 const    d  = dev::normalize(v3((w.fwd[0] + sx * w.right[0]) + sy * w.up[0],
+                                       (w.fwd[1] + sx * w.right[1]) + sy * w.up[1],
+                                       (w.fwd[2] + sx * w.right[2]) + sy * w.up[2]));
+
+    for (int c = 0; c < 3; ++c) {
+        path.throughput[c] = 1.f;
+        path.radiance[c]   = 0.f;
+    }
+    path.bsdf_pdf = 0.f;
+    path.flags    = Flag_singular;
+    w.paths[p]    = path;
+
+    mi_ray r;
+    r.origin[0]      = w.eye[0];
+    r.origin[1]      = w.eye[1];
+    r.origin[2]      = w.eye[2];
+    r.min_t          = 0.f;
+    r.direction[0]   = d.x;
+    r.direction[1]   = d.y;
+    r.direction[2]   = d.z;
+    r.max_t          = Ray_max_t;
+    w.rays[0][p]     = r;
+    w.ray_path[0][p] = p;
+
+    if (0 == p) {
+        w.counters[C_queue0]     = w.num_paths;
+        w.counters[C_queue0 + 1] = 0;
+        w.counters[C_shadow]     = 0;
+    }
+}
+
+// Clears the queue the next shade fills and the shadow queue.
+__global__ void reset_queues(uint32_t* counters, uint32_t out_q) {
+    counters[C_queue0 + out_q] = 0;
+    counters[C_shadow]         = 0;
+}
+
+// Ray totals for the statistics: the queue just traced and the shadow queue.
+__global__ void tally(uint32_t* counters, uint32_t in_q) {
+    uint64_t* totals = reinterpret_cast<uint64_t*>(counters + C_totals);
+    totals[0] += counters[C_queue0 + in_q];
+    totals[1] += counters[C_shadow];
+}
+
+// Emissive triangle by area (first CDF entry above u).
+__device__ __forceinline__ uint32_t pick_light(Wave const& w, float u) {
+    uint32_t lo = 0;
+    uint32_t hi = w.num_lights - 1;
+    while (lo < hi) {
+        uint32_t const mid = (lo + hi) >> 1;
+        if (u < w.light_cdf[mid]) {
+            hi = mid;
+        } else {
+            lo = mid + 1;
+        }
+    }
+    return w.light_tris[lo];
+}
+
+__global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t depth) {
+    uint32_t const count  = w.counters[C_queue0 + in_q];
+    uint32_t const out_q  = in_q ^ 1;
+    uint32_t const stride = gridDim.x * blockDim.x;
+
+    // block-uniform trip count: every lane reaches the wave-wide queue appends
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        uint32_t const i = base + threadIdx.x;
+
+        bool     spawn  = false;
+        bool     shadow = false;
+        uint32_t p      = 0;
+        mi_ray   next_ray;
+        mi_ray   shadow_ray;
+        Shadow   sh;
+
+        if (i < count) {
+            p              = w.ray_path[in_q][i];
+            Path     path  = w.paths[p];
+            mi_hit const h = w.hits[i];
+
+            if (dev::Null != h.triangle) {
+                mi_ray const   r   = w.rays[in_q][i];
+                uint32_t const tri = h.triangle;
+
+                V3 const    a     = vertex(w, w.indices[3 * size_t(tri) + 0]);
+                V3 const    b     = vertex(w, w.indices[3 * size_t(tri) + 1]);
+                V3 const    c     = vertex(w, w.indices[3 * size_t(tri) + 2]);
+                float const bw    = (1.f - h.u) - h.v;
+                V3 const    hit_p = (bw * a + h.u * b) + h.v * c;
+                V3 const    geo_n = dev::normalize(dev::cross(b - a, c - a));
+                V3 const    wo    = v3(-r.direction[0], -r.direction[1], -r.direction[2]);
+
+                Render_material const m = w.materials[w.tri_material[tri]];
+
+                bool live = true;
+
+                if (MI_MATERIAL_LIGHT == m.kind) {
+                    // first vertex: mat_sample.radiance(); later: connect_light
+                    float const cos_l = dev::dot(geo_n, wo);
+                    if (cos_l > 0.f) {
+                        float weight = 1.f;
+                        if (0 != depth && !(path.flags & Flag_singular)) {
+                            float const ls_pdf = (h.t * h.t) / (cos_l * w.light_area);
+                            weight             = power_heuristic(path.bsdf_pdf, ls_pdf);
+                        }
+                        for (int k = 0; k < 3; ++k) {
+                            path.radiance[k] += path.throughput[k] * (weight * m.emission[k]);
+                        }
+                    }
+                    live = false;
+                } else if (0 != depth) {
+                    if (depth >= w.max_bounces) {
+                        live = false;
+                    } else if (depth >= w.min_bounces) {
+                        // russian_roulette, integrator_helper.hpp:42-52
+                        float const q = max_component(path.throughput);
+                        if (path.rng.uniform() > q) {
+                            live = false;
+                        } else {
+                            for (int k = 0; k < 3; ++k) {
+                                path.throughput[k] /= q;
+                            }
+                        }
+                    }
+                }
+
+                if (live && !(dev::dot(geo_n, wo) > 0.f)) {
+                    live = false;  // back face: evaluate and sample give nothing
+                }
+
+                if (live) {
+                    Surface s;
+                    s.kind = m.kind;
+                    s.sub.set(dev::frame_from_normal(geo_n), geo_n, wo, m.color, m.roughness,
+                              m.metallic, m.ior, 1.f, w.e_table);
+
+                    // next-event estimation (sample_lights / evaluate_light)
+                    V3 const p_o = offset_ray(hit_p, geo_n);
+                    if (w.num_lights) {
+                        float const    u0 = path.rng.uniform();
+                        float const    u1 = path.rng.uniform();
+                        float const    u2 = path.rng.uniform();
+                        uint32_t const lt = pick_light(w, u0);
+
+                        V3 const la = vertex(w, w.indices[3 * size_t(lt) + 0]);
+                        V3 const lb = vertex(w, w.indices[3 * size_t(lt) + 1]);
+                        V3 const lc = vertex(w, w.indices[3 * size_t(lt) + 2]);
+                        // sample_triangle_uniform, base/math/sampling.inl:39-43
+                        float const su = sqrtf(u1);
+                        float const lu = 1.f - su;
+                        float const lv = u2 * su;
+                        float const lw = (1.f - lu) - lv;
+                        V3 const    lp = (lw * la + lu * lb) + lv * lc;
+                        V3 const    ln = dev::normalize(dev::cross(lb - la, lc - la));
+
+                        V3 const    axis = offset_ray(lp, ln) - p_o;
+                        float const sl   = dev::dot(axis, axis);
+                        float const dist = sqrtf(sl);
+                        V3 const    wi   = v3(axis.x / dist, axis.y / dist, axis.z / dist);
+                        float const cl   = -dev::dot(ln, wi);
+                        if (cl >= dev::Dot_min) {
+                            float f[4];
+                            s.evaluate(wi, f);
+                            float const light_pdf = sl / (cl * w.light_area);
+                            float const weight    = predivided_power_heuristic(light_pdf, f[3]);
+                            Render_material const lm = w.materials[w.tri_material[lt]];
+                            for (int k = 0; k < 3; ++k) {
+                                sh.contrib[k] =
+                                    path.throughput[k] * (weight * (lm.emission[k] * f[k]));
+                            }
+                            sh.path                 = p;
+                            shadow_ray.origin[0]    = p_o.x;
+                            shadow_ray.origin[1]    = p_o.y;
+                            shadow_ray.origin[2]    = p_o.z;
+                            shadow_ray.min_t        = 0.f;
+                            shadow_ray.direction[0] = wi.x;
+                            shadow_ray.direction[1] = wi.y;
+                            shadow_ray.direction[2] = wi.z;
+                            shadow_ray.max_t        = dist;
+                            shadow = sh.contrib[0] > 0.f || sh.contrib[1] > 0.f ||
+                                     sh.contrib[2] > 0.f;
+                        }
+                    }
+
+                    // BSDF importance sample
+                    float const      xl = path.rng.uniform();
+                    float const      x0 = path.rng.uniform();
+                    float const      x1 = path.rng.uniform();
+                    dev::Bxdf_sample bs;
+                    s.sample(xl, x0, x1, bs);
+                    if (!(bs.pdf > 0.f)) {
+                        live = false;
+                    } else {
+                        if (dev::Type_specular == bs.type) {
+                            path.flags |= Flag_singular;
+                        } else {
+                            path.flags &= ~Flag_singular;
+                        }
+                        path.bsdf_pdf = bs.pdf;
+                        for (int k = 0; k < 3; ++k) {
+                            path.throughput[k] *= bs.refl[k] / bs.pdf;
+                        }
+                        V3 const o =
+                            offset_ray(hit_p, dev::dot(geo_n, bs.wi) > 0.f ? geo_n : -geo_n);
+                        next_ray.origin[0]    = o.x;
+                        next_ray.origin[1]    = o.y;
+                        next_ray.origin[2]    = o.z;
+                        next_ray.min_t        = 0.f;
+                        next_ray.direction[0] = bs.wi.x;
+                        next_ray.direction[1] = bs.wi.y;
+                        next_ray.direction[2] = bs.wi.z;
+                        next_ray.max_t        = Ray_max_t;
+                        spawn                 = true;
+                    }
+                }
+            }
+            w.paths[p] = path;
+        }
+
+        uint32_t const ray_slot = queue_slot(spawn, &w.counters[C_queue0 + out_q]);
+        if (spawn) {
+            w.rays[out_q][ray_slot]     = next_ray;
+            w.ray_path[out_q][ray_slot] = p;
+        }
+        uint32_t const sh_slot = queue_slot(shadow, &w.counters[C_shadow]);
+        if (shadow) {
+            w.shadow_rays[sh_slot] = shadow_ray;
+            w.shadows[sh_slot]     = sh;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) shadow_resolve(Wave w) {
+    uint32_t const count  = w.counters[C_shadow];
+    uint32_t const stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        if (w.occluded[i]) {
+            continue;
+        }
+        Shadow const s = w.shadows[i];
+        Path&        p = w.paths[s.path];
+        for (int k = 0; k < 3; ++k) {
+            p.radiance[k] += s.contrib[k];
+        }
+    }
+}
+
+// Sums the wave's sample planes per pixel, in plane order, into the FP64
+// accumulator (rgb per pixel).
+__global__ void __launch_bounds__(256) resolve(Wave w, uint32_t planes, double* accum) {
+    uint32_t const pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= w.num_pixels) {
+        return;
+    }
+    double s[3] = {0.0, 0.0, 0.0};
+    for (uint32_t k = 0; k < planes; ++k) {
+        Path const& p = w.paths[size_t(k) * w.num_pixels + pixel];
+        for (int c = 0; c < 3; ++c) {
+            s[c] += double(p.radiance[c]);
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        accum[3 * size_t(pixel) + c] += s[c];
+    }
+}
+
+// ---- energy table: sprout's procedure (ggx_integrate.cpp:17-45, 202-240) ----
+
+__device__ float radical_inverse_vdc(uint32_t bits) {
+    bits = (bits << 16) | (bits >> 16);
+    bits = ((bits & 0x55555555u) << 1) | ((bits & 0xAAAAAAAAu) >> 1);
+    bits = ((bits & 0x33333333u) << 2) | ((bits & 0xCCCCCCCCu) >> 2);
+    bits = ((bits & 0x0F0F0F0Fu) << 4) | ((bits & 0xF0F0F0F0u) >> 4);
+    bits = ((bits & 0x00FF00FFu) << 8) | ((bits & 0xFF00FF00u) >> 8);
+    return float(bits) * 2.3283064365386963e-10f;
+}
+
+// One workgroup per table entry; the 1024 sample terms are summed in order
+// by one lane, as the reference's loop does.
+__global__ void __launch_bounds__(256) energy_table(float* table) {
+    __shared__ float terms[1024];
+
+    uint32_t const entry = blockIdx.x;
+    uint32_t const ai    = entry / dev::E_size;
+    uint32_t const ni    = entry % dev::E_size;
+
+    // grid coordinates by float accumulation of the step, as generated
+    float const step  = 1.f / float(dev::E_size - 1);
+    float       alpha = 0.f;
+    for (uint32_t k = 0; k < ai; ++k) {
+        alpha += step;
+    }
+    float n_dot_wo = 0.f;
+    for (uint32_t k = 0; k < ni; ++k) {
+        n_dot_wo += step;
+    }
+
+    if (alpha < dev::Min_rough * dev::Min_rough) {
+        if (0 == threadIdx.x) {
+            table[entry] = 1.f;
+        }
+        return;
+    }
+
+    n_dot_wo    = dev::clampf(n_dot_wo, 0.f, 1.f);
+    V3 const wo = v3(sqrtf(1.f - n_dot_wo * n_dot_wo), 0.f, n_dot_wo);
+
+    dev::Material_closure m;
+    m.layer.t  = v3(1.f, 0.f, 0.f);
+    m.layer.b  = v3(0.f, 1.f, 0.f);
+    m.layer.n  = v3(0.f, 0.f, 1.f);
+    m.geo_n    = m.layer.n;
+    m.wo       = wo;
+    m.alpha    = alpha;
+    m.metallic = 1.f;
+    m.e_table  = nullptr;
+    for (int c = 0; c < 3; ++c) {
+        m.f0[c]     = 1.f;
+        m.albedo[c] = 0.f;
+    }
+
+    for (uint32_t i = threadIdx.x; i < 1024; i += blockDim.x) {
+        dev::Bxdf_sample s;
+        float const n_dot_wi = m.ggx_reflect(n_dot_wo, float(i) / 1024.f, radical_inverse_vdc(i), s);
+        terms[i]             = (n_dot_wi * s.refl[0]) / s.pdf;
+    }
+    __syncthreads();
+    if (0 == threadIdx.x) {
+        float accum = 0.f;
+        for (uint32_t i = 0; i < 1024; ++i) {
+            accum += terms[i];
+        }
+        table[entry] = accum / 1024.f;
+    }
+}
+
+__device__ dev::Material_closure batch_closure(uint32_t i, float const* params, float const* frames,
+                                               float const* wo, float const* table) {
+    float const* f = frames + 12 * size_t(i);
+    float const* q = params + 6 * size_t(i);
+    dev::Layer   l;
+    l.t = v3(f[0], f[1], f[2]);
+    l.b = v3(f[3], f[4], f[5]);
+    l.n = v3(f[6], f[7], f[8]);
+    dev::Material_closure m;
+    m.set(l, v3(f[9], f[10], f[11]), v3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), q, q[3], q[4],
+          q[5], 1.f, table);
+    return m;
+}
+
+__global__ void bsdf_evaluate_batch(uint32_t n, float const* params, float const* frames,
+                                    float const* wo, float const* wi, float const* table,
+                                    float* out) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    dev::Material_closure const m = batch_closure(i, params, frames, wo, table);
+    m.evaluate(v3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), out + 4 * size_t(i));
+}
+
+__global__ void bsdf_sample_batch(uint32_t n, float const* params, float const* frames,
+                                  float const* wo, float const* xi, float const* table,
+                                  float* out) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    dev::Material_closure const m = batch_closure(i, params, frames, wo, table);
+    dev::Bxdf_sample            s;
+    m.sample(xi[3 * i], xi[3 * i + 1], xi[3 * i + 2], s);
+    float* o = out + 8 * size_t(i);
+    if (!(s.pdf > 0.f)) {
+        for (int k = 0; k < 8; ++k) {
+            o[k] = 0.f;
+        }
+        return;
+    }
+    o[0] = s.wi.x;
+    o[1] = s.wi.y;
+    o[2] = s.wi.z;
+    o[3] = s.refl[0];
+    o[4] = s.refl[1];
+    o[5] = s.refl[2];
+    o[6] = s.pdf;
+    o[7] = __uint_as_float(s.type);
+}
+
+uint32_t blocks(uint64_t n, uint32_t b) {
+    uint64_t const g = (n + b - 1) / b;
+    return uint32_t(g > 0 ? g : 1);
+}
+
+bool is_light(mi_material const& m) {
+    return MI_MATERIAL_LIGHT == m.kind;
+}
+
+}  // namespace
+
+int32_t make_energy_table(float** table) {
+    float* t = nullptr;
+    if (hipError_t const e = hipMalloc(&t, dev::E_size * dev::E_size * sizeof(float)); e) {
+        return fail(MI_ERR_HIP, "energy table: %s", hipGetErrorString(e));
+    }
+    hipLaunchKernelGGL(energy_table, dim3(dev::E_size * dev::E_size), dim3(256), 0, nullptr, t);
+    if (hipError_t const e = hipDeviceSynchronize(); e) {
+        (void)hipFree(t);
+        return fail(MI_ERR_HIP, "energy table: %s", hipGetErrorString(e));
+    }
+    *table = t;
+    return MI_OK;
+}
+
+int32_t bsdf_batch(bool sample, uint32_t n, float const* params, float const* frames,
+                   float const* wo, float const* wi_or_xi, float const* table, float* out,
+                   hipStream_t stream) {
+    if (0 == n) {
+        return MI_OK;
+    }
+    if (sample) {
+        hipLaunchKernelGGL(bsdf_sample_batch, dim3(blocks(n, 256)), dim3(256), 0, stream, n,
+                           params, frames, wo, wi_or_xi, table, out);
+    } else {
+        hipLaunchKernelGGL(bsdf_evaluate_batch, dim3(blocks(n, 256)), dim3(256), 0, stream, n,
+                           params, frames, wo, wi_or_xi, table, out);
+    }
+    if (hipError_t const e = hipGetLastError(); e) {
+        return fail(MI_ERR_HIP, "bsdf batch: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+void RenderScene::release() {
+    bvh.release();
+    (void)hipFree(positions);
+    (void)hipFree(indices);
+    (void)hipFree(tri_material);
+    (void)hipFree(materials);
+    (void)hipFree(light_tris);
+    (void)hipFree(light_cdf);
+    *this = RenderScene();
+}
+
+int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t const* indices,
+                     uint32_t num_triangles, uint32_t const* tri_material,
+                     mi_material const* materials, uint32_t num_materials,
+                     mi_bvh_settings const& settings, RenderScene& out) {
+    for (uint64_t i = 0; i < 3ull * num_triangles; ++i) {
+        if (indices[i] >= num_vertices) {
+            return fail(MI_ERR_ARGS, "mi_scene_create: index %llu is %u, of %u vertices",
+                        (unsigned long long)i, indices[i], num_vertices);
+        }
+    }
+    for (uint32_t t = 0; t < num_triangles; ++t) {
+        if (tri_material[t] >= num_materials) {
+            return fail(MI_ERR_ARGS, "mi_scene_create: triangle %u has material %u of %u", t,
+                        tri_material[t], num_materials);
+        }
+    }
+    std::vector<Render_material> mats(num_materials);
+    for (uint32_t m = 0; m < num_materials; ++m) {
+        mi_material const& in = materials[m];
+        if (in.kind > MI_MATERIAL_LIGHT) {
+            return fail(MI_ERR_ARGS, "mi_scene_create: material %u has kind %u", m, in.kind);
+        }
+        Render_material& r = mats[m];
+        std::memset(&r, 0, sizeof(r));
+        for (int c = 0; c < 3; ++c) {
+            r.color[c]    = in.color[c];
+            r.emission[c] = is_light(in) ? in.emission[c] : 0.f;
+        }
+        r.roughness = in.roughness;
+        r.metallic  = in.metallic;
+        r.ior       = in.ior;
+        r.kind      = in.kind;
+    }
+
+    // emissive triangles and their area CDF
+    std::vector<uint32_t> lights;
+    std::vector<double>   areas;
+    for (uint32_t t = 0; t < num_triangles; ++t) {
+        if (!is_light(materials[tri_material[t]])) {
+            continue;
+        }
+        float const* a     = positions + 3 * size_t(indices[3 * size_t(t) + 0]);
+        float const* b     = positions + 3 * size_t(indices[3 * size_t(t) + 1]);
+        float const* c     = positions + 3 * size_t(indices[3 * size_t(t) + 2]);
+        double const e1[3] = {double(b[0]) - a[0], double(b[1]) - a[1], double(b[2]) - a[2]};
+        double const e2[3] = {double(c[0]) - a[0], double(c[1]) - a[1], double(c[2]) - a[2]};
+        double const x     = e1[1] * e2[2] - e1[2] * e2[1];
+        double const y     = e1[2] * e2[0] - e1[0] * e2[2];
+        double const z     = e1[0] * e2[1] - e1[1] * e2[0];
+        double const area  = 0.5 * std::sqrt(x * x + y * y + z * z);
+        if (area > 0.0) {
+            lights.push_back(t);
+            areas.push_back(area);
+        }
+    }
+    double total = 0.0;
+    for (double a : areas) {
+        total += a;
+    }
+    std::vector<float> cdf(lights.size());
+    double             run = 0.0;
+    for (size_t i = 0; i < lights.size(); ++i) {
+        run += areas[i];
+        cdf[i] = i + 1 == lights.size() ? 1.f : float(run / total);
+    }
+
+    RenderScene s;
+    s.num_vertices  = num_vertices;
+    s.num_triangles = num_triangles;
+    s.num_materials = num_materials;
+    s.num_lights    = uint32_t(lights.size());
+    s.light_area    = float(total);
+
+    size_t const nl = std::max<size_t>(1, lights.size());
+    hipError_t   e  = hipMalloc(&s.positions, size_t(num_vertices) * 3 * sizeof(float));
+    if (!e) e = hipMalloc(&s.indices, size_t(num_triangles) * 3 * sizeof(uint32_t));
+    if (!e) e = hipMalloc(&s.tri_material, size_t(num_triangles) * sizeof(uint32_t));
+    if (!e) e = hipMalloc(&s.materials, std::max<size_t>(1, num_materials) * sizeof(Render_material));
+    if (!e) e = hipMalloc(&s.light_tris, nl * sizeof(uint32_t));
+    if (!e) e = hipMalloc(&s.light_cdf, nl * sizeof(float));
+    if (!e) e = hipMemcpy(s.positions, positions, size_t(num_vertices) * 3 * sizeof(float), hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(s.indices, indices, size_t(num_triangles) * 3 * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(s.tri_material, tri_material, size_t(num_triangles) * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(s.materials, mats.data(), num_materials * sizeof(Render_material), hipMemcpyHostToDevice);
+    if (!e && !lights.empty()) e = hipMemcpy(s.light_tris, lights.data(), lights.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (!e && !lights.empty()) e = hipMemcpy(s.light_cdf, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e) {
+        s.release();
+        return fail(MI_ERR_HIP, "mi_scene_create: %s", hipGetErrorString(e));
+    }
+    if (int32_t const st = build_bvh(s.positions, num_vertices, 3, s.indices, num_triangles,
+                                     settings, nullptr, s.bvh);
+        st) {
+        s.release();
+        return st;
+    }
+    out = s;
+    return MI_OK;
+}
+
+void RenderState::release() {
+    (void)hipFree(paths);
+    (void)hipFree(rays[0]);
+    (void)hipFree(rays[1]);
+    (void)hipFree(ray_path[0]);
+    (void)hipFree(ray_path[1]);
+    (void)hipFree(hits);
+    (void)hipFree(shadow_rays);
+    (void)hipFree(shadows);
+    (void)hipFree(occluded);
+    (void)hipFree(counters);
+    if (begin) {
+        (void)hipEventDestroy(begin);
+    }
+    if (end) {
+        (void)hipEventDestroy(end);
+    }
+    if (stream) {
+        (void)hipStreamDestroy(stream);
+    }
+    *this = RenderState();
+}
+
+int32_t RenderState::reserve(uint32_t n) {
+    if (!stream) {
+        hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+        if (!e) e = hipEventCreateWithFlags(&begin, hipEventDisableTiming);
+        if (!e) e = hipEventCreateWithFlags(&end, hipEventDisableTiming);
+        if (!e) e = hipMalloc(&counters, Num_counters * sizeof(uint32_t));
+        if (e) {
+            release();
+            return fail(MI_ERR_HIP, "render state: %s", hipGetErrorString(e));
+        }
+    }
+    if (n <= capacity) {
+        return MI_OK;
+    }
+    (void)hipFree(paths);
+    (void)hipFree(rays[0]);
+    (void)hipFree(rays[1]);
+    (void)hipFree(ray_path[0]);
+    (void)hipFree(ray_path[1]);
+    (void)hipFree(hits);
+    (void)hipFree(shadow_rays);
+    (void)hipFree(shadows);
+    (void)hipFree(occluded);
+    capacity     = 0;
+    hipError_t e = hipMalloc(&paths, size_t(n) * sizeof(Path));
+    for (int q = 0; q < 2 && !e; ++q) {
+        e = hipMalloc(&rays[q], size_t(n) * sizeof(mi_ray));
+        if (!e) e = hipMalloc(&ray_path[q], size_t(n) * sizeof(uint32_t));
+    }
+    if (!e) e = hipMalloc(&hits, size_t(n) * sizeof(mi_hit));
+    if (!e) e = hipMalloc(&shadow_rays, size_t(n) * sizeof(mi_ray));
+    if (!e) e = hipMalloc(&shadows, size_t(n) * sizeof(Shadow));
+    if (!e) e = hipMalloc(&occluded, size_t(n) * sizeof(uint32_t));
+    if (e) {
+        release();
+        return fail(MI_ERR_HIP, "render state of %u paths: %s", n, hipGetErrorString(e));
+    }
+    capacity = n;
+    return MI_OK;
+}
+
+int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_table,
+               RenderState& state, mi_camera const& cam, mi_render_settings const& set,
+               double* accum, hipStream_t stream, mi_render_stats* stats) {
+    uint64_t const num_pixels64 = uint64_t(cam.width) * cam.height;
+    uint32_t const spw          = set.samples_per_wave;
+    uint64_t const paths64      = num_pixels64 * spw;
+    if (0 == num_pixels64 || 0 == spw || 0 == set.samples_per_pixel || paths64 > (1ull << 27) ||
+        set.samples_per_pixel % spw || set.max_bounces > 64 || !(cam.fov_deg > 0.f) ||
+        !(cam.fov_deg < 180.f)) {
+        return fail(MI_ERR_ARGS,
+                    "mi_render: bad camera or settings (%ux%u, %u spp in waves of %u, "
+                    "max_bounces %u)",
+                    cam.width, cam.height, set.samples_per_pixel, spw, set.max_bounces);
+    }
+    uint32_t const num_pixels = uint32_t(num_pixels64);
+    uint32_t const num_paths  = uint32_t(paths64);
+
+    // camera frame: right = fwd x up, true up = right x fwd
+    double       f[3]  = {double(cam.target[0]) - cam.eye[0], double(cam.target[1]) - cam.eye[1],
+                          double(cam.target[2]) - cam.eye[2]};
+    double const fl    = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    double const u0[3] = {cam.up[0], cam.up[1], cam.up[2]};
+    if (!(fl > 0.0)) {
+        return fail(MI_ERR_ARGS, "mi_render: eye and target coincide");
+    }
+    for (double& x : f) {
+        x /= fl;
+    }
+    double r[3] = {f[1] * u0[2] - f[2] * u0[1], f[2] * u0[0] - f[0] * u0[2],
+                   f[0] * u0[1] - f[1] * u0[0]};
+    double const rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (!(rl > 0.0)) {
+        return fail(MI_ERR_ARGS, "mi_render: up is parallel to the view direction");
+    }
+    for (double& x : r) {
+        x /= rl;
+    }
+    double const up[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2],
+                          r[0] * f[1] - r[1] * f[0]};
+
+    if (int32_t const st = state.reserve(num_paths); st) {
+        return st;
+    }
+    if (int32_t const st = prepare_trace(scene.bvh, trace, num_paths); st) {
+        return st;
+    }
+
+    Wave w;
+    w.positions    = scene.positions;
+    w.indices      = scene.indices;
+    w.tri_material = scene.tri_material;
+    w.materials    = scene.materials;
+    w.light_tris   = scene.light_tris;
+    w.light_cdf    = scene.light_cdf;
+    w.num_lights   = scene.num_lights;
+    w.light_area   = scene.light_area;
+    w.e_table      = e_table;
+    for (int k = 0; k < 3; ++k) {
+        w.eye[k]   = cam.eye[k];
+        w.fwd[k]   = float(f[k]);
+        w.right[k] = float(r[k]);
+        w.up[k]    = float(up[k]);
+    }
+    w.tan_half    = float(std::tan(0.5 * double(cam.fov_deg) * M_PI / 180.0));
+    w.aspect      = float(double(cam.width) / double(cam.height));
+    w.width       = cam.width;
+    w.height      = cam.height;
+    w.min_bounces = set.min_bounces;
+    w.max_bounces = set.max_bounces;
+    w.seed        = set.seed;
+    w.num_paths   = num_paths;
+    w.num_pixels  = num_pixels;
+    w.paths       = static_cast<Path*>(state.paths);
+    w.rays[0]     = state.rays[0];
+    w.rays[1]     = state.rays[1];
+    w.ray_path[0] = state.ray_path[0];
+    w.ray_path[1] = state.ray_path[1];
+    w.hits        = state.hits;
+    w.shadow_rays = state.shadow_rays;
+    w.shadows     = static_cast<Shadow*>(state.shadows);
+    w.occluded    = state.occluded;
+    w.counters    = state.counters;
+
+    hipStream_t const s         = state.stream;
+    uint32_t const    waves     = set.samples_per_pixel / spw;
+    uint32_t const    grid_path = std::min<uint32_t>(blocks(num_paths, 256), trace.num_cus * 8);
+    auto const        start     = std::chrono::steady_clock::now();
+
+    // one wave: every kernel reads its counts and sample base on the device
+    auto enqueue_wave = [&]() -> int32_t {
+        hipLaunchKernelGGL(raygen, dim3(blocks(num_paths, 256)), dim3(256), 0, s, w);
+        uint32_t q = 0;
+        for (uint32_t d = 0; d <= set.max_bounces; ++d) {
+            if (int32_t const st = launch_trace_device_count(scene.bvh, trace, state.rays[q],
+                                                             &state.counters[C_queue0 + q],
+                                                             num_paths, state.hits, false, s);
+                st) {
+                return st;
+            }
+            hipLaunchKernelGGL(reset_queues, dim3(1), dim3(1), 0, s, state.counters, q ^ 1);
+            hipLaunchKernelGGL(shade, dim3(grid_path), dim3(256), 0, s, w, q, d);
+            hipLaunchKernelGGL(tally, dim3(1), dim3(1), 0, s, state.counters, q);
+            if (int32_t const st = launch_trace_device_count(scene.bvh, trace, state.shadow_rays,
+                                                             &state.counters[C_shadow], num_paths,
+                                                             state.occluded, true, s);
+                st) {
+                return st;
+            }
+            hipLaunchKernelGGL(shadow_resolve, dim3(grid_path), dim3(256), 0, s, w);
+            q ^= 1;
+        }
+        hipLaunchKernelGGL(resolve, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, w, spw, accum);
+        hipLaunchKernelGGL(advance_base, dim3(1), dim3(1), 0, s, state.counters, spw);
+        return MI_OK;
+    };
+
+    // order after the caller's earlier work on its stream
+    hipError_t e = hipEventRecord(state.begin, stream);
+    if (!e) e = hipStreamWaitEvent(s, state.begin, 0);
+    if (e) {
+        return fail(MI_ERR_HIP, "mi_render: %s", hipGetErrorString(e));
+    }
+    hipLaunchKernelGGL(set_base, dim3(1), dim3(1), 0, s, state.counters, set.first_sample);
+
+    if (set.flags & MI_RENDER_NO_GRAPH) {
+        for (uint32_t wave = 0; wave < waves; ++wave) {
+            if (int32_t const st = enqueue_wave(); st) {
+                return st;
+            }
+        }
+    } else {
+        hipGraph_t     graph = nullptr;
+        hipGraphExec_t exec  = nullptr;
+        e                    = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e) {
+            return fail(MI_ERR_HIP, "mi_render: begin capture: %s", hipGetErrorString(e));
+        }
+        int32_t const    st = enqueue_wave();
+        hipError_t const ce = hipStreamEndCapture(s, &graph);
+        if (st) {
+            if (graph) {
+                (void)hipGraphDestroy(graph);
+            }
+            return st;
+        }
+        if (ce) {
+            return fail(MI_ERR_HIP, "mi_render: end capture: %s", hipGetErrorString(ce));
+        }
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e) {
+            return fail(MI_ERR_HIP, "mi_render: graph instantiate: %s", hipGetErrorString(e));
+        }
+        for (uint32_t wave = 0; wave < waves && !e; ++wave) {
+            e = hipGraphLaunch(exec, s);
+        }
+        // the executable graph is destroyed only after its launches ran
+        hipError_t const se = hipStreamSynchronize(s);
+        (void)hipGraphExecDestroy(exec);
+        if (e || se) {
+            return fail(MI_ERR_HIP, "mi_render: graph launch: %s", hipGetErrorString(e ? e : se));
+        }
+    }
+
+    e = hipGetLastError();
+    if (!e) e = hipEventRecord(state.end, s);
+    if (!e) e = hipStreamWaitEvent(stream, state.end, 0);
+    if (e) {
+        return fail(MI_ERR_HIP, "mi_render: %s", hipGetErrorString(e));
+    }
+    if (stats) {
+        uint64_t totals[2] = {0, 0};
+        e = hipMemcpyAsync(totals, state.counters + C_totals, sizeof(totals), hipMemcpyDeviceToHost,
+                           s);
+        if (!e) e = hipStreamSynchronize(s);
+        if (e) {
+            return fail(MI_ERR_HIP, "mi_render: %s", hipGetErrorString(e));
+        }
+        stats->seconds =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        stats->paths       = double(num_paths) * waves;
+        stats->extend_rays = double(totals[0]);
+        stats->shadow_rays = double(totals[1]);
+    }
+    return MI_OK;
+}
+
+}  // namespace mi

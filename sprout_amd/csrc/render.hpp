@@ -1,0 +1,77 @@
+// Scene, per-render state and entry points of the wavefront path tracer (host code).
+#pragma once
+
+#include "common.hpp"
+#include "trace.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mi {
+
+// Device material record (48 bytes, read as three float4 by the shading kernel).
+struct Render_material {
+    float    color[3];
+    float    roughness;
+    float    emission[3];
+    float    metallic;
+    float    ior;
+    uint32_t kind;  // MI_MATERIAL_*
+    uint32_t pad[2];
+};
+
+struct RenderScene {
+    DeviceBvh        bvh;
+    float*           positions    = nullptr;  // 3 floats per vertex
+    uint32_t*        indices      = nullptr;  // 3 per triangle
+    uint32_t*        tri_material = nullptr;
+    Render_material* materials    = nullptr;
+    uint32_t*        light_tris   = nullptr;  // emissive triangles
+    float*           light_cdf    = nullptr;  // inclusive area CDF over light_tris, last = 1
+
+    uint32_t num_vertices  = 0;
+    uint32_t num_triangles = 0;
+    uint32_t num_materials = 0;
+    uint32_t num_lights    = 0;
+    float    light_area    = 0.f;  // total emissive area
+
+    void release();
+};
+
+// Buffers of one render, kept between calls and grown on demand.
+struct RenderState {
+    uint32_t     capacity    = 0;  // paths
+    void*        paths       = nullptr;
+    mi_ray*      rays[2]     = {nullptr, nullptr};
+    uint32_t*    ray_path[2] = {nullptr, nullptr};
+    mi_hit*      hits        = nullptr;
+    mi_ray*      shadow_rays = nullptr;
+    void*        shadows     = nullptr;
+    uint32_t*    occluded    = nullptr;
+    uint32_t*    counters    = nullptr;  // queue counters, wave base, ray totals
+    hipStream_t  stream      = nullptr;  // internal stream the bounce loop runs (and is captured) on
+    hipEvent_t   begin       = nullptr;
+    hipEvent_t   end         = nullptr;
+
+    int32_t reserve(uint32_t num_paths);
+    void    release();
+};
+
+int32_t make_energy_table(float** table);
+
+int32_t bsdf_batch(bool sample, uint32_t n, float const* params, float const* frames,
+                   float const* wo, float const* wi_or_xi, float const* table, float* out,
+                   hipStream_t stream);
+
+// Host arrays in; builds the tree on the device with the given settings.
+int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t const* indices,
+                     uint32_t num_triangles, uint32_t const* tri_material,
+                     mi_material const* materials, uint32_t num_materials,
+                     mi_bvh_settings const& settings, RenderScene& out);
+
+int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_table,
+               RenderState& state, mi_camera const& camera, mi_render_settings const& settings,
+               double* accum, hipStream_t stream, mi_render_stats* stats);
+
+}  // namespace mi

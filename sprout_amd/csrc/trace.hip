@@ -87,6 +87,7 @@ struct Launch {
     float4 const* tris;
     mi_ray const* rays;
     uint32_t      num_rays;
+    uint32_t const* num_rays_dev;  // if set, trace_refill reads the ray count here
     void*         out;
     uint32_t*     counts;
     uint32_t*     spill;
@@ -413,8 +414,9 @@ __global__ void trace_refill(Launch p) {
     Stack stack;
     stack.init(lds_stack, p.spill, p.lds_entries, p.total_threads);
 
-    uint32_t const lane    = threadIdx.x & 63;
-    uint64_t const lt_mask = (uint64_t(1) << lane) - 1;
+    uint32_t const lane     = threadIdx.x & 63;
+    uint64_t const lt_mask  = (uint64_t(1) << lane) - 1;
+    uint32_t const num_rays = p.num_rays_dev ? *p.num_rays_dev : p.num_rays;
 
     bool     active    = false;
     bool     exhausted = false;  // the global counter is past the last ray (wave-uniform)
@@ -445,13 +447,13 @@ __global__ void trace_refill(Launch p) {
                         base = atomicAdd(p.counter, p.claim_batch);
                     }
                     base = __shfl(base, leader);
-                    if (base >= p.num_rays) {
+                    if (base >= num_rays) {
                         exhausted = true;
                         break;
                     }
                     pool_next = base;
-                    pool_end  = p.num_rays - base <= p.claim_batch ? p.num_rays : base + p.claim_batch;
-                    if (pool_end == p.num_rays) {
+                    pool_end  = num_rays - base <= p.claim_batch ? num_rays : base + p.claim_batch;
+                    if (pool_end == num_rays) {
                         exhausted = true;
                     }
                 }
@@ -526,9 +528,13 @@ __global__ void reset_counter(uint32_t* counter) {
 
 }  // namespace
 
-int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays,
-                     uint32_t num_rays, void* out, uint32_t* counts, bool any_hit,
-                     hipStream_t stream) {
+namespace {
+
+// num_rays_dev set: the refill kernel reads the ray count on the device and
+// num_rays is only its upper bound (it sizes the grid and the stack spill).
+int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, uint32_t num_rays,
+               uint32_t const* num_rays_dev, void* out, uint32_t* counts, bool any_hit,
+               hipStream_t stream) {
     if (0 == num_rays) {
         return MI_OK;
     }
@@ -558,6 +564,7 @@ int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* ra
     p.tris          = bvh.tris;
     p.rays          = rays;
     p.num_rays      = num_rays;
+    p.num_rays_dev  = num_rays_dev;
     p.out           = out;
     p.counts        = counts;
     p.spill         = res.spill;
@@ -573,7 +580,7 @@ int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* ra
         } else {
             hipLaunchKernelGGL(trace_reference<false>, dim3(blocks), dim3(block), lds_bytes, stream, p);
         }
-    } else if (res.kernel != 1) {
+    } else if (res.kernel != 1 || num_rays_dev) {
         hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, res.counter);
         using Kernel    = void (*)(Launch);
         bool const ww   = 2 == res.kernel;
@@ -590,6 +597,33 @@ int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* ra
         return fail(MI_ERR_HIP, "trace launch: %s", hipGetErrorString(e));
     }
     return MI_OK;
+}
+
+}  // namespace
+
+int32_t prepare_trace(DeviceBvh const& bvh, TraceResources& res, uint32_t max_rays) {
+    uint32_t const entries = stack_entries(bvh.max_depth);
+    if (entries > Reference_stack_size) {
+        return fail(MI_ERR_STACK,
+                    "tree depth %u needs %u stack entries, more than the reference's %u",
+                    bvh.max_depth, entries, Reference_stack_size);
+    }
+    uint32_t const threads     = res.grid_blocks(max_rays) * res.block_size;
+    uint32_t const lds_cap     = res.lds_entries ? res.lds_entries : Max_lds_entries;
+    uint32_t const lds_entries = entries < lds_cap ? entries : lds_cap;
+    return res.reserve_spill(size_t(entries - lds_entries) * threads);
+}
+
+int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays,
+                     uint32_t num_rays, void* out, uint32_t* counts, bool any_hit,
+                     hipStream_t stream) {
+    return launch(bvh, res, rays, num_rays, nullptr, out, counts, any_hit, stream);
+}
+
+int32_t launch_trace_device_count(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays,
+                                  uint32_t const* num_rays, uint32_t max_rays, void* out,
+                                  bool any_hit, hipStream_t stream) {
+    return launch(bvh, res, rays, max_rays, num_rays, out, nullptr, any_hit, stream);
 }
 
 }  // namespace mi

@@ -70,4 +70,15 @@ int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* ra
                      uint32_t num_rays, void* out, uint32_t* counts, bool any_hit,
                      hipStream_t stream);
 
+// Reserves what traversals of up to max_rays rays need (the stack spill
+// area), so that later launches allocate nothing (as inside graph capture).
+int32_t prepare_trace(DeviceBvh const& bvh, TraceResources& res, uint32_t max_rays);
+
+// Same traversal over a ray count that lives in device memory (written by an
+// earlier kernel on the stream); max_rays bounds it.  Capturable: it records
+// kernels only.
+int32_t launch_trace_device_count(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays,
+                                  uint32_t const* num_rays, uint32_t max_rays, void* out,
+                                  bool any_hit, hipStream_t stream);
+
 }  // namespace mi

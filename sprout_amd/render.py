@@ -1,0 +1,101 @@
+"""Host side of the wavefront MIS path tracer (mi_scene_create / mi_render).
+
+Mirrors the reference's per-frame boundary: a scene made of triangles with
+one material each (Lambert, Substitute or a one-sided constant light), and
+a render call that adds every pixel's radiance sum over the requested
+samples into an FP64 accumulator on the device, as
+Driver::render_frame_forward (core/rendering/rendering_driver.cpp:295-360)
+drives Pathtracer_MIS (core/rendering/integrator/surface/pathtracer_mis.cpp)
+over all pixels.  Device memory and streams come from torch; every
+computation runs in libsprout_gpu.so.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+from .scenes import MATERIAL_DTYPE, ROOM_CAMERA
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Scene:
+    """Triangles, per-triangle materials and materials (scenes.MATERIAL_DTYPE)."""
+
+    def __init__(self, positions, indices, triangle_materials, materials, settings=(16, 64, 4)):
+        lib = _lib.load()
+        pos = np.ascontiguousarray(positions, np.float32)
+        idx = np.ascontiguousarray(indices, np.uint32)
+        tmat = np.ascontiguousarray(triangle_materials, np.uint32)
+        mats = np.ascontiguousarray(materials, MATERIAL_DTYPE)
+        handle = ctypes.c_uint32(_lib.MI_NULL)
+        s = _lib.MiBvhSettings(*settings)
+        check(lib.mi_scene_create(pos.ctypes.data, len(pos), idx.ctypes.data, len(idx),
+                                  tmat.ctypes.data, mats.ctypes.data, len(mats), ctypes.byref(s),
+                                  ctypes.byref(handle)), "mi_scene_create")
+        self.handle = handle.value
+        self.num_triangles = len(idx)
+
+    def release(self) -> None:
+        if self.handle != _lib.MI_NULL:
+            check(_lib.load().mi_scene_release(self.handle), "mi_scene_release")
+            self.handle = _lib.MI_NULL
+
+
+def camera(width: int, height: int, eye=ROOM_CAMERA["eye"], target=ROOM_CAMERA["target"],
+           up=ROOM_CAMERA["up"], fov_deg=ROOM_CAMERA["fov_deg"]) -> _lib.MiCamera:
+    return _lib.MiCamera(width, height, (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*target),
+                         (ctypes.c_float * 3)(*up), fov_deg)
+
+
+def render(scene: Scene, cam: _lib.MiCamera, spp: int, samples_per_wave: int = 1,
+           min_bounces: int = 4, max_bounces: int = 8, seed: int = 1, first_sample: int = 0,
+           graph: bool = True, accum: torch.Tensor | None = None, stats: bool = False):
+    """Adds the radiance sums of samples first_sample .. first_sample+spp-1 into
+    accum ((height, width, 3) float64 on the device, made zero if None) and
+    returns (accum, stats or None)."""
+    if accum is None:
+        accum = torch.zeros((cam.height, cam.width, 3), dtype=torch.float64, device="cuda")
+    if accum.dtype != torch.float64 or not accum.is_contiguous() or \
+            tuple(accum.shape) != (cam.height, cam.width, 3):
+        raise ValueError("accum must be a contiguous (height, width, 3) float64 device tensor")
+    st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
+                               0 if graph else _lib.MI_RENDER_NO_GRAPH, seed)
+    out = _lib.MiRenderStats() if stats else None
+    check(_lib.load().mi_render(scene.handle, ctypes.byref(cam), ctypes.byref(st),
+                                ctypes.c_void_p(accum.data_ptr()), _stream(),
+                                ctypes.byref(out) if out is not None else None), "mi_render")
+    return accum, out
+
+
+def ggx_e_table() -> torch.Tensor:
+    t = torch.empty((32, 32), dtype=torch.float32, device="cuda")
+    check(_lib.load().mi_ggx_e_table(ctypes.c_void_p(t.data_ptr()), _stream()), "mi_ggx_e_table")
+    return t
+
+
+def _batch(fn: str, width: int, params, frames, wo, w_or_xi) -> np.ndarray:
+    n = len(params)
+    dev = [torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+           for a in (params, frames, wo, w_or_xi)]
+    out = torch.empty((n, width), dtype=torch.float32, device="cuda")
+    check(getattr(_lib.load(), fn)(n, *[ctypes.c_void_p(d.data_ptr()) for d in dev],
+                                   ctypes.c_void_p(out.data_ptr()), _stream()), fn)
+    return out.cpu().numpy()
+
+
+def substitute_evaluate(params, frames, wo, wi) -> np.ndarray:
+    """(n, 4): reflection * n.wi (rgb), pdf; layouts as mi_substitute_evaluate."""
+    return _batch("mi_substitute_evaluate", 4, params, frames, wo, wi)
+
+
+def substitute_sample(params, frames, wo, xi) -> np.ndarray:
+    """(n, 8): wi, reflection (rgb), pdf, type bits (as float32 bits)."""
+    return _batch("mi_substitute_sample", 8, params, frames, wo, xi)

@@ -61,3 +61,35 @@ def bounce_rays(rays: np.ndarray, hits: np.ndarray, positions: np.ndarray, indic
                                          positions.ctypes.data, 3, indices.ctypes.data, seed,
                                          out.ctypes.data, num_out), "mi_gen_bounce_rays")
     return out
+
+
+# mi_material (include/sprout_gpu.h), 40 bytes
+MATERIAL_DTYPE = np.dtype([("color", "<f4", 3), ("emission", "<f4", 3), ("roughness", "<f4"),
+                           ("metallic", "<f4"), ("ior", "<f4"), ("kind", "<u4")])
+
+SUBSTITUTE, LAMBERT, LIGHT = 0, 1, 2
+
+# The room seen through its open +z side.
+ROOM_CAMERA = {"eye": (0.0, 1.0, 3.4), "target": (0.0, 1.0, 0.0), "up": (0.0, 1.0, 0.0),
+               "fov_deg": 40.0}
+
+
+def room(slit: float = 0.0):
+    """The procedural box room: 36 triangles (46 with the slit enclosure).
+
+    Returns positions (v, 3) f32, indices (t, 3) u32, per-triangle material
+    (t,) u32 and materials (m,) MATERIAL_DTYPE: 0 grey 0.75, 1 red
+    0.63/0.06/0.05, 2 green 0.14/0.45/0.09 (all Lambert), 3 the light of
+    radiance 17/12/4."""
+    lib = _lib.load()
+    nv, nt, nm = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0)
+    check(lib.mi_gen_room(slit, None, ctypes.byref(nv), None, ctypes.byref(nt), None, None,
+                          ctypes.byref(nm)), "mi_gen_room")
+    pos = np.zeros((nv.value, 3), np.float32)
+    idx = np.zeros((nt.value, 3), np.uint32)
+    tmat = np.zeros(nt.value, np.uint32)
+    mats = np.zeros(nm.value, MATERIAL_DTYPE)
+    check(lib.mi_gen_room(slit, pos.ctypes.data, ctypes.byref(nv), idx.ctypes.data,
+                          ctypes.byref(nt), tmat.ctypes.data, mats.ctypes.data, ctypes.byref(nm)),
+          "mi_gen_room")
+    return pos, idx, tmat, mats
