@@ -1,0 +1,685 @@
+// BF16 MFMA inference and training kernels of the path-guiding network.
+//
+// Every matrix product runs on v_mfma_f32_16x16x16_bf16 (64-wide wave, FP32
+// accumulation).  Operand lanes, for lane l, r = l % 16, q = l / 16:
+//   A (16 x 16, rows i, depth k): A[r][4q + t], t = 0..3
+//   B (16 x 16, depth k, cols j): B[4q + t][r]
+//   D (16 x 16):                   D[4q + t][r]
+// Weights live in LDS as bf16 W^T [out][in] so a forward B operand is one
+// 8-byte read; activations as bf16 [row][feature] so an A operand is one
+// 8-byte read.
+//
+// forward   a wave computes 16 queries: features (sinpi/cospi of 2^j x), then
+//           four layers through LDS; weights are loaded once per workgroup
+//           and the grid strides over query tiles.
+// gradient  a workgroup of 4 waves takes 64 samples at a time: each wave runs
+//           forward and backward for its 16 (activations and pre-activation
+//           gradients kept in LDS), then the 4 waves split the 96 16x16 tiles
+//           of the weight gradients (depth = the 64 samples) and keep them in
+//           registers across all chunks; partial sums per workgroup are
+//           reduced in a fixed order, so gradients are deterministic.
+// adam      one thread per parameter; refreshes the bf16 W^T copy.
+
+#include "common.hpp"
+#include "guide.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace mi {
+
+namespace {
+
+using namespace guide;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// LDS row strides (elements), padded against bank conflicts; multiples of 4
+constexpr int S_feat = Feat_pad + 8;  // 232
+constexpr int S_hid  = Hidden + 8;    // 72
+constexpr int S_out  = Num_out + 8;   // 40
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t const u = __float_as_uint(f);
+    return uint16_t((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(uint16_t h) {
+    return __uint_as_float(uint32_t(h) << 16);
+}
+
+// Orders this wave's LDS accesses (writes by some lanes, reads by others).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ bf16x4 load4(uint16_t const* p) {
+    return *reinterpret_cast<bf16x4 const*>(p);
+}
+
+__device__ __forceinline__ bf16x4 gather4(uint16_t const* p, int stride) {
+    bf16x4 v;
+    v[0] = short(p[0]);
+    v[1] = short(p[stride]);
+    v[2] = short(p[2 * stride]);
+    v[3] = short(p[3 * stride]);
+    return v;
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+struct Weights {
+    uint16_t w1[Hidden * S_feat];
+    uint16_t w2[Hidden * S_hid];
+    uint16_t w3[Hidden * S_hid];
+    uint16_t w4[Num_out * S_hid];
+    float    b[3 * Hidden + Num_out];
+};
+
+__device__ void load_weights(Weights& s, uint16_t const* wt, float const* params) {
+    for (int i = threadIdx.x; i < Hidden * Feat_pad; i += blockDim.x) {
+        s.w1[(i / Feat_pad) * S_feat + i % Feat_pad] = wt[T_w1 + i];
+    }
+    for (int i = threadIdx.x; i < Hidden * Hidden; i += blockDim.x) {
+        s.w2[(i / Hidden) * S_hid + i % Hidden] = wt[T_w2 + i];
+        s.w3[(i / Hidden) * S_hid + i % Hidden] = wt[T_w3 + i];
+    }
+    for (int i = threadIdx.x; i < Num_out * Hidden; i += blockDim.x) {
+        s.w4[(i / Hidden) * S_hid + i % Hidden] = wt[T_w4 + i];
+    }
+    for (int i = threadIdx.x; i < Hidden; i += blockDim.x) {
+        s.b[i]              = params[Off_b1 + i];
+        s.b[Hidden + i]     = params[Off_b2 + i];
+        s.b[2 * Hidden + i] = params[Off_b3 + i];
+    }
+    for (int i = threadIdx.x; i < Num_out; i += blockDim.x) {
+        s.b[3 * Hidden + i] = params[Off_b4 + i];
+    }
+}
+
+// Features of 16 queries into X [16][S_feat]; row m = query row0 + m, rows
+// past n are zero.  x: 9 inputs per query in [0,1].
+__device__ void features(float const* x, uint32_t row0, uint32_t n, uint16_t* X) {
+    int const      lane = threadIdx.x & 63;
+    int const      m    = lane & 15;
+    int const      g    = lane >> 4;
+    uint32_t const row  = row0 + m;
+    for (int i = g; i < Num_in; i += 4) {
+        float const v = row < n ? x[size_t(row) * Num_in + i] : 0.f;
+        float       f = v;
+        for (int j = 0; j < Num_freq; ++j) {
+            // 2^j v is exact; sinpi/cospi reduce it exactly
+            float const s = row < n ? sinpif(f) : 0.f;
+            float const c = row < n ? cospif(f) : 0.f;
+            X[m * S_feat + 24 * i + 2 * j]     = to_bf16(s);
+            X[m * S_feat + 24 * i + 2 * j + 1] = to_bf16(c);
+            f *= 2.f;
+        }
+    }
+    if (3 == g) {
+        for (int k = Num_feat; k < Feat_pad; ++k) {
+            X[m * S_feat + k] = 0;
+        }
+    }
+}
+
+// out[16][N] = act(A[16][K] W + b): A bf16 in LDS (stride sa), W^T in LDS
+// (stride sw); result bf16 to H (stride sh) or, for the last layer, fp32.
+template <int K, int N, bool Relu>
+__device__ void layer(uint16_t const* A, int sa, uint16_t const* Wt, int sw, float const* bias,
+                      uint16_t* H, int sh, float* F, int sf) {
+    int const lane = threadIdx.x & 63;
+    int const r    = lane & 15;
+    int const q    = lane >> 4;
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < K / 16; ++ks) {
+            bf16x4 const a = load4(A + r * sa + ks * 16 + 4 * q);
+            bf16x4 const b = load4(Wt + (nt * 16 + r) * sw + ks * 16 + 4 * q);
+            acc            = mfma(a, b, acc);
+        }
+        int const j = nt * 16 + r;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            int const i = 4 * q + t;
+            float     v = acc[t] + bias[j];
+            if (Relu) {
+                v                = fmaxf(v, 0.f);
+                H[i * sh + j]    = to_bf16(v);
+            } else {
+                F[i * sf + j] = v;
+            }
+        }
+    }
+}
+
+constexpr int Fwd_waves = 8;
+
+struct Fwd_lds {
+    Weights  w;
+    uint16_t x[Fwd_waves][16 * S_feat];  // features, then the fp32 outputs of the tile
+    uint16_t h[Fwd_waves][2][16 * S_hid];
+};
+static_assert(16 * S_out * sizeof(float) <= 16 * S_feat * sizeof(uint16_t),
+              "a tile's outputs fit in the space of its features");
+static_assert(sizeof(Fwd_lds) <= 160 * 1024, "LDS of a workgroup");
+
+__global__ void __launch_bounds__(Fwd_waves * 64)
+    forward_kernel(uint16_t const* wt, float const* params, float const* x, uint32_t n, float* out) {
+    __shared__ Fwd_lds s;
+    load_weights(s.w, wt, params);
+    __syncthreads();
+
+    int const wave  = threadIdx.x >> 6;
+    int const lane  = threadIdx.x & 63;
+    uint16_t* X     = s.x[wave];
+    uint16_t* H0    = s.h[wave][0];
+    uint16_t* H1    = s.h[wave][1];
+    float*    Y     = reinterpret_cast<float*>(X);
+    float const* bb = s.w.b;
+
+    for (uint32_t tile = blockIdx.x * Fwd_waves + wave; uint64_t(tile) * 16 < n;
+         tile += gridDim.x * Fwd_waves) {
+        uint32_t const row0 = tile * 16;
+        features(x, row0, n, X);
+        wave_sync();
+        layer<Feat_pad, Hidden, true>(X, S_feat, s.w.w1, S_feat, bb, H0, S_hid, nullptr, 0);
+        wave_sync();
+        layer<Hidden, Hidden, true>(H0, S_hid, s.w.w2, S_hid, bb + Hidden, H1, S_hid, nullptr, 0);
+        wave_sync();
+        layer<Hidden, Hidden, true>(H1, S_hid, s.w.w3, S_hid, bb + 2 * Hidden, H0, S_hid, nullptr, 0);
+        wave_sync();
+        layer<Hidden, Num_out, false>(H0, S_hid, s.w.w4, S_hid, bb + 3 * Hidden, nullptr, 0, Y, S_out);
+        wave_sync();
+        // coalesced store of the 16 x 32 outputs
+        for (int e = lane; e < 16 * Num_out; e += 64) {
+            int const      m   = e / Num_out;
+            int const      j   = e % Num_out;
+            uint32_t const row = row0 + m;
+            if (row < n) {
+                out[size_t(row) * Num_out + j] = Y[m * S_out + j];
+            }
+        }
+        wave_sync();
+    }
+}
+
+// ---- training ----
+
+constexpr int Grad_waves = 4;
+constexpr int Chunk      = 16 * Grad_waves;  // samples per workgroup step
+constexpr int Grad_tiles = (Feat_pad / 16) * (Hidden / 16) + 2 * (Hidden / 16) * (Hidden / 16) +
+                           (Hidden / 16) * (Num_out / 16);  // 56 + 32 + 8 = 96
+constexpr int Tiles_per_wave = Grad_tiles / Grad_waves;      // 24
+
+struct Grad_lds {
+    Weights  w;
+    uint16_t x[Chunk * S_feat];
+    uint16_t h[3][Chunk * S_hid];   // post-ReLU activations of the hidden layers
+    uint16_t dz[3][Chunk * S_hid];  // loss gradient at the hidden pre-activations
+    uint16_t dy[Chunk * S_out];     // loss gradient at the outputs
+    float    y[Chunk * S_out];      // outputs
+};
+static_assert(sizeof(Grad_lds) <= 160 * 1024, "LDS of a workgroup");
+constexpr int Num_bias = 3 * Hidden + Num_out;  // 224
+
+// Loss gradient at the 32 outputs of one sample (row of Y), scaled by
+// g = -(t / sum t) / q: the derivative of -(t / sum t) log q.
+__device__ void output_gradient(float const* y, float dirx, float diry, float dirz, float tw,
+                                int lobe, float out[4]) {
+    // mixture density q
+    float dmax = y[3];
+    for (int k = 1; k < Num_lobes; ++k) {
+        dmax = fmaxf(dmax, y[4 * k + 3]);
+    }
+    float esum = 0.f;
+    for (int k = 0; k < Num_lobes; ++k) {
+        esum += expf(y[4 * k + 3] - dmax);
+    }
+    float q  = 0.f;
+    float vk = 0.f, wk = 0.f, kap = 0.f, cosk = 0.f, sa = 0.f, sb = 0.f, theta = 0.f, phi = 0.f;
+    for (int k = 0; k < Num_lobes; ++k) {
+        float const* r  = y + 4 * k;
+        float const  a  = 1.f / (1.f + expf(-r[0]));
+        float const  b  = 1.f / (1.f + expf(-r[1]));
+        float const  th = dev::Pi * a;
+        float const  ph = 2.f * dev::Pi * b;
+        float        st, ct, sp, cp;
+        sincosf(th, &st, &ct);
+        sincosf(ph, &sp, &cp);
+        float const c  = (st * cp) * dirx + (st * sp) * diry + ct * dirz;
+        float const ka = expf(fminf(fmaxf(r[2], Kappa_lo), Kappa_hi));
+        float const v  = ka / (2.f * dev::Pi * -expm1f(-2.f * ka)) * expf(ka * (c - 1.f));
+        float const w  = expf(r[3] - dmax) / esum;
+        q += w * v;
+        if (k == lobe) {
+            vk = v, wk = w, kap = ka, cosk = c, sa = a, sb = b, theta = th, phi = ph;
+        }
+    }
+    float const g    = -tw / fmaxf(q, 1e-30f);
+    float const resp = wk * vk;
+    float const* r   = y + 4 * lobe;
+    // softmax: dq/dd_k = w_k (v_k - q)
+    out[3] = g * wk * (vk - q);
+    // kappa: d log v / d kappa = 1/kappa - 2 e^(-2k) / (1 - e^(-2k)) + (cos - 1)
+    float const e2   = expf(-2.f * kap);
+    float const dlk  = 1.f / kap - 2.f * e2 / (-expm1f(-2.f * kap)) + (cosk - 1.f);
+    bool const  in   = r[2] > Kappa_lo && r[2] < Kappa_hi;
+    out[2]           = in ? g * resp * dlk * kap : 0.f;
+    // direction: d log v / d mu = kappa w
+    float st, ct, sp, cp;
+    sincosf(theta, &st, &ct);
+    sincosf(phi, &sp, &cp);
+    float const dth = kap * ((ct * cp) * dirx + (ct * sp) * diry - st * dirz);
+    float const dph = kap * ((-st * sp) * dirx + (st * cp) * diry);
+    out[0]          = g * resp * dth * dev::Pi * sa * (1.f - sa);
+    out[1]          = g * resp * dph * 2.f * dev::Pi * sb * (1.f - sb);
+}
+
+// dst[16][N] = (src[16][K] W^T) * (act > 0): backward through one layer;
+// B[k][j] = W[j][k] = Wt[k][j] is gathered from 4 rows of Wt.
+template <int K, int N>
+__device__ void back_layer(uint16_t const* src, int ss, uint16_t const* Wt, int sw,
+                           uint16_t const* act, int sact, uint16_t* dst, int sd) {
+    int const lane = threadIdx.x & 63;
+    int const r    = lane & 15;
+    int const q    = lane >> 4;
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < K / 16; ++ks) {
+            bf16x4 const a = load4(src + r * ss + ks * 16 + 4 * q);
+            bf16x4 const b = gather4(Wt + (ks * 16 + 4 * q) * sw + nt * 16 + r, sw);
+            acc            = mfma(a, b, acc);
+        }
+        int const j = nt * 16 + r;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            int const   i = 4 * q + t;
+            float const v = from_bf16(act[i * sact + j]) > 0.f ? acc[t] : 0.f;
+            dst[i * sd + j] = to_bf16(v);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(Grad_waves * 64)
+    gradient_kernel(uint16_t const* wt, float const* params, float const* x, float const* dirs,
+                    float const* weights, uint32_t n, float const* t_sum, float* partial) {
+    __shared__ Grad_lds s;
+    load_weights(s.w, wt, params);
+    __syncthreads();
+
+    // thread b < 224 sums the gradient of bias b over the samples, in order
+    float bias_acc = 0.f;
+
+    int const   wave  = threadIdx.x >> 6;
+    int const   lane  = threadIdx.x & 63;
+    int const   r     = lane & 15;
+    int const   q     = lane >> 4;
+    float const scale = 1.f / *t_sum;
+
+    f32x4 acc[Tiles_per_wave];
+#pragma unroll
+    for (int k = 0; k < Tiles_per_wave; ++k) {
+        acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+
+    uint32_t const chunks = (n + Chunk - 1) / Chunk;
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {  // block-uniform
+        uint32_t const row0 = c * Chunk + wave * 16;
+        uint16_t*      X    = s.x + wave * 16 * S_feat;
+        uint16_t*      H1   = s.h[0] + wave * 16 * S_hid;
+        uint16_t*      H2   = s.h[1] + wave * 16 * S_hid;
+        uint16_t*      H3   = s.h[2] + wave * 16 * S_hid;
+        uint16_t*      DZ1  = s.dz[0] + wave * 16 * S_hid;
+        uint16_t*      DZ2  = s.dz[1] + wave * 16 * S_hid;
+        uint16_t*      DZ3  = s.dz[2] + wave * 16 * S_hid;
+        uint16_t*      DY   = s.dy + wave * 16 * S_out;
+        float*         Y    = s.y + wave * 16 * S_out;
+
+        features(x, row0, n, X);
+        wave_sync();
+        layer<Feat_pad, Hidden, true>(X, S_feat, s.w.w1, S_feat, s.w.b, H1, S_hid, nullptr, 0);
+        wave_sync();
+        layer<Hidden, Hidden, true>(H1, S_hid, s.w.w2, S_hid, s.w.b + Hidden, H2, S_hid, nullptr, 0);
+        wave_sync();
+        layer<Hidden, Hidden, true>(H2, S_hid, s.w.w3, S_hid, s.w.b + 2 * Hidden, H3, S_hid, nullptr, 0);
+        wave_sync();
+        layer<Hidden, Num_out, false>(H3, S_hid, s.w.w4, S_hid, s.w.b + 3 * Hidden, nullptr, 0, Y, S_out);
+        wave_sync();
+
+        // output gradients: lane handles sample r, lobes q and q + 4
+        {
+            uint32_t const row = row0 + r;
+            float          g0[4] = {0.f, 0.f, 0.f, 0.f};
+            float          g1[4] = {0.f, 0.f, 0.f, 0.f};
+            if (row < n) {
+                float const tw = weights[row] * scale;
+                float const dx = dirs[3 * size_t(row) + 0];
+                float const dyv = dirs[3 * size_t(row) + 1];
+                float const dz = dirs[3 * size_t(row) + 2];
+                output_gradient(Y + r * S_out, dx, dyv, dz, tw, q, g0);
+                output_gradient(Y + r * S_out, dx, dyv, dz, tw, q + 4, g1);
+            }
+            wave_sync();  // Y is read by all lanes before DY and DZ1 are written
+            for (int p = 0; p < 4; ++p) {
+                DY[r * S_out + 4 * q + p]       = to_bf16(g0[p]);
+                DY[r * S_out + 4 * (q + 4) + p] = to_bf16(g1[p]);
+            }
+        }
+        wave_sync();
+        back_layer<Num_out, Hidden>(DY, S_out, s.w.w4, S_hid, H3, S_hid, DZ3, S_hid);
+        wave_sync();
+        back_layer<Hidden, Hidden>(DZ3, S_hid, s.w.w3, S_hid, H2, S_hid, DZ2, S_hid);
+        wave_sync();
+        back_layer<Hidden, Hidden>(DZ2, S_hid, s.w.w2, S_hid, H1, S_hid, DZ1, S_hid);
+        __syncthreads();
+
+        // bias gradients: column sums over the chunk's samples
+        if (int(threadIdx.x) < Num_bias) {
+            int const       b   = threadIdx.x;
+            uint16_t const* src = b < 3 * Hidden ? s.dz[b / Hidden] + b % Hidden : s.dy + (b - 3 * Hidden);
+            int const       st  = b < 3 * Hidden ? S_hid : S_out;
+            for (int k = 0; k < Chunk; ++k) {
+                bias_acc += from_bf16(src[k * st]);
+            }
+        }
+
+        // weight gradients over the chunk's 64 samples: dW = act^T dz
+#pragma unroll
+        for (int k = 0; k < Tiles_per_wave; ++k) {
+            int const tile = wave + Grad_waves * k;
+            uint16_t const* act;
+            uint16_t const* dz;
+            int sa, sd, mt, nt;
+            if (tile < 56) {
+                act = s.x, sa = S_feat, dz = s.dz[0], sd = S_hid, mt = tile / 4, nt = tile % 4;
+            } else if (tile < 72) {
+                act = s.h[0], sa = S_hid, dz = s.dz[1], sd = S_hid, mt = (tile - 56) / 4, nt = (tile - 56) % 4;
+            } else if (tile < 88) {
+                act = s.h[1], sa = S_hid, dz = s.dz[2], sd = S_hid, mt = (tile - 72) / 4, nt = (tile - 72) % 4;
+            } else {
+                act = s.h[2], sa = S_hid, dz = s.dy, sd = S_out, mt = (tile - 88) / 2, nt = (tile - 88) % 2;
+            }
+#pragma unroll
+            for (int ks = 0; ks < Chunk / 16; ++ks) {
+                // A[i][k] = act[sample k][feature i]; B[k][j] = dz[sample k][j]
+                bf16x4 const a = gather4(act + (ks * 16 + 4 * q) * sa + mt * 16 + r, sa);
+                bf16x4 const b = gather4(dz + (ks * 16 + 4 * q) * sd + nt * 16 + r, sd);
+                acc[k]         = mfma(a, b, acc[k]);
+            }
+        }
+        __syncthreads();
+    }
+
+    // partial sums of this workgroup, in the parameter layout
+    float* out = partial + size_t(blockIdx.x) * Num_params;
+#pragma unroll
+    for (int k = 0; k < Tiles_per_wave; ++k) {
+        int const tile = wave + Grad_waves * k;
+        int       base, cols, mt, nt;
+        if (tile < 56) {
+            base = Off_w1, cols = Hidden, mt = tile / 4, nt = tile % 4;
+        } else if (tile < 72) {
+            base = Off_w2, cols = Hidden, mt = (tile - 56) / 4, nt = (tile - 56) % 4;
+        } else if (tile < 88) {
+            base = Off_w3, cols = Hidden, mt = (tile - 72) / 4, nt = (tile - 72) % 4;
+        } else {
+            base = Off_w4, cols = Num_out, mt = (tile - 88) / 2, nt = (tile - 88) % 2;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            int const i = mt * 16 + 4 * q + t;
+            int const j = nt * 16 + r;
+            out[base + i * cols + j] = acc[k][t];
+        }
+    }
+    if (int(threadIdx.x) < Num_bias) {
+        int const b       = threadIdx.x;
+        int const offs[4] = {Off_b1, Off_b2, Off_b3, Off_b4};
+        out[offs[b < 3 * Hidden ? b / Hidden : 3] + (b < 3 * Hidden ? b % Hidden : b - 3 * Hidden)] =
+            bias_acc;
+    }
+}
+
+__global__ void sum_weights(float const* w, uint32_t n, float* out) {
+    __shared__ float part[256];
+    float            s = 0.f;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        s += w[i];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) {
+            part[threadIdx.x] += part[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (0 == threadIdx.x) {
+        out[0] = part[0] > 0.f ? part[0] : 1.f;
+    }
+}
+
+__global__ void reduce_partials(float const* partial, uint32_t blocks, float* grad) {
+    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= uint32_t(Num_params)) {
+        return;
+    }
+    float s = 0.f;
+    for (uint32_t b = 0; b < blocks; ++b) {
+        s += partial[size_t(b) * Num_params + p];
+    }
+    grad[p] = s;
+}
+
+// W^T bf16 position of parameter p, or -1 for a bias
+__device__ __host__ inline int transposed_index(int p) {
+    if (p < Off_b1) {
+        return T_w1 + (p % Hidden) * Feat_pad + p / Hidden;
+    }
+    if (p >= Off_w2 && p < Off_b2) {
+        int const e = p - Off_w2;
+        return T_w2 + (e % Hidden) * Hidden + e / Hidden;
+    }
+    if (p >= Off_w3 && p < Off_b3) {
+        int const e = p - Off_w3;
+        return T_w3 + (e % Hidden) * Hidden + e / Hidden;
+    }
+    if (p >= Off_w4 && p < Off_b4) {
+        int const e = p - Off_w4;
+        return T_w4 + (e % Num_out) * Hidden + e / Num_out;
+    }
+    return -1;
+}
+
+__global__ void refresh_bf16(float const* params, uint16_t* wt) {
+    int const p = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= Num_params) {
+        return;
+    }
+    int const t = transposed_index(p);
+    if (t >= 0) {
+        wt[t] = to_bf16(params[p]);
+    }
+}
+
+__global__ void adam_kernel(float* params, float* m, float* v, float const* grad, float lr,
+                            float b1, float b2, float c1, float c2, float eps, uint16_t* wt) {
+    int const p = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= Num_params) {
+        return;
+    }
+    float const g  = grad[p];
+    float const mm = b1 * m[p] + (1.f - b1) * g;
+    float const vv = b2 * v[p] + (1.f - b2) * g * g;
+    m[p]           = mm;
+    v[p]           = vv;
+    float const w  = params[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    params[p]      = w;
+    int const t    = transposed_index(p);
+    if (t >= 0) {
+        wt[t] = to_bf16(w);
+    }
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t b) {
+    uint64_t const g = (n + b - 1) / b;
+    return uint32_t(g > 0 ? g : 1);
+}
+
+struct Pcg {
+    uint64_t state, inc;
+    uint32_t next() {
+        uint64_t const old = state;
+        state              = old * 6364136223846793005ull + (inc | 1);
+        uint32_t const xrs = uint32_t(((old >> 18u) ^ old) >> 27u);
+        uint32_t const rot = uint32_t(old >> 59u);
+        return (xrs >> rot) | (xrs << ((0u - rot) & 31u));
+    }
+    double uniform() { return double(next()) * (1.0 / 4294967296.0); }
+};
+
+hipError_t refresh(GuideNet& net, hipStream_t stream) {
+    hipLaunchKernelGGL(refresh_bf16, dim3(blocks_for(Num_params, 256)), dim3(256), 0, stream,
+                       net.params, net.wt);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+void GuideNet::release() {
+    (void)hipFree(params);
+    (void)hipFree(adam_m);
+    (void)hipFree(adam_v);
+    (void)hipFree(grad);
+    (void)hipFree(partial);
+    (void)hipFree(t_sum);
+    (void)hipFree(wt);
+    *this = GuideNet();
+}
+
+int32_t guide_create(uint64_t seed, uint32_t num_cus, GuideNet& out) {
+    GuideNet n;
+    n.partial_blocks = num_cus ? num_cus : 256;
+    hipError_t e     = hipMalloc(&n.params, Num_params * sizeof(float));
+    if (!e) e = hipMalloc(&n.adam_m, Num_params * sizeof(float));
+    if (!e) e = hipMalloc(&n.adam_v, Num_params * sizeof(float));
+    if (!e) e = hipMalloc(&n.grad, Num_params * sizeof(float));
+    if (!e) e = hipMalloc(&n.partial, size_t(n.partial_blocks) * Num_params * sizeof(float));
+    if (!e) e = hipMalloc(&n.t_sum, sizeof(float));
+    if (!e) e = hipMalloc(&n.wt, Num_t * sizeof(uint16_t));
+    if (e) {
+        n.release();
+        return fail(MI_ERR_HIP, "guide: %s", hipGetErrorString(e));
+    }
+    // He-uniform weights (fan-in of the unpadded features), zero biases
+    std::vector<float> p(Num_params, 0.f);
+    Pcg                rng{0, (seed << 1) | 1};
+    rng.next();
+    rng.state += seed;
+    rng.next();
+    struct L {
+        int off, in, out, fan;
+    } const layers[4] = {{Off_w1, Feat_pad, Hidden, Num_feat},
+                         {Off_w2, Hidden, Hidden, Hidden},
+                         {Off_w3, Hidden, Hidden, Hidden},
+                         {Off_w4, Hidden, Num_out, Hidden}};
+    for (L const& l : layers) {
+        double const bound = std::sqrt(6.0 / l.fan);
+        for (int i = 0; i < l.in; ++i) {
+            for (int j = 0; j < l.out; ++j) {
+                float const w = float((2.0 * rng.uniform() - 1.0) * bound);
+                p[l.off + i * l.out + j] = (l.off == Off_w1 && i >= Num_feat) ? 0.f : w;
+            }
+        }
+    }
+    out = n;
+    return guide_set_params(out, p.data());
+}
+
+int32_t guide_set_params(GuideNet& net, float const* host_params) {
+    hipError_t e = hipMemcpy(net.params, host_params, Num_params * sizeof(float), hipMemcpyHostToDevice);
+    if (!e) e = hipMemset(net.adam_m, 0, Num_params * sizeof(float));
+    if (!e) e = hipMemset(net.adam_v, 0, Num_params * sizeof(float));
+    if (!e) e = refresh(net, nullptr);
+    if (!e) e = hipDeviceSynchronize();
+    if (e) {
+        return fail(MI_ERR_HIP, "guide parameters: %s", hipGetErrorString(e));
+    }
+    net.step = 0;
+    return MI_OK;
+}
+
+int32_t guide_get_params(GuideNet const& net, float* host_params) {
+    if (hipError_t const e = hipMemcpy(host_params, net.params, Num_params * sizeof(float),
+                                       hipMemcpyDeviceToHost);
+        e) {
+        return fail(MI_ERR_HIP, "guide parameters: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+int32_t guide_forward(GuideNet const& net, float const* x, uint32_t n, float* out,
+                      hipStream_t stream) {
+    if (0 == n) {
+        return MI_OK;
+    }
+    uint32_t const tiles  = blocks_for(n, 16);
+    uint32_t const blocks = std::min<uint32_t>(blocks_for(tiles, Fwd_waves), 1024);
+    hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(Fwd_waves * 64), 0, stream, net.wt,
+                       net.params, x, n, out);
+    if (hipError_t const e = hipGetLastError(); e) {
+        return fail(MI_ERR_HIP, "guide forward: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+int32_t guide_gradients(GuideNet& net, float const* x, float const* dirs, float const* weights,
+                        uint32_t n, hipStream_t stream) {
+    if (0 == n) {
+        return fail(MI_ERR_ARGS, "guide gradients: empty batch");
+    }
+    uint32_t const chunks = blocks_for(n, Chunk);
+    uint32_t const blocks = std::min(chunks, net.partial_blocks);
+    hipLaunchKernelGGL(sum_weights, dim3(1), dim3(256), 0, stream, weights, n, net.t_sum);
+    hipLaunchKernelGGL(gradient_kernel, dim3(blocks), dim3(Grad_waves * 64), 0, stream, net.wt,
+                       net.params, x, dirs, weights, n, net.t_sum, net.partial);
+    hipLaunchKernelGGL(reduce_partials, dim3(blocks_for(Num_params, 256)), dim3(256), 0, stream,
+                       net.partial, blocks, net.grad);
+    if (hipError_t const e = hipGetLastError(); e) {
+        return fail(MI_ERR_HIP, "guide gradients: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+int32_t guide_adam_step(GuideNet& net, float const* x, float const* dirs, float const* weights,
+                        uint32_t n, float lr, hipStream_t stream) {
+    if (int32_t const st = guide_gradients(net, x, dirs, weights, n, stream); st) {
+        return st;
+    }
+    net.step += 1;
+    float const b1 = 0.9f;
+    float const b2 = 0.99f;
+    float const c1 = float(1.0 - std::pow(double(b1), double(net.step)));
+    float const c2 = float(1.0 - std::pow(double(b2), double(net.step)));
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(Num_params, 256)), dim3(256), 0, stream,
+                       net.params, net.adam_m, net.adam_v, net.grad, lr, b1, b2, c1, c2, 1e-8f,
+                       net.wt);
+    if (hipError_t const e = hipGetLastError(); e) {
+        return fail(MI_ERR_HIP, "guide adam: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+}  // namespace mi

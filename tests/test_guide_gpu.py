@@ -1,0 +1,169 @@
+"""GPU guiding network (BF16 MFMA forward, gradients, Adam) against the FP64
+oracle (oracle/guide.py).
+
+The forward pass is compared twice: with the oracle emulating the kernel's
+rounding (features, weights and hidden activations rounded to bf16, FP64
+arithmetic otherwise), which pins the MFMA tiling, operand layouts and
+layer order to FP32-accumulation noise; and with the plain FP64 oracle,
+which bounds what BF16 costs.  Gradients are compared with the FP64
+oracle per layer (direction and norm), Adam against the oracle's update of
+the same gradient.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import guide as G
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+
+    from sprout_amd import bvh, guide
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    bvh.init(0)
+    yield guide
+    torch.cuda.synchronize()
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def bf16(a):
+    """round to nearest even bfloat16, as float64"""
+    f = np.asarray(a, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def emulated_forward(params, x):
+    h = bf16(G.encode(x))
+    for li, (w, b) in enumerate(params):
+        z = h @ bf16(w) + np.asarray(b, np.float32).astype(np.float64)
+        h = bf16(np.maximum(z, 0)) if li < 3 else z
+    return h
+
+
+def params_for_test(seed):
+    p = G.init_params(seed)
+    p[3][0] *= 3.0
+    rng = np.random.default_rng(seed + 100)
+    for w, b in p:
+        b[:] = rng.normal(scale=0.1, size=b.shape)
+    p[0][1][:] = np.abs(p[0][1])  # keep layer-1 units alive
+    return p
+
+
+def batch(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.random((n, 9))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    t = rng.exponential(size=n)
+    return x, d, t
+
+
+@pytest.fixture(scope="module")
+def net(gpu):
+    g = gpu.Guide(1)
+    yield g
+    g.release()
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 1000, 40000])
+def test_forward_matches_emulated_bf16(gpu, net, n):
+    params = params_for_test(2)
+    net.set_params(gpu.flatten(params))
+    x, _, _ = batch(n, 3)
+    log(f"forward n={n}")
+    out = net.forward(x).cpu().numpy().astype(np.float64)
+    ref = emulated_forward(params, x)
+    scale = np.abs(ref).max()
+    err = np.abs(out - ref).max() / scale
+    # FP32 accumulation order and an occasional bf16 rounding flip of a feature
+    assert err < 2e-3, err
+    assert np.median(np.abs(out - ref)) / scale < 1e-5
+
+
+def test_forward_close_to_fp64(gpu, net):
+    params = params_for_test(4)
+    net.set_params(gpu.flatten(params))
+    x, _, _ = batch(4096, 5)
+    out = net.forward(x).cpu().numpy().astype(np.float64)
+    ref, _ = G.forward(params, G.encode(x))
+    rel = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert rel < 3e-2, rel
+
+
+def test_params_round_trip(gpu, net):
+    p = gpu.flatten(params_for_test(6))
+    net.set_params(p)
+    assert np.array_equal(net.params(), p)
+
+
+@pytest.mark.parametrize("n", [64, 1000, 50000])
+def test_gradients_match_fp64(gpu, net, n):
+    params = params_for_test(7)
+    net.set_params(gpu.flatten(params))
+    x, d, t = batch(n, 8)
+    log(f"gradients n={n}")
+    g = gpu.unflatten(net.gradients(x, d, t).cpu().numpy())
+    ref = G.backward(params, G.encode(x), d, t)
+    for li in range(4):
+        for k in range(2):
+            a = g[li][k].reshape(-1)
+            b = ref[li][k].reshape(-1)
+            if li == 0 and k == 0:
+                assert np.all(g[0][0][G.NUM_FEAT:] == 0)
+            cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+            assert cos > 0.995 and rel < 0.1, (li, k, cos, rel)
+
+
+def test_gradients_are_deterministic(gpu, net):
+    net.set_params(gpu.flatten(params_for_test(9)))
+    x, d, t = batch(20000, 10)
+    a = net.gradients(x, d, t).cpu().numpy()
+    b = net.gradients(x, d, t).cpu().numpy()
+    assert np.array_equal(a, b)
+
+
+def test_adam_matches_oracle_update(gpu, net):
+    params = params_for_test(11)
+    flat = gpu.flatten(params)
+    net.set_params(flat)
+    x, d, t = batch(3000, 12)
+    g = net.gradients(x, d, t).cpu().numpy().astype(np.float64)
+    net.train_step(x, d, t, lr=2e-3)
+    after = net.params().astype(np.float64)
+    # the oracle's Adam on the same (GPU) gradient
+    p = gpu.unflatten(flat.astype(np.float64))
+    G.adam(p, gpu.unflatten(g), {}, lr=2e-3)
+    expect = gpu.flatten(p).astype(np.float64)
+    assert np.abs(after - expect).max() < 1e-6
+
+
+def test_training_fits_a_lobe(gpu, net):
+    """Online Adam on samples of one vMF lobe drives the loss toward it."""
+    rng = np.random.default_rng(13)
+    n = 8192
+    mu = np.array([0.3, 0.4, np.sqrt(1 - 0.25)])
+    d = np.array([G.sample_vmf(mu, 20.0, rng.random(), rng.random()) for _ in range(n)])
+    x = np.tile(rng.random((1, 9)), (n, 1))
+    t = np.ones(n)
+    params = G.init_params(14)
+    net.set_params(gpu.flatten(params))
+    l0 = G.loss(params, G.encode(x[:256]), d[:256], t[:256])
+    for _ in range(300):
+        net.train_step(x, d, t, lr=3e-3)
+    p = gpu.unflatten(net.params())
+    l1 = G.loss(p, G.encode(x[:256]), d[:256], t[:256])
+    # vMF(kappa = 20) has entropy about -1.2: the fit must come close
+    assert l1 < l0 - 2.0, (l0, l1)
+    assert l1 < -0.8, l1
