@@ -217,6 +217,7 @@ typedef struct mi_render_stats {
     double paths;        /* camera samples */
     double extend_rays;  /* closest-hit rays traced */
     double shadow_rays;  /* any-hit rays traced */
+    double trained;      /* guiding samples trained on (mi_render_guided) */
 } mi_render_stats;
 
 /* Creates a scene from HOST arrays: triangles with one material each; the
@@ -289,6 +290,33 @@ int32_t mi_guide_gradients(uint32_t guide, float const* x, float const* dirs,
 int32_t mi_guide_train_step(uint32_t guide, float const* x, float const* dirs,
                             float const* weights, uint32_t n, float learning_rate,
                             void* stream);
+
+/* Guided rendering (no counterpart in the reference; its plain estimator is
+ * mi_render).  At every non-emissive vertex the network's mixture for
+ * (position, normal, wo) and the BSDF form a one-sample mixture: the guide is
+ * sampled with probability fraction, the BSDF otherwise, and the path
+ * weight uses the mixture pdf fraction * q + (1 - fraction) * p_bsdf, as do
+ * the MIS weights against light sampling, so the estimator stays unbiased
+ * (fraction 0 is plain MIS).  With train != 0 each vertex of depth below
+ * record_depth records (inputs, sampled direction, radiance gathered after
+ * it / throughput before it), and after every wave train_steps Adam steps
+ * run over those records in slices of batch (0: all records over the
+ * steps). */
+typedef struct mi_guide_settings {
+    uint32_t guide;
+    float    fraction;
+    uint32_t train;
+    uint32_t train_steps;
+    uint32_t batch;
+    float    learning_rate;
+    uint32_t record_depth;
+} mi_guide_settings;
+
+/* As mi_render, guided; runs kernel by kernel (no graph), waiting for each
+ * wave's record count when training. */
+int32_t mi_render_guided(uint32_t scene, mi_camera const* camera,
+                         mi_render_settings const* settings, mi_guide_settings const* guide,
+                         double* accum, void* stream, mi_render_stats* stats);
 
 /* ---- synthetic scenes (host-side, deterministic; used by tests and bench) ---- */
 

@@ -80,7 +80,15 @@ class MiRenderSettings(ctypes.Structure):
 
 class MiRenderStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("paths", ctypes.c_double),
-                ("extend_rays", ctypes.c_double), ("shadow_rays", ctypes.c_double)]
+                ("extend_rays", ctypes.c_double), ("shadow_rays", ctypes.c_double),
+                ("trained", ctypes.c_double)]
+
+
+class MiGuideSettings(ctypes.Structure):
+    _fields_ = [("guide", ctypes.c_uint32), ("fraction", ctypes.c_float),
+                ("train", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
+                ("batch", ctypes.c_uint32), ("learning_rate", ctypes.c_float),
+                ("record_depth", ctypes.c_uint32)]
 
 
 MI_MATERIAL_SUBSTITUTE = 0
@@ -119,6 +127,9 @@ SIGNATURES = {
     "mi_scene_release": (_i32, [_u32]),
     "mi_render": (_i32, [_u32, ctypes.POINTER(MiCamera), ctypes.POINTER(MiRenderSettings), _vp,
                          _vp, ctypes.POINTER(MiRenderStats)]),
+    "mi_render_guided": (_i32, [_u32, ctypes.POINTER(MiCamera), ctypes.POINTER(MiRenderSettings),
+                                ctypes.POINTER(MiGuideSettings), _vp, _vp,
+                                ctypes.POINTER(MiRenderStats)]),
     "mi_ggx_e_table": (_i32, [_vp, _vp]),
     "mi_substitute_evaluate": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mi_substitute_sample": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -407,11 +407,20 @@ int32_t mi_scene_release(uint32_t scene) {
     return MI_OK;
 }
 
-int32_t mi_render(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
-                  double* accum, void* stream, mi_render_stats* stats) {
-    clear_error();
+namespace {
+
+int32_t render_call(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
+                    mi_guide_settings const* gs, double* accum, void* stream,
+                    mi_render_stats* stats) {
     if (!context) {
         return fail(MI_ERR_NO_CONTEXT, "mi_render: call mi_init first");
+    }
+    GuideNet* net = nullptr;
+    if (gs) {
+        net = lookup_guide(gs->guide);
+        if (!net) {
+            return fail(MI_ERR_HANDLE, "mi_render_guided: bad guide handle %u", gs->guide);
+        }
     }
     RenderScene* sc = lookup_scene(scene);
     if (!sc) {
@@ -438,8 +447,26 @@ int32_t mi_render(uint32_t scene, mi_camera const* camera, mi_render_settings co
     rt.refill_lanes  = context->trace.refill_lanes;
     rt.claim_batch   = context->trace.claim_batch;
     rt.lds_entries   = context->trace.lds_entries;
-    return render(*sc, rt, table, context->render_state, *camera, *settings, accum,
+    return render(*sc, rt, table, context->render_state, *camera, *settings, net, gs, accum,
                   as_stream(stream), stats);
+}
+
+}  // namespace
+
+int32_t mi_render(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
+                  double* accum, void* stream, mi_render_stats* stats) {
+    clear_error();
+    return render_call(scene, camera, settings, nullptr, accum, stream, stats);
+}
+
+int32_t mi_render_guided(uint32_t scene, mi_camera const* camera,
+                         mi_render_settings const* settings, mi_guide_settings const* guide,
+                         double* accum, void* stream, mi_render_stats* stats) {
+    clear_error();
+    if (!guide) {
+        return fail(MI_ERR_ARGS, "mi_render_guided: no guide settings");
+    }
+    return render_call(scene, camera, settings, guide, accum, stream, stats);
 }
 
 int32_t mi_ggx_e_table(float* table, void* stream) {

@@ -174,8 +174,10 @@ static_assert(16 * S_out * sizeof(float) <= 16 * S_feat * sizeof(uint16_t),
 static_assert(sizeof(Fwd_lds) <= 160 * 1024, "LDS of a workgroup");
 
 __global__ void __launch_bounds__(Fwd_waves * 64)
-    forward_kernel(uint16_t const* wt, float const* params, float const* x, uint32_t n, float* out) {
+    forward_kernel(uint16_t const* wt, float const* params, float const* x, uint32_t n_host,
+                   uint32_t const* n_dev, float* out) {
     __shared__ Fwd_lds s;
+    uint32_t const n = n_dev ? min(*n_dev, n_host) : n_host;
     load_weights(s.w, wt, params);
     __syncthreads();
 
@@ -638,7 +640,22 @@ int32_t guide_forward(GuideNet const& net, float const* x, uint32_t n, float* ou
     uint32_t const tiles  = blocks_for(n, 16);
     uint32_t const blocks = std::min<uint32_t>(blocks_for(tiles, Fwd_waves), 1024);
     hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(Fwd_waves * 64), 0, stream, net.wt,
-                       net.params, x, n, out);
+                       net.params, x, n, static_cast<uint32_t const*>(nullptr), out);
+    if (hipError_t const e = hipGetLastError(); e) {
+        return fail(MI_ERR_HIP, "guide forward: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+int32_t guide_forward_device_count(GuideNet const& net, float const* x, uint32_t const* n,
+                                   uint32_t max_n, float* out, hipStream_t stream) {
+    if (0 == max_n) {
+        return MI_OK;
+    }
+    uint32_t const tiles  = blocks_for(max_n, 16);
+    uint32_t const blocks = std::min<uint32_t>(blocks_for(tiles, Fwd_waves), 1024);
+    hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(Fwd_waves * 64), 0, stream, net.wt,
+                       net.params, x, max_n, n, out);
     if (hipError_t const e = hipGetLastError(); e) {
         return fail(MI_ERR_HIP, "guide forward: %s", hipGetErrorString(e));
     }

@@ -70,6 +70,9 @@ int32_t guide_get_params(GuideNet const& net, float* host_params);
 // x: n x 9 inputs in [0,1]; out: n x 32 raw outputs (device arrays)
 int32_t guide_forward(GuideNet const& net, float const* x, uint32_t n, float* out,
                       hipStream_t stream);
+// Same, over a query count that lives in device memory (at most max_n).
+int32_t guide_forward_device_count(GuideNet const& net, float const* x, uint32_t const* n,
+                                   uint32_t max_n, float* out, hipStream_t stream);
 // Gradient of the batch loss into net.grad (device), no update.
 int32_t guide_gradients(GuideNet& net, float const* x, float const* dirs, float const* weights,
                         uint32_t n, hipStream_t stream);
@@ -126,18 +129,26 @@ struct Vmf_mixture {
 
     // lobe by u0, direction by (u1, u2): w.mu = 1 + log(u1 + (1 - u1) e^(-2 kappa)) / kappa
     __device__ V3 sample(float u0, float u1, float u2) const {
-        int   k   = 0;
-        float acc = weight[0];
-        while (k + 1 < guide::Num_lobes && u0 >= acc) {
-            ++k;
+        // the last lobe unless the running sum passes u0 earlier (static
+        // register indices: every lobe is visited)
+        V3    m     = v3(mu[guide::Num_lobes - 1][0], mu[guide::Num_lobes - 1][1],
+                         mu[guide::Num_lobes - 1][2]);
+        float kap   = kappa[guide::Num_lobes - 1];
+        float acc   = 0.f;
+        bool  found = false;
+#pragma unroll
+        for (int k = 0; k < guide::Num_lobes - 1; ++k) {
             acc += weight[k];
+            bool const take = !found && u0 < acc;
+            m               = take ? v3(mu[k][0], mu[k][1], mu[k][2]) : m;
+            kap             = take ? kappa[k] : kap;
+            found           = found || take;
         }
-        float const kap = kappa[k];
-        float const wz  = fminf(1.f, fmaxf(-1.f, 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kap)) / kap));
-        float const r   = sqrtf(fmaxf(0.f, 1.f - wz * wz));
+        float const wz = fminf(1.f, fmaxf(-1.f, 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kap)) / kap));
+        float const r  = sqrtf(fmaxf(0.f, 1.f - wz * wz));
         float       sp, cp;
         sincosf(2.f * Pi * u2, &sp, &cp);
-        Layer const l = frame_from_normal(v3(mu[k][0], mu[k][1], mu[k][2]));
+        Layer const l = frame_from_normal(m);
         return normalize(l.tangent_to_world(v3(r * cp, r * sp, wz)));
     }
 };

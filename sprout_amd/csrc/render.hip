@@ -41,6 +41,7 @@
 #include "bvh_build.hpp"
 #include "common.hpp"
 #include "device_math.hpp"
+#include "guide.hpp"
 #include "render.hpp"
 #include "trace.hpp"
 
@@ -65,6 +66,7 @@ constexpr float Ray_max_t = 3.40282347e+38f;
 constexpr uint32_t C_queue0     = 0;  // live rays, two ping-pong queues
 constexpr uint32_t C_shadow     = 2;  // shadow rays
 constexpr uint32_t C_base       = 4;  // first sample index of the current wave
+constexpr uint32_t C_records    = 6;  // training samples recorded in the current wave
 constexpr uint32_t C_totals     = 8;  // two uint64: extension and shadow rays traced
 constexpr uint32_t Num_counters = 16;
 
@@ -100,9 +102,11 @@ constexpr uint32_t Flag_singular = 1;  // State::Treat_as_singular
 struct Path {
     float    throughput[3];
     float    radiance[3];
-    float    bsdf_pdf;  // pdf of the BSDF sample that spawned the current ray
+    float    bsdf_pdf;  // pdf of the direction sample that spawned the current ray
     uint32_t flags;
     Rng      rng;
+    uint32_t record;  // training sample of the previous vertex, or Null
+    uint32_t pad;
 };
 
 struct Shadow {
@@ -145,6 +149,20 @@ struct Wave {
     Shadow*   shadows;
     uint32_t* occluded;
     uint32_t* counters;
+    // guiding
+    float        fraction;      // probability of sampling the guide
+    float const* guide_raw;     // 32 network outputs per queue entry
+    float*       guide_x;       // 9 network inputs per queue entry
+    float        box_lo[3];
+    float        box_scale[3];  // 1 / extent of the scene box
+    uint32_t     record_depth;  // vertices of depth below this record samples
+    uint32_t     record_capacity;
+    float*       train_x;       // 9 per record
+    float*       train_dir;     // 3 per record
+    float*       train_w;       // 1 per record
+    float*       rec_t;         // throughput before the direction sample
+    float*       rec_r;         // path radiance before the next vertex
+    uint32_t*    rec_path;
 };
 
 __device__ __forceinline__ float offset_bits(float p, int32_t o) {
@@ -276,6 +294,8 @@ __global__ void __launch_bounds__(256) raygen(Wave w) {
     }
     path.bsdf_pdf = 0.f;
     path.flags    = Flag_singular;
+    path.record   = dev::Null;
+    path.pad      = 0;
     w.paths[p]    = path;
 
     mi_ray r;
@@ -325,6 +345,42 @@ __device__ __forceinline__ uint32_t pick_light(Wave const& w, float u) {
     return w.light_tris[lo];
 }
 
+// Network inputs of every queued hit: position in the scene box, geometric
+// normal and outgoing direction, each mapped to [0,1].
+__global__ void __launch_bounds__(256) guide_inputs(Wave w, uint32_t in_q) {
+    uint32_t const count  = w.counters[C_queue0 + in_q];
+    uint32_t const stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        mi_hit const h  = w.hits[i];
+        float*       xo = w.guide_x + 9 * size_t(i);
+        if (dev::Null == h.triangle) {
+            for (int k = 0; k < 9; ++k) {
+                xo[k] = 0.5f;
+            }
+            continue;
+        }
+        mi_ray const   r   = w.rays[in_q][i];
+        uint32_t const tri = h.triangle;
+        V3 const       a   = vertex(w, w.indices[3 * size_t(tri) + 0]);
+        V3 const       b   = vertex(w, w.indices[3 * size_t(tri) + 1]);
+        V3 const       c   = vertex(w, w.indices[3 * size_t(tri) + 2]);
+        float const    bw  = (1.f - h.u) - h.v;
+        V3 const       hp  = (bw * a + h.u * b) + h.v * c;
+        V3 const       n   = dev::normalize(dev::cross(b - a, c - a));
+        float const    p3[3] = {hp.x, hp.y, hp.z};
+        for (int k = 0; k < 3; ++k) {
+            xo[k] = dev::clampf((p3[k] - w.box_lo[k]) * w.box_scale[k], 0.f, 1.f);
+        }
+        xo[3] = 0.5f * (n.x + 1.f);
+        xo[4] = 0.5f * (n.y + 1.f);
+        xo[5] = 0.5f * (n.z + 1.f);
+        xo[6] = 0.5f * (1.f - r.direction[0]);
+        xo[7] = 0.5f * (1.f - r.direction[1]);
+        xo[8] = 0.5f * (1.f - r.direction[2]);
+    }
+}
+
+template <bool Guided>
 __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t depth) {
     uint32_t const count  = w.counters[C_queue0 + in_q];
     uint32_t const out_q  = in_q ^ 1;
@@ -336,15 +392,26 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
 
         bool     spawn  = false;
         bool     shadow = false;
+        bool     record = false;
         uint32_t p      = 0;
         mi_ray   next_ray;
         mi_ray   shadow_ray;
         Shadow   sh;
+        float    rec_t[3] = {0.f, 0.f, 0.f};
 
         if (i < count) {
             p              = w.ray_path[in_q][i];
             Path     path  = w.paths[p];
             mi_hit const h = w.hits[i];
+
+            if (Guided && dev::Null != path.record) {
+                // radiance before this vertex: what follows is the previous sample's
+                float* rr = w.rec_r + 3 * size_t(path.record);
+                rr[0]     = path.radiance[0];
+                rr[1]     = path.radiance[1];
+                rr[2]     = path.radiance[2];
+                path.record = dev::Null;
+            }
 
             if (dev::Null != h.triangle) {
                 mi_ray const   r   = w.rays[in_q][i];
@@ -430,7 +497,13 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                             float f[4];
                             s.evaluate(wi, f);
                             float const light_pdf = sl / (cl * w.light_area);
-                            float const weight    = predivided_power_heuristic(light_pdf, f[3]);
+                            float       dir_pdf   = f[3];
+                            if (Guided) {
+                                dev::Vmf_mixture mix;
+                                mix.set(w.guide_raw + 32 * size_t(i));
+                                dir_pdf = w.fraction * mix.pdf(wi) + (1.f - w.fraction) * f[3];
+                            }
+                            float const weight = predivided_power_heuristic(light_pdf, dir_pdf);
                             Render_material const lm = w.materials[w.tri_material[lt]];
                             for (int k = 0; k < 3; ++k) {
                                 sh.contrib[k] =
@@ -450,12 +523,47 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                         }
                     }
 
-                    // BSDF importance sample
-                    float const      xl = path.rng.uniform();
-                    float const      x0 = path.rng.uniform();
-                    float const      x1 = path.rng.uniform();
+                    // BSDF importance sample; guided: one-sample mixture of
+                    // the guide (probability fraction) and the BSDF
                     dev::Bxdf_sample bs;
-                    s.sample(xl, x0, x1, bs);
+                    if (Guided) {
+                        dev::Vmf_mixture mix;
+                        mix.set(w.guide_raw + 32 * size_t(i));
+                        float const pick = path.rng.uniform();
+                        float const xl   = path.rng.uniform();
+                        float const x0   = path.rng.uniform();
+                        float const x1   = path.rng.uniform();
+                        if (pick < w.fraction) {
+                            V3 const gw = mix.sample(xl, x0, x1);
+                            float    f[4];
+                            s.evaluate(gw, f);
+                            bs.wi      = gw;
+                            bs.refl[0] = f[0];
+                            bs.refl[1] = f[1];
+                            bs.refl[2] = f[2];
+                            bs.pdf     = f[3];
+                            bs.type    = dev::Type_diffuse;
+                            if (!(dev::dot(geo_n, gw) > 0.f)) {
+                                bs.pdf = 0.f;  // under the surface: the reflection is zero
+                            }
+                        } else {
+                            s.sample(xl, x0, x1, bs);
+                        }
+                        if (bs.pdf > 0.f) {
+                            bs.pdf = w.fraction * mix.pdf(bs.wi) + (1.f - w.fraction) * bs.pdf;
+                        }
+                        if (bs.pdf > 0.f && depth < w.record_depth) {
+                            record   = true;
+                            rec_t[0] = path.throughput[0];
+                            rec_t[1] = path.throughput[1];
+                            rec_t[2] = path.throughput[2];
+                        }
+                    } else {
+                        float const xl = path.rng.uniform();
+                        float const x0 = path.rng.uniform();
+                        float const x1 = path.rng.uniform();
+                        s.sample(xl, x0, x1, bs);
+                    }
                     if (!(bs.pdf > 0.f)) {
                         live = false;
                     } else {
@@ -480,6 +588,26 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                         next_ray.max_t        = Ray_max_t;
                         spawn                 = true;
                     }
+                }
+            }
+            if (Guided) {
+                record = record && spawn;
+                uint32_t const slot = queue_slot(record, &w.counters[C_records]);
+                if (record && slot < w.record_capacity) {
+                    float const* xi = w.guide_x + 9 * size_t(i);
+                    float*       xo = w.train_x + 9 * size_t(slot);
+                    for (int k = 0; k < 9; ++k) {
+                        xo[k] = xi[k];
+                    }
+                    w.train_dir[3 * size_t(slot) + 0] = next_ray.direction[0];
+                    w.train_dir[3 * size_t(slot) + 1] = next_ray.direction[1];
+                    w.train_dir[3 * size_t(slot) + 2] = next_ray.direction[2];
+                    for (int k = 0; k < 3; ++k) {
+                        w.rec_t[3 * size_t(slot) + k] = rec_t[k];
+                        w.rec_r[3 * size_t(slot) + k] = path.radiance[k];
+                    }
+                    w.rec_path[slot] = p;
+                    path.record      = slot;
                 }
             }
             w.paths[p] = path;
@@ -511,6 +639,29 @@ __global__ void __launch_bounds__(256) shadow_resolve(Wave w) {
             p.radiance[k] += s.contrib[k];
         }
     }
+}
+
+// Training weight of every record: average over rgb of (radiance the path
+// gathered after the record's vertex) / (throughput before its direction
+// sample) = incident radiance x reflection / sampling pdf.
+__global__ void __launch_bounds__(256) finish_records(Wave w) {
+    uint32_t const count  = min(w.counters[C_records], w.record_capacity);
+    uint32_t const stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        Path const& p   = w.paths[w.rec_path[i]];
+        float       sum = 0.f;
+        for (int k = 0; k < 3; ++k) {
+            float const t = w.rec_t[3 * size_t(i) + k];
+            float const d = p.radiance[k] - w.rec_r[3 * size_t(i) + k];
+            sum += t > 0.f ? fmaxf(d, 0.f) / t : 0.f;
+        }
+        float const v = sum * (1.f / 3.f);
+        w.train_w[i]  = isfinite(v) ? v : 0.f;
+    }
+}
+
+__global__ void reset_records(uint32_t* counters) {
+    counters[C_records] = 0;
 }
 
 // Sums the wave's sample planes per pixel, in plane order, into the FP64
@@ -781,6 +932,16 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
     s.num_materials = num_materials;
     s.num_lights    = uint32_t(lights.size());
     s.light_area    = float(total);
+    for (int k = 0; k < 3; ++k) {
+        s.box_lo[k] = positions[k];
+        s.box_hi[k] = positions[k];
+    }
+    for (uint32_t v = 0; v < num_vertices; ++v) {
+        for (int k = 0; k < 3; ++k) {
+            s.box_lo[k] = std::min(s.box_lo[k], positions[3 * size_t(v) + k]);
+            s.box_hi[k] = std::max(s.box_hi[k], positions[3 * size_t(v) + k]);
+        }
+    }
 
     size_t const nl = std::max<size_t>(1, lights.size());
     hipError_t   e  = hipMalloc(&s.positions, size_t(num_vertices) * 3 * sizeof(float));
@@ -820,6 +981,14 @@ void RenderState::release() {
     (void)hipFree(shadows);
     (void)hipFree(occluded);
     (void)hipFree(counters);
+    (void)hipFree(guide_x);
+    (void)hipFree(guide_raw);
+    (void)hipFree(train_x);
+    (void)hipFree(train_dir);
+    (void)hipFree(train_w);
+    (void)hipFree(rec_t);
+    (void)hipFree(rec_r);
+    (void)hipFree(rec_path);
     if (begin) {
         (void)hipEventDestroy(begin);
     }
@@ -873,9 +1042,52 @@ int32_t RenderState::reserve(uint32_t n) {
     return MI_OK;
 }
 
+int32_t RenderState::reserve_guiding(uint32_t n, uint32_t records) {
+    if (n > guide_capacity) {
+        (void)hipFree(guide_x);
+        (void)hipFree(guide_raw);
+        guide_x = guide_raw = nullptr;
+        guide_capacity      = 0;
+        hipError_t e        = hipMalloc(&guide_x, size_t(n) * 9 * sizeof(float));
+        if (!e) e = hipMalloc(&guide_raw, size_t(n) * 32 * sizeof(float));
+        if (e) {
+            return fail(MI_ERR_HIP, "guiding buffers of %u paths: %s", n, hipGetErrorString(e));
+        }
+        guide_capacity = n;
+    }
+    if (records > record_capacity) {
+        (void)hipFree(train_x);
+        (void)hipFree(train_dir);
+        (void)hipFree(train_w);
+        (void)hipFree(rec_t);
+        (void)hipFree(rec_r);
+        (void)hipFree(rec_path);
+        train_x = train_dir = train_w = rec_t = rec_r = nullptr;
+        rec_path        = nullptr;
+        record_capacity = 0;
+        hipError_t e    = hipMalloc(&train_x, size_t(records) * 9 * sizeof(float));
+        if (!e) e = hipMalloc(&train_dir, size_t(records) * 3 * sizeof(float));
+        if (!e) e = hipMalloc(&train_w, size_t(records) * sizeof(float));
+        if (!e) e = hipMalloc(&rec_t, size_t(records) * 3 * sizeof(float));
+        if (!e) e = hipMalloc(&rec_r, size_t(records) * 3 * sizeof(float));
+        if (!e) e = hipMalloc(&rec_path, size_t(records) * sizeof(uint32_t));
+        if (e) {
+            return fail(MI_ERR_HIP, "training records (%u): %s", records, hipGetErrorString(e));
+        }
+        record_capacity = records;
+    }
+    return MI_OK;
+}
+
 int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_table,
                RenderState& state, mi_camera const& cam, mi_render_settings const& set,
-               double* accum, hipStream_t stream, mi_render_stats* stats) {
+               GuideNet* net, mi_guide_settings const* gs, double* accum, hipStream_t stream,
+               mi_render_stats* stats) {
+    bool const guided = nullptr != net;
+    if (guided && (!gs || !(gs->fraction >= 0.f) || !(gs->fraction <= 1.f) ||
+                   (gs->train && (0 == gs->train_steps || !(gs->learning_rate > 0.f))))) {
+        return fail(MI_ERR_ARGS, "mi_render_guided: bad guide settings");
+    }
     uint64_t const num_pixels64 = uint64_t(cam.width) * cam.height;
     uint32_t const spw          = set.samples_per_wave;
     uint64_t const paths64      = num_pixels64 * spw;
@@ -919,6 +1131,15 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     if (int32_t const st = prepare_trace(scene.bvh, trace, num_paths); st) {
         return st;
     }
+    uint32_t record_depth = 0;
+    uint32_t records      = 0;
+    if (guided) {
+        record_depth = gs->train ? std::min(gs->record_depth, set.max_bounces + 1) : 0;
+        records      = uint32_t(std::min<uint64_t>(uint64_t(num_paths) * record_depth, 1ull << 26));
+        if (int32_t const st = state.reserve_guiding(num_paths, std::max<uint32_t>(records, 1)); st) {
+            return st;
+        }
+    }
 
     Wave w;
     w.positions    = scene.positions;
@@ -955,6 +1176,22 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.shadows     = static_cast<Shadow*>(state.shadows);
     w.occluded    = state.occluded;
     w.counters    = state.counters;
+    w.fraction        = guided ? gs->fraction : 0.f;
+    w.guide_raw       = state.guide_raw;
+    w.guide_x         = state.guide_x;
+    w.record_depth    = record_depth;
+    w.record_capacity = records;
+    w.train_x         = state.train_x;
+    w.train_dir       = state.train_dir;
+    w.train_w         = state.train_w;
+    w.rec_t           = state.rec_t;
+    w.rec_r           = state.rec_r;
+    w.rec_path        = state.rec_path;
+    for (int k = 0; k < 3; ++k) {
+        float const ext = scene.box_hi[k] - scene.box_lo[k];
+        w.box_lo[k]     = scene.box_lo[k];
+        w.box_scale[k]  = ext > 0.f ? 1.f / ext : 0.f;
+    }
 
     hipStream_t const s         = state.stream;
     uint32_t const    waves     = set.samples_per_pixel / spw;
@@ -973,7 +1210,18 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
                 return st;
             }
             hipLaunchKernelGGL(reset_queues, dim3(1), dim3(1), 0, s, state.counters, q ^ 1);
-            hipLaunchKernelGGL(shade, dim3(grid_path), dim3(256), 0, s, w, q, d);
+            if (guided) {
+                hipLaunchKernelGGL(guide_inputs, dim3(grid_path), dim3(256), 0, s, w, q);
+                if (int32_t const st = guide_forward_device_count(
+                        *net, state.guide_x, &state.counters[C_queue0 + q], num_paths,
+                        state.guide_raw, s);
+                    st) {
+                    return st;
+                }
+                hipLaunchKernelGGL(shade<true>, dim3(grid_path), dim3(256), 0, s, w, q, d);
+            } else {
+                hipLaunchKernelGGL(shade<false>, dim3(grid_path), dim3(256), 0, s, w, q, d);
+            }
             hipLaunchKernelGGL(tally, dim3(1), dim3(1), 0, s, state.counters, q);
             if (int32_t const st = launch_trace_device_count(scene.bvh, trace, state.shadow_rays,
                                                              &state.counters[C_shadow], num_paths,
@@ -989,6 +1237,40 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         return MI_OK;
     };
 
+    // online training on the wave's records: train_steps Adam steps over
+    // consecutive slices of the records
+    uint64_t trained = 0;
+    auto train_wave = [&]() -> int32_t {
+        hipLaunchKernelGGL(finish_records, dim3(grid_path), dim3(256), 0, s, w);
+        uint32_t count = 0;
+        hipError_t e   = hipMemcpyAsync(&count, state.counters + C_records, sizeof(uint32_t),
+                                        hipMemcpyDeviceToHost, s);
+        if (!e) e = hipStreamSynchronize(s);
+        if (e) {
+            return fail(MI_ERR_HIP, "mi_render_guided: %s", hipGetErrorString(e));
+        }
+        hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
+        count = std::min(count, records);
+        if (0 == count) {
+            return MI_OK;
+        }
+        uint32_t const steps = gs->train_steps;
+        uint32_t const batch = gs->batch ? std::min(gs->batch, count)
+                                         : std::max<uint32_t>(1, (count + steps - 1) / steps);
+        for (uint32_t k = 0; k < steps; ++k) {
+            uint32_t const off = uint32_t((uint64_t(k) * batch) % count);
+            uint32_t const nb  = std::min(batch, count - off);
+            if (int32_t const st = guide_adam_step(*net, state.train_x + 9 * size_t(off),
+                                                   state.train_dir + 3 * size_t(off),
+                                                   state.train_w + off, nb, gs->learning_rate, s);
+                st) {
+                return st;
+            }
+            trained += nb;
+        }
+        return MI_OK;
+    };
+
     // order after the caller's earlier work on its stream
     hipError_t e = hipEventRecord(state.begin, stream);
     if (!e) e = hipStreamWaitEvent(s, state.begin, 0);
@@ -996,11 +1278,19 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         return fail(MI_ERR_HIP, "mi_render: %s", hipGetErrorString(e));
     }
     hipLaunchKernelGGL(set_base, dim3(1), dim3(1), 0, s, state.counters, set.first_sample);
+    if (guided) {
+        hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
+    }
 
-    if (set.flags & MI_RENDER_NO_GRAPH) {
+    if (guided || (set.flags & MI_RENDER_NO_GRAPH)) {
         for (uint32_t wave = 0; wave < waves; ++wave) {
             if (int32_t const st = enqueue_wave(); st) {
                 return st;
+            }
+            if (guided && gs->train) {
+                if (int32_t const st = train_wave(); st) {
+                    return st;
+                }
             }
         }
     } else {
@@ -1056,6 +1346,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         stats->paths       = double(num_paths) * waves;
         stats->extend_rays = double(totals[0]);
         stats->shadow_rays = double(totals[1]);
+        stats->trained     = double(trained);
     }
     return MI_OK;
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "guide.hpp"
 #include "trace.hpp"
 
 #include <hip/hip_runtime.h>
@@ -35,6 +36,8 @@ struct RenderScene {
     uint32_t num_materials = 0;
     uint32_t num_lights    = 0;
     float    light_area    = 0.f;  // total emissive area
+    float    box_lo[3]     = {0.f, 0.f, 0.f};  // bounds of all vertices
+    float    box_hi[3]     = {0.f, 0.f, 0.f};
 
     void release();
 };
@@ -53,8 +56,20 @@ struct RenderState {
     hipStream_t  stream      = nullptr;  // internal stream the bounce loop runs (and is captured) on
     hipEvent_t   begin       = nullptr;
     hipEvent_t   end         = nullptr;
+    // guiding
+    uint32_t  guide_capacity  = 0;
+    float*    guide_x         = nullptr;  // network inputs per queue entry
+    float*    guide_raw       = nullptr;  // network outputs per queue entry
+    uint32_t  record_capacity = 0;
+    float*    train_x         = nullptr;
+    float*    train_dir       = nullptr;
+    float*    train_w         = nullptr;
+    float*    rec_t           = nullptr;
+    float*    rec_r           = nullptr;
+    uint32_t* rec_path        = nullptr;
 
     int32_t reserve(uint32_t num_paths);
+    int32_t reserve_guiding(uint32_t num_paths, uint32_t records);
     void    release();
 };
 
@@ -70,8 +85,10 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
                      mi_material const* materials, uint32_t num_materials,
                      mi_bvh_settings const& settings, RenderScene& out);
 
+// net == nullptr: plain MIS; else guided by net with the settings gs.
 int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_table,
                RenderState& state, mi_camera const& camera, mi_render_settings const& settings,
-               double* accum, hipStream_t stream, mi_render_stats* stats);
+               GuideNet* net, mi_guide_settings const* gs, double* accum, hipStream_t stream,
+               mi_render_stats* stats);
 
 }  // namespace mi

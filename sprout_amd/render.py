@@ -99,3 +99,28 @@ def substitute_evaluate(params, frames, wo, wi) -> np.ndarray:
 def substitute_sample(params, frames, wo, xi) -> np.ndarray:
     """(n, 8): wi, reflection (rgb), pdf, type bits (as float32 bits)."""
     return _batch("mi_substitute_sample", 8, params, frames, wo, xi)
+
+
+def render_guided(scene: Scene, cam: _lib.MiCamera, spp: int, guide, fraction: float = 0.5,
+                  train: bool = True, train_steps: int = 4, batch: int = 0,
+                  learning_rate: float = 1e-3, record_depth: int = 4, samples_per_wave: int = 1,
+                  min_bounces: int = 4, max_bounces: int = 8, seed: int = 1,
+                  first_sample: int = 0, accum: torch.Tensor | None = None, stats: bool = False):
+    """mi_render_guided: as render(), sampling directions from a one-sample
+    mixture of the guide (probability fraction) and the BSDF, and training
+    the guide online after every wave when train is set."""
+    if accum is None:
+        accum = torch.zeros((cam.height, cam.width, 3), dtype=torch.float64, device="cuda")
+    if accum.dtype != torch.float64 or not accum.is_contiguous() or \
+            tuple(accum.shape) != (cam.height, cam.width, 3):
+        raise ValueError("accum must be a contiguous (height, width, 3) float64 device tensor")
+    st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces, 0,
+                               seed)
+    gs = _lib.MiGuideSettings(guide.handle, fraction, 1 if train else 0, train_steps, batch,
+                              learning_rate, record_depth)
+    out = _lib.MiRenderStats() if stats else None
+    check(_lib.load().mi_render_guided(scene.handle, ctypes.byref(cam), ctypes.byref(st),
+                                       ctypes.byref(gs), ctypes.c_void_p(accum.data_ptr()),
+                                       _stream(), ctypes.byref(out) if out is not None else None),
+          "mi_render_guided")
+    return accum, out
