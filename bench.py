@@ -137,12 +137,90 @@ def room_render(frames: int, cpu: bool):
 
         threads = min(16, os.cpu_count() or 1)
         t0 = time.perf_counter()
-        O.render(pos, idx, tmat, mats, 256, 256, 4, scenes.ROOM_CAMERA, seed=1, threads=threads)
+        from sprout_amd import lights
+
+        tree = O.scene_light_tree(pos, idx, tmat, mats, lights.build_tree)
+        O.render(pos, idx, tmat, mats, 256, 256, 4, scenes.ROOM_CAMERA, tree, seed=1,
+                 threads=threads)
         dt = time.perf_counter() - t0
         out["cpu_port_samples_per_s"] = round(256 * 256 * 4 / dt, 1)
         out["cpu_port_threads"] = threads
         out["cpu_port_sample"] = "256x256 x 4 spp of the same room, oracle/pt.c"
     return out
+
+
+SLIT_REF = os.path.join(ROOT, "tests", "golden", "slit_room_ref_256.npz")
+
+
+def rel_mse(x, r, eps=1e-2):
+    """mean over pixels and channels of (x - r)^2 / (r^2 + eps), eps = 1e-2"""
+    return float(np.mean((x - r) ** 2 / (r * r + eps)))
+
+
+def slit_equal_time(seconds: float):
+    """Slit room (light behind walls open through a 0.05-wide slit), 256 x 256,
+    max depth 8: plain MIS and guided MIS (trained online from scratch inside
+    the same time budget) each render for `seconds`; relMSE of each against
+    the 65536-spp FP64 reference image."""
+    import torch
+
+    from sprout_amd import guide, render, scenes
+
+    if not os.path.exists(SLIT_REF):
+        return {"skipped": "reference image missing (tests/golden/make_slit_reference.py)"}
+    with np.load(SLIT_REF, allow_pickle=False) as z:
+        ref = z["image"].astype(np.float64)
+        ref_spp = int(z["spp"])
+    pos, idx, tmat, mats = scenes.room(0.05)
+    sc = render.Scene(pos, idx, tmat, mats)
+    cam = render.camera(256, 256, **scenes.ROOM_CAMERA)
+
+    # warm up both paths (kernels, buffers)
+    g = guide.Guide(7)
+    render.render(sc, cam, 4, samples_per_wave=4, seed=2)
+    render.render_guided(sc, cam, 4, g, samples_per_wave=4, seed=3)
+    torch.cuda.synchronize()
+    g.release()
+
+    acc = torch.zeros((256, 256, 3), dtype=torch.float64, device="cuda")
+    spp = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        render.render(sc, cam, 16, samples_per_wave=16, seed=11, first_sample=spp, accum=acc)
+        torch.cuda.synchronize()
+        spp += 16
+    t_plain = time.perf_counter() - t0
+    e_plain = rel_mse(acc.cpu().numpy() / spp, ref)
+
+    g = guide.Guide(8)
+    acc.zero_()
+    gspp = 0
+    trained = 0.0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        _, st = render.render_guided(sc, cam, 4, g, fraction=0.5, train=True, train_steps=4,
+                                     learning_rate=2e-3, record_depth=4, samples_per_wave=4,
+                                     seed=12, first_sample=gspp, accum=acc, stats=True)
+        gspp += 4
+        trained += st.trained
+    torch.cuda.synchronize()
+    t_guided = time.perf_counter() - t0
+    e_guided = rel_mse(acc.cpu().numpy() / gspp, ref)
+    g.release()
+    sc.release()
+    return {
+        "workload": "slit room 256x256, max depth 8, RR from depth 4; reference 65536 spp FP64",
+        "seconds_plain": round(t_plain, 3),
+        "seconds_guided": round(t_guided, 3),
+        "spp_plain": spp,
+        "spp_guided": gspp,
+        "relmse_plain": round(e_plain, 6),
+        "relmse_guided": round(e_guided, 6),
+        "relmse_ratio_plain_over_guided": round(e_plain / e_guided, 3),
+        "relmse_eps": 1e-2,
+        "guide_samples_trained": trained,
+        "reference_spp": ref_spp,
+    }
 
 
 def main() -> int:
@@ -152,6 +230,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--width", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--guide-seconds", type=float, default=10.0,
+                    help="time budget of each side of the guided vs plain comparison")
     ap.add_argument("--copy-probe", action="store_true",
                     help="launch one 256 MiB device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -238,7 +318,7 @@ def main() -> int:
         cpu = cpu_baseline(pos, idx, bounce, threads)
 
     line = {
-        "metric": "Grays/s closest-hit",
+        "metric": "Grays/s closest-hit; relMSE at equal time, guided vs plain GPU MIS PT",
         "value": round(value, 4),
         "unit": "Grays/s",
         "n_gpus": 1,
@@ -284,6 +364,8 @@ def main() -> int:
     line["extra"]["bounce_hit_fraction"] = round(hit_frac, 4)
     log("box room path tracing")
     line["extra"]["room_render"] = room_render(5, not args.no_cpu_baseline)
+    log("slit room: guided vs plain at equal time")
+    line["extra"]["guiding"] = slit_equal_time(args.guide_seconds)
     print(json.dumps(line), flush=True)
     return 0
 

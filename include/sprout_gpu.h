@@ -209,7 +209,10 @@ typedef struct mi_render_settings {
 } mi_render_settings;
 
 enum mi_render_flags {
-    MI_RENDER_NO_GRAPH = 1, /* launch the bounce loop kernel by kernel instead of as a hipGraph */
+    MI_RENDER_NO_GRAPH       = 1, /* launch the bounce loop kernel by kernel instead of as a hipGraph */
+    MI_RENDER_NO_LIGHT_SPLIT = 2, /* one light per vertex (sprout's Light_sampling::Single); by
+                                     default the light tree splits at depths 0-2 (Adaptive,
+                                     take_loader.cpp:624, pathtracer_mis.cpp:423-426) */
 };
 
 typedef struct mi_render_stats {
@@ -220,9 +223,10 @@ typedef struct mi_render_stats {
     double trained;      /* guiding samples trained on (mi_render_guided) */
 } mi_render_stats;
 
-/* Creates a scene from HOST arrays: triangles with one material each; the
- * emissive triangles (kind LIGHT) form the scene's light, sampled uniformly
- * by area.  The tree is built on the device with *bvh_settings (NULL:
+/* Creates a scene from HOST arrays: triangles with one material each; every
+ * emissive triangle (kind LIGHT) is one light of the scene's light tree
+ * (light::Tree, core/scene/light/light_tree.cpp, built on the host as
+ * Tree_builder does, light_tree_builder.cpp:436-540).  The tree is built on the device with *bvh_settings (NULL:
  * {16, 64, 4}).  Mirrors su_create_triangle_mesh_async + material creation +
  * Scene::compile (capi.cpp:378-406, scene.cpp:216-267). */
 int32_t mi_scene_create(float const* positions, uint32_t num_vertices, uint32_t const* indices,
@@ -242,6 +246,49 @@ int32_t mi_scene_release(uint32_t scene);
  * boundary of Driver::render_frame_forward (rendering_driver.cpp:295-360). */
 int32_t mi_render(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
                   double* accum, void* stream, mi_render_stats* stats);
+
+/* ---- light tree ---- */
+
+/* One node of the light tree, 48 bytes: light::Node (light_tree.hpp:32-65). */
+typedef struct mi_light_node {
+    float    center[3];
+    float    radius;
+    float    cone[4]; /* axis, cos of the opening angle */
+    float    power;
+    float    variance;
+    uint32_t bits; /* has_children : 1, two_sided : 1, children_or_light : 30 */
+    uint32_t num_lights;
+} mi_light_node;
+
+/* Builds the light tree of num_lights lights on the host, as Tree_builder
+ * builds the scene tree (light_tree_builder.cpp:436-540, finite lights):
+ * per light a box (min, max: 3 floats each), a cone (4 floats), a power and
+ * a two-sided flag.  HOST arrays; nodes needs 2 * num_lights - 1 entries
+ * (NULL: query *num_nodes only); middles as many; mapping (order -> light)
+ * and orders (light -> order) num_lights each.  No GPU needed. */
+int32_t mi_light_tree_build(float const* box_min, float const* box_max, float const* cones,
+                            float const* powers, uint8_t const* two_sided, uint32_t num_lights,
+                            mi_light_node* nodes, uint32_t* middles, uint32_t* mapping,
+                            uint32_t* orders, uint32_t* num_nodes);
+
+/* Device queries of a scene's light tree (every emissive triangle is a light;
+ * light ids number them in triangle order).  DEVICE arrays: points and
+ * normals n x 3.
+ * sample: randoms n -> picks n x Max (64) x {light, pdf as float bits} and
+ *         counts n (Tree::random_light, light_tree.cpp:363-450)
+ * pdf:    lights n -> pdf n (Tree::pdf, light_tree.cpp:564-635)            */
+int32_t mi_light_tree_sample(uint32_t scene, float const* points, float const* normals,
+                             float const* randoms, uint32_t split, uint32_t n, uint32_t* picks,
+                             uint32_t* counts, void* stream);
+int32_t mi_light_tree_pdf(uint32_t scene, float const* points, float const* normals,
+                          uint32_t const* lights, uint32_t split, uint32_t n, float* pdfs,
+                          void* stream);
+/* HOST: the scene's tree and lights, for checking (sizes from *num_nodes,
+ * *num_lights; NULL arrays query them).  infos: per light center (3),
+ * radius, cone (4), power, two_sided, triangle, area (12 words). */
+int32_t mi_scene_light_tree(uint32_t scene, mi_light_node* nodes, uint32_t* middles,
+                            uint32_t* mapping, uint32_t* orders, void* infos,
+                            uint32_t* num_nodes, uint32_t* num_lights);
 
 /* The 32 x 32 GGX energy table E(n.wo, alpha) (rows alpha) regenerated on the
  * device by sprout's procedure (ggx_integrate.cpp:17-45, 202-240; the data of

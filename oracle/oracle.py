@@ -55,7 +55,24 @@ class _OraclePtSettings(ctypes.Structure):
     _fields_ = [("samples_per_pixel", ctypes.c_uint32), ("first_sample", ctypes.c_uint32),
                 ("min_bounces", ctypes.c_uint32), ("max_bounces", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64), ("mode", ctypes.c_int32),
-                ("num_threads", ctypes.c_uint32)]
+                ("num_threads", ctypes.c_uint32), ("split_lights", ctypes.c_int32)]
+
+
+class _OracleLightTree(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_void_p), ("middles", ctypes.c_void_p),
+                ("mapping", ctypes.c_void_p), ("orders", ctypes.c_void_p),
+                ("lights", ctypes.c_void_p), ("num_nodes", ctypes.c_uint32),
+                ("num_lights", ctypes.c_uint32)]
+
+
+# oracle_light / the library's light records (48 bytes)
+LIGHT_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("cone", "<f4", 4),
+                        ("power", "<f4"), ("two_sided", "<u4"), ("triangle", "<u4"),
+                        ("area", "<f4")])
+# light::Node (48 bytes)
+LIGHT_NODE_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("cone", "<f4", 4),
+                             ("power", "<f4"), ("variance", "<f4"), ("bits", "<u4"),
+                             ("num_lights", "<u4")])
 
 
 PT_MIS, PT_BSDF, PT_NEE = 0, 1, 2
@@ -109,8 +126,16 @@ def lib():
         _lib.oracle_render.restype = ctypes.c_int32
         _lib.oracle_render.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                        ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
-                                       ctypes.c_uint32, ctypes.POINTER(_OracleCamera),
+                                       ctypes.c_uint32, ctypes.POINTER(_OracleLightTree),
+                                       ctypes.POINTER(_OracleCamera),
                                        ctypes.POINTER(_OraclePtSettings), ctypes.c_void_p]
+        _lib.oracle_light_random.restype = ctypes.c_uint32
+        _lib.oracle_light_random.argtypes = [ctypes.POINTER(_OracleLightTree), ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
+                                             ctypes.c_void_p, ctypes.c_void_p]
+        _lib.oracle_light_pdf.restype = ctypes.c_float
+        _lib.oracle_light_pdf.argtypes = [ctypes.POINTER(_OracleLightTree), ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32]
     return _lib
 
 
@@ -343,10 +368,96 @@ def nodes_equal(a: np.ndarray, b: np.ndarray) -> bool:
     )
 
 
+def light_records(positions, indices, triangle_materials, materials) -> np.ndarray:
+    """One light per emissive triangle (kind 2, positive area and radiance), in
+    triangle order: box center, half diagonal, cone (normal, cos 1), power =
+    area * max_component(radiance) (light.cpp:38-48, scene.cpp:504-505),
+    area = 0.5 |cross(b - a, c - a)| in float32."""
+    pos = np.asarray(positions, np.float32)
+    idx = np.asarray(indices, np.uint32)
+    mats = np.asarray(materials).view(MATERIAL_DTYPE)
+    out = []
+    f32 = np.float32
+    for t in range(len(idx)):
+        m = mats[triangle_materials[t]]
+        if m["kind"] != 2:
+            continue
+        a, b, c = (pos[idx[t, k]] for k in range(3))
+        e1 = (b - a).astype(f32)
+        e2 = (c - a).astype(f32)
+        n = np.array([e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                      e1[0] * e2[1] - e1[1] * e2[0]], f32)
+        ln = f32(np.sqrt(f32(f32(n[0] * n[0] + n[1] * n[1]) + n[2] * n[2])))
+        area = f32(0.5) * ln
+        radiance = f32(max(m["emission"]))
+        if not (area > 0 and radiance > 0):
+            continue
+        lo = np.minimum(np.minimum(a, b), c)
+        hi = np.maximum(np.maximum(a, b), c)
+        ext = (hi - lo).astype(f32)
+        r = np.zeros(1, LIGHT_DTYPE)[0]
+        r["center"] = f32(0.5) * (lo + hi)
+        r["radius"] = f32(0.5) * f32(np.sqrt(f32(f32(ext[0] * ext[0] + ext[1] * ext[1]) +
+                                                 ext[2] * ext[2])))
+        r["cone"][:3] = n / ln
+        r["cone"][3] = 1.0
+        r["power"] = area * radiance
+        r["triangle"] = t
+        r["area"] = area
+        out.append(r)
+    return np.array(out, LIGHT_DTYPE)
+
+
+class LightTree:
+    """A built light tree (nodes, middles, mapping, orders) with its lights."""
+
+    def __init__(self, nodes, middles, mapping, orders, lights):
+        self.nodes = np.ascontiguousarray(nodes).view(LIGHT_NODE_DTYPE)
+        self.middles = np.ascontiguousarray(middles, np.uint32)
+        self.mapping = np.ascontiguousarray(mapping, np.uint32)
+        self.orders = np.ascontiguousarray(orders, np.uint32)
+        self.lights = np.ascontiguousarray(lights).view(LIGHT_DTYPE)
+        self.c = _OracleLightTree(_ptr(self.nodes), _ptr(self.middles), _ptr(self.mapping),
+                                  _ptr(self.orders), _ptr(self.lights),
+                                  len(self.nodes) if len(self.lights) else 0, len(self.lights))
+
+    def random_light(self, p, n, random, split):
+        pv = np.ascontiguousarray(p, np.float32)
+        nv = np.ascontiguousarray(n, np.float32)
+        lights = np.zeros(64, np.uint32)
+        pdfs = np.zeros(64, np.float32)
+        k = lib().oracle_light_random(ctypes.byref(self.c), _ptr(pv), _ptr(nv), random,
+                                      1 if split else 0, _ptr(lights), _ptr(pdfs))
+        return lights[:k].copy(), pdfs[:k].copy()
+
+    def pdf(self, p, n, split, light) -> float:
+        pv = np.ascontiguousarray(p, np.float32)
+        nv = np.ascontiguousarray(n, np.float32)
+        return float(lib().oracle_light_pdf(ctypes.byref(self.c), _ptr(pv), _ptr(nv),
+                                            1 if split else 0, int(light)))
+
+
+def scene_light_tree(positions, indices, triangle_materials, materials, build) -> LightTree:
+    """The light tree of a scene: lights from light_records, the tree over
+    them from build(box_min, box_max, cones, powers, two_sided) -> (nodes,
+    middles, mapping, orders), the caller's tree builder."""
+    rec = light_records(positions, indices, triangle_materials, materials)
+    if len(rec) == 0:
+        empty = np.zeros(0, np.uint32)
+        return LightTree(np.zeros(1, LIGHT_NODE_DTYPE), empty, empty, empty, rec)
+    pos = np.asarray(positions, np.float32)
+    tris = pos[np.asarray(indices)[rec["triangle"]]]
+    nodes, middles, mapping, orders = build(tris.min(axis=1), tris.max(axis=1), rec["cone"],
+                                            rec["power"], rec["two_sided"])
+    return LightTree(nodes, middles, mapping, orders, rec)
+
+
 def render(positions, indices, triangle_materials, materials, width, height, spp, camera,
-           first_sample=0, min_bounces=4, max_bounces=8, seed=1, mode=PT_MIS, threads=None):
+           tree: LightTree, first_sample=0, min_bounces=4, max_bounces=8, seed=1, mode=PT_MIS,
+           threads=None, split_lights=True):
     """Radiance sums per pixel, (height, width, 3) float64, of the CPU path
-    tracer (pt.c).  camera: dict with eye, target, up, fov_deg."""
+    tracer (pt.c).  camera: dict with eye, target, up, fov_deg; tree: the
+    scene's light tree."""
     pos = np.ascontiguousarray(positions, np.float32)
     idx = np.ascontiguousarray(indices, np.uint32)
     tmat = np.ascontiguousarray(triangle_materials, np.uint32)
@@ -355,11 +466,12 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
                         (ctypes.c_float * 3)(*camera["target"]),
                         (ctypes.c_float * 3)(*camera["up"]), camera["fov_deg"])
     n = threads or min(os.cpu_count() or 1, 16)
-    st = _OraclePtSettings(spp, first_sample, min_bounces, max_bounces, seed, mode, n)
+    st = _OraclePtSettings(spp, first_sample, min_bounces, max_bounces, seed, mode, n,
+                           1 if split_lights else 0)
     accum = np.zeros((height, width, 3), np.float64)
     status = lib().oracle_render(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(tmat),
-                                 _ptr(mats), len(mats), ctypes.byref(cam), ctypes.byref(st),
-                                 _ptr(accum))
+                                 _ptr(mats), len(mats), ctypes.byref(tree.c), ctypes.byref(cam),
+                                 ctypes.byref(st), _ptr(accum))
     if status:
         raise RuntimeError(f"oracle_render: status {status}")
     return accum

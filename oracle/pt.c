@@ -17,10 +17,11 @@
  * Paths draw their random numbers from PCG32 started as start(seed,
  * sample * W * H + y * W + x), in this order: 2 for the pixel jitter; per
  * vertex 1 for Russian roulette when it applies, 3 for the light sample when
- * the scene has lights, 3 for the BSDF sample.  Emitters are chosen by area
- * among all emissive triangles (the reference's single-light tree pick times
- * its per-part area choice); the pinhole camera jitters uniformly inside the
- * pixel.  Closest and any hits come from the oracle's own SBVH (sbvh.c,
+ * the scene has lights (1 to pick them, then 2 per picked light), 3 for the
+ * BSDF sample.  Lights (one per emissive triangle) are picked by the light
+ * tree (light_tree.c), split below depth 3 when asked (sprout's Adaptive
+ * light sampling, pathtracer_mis.cpp:423-426); the pinhole camera jitters
+ * uniformly inside the pixel.  Closest and any hits come from the oracle's own SBVH (sbvh.c,
  * trace.c), the Substitute material from bsdf.c.
  *
  * Besides the reference's estimator (ORACLE_PT_MIS) it has two more that
@@ -118,10 +119,8 @@ typedef struct {
     uint32_t const*         tri_material;
     oracle_material const*  materials;
     oracle_bvh              bvh;
-    uint32_t*               lights;
-    float*                  light_cdf;
-    uint32_t                num_lights;
-    float                   light_area;
+    oracle_light_tree const* tree;
+    uint32_t*               tri_light;
     oracle_camera const*    cam;
     oracle_pt_settings const* set;
     v3                      fwd, right, up;
@@ -240,20 +239,6 @@ static float predivided_power_heuristic(float f, float g) {
     return f / (f * f + g * g);
 }
 
-static uint32_t pick_light(scene_t const* s, float u) {
-    uint32_t lo = 0;
-    uint32_t hi = s->num_lights - 1;
-    while (lo < hi) {
-        uint32_t const mid = (lo + hi) >> 1;
-        if (u < s->light_cdf[mid]) {
-            hi = mid;
-        } else {
-            lo = mid + 1;
-        }
-    }
-    return s->lights[lo];
-}
-
 /* One camera sample; adds its radiance to rgb. */
 static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, double rgb[3]) {
     oracle_pt_settings const* set = s->set;
@@ -277,6 +262,7 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
     float    radiance[3]   = {0.f, 0.f, 0.f};
     float    bsdf_pdf      = 0.f;
     int      singular      = 1;
+    float    prev_n[3]     = {0.f, 0.f, 0.f};
 
     for (uint32_t depth = 0;; ++depth) {
         uint32_t hit[4];
@@ -304,11 +290,15 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
         if (KIND_LIGHT == m->kind) {
             float const cos_l = dot(geo_n, wo);
             if (cos_l > 0.f) {
-                float weight = 1.f;
-                if (0 != depth && !singular) {
+                float          weight = 1.f;
+                uint32_t const l      = s->tri_light[tri];
+                if (0 != depth && !singular && SENTINEL != l) {
                     if (ORACLE_PT_MIS == mode) {
-                        float const ls_pdf = (t * t) / (cos_l * s->light_area);
-                        weight             = power_heuristic(bsdf_pdf, ls_pdf);
+                        int const   split    = set->split_lights && depth < 3;
+                        float const origin[3] = {o.x, o.y, o.z};
+                        float const tree_pdf = oracle_light_pdf(s->tree, origin, prev_n, split, l);
+                        float const ls_pdf   = (t * t) / (cos_l * s->tree->lights[l].area);
+                        weight               = power_heuristic(bsdf_pdf, ls_pdf * tree_pdf);
                     } else if (ORACLE_PT_NEE == mode) {
                         weight = 0.f;
                     }
@@ -343,12 +333,22 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
         surface_set(&sf, m, geo_n, wo);
 
         v3 const p_o = offset_ray(hit_p, geo_n);
-        if (s->num_lights) {
-            float const    u0 = pcg_float(&rng);
-            float const    u1 = pcg_float(&rng);
-            float const    u2 = pcg_float(&rng);
-            uint32_t const lt = pick_light(s, u0);
-            if (ORACLE_PT_BSDF != mode) {
+        if (s->tree->num_nodes) {
+            int const      split     = set->split_lights && depth < 3;
+            float const    select    = pcg_float(&rng);
+            float const    po[3]     = {p_o.x, p_o.y, p_o.z};
+            float const    gn[3]     = {geo_n.x, geo_n.y, geo_n.z};
+            uint32_t       picks[ORACLE_MAX_LIGHTS];
+            float          ppdf[ORACLE_MAX_LIGHTS];
+            uint32_t const num_picks = oracle_light_random(s->tree, po, gn, select, split, picks, ppdf);
+            for (uint32_t k = 0; k < num_picks; ++k) {
+                oracle_light const* L  = &s->tree->lights[picks[k]];
+                uint32_t const      lt = L->triangle;
+                float const         u1 = pcg_float(&rng);
+                float const         u2 = pcg_float(&rng);
+                if (ORACLE_PT_BSDF == mode) {
+                    continue;
+                }
                 v3 la, lb, lc;
                 tri_vertices(s, lt, &la, &lb, &lc);
                 float const su = sqrtf(u1);
@@ -366,14 +366,14 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
                 if (cl >= DOT_MIN) {
                     float f[4];
                     surface_evaluate(&sf, wi, f);
-                    float const light_pdf = sl / (cl * s->light_area);
+                    float const light_pdf = sl / (cl * L->area) * ppdf[k];
                     float const weight    = ORACLE_PT_MIS == mode
                                                 ? predivided_power_heuristic(light_pdf, f[3])
                                                 : 1.f / light_pdf;
                     oracle_material const* lm = &s->materials[s->tri_material[lt]];
                     float                  contrib[3];
-                    for (int k = 0; k < 3; ++k) {
-                        contrib[k] = throughput[k] * (weight * (lm->emission[k] * f[k]));
+                    for (int c = 0; c < 3; ++c) {
+                        contrib[c] = throughput[c] * (weight * (lm->emission[c] * f[c]));
                     }
                     if (contrib[0] > 0.f || contrib[1] > 0.f || contrib[2] > 0.f) {
                         uint32_t occ = 0;
@@ -381,8 +381,8 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
                             return ORACLE_ERR_STACK;
                         }
                         if (!occ) {
-                            for (int k = 0; k < 3; ++k) {
-                                radiance[k] += contrib[k];
+                            for (int c = 0; c < 3; ++c) {
+                                radiance[c] += contrib[c];
                             }
                         }
                     }
@@ -408,6 +408,9 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
         v3 const wi = mk(bs[0], bs[1], bs[2]);
         o           = offset_ray(hit_p, dot(geo_n, wi) > 0.f ? geo_n : mk(-geo_n.x, -geo_n.y, -geo_n.z));
         d           = wi;
+        prev_n[0]   = geo_n.x;
+        prev_n[1]   = geo_n.y;
+        prev_n[2]   = geo_n.z;
     }
 
     for (int k = 0; k < 3; ++k) {
@@ -446,8 +449,9 @@ static void* render_rows(void* arg) {
 int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t const* indices,
                       uint32_t num_triangles, uint32_t const* tri_material,
                       oracle_material const* materials, uint32_t num_materials,
-                      oracle_camera const* cam, oracle_pt_settings const* set, double* accum) {
-    if (!positions || !indices || !tri_material || !materials || !cam || !set || !accum ||
+                      oracle_light_tree const* tree, oracle_camera const* cam,
+                      oracle_pt_settings const* set, double* accum) {
+    if (!positions || !indices || !tri_material || !materials || !tree || !cam || !set || !accum ||
         0 == cam->width || 0 == cam->height || 0 == num_triangles) {
         return ORACLE_ERR_ARGS;
     }
@@ -473,37 +477,17 @@ int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t co
         return st;
     }
 
-    /* emissive triangles and their area CDF (areas in double) */
-    s.lights    = (uint32_t*)malloc(sizeof(uint32_t) * num_triangles);
-    s.light_cdf = (float*)malloc(sizeof(float) * num_triangles);
-    double* areas = (double*)malloc(sizeof(double) * num_triangles);
-    double  total = 0.0;
+    /* the light of each emissive triangle */
+    s.tree      = tree;
+    s.tri_light = (uint32_t*)malloc(sizeof(uint32_t) * num_triangles);
     for (uint32_t t = 0; t < num_triangles; ++t) {
-        if (KIND_LIGHT != materials[tri_material[t]].kind) {
-            continue;
-        }
-        float const* a = positions + 3 * (size_t)indices[3 * (size_t)t + 0];
-        float const* b = positions + 3 * (size_t)indices[3 * (size_t)t + 1];
-        float const* c = positions + 3 * (size_t)indices[3 * (size_t)t + 2];
-        double const e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
-        double const e2[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
-        double const x = e1[1] * e2[2] - e1[2] * e2[1];
-        double const y = e1[2] * e2[0] - e1[0] * e2[2];
-        double const z = e1[0] * e2[1] - e1[1] * e2[0];
-        double const area = 0.5 * sqrt(x * x + y * y + z * z);
-        if (area > 0.0) {
-            s.lights[s.num_lights] = t;
-            areas[s.num_lights++]  = area;
-            total += area;
+        s.tri_light[t] = SENTINEL;
+    }
+    for (uint32_t l = 0; l < tree->num_lights; ++l) {
+        if (tree->lights[l].triangle < num_triangles) {
+            s.tri_light[tree->lights[l].triangle] = l;
         }
     }
-    double run = 0.0;
-    for (uint32_t i = 0; i < s.num_lights; ++i) {
-        run += areas[i];
-        s.light_cdf[i] = i + 1 == s.num_lights ? 1.f : (float)(run / total);
-    }
-    free(areas);
-    s.light_area = (float)total;
 
     /* camera frame in double, rounded to float: right = fwd x up, up' = right x fwd */
     double f[3] = {(double)cam->target[0] - cam->eye[0], (double)cam->target[1] - cam->eye[1],
@@ -541,8 +525,7 @@ int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t co
     }
     pthread_mutex_destroy(&s.lock);
 
-    free(s.lights);
-    free(s.light_cdf);
+    free(s.tri_light);
     oracle_bvh_free(&s.bvh);
     return s.status;
 }

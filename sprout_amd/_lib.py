@@ -95,6 +95,7 @@ MI_MATERIAL_SUBSTITUTE = 0
 MI_MATERIAL_LAMBERT = 1
 MI_MATERIAL_LIGHT = 2
 MI_RENDER_NO_GRAPH = 1
+MI_RENDER_NO_LIGHT_SPLIT = 2
 
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int32
@@ -130,6 +131,10 @@ SIGNATURES = {
     "mi_render_guided": (_i32, [_u32, ctypes.POINTER(MiCamera), ctypes.POINTER(MiRenderSettings),
                                 ctypes.POINTER(MiGuideSettings), _vp, _vp,
                                 ctypes.POINTER(MiRenderStats)]),
+    "mi_light_tree_build": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _pu32]),
+    "mi_light_tree_sample": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp]),
+    "mi_light_tree_pdf": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp]),
+    "mi_scene_light_tree": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _pu32, _pu32]),
     "mi_ggx_e_table": (_i32, [_vp, _vp]),
     "mi_substitute_evaluate": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mi_substitute_sample": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -3,6 +3,7 @@
 #include "bvh_build.hpp"
 #include "common.hpp"
 #include "guide.hpp"
+#include "light_tree.hpp"
 #include "render.hpp"
 #include "trace.hpp"
 
@@ -613,6 +614,66 @@ int32_t mi_guide_train_step(uint32_t guide, float const* x, float const* dirs,
         return fail(g ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_guide_train_step: bad arguments");
     }
     return guide_adam_step(*g, x, dirs, weights, n, learning_rate, as_stream(stream));
+}
+
+int32_t mi_light_tree_build(float const* box_min, float const* box_max, float const* cones,
+                            float const* powers, uint8_t const* two_sided, uint32_t num_lights,
+                            mi_light_node* nodes, uint32_t* middles, uint32_t* mapping,
+                            uint32_t* orders, uint32_t* num_nodes) {
+    clear_error();
+    if (!num_nodes || (num_lights && (!box_min || !box_max || !cones || !powers || !two_sided))) {
+        return fail(MI_ERR_ARGS, "mi_light_tree_build: bad arguments");
+    }
+    HostLightTree t;
+    build_light_tree(box_min, box_max, cones, powers, two_sided, num_lights, t);
+    *num_nodes = uint32_t(t.nodes.size());
+    if (nodes) {
+        if (!middles || !mapping || !orders) {
+            return fail(MI_ERR_ARGS, "mi_light_tree_build: bad arguments");
+        }
+        std::memcpy(nodes, t.nodes.data(), t.nodes.size() * sizeof(mi_light_node));
+        std::memcpy(middles, t.middles.data(), t.middles.size() * sizeof(uint32_t));
+        std::memcpy(mapping, t.light_mapping.data(), t.light_mapping.size() * sizeof(uint32_t));
+        std::memcpy(orders, t.light_orders.data(), t.light_orders.size() * sizeof(uint32_t));
+    }
+    return MI_OK;
+}
+
+int32_t mi_light_tree_sample(uint32_t scene, float const* points, float const* normals,
+                             float const* randoms, uint32_t split, uint32_t n, uint32_t* picks,
+                             uint32_t* counts, void* stream) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc || (n && (!points || !normals || !randoms || !picks || !counts))) {
+        return fail(sc ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_light_tree_sample: bad arguments");
+    }
+    return light_tree_queries(*sc, true, points, normals, randoms, split, n, picks, counts,
+                              as_stream(stream));
+}
+
+int32_t mi_light_tree_pdf(uint32_t scene, float const* points, float const* normals,
+                          uint32_t const* lights, uint32_t split, uint32_t n, float* pdfs,
+                          void* stream) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc || (n && (!points || !normals || !lights || !pdfs))) {
+        return fail(sc ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_light_tree_pdf: bad arguments");
+    }
+    return light_tree_queries(*sc, false, points, normals, lights, split, n, pdfs, nullptr,
+                              as_stream(stream));
+}
+
+int32_t mi_scene_light_tree(uint32_t scene, mi_light_node* nodes, uint32_t* middles,
+                            uint32_t* mapping, uint32_t* orders, void* infos,
+                            uint32_t* num_nodes, uint32_t* num_lights) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc || !num_nodes || !num_lights) {
+        return fail(sc ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_scene_light_tree: bad arguments");
+    }
+    *num_nodes  = sc->num_lights ? sc->num_light_nodes : 0;
+    *num_lights = sc->num_lights;
+    return download_light_tree(*sc, nodes, middles, mapping, orders, infos);
 }
 
 }  // extern "C"

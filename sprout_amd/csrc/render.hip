@@ -42,6 +42,7 @@
 #include "common.hpp"
 #include "device_math.hpp"
 #include "guide.hpp"
+#include "light_tree.hpp"
 #include "render.hpp"
 #include "trace.hpp"
 
@@ -99,14 +100,17 @@ struct Rng {
 
 constexpr uint32_t Flag_singular = 1;  // State::Treat_as_singular
 
+// Pathtracer_MIS::Num_dedicated_samplers: light-tree splitting below this depth
+constexpr uint32_t Num_split_bounces = 3;
+
 struct Path {
     float    throughput[3];
     float    radiance[3];
     float    bsdf_pdf;  // pdf of the direction sample that spawned the current ray
     uint32_t flags;
     Rng      rng;
-    uint32_t record;  // training sample of the previous vertex, or Null
-    uint32_t pad;
+    uint32_t record;     // training sample of the previous vertex, or Null
+    float    prev_n[3];  // geometric normal of the previous vertex
 };
 
 struct Shadow {
@@ -120,10 +124,9 @@ struct Wave {
     uint32_t const*        indices;
     uint32_t const*        tri_material;
     Render_material const* materials;
-    uint32_t const*        light_tris;
-    float const*           light_cdf;
-    uint32_t               num_lights;
-    float                  light_area;
+    dev::Light_tree_view   tree;
+    uint32_t const*        tri_light;  // light of an emissive triangle
+    uint32_t               split_lights;
     float const*           e_table;
     // camera
     float    eye[3];
@@ -148,6 +151,7 @@ struct Wave {
     mi_ray*   shadow_rays;
     Shadow*   shadows;
     uint32_t* occluded;
+    uint2*    shadow_range;  // first shadow entry and count of each queue entry's vertex
     uint32_t* counters;
     // guiding
     float        fraction;      // probability of sampling the guide
@@ -217,6 +221,24 @@ __device__ __forceinline__ uint32_t queue_slot(bool want, uint32_t* counter) {
     }
     base = __shfl(base, leader);
     return base + uint32_t(__popcll(mask & ((uint64_t(1) << lane) - 1)));
+}
+
+// Reserves n consecutive queue entries per lane: one atomic per wave, the
+// lanes' ranges in lane order.  Every lane of the wave must call it.
+__device__ __forceinline__ uint32_t queue_range(uint32_t n, uint32_t* counter) {
+    uint32_t const lane = threadIdx.x & 63;
+    uint32_t       inc  = n;  // inclusive prefix sum over the lanes
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint32_t const up = __shfl_up(inc, d, 64);
+        inc += lane >= d ? up : 0u;
+    }
+    uint32_t const total = __shfl(inc, 63);
+    uint32_t       base  = 0;
+    if (63 == lane && total) {
+        base = atomicAdd(counter, total);
+    }
+    base = __shfl(base, 63);
+    return base + inc - n;
 }
 
 // The material of a hit, Lambert or Substitute (both one-sided).
@@ -295,7 +317,7 @@ __global__ void __launch_bounds__(256) raygen(Wave w) {
     path.bsdf_pdf = 0.f;
     path.flags    = Flag_singular;
     path.record   = dev::Null;
-    path.pad      = 0;
+    path.prev_n[0] = path.prev_n[1] = path.prev_n[2] = 0.f;
     w.paths[p]    = path;
 
     mi_ray r;
@@ -328,21 +350,6 @@ __global__ void tally(uint32_t* counters, uint32_t in_q) {
     uint64_t* totals = reinterpret_cast<uint64_t*>(counters + C_totals);
     totals[0] += counters[C_queue0 + in_q];
     totals[1] += counters[C_shadow];
-}
-
-// Emissive triangle by area (first CDF entry above u).
-__device__ __forceinline__ uint32_t pick_light(Wave const& w, float u) {
-    uint32_t lo = 0;
-    uint32_t hi = w.num_lights - 1;
-    while (lo < hi) {
-        uint32_t const mid = (lo + hi) >> 1;
-        if (u < w.light_cdf[mid]) {
-            hi = mid;
-        } else {
-            lo = mid + 1;
-        }
-    }
-    return w.light_tris[lo];
 }
 
 // Network inputs of every queued hit: position in the scene box, geometric
@@ -385,31 +392,41 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
     uint32_t const count  = w.counters[C_queue0 + in_q];
     uint32_t const out_q  = in_q ^ 1;
     uint32_t const stride = gridDim.x * blockDim.x;
+    // sample_lights splits the light tree at the first bounces
+    // (Pathtracer_MIS::splitting, pathtracer_mis.cpp:423-426)
+    bool const split = w.split_lights && depth < Num_split_bounces;
 
     // block-uniform trip count: every lane reaches the wave-wide queue appends
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         uint32_t const i = base + threadIdx.x;
 
         bool     spawn  = false;
-        bool     shadow = false;
         bool     record = false;
+        bool     live   = false;  // a shading vertex that continues the path
         uint32_t p      = 0;
+        Path     path;
         mi_ray   next_ray;
-        mi_ray   shadow_ray;
-        Shadow   sh;
         float    rec_t[3] = {0.f, 0.f, 0.f};
+        V3       hit_p    = v3(0.f, 0.f, 0.f);
+        V3       geo_n    = v3(0.f, 0.f, 1.f);
+        V3       p_o      = v3(0.f, 0.f, 0.f);
+        Surface  s;
+        s.kind = MI_MATERIAL_LAMBERT;
+
+        dev::Light_pick picks[dev::Max_lights];
+        uint32_t        num_picks = 0;
 
         if (i < count) {
             p              = w.ray_path[in_q][i];
-            Path     path  = w.paths[p];
+            path           = w.paths[p];
             mi_hit const h = w.hits[i];
 
             if (Guided && dev::Null != path.record) {
                 // radiance before this vertex: what follows is the previous sample's
-                float* rr = w.rec_r + 3 * size_t(path.record);
-                rr[0]     = path.radiance[0];
-                rr[1]     = path.radiance[1];
-                rr[2]     = path.radiance[2];
+                float* rr   = w.rec_r + 3 * size_t(path.record);
+                rr[0]       = path.radiance[0];
+                rr[1]       = path.radiance[1];
+                rr[2]       = path.radiance[2];
                 path.record = dev::Null;
             }
 
@@ -417,26 +434,33 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                 mi_ray const   r   = w.rays[in_q][i];
                 uint32_t const tri = h.triangle;
 
-                V3 const    a     = vertex(w, w.indices[3 * size_t(tri) + 0]);
-                V3 const    b     = vertex(w, w.indices[3 * size_t(tri) + 1]);
-                V3 const    c     = vertex(w, w.indices[3 * size_t(tri) + 2]);
-                float const bw    = (1.f - h.u) - h.v;
-                V3 const    hit_p = (bw * a + h.u * b) + h.v * c;
-                V3 const    geo_n = dev::normalize(dev::cross(b - a, c - a));
-                V3 const    wo    = v3(-r.direction[0], -r.direction[1], -r.direction[2]);
+                V3 const    a  = vertex(w, w.indices[3 * size_t(tri) + 0]);
+                V3 const    b  = vertex(w, w.indices[3 * size_t(tri) + 1]);
+                V3 const    c  = vertex(w, w.indices[3 * size_t(tri) + 2]);
+                float const bw = (1.f - h.u) - h.v;
+                hit_p          = (bw * a + h.u * b) + h.v * c;
+                geo_n          = dev::normalize(dev::cross(b - a, c - a));
+                V3 const wo    = v3(-r.direction[0], -r.direction[1], -r.direction[2]);
 
                 Render_material const m = w.materials[w.tri_material[tri]];
 
-                bool live = true;
+                live = true;
 
                 if (MI_MATERIAL_LIGHT == m.kind) {
                     // first vertex: mat_sample.radiance(); later: connect_light
                     float const cos_l = dev::dot(geo_n, wo);
                     if (cos_l > 0.f) {
                         float weight = 1.f;
-                        if (0 != depth && !(path.flags & Flag_singular)) {
-                            float const ls_pdf = (h.t * h.t) / (cos_l * w.light_area);
-                            weight             = power_heuristic(path.bsdf_pdf, ls_pdf);
+                        uint32_t const l = w.tri_light[tri];
+                        // an emitter light sampling cannot pick (zero power)
+                        // is found by BSDF sampling alone: weight 1
+                        if (0 != depth && !(path.flags & Flag_singular) && dev::Null != l) {
+                            // Mesh::pdf times Tree::pdf at the previous vertex
+                            float const tree_pdf = dev::light_pdf(
+                                w.tree, v3(r.origin[0], r.origin[1], r.origin[2]),
+                                v3(path.prev_n[0], path.prev_n[1], path.prev_n[2]), false, split, l);
+                            float const ls_pdf = (h.t * h.t) / (cos_l * w.tree.lights[l].area);
+                            weight             = power_heuristic(path.bsdf_pdf, ls_pdf * tree_pdf);
                         }
                         for (int k = 0; k < 3; ++k) {
                             path.radiance[k] += path.throughput[k] * (weight * m.emission[k]);
@@ -464,152 +488,183 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                 }
 
                 if (live) {
-                    Surface s;
                     s.kind = m.kind;
                     s.sub.set(dev::frame_from_normal(geo_n), geo_n, wo, m.color, m.roughness,
                               m.metallic, m.ior, 1.f, w.e_table);
-
-                    // next-event estimation (sample_lights / evaluate_light)
-                    V3 const p_o = offset_ray(hit_p, geo_n);
-                    if (w.num_lights) {
-                        float const    u0 = path.rng.uniform();
-                        float const    u1 = path.rng.uniform();
-                        float const    u2 = path.rng.uniform();
-                        uint32_t const lt = pick_light(w, u0);
-
-                        V3 const la = vertex(w, w.indices[3 * size_t(lt) + 0]);
-                        V3 const lb = vertex(w, w.indices[3 * size_t(lt) + 1]);
-                        V3 const lc = vertex(w, w.indices[3 * size_t(lt) + 2]);
-                        // sample_triangle_uniform, base/math/sampling.inl:39-43
-                        float const su = sqrtf(u1);
-                        float const lu = 1.f - su;
-                        float const lv = u2 * su;
-                        float const lw = (1.f - lu) - lv;
-                        V3 const    lp = (lw * la + lu * lb) + lv * lc;
-                        V3 const    ln = dev::normalize(dev::cross(lb - la, lc - la));
-
-                        V3 const    axis = offset_ray(lp, ln) - p_o;
-                        float const sl   = dev::dot(axis, axis);
-                        float const dist = sqrtf(sl);
-                        V3 const    wi   = v3(axis.x / dist, axis.y / dist, axis.z / dist);
-                        float const cl   = -dev::dot(ln, wi);
-                        if (cl >= dev::Dot_min) {
-                            float f[4];
-                            s.evaluate(wi, f);
-                            float const light_pdf = sl / (cl * w.light_area);
-                            float       dir_pdf   = f[3];
-                            if (Guided) {
-                                dev::Vmf_mixture mix;
-                                mix.set(w.guide_raw + 32 * size_t(i));
-                                dir_pdf = w.fraction * mix.pdf(wi) + (1.f - w.fraction) * f[3];
-                            }
-                            float const weight = predivided_power_heuristic(light_pdf, dir_pdf);
-                            Render_material const lm = w.materials[w.tri_material[lt]];
-                            for (int k = 0; k < 3; ++k) {
-                                sh.contrib[k] =
-                                    path.throughput[k] * (weight * (lm.emission[k] * f[k]));
-                            }
-                            sh.path                 = p;
-                            shadow_ray.origin[0]    = p_o.x;
-                            shadow_ray.origin[1]    = p_o.y;
-                            shadow_ray.origin[2]    = p_o.z;
-                            shadow_ray.min_t        = 0.f;
-                            shadow_ray.direction[0] = wi.x;
-                            shadow_ray.direction[1] = wi.y;
-                            shadow_ray.direction[2] = wi.z;
-                            shadow_ray.max_t        = dist;
-                            shadow = sh.contrib[0] > 0.f || sh.contrib[1] > 0.f ||
-                                     sh.contrib[2] > 0.f;
-                        }
+                    // next-event estimation: the lights the tree picks (sample_lights)
+                    p_o = offset_ray(hit_p, geo_n);
+                    if (w.tree.num_nodes) {
+                        float const select = path.rng.uniform();
+                        num_picks = dev::random_light(w.tree, p_o, geo_n, false, select, split, picks);
                     }
+                }
+            }
+        }
 
-                    // BSDF importance sample; guided: one-sample mixture of
-                    // the guide (probability fraction) and the BSDF
-                    dev::Bxdf_sample bs;
+        // one shadow ray per picked light (evaluate_light); a vertex's rays
+        // take consecutive queue slots, reserved with one atomic per wave
+        uint32_t const sbase = queue_range(num_picks, &w.counters[C_shadow]);
+        if (i < count) {
+            w.shadow_range[i] = make_uint2(sbase, num_picks);
+        }
+        for (uint32_t k = 0; k < num_picks; ++k) {
+            bool   want = false;
+            mi_ray shadow_ray;
+            Shadow sh;
+            {
+                dev::Light_pick const pick = picks[k];
+                Light_info const      L    = w.tree.lights[pick.light];
+                uint32_t const        lt   = L.triangle;
+                float const           u1   = path.rng.uniform();
+                float const           u2   = path.rng.uniform();
+
+                V3 const la = vertex(w, w.indices[3 * size_t(lt) + 0]);
+                V3 const lb = vertex(w, w.indices[3 * size_t(lt) + 1]);
+                V3 const lc = vertex(w, w.indices[3 * size_t(lt) + 2]);
+                // sample_triangle_uniform, base/math/sampling.inl:39-43
+                float const su = sqrtf(u1);
+                float const lu = 1.f - su;
+                float const lv = u2 * su;
+                float const lw = (1.f - lu) - lv;
+                V3 const    lp = (lw * la + lu * lb) + lv * lc;
+                V3 const    ln = dev::normalize(dev::cross(lb - la, lc - la));
+
+                V3 const    axis = offset_ray(lp, ln) - p_o;
+                float const sl   = dev::dot(axis, axis);
+                float const dist = sqrtf(sl);
+                V3 const    wi   = v3(axis.x / dist, axis.y / dist, axis.z / dist);
+                float const cl   = -dev::dot(ln, wi);
+                if (cl >= dev::Dot_min) {
+                    float f[4];
+                    s.evaluate(wi, f);
+                    float const light_pdf = sl / (cl * L.area) * pick.pdf;
+                    float       dir_pdf   = f[3];
                     if (Guided) {
                         dev::Vmf_mixture mix;
                         mix.set(w.guide_raw + 32 * size_t(i));
-                        float const pick = path.rng.uniform();
-                        float const xl   = path.rng.uniform();
-                        float const x0   = path.rng.uniform();
-                        float const x1   = path.rng.uniform();
-                        if (pick < w.fraction) {
-                            V3 const gw = mix.sample(xl, x0, x1);
-                            float    f[4];
-                            s.evaluate(gw, f);
-                            bs.wi      = gw;
-                            bs.refl[0] = f[0];
-                            bs.refl[1] = f[1];
-                            bs.refl[2] = f[2];
-                            bs.pdf     = f[3];
-                            bs.type    = dev::Type_diffuse;
-                            if (!(dev::dot(geo_n, gw) > 0.f)) {
-                                bs.pdf = 0.f;  // under the surface: the reflection is zero
-                            }
-                        } else {
-                            s.sample(xl, x0, x1, bs);
-                        }
-                        if (bs.pdf > 0.f) {
-                            bs.pdf = w.fraction * mix.pdf(bs.wi) + (1.f - w.fraction) * bs.pdf;
-                        }
-                        if (bs.pdf > 0.f && depth < w.record_depth) {
-                            record   = true;
-                            rec_t[0] = path.throughput[0];
-                            rec_t[1] = path.throughput[1];
-                            rec_t[2] = path.throughput[2];
-                        }
-                    } else {
-                        float const xl = path.rng.uniform();
-                        float const x0 = path.rng.uniform();
-                        float const x1 = path.rng.uniform();
-                        s.sample(xl, x0, x1, bs);
+                        dir_pdf = w.fraction * mix.pdf(wi) + (1.f - w.fraction) * f[3];
                     }
-                    if (!(bs.pdf > 0.f)) {
-                        live = false;
-                    } else {
-                        if (dev::Type_specular == bs.type) {
-                            path.flags |= Flag_singular;
-                        } else {
-                            path.flags &= ~Flag_singular;
-                        }
-                        path.bsdf_pdf = bs.pdf;
-                        for (int k = 0; k < 3; ++k) {
-                            path.throughput[k] *= bs.refl[k] / bs.pdf;
-                        }
-                        V3 const o =
-                            offset_ray(hit_p, dev::dot(geo_n, bs.wi) > 0.f ? geo_n : -geo_n);
-                        next_ray.origin[0]    = o.x;
-                        next_ray.origin[1]    = o.y;
-                        next_ray.origin[2]    = o.z;
-                        next_ray.min_t        = 0.f;
-                        next_ray.direction[0] = bs.wi.x;
-                        next_ray.direction[1] = bs.wi.y;
-                        next_ray.direction[2] = bs.wi.z;
-                        next_ray.max_t        = Ray_max_t;
-                        spawn                 = true;
+                    float const weight = predivided_power_heuristic(light_pdf, dir_pdf);
+                    Render_material const lm = w.materials[w.tri_material[lt]];
+                    for (int c = 0; c < 3; ++c) {
+                        sh.contrib[c] = path.throughput[c] * (weight * (lm.emission[c] * f[c]));
                     }
+                    sh.path                 = p;
+                    shadow_ray.origin[0]    = p_o.x;
+                    shadow_ray.origin[1]    = p_o.y;
+                    shadow_ray.origin[2]    = p_o.z;
+                    shadow_ray.min_t        = 0.f;
+                    shadow_ray.direction[0] = wi.x;
+                    shadow_ray.direction[1] = wi.y;
+                    shadow_ray.direction[2] = wi.z;
+                    shadow_ray.max_t        = dist;
+                    want = sh.contrib[0] > 0.f || sh.contrib[1] > 0.f || sh.contrib[2] > 0.f;
                 }
             }
+            if (!want) {
+                // nothing to add: an empty interval, resolved as not occluded
+                shadow_ray.origin[0] = shadow_ray.origin[1] = shadow_ray.origin[2] = 0.f;
+                shadow_ray.direction[0] = 0.f;
+                shadow_ray.direction[1] = 0.f;
+                shadow_ray.direction[2] = 1.f;
+                shadow_ray.min_t = 1.f;
+                shadow_ray.max_t = 0.f;
+                sh.contrib[0] = sh.contrib[1] = sh.contrib[2] = 0.f;
+                sh.path = p;
+            }
+            w.shadow_rays[sbase + k] = shadow_ray;
+            w.shadows[sbase + k]     = sh;
+        }
+
+        if (live) {
+            // BSDF importance sample; guided: one-sample mixture of the guide
+            // (probability fraction) and the BSDF
+            dev::Bxdf_sample bs;
             if (Guided) {
-                record = record && spawn;
-                uint32_t const slot = queue_slot(record, &w.counters[C_records]);
-                if (record && slot < w.record_capacity) {
-                    float const* xi = w.guide_x + 9 * size_t(i);
-                    float*       xo = w.train_x + 9 * size_t(slot);
-                    for (int k = 0; k < 9; ++k) {
-                        xo[k] = xi[k];
+                dev::Vmf_mixture mix;
+                mix.set(w.guide_raw + 32 * size_t(i));
+                float const pick = path.rng.uniform();
+                float const xl   = path.rng.uniform();
+                float const x0   = path.rng.uniform();
+                float const x1   = path.rng.uniform();
+                if (pick < w.fraction) {
+                    V3 const gw = mix.sample(xl, x0, x1);
+                    float    f[4];
+                    s.evaluate(gw, f);
+                    bs.wi      = gw;
+                    bs.refl[0] = f[0];
+                    bs.refl[1] = f[1];
+                    bs.refl[2] = f[2];
+                    bs.pdf     = f[3];
+                    bs.type    = dev::Type_diffuse;
+                    if (!(dev::dot(geo_n, gw) > 0.f)) {
+                        bs.pdf = 0.f;  // under the surface: the reflection is zero
                     }
-                    w.train_dir[3 * size_t(slot) + 0] = next_ray.direction[0];
-                    w.train_dir[3 * size_t(slot) + 1] = next_ray.direction[1];
-                    w.train_dir[3 * size_t(slot) + 2] = next_ray.direction[2];
-                    for (int k = 0; k < 3; ++k) {
-                        w.rec_t[3 * size_t(slot) + k] = rec_t[k];
-                        w.rec_r[3 * size_t(slot) + k] = path.radiance[k];
-                    }
-                    w.rec_path[slot] = p;
-                    path.record      = slot;
+                } else {
+                    s.sample(xl, x0, x1, bs);
                 }
+                if (bs.pdf > 0.f) {
+                    bs.pdf = w.fraction * mix.pdf(bs.wi) + (1.f - w.fraction) * bs.pdf;
+                }
+                if (bs.pdf > 0.f && depth < w.record_depth) {
+                    record   = true;
+                    rec_t[0] = path.throughput[0];
+                    rec_t[1] = path.throughput[1];
+                    rec_t[2] = path.throughput[2];
+                }
+            } else {
+                float const xl = path.rng.uniform();
+                float const x0 = path.rng.uniform();
+                float const x1 = path.rng.uniform();
+                s.sample(xl, x0, x1, bs);
             }
+            if (bs.pdf > 0.f) {
+                if (dev::Type_specular == bs.type) {
+                    path.flags |= Flag_singular;
+                } else {
+                    path.flags &= ~Flag_singular;
+                }
+                path.bsdf_pdf = bs.pdf;
+                for (int k = 0; k < 3; ++k) {
+                    path.throughput[k] *= bs.refl[k] / bs.pdf;
+                }
+                path.prev_n[0]        = geo_n.x;
+                path.prev_n[1]        = geo_n.y;
+                path.prev_n[2]        = geo_n.z;
+                V3 const o            = offset_ray(hit_p, dev::dot(geo_n, bs.wi) > 0.f ? geo_n : -geo_n);
+                next_ray.origin[0]    = o.x;
+                next_ray.origin[1]    = o.y;
+                next_ray.origin[2]    = o.z;
+                next_ray.min_t        = 0.f;
+                next_ray.direction[0] = bs.wi.x;
+                next_ray.direction[1] = bs.wi.y;
+                next_ray.direction[2] = bs.wi.z;
+                next_ray.max_t        = Ray_max_t;
+                spawn                 = true;
+            }
+        }
+
+        if (Guided) {
+            record              = record && spawn;
+            uint32_t const slot = queue_slot(record, &w.counters[C_records]);
+            if (record && slot < w.record_capacity) {
+                float const* xi = w.guide_x + 9 * size_t(i);
+                float*       xo = w.train_x + 9 * size_t(slot);
+                for (int k = 0; k < 9; ++k) {
+                    xo[k] = xi[k];
+                }
+                w.train_dir[3 * size_t(slot) + 0] = next_ray.direction[0];
+                w.train_dir[3 * size_t(slot) + 1] = next_ray.direction[1];
+                w.train_dir[3 * size_t(slot) + 2] = next_ray.direction[2];
+                for (int k = 0; k < 3; ++k) {
+                    w.rec_t[3 * size_t(slot) + k] = rec_t[k];
+                    w.rec_r[3 * size_t(slot) + k] = path.radiance[k];
+                }
+                w.rec_path[slot] = p;
+                path.record      = slot;
+            }
+        }
+        if (i < count) {
             w.paths[p] = path;
         }
 
@@ -618,25 +673,28 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             w.rays[out_q][ray_slot]     = next_ray;
             w.ray_path[out_q][ray_slot] = p;
         }
-        uint32_t const sh_slot = queue_slot(shadow, &w.counters[C_shadow]);
-        if (shadow) {
-            w.shadow_rays[sh_slot] = shadow_ray;
-            w.shadows[sh_slot]     = sh;
-        }
     }
 }
 
-__global__ void __launch_bounds__(256) shadow_resolve(Wave w) {
-    uint32_t const count  = w.counters[C_shadow];
+// Adds the unoccluded light contributions of every vertex shaded this
+// bounce to its path, in the order the lights were picked.
+__global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q) {
+    uint32_t const count  = w.counters[C_queue0 + in_q];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        if (w.occluded[i]) {
+        uint2 const range = w.shadow_range[i];
+        if (0 == range.y) {
             continue;
         }
-        Shadow const s = w.shadows[i];
-        Path&        p = w.paths[s.path];
-        for (int k = 0; k < 3; ++k) {
-            p.radiance[k] += s.contrib[k];
+        Path& p = w.paths[w.ray_path[in_q][i]];
+        for (uint32_t k = 0; k < range.y; ++k) {
+            if (w.occluded[range.x + k]) {
+                continue;
+            }
+            Shadow const s = w.shadows[range.x + k];
+            for (int c = 0; c < 3; ++c) {
+                p.radiance[c] += s.contrib[c];
+            }
         }
     }
 }
@@ -805,6 +863,37 @@ __global__ void bsdf_sample_batch(uint32_t n, float const* params, float const* 
     o[7] = __uint_as_float(s.type);
 }
 
+__global__ void light_tree_sample_batch(dev::Light_tree_view t, float const* points,
+                                        float const* normals, float const* randoms, uint32_t split,
+                                        uint32_t n, uint32_t* picks, uint32_t* counts) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    dev::Light_pick out[dev::Max_lights];
+    V3 const        p = v3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
+    V3 const        nn = v3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+    uint32_t const  c = dev::random_light(t, p, nn, false, randoms[i], 0 != split, out);
+    counts[i]         = c;
+    uint32_t* o       = picks + 2 * size_t(i) * dev::Max_lights;
+    for (uint32_t k = 0; k < c; ++k) {
+        o[2 * k]     = out[k].light;
+        o[2 * k + 1] = __float_as_uint(out[k].pdf);
+    }
+}
+
+__global__ void light_tree_pdf_batch(dev::Light_tree_view t, float const* points,
+                                     float const* normals, uint32_t const* lights, uint32_t split,
+                                     uint32_t n, float* pdfs) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    V3 const p  = v3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
+    V3 const nn = v3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+    pdfs[i]     = dev::light_pdf(t, p, nn, false, 0 != split, lights[i]);
+}
+
 uint32_t blocks(uint64_t n, uint32_t b) {
     uint64_t const g = (n + b - 1) / b;
     return uint32_t(g > 0 ? g : 1);
@@ -849,14 +938,71 @@ int32_t bsdf_batch(bool sample, uint32_t n, float const* params, float const* fr
     return MI_OK;
 }
 
+namespace {
+
+dev::Light_tree_view tree_view(RenderScene const& scene) {
+    dev::Light_tree_view t;
+    t.nodes     = scene.light_nodes;
+    t.middles   = scene.light_middles;
+    t.mapping   = scene.light_mapping;
+    t.orders    = scene.light_orders;
+    t.lights    = scene.light_infos;
+    t.num_nodes = scene.num_lights ? scene.num_light_nodes : 0;
+    return t;
+}
+
+}  // namespace
+
+int32_t light_tree_queries(RenderScene const& scene, bool sample, float const* points,
+                           float const* normals, void const* arg, uint32_t split, uint32_t n,
+                           void* out0, uint32_t* out1, hipStream_t stream) {
+    if (0 == n) {
+        return MI_OK;
+    }
+    dev::Light_tree_view const t = tree_view(scene);
+    if (sample) {
+        hipLaunchKernelGGL(light_tree_sample_batch, dim3(blocks(n, 128)), dim3(128), 0, stream, t,
+                           points, normals, static_cast<float const*>(arg), split, n,
+                           static_cast<uint32_t*>(out0), out1);
+    } else {
+        hipLaunchKernelGGL(light_tree_pdf_batch, dim3(blocks(n, 128)), dim3(128), 0, stream, t,
+                           points, normals, static_cast<uint32_t const*>(arg), split, n,
+                           static_cast<float*>(out0));
+    }
+    if (hipError_t const e = hipGetLastError(); e) {
+        return fail(MI_ERR_HIP, "light tree query: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+int32_t download_light_tree(RenderScene const& scene, mi_light_node* nodes, uint32_t* middles,
+                            uint32_t* mapping, uint32_t* orders, void* infos) {
+    size_t const nn = scene.num_lights ? scene.num_light_nodes : 0;
+    size_t const nl = scene.num_lights;
+    hipError_t   e  = hipSuccess;
+    if (nodes && nn) e = hipMemcpy(nodes, scene.light_nodes, nn * sizeof(Light_node), hipMemcpyDeviceToHost);
+    if (!e && middles && nn) e = hipMemcpy(middles, scene.light_middles, nn * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (!e && mapping && nl) e = hipMemcpy(mapping, scene.light_mapping, nl * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (!e && orders && nl) e = hipMemcpy(orders, scene.light_orders, nl * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (!e && infos && nl) e = hipMemcpy(infos, scene.light_infos, nl * sizeof(Light_info), hipMemcpyDeviceToHost);
+    if (e) {
+        return fail(MI_ERR_HIP, "light tree download: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
 void RenderScene::release() {
     bvh.release();
     (void)hipFree(positions);
     (void)hipFree(indices);
     (void)hipFree(tri_material);
     (void)hipFree(materials);
-    (void)hipFree(light_tris);
-    (void)hipFree(light_cdf);
+    (void)hipFree(light_nodes);
+    (void)hipFree(light_middles);
+    (void)hipFree(light_mapping);
+    (void)hipFree(light_orders);
+    (void)hipFree(light_infos);
+    (void)hipFree(tri_light);
     *this = RenderScene();
 }
 
@@ -894,44 +1040,69 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
         r.kind      = in.kind;
     }
 
-    // emissive triangles and their area CDF
-    std::vector<uint32_t> lights;
-    std::vector<double>   areas;
+    // emissive triangles: one light each (a single-triangle part), the
+    // box, cone and power Scene::prop_prepare_sampling gives it
+    // (scene.cpp:440-506, light.cpp:38-48)
+    std::vector<Light_info> infos;
+    std::vector<float>      bmin, bmax, cones, powers;
+    std::vector<uint8_t>    two_sided;
+    std::vector<uint32_t>   tri_light(num_triangles, dev::Null);
     for (uint32_t t = 0; t < num_triangles; ++t) {
-        if (!is_light(materials[tri_material[t]])) {
+        mi_material const& mat = materials[tri_material[t]];
+        if (!is_light(mat)) {
             continue;
         }
-        float const* a     = positions + 3 * size_t(indices[3 * size_t(t) + 0]);
-        float const* b     = positions + 3 * size_t(indices[3 * size_t(t) + 1]);
-        float const* c     = positions + 3 * size_t(indices[3 * size_t(t) + 2]);
-        double const e1[3] = {double(b[0]) - a[0], double(b[1]) - a[1], double(b[2]) - a[2]};
-        double const e2[3] = {double(c[0]) - a[0], double(c[1]) - a[1], double(c[2]) - a[2]};
-        double const x     = e1[1] * e2[2] - e1[2] * e2[1];
-        double const y     = e1[2] * e2[0] - e1[0] * e2[2];
-        double const z     = e1[0] * e2[1] - e1[1] * e2[0];
-        double const area  = 0.5 * std::sqrt(x * x + y * y + z * z);
-        if (area > 0.0) {
-            lights.push_back(t);
-            areas.push_back(area);
+        float const* a = positions + 3 * size_t(indices[3 * size_t(t) + 0]);
+        float const* b = positions + 3 * size_t(indices[3 * size_t(t) + 1]);
+        float const* c = positions + 3 * size_t(indices[3 * size_t(t) + 2]);
+        float const  e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        float const  e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        float        n[3]  = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                              e1[0] * e2[1] - e1[1] * e2[0]};
+        float const  len   = std::sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+        float const  area  = 0.5f * len;
+        float const  radiance = std::max(std::max(mat.emission[0], mat.emission[1]), mat.emission[2]);
+        if (!(area > 0.f) || !(radiance > 0.f)) {
+            continue;
         }
+        for (float& x : n) {
+            x /= len;
+        }
+        Light_info li;
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(std::min(a[k], b[k]), c[k]);
+            hi[k] = std::max(std::max(a[k], b[k]), c[k]);
+            bmin.push_back(lo[k]);
+            bmax.push_back(hi[k]);
+            li.center[k] = 0.5f * (lo[k] + hi[k]);
+            li.cone[k]   = n[k];
+        }
+        float const ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        li.radius   = 0.5f * std::sqrt((ext[0] * ext[0] + ext[1] * ext[1]) + ext[2] * ext[2]);
+        li.cone[3]  = 1.f;
+        li.power    = area * radiance;
+        li.two_sided = 0;
+        li.triangle = t;
+        li.area     = area;
+        for (int k = 0; k < 4; ++k) {
+            cones.push_back(li.cone[k]);
+        }
+        powers.push_back(li.power);
+        two_sided.push_back(0);
+        tri_light[t] = uint32_t(infos.size());
+        infos.push_back(li);
     }
-    double total = 0.0;
-    for (double a : areas) {
-        total += a;
-    }
-    std::vector<float> cdf(lights.size());
-    double             run = 0.0;
-    for (size_t i = 0; i < lights.size(); ++i) {
-        run += areas[i];
-        cdf[i] = i + 1 == lights.size() ? 1.f : float(run / total);
-    }
+    HostLightTree tree;
+    build_light_tree(bmin.data(), bmax.data(), cones.data(), powers.data(), two_sided.data(),
+                     uint32_t(infos.size()), tree);
 
     RenderScene s;
     s.num_vertices  = num_vertices;
     s.num_triangles = num_triangles;
     s.num_materials = num_materials;
-    s.num_lights    = uint32_t(lights.size());
-    s.light_area    = float(total);
+    s.num_lights    = uint32_t(infos.size());
+    s.num_light_nodes = uint32_t(tree.nodes.size());
     for (int k = 0; k < 3; ++k) {
         s.box_lo[k] = positions[k];
         s.box_hi[k] = positions[k];
@@ -943,19 +1114,30 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
         }
     }
 
-    size_t const nl = std::max<size_t>(1, lights.size());
+    size_t const nl = std::max<size_t>(1, infos.size());
+    size_t const nn = std::max<size_t>(1, tree.nodes.size());
     hipError_t   e  = hipMalloc(&s.positions, size_t(num_vertices) * 3 * sizeof(float));
     if (!e) e = hipMalloc(&s.indices, size_t(num_triangles) * 3 * sizeof(uint32_t));
     if (!e) e = hipMalloc(&s.tri_material, size_t(num_triangles) * sizeof(uint32_t));
     if (!e) e = hipMalloc(&s.materials, std::max<size_t>(1, num_materials) * sizeof(Render_material));
-    if (!e) e = hipMalloc(&s.light_tris, nl * sizeof(uint32_t));
-    if (!e) e = hipMalloc(&s.light_cdf, nl * sizeof(float));
+    if (!e) e = hipMalloc(&s.light_nodes, nn * sizeof(Light_node));
+    if (!e) e = hipMalloc(&s.light_middles, nn * sizeof(uint32_t));
+    if (!e) e = hipMalloc(&s.light_mapping, nl * sizeof(uint32_t));
+    if (!e) e = hipMalloc(&s.light_orders, nl * sizeof(uint32_t));
+    if (!e) e = hipMalloc(&s.light_infos, nl * sizeof(Light_info));
+    if (!e) e = hipMalloc(&s.tri_light, size_t(num_triangles) * sizeof(uint32_t));
     if (!e) e = hipMemcpy(s.positions, positions, size_t(num_vertices) * 3 * sizeof(float), hipMemcpyHostToDevice);
     if (!e) e = hipMemcpy(s.indices, indices, size_t(num_triangles) * 3 * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (!e) e = hipMemcpy(s.tri_material, tri_material, size_t(num_triangles) * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (!e) e = hipMemcpy(s.materials, mats.data(), num_materials * sizeof(Render_material), hipMemcpyHostToDevice);
-    if (!e && !lights.empty()) e = hipMemcpy(s.light_tris, lights.data(), lights.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (!e && !lights.empty()) e = hipMemcpy(s.light_cdf, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (!e && !infos.empty()) {
+        e = hipMemcpy(s.light_nodes, tree.nodes.data(), tree.nodes.size() * sizeof(Light_node), hipMemcpyHostToDevice);
+        if (!e) e = hipMemcpy(s.light_middles, tree.middles.data(), tree.middles.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (!e) e = hipMemcpy(s.light_mapping, tree.light_mapping.data(), infos.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (!e) e = hipMemcpy(s.light_orders, tree.light_orders.data(), infos.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (!e) e = hipMemcpy(s.light_infos, infos.data(), infos.size() * sizeof(Light_info), hipMemcpyHostToDevice);
+    }
+    if (!e) e = hipMemcpy(s.tri_light, tri_light.data(), size_t(num_triangles) * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e) {
         s.release();
         return fail(MI_ERR_HIP, "mi_scene_create: %s", hipGetErrorString(e));
@@ -980,6 +1162,7 @@ void RenderState::release() {
     (void)hipFree(shadow_rays);
     (void)hipFree(shadows);
     (void)hipFree(occluded);
+    (void)hipFree(shadow_range);
     (void)hipFree(counters);
     (void)hipFree(guide_x);
     (void)hipFree(guide_raw);
@@ -1001,7 +1184,7 @@ void RenderState::release() {
     *this = RenderState();
 }
 
-int32_t RenderState::reserve(uint32_t n) {
+int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
     if (!stream) {
         hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
         if (!e) e = hipEventCreateWithFlags(&begin, hipEventDisableTiming);
@@ -1012,6 +1195,22 @@ int32_t RenderState::reserve(uint32_t n) {
             return fail(MI_ERR_HIP, "render state: %s", hipGetErrorString(e));
         }
     }
+    if (num_shadow > shadow_capacity) {
+        (void)hipFree(shadow_rays);
+        (void)hipFree(shadows);
+        (void)hipFree(occluded);
+        shadow_rays     = nullptr;
+        shadows         = nullptr;
+        occluded        = nullptr;
+        shadow_capacity = 0;
+        hipError_t e    = hipMalloc(&shadow_rays, size_t(num_shadow) * sizeof(mi_ray));
+        if (!e) e = hipMalloc(&shadows, size_t(num_shadow) * sizeof(Shadow));
+        if (!e) e = hipMalloc(&occluded, size_t(num_shadow) * sizeof(uint32_t));
+        if (e) {
+            return fail(MI_ERR_HIP, "shadow queue of %u rays: %s", num_shadow, hipGetErrorString(e));
+        }
+        shadow_capacity = num_shadow;
+    }
     if (n <= capacity) {
         return MI_OK;
     }
@@ -1021,9 +1220,7 @@ int32_t RenderState::reserve(uint32_t n) {
     (void)hipFree(ray_path[0]);
     (void)hipFree(ray_path[1]);
     (void)hipFree(hits);
-    (void)hipFree(shadow_rays);
-    (void)hipFree(shadows);
-    (void)hipFree(occluded);
+    (void)hipFree(shadow_range);
     capacity     = 0;
     hipError_t e = hipMalloc(&paths, size_t(n) * sizeof(Path));
     for (int q = 0; q < 2 && !e; ++q) {
@@ -1031,9 +1228,7 @@ int32_t RenderState::reserve(uint32_t n) {
         if (!e) e = hipMalloc(&ray_path[q], size_t(n) * sizeof(uint32_t));
     }
     if (!e) e = hipMalloc(&hits, size_t(n) * sizeof(mi_hit));
-    if (!e) e = hipMalloc(&shadow_rays, size_t(n) * sizeof(mi_ray));
-    if (!e) e = hipMalloc(&shadows, size_t(n) * sizeof(Shadow));
-    if (!e) e = hipMalloc(&occluded, size_t(n) * sizeof(uint32_t));
+    if (!e) e = hipMalloc(&shadow_range, size_t(n) * sizeof(uint2));
     if (e) {
         release();
         return fail(MI_ERR_HIP, "render state of %u paths: %s", n, hipGetErrorString(e));
@@ -1125,10 +1320,21 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     double const up[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2],
                           r[0] * f[1] - r[1] * f[0]};
 
-    if (int32_t const st = state.reserve(num_paths); st) {
+    // a vertex sends one shadow ray per picked light: up to Max_lights where
+    // the tree splits
+    bool const     split_lights = !(set.flags & MI_RENDER_NO_LIGHT_SPLIT) && scene.num_lights > 1;
+    uint64_t const shadow64 =
+        uint64_t(num_paths) * (split_lights ? std::min(scene.num_lights, dev::Max_lights) : 1u);
+    if (shadow64 > (1ull << 30)) {
+        return fail(MI_ERR_ARGS, "mi_render: %llu paths per wave need a shadow queue of %llu rays; "
+                    "use fewer samples per wave", (unsigned long long)num_paths,
+                    (unsigned long long)shadow64);
+    }
+    uint32_t const num_shadow = uint32_t(shadow64);
+    if (int32_t const st = state.reserve(num_paths, num_shadow); st) {
         return st;
     }
-    if (int32_t const st = prepare_trace(scene.bvh, trace, num_paths); st) {
+    if (int32_t const st = prepare_trace(scene.bvh, trace, std::max(num_paths, num_shadow)); st) {
         return st;
     }
     uint32_t record_depth = 0;
@@ -1146,10 +1352,14 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.indices      = scene.indices;
     w.tri_material = scene.tri_material;
     w.materials    = scene.materials;
-    w.light_tris   = scene.light_tris;
-    w.light_cdf    = scene.light_cdf;
-    w.num_lights   = scene.num_lights;
-    w.light_area   = scene.light_area;
+    w.tree.nodes     = scene.light_nodes;
+    w.tree.middles   = scene.light_middles;
+    w.tree.mapping   = scene.light_mapping;
+    w.tree.orders    = scene.light_orders;
+    w.tree.lights    = scene.light_infos;
+    w.tree.num_nodes = scene.num_lights ? scene.num_light_nodes : 0;
+    w.tri_light      = scene.tri_light;
+    w.split_lights   = split_lights ? 1u : 0u;
     w.e_table      = e_table;
     for (int k = 0; k < 3; ++k) {
         w.eye[k]   = cam.eye[k];
@@ -1175,6 +1385,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.shadow_rays = state.shadow_rays;
     w.shadows     = static_cast<Shadow*>(state.shadows);
     w.occluded    = state.occluded;
+    w.shadow_range = state.shadow_range;
     w.counters    = state.counters;
     w.fraction        = guided ? gs->fraction : 0.f;
     w.guide_raw       = state.guide_raw;
@@ -1224,12 +1435,12 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
             }
             hipLaunchKernelGGL(tally, dim3(1), dim3(1), 0, s, state.counters, q);
             if (int32_t const st = launch_trace_device_count(scene.bvh, trace, state.shadow_rays,
-                                                             &state.counters[C_shadow], num_paths,
+                                                             &state.counters[C_shadow], num_shadow,
                                                              state.occluded, true, s);
                 st) {
                 return st;
             }
-            hipLaunchKernelGGL(shadow_resolve, dim3(grid_path), dim3(256), 0, s, w);
+            hipLaunchKernelGGL(shadow_resolve, dim3(grid_path), dim3(256), 0, s, w, q);
             q ^= 1;
         }
         hipLaunchKernelGGL(resolve, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, w, spw, accum);

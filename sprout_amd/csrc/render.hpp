@@ -3,6 +3,7 @@
 
 #include "common.hpp"
 #include "guide.hpp"
+#include "light_tree.hpp"
 #include "trace.hpp"
 
 #include <hip/hip_runtime.h>
@@ -28,14 +29,18 @@ struct RenderScene {
     uint32_t*        indices      = nullptr;  // 3 per triangle
     uint32_t*        tri_material = nullptr;
     Render_material* materials    = nullptr;
-    uint32_t*        light_tris   = nullptr;  // emissive triangles
-    float*           light_cdf    = nullptr;  // inclusive area CDF over light_tris, last = 1
+    Light_node*      light_nodes   = nullptr;  // light tree over the emissive triangles
+    uint32_t*        light_middles = nullptr;
+    uint32_t*        light_mapping = nullptr;
+    uint32_t*        light_orders  = nullptr;
+    Light_info*      light_infos   = nullptr;
+    uint32_t*        tri_light     = nullptr;  // light of each triangle, or Null
 
     uint32_t num_vertices  = 0;
     uint32_t num_triangles = 0;
     uint32_t num_materials = 0;
-    uint32_t num_lights    = 0;
-    float    light_area    = 0.f;  // total emissive area
+    uint32_t num_lights      = 0;
+    uint32_t num_light_nodes = 0;
     float    box_lo[3]     = {0.f, 0.f, 0.f};  // bounds of all vertices
     float    box_hi[3]     = {0.f, 0.f, 0.f};
 
@@ -52,6 +57,7 @@ struct RenderState {
     mi_ray*      shadow_rays = nullptr;
     void*        shadows     = nullptr;
     uint32_t*    occluded    = nullptr;
+    uint2*       shadow_range = nullptr;
     uint32_t*    counters    = nullptr;  // queue counters, wave base, ray totals
     hipStream_t  stream      = nullptr;  // internal stream the bounce loop runs (and is captured) on
     hipEvent_t   begin       = nullptr;
@@ -68,7 +74,9 @@ struct RenderState {
     float*    rec_r           = nullptr;
     uint32_t* rec_path        = nullptr;
 
-    int32_t reserve(uint32_t num_paths);
+    uint32_t  shadow_capacity = 0;
+
+    int32_t reserve(uint32_t num_paths, uint32_t num_shadow);
     int32_t reserve_guiding(uint32_t num_paths, uint32_t records);
     void    release();
 };
@@ -84,6 +92,15 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
                      uint32_t num_triangles, uint32_t const* tri_material,
                      mi_material const* materials, uint32_t num_materials,
                      mi_bvh_settings const& settings, RenderScene& out);
+
+// Batch light-tree queries on device arrays (sample: arg = randoms,
+// out0 = picks, out1 = counts; pdf: arg = light ids, out0 = pdfs).
+int32_t light_tree_queries(RenderScene const& scene, bool sample, float const* points,
+                           float const* normals, void const* arg, uint32_t split, uint32_t n,
+                           void* out0, uint32_t* out1, hipStream_t stream);
+
+int32_t download_light_tree(RenderScene const& scene, mi_light_node* nodes, uint32_t* middles,
+                            uint32_t* mapping, uint32_t* orders, void* infos);
 
 // net == nullptr: plain MIS; else guided by net with the settings gs.
 int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_table,

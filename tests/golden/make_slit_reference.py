@@ -1,0 +1,72 @@
+"""Writes the converged image of the slit room that the equal-time guiding
+comparison is measured against (needs a GPU).
+
+Slit room (the box room with its light enclosed behind walls open only
+through a 0.05-wide slit), 256 x 256, plain MIS (mi_render), max depth 8,
+Russian roulette from depth 4, 65536 samples per pixel accumulated in FP64,
+seed 1000 (independent of every render compared with it).  The image is
+stored as float32 radiance.
+
+    python tests/golden/make_slit_reference.py [--spp 65536] [--out PATH]
+"""
+
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+SIZE = 256
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--spp", type=int, default=65536)
+    ap.add_argument("--chunk", type=int, default=1024)
+    ap.add_argument("--out", default=os.path.join(HERE, "slit_room_ref_256.npz"))
+    ap.add_argument("--partial", default=None,
+                    help="FP64 running sum kept here after every chunk; an existing one is resumed")
+    ap.add_argument("--seconds", type=float, default=1e9, help="stop (resumable) after this long")
+    args = ap.parse_args()
+
+    import torch
+
+    from sprout_amd import bvh, render, scenes
+
+    bvh.init(0)
+    pos, idx, tmat, mats = scenes.room(0.05)
+    sc = render.Scene(pos, idx, tmat, mats)
+    cam = render.camera(SIZE, SIZE, **scenes.ROOM_CAMERA)
+    acc = torch.zeros((SIZE, SIZE, 3), dtype=torch.float64, device="cuda")
+    done = 0
+    if args.partial and os.path.exists(args.partial):
+        with np.load(args.partial, allow_pickle=False) as z:
+            acc.copy_(torch.from_numpy(z["sum"]))
+            done = int(z["done"])
+        print(f"resuming at {done} spp", flush=True)
+    t0 = time.perf_counter()
+    while done < args.spp and time.perf_counter() - t0 < args.seconds:
+        n = min(args.chunk, args.spp - done)
+        render.render(sc, cam, n, samples_per_wave=16 if n % 16 == 0 else 1, seed=1000,
+                      first_sample=done, accum=acc)
+        torch.cuda.synchronize()
+        done += n
+        if args.partial:
+            np.savez(args.partial, sum=acc.cpu().numpy(), done=np.int64(done))
+        print(f"{done} spp, {time.perf_counter() - t0:.1f} s", flush=True)
+    if done < args.spp:
+        print(f"stopped at {done} of {args.spp} spp", flush=True)
+        return
+    img = (acc / args.spp).cpu().numpy().astype(np.float32)
+    np.savez_compressed(args.out, image=img, spp=np.int64(args.spp), seed=np.int64(1000))
+    print(f"wrote {args.out}: mean {img.mean():.5f}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

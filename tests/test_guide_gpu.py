@@ -62,7 +62,9 @@ def params_for_test(seed):
 
 def batch(n, seed):
     rng = np.random.default_rng(seed)
-    x = rng.random((n, 9))
+    # float32 inputs, as the device sees them: 2^11 pi x amplifies the
+    # float64 -> float32 rounding of x to about 4e-4 in the top features
+    x = rng.random((n, 9)).astype(np.float32).astype(np.float64)
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     t = rng.exponential(size=n)

@@ -1,0 +1,101 @@
+"""Light tree on the GPU: the scene's tree and lights against the oracle's,
+device sampling and pdf against the oracle's restatement on the same tree,
+and a many-lights render with the same random numbers as the oracle's path
+tracer."""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from scene_util import light_tree, room_with_lights
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+
+    from sprout_amd import bvh, render
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    bvh.init(0)
+    yield render
+    torch.cuda.synchronize()
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def lit(gpu):
+    pos, idx, tmat, mats = room_with_lights()
+    sc = gpu.Scene(pos, idx, tmat, mats)
+    yield sc, (pos, idx, tmat, mats)
+    sc.release()
+
+
+def test_scene_tree_matches_oracle_lights(gpu, lit):
+    from sprout_amd import lights
+
+    sc, scene = lit
+    nodes, middles, mapping, orders, infos = lights.scene_tree(sc)
+    ref = light_tree(*scene)
+    assert len(infos) == len(ref.lights) == 50  # 48 + the two triangles of the ceiling light
+    for f in ("center", "radius", "cone", "power", "triangle", "area"):
+        assert np.array_equal(infos[f], ref.lights[f]), f
+    assert np.array_equal(nodes.view(np.uint8), ref.nodes.view(np.uint8))
+    assert np.array_equal(middles, ref.middles)
+    assert np.array_equal(mapping, ref.mapping)
+    assert np.array_equal(orders, ref.orders)
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_device_sampling_matches_oracle(gpu, lit, split):
+    from sprout_amd import lights
+
+    sc, scene = lit
+    tree = light_tree(*scene)
+    rng = np.random.default_rng(4)
+    m = 2000
+    p = np.concatenate([rng.uniform([-1, 0, -1], [1, 2, 1], (m - 200, 3)),
+                        tree.lights["center"][rng.integers(0, 50, 200)] - [0, 0.02, 0]]).astype(np.float32)
+    nrm = rng.normal(size=(m, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    nrm = nrm.astype(np.float32)
+    u = rng.random(m).astype(np.float32)
+    log(f"device light sampling, split {split}")
+    picks, pdfs, counts = lights.sample(sc, p, nrm, u, split)
+    multi = 0
+    for k in range(m):
+        ls, ps = tree.random_light(p[k], nrm[k], float(u[k]), split)
+        assert counts[k] == len(ls), k
+        assert np.array_equal(picks[k, :len(ls)], ls), k
+        assert np.allclose(pdfs[k, :len(ls)], ps, rtol=1e-5, atol=0), k
+        multi += len(ls) > 1
+    if split:
+        assert multi > 0
+    # pdf queries of the picked lights
+    first = picks[:, 0]
+    dev = lights.pdf(sc, p, nrm, first, split)
+    ref = np.array([tree.pdf(p[k], nrm[k], split, first[k]) for k in range(m)], np.float32)
+    assert np.allclose(dev, ref, rtol=1e-5, atol=0)
+
+
+def test_many_lights_render_same_seeds_as_oracle(gpu, lit):
+    import torch
+
+    from sprout_amd import scenes
+
+    sc, scene = lit
+    w, h, spp = 48, 40, 16
+    log("many-lights room, same seeds as the oracle")
+    cam = gpu.camera(w, h, **scenes.ROOM_CAMERA)
+    img, _ = gpu.render(sc, cam, spp, samples_per_wave=4, seed=8)
+    torch.cuda.synchronize()
+    img = img.cpu().numpy() / spp
+    ref = O.render(*scene, w, h, spp, scenes.ROOM_CAMERA, light_tree(*scene), seed=8) / spp
+    diff = np.abs(img - ref) / np.maximum(ref, 1e-3)
+    assert np.mean(diff < 1e-4) > 0.95, np.mean(diff < 1e-4)
+    assert float(np.mean((img - ref) ** 2 / (ref * ref + 1e-2))) < 1e-5

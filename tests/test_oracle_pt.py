@@ -22,6 +22,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from scene_util import light_tree
 from sprout_amd import scenes
 
 M = O.MATERIAL_DTYPE
@@ -69,8 +70,8 @@ def center_radiance(seed, spp, mode, half=0.5, height=1.0, albedo=0.6, radiance=
     pos, idx, tmat, mats = floor_and_light(half, height, albedo, radiance)
     cam = {"eye": (0.0, 0.5 * height, 0.0), "target": (0.0, 0.0, 0.0), "up": (0.0, 0.0, -1.0),
            "fov_deg": 0.5}
-    img = O.render(pos, idx, tmat, mats, 3, 3, spp, cam, max_bounces=0, min_bounces=0,
-                   seed=seed, mode=mode)
+    img = O.render(pos, idx, tmat, mats, 3, 3, spp, cam, light_tree(pos, idx, tmat, mats),
+                   max_bounces=0, min_bounces=0, seed=seed, mode=mode)
     return img[1, 1] / spp
 
 
@@ -93,9 +94,10 @@ def block_means(img, k):
 
 def room_batches(mode, batches, spp, size=16, **kw):
     pos, idx, tmat, mats = scenes.room()
+    tree = light_tree(pos, idx, tmat, mats)
     out = []
     for b in range(batches):
-        img = O.render(pos, idx, tmat, mats, size, size, spp, scenes.ROOM_CAMERA,
+        img = O.render(pos, idx, tmat, mats, size, size, spp, scenes.ROOM_CAMERA, tree,
                        first_sample=b * spp, seed=100 + mode, mode=mode, **kw)
         out.append(block_means(img / spp, 4))
     out = np.array(out)
@@ -124,9 +126,10 @@ def test_more_bounces_add_light():
     """The depth test follows each BSDF bounce (pathtracer_mis.cpp:262-264), so
     max_bounces 0 and 1 are the same estimator; beyond, light only adds up."""
     pos, idx, tmat, mats = scenes.room()
+    tree = light_tree(pos, idx, tmat, mats)
     imgs = []
     for mb in (0, 1, 2, 3, 5):
-        imgs.append(O.render(pos, idx, tmat, mats, 16, 16, 256, scenes.ROOM_CAMERA,
+        imgs.append(O.render(pos, idx, tmat, mats, 16, 16, 256, scenes.ROOM_CAMERA, tree,
                              max_bounces=mb, min_bounces=mb, seed=3))
     assert np.array_equal(imgs[0], imgs[1])
     means = [i.mean() for i in imgs[1:]]
@@ -138,7 +141,8 @@ def test_camera_sees_the_light_unweighted():
     pos, idx, tmat, mats = floor_and_light(radiance=5.0)
     cam = {"eye": (0.0, 0.0 + 1e-3, 0.0), "target": (0.0, 1.0, 0.0), "up": (0.0, 0.0, -1.0),
            "fov_deg": 10.0}
-    img = O.render(pos, idx, tmat, mats, 4, 4, 8, cam, max_bounces=3)
+    img = O.render(pos, idx, tmat, mats, 4, 4, 8, cam, light_tree(pos, idx, tmat, mats),
+                   max_bounces=3)
     assert np.all(img / 8 == 5.0)
 
 
@@ -146,5 +150,5 @@ def test_back_of_light_is_dark():
     pos, idx, tmat, mats = floor_and_light()
     cam = {"eye": (0.0, 3.0, 0.0), "target": (0.0, 0.0, 0.0), "up": (0.0, 0.0, -1.0),
            "fov_deg": 5.0}
-    img = O.render(pos, idx, tmat, mats, 4, 4, 16, cam)
+    img = O.render(pos, idx, tmat, mats, 4, 4, 16, cam, light_tree(pos, idx, tmat, mats))
     assert np.all(img == 0.0)

