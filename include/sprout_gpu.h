@@ -271,9 +271,14 @@ int32_t mi_light_tree_build(float const* box_min, float const* box_max, float co
                             mi_light_node* nodes, uint32_t* middles, uint32_t* mapping,
                             uint32_t* orders, uint32_t* num_nodes);
 
+enum mi_light_flags {
+    MI_LIGHT_SPLIT        = 1, /* split the tree (Adaptive light sampling) */
+    MI_LIGHT_TOTAL_SPHERE = 2, /* the point is in a medium: no horizon test (translucent) */
+};
+
 /* Device queries of a scene's light tree (every emissive triangle is a light;
  * light ids number them in triangle order).  DEVICE arrays: points and
- * normals n x 3.
+ * normals n x 3; split: MI_LIGHT_* flags.
  * sample: randoms n -> picks n x Max (64) x {light, pdf as float bits} and
  *         counts n (Tree::random_light, light_tree.cpp:363-450)
  * pdf:    lights n -> pdf n (Tree::pdf, light_tree.cpp:564-635)            */
@@ -289,6 +294,49 @@ int32_t mi_light_tree_pdf(uint32_t scene, float const* points, float const* norm
 int32_t mi_scene_light_tree(uint32_t scene, mi_light_node* nodes, uint32_t* middles,
                             uint32_t* mapping, uint32_t* orders, void* infos,
                             uint32_t* num_nodes, uint32_t* num_lights);
+
+/* ---- participating medium ---- */
+
+/* Gives a scene a heterogeneous medium: a density grid of dims[0] x dims[1] x
+ * dims[2] floats (HOST, x fastest, >= 0) stretched over the axis-aligned box
+ * [box_lo, box_hi]; absorption cc_a and scattering cc_s per unit density
+ * (rgb) and Henyey-Greenstein anisotropy g in (-1, 1).  Every ray segment in
+ * the box is tracked (Tracking_multi, tracking_multi.cpp:44-183: delta
+ * tracking with history-aware channel weights, majorants from the gridtree
+ * Octree_builder builds, volumetric_octree_builder.cpp:13-296); shadow rays
+ * through the box are attenuated by (residual) ratio tracking
+ * (tracking.cpp:32-129).  Scattering vertices sample the phase function and
+ * the light tree over the whole sphere.  Replaces the scene's medium;
+ * density NULL removes it.  Mirrors the volumetric Grid material on a
+ * box-shaped prop (volumetric_grid.cpp:18-70). */
+int32_t mi_scene_set_medium(uint32_t scene, float const* density, uint32_t const dims[3],
+                            float const box_lo[3], float const box_hi[3], float const cc_a[3],
+                            float const cc_s[3], float g);
+
+/* HOST, no GPU: the majorant gridtree of a density grid (Octree_builder::build).
+ * num_cells (3): top-level cells of 32^3 voxels per axis.  nodes: uint32 per
+ * node, bit 0 = has children, bits 1-31 = first child (8 consecutive) or
+ * data index (0x7FFFFFFF: empty); data: 4 floats per leaf {minorant mu_a,
+ * minorant mu_s, majorant mu_a, majorant mu_s}.  NULL arrays query the counts. */
+int32_t mi_medium_gridtree(float const* density, uint32_t const dims[3], float const cc_a[3],
+                           float const cc_s[3], uint32_t* num_cells, uint32_t* nodes,
+                           uint32_t* num_nodes, float* data, uint32_t* num_data);
+
+enum mi_medium_flags {
+    MI_MEDIUM_TRANSMITTANCE = 1, /* Tracking::transmittance instead of Tracking_multi::integrate */
+    MI_MEDIUM_NEAREST       = 2, /* unfiltered density lookups (Filter::Nearest) */
+};
+
+/* Batch tracking through a scene's medium, for checking: n DEVICE rays (each
+ * clipped to the medium box as the renderer clips it), one PCG32 generator
+ * per ray started at (seed, index), similarity-relation depth `depth`.
+ * out: n x 5 DEVICE floats.
+ * default: Tracking_multi::integrate -> {event (0 passed, 1 scattered,
+ *   2 aborted), scattering t, weight rgb}
+ * MI_MEDIUM_TRANSMITTANCE: Tracking::transmittance -> {1 transmitted /
+ *   0 blocked, 0, tr rgb} */
+int32_t mi_medium_track(uint32_t scene, mi_ray const* rays, uint32_t depth, uint64_t seed,
+                        uint32_t flags, uint32_t n, float* out, void* stream);
 
 /* The 32 x 32 GGX energy table E(n.wo, alpha) (rows alpha) regenerated on the
  * device by sprout's procedure (ggx_integrate.cpp:17-45, 202-240; the data of
@@ -379,6 +427,21 @@ int32_t mi_gen_room(float slit, float* positions, uint32_t* num_vertices, uint32
                     uint32_t* num_triangles, uint32_t* triangle_materials,
                     mi_material* materials, uint32_t* num_materials);
 
+
+/* Volume scene: a grey ground quad (y = 0, 8 x 8) and above it, facing
+ * down at y = 3.5, an emissive quad of 6 x 6 split into 1000 triangles
+ * (radiance 4 / 4.5 / 5); material 0 ground (Lambert 0.5), 1 light.  The
+ * medium is given separately (mi_scene_set_medium).  Pass NULL buffers to
+ * query the counts.  HOST pointers. */
+int32_t mi_gen_cloud_scene(float* positions, uint32_t* num_vertices, uint32_t* indices,
+                           uint32_t* num_triangles, uint32_t* triangle_materials,
+                           mi_material* materials, uint32_t* num_materials);
+
+/* Density grid nx x ny x nz (x fastest, HOST) from seeded fBm of `octaves`
+ * octaves of value noise inside a spherical envelope, scaled to [0,
+ * max_density]. */
+int32_t mi_gen_fbm_density(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t octaves, uint64_t seed,
+                           float max_density, float* density);
 
 /* Icosphere of num_subdivisions subdivisions of an icosahedron (20 * 4^s
  * triangles, 10 * 4^s + 2 vertices), radius 1, radially displaced by

@@ -134,7 +134,7 @@ static v3 hemisphere_cosine(float u, float v) {
 }
 
 void oracle_orthonormal_basis(float const n[3], float t[3], float b[3]) {
-    float const sign = n[2] < 0.f ? -1.f : 1.f;
+    float const sign = copysignf(1.f, n[2]);  // copysign1, base/math/math.hpp:124-128
     float const c    = -1.f / (sign + n[2]);
     float const d    = n[0] * n[1] * c;
     t[0]             = 1.f + sign * n[0] * n[0] * c;

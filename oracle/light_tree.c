@@ -38,7 +38,8 @@ static float clamped_sin_sub(float cos_a, float cos_b, float sin_a, float sin_b)
 }
 
 float oracle_light_importance(float const p[3], float const n[3], float const center[3],
-                              float const cone[4], float radius, float power, int two_sided) {
+                              float const cone[4], float radius, float power, int two_sided,
+                              int total_sphere) {
     float const axis[3] = {p[0] - center[0], p[1] - center[1], p[2] - center[2]};
     float const il      = 1.f / sqrtf(dot3(axis, axis));
     float const na[3]   = {il * axis[0], il * axis[1], il * axis[2]};
@@ -58,7 +59,7 @@ float oracle_light_importance(float const p[3], float const n[3], float const ce
     float const tc = clamped_cos_sub(ta, cos_cu, tb, sin_cu);
     float const tn = clamped_cos_sub(cos_n, cos_cu, sin_n, sin_cu);
 
-    float const ra     = tn; /* total_sphere = false */
+    float const ra     = total_sphere ? 1.f : tn;
     float const rb     = fmaxf(tc, 0.f);
     float const id_min = fminf(2.f / radius, il);
     float const base   = power * (id_min * id_min);
@@ -66,14 +67,14 @@ float oracle_light_importance(float const p[3], float const n[3], float const ce
     return fmaxf(ra * rb * base, DOT_MIN);
 }
 
-static float node_weight(oracle_light_node const* nd, float const p[3], float const n[3]) {
+static float node_weight(oracle_light_node const* nd, float const p[3], float const n[3], int ts) {
     return oracle_light_importance(p, n, nd->center, nd->cone, nd->radius, nd->power,
-                                   (int)((nd->bits >> 1) & 1u));
+                                   (int)((nd->bits >> 1) & 1u), ts);
 }
 
-static float light_weight(oracle_light const* l, float const p[3], float const n[3]) {
+static float light_weight(oracle_light const* l, float const p[3], float const n[3], int ts) {
     return oracle_light_importance(p, n, l->center, l->cone, l->radius, l->power,
-                                   (int)l->two_sided);
+                                   (int)l->two_sided, ts);
 }
 
 static int node_split(oracle_light_node const* nd, float const p[3]) {
@@ -104,11 +105,11 @@ static uint32_t children_or_light(oracle_light_node const* nd) {
 }
 
 static void leaf_cdf(oracle_light_tree const* t, oracle_light_node const* nd, float const p[3],
-                     float const n[3], float cdf[5]) {
+                     float const n[3], int ts, float cdf[5]) {
     float          w[4];
     uint32_t const begin = children_or_light(nd);
     for (uint32_t i = 0; i < 4; ++i) {
-        w[i] = i < nd->num_lights ? light_weight(&t->lights[t->mapping[begin + i]], p, n) : 0.f;
+        w[i] = i < nd->num_lights ? light_weight(&t->lights[t->mapping[begin + i]], p, n, ts) : 0.f;
     }
     float const ii = 1.f / (((w[0] + w[1]) + w[2]) + w[3]);
     cdf[0]         = 0.f;
@@ -126,7 +127,8 @@ typedef struct {
 } entry;
 
 uint32_t oracle_light_random(oracle_light_tree const* t, float const p[3], float const n[3],
-                             float random, int32_t split, uint32_t* lights, float* pdfs) {
+                             int32_t ts, float random, int32_t split, uint32_t* lights,
+                             float* pdfs) {
     if (0 == t->num_nodes) {
         return 0;
     }
@@ -148,8 +150,8 @@ uint32_t oracle_light_random(oracle_light_tree const* t, float const p[3], float
                 entry const push = {e.pdf, e.random, c1, e.depth};
                 stack[sp++]      = push;
             } else {
-                float       p0 = node_weight(&t->nodes[c0], p, n);
-                float       p1 = node_weight(&t->nodes[c1], p, n);
+                float       p0 = node_weight(&t->nodes[c0], p, n, ts);
+                float       p1 = node_weight(&t->nodes[c1], p, n, ts);
                 float const pt = p0 + p1;
                 p0 /= pt;
                 p1 /= pt;
@@ -178,7 +180,7 @@ uint32_t oracle_light_random(oracle_light_tree const* t, float const p[3], float
                 }
             } else {
                 float cdf[5];
-                leaf_cdf(t, nd, p, n, cdf);
+                leaf_cdf(t, nd, p, n, ts, cdf);
                 uint32_t it = nd->num_lights;
                 for (uint32_t i = 0; i < nd->num_lights; ++i) {
                     if (cdf[i] >= e.random) {
@@ -199,7 +201,7 @@ uint32_t oracle_light_random(oracle_light_tree const* t, float const p[3], float
 }
 
 float oracle_light_pdf(oracle_light_tree const* t, float const p[3], float const n[3],
-                       int32_t split, uint32_t light) {
+                       int32_t ts, int32_t split, uint32_t light) {
     if (0 == t->num_nodes) {
         return 0.f;
     }
@@ -214,8 +216,8 @@ float oracle_light_pdf(oracle_light_tree const* t, float const p[3], float const
             if (do_split) {
                 nid = lo < middle ? c0 : c0 + 1;
             } else {
-                float const p0 = node_weight(&t->nodes[c0], p, n);
-                float const p1 = node_weight(&t->nodes[c0 + 1], p, n);
+                float const p0 = node_weight(&t->nodes[c0], p, n, ts);
+                float const p1 = node_weight(&t->nodes[c0 + 1], p, n, ts);
                 float const pt = p0 + p1;
                 if (lo < middle) {
                     nid = c0;
@@ -231,7 +233,7 @@ float oracle_light_pdf(oracle_light_tree const* t, float const p[3], float const
                 return pdf;
             }
             float cdf[5];
-            leaf_cdf(t, nd, p, n, cdf);
+            leaf_cdf(t, nd, p, n, ts, cdf);
             uint32_t const idx = lo - children_or_light(nd);
             return pdf * (cdf[idx + 1] - cdf[idx]);
         }

@@ -55,7 +55,21 @@ class _OraclePtSettings(ctypes.Structure):
     _fields_ = [("samples_per_pixel", ctypes.c_uint32), ("first_sample", ctypes.c_uint32),
                 ("min_bounces", ctypes.c_uint32), ("max_bounces", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64), ("mode", ctypes.c_int32),
-                ("num_threads", ctypes.c_uint32), ("split_lights", ctypes.c_int32)]
+                ("num_threads", ctypes.c_uint32), ("split_lights", ctypes.c_int32),
+                ("medium", ctypes.c_void_p)]
+
+
+class _OracleGridtree(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_void_p), ("data", ctypes.c_void_p),
+                ("num_nodes", ctypes.c_uint32), ("num_data", ctypes.c_uint32),
+                ("num_cells", ctypes.c_int32 * 3), ("dims", ctypes.c_int32 * 3)]
+
+
+class _OracleMedium(ctypes.Structure):
+    _fields_ = [("density", ctypes.c_void_p), ("dims", ctypes.c_int32 * 3),
+                ("tree", ctypes.POINTER(_OracleGridtree)), ("box_lo", ctypes.c_float * 3),
+                ("box_hi", ctypes.c_float * 3), ("cc_a", ctypes.c_float * 3),
+                ("cc_s", ctypes.c_float * 3), ("g", ctypes.c_float)]
 
 
 class _OracleLightTree(ctypes.Structure):
@@ -129,13 +143,34 @@ def lib():
                                        ctypes.c_uint32, ctypes.POINTER(_OracleLightTree),
                                        ctypes.POINTER(_OracleCamera),
                                        ctypes.POINTER(_OraclePtSettings), ctypes.c_void_p]
+        _lib.oracle_gridtree_build.restype = ctypes.c_int32
+        _lib.oracle_gridtree_build.argtypes = [ctypes.c_void_p] * 4 + [ctypes.POINTER(_OracleGridtree)]
+        _lib.oracle_gridtree_free.restype = None
+        _lib.oracle_gridtree_free.argtypes = [ctypes.POINTER(_OracleGridtree)]
+        _lib.oracle_medium_batch.restype = ctypes.c_int32
+        _lib.oracle_medium_batch.argtypes = [ctypes.POINTER(_OracleMedium), ctypes.c_void_p,
+                                             ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
+        _lib.oracle_medium_density.restype = ctypes.c_float
+        _lib.oracle_medium_density.argtypes = [ctypes.POINTER(_OracleMedium), ctypes.c_void_p,
+                                               ctypes.c_int32]
+        _lib.oracle_phase_hg.restype = ctypes.c_float
+        _lib.oracle_phase_hg.argtypes = [ctypes.c_float, ctypes.c_float]
+        _lib.oracle_sample_phase.restype = None
+        _lib.oracle_sample_phase.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.oracle_offset_f.restype = ctypes.c_float
+        _lib.oracle_offset_f.argtypes = [ctypes.c_float]
+        _lib.oracle_similarity_scale.restype = ctypes.c_float
+        _lib.oracle_similarity_scale.argtypes = [ctypes.c_float, ctypes.c_uint32]
         _lib.oracle_light_random.restype = ctypes.c_uint32
         _lib.oracle_light_random.argtypes = [ctypes.POINTER(_OracleLightTree), ctypes.c_void_p,
-                                             ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
-                                             ctypes.c_void_p, ctypes.c_void_p]
+                                             ctypes.c_void_p, ctypes.c_int32, ctypes.c_float,
+                                             ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
         _lib.oracle_light_pdf.restype = ctypes.c_float
         _lib.oracle_light_pdf.argtypes = [ctypes.POINTER(_OracleLightTree), ctypes.c_void_p,
-                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32]
+                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_uint32]
     return _lib
 
 
@@ -421,20 +456,22 @@ class LightTree:
                                   _ptr(self.orders), _ptr(self.lights),
                                   len(self.nodes) if len(self.lights) else 0, len(self.lights))
 
-    def random_light(self, p, n, random, split):
+    def random_light(self, p, n, random, split, total_sphere=False):
         pv = np.ascontiguousarray(p, np.float32)
         nv = np.ascontiguousarray(n, np.float32)
         lights = np.zeros(64, np.uint32)
         pdfs = np.zeros(64, np.float32)
-        k = lib().oracle_light_random(ctypes.byref(self.c), _ptr(pv), _ptr(nv), random,
-                                      1 if split else 0, _ptr(lights), _ptr(pdfs))
+        k = lib().oracle_light_random(ctypes.byref(self.c), _ptr(pv), _ptr(nv),
+                                      1 if total_sphere else 0, random, 1 if split else 0,
+                                      _ptr(lights), _ptr(pdfs))
         return lights[:k].copy(), pdfs[:k].copy()
 
-    def pdf(self, p, n, split, light) -> float:
+    def pdf(self, p, n, split, light, total_sphere=False) -> float:
         pv = np.ascontiguousarray(p, np.float32)
         nv = np.ascontiguousarray(n, np.float32)
         return float(lib().oracle_light_pdf(ctypes.byref(self.c), _ptr(pv), _ptr(nv),
-                                            1 if split else 0, int(light)))
+                                            1 if total_sphere else 0, 1 if split else 0,
+                                            int(light)))
 
 
 def scene_light_tree(positions, indices, triangle_materials, materials, build) -> LightTree:
@@ -452,12 +489,82 @@ def scene_light_tree(positions, indices, triangle_materials, materials, build) -
     return LightTree(nodes, middles, mapping, orders, rec)
 
 
+class Medium:
+    """A density grid (nz, ny, nx) float32, x fastest, filling the box
+    [box_lo, box_hi], with absorption / scattering per unit density (rgb) and
+    Henyey-Greenstein g; its majorant gridtree is built by medium.c."""
+
+    def __init__(self, density, box_lo, box_hi, cc_a, cc_s, g):
+        self.density = np.ascontiguousarray(density, np.float32)
+        assert self.density.ndim == 3
+        nz, ny, nx = self.density.shape
+        self.dims = (nx, ny, nz)
+        self.tree = _OracleGridtree()
+        dims = np.array(self.dims, np.int32)
+        a = np.asarray(cc_a, np.float32)
+        s_ = np.asarray(cc_s, np.float32)
+        st = lib().oracle_gridtree_build(_ptr(self.density), _ptr(dims), _ptr(a), _ptr(s_),
+                                         ctypes.byref(self.tree))
+        if st:
+            raise RuntimeError(f"oracle_gridtree_build: status {st}")
+        f3 = ctypes.c_float * 3
+        self.c = _OracleMedium(_ptr(self.density), (ctypes.c_int32 * 3)(*self.dims),
+                               ctypes.pointer(self.tree), f3(*box_lo), f3(*box_hi), f3(*a),
+                               f3(*s_), float(g))
+
+    def __del__(self):
+        try:
+            lib().oracle_gridtree_free(ctypes.byref(self.tree))
+        except Exception:
+            pass
+
+    def gridtree(self):
+        """(nodes uint32, data (n, 4) float32, num_cells)"""
+        nodes = np.ctypeslib.as_array(ctypes.cast(self.tree.nodes, ctypes.POINTER(ctypes.c_uint32)),
+                                      (self.tree.num_nodes,)).copy()
+        data = np.ctypeslib.as_array(ctypes.cast(self.tree.data, ctypes.POINTER(ctypes.c_float)),
+                                     (max(self.tree.num_data, 1) * 4,)).copy().reshape(-1, 4)
+        return nodes, data[:self.tree.num_data], tuple(self.tree.num_cells)
+
+    def density_at(self, uvw, nearest=False) -> float:
+        u = np.ascontiguousarray(uvw, np.float32)
+        return float(lib().oracle_medium_density(ctypes.byref(self.c), _ptr(u), 1 if nearest else 0))
+
+    def batch(self, rays, depth=0, seed=1, transmittance=False, nearest=False):
+        """rays (n, 8): origin, min_t, direction, max_t -> (n, 5) as mi_medium_track."""
+        r = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((len(r), 5), np.float32)
+        lib().oracle_medium_batch(ctypes.byref(self.c), _ptr(r), len(r), depth, seed,
+                                  1 if transmittance else 0, 1 if nearest else 0, _ptr(out))
+        return out
+
+
+def phase_hg(cos_theta, g) -> float:
+    return float(lib().oracle_phase_hg(cos_theta, g))
+
+
+def sample_phase(wo, r0, r1, g):
+    w = np.ascontiguousarray(wo, np.float32)
+    wi = np.zeros(3, np.float32)
+    pdf = ctypes.c_float()
+    lib().oracle_sample_phase(_ptr(w), r0, r1, g, _ptr(wi), ctypes.byref(pdf))
+    return wi, pdf.value
+
+
+def offset_f(t) -> float:
+    return float(lib().oracle_offset_f(float(t)))
+
+
+def similarity_scale(g, depth) -> float:
+    return float(lib().oracle_similarity_scale(g, depth))
+
+
 def render(positions, indices, triangle_materials, materials, width, height, spp, camera,
            tree: LightTree, first_sample=0, min_bounces=4, max_bounces=8, seed=1, mode=PT_MIS,
-           threads=None, split_lights=True):
+           threads=None, split_lights=True, medium: Medium | None = None):
     """Radiance sums per pixel, (height, width, 3) float64, of the CPU path
     tracer (pt.c).  camera: dict with eye, target, up, fov_deg; tree: the
-    scene's light tree."""
+    scene's light tree; medium: a participating medium or None."""
     pos = np.ascontiguousarray(positions, np.float32)
     idx = np.ascontiguousarray(indices, np.uint32)
     tmat = np.ascontiguousarray(triangle_materials, np.uint32)
@@ -467,7 +574,8 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
                         (ctypes.c_float * 3)(*camera["up"]), camera["fov_deg"])
     n = threads or min(os.cpu_count() or 1, 16)
     st = _OraclePtSettings(spp, first_sample, min_bounces, max_bounces, seed, mode, n,
-                           1 if split_lights else 0)
+                           1 if split_lights else 0,
+                           ctypes.cast(ctypes.pointer(medium.c), ctypes.c_void_p) if medium else None)
     accum = np.zeros((height, width, 3), np.float64)
     status = lib().oracle_render(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(tmat),
                                  _ptr(mats), len(mats), ctypes.byref(tree.c), ctypes.byref(cam),

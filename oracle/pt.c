@@ -18,7 +18,8 @@
  * sample * W * H + y * W + x), in this order: 2 for the pixel jitter; per
  * vertex 1 for Russian roulette when it applies, 3 for the light sample when
  * the scene has lights (1 to pick them, then 2 per picked light), 3 for the
- * BSDF sample.  Lights (one per emissive triangle) are picked by the light
+ * BSDF sample (see path() for the additions a medium makes).  Lights (one
+ * per emissive triangle) are picked by the light
  * tree (light_tree.c), split below depth 3 when asked (sprout's Adaptive
  * light sampling, pathtracer_mis.cpp:423-426); the pinhole camera jitters
  * uniformly inside the pixel.  Closest and any hits come from the oracle's own SBVH (sbvh.c,
@@ -72,6 +73,14 @@ static float pcg_float(pcg* g) {
     float    f;
     memcpy(&f, &bits, 4);
     return f - 1.f;
+}
+
+void oracle_pcg_start(oracle_pcg* g, uint64_t seed, uint64_t sequence) {
+    pcg_start((pcg*)g, seed, sequence);
+}
+
+float oracle_pcg_float(void* g) {
+    return pcg_float((pcg*)g);
 }
 
 typedef struct {
@@ -154,8 +163,11 @@ static int32_t occluded(scene_t const* s, v3 o, v3 d, float max_t, uint32_t* occ
                         ORACLE_RCP_PINNED, occ, NULL, 1);
 }
 
+#define KIND_PHASE 0x100 /* scattering vertex in the medium */
+
 typedef struct {
     int32_t kind;
+    float   g;          /* phase anisotropy */
     float   params[6];  /* color, roughness, metallic, ior */
     float   frame[12];  /* t, b, n, geo_n */
     v3      n;
@@ -190,6 +202,11 @@ static void surface_set(surface* sf, oracle_material const* m, v3 geo_n, v3 wo) 
 
 /* out: reflection * n.wi (rgb), pdf */
 static void surface_evaluate(surface const* sf, v3 wi, float out[4]) {
+    if (KIND_PHASE == sf->kind) {
+        float const ph = oracle_phase_hg(dot(sf->wo, wi), sf->g);
+        out[0] = out[1] = out[2] = out[3] = ph;
+        return;
+    }
     if (KIND_LAMBERT == sf->kind) {
         float const n_dot_wi = clampf(dot(sf->n, wi), DOT_MIN, 1.f);
         for (int c = 0; c < 3; ++c) {
@@ -205,6 +222,16 @@ static void surface_evaluate(surface const* sf, v3 wi, float out[4]) {
 
 /* out: wi, reflection (rgb), pdf, type */
 static void surface_sample(surface const* sf, float xl, float x0, float x1, float out[8]) {
+    if (KIND_PHASE == sf->kind) {
+        float const wo[3] = {sf->wo.x, sf->wo.y, sf->wo.z};
+        float       pdf;
+        oracle_sample_phase(wo, x0, x1, sf->g, out, &pdf);
+        out[3] = out[4] = out[5] = pdf;
+        out[6]              = pdf;
+        uint32_t const type = TYPE_DIFFUSE;
+        memcpy(&out[7], &type, 4);
+        return;
+    }
     if (KIND_LAMBERT == sf->kind) {
         float xy[2];
         oracle_disk_concentric(x0, x1, xy);
@@ -239,9 +266,28 @@ static float predivided_power_heuristic(float f, float g) {
     return f / (f * f + g * g);
 }
 
-/* One camera sample; adds its radiance to rgb. */
+/* a shadow ray of the current vertex, resolved after its direction sample */
+typedef struct {
+    float contrib[3];
+    v3    o;
+    v3    wi;
+    float dist;
+} shadow_t;
+
+/* One camera sample; adds its radiance to rgb.
+ *
+ * With a medium (Tracking_multi, pathtracer_mis.cpp:215-250): the part of
+ * every ray before its hit that lies in the medium box is tracked; the
+ * tracking weight multiplies the throughput, a scattering point becomes a
+ * vertex that samples the phase function and the light tree over the whole
+ * sphere, and an abort ends the path.  The unoccluded shadow rays of a vertex
+ * are attenuated by the medium after the vertex's direction sample, in the
+ * order their lights were picked.  The generator order per vertex is then:
+ * tracking of the ray that reached it, [roulette 1], [light select 1, 2 per
+ * picked light], direction 3, transmittance of its shadow rays. */
 static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, double rgb[3]) {
-    oracle_pt_settings const* set = s->set;
+    oracle_pt_settings const* set    = s->set;
+    oracle_medium const*      medium = set->medium;
     uint32_t const W = s->cam->width;
     uint32_t const H = s->cam->height;
     int32_t const  mode = set->mode;
@@ -262,15 +308,16 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
     float    radiance[3]   = {0.f, 0.f, 0.f};
     float    bsdf_pdf      = 0.f;
     int      singular      = 1;
+    int      primary       = 1; /* State::Primary_ray */
+    int      translucent   = 0; /* the previous vertex was in the medium */
+    int      nearest       = 0; /* unfiltered density lookups (Filter::Nearest) */
     float    prev_n[3]     = {0.f, 0.f, 0.f};
+    shadow_t shadows[ORACLE_MAX_LIGHTS];
 
     for (uint32_t depth = 0;; ++depth) {
         uint32_t hit[4];
         if (closest(s, o, d, RAY_MAX_T, hit)) {
             return ORACLE_ERR_STACK;
-        }
-        if (SENTINEL == hit[3]) {
-            break;
         }
         float t, hu, hv;
         memcpy(&t, &hit[0], 4);
@@ -278,36 +325,76 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
         memcpy(&hv, &hit[2], 4);
         uint32_t const tri = hit[3];
 
-        v3 a, b, c;
-        tri_vertices(s, tri, &a, &b, &c);
-        float const bw    = (1.f - hu) - hv;
-        v3 const    hit_p = add(add(scale(bw, a), scale(hu, b)), scale(hv, c));
-        v3 const    geo_n = normalize(cross(sub(b, a), sub(c, a)));
-        v3 const    wo    = mk(-d.x, -d.y, -d.z);
-
-        oracle_material const* m = &s->materials[s->tri_material[tri]];
-
-        if (KIND_LIGHT == m->kind) {
-            float const cos_l = dot(geo_n, wo);
-            if (cos_l > 0.f) {
-                float          weight = 1.f;
-                uint32_t const l      = s->tri_light[tri];
-                if (0 != depth && !singular && SENTINEL != l) {
-                    if (ORACLE_PT_MIS == mode) {
-                        int const   split    = set->split_lights && depth < 3;
-                        float const origin[3] = {o.x, o.y, o.z};
-                        float const tree_pdf = oracle_light_pdf(s->tree, origin, prev_n, split, l);
-                        float const ls_pdf   = (t * t) / (cos_l * s->tree->lights[l].area);
-                        weight               = power_heuristic(bsdf_pdf, ls_pdf * tree_pdf);
-                    } else if (ORACLE_PT_NEE == mode) {
-                        weight = 0.f;
-                    }
-                }
+        int   scatter = 0;
+        v3    hit_p   = mk(0.f, 0.f, 0.f);
+        v3    geo_n   = mk(0.f, 0.f, 1.f);
+        if (medium) {
+            float const oa[3] = {o.x, o.y, o.z};
+            float const da[3] = {d.x, d.y, d.z};
+            float       t0, t1;
+            if (oracle_medium_overlap(medium, oa, da, 0.f, SENTINEL == tri ? RAY_MAX_T : t, &t0, &t1)) {
+                float         ts = 0.f;
+                float         w[3];
+                int32_t const e = oracle_medium_track(medium, oa, da, t0, t1, depth, nearest,
+                                                      oracle_pcg_float, &rng, &ts, w);
                 for (int k = 0; k < 3; ++k) {
-                    radiance[k] += throughput[k] * (weight * m->emission[k]);
+                    throughput[k] *= w[k];
+                }
+                if (2 == e) {
+                    break; /* Event::Abort */
+                }
+                if (1 == e) {
+                    scatter = 1;
+                    hit_p   = mk(o.x + ts * d.x, o.y + ts * d.y, o.z + ts * d.z);
                 }
             }
+        }
+        if (!scatter && SENTINEL == tri) {
             break;
+        }
+
+        v3 const wo = mk(-d.x, -d.y, -d.z);
+        surface  sf;
+        v3       p_o;
+
+        if (!scatter) {
+            v3 a, b, c;
+            tri_vertices(s, tri, &a, &b, &c);
+            float const bw = (1.f - hu) - hv;
+            hit_p          = add(add(scale(bw, a), scale(hu, b)), scale(hv, c));
+            geo_n          = normalize(cross(sub(b, a), sub(c, a)));
+
+            oracle_material const* m = &s->materials[s->tri_material[tri]];
+
+            if (KIND_LIGHT == m->kind) {
+                float const cos_l = dot(geo_n, wo);
+                if (cos_l > 0.f) {
+                    float          weight = 1.f;
+                    uint32_t const l      = s->tri_light[tri];
+                    if (0 != depth && !singular && SENTINEL != l) {
+                        if (ORACLE_PT_MIS == mode) {
+                            int const   split     = set->split_lights && depth < 3;
+                            float const origin[3] = {o.x, o.y, o.z};
+                            float const tree_pdf =
+                                oracle_light_pdf(s->tree, origin, prev_n, translucent, split, l);
+                            float const ls_pdf = (t * t) / (cos_l * s->tree->lights[l].area);
+                            weight             = power_heuristic(bsdf_pdf, ls_pdf * tree_pdf);
+                        } else if (ORACLE_PT_NEE == mode) {
+                            weight = 0.f;
+                        }
+                    }
+                    for (int k = 0; k < 3; ++k) {
+                        radiance[k] += throughput[k] * (weight * m->emission[k]);
+                    }
+                }
+                break;
+            }
+            surface_set(&sf, m, geo_n, wo);
+        } else {
+            memset(&sf, 0, sizeof(sf));
+            sf.kind = KIND_PHASE;
+            sf.g    = medium->g;
+            sf.wo   = wo;
         }
 
         if (0 != depth) {
@@ -325,14 +412,15 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
             }
         }
 
-        if (!(dot(geo_n, wo) > 0.f)) {
+        if (!scatter && !(dot(geo_n, wo) > 0.f)) {
             break;
         }
 
-        surface sf;
-        surface_set(&sf, m, geo_n, wo);
+        nearest = !(depth <= 1 || primary);
 
-        v3 const p_o = offset_ray(hit_p, geo_n);
+        /* a scattering vertex is its own shadow-ray origin (subsurface offset_p) */
+        p_o = scatter ? hit_p : offset_ray(hit_p, geo_n);
+        uint32_t num_shadows = 0;
         if (s->tree->num_nodes) {
             int const      split     = set->split_lights && depth < 3;
             float const    select    = pcg_float(&rng);
@@ -340,7 +428,8 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
             float const    gn[3]     = {geo_n.x, geo_n.y, geo_n.z};
             uint32_t       picks[ORACLE_MAX_LIGHTS];
             float          ppdf[ORACLE_MAX_LIGHTS];
-            uint32_t const num_picks = oracle_light_random(s->tree, po, gn, select, split, picks, ppdf);
+            uint32_t const num_picks =
+                oracle_light_random(s->tree, po, gn, scatter, select, split, picks, ppdf);
             for (uint32_t k = 0; k < num_picks; ++k) {
                 oracle_light const* L  = &s->tree->lights[picks[k]];
                 uint32_t const      lt = L->triangle;
@@ -371,19 +460,20 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
                                                 ? predivided_power_heuristic(light_pdf, f[3])
                                                 : 1.f / light_pdf;
                     oracle_material const* lm = &s->materials[s->tri_material[lt]];
-                    float                  contrib[3];
+                    shadow_t*              sh = &shadows[num_shadows];
                     for (int c = 0; c < 3; ++c) {
-                        contrib[c] = throughput[c] * (weight * (lm->emission[c] * f[c]));
+                        sh->contrib[c] = throughput[c] * (weight * (lm->emission[c] * f[c]));
                     }
-                    if (contrib[0] > 0.f || contrib[1] > 0.f || contrib[2] > 0.f) {
+                    if (sh->contrib[0] > 0.f || sh->contrib[1] > 0.f || sh->contrib[2] > 0.f) {
                         uint32_t occ = 0;
                         if (occluded(s, p_o, wi, dist, &occ)) {
                             return ORACLE_ERR_STACK;
                         }
                         if (!occ) {
-                            for (int c = 0; c < 3; ++c) {
-                                radiance[c] += contrib[c];
-                            }
+                            sh->o    = p_o;
+                            sh->wi   = wi;
+                            sh->dist = dist;
+                            ++num_shadows;
                         }
                     }
                 }
@@ -395,22 +485,47 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
         float const x1 = pcg_float(&rng);
         float       bs[8];
         surface_sample(&sf, xl, x0, x1, bs);
+
+        /* the vertex's unoccluded shadow rays, after its direction sample */
+        for (uint32_t k = 0; k < num_shadows; ++k) {
+            shadow_t const* sh    = &shadows[k];
+            float           tr[3] = {1.f, 1.f, 1.f};
+            if (medium) {
+                float const oa[3] = {sh->o.x, sh->o.y, sh->o.z};
+                float const da[3] = {sh->wi.x, sh->wi.y, sh->wi.z};
+                float       t0, t1;
+                if (oracle_medium_overlap(medium, oa, da, 0.f, sh->dist, &t0, &t1) &&
+                    !oracle_medium_transmittance(medium, oa, da, t0, t1, depth, nearest,
+                                                 oracle_pcg_float, &rng, tr)) {
+                    continue;
+                }
+            }
+            for (int c = 0; c < 3; ++c) {
+                radiance[c] += tr[c] * sh->contrib[c];
+            }
+        }
+
         if (!(bs[6] > 0.f)) {
             break;
         }
         uint32_t type;
         memcpy(&type, &bs[7], 4);
         singular = TYPE_SPECULAR == type;
-        bsdf_pdf = bs[6];
+        if (!singular) {
+            primary = 0;
+        }
+        translucent = scatter;
+        bsdf_pdf    = bs[6];
         for (int k = 0; k < 3; ++k) {
             throughput[k] *= bs[3 + k] / bs[6];
         }
         v3 const wi = mk(bs[0], bs[1], bs[2]);
-        o           = offset_ray(hit_p, dot(geo_n, wi) > 0.f ? geo_n : mk(-geo_n.x, -geo_n.y, -geo_n.z));
-        d           = wi;
-        prev_n[0]   = geo_n.x;
-        prev_n[1]   = geo_n.y;
-        prev_n[2]   = geo_n.z;
+        o = scatter ? hit_p
+                    : offset_ray(hit_p, dot(geo_n, wi) > 0.f ? geo_n : mk(-geo_n.x, -geo_n.y, -geo_n.z));
+        d         = wi;
+        prev_n[0] = geo_n.x;
+        prev_n[1] = geo_n.y;
+        prev_n[2] = geo_n.z;
     }
 
     for (int k = 0; k < 3; ++k) {

@@ -135,10 +135,15 @@ SIGNATURES = {
     "mi_light_tree_sample": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp]),
     "mi_light_tree_pdf": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp]),
     "mi_scene_light_tree": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _pu32, _pu32]),
+    "mi_scene_set_medium": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp, _f32]),
+    "mi_medium_gridtree": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _pu32, _vp, _pu32]),
+    "mi_medium_track": (_i32, [_u32, _vp, _u32, ctypes.c_uint64, _u32, _u32, _vp, _vp]),
     "mi_ggx_e_table": (_i32, [_vp, _vp]),
     "mi_substitute_evaluate": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mi_substitute_sample": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mi_gen_room": (_i32, [_f32, _vp, _pu32, _vp, _pu32, _vp, _vp, _pu32]),
+    "mi_gen_cloud_scene": (_i32, [_vp, _pu32, _vp, _pu32, _vp, _vp, _pu32]),
+    "mi_gen_fbm_density": (_i32, [_u32, _u32, _u32, _u32, ctypes.c_uint64, _f32, _vp]),
     "mi_guide_create": (_i32, [_u64, _pu32]),
     "mi_guide_release": (_i32, [_u32]),
     "mi_guide_set_params": (_i32, [_u32, _vp]),

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsprout_gpu.so")
 
-SOURCES = ["capi.cpp", "gen.cpp", "light_tree.cpp", "bvh_device.hip", "bvh_build.hip", "trace.hip",
+SOURCES = ["capi.cpp", "gen.cpp", "light_tree.cpp", "medium.cpp", "bvh_device.hip", "bvh_build.hip", "trace.hip",
            "render.hip",
            "guide.hip"]
 

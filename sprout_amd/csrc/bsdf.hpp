@@ -92,7 +92,7 @@ struct Layer {
 
 // orthonormal_basis, base/math/vector3.inl:523-534
 __device__ __forceinline__ Layer frame_from_normal(V3 n) {
-    float const sign = n.z < 0.f ? -1.f : 1.f;
+    float const sign = copysignf(1.f, n.z);  // copysign1, base/math/math.hpp:124-128
     float const c    = -1.f / (sign + n.z);
     float const d    = n.x * n.y * c;
     Layer       l;

@@ -663,6 +663,76 @@ int32_t mi_light_tree_pdf(uint32_t scene, float const* points, float const* norm
                               as_stream(stream));
 }
 
+int32_t mi_scene_set_medium(uint32_t scene, float const* density, uint32_t const dims[3],
+                            float const box_lo[3], float const box_hi[3], float const cc_a[3],
+                            float const cc_s[3], float g) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT,
+                    "mi_scene_set_medium: bad handle %u", scene);
+    }
+    if (density && (!dims || !box_lo || !box_hi || !cc_a || !cc_s)) {
+        return fail(MI_ERR_ARGS, "mi_scene_set_medium: null argument");
+    }
+    (void)hipDeviceSynchronize();  // renders in flight may read the old medium
+    return set_medium(*sc, density, dims, box_lo, box_hi, cc_a, cc_s, g);
+}
+
+int32_t mi_medium_gridtree(float const* density, uint32_t const dims[3], float const cc_a[3],
+                           float const cc_s[3], uint32_t* num_cells, uint32_t* nodes,
+                           uint32_t* num_nodes, float* data, uint32_t* num_data) {
+    clear_error();
+    if (!density || !dims || !cc_a || !cc_s || !num_nodes || !num_data) {
+        return fail(MI_ERR_ARGS, "mi_medium_gridtree: null argument");
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (0 == dims[k] || dims[k] > 4096) {
+            return fail(MI_ERR_ARGS, "mi_medium_gridtree: dimension %u out of [1, 4096]", dims[k]);
+        }
+    }
+    int32_t const idims[3] = {int32_t(dims[0]), int32_t(dims[1]), int32_t(dims[2])};
+    HostGridtree  tree;
+    build_gridtree(density, idims, cc_a, cc_s, tree);
+    if (num_cells) {
+        for (int k = 0; k < 3; ++k) {
+            num_cells[k] = uint32_t(tree.num_cells[k]);
+        }
+    }
+    if (nodes) {
+        if (*num_nodes < tree.nodes.size()) {
+            return fail(MI_ERR_ARGS, "mi_medium_gridtree: %zu nodes do not fit in %u",
+                        tree.nodes.size(), *num_nodes);
+        }
+        std::memcpy(nodes, tree.nodes.data(), tree.nodes.size() * sizeof(uint32_t));
+    }
+    if (data) {
+        if (*num_data < tree.data.size()) {
+            return fail(MI_ERR_ARGS, "mi_medium_gridtree: %zu leaves do not fit in %u",
+                        tree.data.size(), *num_data);
+        }
+        std::memcpy(data, tree.data.data(), tree.data.size() * sizeof(Medium_cm));
+    }
+    *num_nodes = uint32_t(tree.nodes.size());
+    *num_data  = uint32_t(tree.data.size());
+    return MI_OK;
+}
+
+int32_t mi_medium_track(uint32_t scene, mi_ray const* rays, uint32_t depth, uint64_t seed,
+                        uint32_t flags, uint32_t n, float* out, void* stream) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT, "mi_medium_track: bad handle %u",
+                    scene);
+    }
+    if (n && (!rays || !out)) {
+        return fail(MI_ERR_ARGS, "mi_medium_track: null argument");
+    }
+    return medium_queries(*sc, rays, depth, seed, flags & (MI_MEDIUM_TRANSMITTANCE | MI_MEDIUM_NEAREST),
+                          n, out, as_stream(stream));
+}
+
 int32_t mi_scene_light_tree(uint32_t scene, mi_light_node* nodes, uint32_t* middles,
                             uint32_t* mapping, uint32_t* orders, void* infos,
                             uint32_t* num_nodes, uint32_t* num_lights) {

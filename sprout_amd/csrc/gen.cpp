@@ -8,8 +8,10 @@
 
 #include "common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -473,6 +475,97 @@ extern "C" int32_t mi_gen_room(float slit, float* positions, uint32_t* num_verti
         materials[1] = red;
         materials[2] = green;
         materials[3] = light;
+    }
+    return MI_OK;
+}
+
+namespace {
+
+Room make_cloud_scene() {
+    Room r;
+    // ground, grey, facing up
+    r.quad({-4, 0, -4}, {4, 0, -4}, {4, 0, 4}, {-4, 0, 4}, {0, 1, 0}, 0);
+    // emissive quad above the cloud, facing down: 25 x 20 cells, 1000 triangles
+    double const x0 = -3.0, x1 = 3.0, z0 = -3.0, z1 = 3.0, y = 3.5;
+    uint32_t const nx = 25, nz = 20;
+    for (uint32_t j = 0; j < nz; ++j) {
+        for (uint32_t i = 0; i < nx; ++i) {
+            double const ax = x0 + (x1 - x0) * double(i) / nx;
+            double const bx = x0 + (x1 - x0) * double(i + 1) / nx;
+            double const az = z0 + (z1 - z0) * double(j) / nz;
+            double const bz = z0 + (z1 - z0) * double(j + 1) / nz;
+            r.quad({ax, y, az}, {bx, y, az}, {bx, y, bz}, {ax, y, bz}, {0, -1, 0}, 1);
+        }
+    }
+    return r;
+}
+
+}  // namespace
+
+extern "C" int32_t mi_gen_cloud_scene(float* positions, uint32_t* num_vertices, uint32_t* indices,
+                                      uint32_t* num_triangles, uint32_t* triangle_materials,
+                                      mi_material* materials, uint32_t* num_materials) {
+    if (!num_vertices || !num_triangles || !num_materials) {
+        return mi::fail(MI_ERR_ARGS, "mi_gen_cloud_scene: bad arguments");
+    }
+    Room const r = make_cloud_scene();
+    *num_vertices  = uint32_t(r.positions.size() / 3);
+    *num_triangles = uint32_t(r.indices.size() / 3);
+    *num_materials = 2;
+    if (positions) {
+        std::memcpy(positions, r.positions.data(), r.positions.size() * sizeof(float));
+    }
+    if (indices) {
+        std::memcpy(indices, r.indices.data(), r.indices.size() * sizeof(uint32_t));
+    }
+    if (triangle_materials) {
+        std::memcpy(triangle_materials, r.materials.data(), r.materials.size() * sizeof(uint32_t));
+    }
+    if (materials) {
+        mi_material const ground = {{0.5f, 0.5f, 0.5f}, {0, 0, 0}, 1.f, 0.f, 1.46f, MI_MATERIAL_LAMBERT};
+        mi_material const sky    = {{0, 0, 0}, {4.f, 4.5f, 5.f}, 1.f, 0.f, 1.f, MI_MATERIAL_LIGHT};
+        materials[0] = ground;
+        materials[1] = sky;
+    }
+    return MI_OK;
+}
+
+extern "C" int32_t mi_gen_fbm_density(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t octaves,
+                                      uint64_t seed, float max_density, float* density) {
+    if (!density || 0 == nx || 0 == ny || 0 == nz || nx > 4096 || ny > 4096 || nz > 4096 ||
+        0 == octaves || octaves > 16 || !(max_density >= 0.f)) {
+        return mi::fail(MI_ERR_ARGS, "mi_gen_fbm_density: bad arguments");
+    }
+    auto slab = [&](uint32_t z_begin, uint32_t z_end) {
+        for (uint32_t z = z_begin; z < z_end; ++z) {
+            for (uint32_t y = 0; y < ny; ++y) {
+                for (uint32_t x = 0; x < nx; ++x) {
+                    // voxel centre in [0, 1]^3, offset to the cube centre
+                    double const u = (double(x) + 0.5) / nx - 0.5;
+                    double const v = (double(y) + 0.5) / ny - 0.5;
+                    double const w = (double(z) + 0.5) / nz - 0.5;
+                    double const n = fbm(u * 2.0 + 7.31, v * 2.0 + 3.17, w * 2.0 + 5.53, octaves, seed);
+                    // spherical envelope keeps the cloud clear of the box faces
+                    double const r2  = 4.0 * (u * u + v * v + w * w);
+                    double const env = std::max(0.0, 1.0 - r2);
+                    double const c   = std::min(1.0, std::max(0.0, (0.5 * n + 0.5) * 2.2 * env - 0.35) * 2.0);
+                    density[(size_t(z) * ny + y) * nx + x] = float(double(max_density) * c);
+                }
+            }
+        }
+    };
+    uint32_t const threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    uint32_t const per = (nz + threads - 1) / threads;
+    for (uint32_t t = 0; t < threads; ++t) {
+        uint32_t const b = std::min(nz, t * per);
+        uint32_t const e = std::min(nz, b + per);
+        if (b < e) {
+            pool.emplace_back(slab, b, e);
+        }
+    }
+    for (auto& th : pool) {
+        th.join();
     }
     return MI_OK;
 }

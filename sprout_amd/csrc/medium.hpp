@@ -75,7 +75,15 @@ __device__ __forceinline__ float saturate(float x) {
     return fminf(fmaxf(x, 0.f), 1.f);
 }
 
-// Linear_3D<clamp>::sample_1
+// Nearest_3D<clamp>::sample_1 (texture_sampler.cpp:124-172)
+__device__ inline float density_nearest(Medium_view const& m, V3 uvw) {
+    int32_t const x = min(int32_t(saturate(uvw.x) * float(m.dims[0])), m.dims[0] - 1);
+    int32_t const y = min(int32_t(saturate(uvw.y) * float(m.dims[1])), m.dims[1] - 1);
+    int32_t const z = min(int32_t(saturate(uvw.z) * float(m.dims[2])), m.dims[2] - 1);
+    return m.density[(size_t(z) * size_t(m.dims[1]) + size_t(y)) * size_t(m.dims[0]) + size_t(x)];
+}
+
+// Linear_3D<clamp>::sample_1 (texture_sampler.cpp:175-188, 256-282)
 __device__ inline float density(Medium_view const& m, V3 uvw) {
     float const u  = saturate(uvw.x) * float(m.dims[0]) - 0.5f;
     float const v  = saturate(uvw.y) * float(m.dims[1]) - 0.5f;
@@ -239,3 +247,227 @@ __device__ inline V3 sample_phase(V3 wo, float r0, float r1, float g, float& pdf
 }  // namespace dev
 
 }  // namespace mi
+
+namespace mi::dev {
+
+// Per-lane random numbers for the tracking loops (the path's generator).
+template <typename Rng>
+__device__ __forceinline__ float medium_uniform(Rng& rng) {
+    return rng.uniform();
+}
+
+__device__ __forceinline__ float average3(float const v[3]) {
+    return (v[0] + v[1] + v[2]) / 3.f;
+}
+
+// collision coefficients at uvw, mu_s scaled by srs; nearest: the filter
+// the path tracer asks for after its second vertex (Filter::Nearest,
+// pathtracer_mis.cpp:121), else the grid's linear sampler
+__device__ __forceinline__ void coefficients(Medium_view const& m, V3 uvw, float srs, bool nearest,
+                                             float a[3], float s[3]) {
+    float const dn = nearest ? density_nearest(m, uvw) : density(m, uvw);
+    for (int c = 0; c < 3; ++c) {
+        a[c] = dn * m.cc_a[c];
+        s[c] = (dn * m.cc_s[c]) * srs;
+    }
+}
+
+// Tracking::tracking (scattering, no emission): true = scattered at t_out
+template <typename Rng>
+__device__ inline bool tracking(Medium_view const& m, Tray const& r, Medium_cm const& cm, float srs,
+                                bool nearest, Rng& rng, float& t_out, float w[3]) {
+    float const mt = cm.majorant_mu_a + cm.majorant_mu_s;
+    if (mt < Min_mt) {
+        return false;
+    }
+    float       lw[3] = {w[0], w[1], w[2]};
+    float const imt   = 1.f / mt;
+    for (float t = r.min_t, d = r.max_t;;) {
+        float const r0 = medium_uniform(rng);
+        t -= logf(1.f - r0) * imt;
+        if (t > d) {
+            for (int c = 0; c < 3; ++c) {
+                w[c] = lw[c];
+            }
+            return false;
+        }
+        float mu_a[3], mu_s[3];
+        coefficients(m, r.point(t), srs, nearest, mu_a, mu_s);
+        float mu_n[3], msw[3], mnw[3];
+        for (int c = 0; c < 3; ++c) {
+            mu_n[c] = mt - (mu_a[c] + mu_s[c]);
+            msw[c]  = mu_s[c] * lw[c];
+            mnw[c]  = mu_n[c] * lw[c];
+        }
+        float const ms = average3(msw);
+        float const mn = average3(mnw);
+        float const cc = 1.f / (ms + mn);
+        float const ps = ms * cc;
+        float const pn = mn * cc;
+        float const r1 = medium_uniform(rng);
+        if ((r1 <= 1.f - pn) & (ps > 0.f)) {
+            for (int c = 0; c < 3; ++c) {
+                w[c] = lw[c] * (mu_s[c] / (mt * ps));
+            }
+            t_out = t;
+            return true;
+        }
+        for (int c = 0; c < 3; ++c) {
+            lw[c] *= mu_n[c] / (mt * pn);
+        }
+    }
+}
+
+__device__ __forceinline__ bool all_below(float const w[3], float e) {
+    return w[0] < e && w[1] < e && w[2] < e;
+}
+
+// tracking_transmitted with its residual-ratio-tracking branch
+template <typename Rng>
+__device__ inline bool tracking_transmitted(Medium_view const& m, Tray const& r,
+                                            Medium_cm const& cm, float srs, bool nearest, Rng& rng,
+                                            float w[3]) {
+    float const mt = cm.majorant_mu_a + cm.majorant_mu_s;
+    if (mt < Min_mt) {
+        return true;
+    }
+    float const minorant = cm.minorant_mu_a + cm.minorant_mu_s;
+    float       control  = 0.f;
+    float       mr       = mt;
+    if (minorant > 0.f) {
+        float const att = expf(-(r.max_t - r.min_t) * minorant);
+        for (int c = 0; c < 3; ++c) {
+            w[c] *= att;
+        }
+        if (all_below(w, Abort_epsilon)) {
+            return false;
+        }
+        mr = mt - minorant;
+        if (mr < Min_mt) {
+            return true;
+        }
+        control = minorant;
+    }
+    float const imt = 1.f / mr;
+    for (float t = r.min_t, d = r.max_t;;) {
+        float const r0 = medium_uniform(rng);
+        t -= logf(1.f - r0) * imt;
+        if (t > d) {
+            return true;
+        }
+        float mu_a[3], mu_s[3];
+        coefficients(m, r.point(t), srs, nearest, mu_a, mu_s);
+        for (int c = 0; c < 3; ++c) {
+            float const mu_t = (mu_a[c] + mu_s[c]) - control;
+            w[c] *= imt * (mr - mu_t);
+        }
+        if (all_below(w, Abort_epsilon)) {
+            return false;
+        }
+    }
+}
+
+// Texture-space ray of a world segment [t0, t1] (the part inside the box).
+__device__ inline Tray texture_ray(Medium_view const& m, V3 o, V3 d, float t0, float t1) {
+    Tray r;
+    r.o     = v3((o.x - m.box_lo[0]) * m.iext[0], (o.y - m.box_lo[1]) * m.iext[1],
+                 (o.z - m.box_lo[2]) * m.iext[2]);
+    r.d     = v3(d.x * m.iext[0], d.y * m.iext[1], d.z * m.iext[2]);
+    r.inv   = v3(reciprocal_lane(r.d.x), reciprocal_lane(r.d.y), reciprocal_lane(r.d.z));
+    r.min_t = t0;
+    r.max_t = t1;
+    return r;
+}
+
+// World-space overlap [t0, t1] of a ray with the medium box within
+// [min_t, max_t].  A ray that starts outside the box is tracked from
+// offset_f of its entry distance, as a ray continued through a medium's
+// boundary surface is (pathtracer_mis.cpp:185-186).
+__device__ inline bool box_overlap(Medium_view const& m, V3 o, V3 d, float min_t, float max_t,
+                                   float& t0, float& t1) {
+    float const lo[3] = {m.box_lo[0], m.box_lo[1], m.box_lo[2]};
+    float const hi[3] = {m.box_lo[0] + 1.f / m.iext[0], m.box_lo[1] + 1.f / m.iext[1],
+                         m.box_lo[2] + 1.f / m.iext[2]};
+    float const oo[3] = {o.x, o.y, o.z};
+    float const dd[3] = {d.x, d.y, d.z};
+    float       a     = min_t;
+    float       b     = max_t;
+    for (int k = 0; k < 3; ++k) {
+        if (0.f == dd[k]) {
+            if (oo[k] < lo[k] || oo[k] > hi[k]) {
+                return false;
+            }
+            continue;
+        }
+        float const inv = reciprocal_lane(dd[k]);
+        float       n   = (lo[k] - oo[k]) * inv;
+        float       f   = (hi[k] - oo[k]) * inv;
+        if (n > f) {
+            float const x = n;
+            n             = f;
+            f             = x;
+        }
+        // NaN (0 * inf) leaves the interval as it is
+        a = n > a ? n : a;
+        b = f < b ? f : b;
+    }
+    t0 = a > min_t ? offset_f(a) : a;
+    t1 = b;
+    return t0 < t1;
+}
+
+// Tracking_multi::integrate for one segment (tracking_multi.cpp:101-162):
+// 1 scattered at t_out, 0 passed, 2 aborted (every channel of w below
+// Abort_epsilon, scattered or not).  w multiplies the throughput.
+template <typename Rng>
+__device__ inline int track_segment(Medium_view const& m, V3 o, V3 d, float t0, float t1,
+                                    uint32_t depth, bool nearest, Rng& rng, float& t_out,
+                                    float w[3]) {
+    Tray        r     = texture_ray(m, o, d, t0, t1);
+    float const srs   = similarity_scale(m.g, depth);
+    int         event = 0;
+    w[0] = w[1] = w[2] = 1.f;
+    for (float const end = t1; r.min_t < end;) {
+        Medium_cm cm;
+        if (gridtree_intersect(m, r, cm)) {
+            cm.minorant_mu_s *= srs;
+            cm.majorant_mu_s *= srs;
+            if (tracking(m, r, cm, srs, nearest, rng, t_out, w)) {
+                event = 1;
+                break;
+            }
+        }
+        r.min_t = offset_f(r.max_t);
+        r.max_t = end;
+    }
+    return all_below(w, Abort_epsilon) ? 2 : event;
+}
+
+// Tracking::transmittance for the part of a segment inside the box:
+// false = blocked (tr below Abort_epsilon), else tr.
+template <typename Rng>
+__device__ inline bool transmittance(Medium_view const& m, V3 o, V3 d, float t0, float t1,
+                                     uint32_t depth, bool nearest, Rng& rng, float tr[3]) {
+    tr[0] = tr[1] = tr[2] = 1.f;
+    if (offset_f(t0) >= t1) {
+        return true;
+    }
+    Tray        r   = texture_ray(m, o, d, t0, t1);
+    float const srs = similarity_scale(m.g, depth);
+    for (float const end = t1; r.min_t < end;) {
+        Medium_cm cm;
+        if (gridtree_intersect(m, r, cm)) {
+            cm.minorant_mu_s *= srs;
+            cm.majorant_mu_s *= srs;
+            if (!tracking_transmitted(m, r, cm, srs, nearest, rng, tr)) {
+                tr[0] = tr[1] = tr[2] = 0.f;
+                return false;
+            }
+        }
+        r.min_t = offset_f(r.max_t);
+        r.max_t = end;
+    }
+    return true;
+}
+
+}  // namespace mi::dev

@@ -43,6 +43,7 @@
 #include "device_math.hpp"
 #include "guide.hpp"
 #include "light_tree.hpp"
+#include "medium.hpp"
 #include "render.hpp"
 #include "trace.hpp"
 
@@ -98,7 +99,19 @@ struct Rng {
     }
 };
 
-constexpr uint32_t Flag_singular = 1;  // State::Treat_as_singular
+constexpr uint32_t Flag_singular    = 1;  // State::Treat_as_singular
+constexpr uint32_t Flag_translucent = 2;  // State::Is_translucent: the vertex was in a medium
+constexpr uint32_t Flag_primary     = 4;  // State::Primary_ray: no non-specular bounce yet
+// the medium's density is looked up without filtering for the shadow rays of
+// the current vertex and the ray it spawned: Filter::Nearest once the
+// vertex's depth is above 1 and the path is no longer primary
+// (pathtracer_mis.cpp:121)
+constexpr uint32_t Flag_nearest     = 8;
+
+// medium_event values per queue entry
+constexpr uint32_t Medium_none     = 0;
+constexpr uint32_t Medium_scatter  = 1;
+constexpr uint32_t Medium_absorbed = 2;
 
 // Pathtracer_MIS::Num_dedicated_samplers: light-tree splitting below this depth
 constexpr uint32_t Num_split_bounces = 3;
@@ -127,6 +140,9 @@ struct Wave {
     dev::Light_tree_view   tree;
     uint32_t const*        tri_light;  // light of an emissive triangle
     uint32_t               split_lights;
+    dev::Medium_view       medium;
+    uint32_t*              medium_event;  // per queue entry: Medium_*
+    float*                 medium_p;      // scattering point per queue entry
     float const*           e_table;
     // camera
     float    eye[3];
@@ -241,12 +257,22 @@ __device__ __forceinline__ uint32_t queue_range(uint32_t n, uint32_t* counter) {
     return base + inc - n;
 }
 
-// The material of a hit, Lambert or Substitute (both one-sided).
+// Scattering vertex in the medium: Henyey-Greenstein phase function
+constexpr uint32_t Kind_phase = 0x100;
+
+// The material of a vertex: Lambert or Substitute (both one-sided), or the
+// medium's phase function (volumetric::Sample, volumetric_sample.cpp:15-60).
 struct Surface {
-    dev::Material_closure sub;
+    dev::Material_closure sub;  // sub.wo is the outgoing direction for every kind
     uint32_t              kind;
+    float                 g;  // phase anisotropy
 
     __device__ void evaluate(V3 wi, float out[4]) const {
+        if (Kind_phase == kind) {
+            float const ph = dev::phase_hg(dev::dot(sub.wo, wi), g);
+            out[0] = out[1] = out[2] = out[3] = ph;
+            return;
+        }
         if (MI_MATERIAL_LAMBERT == kind) {
             // lambert::Iso::reflection times n.wi, lambert.inl:15-26
             float const n_dot_wi = dev::clamp_dot(sub.layer.n, wi);
@@ -260,6 +286,14 @@ struct Surface {
     }
 
     __device__ void sample(float xl, float x0, float x1, dev::Bxdf_sample& s) const {
+        if (Kind_phase == kind) {
+            float pdf;
+            s.wi      = dev::sample_phase(sub.wo, x0, x1, g, pdf);
+            s.refl[0] = s.refl[1] = s.refl[2] = pdf;
+            s.pdf     = pdf;
+            s.type    = dev::Type_diffuse;
+            return;
+        }
         if (MI_MATERIAL_LAMBERT == kind) {
             // lambert::Iso::reflect, lambert.inl:28-45
             V3 const    is       = dev::hemisphere_cosine(x0, x1);
@@ -315,7 +349,7 @@ __global__ void __launch_bounds__(256) raygen(Wave w) {
         path.radiance[c]   = 0.f;
     }
     path.bsdf_pdf = 0.f;
-    path.flags    = Flag_singular;
+    path.flags    = Flag_singular | Flag_primary;
     path.record   = dev::Null;
     path.prev_n[0] = path.prev_n[1] = path.prev_n[2] = 0.f;
     w.paths[p]    = path;
@@ -350,6 +384,43 @@ __global__ void tally(uint32_t* counters, uint32_t in_q) {
     uint64_t* totals = reinterpret_cast<uint64_t*>(counters + C_totals);
     totals[0] += counters[C_queue0 + in_q];
     totals[1] += counters[C_shadow];
+}
+
+// Tracking_multi::integrate for every queued ray: the part of the ray before
+// its surface hit that lies in the medium box is tracked; the tracking weight
+// multiplies the path throughput, and a scattering point replaces the hit.
+__global__ void __launch_bounds__(256) medium_track(Wave w, uint32_t in_q, uint32_t depth) {
+    uint32_t const count  = w.counters[C_queue0 + in_q];
+    uint32_t const stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        mi_ray const r     = w.rays[in_q][i];
+        mi_hit const h     = w.hits[i];
+        V3 const     o     = v3(r.origin[0], r.origin[1], r.origin[2]);
+        V3 const     d     = v3(r.direction[0], r.direction[1], r.direction[2]);
+        uint32_t     event = Medium_none;
+        float        t0, t1;
+        if (dev::box_overlap(w.medium, o, d, r.min_t, h.t, t0, t1)) {
+            uint32_t const p    = w.ray_path[in_q][i];
+            Path           path = w.paths[p];
+            float          t_out = 0.f;
+            float          tw[3];
+            int const      e = dev::track_segment(w.medium, o, d, t0, t1, depth,
+                                                      0 != (path.flags & Flag_nearest), path.rng, t_out, tw);
+            for (int c = 0; c < 3; ++c) {
+                path.throughput[c] *= tw[c];
+            }
+            if (1 == e) {
+                event                  = Medium_scatter;
+                w.medium_p[3 * i + 0] = o.x + t_out * d.x;
+                w.medium_p[3 * i + 1] = o.y + t_out * d.y;
+                w.medium_p[3 * i + 2] = o.z + t_out * d.z;
+            } else if (2 == e) {
+                event = Medium_absorbed;
+            }
+            w.paths[p] = path;
+        }
+        w.medium_event[i] = event;
+    }
 }
 
 // Network inputs of every queued hit: position in the scene box, geometric
@@ -410,6 +481,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
         V3       hit_p    = v3(0.f, 0.f, 0.f);
         V3       geo_n    = v3(0.f, 0.f, 1.f);
         V3       p_o      = v3(0.f, 0.f, 0.f);
+        bool     in_medium = false;  // a scattering vertex of the medium
         Surface  s;
         s.kind = MI_MATERIAL_LAMBERT;
 
@@ -430,7 +502,46 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                 path.record = dev::Null;
             }
 
-            if (dev::Null != h.triangle) {
+            uint32_t const event = w.medium.present ? w.medium_event[i] : Medium_none;
+
+            if (Medium_scatter == event) {
+                // Event::Scatter: the vertex is the scattering point
+                // (pathtracer_mis.cpp:215-250); no emission, the same depth
+                // and roulette tests as a surface
+                mi_ray const r = w.rays[in_q][i];
+                hit_p = v3(w.medium_p[3 * size_t(i) + 0], w.medium_p[3 * size_t(i) + 1],
+                           w.medium_p[3 * size_t(i) + 2]);
+                in_medium = true;
+                live      = true;
+                if (0 != depth) {
+                    if (depth >= w.max_bounces) {
+                        live = false;
+                    } else if (depth >= w.min_bounces) {
+                        float const q = max_component(path.throughput);
+                        if (path.rng.uniform() > q) {
+                            live = false;
+                        } else {
+                            for (int k = 0; k < 3; ++k) {
+                                path.throughput[k] /= q;
+                            }
+                        }
+                    }
+                }
+                if (live) {
+                    s.kind   = Kind_phase;
+                    s.g      = w.medium.g;
+                    s.sub.wo = v3(-r.direction[0], -r.direction[1], -r.direction[2]);
+                    // offset_p(n, translucent) of a subsurface vertex: the point itself;
+                    // the tree weighs lights over the whole sphere
+                    p_o = hit_p;
+                    if (w.tree.num_nodes) {
+                        float const select = path.rng.uniform();
+                        num_picks = dev::random_light(w.tree, p_o, geo_n, true, select, split, picks);
+                    }
+                }
+            } else if (Medium_absorbed == event) {
+                // Event::Abort: the tracking weight fell below Abort_epsilon
+            } else if (dev::Null != h.triangle) {
                 mi_ray const   r   = w.rays[in_q][i];
                 uint32_t const tri = h.triangle;
 
@@ -458,7 +569,8 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                             // Mesh::pdf times Tree::pdf at the previous vertex
                             float const tree_pdf = dev::light_pdf(
                                 w.tree, v3(r.origin[0], r.origin[1], r.origin[2]),
-                                v3(path.prev_n[0], path.prev_n[1], path.prev_n[2]), false, split, l);
+                                v3(path.prev_n[0], path.prev_n[1], path.prev_n[2]),
+                                0 != (path.flags & Flag_translucent), split, l);
                             float const ls_pdf = (h.t * h.t) / (cos_l * w.tree.lights[l].area);
                             weight             = power_heuristic(path.bsdf_pdf, ls_pdf * tree_pdf);
                         }
@@ -501,6 +613,11 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             }
         }
 
+        if (live) {
+            bool const linear = depth <= 1 || 0 != (path.flags & Flag_primary);
+            path.flags        = linear ? (path.flags & ~Flag_nearest) : (path.flags | Flag_nearest);
+        }
+
         // one shadow ray per picked light (evaluate_light); a vertex's rays
         // take consecutive queue slots, reserved with one atomic per wave
         uint32_t const sbase = queue_range(num_picks, &w.counters[C_shadow]);
@@ -539,7 +656,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                     s.evaluate(wi, f);
                     float const light_pdf = sl / (cl * L.area) * pick.pdf;
                     float       dir_pdf   = f[3];
-                    if (Guided) {
+                    if (Guided && !in_medium) {
                         dev::Vmf_mixture mix;
                         mix.set(w.guide_raw + 32 * size_t(i));
                         dir_pdf = w.fraction * mix.pdf(wi) + (1.f - w.fraction) * f[3];
@@ -580,7 +697,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             // BSDF importance sample; guided: one-sample mixture of the guide
             // (probability fraction) and the BSDF
             dev::Bxdf_sample bs;
-            if (Guided) {
+            if (Guided && !in_medium) {
                 dev::Vmf_mixture mix;
                 mix.set(w.guide_raw + 32 * size_t(i));
                 float const pick = path.rng.uniform();
@@ -622,7 +739,12 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                 if (dev::Type_specular == bs.type) {
                     path.flags |= Flag_singular;
                 } else {
-                    path.flags &= ~Flag_singular;
+                    path.flags &= ~(Flag_singular | Flag_primary);
+                }
+                if (in_medium) {
+                    path.flags |= Flag_translucent;
+                } else {
+                    path.flags &= ~Flag_translucent;
                 }
                 path.bsdf_pdf = bs.pdf;
                 for (int k = 0; k < 3; ++k) {
@@ -631,7 +753,9 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                 path.prev_n[0]        = geo_n.x;
                 path.prev_n[1]        = geo_n.y;
                 path.prev_n[2]        = geo_n.z;
-                V3 const o            = offset_ray(hit_p, dev::dot(geo_n, bs.wi) > 0.f ? geo_n : -geo_n);
+                // a scattering vertex spawns from the point itself (subsurface offset_p)
+                V3 const o = in_medium ? hit_p
+                                       : offset_ray(hit_p, dev::dot(geo_n, bs.wi) > 0.f ? geo_n : -geo_n);
                 next_ray.origin[0]    = o.x;
                 next_ray.origin[1]    = o.y;
                 next_ray.origin[2]    = o.z;
@@ -678,7 +802,11 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
 
 // Adds the unoccluded light contributions of every vertex shaded this
 // bounce to its path, in the order the lights were picked.
-__global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q) {
+// With a medium, the part of each unoccluded shadow ray inside its box is
+// attenuated by ratio tracking (Worker::transmitted: visibility, then
+// transmittance, rendering_worker.cpp:143-153), drawing from the path's
+// generator after the vertex's direction sample.
+__global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q, uint32_t depth) {
     uint32_t const count  = w.counters[C_queue0 + in_q];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
@@ -686,16 +814,35 @@ __global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q) {
         if (0 == range.y) {
             continue;
         }
-        Path& p = w.paths[w.ray_path[in_q][i]];
+        Path* const p   = &w.paths[w.ray_path[in_q][i]];
+        Rng         rng = p->rng;
+        float       radiance[3] = {p->radiance[0], p->radiance[1], p->radiance[2]};
         for (uint32_t k = 0; k < range.y; ++k) {
             if (w.occluded[range.x + k]) {
                 continue;
             }
-            Shadow const s = w.shadows[range.x + k];
+            Shadow const s     = w.shadows[range.x + k];
+            float        tr[3] = {1.f, 1.f, 1.f};
+            if (w.medium.present) {
+                mi_ray const r = w.shadow_rays[range.x + k];
+                V3 const     o = v3(r.origin[0], r.origin[1], r.origin[2]);
+                V3 const     d = v3(r.direction[0], r.direction[1], r.direction[2]);
+                float        t0, t1;
+                if (r.min_t < r.max_t && dev::box_overlap(w.medium, o, d, r.min_t, r.max_t, t0, t1)) {
+                    if (!dev::transmittance(w.medium, o, d, t0, t1, depth,
+                                            0 != (p->flags & Flag_nearest), rng, tr)) {
+                        continue;
+                    }
+                }
+            }
             for (int c = 0; c < 3; ++c) {
-                p.radiance[c] += s.contrib[c];
+                radiance[c] += tr[c] * s.contrib[c];
             }
         }
+        for (int c = 0; c < 3; ++c) {
+            p->radiance[c] = radiance[c];
+        }
+        p->rng = rng;
     }
 }
 
@@ -863,6 +1010,41 @@ __global__ void bsdf_sample_batch(uint32_t n, float const* params, float const* 
     o[7] = __uint_as_float(s.type);
 }
 
+// Batch tracking along rays through a scene's medium (each ray clipped to
+// the box as the renderer clips it), one generator per ray started at
+// (seed, index): track_segment -> {event, t, w rgb}, or with
+// MI_MEDIUM_TRANSMITTANCE transmittance -> {transmitted, 0, tr rgb}.
+__global__ void medium_batch(dev::Medium_view m, mi_ray const* rays, uint32_t depth, uint64_t seed,
+                             uint32_t flags, uint32_t n, float* out) {
+    bool const transmittance = 0 != (flags & MI_MEDIUM_TRANSMITTANCE);
+    bool const nearest       = 0 != (flags & MI_MEDIUM_NEAREST);
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    mi_ray const r = rays[i];
+    V3 const     o = v3(r.origin[0], r.origin[1], r.origin[2]);
+    V3 const     d = v3(r.direction[0], r.direction[1], r.direction[2]);
+    Rng          rng;
+    rng.start(seed, i);
+    float res[5] = {0.f, 0.f, 1.f, 1.f, 1.f};
+    float t0, t1;
+    if (dev::box_overlap(m, o, d, r.min_t, r.max_t, t0, t1)) {
+        if (transmittance) {
+            res[0] = dev::transmittance(m, o, d, t0, t1, depth, nearest, rng, res + 2) ? 1.f : 0.f;
+        } else {
+            float t_out = 0.f;
+            res[0]      = float(dev::track_segment(m, o, d, t0, t1, depth, nearest, rng, t_out, res + 2));
+            res[1]      = t_out;
+        }
+    } else if (transmittance) {
+        res[0] = 1.f;
+    }
+    for (int k = 0; k < 5; ++k) {
+        out[5 * size_t(i) + k] = res[k];
+    }
+}
+
 __global__ void light_tree_sample_batch(dev::Light_tree_view t, float const* points,
                                         float const* normals, float const* randoms, uint32_t split,
                                         uint32_t n, uint32_t* picks, uint32_t* counts) {
@@ -873,7 +1055,8 @@ __global__ void light_tree_sample_batch(dev::Light_tree_view t, float const* poi
     dev::Light_pick out[dev::Max_lights];
     V3 const        p = v3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
     V3 const        nn = v3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
-    uint32_t const  c = dev::random_light(t, p, nn, false, randoms[i], 0 != split, out);
+    uint32_t const  c = dev::random_light(t, p, nn, 0 != (split & MI_LIGHT_TOTAL_SPHERE), randoms[i],
+                                          0 != (split & MI_LIGHT_SPLIT), out);
     counts[i]         = c;
     uint32_t* o       = picks + 2 * size_t(i) * dev::Max_lights;
     for (uint32_t k = 0; k < c; ++k) {
@@ -891,7 +1074,8 @@ __global__ void light_tree_pdf_batch(dev::Light_tree_view t, float const* points
     }
     V3 const p  = v3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
     V3 const nn = v3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
-    pdfs[i]     = dev::light_pdf(t, p, nn, false, 0 != split, lights[i]);
+    pdfs[i]     = dev::light_pdf(t, p, nn, 0 != (split & MI_LIGHT_TOTAL_SPHERE),
+                                 0 != (split & MI_LIGHT_SPLIT), lights[i]);
 }
 
 uint32_t blocks(uint64_t n, uint32_t b) {
@@ -975,6 +1159,22 @@ int32_t light_tree_queries(RenderScene const& scene, bool sample, float const* p
     return MI_OK;
 }
 
+int32_t medium_queries(RenderScene const& scene, mi_ray const* rays, uint32_t depth, uint64_t seed,
+                       uint32_t flags, uint32_t n, float* out, hipStream_t stream) {
+    if (!scene.medium.present) {
+        return fail(MI_ERR_ARGS, "mi_medium_track: the scene has no medium");
+    }
+    if (0 == n) {
+        return MI_OK;
+    }
+    hipLaunchKernelGGL(medium_batch, dim3(blocks(n, 128)), dim3(128), 0, stream, scene.medium, rays,
+                       depth, seed, flags, n, out);
+    if (hipError_t const e = hipGetLastError(); e) {
+        return fail(MI_ERR_HIP, "mi_medium_track: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
 int32_t download_light_tree(RenderScene const& scene, mi_light_node* nodes, uint32_t* middles,
                             uint32_t* mapping, uint32_t* orders, void* infos) {
     size_t const nn = scene.num_lights ? scene.num_light_nodes : 0;
@@ -1003,7 +1203,99 @@ void RenderScene::release() {
     (void)hipFree(light_orders);
     (void)hipFree(light_infos);
     (void)hipFree(tri_light);
+    release_medium();
     *this = RenderScene();
+}
+
+void RenderScene::release_medium() {
+    (void)hipFree(medium_density);
+    (void)hipFree(medium_nodes);
+    (void)hipFree(medium_data);
+    medium_density = nullptr;
+    medium_nodes   = nullptr;
+    medium_data    = nullptr;
+    medium         = dev::Medium_view{};
+}
+
+constexpr double Max_optical_depth = 1.0e4;
+
+int32_t set_medium(RenderScene& scene, float const* density, uint32_t const dims[3],
+                   float const box_lo[3], float const box_hi[3], float const cc_a[3],
+                   float const cc_s[3], float g) {
+    scene.release_medium();
+    if (!density) {
+        return MI_OK;
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (0 == dims[k] || dims[k] > 4096) {
+            return fail(MI_ERR_ARGS, "medium grid dimension %u out of [1, 4096]", dims[k]);
+        }
+        if (!(box_hi[k] > box_lo[k]) || !std::isfinite(box_lo[k]) || !std::isfinite(box_hi[k])) {
+            return fail(MI_ERR_ARGS, "medium box is empty or not finite");
+        }
+        if (!(cc_a[k] >= 0.f) || !(cc_s[k] >= 0.f) || !std::isfinite(cc_a[k]) ||
+            !std::isfinite(cc_s[k])) {
+            return fail(MI_ERR_ARGS, "medium coefficients must be finite and non-negative");
+        }
+    }
+    if (!(g > -1.f && g < 1.f)) {
+        return fail(MI_ERR_ARGS, "phase anisotropy g = %g outside (-1, 1)", double(g));
+    }
+    size_t const n     = size_t(dims[0]) * dims[1] * dims[2];
+    float        max_d = 0.f;
+    for (size_t i = 0; i < n; ++i) {
+        if (!(density[i] >= 0.f) || !std::isfinite(density[i])) {
+            return fail(MI_ERR_ARGS, "medium density %zu is negative or not finite", i);
+        }
+        max_d = std::max(max_d, density[i]);
+    }
+    // delta tracking takes about (majorant x length) steps per ray: bound the
+    // optical depth of the box diagonal so that every tracking loop ends soon
+    double diag2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double const e = double(box_hi[k]) - double(box_lo[k]);
+        diag2 += e * e;
+    }
+    double const mt_max = double(max_d) * (double(std::max({cc_a[0], cc_a[1], cc_a[2]})) +
+                                           double(std::max({cc_s[0], cc_s[1], cc_s[2]})));
+    if (mt_max * std::sqrt(diag2) > Max_optical_depth) {
+        return fail(MI_ERR_ARGS, "medium optical depth %g across the box exceeds %g",
+                    mt_max * std::sqrt(diag2), Max_optical_depth);
+    }
+    int32_t const idims[3] = {int32_t(dims[0]), int32_t(dims[1]), int32_t(dims[2])};
+    HostGridtree  tree;
+    build_gridtree(density, idims, cc_a, cc_s, tree);
+
+    hipError_t e = hipMalloc(&scene.medium_density, n * sizeof(float));
+    if (!e) e = hipMalloc(&scene.medium_nodes, tree.nodes.size() * sizeof(uint32_t));
+    if (!e) e = hipMalloc(&scene.medium_data, std::max<size_t>(tree.data.size(), 1) * sizeof(Medium_cm));
+    if (!e) e = hipMemcpy(scene.medium_density, density, n * sizeof(float), hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(scene.medium_nodes, tree.nodes.data(), tree.nodes.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice);
+    if (!e && !tree.data.empty()) {
+        e = hipMemcpy(scene.medium_data, tree.data.data(), tree.data.size() * sizeof(Medium_cm),
+                      hipMemcpyHostToDevice);
+    }
+    if (e) {
+        scene.release_medium();
+        return fail(MI_ERR_HIP, "medium upload: %s", hipGetErrorString(e));
+    }
+    dev::Medium_view& m = scene.medium;
+    m.density           = scene.medium_density;
+    for (int k = 0; k < 3; ++k) {
+        m.dims[k]      = idims[k];
+        m.inv_dims[k]  = 1.f / float(idims[k]);
+        m.num_cells[k] = uint32_t(tree.num_cells[k]);
+        m.box_lo[k]    = box_lo[k];
+        m.iext[k]      = 1.f / (box_hi[k] - box_lo[k]);
+        m.cc_a[k]      = cc_a[k];
+        m.cc_s[k]      = cc_s[k];
+    }
+    m.nodes   = scene.medium_nodes;
+    m.data    = scene.medium_data;
+    m.g       = g;
+    m.present = 1;
+    return MI_OK;
 }
 
 int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t const* indices,
@@ -1163,6 +1455,8 @@ void RenderState::release() {
     (void)hipFree(shadows);
     (void)hipFree(occluded);
     (void)hipFree(shadow_range);
+    (void)hipFree(medium_event);
+    (void)hipFree(medium_p);
     (void)hipFree(counters);
     (void)hipFree(guide_x);
     (void)hipFree(guide_raw);
@@ -1221,6 +1515,8 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
     (void)hipFree(ray_path[1]);
     (void)hipFree(hits);
     (void)hipFree(shadow_range);
+    (void)hipFree(medium_event);
+    (void)hipFree(medium_p);
     capacity     = 0;
     hipError_t e = hipMalloc(&paths, size_t(n) * sizeof(Path));
     for (int q = 0; q < 2 && !e; ++q) {
@@ -1229,6 +1525,8 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
     }
     if (!e) e = hipMalloc(&hits, size_t(n) * sizeof(mi_hit));
     if (!e) e = hipMalloc(&shadow_range, size_t(n) * sizeof(uint2));
+    if (!e) e = hipMalloc(&medium_event, size_t(n) * sizeof(uint32_t));
+    if (!e) e = hipMalloc(&medium_p, size_t(n) * 3 * sizeof(float));
     if (e) {
         release();
         return fail(MI_ERR_HIP, "render state of %u paths: %s", n, hipGetErrorString(e));
@@ -1360,6 +1658,9 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.tree.num_nodes = scene.num_lights ? scene.num_light_nodes : 0;
     w.tri_light      = scene.tri_light;
     w.split_lights   = split_lights ? 1u : 0u;
+    w.medium         = scene.medium;
+    w.medium_event   = state.medium_event;
+    w.medium_p       = state.medium_p;
     w.e_table      = e_table;
     for (int k = 0; k < 3; ++k) {
         w.eye[k]   = cam.eye[k];
@@ -1420,6 +1721,9 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
                 st) {
                 return st;
             }
+            if (scene.medium.present) {
+                hipLaunchKernelGGL(medium_track, dim3(grid_path), dim3(256), 0, s, w, q, d);
+            }
             hipLaunchKernelGGL(reset_queues, dim3(1), dim3(1), 0, s, state.counters, q ^ 1);
             if (guided) {
                 hipLaunchKernelGGL(guide_inputs, dim3(grid_path), dim3(256), 0, s, w, q);
@@ -1440,7 +1744,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
                 st) {
                 return st;
             }
-            hipLaunchKernelGGL(shadow_resolve, dim3(grid_path), dim3(256), 0, s, w, q);
+            hipLaunchKernelGGL(shadow_resolve, dim3(grid_path), dim3(256), 0, s, w, q, d);
             q ^= 1;
         }
         hipLaunchKernelGGL(resolve, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, w, spw, accum);

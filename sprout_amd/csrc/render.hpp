@@ -4,6 +4,7 @@
 #include "common.hpp"
 #include "guide.hpp"
 #include "light_tree.hpp"
+#include "medium.hpp"
 #include "trace.hpp"
 
 #include <hip/hip_runtime.h>
@@ -35,6 +36,11 @@ struct RenderScene {
     uint32_t*        light_orders  = nullptr;
     Light_info*      light_infos   = nullptr;
     uint32_t*        tri_light     = nullptr;  // light of each triangle, or Null
+    // participating medium in an axis-aligned box (none while density is null)
+    float*           medium_density = nullptr;  // nx * ny * nz, x fastest
+    uint32_t*        medium_nodes   = nullptr;  // majorant gridtree
+    Medium_cm*       medium_data    = nullptr;
+    dev::Medium_view medium{};
 
     uint32_t num_vertices  = 0;
     uint32_t num_triangles = 0;
@@ -45,7 +51,16 @@ struct RenderScene {
     float    box_hi[3]     = {0.f, 0.f, 0.f};
 
     void release();
+    void release_medium();
 };
+
+// Density grid (nx * ny * nz floats, x fastest, host memory) filling the box
+// [box_lo, box_hi]; cc_a / cc_s: absorption and scattering coefficients per
+// unit density (rgb); g: Henyey-Greenstein anisotropy.  Replaces any medium
+// the scene had; density == nullptr removes it.
+int32_t set_medium(RenderScene& scene, float const* density, uint32_t const dims[3],
+                   float const box_lo[3], float const box_hi[3], float const cc_a[3],
+                   float const cc_s[3], float g);
 
 // Buffers of one render, kept between calls and grown on demand.
 struct RenderState {
@@ -58,6 +73,8 @@ struct RenderState {
     void*        shadows     = nullptr;
     uint32_t*    occluded    = nullptr;
     uint2*       shadow_range = nullptr;
+    uint32_t*    medium_event = nullptr;  // per queue entry
+    float*       medium_p     = nullptr;  // 3 per queue entry
     uint32_t*    counters    = nullptr;  // queue counters, wave base, ray totals
     hipStream_t  stream      = nullptr;  // internal stream the bounce loop runs (and is captured) on
     hipEvent_t   begin       = nullptr;
@@ -98,6 +115,10 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
 int32_t light_tree_queries(RenderScene const& scene, bool sample, float const* points,
                            float const* normals, void const* arg, uint32_t split, uint32_t n,
                            void* out0, uint32_t* out1, hipStream_t stream);
+
+// Batch medium tracking on device rays (flags: MI_MEDIUM_*); out n x 5 floats.
+int32_t medium_queries(RenderScene const& scene, mi_ray const* rays, uint32_t depth, uint64_t seed,
+                       uint32_t flags, uint32_t n, float* out, hipStream_t stream);
 
 int32_t download_light_tree(RenderScene const& scene, mi_light_node* nodes, uint32_t* middles,
                             uint32_t* mapping, uint32_t* orders, void* infos);

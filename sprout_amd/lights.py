@@ -65,8 +65,9 @@ def scene_tree(scene):
         lights[:nl.value]
 
 
-def sample(scene, points, normals, randoms, split: bool):
-    """Device batch Tree::random_light -> (picks (n, 64) light ids, pdfs (n, 64), counts)."""
+def sample(scene, points, normals, randoms, split: bool, total_sphere: bool = False):
+    """Device batch Tree::random_light -> (picks (n, 64) light ids, pdfs (n, 64), counts).
+    total_sphere: the points lie in a medium (no horizon test)."""
     import torch
 
     p = torch.from_numpy(np.ascontiguousarray(points, np.float32)).cuda()
@@ -78,7 +79,8 @@ def sample(scene, points, normals, randoms, split: bool):
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     check(_lib.load().mi_light_tree_sample(scene.handle, ctypes.c_void_p(p.data_ptr()),
                                            ctypes.c_void_p(nv.data_ptr()),
-                                           ctypes.c_void_p(r.data_ptr()), 1 if split else 0, n,
+                                           ctypes.c_void_p(r.data_ptr()),
+                                           _flags(split, total_sphere), n,
                                            ctypes.c_void_p(picks.data_ptr()),
                                            ctypes.c_void_p(counts.data_ptr()), stream),
           "mi_light_tree_sample")
@@ -87,7 +89,11 @@ def sample(scene, points, normals, randoms, split: bool):
         counts.cpu().numpy().astype(np.int64)
 
 
-def pdf(scene, points, normals, lights, split: bool) -> np.ndarray:
+def _flags(split: bool, total_sphere: bool) -> int:
+    return (1 if split else 0) | (2 if total_sphere else 0)  # MI_LIGHT_SPLIT, MI_LIGHT_TOTAL_SPHERE
+
+
+def pdf(scene, points, normals, lights, split: bool, total_sphere: bool = False) -> np.ndarray:
     import torch
 
     p = torch.from_numpy(np.ascontiguousarray(points, np.float32)).cuda()
@@ -98,7 +104,8 @@ def pdf(scene, points, normals, lights, split: bool) -> np.ndarray:
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     check(_lib.load().mi_light_tree_pdf(scene.handle, ctypes.c_void_p(p.data_ptr()),
                                         ctypes.c_void_p(nv.data_ptr()),
-                                        ctypes.c_void_p(li.data_ptr()), 1 if split else 0, n,
+                                        ctypes.c_void_p(li.data_ptr()),
+                                        _flags(split, total_sphere), n,
                                         ctypes.c_void_p(out.data_ptr()), stream),
           "mi_light_tree_pdf")
     return out.cpu().numpy()

@@ -93,3 +93,42 @@ def room(slit: float = 0.0):
                           ctypes.byref(nt), tmat.ctypes.data, mats.ctypes.data, ctypes.byref(nm)),
           "mi_gen_room")
     return pos, idx, tmat, mats
+
+
+# camera of the cloud scene: looking at the medium box from the front
+CLOUD_CAMERA = {"eye": (0.0, 1.4, 4.6), "target": (0.0, 1.2, 0.0), "up": (0.0, 1.0, 0.0),
+                "fov_deg": 40.0}
+
+# the medium box of the cloud scene and its coefficients per unit density:
+# scattering albedo 0.9, Henyey-Greenstein g 0.6
+CLOUD_BOX = ((-1.0, 0.2, -1.0), (1.0, 2.2, 1.0))
+CLOUD_CC_A = (0.1, 0.1, 0.1)
+CLOUD_CC_S = (0.9, 0.9, 0.9)
+CLOUD_G = 0.6
+
+
+def cloud_scene():
+    """Ground quad and a 1000-triangle emissive quad above it (1002
+    triangles): positions, indices, per-triangle material, materials
+    (0 ground Lambert 0.5, 1 light 4 / 4.5 / 5)."""
+    lib = _lib.load()
+    nv, nt, nm = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0)
+    check(lib.mi_gen_cloud_scene(None, ctypes.byref(nv), None, ctypes.byref(nt), None, None,
+                                 ctypes.byref(nm)), "mi_gen_cloud_scene")
+    pos = np.zeros((nv.value, 3), np.float32)
+    idx = np.zeros((nt.value, 3), np.uint32)
+    tmat = np.zeros(nt.value, np.uint32)
+    mats = np.zeros(nm.value, MATERIAL_DTYPE)
+    check(lib.mi_gen_cloud_scene(pos.ctypes.data, ctypes.byref(nv), idx.ctypes.data,
+                                 ctypes.byref(nt), tmat.ctypes.data, mats.ctypes.data,
+                                 ctypes.byref(nm)), "mi_gen_cloud_scene")
+    return pos, idx, tmat, mats
+
+
+def fbm_density(n: int = 256, octaves: int = 6, seed: int = 1, max_density: float = 20.0):
+    """(n, n, n) float32 density grid (z, y, x; x fastest) of seeded fBm in a
+    spherical envelope, in [0, max_density]."""
+    out = np.zeros((n, n, n), np.float32)
+    check(_lib.load().mi_gen_fbm_density(n, n, n, octaves, seed, max_density, out.ctypes.data),
+          "mi_gen_fbm_density")
+    return out

@@ -51,11 +51,20 @@ def test_scene_tree_matches_oracle_lights(gpu, lit):
     assert np.array_equal(orders, ref.orders)
 
 
+@pytest.fixture(scope="module")
+def bright(gpu):
+    # lights bright enough that the variance test splits nodes near them
+    pos, idx, tmat, mats = room_with_lights(radiance_scale=2000.0)
+    sc = gpu.Scene(pos, idx, tmat, mats)
+    yield sc, (pos, idx, tmat, mats)
+    sc.release()
+
+
 @pytest.mark.parametrize("split", [False, True])
-def test_device_sampling_matches_oracle(gpu, lit, split):
+def test_device_sampling_matches_oracle(gpu, bright, split):
     from sprout_amd import lights
 
-    sc, scene = lit
+    sc, scene = bright
     tree = light_tree(*scene)
     rng = np.random.default_rng(4)
     m = 2000
