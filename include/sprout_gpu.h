@@ -54,6 +54,17 @@ typedef struct mi_hit {
     uint32_t triangle;
 } mi_hit;
 
+/* Closest hit of an instanced scene: t, barycentrics u / v and the
+ * triangle (input index in its mesh) as mi_hit, plus the instance. */
+typedef struct mi_instance_hit {
+    float    t;
+    float    u;
+    float    v;
+    uint32_t triangle;
+    uint32_t instance; /* MI_NULL on a miss */
+    uint32_t pad[3];
+} mi_instance_hit;
+
 /* SAH builder settings; sprout meshes use {16, 64, 4}
  * (triangle_mesh_provider.cpp:377, Builder_SAH(num_slices, sweep_threshold,
  * max_primitives)). */
@@ -246,6 +257,38 @@ int32_t mi_scene_release(uint32_t scene);
  * boundary of Driver::render_frame_forward (rendering_driver.cpp:295-360). */
 int32_t mi_render(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
                   double* accum, void* stream, mi_render_stats* stats);
+
+/* ---- instanced meshes (sprout's props) ---- */
+
+/* A top-level tree over num_instances instances of mesh trees: instance i
+ * places the tree meshes[i] (a handle of mi_bvh_build / mi_bvh_upload; the
+ * same mesh may be placed many times) with the row-vector 4x4 matrices of
+ * sprout's Composed_transformation, world_to_object and object_to_world (16
+ * floats each, rows r0..r3, p' = x r0 + y r1 + z r2 + r3; HOST).  Its world
+ * box is the mesh's box transformed as AABB::transform does (aabb.inl:263-280);
+ * the top-level tree is the SBVH scene::bvh::Builder builds over those boxes
+ * (scene_bvh_builder.cpp:14-107, settings as mi_bvh_build's).  The mesh
+ * trees must outlive it.  Mirrors su_create_prop + su_entity_set_transformation
+ * (capi/capi.cpp:521-541) and the prop tree of Scene::compile (scene.cpp:235). */
+int32_t mi_instances_create(uint32_t const* meshes, float const* world_to_object,
+                            float const* object_to_world, uint32_t num_instances,
+                            mi_bvh_settings const* settings, void* stream, uint32_t* instances);
+int32_t mi_instances_release(uint32_t instances);
+
+/* Top-level then mesh traversal of DEVICE rays (world space), as
+ * BVH_wrapper::intersect / intersect_p (prop_bvh_wrapper.cpp:33-106,
+ * 183-247) over Prop::intersect (prop.cpp:68-126, the prop box first) and
+ * Mesh::intersect (triangle_mesh.cpp:366-446, the ray in object space,
+ * same t).  hits: n mi_instance_hit; occluded: n uint32 (1 = hit). */
+int32_t mi_trace_instances_closest(uint32_t instances, mi_ray const* rays, uint32_t num_rays,
+                                   mi_instance_hit* hits, void* stream);
+int32_t mi_trace_instances_any(uint32_t instances, mi_ray const* rays, uint32_t num_rays,
+                               uint32_t* occluded, void* stream);
+/* HOST: the top-level tree (nodes: 32-byte sprout nodes, slots: instance per
+ * slot) and each instance's world box (6 floats: lo, hi).  NULL arrays query
+ * *num_nodes / *num_slots. */
+int32_t mi_instances_download(uint32_t instances, void* nodes, uint32_t* slots, float* boxes,
+                              uint32_t* num_nodes, uint32_t* num_slots);
 
 /* ---- light tree ---- */
 

@@ -69,6 +69,43 @@ int32_t oracle_trace(onode const* nodes, uint32_t const* slots, float const* pos
 void oracle_brute_force(float const* positions, uint32_t const* indices, uint32_t num_triangles,
                         float const* rays, uint64_t num_rays, int32_t rcp_mode, uint32_t* hits);
 
+/* triangle::intersect (Möller-Trumbore as the reference compiles it):
+ * 1 and *max_t, *u, *v updated on a hit in [min_t, *max_t] */
+int32_t oracle_triangle(float const* pa, float const* pb, float const* pc, float const o[3],
+                        float const d[3], float min_t, float* max_t, float* u, float* v,
+                        int32_t rcp_mode);
+
+/* ---- instanced meshes (instances.c) ---- */
+
+typedef struct {
+    oracle_bvh const* bvh;
+    float const*      positions;
+    uint32_t const*   indices;
+} oracle_mesh;
+
+typedef struct {
+    oracle_bvh         top;   /* over the instances' world boxes */
+    float*             boxes; /* 6 per instance: lo, hi */
+    float const*       w2o;   /* 16 per instance */
+    uint32_t const*    mesh_of;
+    oracle_mesh const* meshes;
+    uint32_t           num_instances;
+} oracle_instances;
+
+void    oracle_transform_point(float const m[16], float const v[3], float out[3]);
+void    oracle_transform_vector(float const m[16], float const v[3], float out[3]);
+void    oracle_transform_box(float const lo[3], float const hi[3], float const m[16],
+                             float out_lo[3], float out_hi[3]);
+int32_t oracle_instances_build(oracle_mesh const* meshes, uint32_t num_meshes,
+                               uint32_t const* mesh_of, float const* world_to_object,
+                               float const* object_to_world, uint32_t num_instances,
+                               oracle_instances* out);
+void    oracle_instances_free(oracle_instances* t);
+/* rays 8 floats each; any: 1 word per ray; closest: 5 words {t, u, v,
+ * triangle, instance} (pinned reciprocal) */
+int32_t oracle_instances_trace(oracle_instances const* t, float const* rays, uint64_t num_rays,
+                               int32_t any, uint32_t* hits);
+
 /* The reciprocals as defined above, exposed for tests. */
 float oracle_reciprocal(float x, int32_t rcp_mode);
 void  oracle_reciprocal3(float const d[3], int32_t rcp_mode, float out[3]);

@@ -163,6 +163,20 @@ def lib():
         _lib.oracle_offset_f.argtypes = [ctypes.c_float]
         _lib.oracle_similarity_scale.restype = ctypes.c_float
         _lib.oracle_similarity_scale.argtypes = [ctypes.c_float, ctypes.c_uint32]
+        for name in ("oracle_transform_point", "oracle_transform_vector"):
+            getattr(_lib, name).restype = None
+            getattr(_lib, name).argtypes = [ctypes.c_void_p] * 3
+        _lib.oracle_transform_box.restype = None
+        _lib.oracle_transform_box.argtypes = [ctypes.c_void_p] * 5
+        _lib.oracle_instances_build.restype = ctypes.c_int32
+        _lib.oracle_instances_build.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                ctypes.c_void_p]
+        _lib.oracle_instances_free.restype = None
+        _lib.oracle_instances_free.argtypes = [ctypes.c_void_p]
+        _lib.oracle_instances_trace.restype = ctypes.c_int32
+        _lib.oracle_instances_trace.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                                ctypes.c_int32, ctypes.c_void_p]
         _lib.oracle_light_random.restype = ctypes.c_uint32
         _lib.oracle_light_random.argtypes = [ctypes.POINTER(_OracleLightTree), ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_int32, ctypes.c_float,
@@ -390,6 +404,126 @@ class RefTree:
         secs = self._lib.sref_trace(self._h, _ptr(rays), n, int(any_hit), _ptr(hits),
                                     threads or self.threads)
         return hits, secs
+
+
+def ref_prop_lib(arch: str = "native"):
+    """The reference's prop-tree builder and transforms (oracle/_ref), or None."""
+    path = ref_lib_path(arch)
+    if not os.path.exists(path):
+        return None
+    l = ctypes.CDLL(path)
+    if not hasattr(l, "sref_build_props"):
+        return None
+    for name in ("sref_transform_point", "sref_transform_vector"):
+        getattr(l, name).argtypes = [ctypes.c_void_p] * 3
+    l.sref_transform_box.argtypes = [ctypes.c_void_p] * 4
+    l.sref_build_props.restype = ctypes.c_int32
+    l.sref_build_props.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    return l
+
+
+def ref_build_props(l, boxes, threads=4):
+    """scene::bvh::Builder over prop boxes (n, 6) -> (nodes, prop per slot)."""
+    b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+    nn, ni = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    l.sref_build_props(_ptr(b), len(b), threads, None, None, ctypes.byref(nn), ctypes.byref(ni))
+    nodes = np.zeros(nn.value, NODE_DTYPE)
+    idx = np.zeros(ni.value, np.uint32)
+    l.sref_build_props(_ptr(b), len(b), threads, _ptr(nodes), _ptr(idx), ctypes.byref(nn),
+                       ctypes.byref(ni))
+    return nodes, idx
+
+
+# ---------------------------------------------------------------------------
+# Instanced meshes (instances.c)
+
+
+class _OracleMesh(ctypes.Structure):
+    _fields_ = [("bvh", ctypes.POINTER(_OracleBvh)), ("positions", ctypes.c_void_p),
+                ("indices", ctypes.c_void_p)]
+
+
+class _OracleInstances(ctypes.Structure):
+    _fields_ = [("top", _OracleBvh), ("boxes", ctypes.c_void_p), ("w2o", ctypes.c_void_p),
+                ("mesh_of", ctypes.c_void_p), ("meshes", ctypes.POINTER(_OracleMesh)),
+                ("num_instances", ctypes.c_uint32)]
+
+
+def _mat(m):
+    return np.ascontiguousarray(m, np.float32).reshape(16)
+
+
+def transform_point(m, v):
+    out = np.zeros(3, np.float32)
+    lib().oracle_transform_point(_ptr(_mat(m)), _ptr(np.ascontiguousarray(v, np.float32)), _ptr(out))
+    return out
+
+
+def transform_vector(m, v):
+    out = np.zeros(3, np.float32)
+    lib().oracle_transform_vector(_ptr(_mat(m)), _ptr(np.ascontiguousarray(v, np.float32)), _ptr(out))
+    return out
+
+
+def transform_box(lo, hi, m):
+    out = np.zeros(6, np.float32)
+    lib().oracle_transform_box(_ptr(np.ascontiguousarray(lo, np.float32)),
+                               _ptr(np.ascontiguousarray(hi, np.float32)), _ptr(_mat(m)),
+                               _ptr(out[:3]), _ptr(out[3:]))
+    return out
+
+
+class Instances:
+    """Meshes (positions, indices, Tree) placed by instances with row-vector
+    4x4 matrices; the top-level tree is built by the oracle's SBVH over the
+    instances' world boxes (instances.c)."""
+
+    def __init__(self, meshes, mesh_of, world_to_object, object_to_world):
+        self._keep = []
+        self._bvhs = (_OracleBvh * len(meshes))()
+        self._meshes = (_OracleMesh * len(meshes))()
+        for k, (pos, idx, tree) in enumerate(meshes):
+            pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+            idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+            self._keep += [pos, idx, tree.nodes, tree.slots]
+            self._bvhs[k] = _OracleBvh(_ptr(tree.nodes), _ptr(tree.slots), len(tree.nodes),
+                                       len(tree.slots), 0)
+            self._meshes[k] = _OracleMesh(ctypes.pointer(self._bvhs[k]), _ptr(pos), _ptr(idx))
+        self.mesh_of = np.ascontiguousarray(mesh_of, np.uint32)
+        self.w2o = np.ascontiguousarray(world_to_object, np.float32).reshape(-1, 16)
+        self.o2w = np.ascontiguousarray(object_to_world, np.float32).reshape(-1, 16)
+        self.c = _OracleInstances()
+        st = lib().oracle_instances_build(self._meshes, len(meshes), _ptr(self.mesh_of),
+                                          _ptr(self.w2o), _ptr(self.o2w), len(self.mesh_of),
+                                          ctypes.byref(self.c))
+        if st:
+            raise RuntimeError(f"oracle_instances_build: status {st}")
+        n = len(self.mesh_of)
+        self.boxes = np.ctypeslib.as_array(ctypes.cast(self.c.boxes, ctypes.POINTER(ctypes.c_float)),
+                                           (n * 6,)).copy().reshape(n, 6)
+        top = self.c.top
+        self.top_nodes = np.ctypeslib.as_array(
+            ctypes.cast(top.nodes, ctypes.POINTER(ctypes.c_uint8)), (top.num_nodes * 32,)
+        ).view(NODE_DTYPE).copy()
+        self.top_slots = np.ctypeslib.as_array(
+            ctypes.cast(top.slots, ctypes.POINTER(ctypes.c_uint32)), (top.num_slots,)).copy()
+
+    def __del__(self):
+        try:
+            lib().oracle_instances_free(ctypes.byref(self.c))
+        except Exception:
+            pass
+
+    def trace(self, rays, any_hit=False):
+        """closest: (n, 5) uint32 words t, u, v, triangle, instance; any: (n,)"""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((len(r),) if any_hit else (len(r), 5), np.uint32)
+        st = lib().oracle_instances_trace(ctypes.byref(self.c), _ptr(r), len(r), int(any_hit),
+                                          _ptr(hits))
+        if st:
+            raise RuntimeError(f"oracle_instances_trace: status {st}")
+        return hits
 
 
 def nodes_equal(a: np.ndarray, b: np.ndarray) -> bool:

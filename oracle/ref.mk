@@ -28,6 +28,7 @@ CXXFLAGS := -std=c++20 -O3 -DNDEBUG -march=$(ARCH) -fno-rtti -fno-exceptions -fP
 
 REF_SOURCES := \
   $(SRC)/core/scene/bvh/scene_bvh_builder_base.cpp \
+  $(SRC)/core/scene/bvh/scene_bvh_builder.cpp \
   $(SRC)/core/scene/shape/triangle/bvh/triangle_bvh_builder_sah.cpp \
   $(SRC)/core/scene/shape/triangle/bvh/triangle_bvh_helper.cpp \
   $(SRC)/core/scene/shape/triangle/bvh/triangle_bvh_tree.cpp \

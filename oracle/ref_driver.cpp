@@ -15,7 +15,11 @@
 #include "scene/shape/triangle/bvh/triangle_bvh_tree.hpp"
 #undef private
 
+#include "base/math/aabb.inl"
+#include "base/math/matrix4x4.inl"
 #include "base/math/vector3.inl"
+#include "scene/bvh/scene_bvh_builder.hpp"
+#include "scene/bvh/scene_bvh_tree.inl"
 #include "base/thread/thread_pool.hpp"
 #include "scene/bvh/scene_bvh_node.inl"
 #include "scene/shape/node_stack.inl"
@@ -173,6 +177,62 @@ double sref_trace(void* handle, float const* rays, uint64_t num_rays, int32_t an
         0, int32_t(num_rays));
 
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+}
+
+// The reference's math::transform_point / transform_vector (matrix4x4.inl:
+// 319-342) and AABB::transform (aabb.inl:263-280) as the reference build
+// compiles them.  m: 16 floats, row-vector rows r0..r3.
+void sref_transform_point(float const* m, float const* v, float* out) {
+    math::Simd4x4f const mm(*reinterpret_cast<float4x4 const*>(m));
+    float3 const         r(math::transform_point(mm, Simdf(_mm_setr_ps(v[0], v[1], v[2], 0.f))));
+    out[0] = r[0];
+    out[1] = r[1];
+    out[2] = r[2];
+}
+
+void sref_transform_vector(float const* m, float const* v, float* out) {
+    math::Simd4x4f const mm(*reinterpret_cast<float4x4 const*>(m));
+    float3 const         r(math::transform_vector(mm, Simdf(_mm_setr_ps(v[0], v[1], v[2], 0.f))));
+    out[0] = r[0];
+    out[1] = r[1];
+    out[2] = r[2];
+}
+
+void sref_transform_box(float const* m, float const* lo, float const* hi, float* out) {
+    AABB const b(float3(lo[0], lo[1], lo[2]), float3(hi[0], hi[1], hi[2]));
+    AABB const t = b.transform(*reinterpret_cast<float4x4 const*>(m));
+    for (int k = 0; k < 3; ++k) {
+        out[k]     = t.bounds[0][k];
+        out[3 + k] = t.bounds[1][k];
+    }
+}
+
+// scene::bvh::Builder (scene_bvh_builder.cpp:14-107) over n prop boxes (6
+// floats each: lo, hi), as Scene::compile builds the prop tree
+// (scene.cpp:235).  nodes (32 B each) / indices may be null to query the
+// counts.
+int32_t sref_build_props(float const* boxes, uint32_t n, uint32_t num_threads, void* nodes,
+                         uint32_t* indices, uint32_t* num_nodes, uint32_t* num_indices) {
+    std::vector<uint32_t> ids(n);
+    std::vector<AABB>     aabbs(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        ids[i]   = i;
+        aabbs[i] = AABB(float3(boxes[6 * i + 0], boxes[6 * i + 1], boxes[6 * i + 2]),
+                        float3(boxes[6 * i + 3], boxes[6 * i + 4], boxes[6 * i + 5]));
+    }
+    thread::Pool        threads(num_threads ? num_threads : 1);
+    scene::bvh::Builder builder;
+    scene::bvh::Tree    tree;
+    builder.build(tree, ids, aabbs, threads);
+    *num_nodes   = tree.num_nodes_;
+    *num_indices = tree.num_indices_;
+    if (nodes) {
+        std::memcpy(nodes, tree.nodes_, size_t(tree.num_nodes_) * 32);
+    }
+    if (indices) {
+        std::memcpy(indices, tree.indices_, size_t(tree.num_indices_) * 4);
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -130,6 +130,12 @@ static int triangle(float const* pa, float const* pb, float const* pc, float con
     return 0;
 }
 
+int32_t oracle_triangle(float const* pa, float const* pb, float const* pc, float const o[3],
+                        float const d[3], float min_t, float* max_t, float* u, float* v,
+                        int32_t rcp_mode) {
+    return triangle(pa, pb, pc, o, d, min_t, max_t, u, v, rcp_mode);
+}
+
 typedef struct {
     onode const*    nodes;
     uint32_t const* slots;

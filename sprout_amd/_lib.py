@@ -135,6 +135,11 @@ SIGNATURES = {
     "mi_light_tree_sample": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp]),
     "mi_light_tree_pdf": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp]),
     "mi_scene_light_tree": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _pu32, _pu32]),
+    "mi_instances_create": (_i32, [_vp, _vp, _vp, _u32, ctypes.POINTER(MiBvhSettings), _vp, _pu32]),
+    "mi_instances_release": (_i32, [_u32]),
+    "mi_trace_instances_closest": (_i32, [_u32, _vp, _u32, _vp, _vp]),
+    "mi_trace_instances_any": (_i32, [_u32, _vp, _u32, _vp, _vp]),
+    "mi_instances_download": (_i32, [_u32, _vp, _vp, _vp, _pu32, _pu32]),
     "mi_scene_set_medium": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp, _f32]),
     "mi_medium_gridtree": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _pu32, _vp, _pu32]),
     "mi_medium_track": (_i32, [_u32, _vp, _u32, ctypes.c_uint64, _u32, _u32, _vp, _vp]),

@@ -3,12 +3,14 @@
 #include "bvh_build.hpp"
 #include "common.hpp"
 #include "guide.hpp"
+#include "instances.hpp"
 #include "light_tree.hpp"
 #include "render.hpp"
 #include "trace.hpp"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -26,6 +28,7 @@ struct Context {
     std::vector<DeviceBvh*>   bvhs;    // handle = index
     std::vector<RenderScene*> scenes;  // handle = index
     std::vector<GuideNet*>    guides;  // handle = index
+    std::vector<Tlas*>        tlases;  // handle = index
     TraceResources            render_trace;  // the renderer's own ray counter and spill area
     RenderState               render_state;
     float*                    e_table = nullptr;  // GGX energy table, made on first use
@@ -61,6 +64,13 @@ RenderScene* lookup_scene(uint32_t handle) {
         return nullptr;
     }
     return context->scenes[handle];
+}
+
+Tlas* lookup_tlas(uint32_t handle) {
+    if (!context || handle >= context->tlases.size()) {
+        return nullptr;
+    }
+    return context->tlases[handle];
 }
 
 GuideNet* lookup_guide(uint32_t handle) {
@@ -158,6 +168,12 @@ int32_t mi_release(void) {
             delete g;
         }
     }
+    for (Tlas* t : context->tlases) {
+        if (t) {
+            t->release();
+            delete t;
+        }
+    }
     context->render_state.release();
     context->render_trace.release();
     (void)hipFree(context->e_table);
@@ -227,6 +243,12 @@ int32_t mi_bvh_release(uint32_t bvh) {
     if (!b) {
         return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT, "mi_bvh_release: bad handle %u",
                     bvh);
+    }
+    for (uint32_t i = 0; i < context->tlases.size(); ++i) {
+        Tlas const* t = context->tlases[i];
+        if (t && std::find(t->mesh_handles.begin(), t->mesh_handles.end(), bvh) != t->mesh_handles.end()) {
+            return fail(MI_ERR_ARGS, "mi_bvh_release: bvh %u is placed by instances %u", bvh, i);
+        }
     }
     (void)hipDeviceSynchronize();
     b->release();
@@ -661,6 +683,130 @@ int32_t mi_light_tree_pdf(uint32_t scene, float const* points, float const* norm
     }
     return light_tree_queries(*sc, false, points, normals, lights, split, n, pdfs, nullptr,
                               as_stream(stream));
+}
+
+int32_t mi_instances_create(uint32_t const* meshes, float const* world_to_object,
+                            float const* object_to_world, uint32_t num_instances,
+                            mi_bvh_settings const* settings, void* stream, uint32_t* instances) {
+    clear_error();
+    if (!context) {
+        return fail(MI_ERR_NO_CONTEXT, "mi_instances_create: call mi_init first");
+    }
+    if (!meshes || !world_to_object || !object_to_world || !instances || 0 == num_instances) {
+        return fail(MI_ERR_ARGS, "mi_instances_create: bad arguments");
+    }
+    mi_bvh_settings const st = settings ? *settings : mi_bvh_settings{16, 64, 4};
+    // distinct mesh trees, in order of first use
+    std::vector<uint32_t>         handles;
+    std::vector<DeviceBvh const*> trees;
+    std::vector<uint32_t>         mesh_of(num_instances);
+    for (uint32_t i = 0; i < num_instances; ++i) {
+        DeviceBvh const* b = lookup(meshes[i]);
+        if (!b) {
+            return fail(MI_ERR_HANDLE, "mi_instances_create: instance %u: bad bvh handle %u", i,
+                        meshes[i]);
+        }
+        auto const it = std::find(handles.begin(), handles.end(), meshes[i]);
+        if (it == handles.end()) {
+            mesh_of[i] = uint32_t(handles.size());
+            handles.push_back(meshes[i]);
+            trees.push_back(b);
+        } else {
+            mesh_of[i] = uint32_t(it - handles.begin());
+        }
+    }
+    auto* t = new Tlas();
+    if (int32_t const r = create_tlas(trees.data(), uint32_t(trees.size()), mesh_of.data(),
+                                      world_to_object, object_to_world, num_instances, st,
+                                      as_stream(stream), *t);
+        r) {
+        delete t;
+        return r;
+    }
+    t->mesh_handles = handles;
+    for (uint32_t i = 0; i < context->tlases.size(); ++i) {
+        if (!context->tlases[i]) {
+            context->tlases[i] = t;
+            *instances         = i;
+            return MI_OK;
+        }
+    }
+    context->tlases.push_back(t);
+    *instances = uint32_t(context->tlases.size() - 1);
+    return MI_OK;
+}
+
+int32_t mi_instances_release(uint32_t instances) {
+    clear_error();
+    Tlas* t = lookup_tlas(instances);
+    if (!t) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT,
+                    "mi_instances_release: bad handle %u", instances);
+    }
+    (void)hipDeviceSynchronize();
+    t->release();
+    delete t;
+    context->tlases[instances] = nullptr;
+    return MI_OK;
+}
+
+namespace {
+
+int32_t trace_instances_call(uint32_t instances, mi_ray const* rays, uint32_t n, void* out,
+                             bool any_hit, void* stream) {
+    clear_error();
+    Tlas const* t = lookup_tlas(instances);
+    if (!t) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT, "instance trace: bad handle %u",
+                    instances);
+    }
+    if (n && (!rays || !out)) {
+        return fail(MI_ERR_ARGS, "instance trace: null buffer");
+    }
+    return trace_instances_launch(*t, context->trace.num_cus, rays, n, out, any_hit,
+                                  as_stream(stream));
+}
+
+}  // namespace
+
+int32_t mi_trace_instances_closest(uint32_t instances, mi_ray const* rays, uint32_t num_rays,
+                                   mi_instance_hit* hits, void* stream) {
+    return trace_instances_call(instances, rays, num_rays, hits, false, stream);
+}
+
+int32_t mi_trace_instances_any(uint32_t instances, mi_ray const* rays, uint32_t num_rays,
+                               uint32_t* occluded, void* stream) {
+    return trace_instances_call(instances, rays, num_rays, occluded, true, stream);
+}
+
+int32_t mi_instances_download(uint32_t instances, void* nodes, uint32_t* slots, float* boxes,
+                              uint32_t* num_nodes, uint32_t* num_slots) {
+    clear_error();
+    Tlas const* t = lookup_tlas(instances);
+    if (!t || !num_nodes || !num_slots) {
+        return fail(t ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_instances_download: bad arguments");
+    }
+    *num_nodes = t->tree.num_nodes;
+    *num_slots = t->tree.num_slots;
+    if (boxes) {
+        for (uint32_t i = 0; i < t->num_instances; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                boxes[6 * size_t(i) + k]     = t->records[i].lo[k];
+                boxes[6 * size_t(i) + 3 + k] = t->records[i].hi[k];
+            }
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (nodes) {
+        e = hipMemcpy(nodes, t->tree.nodes, size_t(t->tree.num_nodes) * 32, hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess && slots && t->tree.num_slots) {
+        e = hipMemcpy(slots, t->tree.slots, size_t(t->tree.num_slots) * 4, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) {
+        return fail(MI_ERR_HIP, "mi_instances_download: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
 }
 
 int32_t mi_scene_set_medium(uint32_t scene, float const* density, uint32_t const dims[3],

@@ -1,0 +1,63 @@
+// Instanced meshes under a top-level tree (host interface); see instances.hip.
+#pragma once
+
+#include "bvh_build.hpp"
+#include "common.hpp"
+#include "trace.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace mi {
+
+// One instance on the device: world-to-object rows (row-vector 4x4, as
+// sprout's Composed_transformation::world_to_object), world box, mesh.
+struct Instance_record {
+    float    world_to_object[16];
+    float    lo[3];
+    float    hi[3];
+    uint32_t mesh;
+    uint32_t pad;
+};
+
+struct Mesh_view {
+    float4 const* nodes;
+    float4 const* tris;
+};
+
+struct Tlas_view {
+    float4 const*          nodes;
+    float4 const*          slots;  // 3 float4 per top-level slot; .w of the first = instance
+    uint32_t               num_nodes;
+    Instance_record const* instances;
+    Mesh_view const*       meshes;
+};
+
+struct Tlas {
+    DeviceBvh                    tree;  // over one box per instance
+    Instance_record*             instances = nullptr;
+    Mesh_view*                   meshes    = nullptr;
+    std::vector<Instance_record> records;       // host copy
+    std::vector<uint32_t>        mesh_handles;  // the BVH handles the instances use
+    uint32_t                     num_instances = 0;
+    uint32_t                     num_meshes    = 0;
+    uint32_t                     max_stack     = 0;
+
+    void release();
+};
+
+// meshes[k]: device trees; instance i uses meshes[mesh_of[i]] with the
+// row-vector 4x4 matrices world_to_object / object_to_world (16 floats each,
+// host).  Builds the top-level tree on the device.
+int32_t create_tlas(DeviceBvh const* const* meshes, uint32_t num_meshes, uint32_t const* mesh_of,
+                    float const* world_to_object, float const* object_to_world,
+                    uint32_t num_instances, mi_bvh_settings const& settings, hipStream_t stream,
+                    Tlas& out);
+
+// Closest (out: mi_instance_hit per ray) or any hit (out: uint32 per ray).
+int32_t trace_instances_launch(Tlas const& t, uint32_t num_cus, mi_ray const* rays, uint32_t n,
+                               void* out, bool any_hit, hipStream_t stream);
+
+}  // namespace mi

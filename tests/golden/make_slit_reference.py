@@ -32,6 +32,8 @@ def main() -> None:
     ap.add_argument("--out", default=os.path.join(HERE, "slit_room_ref_256.npz"))
     ap.add_argument("--partial", default=None,
                     help="FP64 running sum kept here after every chunk; an existing one is resumed")
+    ap.add_argument("--resume", default=None,
+                    help="start from this running sum (written by --partial) instead")
     ap.add_argument("--seconds", type=float, default=1e9, help="stop (resumable) after this long")
     args = ap.parse_args()
 
@@ -45,8 +47,9 @@ def main() -> None:
     cam = render.camera(SIZE, SIZE, **scenes.ROOM_CAMERA)
     acc = torch.zeros((SIZE, SIZE, 3), dtype=torch.float64, device="cuda")
     done = 0
-    if args.partial and os.path.exists(args.partial):
-        with np.load(args.partial, allow_pickle=False) as z:
+    start = args.resume or args.partial
+    if start and os.path.exists(start):
+        with open(start, "rb") as f, np.load(f, allow_pickle=False) as z:
             acc.copy_(torch.from_numpy(z["sum"]))
             done = int(z["done"])
         print(f"resuming at {done} spp", flush=True)
@@ -58,7 +61,8 @@ def main() -> None:
         torch.cuda.synchronize()
         done += n
         if args.partial:
-            np.savez(args.partial, sum=acc.cpu().numpy(), done=np.int64(done))
+            with open(args.partial, "wb") as f:  # the name is kept as given
+                np.savez(f, sum=acc.cpu().numpy(), done=np.int64(done))
         print(f"{done} spp, {time.perf_counter() - t0:.1f} s", flush=True)
     if done < args.spp:
         print(f"stopped at {done} of {args.spp} spp", flush=True)

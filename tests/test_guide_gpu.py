@@ -113,23 +113,58 @@ def test_params_round_trip(gpu, net):
     assert np.array_equal(net.params(), p)
 
 
+def emulated_backward(params, x, w, t):
+    """G.backward with the kernel's roundings: features, weights, hidden
+    activations and the back-propagated gradients (output gradient and
+    each hidden layer's) rounded to bf16, FP64 otherwise."""
+    feat = bf16(G.encode(x))
+    acts = []
+    h = feat
+    for li, (wl, b) in enumerate(params):
+        acts.append(h)
+        z = h @ bf16(wl) + np.asarray(b, np.float32).astype(np.float64)
+        h = bf16(np.maximum(z, 0)) if li < 3 else z
+    dz = bf16(G.d_raw(h, w, t))
+    grads = [None] * len(params)
+    for li in range(len(params) - 1, -1, -1):
+        wl, _ = params[li]
+        grads[li] = [acts[li].T @ dz, dz.sum(0)]
+        if li > 0:
+            dz = bf16((dz @ bf16(wl).T) * (acts[li] > 0))
+    return grads
+
+
+def compare(a, b):
+    a = a.reshape(-1)
+    b = b.reshape(-1)
+    cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+    rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+    return cos, rel
+
+
 @pytest.mark.parametrize("n", [64, 1000, 50000])
-def test_gradients_match_fp64(gpu, net, n):
+def test_gradients_match_emulated_bf16_and_fp64(gpu, net, n):
+    """Against the oracle emulating the kernel's bf16 roundings the
+    gradients agree to FP32-accumulation noise (a tiling, layout or
+    reduction mistake moves them by their own size).  Against plain FP64
+    they agree to what bf16 costs: a few samples with a small mixture
+    density q carry weights t / q, and q depends exponentially on the
+    rounded concentrations (at n = 1000 the emulated oracle itself is at
+    cos 0.985, relative 0.17 from FP64 in the first layer)."""
     params = params_for_test(7)
     net.set_params(gpu.flatten(params))
     x, d, t = batch(n, 8)
     log(f"gradients n={n}")
     g = gpu.unflatten(net.gradients(x, d, t).cpu().numpy())
+    assert np.all(g[0][0][G.NUM_FEAT:] == 0)
+    emu = emulated_backward(params, x, d, t)
     ref = G.backward(params, G.encode(x), d, t)
     for li in range(4):
         for k in range(2):
-            a = g[li][k].reshape(-1)
-            b = ref[li][k].reshape(-1)
-            if li == 0 and k == 0:
-                assert np.all(g[0][0][G.NUM_FEAT:] == 0)
-            cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
-            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
-            assert cos > 0.995 and rel < 0.1, (li, k, cos, rel)
+            cos, rel = compare(g[li][k], emu[li][k])
+            assert cos > 0.9995 and rel < 0.03, ("emulated", li, k, cos, rel)
+            cos, rel = compare(g[li][k], ref[li][k])
+            assert cos > 0.97 and rel < 0.25, ("fp64", li, k, cos, rel)
 
 
 def test_gradients_are_deterministic(gpu, net):
@@ -170,6 +205,11 @@ def test_training_fits_a_lobe(gpu, net):
         net.train_step(x, d, t, lr=3e-3)
     p = gpu.unflatten(net.params())
     l1 = G.loss(p, G.encode(x[:256]), d[:256], t[:256])
-    # vMF(kappa = 20) has entropy about -1.2: the fit must come close
+    # the loss cannot go below the entropy of the target lobe,
+    # -(log C(k) + k A(k)) with C(k) = k / (4 pi sinh k), A(k) = coth k - 1 / k
+    # (about -0.16 at k = 20): the fit must come close to it
+    k = 20.0
+    log_c = np.log(k) - np.log(4 * np.pi) - (k + np.log1p(-np.exp(-2 * k)) - np.log(2.0))
+    entropy = -(log_c + k * (1 / np.tanh(k) - 1 / k))
     assert l1 < l0 - 2.0, (l0, l1)
-    assert l1 < -0.8, l1
+    assert l1 < entropy + 0.15, (l1, entropy)
