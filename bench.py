@@ -205,6 +205,72 @@ def cloud_render(frames: int, cpu: bool):
     return out
 
 
+def instance_trace(steps: int, warmup: int, cpu: bool = False):
+    """Instanced meshes: 64 instances of 4 distinct 20 480-triangle displaced
+    spheres (seeded similarity transforms in a 20-unit cube, 1.31 M instanced
+    triangles); closest and any hit of 4 M rays from a sphere of radius 25
+    toward random points in the cube."""
+    import torch
+
+    from sprout_amd import bvh, instances, scenes
+
+    trees, meshes = [], []
+    for k in range(4):
+        pos, idx = scenes.displaced_icosphere(5, octaves=4, amplitude=0.15, noise_seed=7 + k,
+                                              num_random=0)
+        meshes.append((pos, idx))
+        trees.append(bvh.Builder_SAH(16, 64, 4).build(torch.from_numpy(pos).cuda(),
+                                                      torch.from_numpy(idx.view(np.int32)).cuda()))
+    n_inst = 64
+    w2o, o2w = instances.random_transforms(n_inst, 107, extent=10.0)
+    mesh_of = np.arange(n_inst) % 4
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    inst = instances.Instances([trees[m] for m in mesh_of], w2o, o2w)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+
+    rng = np.random.default_rng(8)
+    n = 1 << 22
+    o = rng.normal(size=(n, 3))
+    o = o / np.linalg.norm(o, axis=1, keepdims=True) * 25.0
+    d = rng.uniform(-10, 10, (n, 3)) - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = d
+    rays[:, 7] = 3.4e38
+    d_rays = torch.from_numpy(rays).cuda()
+    hits = torch.empty((n, 8), dtype=torch.int32, device="cuda")
+    _, ms_closest = timed_launches(lambda: inst.intersect(d_rays), steps, warmup)
+    _, ms_any = timed_launches(lambda: inst.intersect_p(d_rays), max(10, steps // 5), 2)
+    hits = inst.intersect(d_rays)
+    torch.cuda.synchronize()
+    frac = float((hits.cpu().numpy().view(np.uint32)[:, 4] != 0xFFFFFFFF).mean())
+    out = {
+        "workload": "64 instances of 4 x 20480-tri displaced spheres, 4M rays into a 20-unit cube",
+        "closest_grays": round(n / (ms_closest * 1e-3) / 1e9, 4),
+        "any_grays": round(n / (ms_any * 1e-3) / 1e9, 4),
+        "closest_kernel_ms": round(ms_closest, 4),
+        "hit_fraction": round(frac, 4),
+        "top_level_build_ms": round(build_ms, 2),
+    }
+    if cpu:
+        from oracle import oracle as O
+
+        oms = [(p, i, O.build(p, i)) for p, i in meshes]
+        oi = O.Instances(oms, mesh_of.astype(np.uint32), w2o, o2w)
+        m = 200000
+        t0 = time.perf_counter()
+        oi.trace(rays[:m])
+        dt = time.perf_counter() - t0
+        out["cpu_port_grays"] = round(m / dt / 1e9, 5)
+        out["cpu_port_threads"] = 1
+        out["cpu_port_sample"] = f"{m} of the rays, oracle/instances.c, one thread"
+    inst.release()
+    return out
+
+
 SLIT_REF = os.path.join(ROOT, "tests", "golden", "slit_room_ref_256.npz")
 
 
@@ -288,6 +354,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--guide-seconds", type=float, default=2.0,
                     help="time budget of each side of the guided vs plain comparison")
+    ap.add_argument("--part", choices=["all", "room", "volume", "guiding", "instances"],
+                    default="all",
+                    help="run one workload only and print its figures (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="launch one 256 MiB device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -304,6 +373,13 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     bvh.init(0)
+
+    if args.part != "all":
+        parts = {"room": lambda: room_render(5, False), "volume": lambda: cloud_render(3, False),
+                 "guiding": lambda: slit_equal_time(args.guide_seconds),
+                 "instances": lambda: instance_trace(args.steps, args.warmup)}
+        print(json.dumps({"part": args.part, "figures": parts[args.part]()}), flush=True)
+        return 0
 
     if args.copy_probe:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
@@ -420,6 +496,8 @@ def main() -> int:
     line["extra"]["bounce_hit_fraction"] = round(hit_frac, 4)
     log("box room path tracing")
     line["extra"]["room_render"] = room_render(5, not args.no_cpu_baseline)
+    log("instanced meshes")
+    line["extra"]["instances"] = instance_trace(args.steps, args.warmup, not args.no_cpu_baseline)
     log("fBm cloud path tracing")
     line["extra"]["volume_render"] = cloud_render(3, not args.no_cpu_baseline)
     log("slit room: guided vs plain at equal time")

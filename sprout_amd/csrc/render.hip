@@ -865,6 +865,26 @@ __global__ void __launch_bounds__(256) finish_records(Wave w) {
     }
 }
 
+// Moves record j of the wave to slot (j P + salt) mod count, P = 2^31 - 1
+// (prime, so a bijection for every count below it): the records arrive in
+// queue order, which follows the pixels; consecutive training slices of the
+// shuffled copy mix the whole image.
+__global__ void __launch_bounds__(256) shuffle_records(Wave w, uint32_t salt, float* sx, float* sd,
+                                                       float* sw) {
+    uint32_t const count  = min(w.counters[C_records], w.record_capacity);
+    uint32_t const stride = gridDim.x * blockDim.x;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
+        uint32_t const k = uint32_t((uint64_t(j) * 2147483647ull + salt) % count);
+        for (int c = 0; c < 9; ++c) {
+            sx[9 * size_t(k) + c] = w.train_x[9 * size_t(j) + c];
+        }
+        for (int c = 0; c < 3; ++c) {
+            sd[3 * size_t(k) + c] = w.train_dir[3 * size_t(j) + c];
+        }
+        sw[k] = w.train_w[j];
+    }
+}
+
 __global__ void reset_records(uint32_t* counters) {
     counters[C_records] = 0;
 }
@@ -1463,6 +1483,9 @@ void RenderState::release() {
     (void)hipFree(train_x);
     (void)hipFree(train_dir);
     (void)hipFree(train_w);
+    (void)hipFree(shuf_x);
+    (void)hipFree(shuf_dir);
+    (void)hipFree(shuf_w);
     (void)hipFree(rec_t);
     (void)hipFree(rec_r);
     (void)hipFree(rec_path);
@@ -1555,10 +1578,17 @@ int32_t RenderState::reserve_guiding(uint32_t n, uint32_t records) {
         (void)hipFree(rec_t);
         (void)hipFree(rec_r);
         (void)hipFree(rec_path);
+        (void)hipFree(shuf_x);
+        (void)hipFree(shuf_dir);
+        (void)hipFree(shuf_w);
         train_x = train_dir = train_w = rec_t = rec_r = nullptr;
+        shuf_x = shuf_dir = shuf_w = nullptr;
         rec_path        = nullptr;
         record_capacity = 0;
         hipError_t e    = hipMalloc(&train_x, size_t(records) * 9 * sizeof(float));
+        if (!e) e = hipMalloc(&shuf_x, size_t(records) * 9 * sizeof(float));
+        if (!e) e = hipMalloc(&shuf_dir, size_t(records) * 3 * sizeof(float));
+        if (!e) e = hipMalloc(&shuf_w, size_t(records) * sizeof(float));
         if (!e) e = hipMalloc(&train_dir, size_t(records) * 3 * sizeof(float));
         if (!e) e = hipMalloc(&train_w, size_t(records) * sizeof(float));
         if (!e) e = hipMalloc(&rec_t, size_t(records) * 3 * sizeof(float));
@@ -1754,7 +1784,8 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
 
     // online training on the wave's records: train_steps Adam steps over
     // consecutive slices of the records
-    uint64_t trained = 0;
+    uint64_t trained       = 0;
+    uint32_t trained_waves = 0;
     auto train_wave = [&]() -> int32_t {
         hipLaunchKernelGGL(finish_records, dim3(grid_path), dim3(256), 0, s, w);
         uint32_t count = 0;
@@ -1764,20 +1795,25 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         if (e) {
             return fail(MI_ERR_HIP, "mi_render_guided: %s", hipGetErrorString(e));
         }
-        hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
         count = std::min(count, records);
         if (0 == count) {
+            hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
             return MI_OK;
         }
+        hipLaunchKernelGGL(shuffle_records, dim3(grid_path), dim3(256), 0, s, w,
+                           uint32_t(trained_waves * 2654435761u), state.shuf_x, state.shuf_dir,
+                           state.shuf_w);
+        hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
+        ++trained_waves;
         uint32_t const steps = gs->train_steps;
         uint32_t const batch = gs->batch ? std::min(gs->batch, count)
                                          : std::max<uint32_t>(1, (count + steps - 1) / steps);
         for (uint32_t k = 0; k < steps; ++k) {
             uint32_t const off = uint32_t((uint64_t(k) * batch) % count);
             uint32_t const nb  = std::min(batch, count - off);
-            if (int32_t const st = guide_adam_step(*net, state.train_x + 9 * size_t(off),
-                                                   state.train_dir + 3 * size_t(off),
-                                                   state.train_w + off, nb, gs->learning_rate, s);
+            if (int32_t const st = guide_adam_step(*net, state.shuf_x + 9 * size_t(off),
+                                                   state.shuf_dir + 3 * size_t(off),
+                                                   state.shuf_w + off, nb, gs->learning_rate, s);
                 st) {
                 return st;
             }

@@ -90,6 +90,9 @@ struct RenderState {
     float*    rec_t           = nullptr;
     float*    rec_r           = nullptr;
     uint32_t* rec_path        = nullptr;
+    float*    shuf_x          = nullptr;  // the wave's records, shuffled for training
+    float*    shuf_dir        = nullptr;
+    float*    shuf_w          = nullptr;
 
     uint32_t  shadow_capacity = 0;
 
