@@ -450,8 +450,11 @@ typedef struct mi_guide_settings {
     uint32_t record_depth;
 } mi_guide_settings;
 
-/* As mi_render, guided; runs kernel by kernel (no graph), waiting for each
- * wave's record count when training. */
+/* As mi_render, guided.  With training, every wave is followed on the
+ * device by the training weights of its records, a shuffled copy of them
+ * and train_steps Adam steps over slices of `batch` records (0: all records
+ * in train_steps slices); the whole wave, training included, is one hipGraph
+ * (kernel by kernel with MI_RENDER_NO_GRAPH). */
 int32_t mi_render_guided(uint32_t scene, mi_camera const* camera,
                          mi_render_settings const* settings, mi_guide_settings const* guide,
                          double* accum, void* stream, mi_render_stats* stats);

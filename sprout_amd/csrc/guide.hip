@@ -316,8 +316,17 @@ __device__ void back_layer(uint16_t const* src, int ss, uint16_t const* Wt, int 
 
 __global__ void __launch_bounds__(Grad_waves * 64)
     gradient_kernel(uint16_t const* wt, float const* params, float const* x, float const* dirs,
-                    float const* weights, uint32_t n, float const* t_sum, float* partial) {
+                    float const* weights, uint32_t n, uint32_t const* window, float const* t_sum,
+                    float* partial) {
     __shared__ Grad_lds s;
+    if (window) {
+        // the batch is records [window[0], window[0] + window[1]) of the arrays
+        uint32_t const off = window[0];
+        n                  = window[1];
+        x += 9 * size_t(off);
+        dirs += 3 * size_t(off);
+        weights += off;
+    }
     load_weights(s.w, wt, params);
     __syncthreads();
 
@@ -454,9 +463,13 @@ __global__ void __launch_bounds__(Grad_waves * 64)
     }
 }
 
-__global__ void sum_weights(float const* w, uint32_t n, float* out) {
+__global__ void sum_weights(float const* w, uint32_t n, uint32_t const* window, float* out) {
     __shared__ float part[256];
     float            s = 0.f;
+    if (window) {
+        w += window[0];
+        n = window[1];
+    }
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         s += w[i];
     }
@@ -516,12 +529,42 @@ __global__ void refresh_bf16(float const* params, uint16_t* wt) {
     }
 }
 
+// Training window of one Adam step: window = {first record, count, step t}.
+// Records [k batch mod count, + batch) of the first min(*records, cap); with
+// batch 0 the count is split into `steps` slices.  Empty windows take no
+// step.  trained (optional) sums the records used.
+__global__ void set_window(uint32_t const* records, uint32_t cap, uint32_t fixed_n, uint32_t k,
+                           uint32_t steps, uint32_t batch, uint32_t* window, uint32_t* step,
+                           unsigned long long* trained) {
+    uint32_t off = 0;
+    uint32_t nb  = fixed_n;
+    if (records) {
+        uint32_t const count = min(*records, cap);
+        uint32_t const b     = batch ? min(batch, count) : (count + steps - 1) / max(steps, 1u);
+        off                  = count ? uint32_t((uint64_t(k) * b) % count) : 0u;
+        nb                   = count ? min(b, count - off) : 0u;
+    }
+    window[0] = off;
+    window[1] = nb;
+    if (nb) {
+        *step += 1;
+        if (trained) {
+            *trained += nb;
+        }
+    }
+    window[2] = *step;
+}
+
+// Adam with the bias corrections of the window's step (no step on an empty window).
 __global__ void adam_kernel(float* params, float* m, float* v, float const* grad, float lr,
-                            float b1, float b2, float c1, float c2, float eps, uint16_t* wt) {
+                            float b1, float b2, uint32_t const* window, float eps, uint16_t* wt) {
     int const p = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (p >= Num_params) {
+    if (p >= Num_params || 0 == window[1]) {
         return;
     }
+    double const step = double(window[2]);
+    float const  c1   = float(1.0 - pow(double(b1), step));
+    float const  c2   = float(1.0 - pow(double(b2), step));
     float const g  = grad[p];
     float const mm = b1 * m[p] + (1.f - b1) * g;
     float const vv = b2 * v[p] + (1.f - b2) * g * g;
@@ -568,6 +611,7 @@ void GuideNet::release() {
     (void)hipFree(partial);
     (void)hipFree(t_sum);
     (void)hipFree(wt);
+    (void)hipFree(window);
     *this = GuideNet();
 }
 
@@ -581,10 +625,13 @@ int32_t guide_create(uint64_t seed, uint32_t num_cus, GuideNet& out) {
     if (!e) e = hipMalloc(&n.partial, size_t(n.partial_blocks) * Num_params * sizeof(float));
     if (!e) e = hipMalloc(&n.t_sum, sizeof(float));
     if (!e) e = hipMalloc(&n.wt, Num_t * sizeof(uint16_t));
+    if (!e) e = hipMalloc(&n.window, 8 * sizeof(uint32_t));
     if (e) {
         n.release();
         return fail(MI_ERR_HIP, "guide: %s", hipGetErrorString(e));
     }
+    n.step_dev     = n.window + 4;
+    n.scratch_step = n.window + 5;
     // He-uniform weights (fan-in of the unpadded features), zero biases
     std::vector<float> p(Num_params, 0.f);
     Pcg                rng{0, (seed << 1) | 1};
@@ -614,12 +661,12 @@ int32_t guide_set_params(GuideNet& net, float const* host_params) {
     hipError_t e = hipMemcpy(net.params, host_params, Num_params * sizeof(float), hipMemcpyHostToDevice);
     if (!e) e = hipMemset(net.adam_m, 0, Num_params * sizeof(float));
     if (!e) e = hipMemset(net.adam_v, 0, Num_params * sizeof(float));
+    if (!e) e = hipMemset(net.window, 0, 8 * sizeof(uint32_t));  // step 0
     if (!e) e = refresh(net, nullptr);
     if (!e) e = hipDeviceSynchronize();
     if (e) {
         return fail(MI_ERR_HIP, "guide parameters: %s", hipGetErrorString(e));
     }
-    net.step = 0;
     return MI_OK;
 }
 
@@ -662,18 +709,39 @@ int32_t guide_forward_device_count(GuideNet const& net, float const* x, uint32_t
     return MI_OK;
 }
 
+namespace {
+
+// gradient of the batch described by net.window (max_n bounds its count)
+void enqueue_gradients(GuideNet& net, float const* x, float const* dirs, float const* weights,
+                       uint32_t max_n, hipStream_t stream) {
+    uint32_t const blocks = std::min(blocks_for(max_n, Chunk), net.partial_blocks);
+    hipLaunchKernelGGL(sum_weights, dim3(1), dim3(256), 0, stream, weights, 0u,
+                       static_cast<uint32_t const*>(net.window), net.t_sum);
+    hipLaunchKernelGGL(gradient_kernel, dim3(blocks), dim3(Grad_waves * 64), 0, stream, net.wt,
+                       net.params, x, dirs, weights, 0u, static_cast<uint32_t const*>(net.window),
+                       net.t_sum, net.partial);
+    hipLaunchKernelGGL(reduce_partials, dim3(blocks_for(Num_params, 256)), dim3(256), 0, stream,
+                       net.partial, blocks, net.grad);
+}
+
+void enqueue_adam(GuideNet& net, float lr, hipStream_t stream) {
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(Num_params, 256)), dim3(256), 0, stream,
+                       net.params, net.adam_m, net.adam_v, net.grad, lr, 0.9f, 0.99f,
+                       static_cast<uint32_t const*>(net.window), 1e-8f, net.wt);
+}
+
+}  // namespace
+
 int32_t guide_gradients(GuideNet& net, float const* x, float const* dirs, float const* weights,
                         uint32_t n, hipStream_t stream) {
     if (0 == n) {
         return fail(MI_ERR_ARGS, "guide gradients: empty batch");
     }
-    uint32_t const chunks = blocks_for(n, Chunk);
-    uint32_t const blocks = std::min(chunks, net.partial_blocks);
-    hipLaunchKernelGGL(sum_weights, dim3(1), dim3(256), 0, stream, weights, n, net.t_sum);
-    hipLaunchKernelGGL(gradient_kernel, dim3(blocks), dim3(Grad_waves * 64), 0, stream, net.wt,
-                       net.params, x, dirs, weights, n, net.t_sum, net.partial);
-    hipLaunchKernelGGL(reduce_partials, dim3(blocks_for(Num_params, 256)), dim3(256), 0, stream,
-                       net.partial, blocks, net.grad);
+    // a fixed window of n records that takes no Adam step
+    hipLaunchKernelGGL(set_window, dim3(1), dim3(1), 0, stream, static_cast<uint32_t const*>(nullptr),
+                       0u, n, 0u, 1u, 0u, net.window, net.scratch_step,
+                       static_cast<unsigned long long*>(nullptr));
+    enqueue_gradients(net, x, dirs, weights, n, stream);
     if (hipError_t const e = hipGetLastError(); e) {
         return fail(MI_ERR_HIP, "guide gradients: %s", hipGetErrorString(e));
     }
@@ -682,19 +750,32 @@ int32_t guide_gradients(GuideNet& net, float const* x, float const* dirs, float 
 
 int32_t guide_adam_step(GuideNet& net, float const* x, float const* dirs, float const* weights,
                         uint32_t n, float lr, hipStream_t stream) {
-    if (int32_t const st = guide_gradients(net, x, dirs, weights, n, stream); st) {
-        return st;
+    if (0 == n) {
+        return fail(MI_ERR_ARGS, "guide adam: empty batch");
     }
-    net.step += 1;
-    float const b1 = 0.9f;
-    float const b2 = 0.99f;
-    float const c1 = float(1.0 - std::pow(double(b1), double(net.step)));
-    float const c2 = float(1.0 - std::pow(double(b2), double(net.step)));
-    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(Num_params, 256)), dim3(256), 0, stream,
-                       net.params, net.adam_m, net.adam_v, net.grad, lr, b1, b2, c1, c2, 1e-8f,
-                       net.wt);
+    hipLaunchKernelGGL(set_window, dim3(1), dim3(1), 0, stream, static_cast<uint32_t const*>(nullptr),
+                       0u, n, 0u, 1u, 0u, net.window, net.step_dev,
+                       static_cast<unsigned long long*>(nullptr));
+    enqueue_gradients(net, x, dirs, weights, n, stream);
+    enqueue_adam(net, lr, stream);
     if (hipError_t const e = hipGetLastError(); e) {
         return fail(MI_ERR_HIP, "guide adam: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
+int32_t guide_train_records(GuideNet& net, float const* x, float const* dirs, float const* weights,
+                            uint32_t const* num_records, uint32_t capacity, uint32_t steps,
+                            uint32_t batch, float lr, unsigned long long* trained,
+                            hipStream_t stream) {
+    for (uint32_t k = 0; k < steps; ++k) {
+        hipLaunchKernelGGL(set_window, dim3(1), dim3(1), 0, stream, num_records, capacity, 0u, k,
+                           steps, batch, net.window, net.step_dev, trained);
+        enqueue_gradients(net, x, dirs, weights, batch ? std::min(batch, capacity) : capacity, stream);
+        enqueue_adam(net, lr, stream);
+    }
+    if (hipError_t const e = hipGetLastError(); e) {
+        return fail(MI_ERR_HIP, "guide training: %s", hipGetErrorString(e));
     }
     return MI_OK;
 }

@@ -58,8 +58,10 @@ struct GuideNet {
     float*    partial = nullptr;  // per-block gradient partial sums
     float*    t_sum   = nullptr;  // batch weight sum (one float)
     uint16_t* wt      = nullptr;  // bf16 W^T of every layer
+    uint32_t* window  = nullptr;  // {first record, count, Adam step} of the current batch
+    uint32_t* step_dev     = nullptr;  // Adam steps taken (device word)
+    uint32_t* scratch_step = nullptr;  // step word of gradient-only windows
     uint32_t  partial_blocks = 0;
-    uint32_t  step           = 0;
 
     void release();
 };
@@ -79,6 +81,13 @@ int32_t guide_gradients(GuideNet& net, float const* x, float const* dirs, float 
 // Gradient + one Adam step.
 int32_t guide_adam_step(GuideNet& net, float const* x, float const* dirs, float const* weights,
                         uint32_t n, float lr, hipStream_t stream);
+// `steps` Adam steps over the first min(*num_records, capacity) records (a
+// count in device memory), in slices of `batch` (0: count / steps); trained
+// (device, optional) sums the records used.  Kernels only: capturable.
+int32_t guide_train_records(GuideNet& net, float const* x, float const* dirs, float const* weights,
+                            uint32_t const* num_records, uint32_t capacity, uint32_t steps,
+                            uint32_t batch, float lr, unsigned long long* trained,
+                            hipStream_t stream);
 
 namespace dev {
 

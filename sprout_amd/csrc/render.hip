@@ -69,7 +69,8 @@ constexpr uint32_t C_queue0     = 0;  // live rays, two ping-pong queues
 constexpr uint32_t C_shadow     = 2;  // shadow rays
 constexpr uint32_t C_base       = 4;  // first sample index of the current wave
 constexpr uint32_t C_records    = 6;  // training samples recorded in the current wave
-constexpr uint32_t C_totals     = 8;  // two uint64: extension and shadow rays traced
+constexpr uint32_t C_totals     = 8;  // two uint64: extension and shadow rays traced,
+                                      // then one uint64: guiding records trained on
 constexpr uint32_t Num_counters = 16;
 
 // rnd::Generator, base/random/generator.inl:16-49
@@ -313,7 +314,7 @@ struct Surface {
 
 __global__ void set_base(uint32_t* counters, uint32_t first_sample) {
     counters[C_base] = first_sample;
-    for (uint32_t k = 0; k < 4; ++k) {
+    for (uint32_t k = 0; k < 6; ++k) {
         counters[C_totals + k] = 0;
     }
 }
@@ -869,9 +870,10 @@ __global__ void __launch_bounds__(256) finish_records(Wave w) {
 // (prime, so a bijection for every count below it): the records arrive in
 // queue order, which follows the pixels; consecutive training slices of the
 // shuffled copy mix the whole image.
-__global__ void __launch_bounds__(256) shuffle_records(Wave w, uint32_t salt, float* sx, float* sd,
-                                                       float* sw) {
+__global__ void __launch_bounds__(256) shuffle_records(Wave w, uint32_t const* step, float* sx,
+                                                       float* sd, float* sw) {
     uint32_t const count  = min(w.counters[C_records], w.record_capacity);
+    uint32_t const salt   = *step * 2654435761u;  // a new permutation after every Adam step
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
         uint32_t const k = uint32_t((uint64_t(j) * 2147483647ull + salt) % count);
@@ -1489,6 +1491,9 @@ void RenderState::release() {
     (void)hipFree(rec_t);
     (void)hipFree(rec_r);
     (void)hipFree(rec_path);
+    if (graph) {
+        (void)hipGraphExecDestroy(graph);
+    }
     if (begin) {
         (void)hipEventDestroy(begin);
     }
@@ -1511,6 +1516,9 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
             release();
             return fail(MI_ERR_HIP, "render state: %s", hipGetErrorString(e));
         }
+    }
+    if ((num_shadow > shadow_capacity || n > capacity) && stream) {
+        (void)hipStreamSynchronize(stream);  // earlier waves may still use the buffers
     }
     if (num_shadow > shadow_capacity) {
         (void)hipFree(shadow_rays);
@@ -1559,6 +1567,9 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
 }
 
 int32_t RenderState::reserve_guiding(uint32_t n, uint32_t records) {
+    if ((n > guide_capacity || records > record_capacity) && stream) {
+        (void)hipStreamSynchronize(stream);  // earlier waves may still use the buffers
+    }
     if (n > guide_capacity) {
         (void)hipFree(guide_x);
         (void)hipFree(guide_raw);
@@ -1668,7 +1679,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     uint32_t record_depth = 0;
     uint32_t records      = 0;
     if (guided) {
-        record_depth = gs->train ? std::min(gs->record_depth, set.max_bounces + 1) : 0;
+        record_depth = (gs->train && gs->train_steps) ? std::min(gs->record_depth, set.max_bounces + 1) : 0;
         records      = uint32_t(std::min<uint64_t>(uint64_t(num_paths) * record_depth, 1ull << 26));
         if (int32_t const st = state.reserve_guiding(num_paths, std::max<uint32_t>(records, 1)); st) {
             return st;
@@ -1676,6 +1687,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     }
 
     Wave w;
+    std::memset(&w, 0, sizeof(w));  // padding too: the bytes key the cached graph
     w.positions    = scene.positions;
     w.indices      = scene.indices;
     w.tri_material = scene.tri_material;
@@ -1784,43 +1796,24 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
 
     // online training on the wave's records: train_steps Adam steps over
     // consecutive slices of the records
-    uint64_t trained       = 0;
-    uint32_t trained_waves = 0;
-    auto train_wave = [&]() -> int32_t {
+    // online training on the wave's records, all on the device (capturable):
+    // training weights, a shuffled copy, train_steps Adam steps over slices
+    auto enqueue_training = [&]() -> int32_t {
         hipLaunchKernelGGL(finish_records, dim3(grid_path), dim3(256), 0, s, w);
-        uint32_t count = 0;
-        hipError_t e   = hipMemcpyAsync(&count, state.counters + C_records, sizeof(uint32_t),
-                                        hipMemcpyDeviceToHost, s);
-        if (!e) e = hipStreamSynchronize(s);
-        if (e) {
-            return fail(MI_ERR_HIP, "mi_render_guided: %s", hipGetErrorString(e));
-        }
-        count = std::min(count, records);
-        if (0 == count) {
-            hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
-            return MI_OK;
-        }
         hipLaunchKernelGGL(shuffle_records, dim3(grid_path), dim3(256), 0, s, w,
-                           uint32_t(trained_waves * 2654435761u), state.shuf_x, state.shuf_dir,
+                           static_cast<uint32_t const*>(net->step_dev), state.shuf_x, state.shuf_dir,
                            state.shuf_w);
-        hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
-        ++trained_waves;
-        uint32_t const steps = gs->train_steps;
-        uint32_t const batch = gs->batch ? std::min(gs->batch, count)
-                                         : std::max<uint32_t>(1, (count + steps - 1) / steps);
-        for (uint32_t k = 0; k < steps; ++k) {
-            uint32_t const off = uint32_t((uint64_t(k) * batch) % count);
-            uint32_t const nb  = std::min(batch, count - off);
-            if (int32_t const st = guide_adam_step(*net, state.shuf_x + 9 * size_t(off),
-                                                   state.shuf_dir + 3 * size_t(off),
-                                                   state.shuf_w + off, nb, gs->learning_rate, s);
-                st) {
-                return st;
-            }
-            trained += nb;
+        if (int32_t const st = guide_train_records(
+                *net, state.shuf_x, state.shuf_dir, state.shuf_w, &state.counters[C_records], records,
+                gs->train_steps, gs->batch, gs->learning_rate,
+                reinterpret_cast<unsigned long long*>(state.counters + C_totals + 4), s);
+            st) {
+            return st;
         }
+        hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
         return MI_OK;
     };
+    bool const training = guided && gs->train && gs->train_steps > 0;
 
     // order after the caller's earlier work on its stream
     hipError_t e = hipEventRecord(state.begin, stream);
@@ -1833,48 +1826,89 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
     }
 
-    if (guided || (set.flags & MI_RENDER_NO_GRAPH)) {
+    if (set.flags & MI_RENDER_NO_GRAPH) {
         for (uint32_t wave = 0; wave < waves; ++wave) {
             if (int32_t const st = enqueue_wave(); st) {
                 return st;
             }
-            if (guided && gs->train) {
-                if (int32_t const st = train_wave(); st) {
+            if (training) {
+                if (int32_t const st = enqueue_training(); st) {
                     return st;
                 }
             }
         }
     } else {
-        hipGraph_t     graph = nullptr;
-        hipGraphExec_t exec  = nullptr;
-        e                    = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-        if (e) {
-            return fail(MI_ERR_HIP, "mi_render: begin capture: %s", hipGetErrorString(e));
+        // The wave's graph is kept between calls: everything its kernels take
+        // (the wave description, buffers, launch shapes, training settings)
+        // forms the key, and an equal key replays the cached graph.
+        std::vector<uint8_t> key(sizeof(Wave));
+        std::memcpy(key.data(), &w, sizeof(Wave));
+        auto put = [&key](auto const& v) {
+            uint8_t const* b = reinterpret_cast<uint8_t const*>(&v);
+            key.insert(key.end(), b, b + sizeof(v));
+        };
+        put(accum);
+        put(grid_path);
+        put(num_shadow);
+        put(spw);
+        put(scene.bvh.nodes);
+        put(scene.bvh.tris);
+        put(trace.counter);
+        put(trace.spill);
+        put(trace.lds_entries);
+        put(trace.kernel);
+        put(trace.block_size);
+        put(trace.blocks_per_cu);
+        put(trace.refill_lanes);
+        put(trace.claim_batch);
+        put(training);
+        put(net);
+        if (training) {
+            put(gs->train_steps);
+            put(gs->batch);
+            put(gs->learning_rate);
+            put(records);
+            put(state.shuf_x);
         }
-        int32_t const    st = enqueue_wave();
-        hipError_t const ce = hipStreamEndCapture(s, &graph);
-        if (st) {
-            if (graph) {
-                (void)hipGraphDestroy(graph);
+        if (!state.graph || key != state.graph_key) {
+            if (state.graph) {
+                (void)hipStreamSynchronize(s);  // its earlier launches may still run
+                (void)hipGraphExecDestroy(state.graph);
+                state.graph = nullptr;
+                state.graph_key.clear();
             }
-            return st;
-        }
-        if (ce) {
-            return fail(MI_ERR_HIP, "mi_render: end capture: %s", hipGetErrorString(ce));
-        }
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e) {
-            return fail(MI_ERR_HIP, "mi_render: graph instantiate: %s", hipGetErrorString(e));
+            hipGraph_t graph = nullptr;
+            e                = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (e) {
+                return fail(MI_ERR_HIP, "mi_render: begin capture: %s", hipGetErrorString(e));
+            }
+            int32_t st = enqueue_wave();
+            if (!st && training) {
+                st = enqueue_training();
+            }
+            hipError_t const ce = hipStreamEndCapture(s, &graph);
+            if (st) {
+                if (graph) {
+                    (void)hipGraphDestroy(graph);
+                }
+                return st;
+            }
+            if (ce) {
+                return fail(MI_ERR_HIP, "mi_render: end capture: %s", hipGetErrorString(ce));
+            }
+            e = hipGraphInstantiate(&state.graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e) {
+                state.graph = nullptr;
+                return fail(MI_ERR_HIP, "mi_render: graph instantiate: %s", hipGetErrorString(e));
+            }
+            state.graph_key = std::move(key);
         }
         for (uint32_t wave = 0; wave < waves && !e; ++wave) {
-            e = hipGraphLaunch(exec, s);
+            e = hipGraphLaunch(state.graph, s);
         }
-        // the executable graph is destroyed only after its launches ran
-        hipError_t const se = hipStreamSynchronize(s);
-        (void)hipGraphExecDestroy(exec);
-        if (e || se) {
-            return fail(MI_ERR_HIP, "mi_render: graph launch: %s", hipGetErrorString(e ? e : se));
+        if (e) {
+            return fail(MI_ERR_HIP, "mi_render: graph launch: %s", hipGetErrorString(e));
         }
     }
 
@@ -1885,7 +1919,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         return fail(MI_ERR_HIP, "mi_render: %s", hipGetErrorString(e));
     }
     if (stats) {
-        uint64_t totals[2] = {0, 0};
+        uint64_t totals[3] = {0, 0, 0};
         e = hipMemcpyAsync(totals, state.counters + C_totals, sizeof(totals), hipMemcpyDeviceToHost,
                            s);
         if (!e) e = hipStreamSynchronize(s);
@@ -1897,7 +1931,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         stats->paths       = double(num_paths) * waves;
         stats->extend_rays = double(totals[0]);
         stats->shadow_rays = double(totals[1]);
-        stats->trained     = double(trained);
+        stats->trained     = double(totals[2]);
     }
     return MI_OK;
 }

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace mi {
 
@@ -79,6 +80,8 @@ struct RenderState {
     hipStream_t  stream      = nullptr;  // internal stream the bounce loop runs (and is captured) on
     hipEvent_t   begin       = nullptr;
     hipEvent_t   end         = nullptr;
+    hipGraphExec_t       graph = nullptr;  // the last wave's graph, replayed while its key holds
+    std::vector<uint8_t> graph_key;
     // guiding
     uint32_t  guide_capacity  = 0;
     float*    guide_x         = nullptr;  // network inputs per queue entry

@@ -105,7 +105,8 @@ def render_guided(scene: Scene, cam: _lib.MiCamera, spp: int, guide, fraction: f
                   train: bool = True, train_steps: int = 4, batch: int = 0,
                   learning_rate: float = 1e-3, record_depth: int = 4, samples_per_wave: int = 1,
                   min_bounces: int = 4, max_bounces: int = 8, seed: int = 1,
-                  first_sample: int = 0, accum: torch.Tensor | None = None, stats: bool = False):
+                  first_sample: int = 0, accum: torch.Tensor | None = None, stats: bool = False,
+                  graph: bool = True):
     """mi_render_guided: as render(), sampling directions from a one-sample
     mixture of the guide (probability fraction) and the BSDF, and training
     the guide online after every wave when train is set."""
@@ -114,8 +115,8 @@ def render_guided(scene: Scene, cam: _lib.MiCamera, spp: int, guide, fraction: f
     if accum.dtype != torch.float64 or not accum.is_contiguous() or \
             tuple(accum.shape) != (cam.height, cam.width, 3):
         raise ValueError("accum must be a contiguous (height, width, 3) float64 device tensor")
-    st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces, 0,
-                               seed)
+    st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
+                               0 if graph else _lib.MI_RENDER_NO_GRAPH, seed)
     gs = _lib.MiGuideSettings(guide.handle, fraction, 1 if train else 0, train_steps, batch,
                               learning_rate, record_depth)
     out = _lib.MiRenderStats() if stats else None

@@ -124,7 +124,34 @@ def read_figures(path):
     return out
 
 
+def stats_only(argv):
+    """Kernel table of one --kernel-trace --stats run:
+    python -m sprout_amd.rocprof_summary --stats-only <kt> --title T --out F --command C"""
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stats-only", dest="kt", required=True)
+    ap.add_argument("--title", required=True)
+    ap.add_argument("--out", default="")
+    ap.add_argument("--command", default="")
+    ap.add_argument("--top", type=int, default=16)
+    ap.add_argument("--note", default="")
+    args = ap.parse_args(argv)
+    text = (f"# rocprofv3: {args.title}\n\n"
+            f"Command (run from the repository root on the GPU box, after `cd /tmp && export TMPDIR=/tmp`):\n\n"
+            f"```\n{args.command}\n```\n\n"
+            f"Kernel statistics of the `--kernel-trace --stats` run:\n\n{kernel_table(args.kt, args.top)}\n"
+            + (f"\n{args.note}\n" if args.note else ""))
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
+    print(text)
+
+
 def main():
+    import sys
+
+    if "--stats-only" in sys.argv:
+        stats_only(sys.argv[1:])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("kt")
     ap.add_argument("fetch")

@@ -99,3 +99,28 @@ def test_training_lowers_error_in_the_slit_room(gpu):
     assert e_guided < 0.9 * e_plain, (e_plain, e_guided)
     g.release()
     sc.release()
+
+
+def test_guided_graph_equals_launches(gpu):
+    """The captured guided wave (render + on-device training) replays the
+    kernel-by-kernel run bit for bit: image and trained parameters."""
+    import torch
+
+    from sprout_amd import scenes
+
+    render, guide = gpu
+    sc = scene(gpu, 0.05)
+    cam = render.camera(32, 32, **scenes.ROOM_CAMERA)
+    out = []
+    for graph in (True, False):
+        g = guide.Guide(3)
+        img, st = render.render_guided(sc, cam, 64, g, fraction=0.5, train=True, train_steps=3,
+                                       batch=4096, learning_rate=2e-3, samples_per_wave=8, seed=5,
+                                       graph=graph, stats=True)
+        torch.cuda.synchronize()
+        out.append((img.cpu().numpy(), g.params(), st.trained))
+        g.release()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2] > 0
+
