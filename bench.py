@@ -205,6 +205,47 @@ def cloud_render(frames: int, cpu: bool):
     return out
 
 
+def batch_render(frames: int):
+    """Batched path tracing: 64 placements of 4 distinct 20 480-triangle
+    displaced spheres with per-instance Substitute materials, a ground plane
+    and 10 000 emissive triangles in the light tree (1.32 M triangles, baked
+    into world space); 1920 x 1080, 4 spp per frame in waves of 1 spp,
+    Russian roulette from depth 3, max depth 16."""
+    import torch
+
+    from sprout_amd import render, scenes
+
+    t0 = time.perf_counter()
+    pos, idx, tmat, mats = scenes.batch_scene()
+    sc = render.Scene(pos, idx, tmat, mats)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    cam = render.camera(1920, 1080, **scenes.BATCH_CAMERA)
+    acc = torch.zeros((1080, 1920, 3), dtype=torch.float64, device="cuda")
+    render.render(sc, cam, 1, samples_per_wave=1, min_bounces=3, max_bounces=16, seed=1, accum=acc)
+    torch.cuda.synchronize()
+    paths = rays = 0.0
+    t0 = time.perf_counter()
+    for f in range(frames):
+        _, st = render.render(sc, cam, 4, samples_per_wave=1, min_bounces=3, max_bounces=16, seed=1,
+                              first_sample=1 + 4 * f, accum=acc, stats=True)
+        paths += st.paths
+        rays += st.extend_rays + st.shadow_rays
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sc.release()
+    return {
+        "workload": "64 instances x 20480-tri spheres (baked, 1.32M tris), 10000 lights, "
+                    "1920x1080, 4 spp per frame, RR from depth 3, max depth 16",
+        "scene_setup_s": round(setup, 2),
+        "ms_per_frame": round(wall / frames * 1e3, 2),
+        "samples_per_s": round(paths / wall, 1),
+        "grays_all_rays": round(rays / wall / 1e9, 4),
+        "rays_per_sample": round(rays / paths, 3),
+        "seconds_for_64_spp": round(1920 * 1080 * 64 / (paths / wall), 2),
+    }
+
+
 def instance_trace(steps: int, warmup: int, cpu: bool = False):
     """Instanced meshes: 64 instances of 4 distinct 20 480-triangle displaced
     spheres (seeded similarity transforms in a 20-unit cube, 1.31 M instanced
@@ -354,7 +395,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--guide-seconds", type=float, default=2.0,
                     help="time budget of each side of the guided vs plain comparison")
-    ap.add_argument("--part", choices=["all", "room", "volume", "guiding", "instances"],
+    ap.add_argument("--part", choices=["all", "room", "volume", "guiding", "instances", "batch"],
                     default="all",
                     help="run one workload only and print its figures (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -376,6 +417,7 @@ def main() -> int:
 
     if args.part != "all":
         parts = {"room": lambda: room_render(5, False), "volume": lambda: cloud_render(3, False),
+                 "batch": lambda: batch_render(2),
                  "guiding": lambda: slit_equal_time(args.guide_seconds),
                  "instances": lambda: instance_trace(args.steps, args.warmup)}
         print(json.dumps({"part": args.part, "figures": parts[args.part]()}), flush=True)
@@ -496,6 +538,8 @@ def main() -> int:
     line["extra"]["bounce_hit_fraction"] = round(hit_frac, 4)
     log("box room path tracing")
     line["extra"]["room_render"] = room_render(5, not args.no_cpu_baseline)
+    log("batched path tracing")
+    line["extra"]["batch_render"] = batch_render(2)
     log("instanced meshes")
     line["extra"]["instances"] = instance_trace(args.steps, args.warmup, not args.no_cpu_baseline)
     log("fBm cloud path tracing")

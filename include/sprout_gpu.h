@@ -62,7 +62,8 @@ typedef struct mi_instance_hit {
     float    v;
     uint32_t triangle;
     uint32_t instance; /* MI_NULL on a miss */
-    uint32_t pad[3];
+    uint32_t slot;     /* the triangle's slot in its mesh tree (Intersection::index) */
+    uint32_t pad[2];
 } mi_instance_hit;
 
 /* SAH builder settings; sprout meshes use {16, 64, 4}

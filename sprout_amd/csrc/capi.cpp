@@ -763,7 +763,7 @@ int32_t trace_instances_call(uint32_t instances, mi_ray const* rays, uint32_t n,
     if (n && (!rays || !out)) {
         return fail(MI_ERR_ARGS, "instance trace: null buffer");
     }
-    return trace_instances_launch(*t, context->trace.num_cus, rays, n, out, any_hit,
+    return trace_instances_launch(*t, context->trace.num_cus, rays, n, nullptr, out, any_hit,
                                   as_stream(stream));
 }
 

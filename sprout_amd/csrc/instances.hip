@@ -55,23 +55,8 @@ __device__ __forceinline__ void set_inverse(Iray& r) {
     r.sign_bits = (r.inv.x < 0.f ? 1u : 0u) | (r.inv.y < 0.f ? 2u : 0u) | (r.inv.z < 0.f ? 4u : 0u);
 }
 
-// transform_point / transform_vector with the rounding the reference build
-// performs (rows r0..r3 of a row-vector 4x4)
-__device__ __forceinline__ F3 xform_point(float const* m, F3 v) {
-    F3 r;
-    r.x = __builtin_fmaf(v.z, m[8], __builtin_fmaf(v.x, m[0], v.y * m[4])) + m[12];
-    r.y = __builtin_fmaf(v.z, m[9], __builtin_fmaf(v.x, m[1], v.y * m[5])) + m[13];
-    r.z = __builtin_fmaf(v.z, m[10], __builtin_fmaf(v.x, m[2], v.y * m[6])) + m[14];
-    return r;
-}
-
-__device__ __forceinline__ F3 xform_vector(float const* m, F3 v) {
-    F3 r;
-    r.x = __builtin_fmaf(v.z, m[8], __builtin_fmaf(v.x, m[0], v.y * m[4]));
-    r.y = __builtin_fmaf(v.z, m[9], __builtin_fmaf(v.x, m[1], v.y * m[5]));
-    r.z = __builtin_fmaf(v.z, m[10], __builtin_fmaf(v.x, m[2], v.y * m[6]));
-    return r;
-}
+using dev::xform_point;
+using dev::xform_vector;
 
 __device__ __forceinline__ uint32_t node_num(uint32_t meta) {
     return (meta >> 8) & 0xFFu;
@@ -102,7 +87,8 @@ struct Instance_stack {
 // shared stack above a sentinel.  Closest: shrinks r.max_t on every hit.
 template <bool Any>
 __device__ bool mesh_trace(float4 const* __restrict__ nodes, float4 const* __restrict__ tris,
-                           Iray& r, Instance_stack& stack, uint32_t& index, float& u, float& v) {
+                           Iray& r, Instance_stack& stack, uint32_t& index, uint32_t& slot, float& u,
+                           float& v) {
     bool hit = false;
     stack.push(Null);
     uint32_t n = 0;
@@ -141,6 +127,7 @@ __device__ bool mesh_trace(float4 const* __restrict__ nodes, float4 const* __res
                     u       = tu;
                     v       = tv;
                     index   = __float_as_uint(ta.w);
+                    slot    = i;
                 }
             }
         }
@@ -151,11 +138,15 @@ __device__ bool mesh_trace(float4 const* __restrict__ nodes, float4 const* __res
 
 template <bool Any>
 __global__ void __launch_bounds__(Block) trace_instances(Tlas_view t, mi_ray const* rays,
-                                                         uint32_t num_rays, void* out) {
+                                                         uint32_t num_rays,
+                                                         uint32_t const* num_rays_dev, void* out) {
     extern __shared__ uint32_t lds[];
     Instance_stack stack;
     stack.s = lds;
 
+    if (num_rays_dev) {
+        num_rays = min(num_rays, *num_rays_dev);
+    }
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t ri = blockIdx.x * blockDim.x + threadIdx.x; ri < num_rays; ri += stride) {
         float4 const* rp = reinterpret_cast<float4 const*>(rays) + 2 * size_t(ri);
@@ -170,6 +161,7 @@ __global__ void __launch_bounds__(Block) trace_instances(Tlas_view t, mi_ray con
 
         uint32_t instance = Null;
         uint32_t index    = Null;
+        uint32_t slot     = Null;
         float    u = 0.f, v = 0.f;
         bool     any = false;
 
@@ -209,9 +201,9 @@ __global__ void __launch_bounds__(Block) trace_instances(Tlas_view t, mi_ray con
                     o.max_t = w.max_t;
                     set_inverse(o);
                     Mesh_view const mv = t.meshes[ir.mesh];
-                    uint32_t        idx = Null;
+                    uint32_t        idx = Null, hslot = Null;
                     float           hu = 0.f, hv = 0.f;
-                    if (mesh_trace<Any>(mv.nodes, mv.tris, o, stack, idx, hu, hv)) {
+                    if (mesh_trace<Any>(mv.nodes, mv.tris, o, stack, idx, hslot, hu, hv)) {
                         if (Any) {
                             any = true;
                             break;
@@ -219,6 +211,7 @@ __global__ void __launch_bounds__(Block) trace_instances(Tlas_view t, mi_ray con
                         w.max_t  = o.max_t;
                         instance = p;
                         index    = idx;
+                        slot     = hslot;
                         u        = hu;
                         v        = hv;
                     }
@@ -239,7 +232,8 @@ __global__ void __launch_bounds__(Block) trace_instances(Tlas_view t, mi_ray con
             h.v        = Null != index ? v : 0.f;
             h.triangle = index;
             h.instance = instance;
-            h.pad[0] = h.pad[1] = h.pad[2] = 0;
+            h.slot     = slot;
+            h.pad[0] = h.pad[1] = 0;
             static_cast<mi_instance_hit*>(out)[ri] = h;
         }
     }
@@ -357,7 +351,7 @@ int32_t create_tlas(DeviceBvh const* const* meshes, uint32_t num_meshes, uint32_
 }
 
 int32_t trace_instances_launch(Tlas const& t, uint32_t num_cus, mi_ray const* rays, uint32_t n,
-                               void* out, bool any_hit, hipStream_t stream) {
+                               uint32_t const* n_dev, void* out, bool any_hit, hipStream_t stream) {
     if (0 == n) {
         return MI_OK;
     }
@@ -371,9 +365,11 @@ int32_t trace_instances_launch(Tlas const& t, uint32_t num_cus, mi_ray const* ra
     uint32_t const blocks = uint32_t(std::min<uint64_t>(wanted, uint64_t(num_cus) * 8));
     size_t const   lds    = size_t(Stack_entries) * Block * sizeof(uint32_t);
     if (any_hit) {
-        hipLaunchKernelGGL(trace_instances<true>, dim3(blocks), dim3(Block), lds, stream, v, rays, n, out);
+        hipLaunchKernelGGL(trace_instances<true>, dim3(blocks), dim3(Block), lds, stream, v, rays, n,
+                           n_dev, out);
     } else {
-        hipLaunchKernelGGL(trace_instances<false>, dim3(blocks), dim3(Block), lds, stream, v, rays, n, out);
+        hipLaunchKernelGGL(trace_instances<false>, dim3(blocks), dim3(Block), lds, stream, v, rays, n,
+                           n_dev, out);
     }
     if (hipError_t const e = hipGetLastError(); e) {
         return fail(MI_ERR_HIP, "instance trace: %s", hipGetErrorString(e));

@@ -3,6 +3,7 @@
 
 #include "bvh_build.hpp"
 #include "common.hpp"
+#include "device_math.hpp"
 #include "trace.hpp"
 
 #include <hip/hip_runtime.h>
@@ -21,6 +22,29 @@ struct Instance_record {
     uint32_t mesh;
     uint32_t pad;
 };
+
+namespace dev {
+
+// transform_point / transform_vector (base/math/matrix4x4.inl:319-342) with
+// the rounding the reference build performs: fma(z, r2, fma(x, r0, y r1))
+// (+ r3), rows r0..r3 of a row-vector 4x4
+__device__ __forceinline__ F3 xform_point(float const* m, F3 v) {
+    F3 r;
+    r.x = __builtin_fmaf(v.z, m[8], __builtin_fmaf(v.x, m[0], v.y * m[4])) + m[12];
+    r.y = __builtin_fmaf(v.z, m[9], __builtin_fmaf(v.x, m[1], v.y * m[5])) + m[13];
+    r.z = __builtin_fmaf(v.z, m[10], __builtin_fmaf(v.x, m[2], v.y * m[6])) + m[14];
+    return r;
+}
+
+__device__ __forceinline__ F3 xform_vector(float const* m, F3 v) {
+    F3 r;
+    r.x = __builtin_fmaf(v.z, m[8], __builtin_fmaf(v.x, m[0], v.y * m[4]));
+    r.y = __builtin_fmaf(v.z, m[9], __builtin_fmaf(v.x, m[1], v.y * m[5]));
+    r.z = __builtin_fmaf(v.z, m[10], __builtin_fmaf(v.x, m[2], v.y * m[6]));
+    return r;
+}
+
+}  // namespace dev
 
 struct Mesh_view {
     float4 const* nodes;
@@ -56,8 +80,10 @@ int32_t create_tlas(DeviceBvh const* const* meshes, uint32_t num_meshes, uint32_
                     uint32_t num_instances, mi_bvh_settings const& settings, hipStream_t stream,
                     Tlas& out);
 
-// Closest (out: mi_instance_hit per ray) or any hit (out: uint32 per ray).
+// Closest (out: mi_instance_hit per ray, pad[0] = the mesh tree's slot) or
+// any hit (out: uint32 per ray).  n_dev (optional): a ray count in device
+// memory, at most n (capturable).
 int32_t trace_instances_launch(Tlas const& t, uint32_t num_cus, mi_ray const* rays, uint32_t n,
-                               void* out, bool any_hit, hipStream_t stream);
+                               uint32_t const* n_dev, void* out, bool any_hit, hipStream_t stream);
 
 }  // namespace mi

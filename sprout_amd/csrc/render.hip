@@ -142,6 +142,13 @@ struct Wave {
     uint32_t const*        tri_light;  // light of an emissive triangle
     uint32_t               split_lights;
     dev::Medium_view       medium;
+    // instanced scenes
+    uint32_t               instanced;
+    uint32_t               emitter_instance;  // the instance of the world triangles
+    mi_instance_hit const* ihits;
+    Tlas_view              tlas;
+    float const*           inst_o2w;
+    uint32_t const*        inst_material;
     uint32_t*              medium_event;  // per queue entry: Medium_*
     float*                 medium_p;      // scattering point per queue entry
     float const*           e_table;
@@ -206,6 +213,69 @@ __device__ __forceinline__ V3 offset_ray(V3 p, V3 n) {
 __device__ __forceinline__ V3 vertex(Wave const& w, uint32_t i) {
     float const* p = w.positions + 3 * size_t(i);
     return {p[0], p[1], p[2]};
+}
+
+// A closest hit of either scene kind: world triangles only, or instances.
+struct Hit {
+    float    t;
+    float    u;
+    float    v;
+    uint32_t triangle;  // Null on a miss
+    uint32_t instance;  // Null in a scene of world triangles
+    uint32_t slot;      // the triangle's slot in its mesh tree (instances)
+};
+
+__device__ __forceinline__ Hit load_hit(Wave const& w, uint32_t i) {
+    if (w.instanced) {
+        mi_instance_hit const h = w.ihits[i];
+        return {h.t, h.u, h.v, h.triangle, h.instance, h.slot};
+    }
+    mi_hit const h = w.hits[i];
+    return {h.t, h.u, h.v, h.triangle, dev::Null, dev::Null};
+}
+
+// Point and geometric normal of a (non-missed) hit, its material and light.
+struct Surface_point {
+    V3       p;
+    V3       n;
+    uint32_t material;
+    uint32_t light;  // Null unless an emitter of the light tree
+};
+
+__device__ Surface_point surface_point(Wave const& w, Hit const& h) {
+    Surface_point sp;
+    if (dev::Null == h.instance || h.instance == w.emitter_instance) {
+        // a world triangle (the instance of the world triangles is the identity)
+        uint32_t const tri = h.triangle;
+        V3 const       a   = vertex(w, w.indices[3 * size_t(tri) + 0]);
+        V3 const       b   = vertex(w, w.indices[3 * size_t(tri) + 1]);
+        V3 const       c   = vertex(w, w.indices[3 * size_t(tri) + 2]);
+        float const    bw  = (1.f - h.u) - h.v;
+        sp.p               = (bw * a + h.u * b) + h.v * c;
+        sp.n               = dev::normalize(dev::cross(b - a, c - a));
+        sp.material        = w.tri_material[tri];
+        sp.light           = w.tri_light[tri];
+        return sp;
+    }
+    // a mesh instance: the slot's triangle in object space, then to world
+    // space (Mesh::intersect, triangle_mesh.cpp:379-392); normals through the
+    // matrix's upper 3x3, then normalized (similarity transforms)
+    Instance_record const& ir = w.tlas.instances[h.instance];
+    float4 const*          tr = w.tlas.meshes[ir.mesh].tris + 3 * size_t(h.slot);
+    float4 const           a  = tr[0];
+    float4 const           e1 = tr[1];
+    float4 const           e2 = tr[2];
+    dev::F3 const po = {(a.x + h.u * e1.x) + h.v * e2.x, (a.y + h.u * e1.y) + h.v * e2.y,
+                        (a.z + h.u * e1.z) + h.v * e2.z};
+    float const*  m  = w.inst_o2w + 16 * size_t(h.instance);
+    dev::F3 const pw = dev::xform_point(m, po);
+    V3 const      no = dev::cross(v3(e1.x, e1.y, e1.z), v3(e2.x, e2.y, e2.z));
+    dev::F3 const nw = dev::xform_vector(m, {no.x, no.y, no.z});
+    sp.p             = v3(pw.x, pw.y, pw.z);
+    sp.n             = dev::normalize(v3(nw.x, nw.y, nw.z));
+    sp.material      = w.inst_material[h.instance];
+    sp.light         = dev::Null;
+    return sp;
 }
 
 __device__ __forceinline__ float max_component(float const t[3]) {
@@ -395,7 +465,7 @@ __global__ void __launch_bounds__(256) medium_track(Wave w, uint32_t in_q, uint3
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         mi_ray const r     = w.rays[in_q][i];
-        mi_hit const h     = w.hits[i];
+        Hit const    h     = load_hit(w, i);
         V3 const     o     = v3(r.origin[0], r.origin[1], r.origin[2]);
         V3 const     d     = v3(r.direction[0], r.direction[1], r.direction[2]);
         uint32_t     event = Medium_none;
@@ -430,23 +500,18 @@ __global__ void __launch_bounds__(256) guide_inputs(Wave w, uint32_t in_q) {
     uint32_t const count  = w.counters[C_queue0 + in_q];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        mi_hit const h  = w.hits[i];
-        float*       xo = w.guide_x + 9 * size_t(i);
+        Hit const h  = load_hit(w, i);
+        float*    xo = w.guide_x + 9 * size_t(i);
         if (dev::Null == h.triangle) {
             for (int k = 0; k < 9; ++k) {
                 xo[k] = 0.5f;
             }
             continue;
         }
-        mi_ray const   r   = w.rays[in_q][i];
-        uint32_t const tri = h.triangle;
-        V3 const       a   = vertex(w, w.indices[3 * size_t(tri) + 0]);
-        V3 const       b   = vertex(w, w.indices[3 * size_t(tri) + 1]);
-        V3 const       c   = vertex(w, w.indices[3 * size_t(tri) + 2]);
-        float const    bw  = (1.f - h.u) - h.v;
-        V3 const       hp  = (bw * a + h.u * b) + h.v * c;
-        V3 const       n   = dev::normalize(dev::cross(b - a, c - a));
-        float const    p3[3] = {hp.x, hp.y, hp.z};
+        mi_ray const        r     = w.rays[in_q][i];
+        Surface_point const sp    = surface_point(w, h);
+        V3 const            n     = sp.n;
+        float const         p3[3] = {sp.p.x, sp.p.y, sp.p.z};
         for (int k = 0; k < 3; ++k) {
             xo[k] = dev::clampf((p3[k] - w.box_lo[k]) * w.box_scale[k], 0.f, 1.f);
         }
@@ -492,7 +557,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
         if (i < count) {
             p              = w.ray_path[in_q][i];
             path           = w.paths[p];
-            mi_hit const h = w.hits[i];
+            Hit const h = load_hit(w, i);
 
             if (Guided && dev::Null != path.record) {
                 // radiance before this vertex: what follows is the previous sample's
@@ -543,18 +608,13 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             } else if (Medium_absorbed == event) {
                 // Event::Abort: the tracking weight fell below Abort_epsilon
             } else if (dev::Null != h.triangle) {
-                mi_ray const   r   = w.rays[in_q][i];
-                uint32_t const tri = h.triangle;
+                mi_ray const        r  = w.rays[in_q][i];
+                Surface_point const sp = surface_point(w, h);
+                hit_p                  = sp.p;
+                geo_n                  = sp.n;
+                V3 const wo            = v3(-r.direction[0], -r.direction[1], -r.direction[2]);
 
-                V3 const    a  = vertex(w, w.indices[3 * size_t(tri) + 0]);
-                V3 const    b  = vertex(w, w.indices[3 * size_t(tri) + 1]);
-                V3 const    c  = vertex(w, w.indices[3 * size_t(tri) + 2]);
-                float const bw = (1.f - h.u) - h.v;
-                hit_p          = (bw * a + h.u * b) + h.v * c;
-                geo_n          = dev::normalize(dev::cross(b - a, c - a));
-                V3 const wo    = v3(-r.direction[0], -r.direction[1], -r.direction[2]);
-
-                Render_material const m = w.materials[w.tri_material[tri]];
+                Render_material const m = w.materials[sp.material];
 
                 live = true;
 
@@ -563,7 +623,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                     float const cos_l = dev::dot(geo_n, wo);
                     if (cos_l > 0.f) {
                         float weight = 1.f;
-                        uint32_t const l = w.tri_light[tri];
+                        uint32_t const l = sp.light;
                         // an emitter light sampling cannot pick (zero power)
                         // is found by BSDF sampling alone: weight 1
                         if (0 != depth && !(path.flags & Flag_singular) && dev::Null != l) {

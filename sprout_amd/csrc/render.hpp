@@ -3,6 +3,7 @@
 
 #include "common.hpp"
 #include "guide.hpp"
+#include "instances.hpp"
 #include "light_tree.hpp"
 #include "medium.hpp"
 #include "trace.hpp"
@@ -42,6 +43,13 @@ struct RenderScene {
     uint32_t*        medium_nodes   = nullptr;  // majorant gridtree
     Medium_cm*       medium_data    = nullptr;
     dev::Medium_view medium{};
+    // instanced scenes: the mesh instances and, last, the triangles above
+    // (emitters and other world-space triangles) as one more instance
+    Tlas      tlas;
+    float*    inst_o2w         = nullptr;  // 16 per instance
+    uint32_t* inst_material    = nullptr;  // per instance; Null for the world triangles
+    uint32_t  emitter_instance = 0xFFFFFFFFu;
+    bool      instanced        = false;
 
     uint32_t num_vertices  = 0;
     uint32_t num_triangles = 0;
@@ -54,6 +62,18 @@ struct RenderScene {
     void release();
     void release_medium();
 };
+
+// An instanced scene: num_instances placements of mesh trees (row-vector
+// 4x4 matrices, one material each, not emissive) plus the world-space
+// triangles of the arrays (emitters among them), which form the light tree
+// as in create_scene and are traced as one more instance.
+int32_t create_scene_instanced(DeviceBvh const* const* meshes, float const* world_to_object,
+                               float const* object_to_world, uint32_t const* instance_materials,
+                               uint32_t num_instances, float const* positions,
+                               uint32_t num_vertices, uint32_t const* indices,
+                               uint32_t num_triangles, uint32_t const* tri_material,
+                               mi_material const* materials, uint32_t num_materials,
+                               mi_bvh_settings const& settings, RenderScene& out);
 
 // Density grid (nx * ny * nz floats, x fastest, host memory) filling the box
 // [box_lo, box_hi]; cc_a / cc_s: absorption and scattering coefficients per
@@ -70,6 +90,7 @@ struct RenderState {
     mi_ray*      rays[2]     = {nullptr, nullptr};
     uint32_t*    ray_path[2] = {nullptr, nullptr};
     mi_hit*      hits        = nullptr;
+    mi_instance_hit* inst_hits = nullptr;  // closest hits of instanced scenes
     mi_ray*      shadow_rays = nullptr;
     void*        shadows     = nullptr;
     uint32_t*    occluded    = nullptr;
