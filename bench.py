@@ -205,19 +205,26 @@ def cloud_render(frames: int, cpu: bool):
     return out
 
 
-def batch_render(frames: int):
+def batch_render(frames: int, instanced: bool):
     """Batched path tracing: 64 placements of 4 distinct 20 480-triangle
     displaced spheres with per-instance Substitute materials, a ground plane
-    and 10 000 emissive triangles in the light tree (1.32 M triangles, baked
-    into world space); 1920 x 1080, 4 spp per frame in waves of 1 spp,
-    Russian roulette from depth 3, max depth 16."""
+    and 10 000 emissive triangles in the light tree (1.32 M triangles);
+    instanced (a tree per mesh under a top-level tree) or baked into one
+    world-space soup; 1920 x 1080, 4 spp per frame in waves of 1 spp, Russian
+    roulette from depth 3, max depth 16."""
     import torch
 
     from sprout_amd import render, scenes
 
     t0 = time.perf_counter()
-    pos, idx, tmat, mats = scenes.batch_scene()
-    sc = render.Scene(pos, idx, tmat, mats)
+    if instanced:
+        b = scenes.batch_scene_instanced()
+        sc = render.InstancedScene(b["meshes"], b["mesh_of"], b["world_to_object"],
+                                   b["object_to_world"], b["instance_materials"], b["positions"],
+                                   b["indices"], b["triangle_materials"], b["materials"])
+    else:
+        pos, idx, tmat, mats = scenes.batch_scene()
+        sc = render.Scene(pos, idx, tmat, mats)
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
     cam = render.camera(1920, 1080, **scenes.BATCH_CAMERA)
@@ -234,8 +241,9 @@ def batch_render(frames: int):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     sc.release()
+    form = "instanced: 4 mesh trees under a top-level tree" if instanced else "baked, one tree"
     return {
-        "workload": "64 instances x 20480-tri spheres (baked, 1.32M tris), 10000 lights, "
+        "workload": f"64 instances x 20480-tri spheres ({form}, 1.32M tris), 10000 lights, "
                     "1920x1080, 4 spp per frame, RR from depth 3, max depth 16",
         "scene_setup_s": round(setup, 2),
         "ms_per_frame": round(wall / frames * 1e3, 2),
@@ -395,7 +403,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--guide-seconds", type=float, default=2.0,
                     help="time budget of each side of the guided vs plain comparison")
-    ap.add_argument("--part", choices=["all", "room", "volume", "guiding", "instances", "batch"],
+    ap.add_argument("--part", choices=["all", "room", "volume", "guiding", "instances", "batch",
+                                       "batch_instanced"],
                     default="all",
                     help="run one workload only and print its figures (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -417,7 +426,8 @@ def main() -> int:
 
     if args.part != "all":
         parts = {"room": lambda: room_render(5, False), "volume": lambda: cloud_render(3, False),
-                 "batch": lambda: batch_render(2),
+                 "batch": lambda: batch_render(2, False),
+                 "batch_instanced": lambda: batch_render(2, True),
                  "guiding": lambda: slit_equal_time(args.guide_seconds),
                  "instances": lambda: instance_trace(args.steps, args.warmup)}
         print(json.dumps({"part": args.part, "figures": parts[args.part]()}), flush=True)
@@ -539,7 +549,8 @@ def main() -> int:
     log("box room path tracing")
     line["extra"]["room_render"] = room_render(5, not args.no_cpu_baseline)
     log("batched path tracing")
-    line["extra"]["batch_render"] = batch_render(2)
+    line["extra"]["batch_render"] = batch_render(2, False)
+    line["extra"]["batch_render_instanced"] = batch_render(2, True)
     log("instanced meshes")
     line["extra"]["instances"] = instance_trace(args.steps, args.warmup, not args.no_cpu_baseline)
     log("fBm cloud path tracing")

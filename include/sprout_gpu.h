@@ -246,6 +246,38 @@ int32_t mi_scene_create(float const* positions, uint32_t num_vertices, uint32_t 
                         mi_material const* materials, uint32_t num_materials,
                         mi_bvh_settings const* bvh_settings, uint32_t* scene);
 
+/* A mesh for mi_scene_create_instanced (HOST arrays). */
+typedef struct mi_mesh {
+    float const*    positions; /* 3 floats per vertex */
+    uint32_t        num_vertices;
+    uint32_t const* indices;   /* 3 per triangle */
+    uint32_t        num_triangles;
+} mi_mesh;
+
+/* Creates an instanced scene: num_instances placements of the meshes
+ * (instance i shows meshes[instance_mesh[i]] with the row-vector 4x4
+ * matrices world_to_object / object_to_world, as mi_instances_create, and
+ * the one material instance_materials[i], which may not be emissive), plus
+ * the world-space triangles of positions / indices with their materials as in
+ * mi_scene_create (the emissive ones form the light tree; at least one
+ * triangle).  Each mesh gets its own tree, and the top-level tree spans the
+ * instances and the world triangles (one more instance, identity matrices).
+ * The hit point of an instance is interpolated in object space and
+ * transformed to world space, its normal transformed and normalized
+ * (Mesh::intersect, triangle_mesh.cpp:364-392; similarity transforms).
+ * Mirrors su_create_triangle_mesh_async + su_create_prop with one material +
+ * su_entity_set_transformation (capi.cpp:378-456, 521-541) and the prop tree of
+ * Scene::compile (scene.cpp:216-267).  HOST arrays throughout. */
+int32_t mi_scene_create_instanced(mi_mesh const* meshes, uint32_t num_meshes,
+                                  uint32_t const* instance_mesh, float const* world_to_object,
+                                  float const* object_to_world,
+                                  uint32_t const* instance_materials, uint32_t num_instances,
+                                  float const* positions, uint32_t num_vertices,
+                                  uint32_t const* indices, uint32_t num_triangles,
+                                  uint32_t const* triangle_materials, mi_material const* materials,
+                                  uint32_t num_materials, mi_bvh_settings const* bvh_settings,
+                                  uint32_t* scene);
+
 int32_t mi_scene_release(uint32_t scene);
 
 /* Renders and ADDS each pixel's radiance sum over the samples into accum,

@@ -263,6 +263,17 @@ typedef struct {
 
 enum { ORACLE_PT_MIS = 0, ORACLE_PT_BSDF = 1, ORACLE_PT_NEE = 2 };
 
+/* An instanced scene for oracle_render: tlas places the meshes and, as its
+ * instance `world` (identity matrices), the tree over the render's triangles,
+ * whose materials and lights apply to it; every other instance shows its
+ * mesh with the one material material[instance]. */
+typedef struct {
+    oracle_instances const* tlas;
+    float const*            o2w;      /* 16 per instance */
+    uint32_t const*         material; /* per instance */
+    uint32_t                world;
+} oracle_pt_instances;
+
 typedef struct {
     uint32_t samples_per_pixel;
     uint32_t first_sample;
@@ -273,6 +284,7 @@ typedef struct {
     uint32_t num_threads;
     int32_t  split_lights; /* split the light tree below depth 3 (sprout's Adaptive) */
     oracle_medium const* medium; /* NULL: none */
+    oracle_pt_instances const* instances; /* NULL: the triangles only */
 } oracle_pt_settings;
 
 /* Adds each pixel's radiance sum over the samples into accum (W * H * 3).

@@ -56,7 +56,7 @@ class _OraclePtSettings(ctypes.Structure):
                 ("min_bounces", ctypes.c_uint32), ("max_bounces", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64), ("mode", ctypes.c_int32),
                 ("num_threads", ctypes.c_uint32), ("split_lights", ctypes.c_int32),
-                ("medium", ctypes.c_void_p)]
+                ("medium", ctypes.c_void_p), ("instances", ctypes.c_void_p)]
 
 
 class _OracleGridtree(ctypes.Structure):
@@ -689,16 +689,44 @@ def offset_f(t) -> float:
     return float(lib().oracle_offset_f(float(t)))
 
 
+class _OraclePtInstances(ctypes.Structure):
+    _fields_ = [("tlas", ctypes.POINTER(_OracleInstances)), ("o2w", ctypes.c_void_p),
+                ("material", ctypes.c_void_p), ("world", ctypes.c_uint32)]
+
+
+class PtInstances:
+    """An instanced scene for render(): meshes [(positions, indices)] placed
+    by mesh_of / world_to_object / object_to_world with one material each,
+    plus the render's own triangles (positions, indices) as the last instance
+    with identity matrices; every tree is the oracle's SBVH."""
+
+    def __init__(self, meshes, mesh_of, world_to_object, object_to_world, materials, positions,
+                 indices):
+        trees = [(p, i, build(p, i)) for p, i in meshes]
+        trees.append((positions, indices, build(positions, indices)))
+        n = len(mesh_of)
+        eye = np.eye(4, dtype=np.float32).reshape(1, 16)
+        self.inst = Instances(trees, np.append(np.asarray(mesh_of, np.uint32), len(meshes)),
+                              np.vstack([np.asarray(world_to_object, np.float32).reshape(-1, 16), eye]),
+                              np.vstack([np.asarray(object_to_world, np.float32).reshape(-1, 16), eye]))
+        self.material = np.append(np.asarray(materials, np.uint32), np.uint32(0xFFFFFFFF))
+        self.world = n
+        self.c = _OraclePtInstances(ctypes.pointer(self.inst.c), _ptr(self.inst.o2w),
+                                    _ptr(self.material), n)
+
+
 def similarity_scale(g, depth) -> float:
     return float(lib().oracle_similarity_scale(g, depth))
 
 
 def render(positions, indices, triangle_materials, materials, width, height, spp, camera,
            tree: LightTree, first_sample=0, min_bounces=4, max_bounces=8, seed=1, mode=PT_MIS,
-           threads=None, split_lights=True, medium: Medium | None = None):
+           threads=None, split_lights=True, medium: Medium | None = None,
+           instances: PtInstances | None = None):
     """Radiance sums per pixel, (height, width, 3) float64, of the CPU path
     tracer (pt.c).  camera: dict with eye, target, up, fov_deg; tree: the
-    scene's light tree; medium: a participating medium or None."""
+    scene's light tree; medium: a participating medium or None; instances:
+    mesh instances beside the triangles, or None."""
     pos = np.ascontiguousarray(positions, np.float32)
     idx = np.ascontiguousarray(indices, np.uint32)
     tmat = np.ascontiguousarray(triangle_materials, np.uint32)
@@ -709,7 +737,8 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
     n = threads or min(os.cpu_count() or 1, 16)
     st = _OraclePtSettings(spp, first_sample, min_bounces, max_bounces, seed, mode, n,
                            1 if split_lights else 0,
-                           ctypes.cast(ctypes.pointer(medium.c), ctypes.c_void_p) if medium else None)
+                           ctypes.cast(ctypes.pointer(medium.c), ctypes.c_void_p) if medium else None,
+                           ctypes.cast(ctypes.pointer(instances.c), ctypes.c_void_p) if instances else None)
     accum = np.zeros((height, width, 3), np.float64)
     status = lib().oracle_render(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(tmat),
                                  _ptr(mats), len(mats), ctypes.byref(tree.c), ctypes.byref(cam),

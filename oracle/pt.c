@@ -151,16 +151,51 @@ static void tri_vertices(scene_t const* s, uint32_t t, v3* a, v3* b, v3* c) {
     *c = vertex(s, s->indices[3 * (size_t)t + 2]);
 }
 
-static int32_t closest(scene_t const* s, v3 o, v3 d, float max_t, uint32_t hit[4]) {
+/* hit: t, u, v, triangle, instance (SENTINEL without instances) */
+static int32_t closest(scene_t const* s, v3 o, v3 d, float max_t, uint32_t hit[5]) {
     float const ray[8] = {o.x, o.y, o.z, 0.f, d.x, d.y, d.z, max_t};
+    if (s->set->instances) {
+        return oracle_instances_trace(s->set->instances->tlas, ray, 1, 0, hit);
+    }
+    hit[4] = SENTINEL;
     return oracle_trace(s->bvh.nodes, s->bvh.slots, s->positions, s->indices, ray, 1, 0,
                         ORACLE_RCP_PINNED, hit, NULL, 1);
 }
 
 static int32_t occluded(scene_t const* s, v3 o, v3 d, float max_t, uint32_t* occ) {
     float const ray[8] = {o.x, o.y, o.z, 0.f, d.x, d.y, d.z, max_t};
+    if (s->set->instances) {
+        return oracle_instances_trace(s->set->instances->tlas, ray, 1, 1, occ);
+    }
     return oracle_trace(s->bvh.nodes, s->bvh.slots, s->positions, s->indices, ray, 1, 1,
                         ORACLE_RCP_PINNED, occ, NULL, 1);
+}
+
+/* The hit of a mesh instance: the triangle interpolated in object space and
+ * moved to world space, its normal normalized, transformed and normalized
+ * again (Mesh::intersect, triangle_mesh.cpp:379-392; Indexed_data::normal,
+ * triangle_bvh_indexed_data.inl:282-293; a normalized transform of the normal
+ * in place of the rotation, equal for similarity transforms). */
+static void instance_point(oracle_pt_instances const* pi, uint32_t inst, uint32_t tri, float hu,
+                           float hv, v3* p, v3* n) {
+    oracle_mesh const* mesh = &pi->tlas->meshes[pi->tlas->mesh_of[inst]];
+    float const*       pa   = mesh->positions + 3 * (size_t)mesh->indices[3 * (size_t)tri + 0];
+    float const*       pb   = mesh->positions + 3 * (size_t)mesh->indices[3 * (size_t)tri + 1];
+    float const*       pc   = mesh->positions + 3 * (size_t)mesh->indices[3 * (size_t)tri + 2];
+    v3 const           a    = mk(pa[0], pa[1], pa[2]);
+    v3 const           b    = mk(pb[0], pb[1], pb[2]);
+    v3 const           c    = mk(pc[0], pc[1], pc[2]);
+    float const        bw   = (1.f - hu) - hv;
+    v3 const           po   = add(add(scale(bw, a), scale(hu, b)), scale(hv, c));
+    v3 const           no   = normalize(cross(sub(b, a), sub(c, a)));
+    float const*       m    = pi->o2w + 16 * (size_t)inst;
+    float const        pi3[3] = {po.x, po.y, po.z};
+    float const        ni3[3] = {no.x, no.y, no.z};
+    float              pw[3], nw[3];
+    oracle_transform_point(m, pi3, pw);
+    oracle_transform_vector(m, ni3, nw);
+    *p = mk(pw[0], pw[1], pw[2]);
+    *n = normalize(mk(nw[0], nw[1], nw[2]));
 }
 
 #define KIND_PHASE 0x100 /* scattering vertex in the medium */
@@ -315,7 +350,7 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
     shadow_t shadows[ORACLE_MAX_LIGHTS];
 
     for (uint32_t depth = 0;; ++depth) {
-        uint32_t hit[4];
+        uint32_t hit[5];
         if (closest(s, o, d, RAY_MAX_T, hit)) {
             return ORACLE_ERR_STACK;
         }
@@ -323,7 +358,8 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
         memcpy(&t, &hit[0], 4);
         memcpy(&hu, &hit[1], 4);
         memcpy(&hv, &hit[2], 4);
-        uint32_t const tri = hit[3];
+        uint32_t const tri  = hit[3];
+        uint32_t const inst = hit[4];
 
         int   scatter = 0;
         v3    hit_p   = mk(0.f, 0.f, 0.f);
@@ -358,19 +394,27 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
         v3       p_o;
 
         if (!scatter) {
-            v3 a, b, c;
-            tri_vertices(s, tri, &a, &b, &c);
-            float const bw = (1.f - hu) - hv;
-            hit_p          = add(add(scale(bw, a), scale(hu, b)), scale(hv, c));
-            geo_n          = normalize(cross(sub(b, a), sub(c, a)));
-
-            oracle_material const* m = &s->materials[s->tri_material[tri]];
+            oracle_pt_instances const* pi = set->instances;
+            oracle_material const*     m;
+            uint32_t                   light = SENTINEL;
+            if (pi && inst != pi->world) {
+                instance_point(pi, inst, tri, hu, hv, &hit_p, &geo_n);
+                m = &s->materials[pi->material[inst]];
+            } else {
+                v3 a, b, c;
+                tri_vertices(s, tri, &a, &b, &c);
+                float const bw = (1.f - hu) - hv;
+                hit_p          = add(add(scale(bw, a), scale(hu, b)), scale(hv, c));
+                geo_n          = normalize(cross(sub(b, a), sub(c, a)));
+                m              = &s->materials[s->tri_material[tri]];
+                light          = s->tri_light[tri];
+            }
 
             if (KIND_LIGHT == m->kind) {
                 float const cos_l = dot(geo_n, wo);
                 if (cos_l > 0.f) {
                     float          weight = 1.f;
-                    uint32_t const l      = s->tri_light[tri];
+                    uint32_t const l      = light;
                     if (0 != depth && !singular && SENTINEL != l) {
                         if (ORACLE_PT_MIS == mode) {
                             int const   split     = set->split_lights && depth < 3;

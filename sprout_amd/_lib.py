@@ -65,6 +65,11 @@ class MiMaterial(ctypes.Structure):
                 ("ior", ctypes.c_float), ("kind", ctypes.c_uint32)]
 
 
+class MiMesh(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("num_vertices", ctypes.c_uint32),
+                ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint32)]
+
+
 class MiCamera(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("eye", ctypes.c_float * 3), ("target", ctypes.c_float * 3),
@@ -125,6 +130,9 @@ SIGNATURES = {
     "mi_gen_bounce_rays": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp, _u32]),
     "mi_scene_create": (_i32, [_vp, _u32, _vp, _u32, _vp, _vp, _u32,
                                ctypes.POINTER(MiBvhSettings), _pu32]),
+    "mi_scene_create_instanced": (_i32, [ctypes.POINTER(MiMesh), _u32, _vp, _vp, _vp, _vp, _u32,
+                                         _vp, _u32, _vp, _u32, _vp, _vp, _u32,
+                                         ctypes.POINTER(MiBvhSettings), _pu32]),
     "mi_scene_release": (_i32, [_u32]),
     "mi_render": (_i32, [_u32, ctypes.POINTER(MiCamera), ctypes.POINTER(MiRenderSettings), _vp,
                          _vp, ctypes.POINTER(MiRenderStats)]),

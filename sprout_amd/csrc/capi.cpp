@@ -416,6 +416,51 @@ int32_t mi_scene_create(float const* positions, uint32_t num_vertices, uint32_t 
     return MI_OK;
 }
 
+int32_t mi_scene_create_instanced(mi_mesh const* meshes, uint32_t num_meshes,
+                                  uint32_t const* instance_mesh, float const* world_to_object,
+                                  float const* object_to_world,
+                                  uint32_t const* instance_materials, uint32_t num_instances,
+                                  float const* positions, uint32_t num_vertices,
+                                  uint32_t const* indices, uint32_t num_triangles,
+                                  uint32_t const* triangle_materials, mi_material const* materials,
+                                  uint32_t num_materials, mi_bvh_settings const* bvh_settings,
+                                  uint32_t* scene) {
+    clear_error();
+    if (!context) {
+        return fail(MI_ERR_NO_CONTEXT, "mi_scene_create_instanced: call mi_init first");
+    }
+    if (!meshes || 0 == num_meshes || !instance_mesh || !world_to_object || !object_to_world ||
+        !instance_materials || 0 == num_instances || !positions || !indices ||
+        !triangle_materials || !materials || !scene || 0 == num_vertices || 0 == num_triangles ||
+        0 == num_materials) {
+        return fail(MI_ERR_ARGS, "mi_scene_create_instanced: bad arguments");
+    }
+    mi_bvh_settings const st = bvh_settings ? *bvh_settings : mi_bvh_settings{16, 64, 4};
+    if (0 == st.num_slices || 0 == st.max_primitives || st.max_primitives > 0xFF) {
+        return fail(MI_ERR_ARGS, "mi_scene_create_instanced: bad tree settings");
+    }
+    RenderScene out;
+    if (int32_t const r = create_scene_instanced(meshes, num_meshes, instance_mesh, world_to_object,
+                                                 object_to_world, instance_materials, num_instances,
+                                                 positions, num_vertices, indices, num_triangles,
+                                                 triangle_materials, materials, num_materials, st,
+                                                 out);
+        r) {
+        return r;
+    }
+    auto* sc = new RenderScene(out);
+    for (uint32_t i = 0; i < context->scenes.size(); ++i) {
+        if (!context->scenes[i]) {
+            context->scenes[i] = sc;
+            *scene             = i;
+            return MI_OK;
+        }
+    }
+    context->scenes.push_back(sc);
+    *scene = uint32_t(context->scenes.size() - 1);
+    return MI_OK;
+}
+
 int32_t mi_scene_release(uint32_t scene) {
     clear_error();
     RenderScene* sc = lookup_scene(scene);

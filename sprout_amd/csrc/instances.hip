@@ -355,13 +355,8 @@ int32_t trace_instances_launch(Tlas const& t, uint32_t num_cus, mi_ray const* ra
     if (0 == n) {
         return MI_OK;
     }
-    Tlas_view v;
-    v.nodes     = t.tree.nodes;
-    v.slots     = t.tree.tris;
-    v.num_nodes = t.tree.num_nodes;
-    v.instances = t.instances;
-    v.meshes    = t.meshes;
-    uint64_t const wanted = (uint64_t(n) + Block - 1) / Block;
+    Tlas_view const v      = tlas_view(t);
+    uint64_t const  wanted = (uint64_t(n) + Block - 1) / Block;
     uint32_t const blocks = uint32_t(std::min<uint64_t>(wanted, uint64_t(num_cus) * 8));
     size_t const   lds    = size_t(Stack_entries) * Block * sizeof(uint32_t);
     if (any_hit) {

@@ -72,6 +72,16 @@ struct Tlas {
     void release();
 };
 
+inline Tlas_view tlas_view(Tlas const& t) {
+    Tlas_view v;
+    v.nodes     = t.tree.nodes;
+    v.slots     = t.tree.tris;
+    v.num_nodes = t.tree.num_nodes;
+    v.instances = t.instances;
+    v.meshes    = t.meshes;
+    return v;
+}
+
 // meshes[k]: device trees; instance i uses meshes[mesh_of[i]] with the
 // row-vector 4x4 matrices world_to_object / object_to_world (16 floats each,
 // host).  Builds the top-level tree on the device.

@@ -147,6 +147,7 @@ struct Wave {
     uint32_t               emitter_instance;  // the instance of the world triangles
     mi_instance_hit const* ihits;
     Tlas_view              tlas;
+    Mesh_geometry const*   mesh_geo;
     float const*           inst_o2w;
     uint32_t const*        inst_material;
     uint32_t*              medium_event;  // per queue entry: Medium_*
@@ -257,19 +258,23 @@ __device__ Surface_point surface_point(Wave const& w, Hit const& h) {
         sp.light           = w.tri_light[tri];
         return sp;
     }
-    // a mesh instance: the slot's triangle in object space, then to world
-    // space (Mesh::intersect, triangle_mesh.cpp:379-392); normals through the
-    // matrix's upper 3x3, then normalized (similarity transforms)
-    Instance_record const& ir = w.tlas.instances[h.instance];
-    float4 const*          tr = w.tlas.meshes[ir.mesh].tris + 3 * size_t(h.slot);
-    float4 const           a  = tr[0];
-    float4 const           e1 = tr[1];
-    float4 const           e2 = tr[2];
-    dev::F3 const po = {(a.x + h.u * e1.x) + h.v * e2.x, (a.y + h.u * e1.y) + h.v * e2.y,
-                        (a.z + h.u * e1.z) + h.v * e2.z};
+    // a mesh instance: the triangle interpolated in object space and moved to
+    // world space, its normal normalized, transformed and normalized again
+    // (Mesh::intersect, triangle_mesh.cpp:379-392; Indexed_data::normal,
+    // triangle_bvh_indexed_data.inl:282-293)
+    Mesh_geometry const g  = w.mesh_geo[w.tlas.instances[h.instance].mesh];
+    auto const          at = [&g](uint32_t k) {
+        float const* q = g.positions + 3 * size_t(k);
+        return v3(q[0], q[1], q[2]);
+    };
+    V3 const      a  = at(g.indices[3 * size_t(h.triangle) + 0]);
+    V3 const      b  = at(g.indices[3 * size_t(h.triangle) + 1]);
+    V3 const      c  = at(g.indices[3 * size_t(h.triangle) + 2]);
+    float const   bw = (1.f - h.u) - h.v;
+    V3 const      po = (bw * a + h.u * b) + h.v * c;
+    V3 const      no = dev::normalize(dev::cross(b - a, c - a));
     float const*  m  = w.inst_o2w + 16 * size_t(h.instance);
-    dev::F3 const pw = dev::xform_point(m, po);
-    V3 const      no = dev::cross(v3(e1.x, e1.y, e1.z), v3(e2.x, e2.y, e2.z));
+    dev::F3 const pw = dev::xform_point(m, {po.x, po.y, po.z});
     dev::F3 const nw = dev::xform_vector(m, {no.x, no.y, no.z});
     sp.p             = v3(pw.x, pw.y, pw.z);
     sp.n             = dev::normalize(v3(nw.x, nw.y, nw.z));
@@ -1286,6 +1291,16 @@ void RenderScene::release() {
     (void)hipFree(light_infos);
     (void)hipFree(tri_light);
     release_medium();
+    tlas.release();
+    for (DeviceBvh& t : mesh_trees) {
+        t.release();
+    }
+    for (void* p : mesh_buffers) {
+        (void)hipFree(p);
+    }
+    (void)hipFree(mesh_geo);
+    (void)hipFree(inst_o2w);
+    (void)hipFree(inst_material);
     *this = RenderScene();
 }
 
@@ -1526,6 +1541,122 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
     return MI_OK;
 }
 
+int32_t create_scene_instanced(mi_mesh const* meshes, uint32_t num_meshes,
+                               uint32_t const* instance_mesh, float const* world_to_object,
+                               float const* object_to_world, uint32_t const* instance_materials,
+                               uint32_t num_instances, float const* positions,
+                               uint32_t num_vertices, uint32_t const* indices,
+                               uint32_t num_triangles, uint32_t const* tri_material,
+                               mi_material const* materials, uint32_t num_materials,
+                               mi_bvh_settings const& settings, RenderScene& out) {
+    for (uint32_t k = 0; k < num_meshes; ++k) {
+        mi_mesh const& m = meshes[k];
+        if (!m.positions || !m.indices || 0 == m.num_vertices || 0 == m.num_triangles) {
+            return fail(MI_ERR_ARGS, "mi_scene_create_instanced: mesh %u is empty", k);
+        }
+        for (uint64_t i = 0; i < 3ull * m.num_triangles; ++i) {
+            if (m.indices[i] >= m.num_vertices) {
+                return fail(MI_ERR_ARGS, "mi_scene_create_instanced: mesh %u: index %llu is %u, of %u vertices",
+                            k, (unsigned long long)i, m.indices[i], m.num_vertices);
+            }
+        }
+    }
+    for (uint32_t i = 0; i < num_instances; ++i) {
+        if (instance_mesh[i] >= num_meshes) {
+            return fail(MI_ERR_ARGS, "mi_scene_create_instanced: instance %u: mesh %u of %u", i,
+                        instance_mesh[i], num_meshes);
+        }
+        if (instance_materials[i] >= num_materials) {
+            return fail(MI_ERR_ARGS, "mi_scene_create_instanced: instance %u: material %u of %u", i,
+                        instance_materials[i], num_materials);
+        }
+        if (is_light(materials[instance_materials[i]])) {
+            return fail(MI_ERR_ARGS, "mi_scene_create_instanced: instance %u: emissive material %u",
+                        i, instance_materials[i]);
+        }
+    }
+
+    // the world triangles with their light tree, as a scene of their own
+    RenderScene s;
+    if (int32_t const st = create_scene(positions, num_vertices, indices, num_triangles, tri_material,
+                                        materials, num_materials, settings, s);
+        st) {
+        return st;
+    }
+    auto failed = [&s](int32_t st) {
+        s.release();
+        return st;
+    };
+
+    // the meshes: vertex data on the device and one tree each
+    std::vector<Mesh_geometry> geo(num_meshes);
+    s.mesh_trees.resize(num_meshes);
+    for (uint32_t k = 0; k < num_meshes; ++k) {
+        mi_mesh const& m   = meshes[k];
+        float*         pos = nullptr;
+        uint32_t*      idx = nullptr;
+        hipError_t     e   = hipMalloc(&pos, size_t(m.num_vertices) * 3 * sizeof(float));
+        if (!e) s.mesh_buffers.push_back(pos);
+        if (!e) e = hipMalloc(&idx, size_t(m.num_triangles) * 3 * sizeof(uint32_t));
+        if (!e) s.mesh_buffers.push_back(idx);
+        if (!e) e = hipMemcpy(pos, m.positions, size_t(m.num_vertices) * 3 * sizeof(float), hipMemcpyHostToDevice);
+        if (!e) e = hipMemcpy(idx, m.indices, size_t(m.num_triangles) * 3 * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e) {
+            return failed(fail(MI_ERR_HIP, "mi_scene_create_instanced: mesh %u: %s", k, hipGetErrorString(e)));
+        }
+        geo[k] = {pos, idx};
+        if (int32_t const st = build_bvh(pos, m.num_vertices, 3, idx, m.num_triangles, settings,
+                                         nullptr, s.mesh_trees[k]);
+            st) {
+            return failed(st);
+        }
+    }
+
+    // the top-level tree over the instances and, last, the world triangles
+    uint32_t const                n_all = num_instances + 1;
+    std::vector<DeviceBvh const*> trees(num_meshes + 1);
+    for (uint32_t k = 0; k < num_meshes; ++k) {
+        trees[k] = &s.mesh_trees[k];
+    }
+    trees[num_meshes] = &s.bvh;
+    std::vector<uint32_t> mesh_of(instance_mesh, instance_mesh + num_instances);
+    mesh_of.push_back(num_meshes);
+    std::vector<float> w2o(world_to_object, world_to_object + 16 * size_t(num_instances));
+    std::vector<float> o2w(object_to_world, object_to_world + 16 * size_t(num_instances));
+    for (int k = 0; k < 16; ++k) {
+        float const id = (k % 5 == 0) ? 1.f : 0.f;
+        w2o.push_back(id);
+        o2w.push_back(id);
+    }
+    if (int32_t const st = create_tlas(trees.data(), num_meshes + 1, mesh_of.data(), w2o.data(),
+                                       o2w.data(), n_all, settings, nullptr, s.tlas);
+        st) {
+        return failed(st);
+    }
+    std::vector<uint32_t> mats(instance_materials, instance_materials + num_instances);
+    mats.push_back(dev::Null);
+    hipError_t e = hipMalloc(&s.mesh_geo, std::max<size_t>(1, num_meshes) * sizeof(Mesh_geometry));
+    if (!e) e = hipMalloc(&s.inst_o2w, o2w.size() * sizeof(float));
+    if (!e) e = hipMalloc(&s.inst_material, mats.size() * sizeof(uint32_t));
+    if (!e && num_meshes) e = hipMemcpy(s.mesh_geo, geo.data(), num_meshes * sizeof(Mesh_geometry), hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(s.inst_o2w, o2w.data(), o2w.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(s.inst_material, mats.data(), mats.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e) {
+        return failed(fail(MI_ERR_HIP, "mi_scene_create_instanced: %s", hipGetErrorString(e)));
+    }
+    // the guide's position normalization spans every instance
+    for (Instance_record const& r : s.tlas.records) {
+        for (int k = 0; k < 3; ++k) {
+            s.box_lo[k] = std::min(s.box_lo[k], r.lo[k]);
+            s.box_hi[k] = std::max(s.box_hi[k], r.hi[k]);
+        }
+    }
+    s.instanced        = true;
+    s.emitter_instance = num_instances;
+    out                = s;
+    return MI_OK;
+}
+
 void RenderState::release() {
     (void)hipFree(paths);
     (void)hipFree(rays[0]);
@@ -1614,7 +1745,10 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
         e = hipMalloc(&rays[q], size_t(n) * sizeof(mi_ray));
         if (!e) e = hipMalloc(&ray_path[q], size_t(n) * sizeof(uint32_t));
     }
-    if (!e) e = hipMalloc(&hits, size_t(n) * sizeof(mi_hit));
+    // room for the closest hits of either scene kind
+    static_assert(sizeof(mi_instance_hit) >= sizeof(mi_hit));
+    if (!e) e = hipMalloc(&hits, size_t(n) * sizeof(mi_instance_hit));
+    inst_hits = reinterpret_cast<mi_instance_hit*>(hits);
     if (!e) e = hipMalloc(&shadow_range, size_t(n) * sizeof(uint2));
     if (!e) e = hipMalloc(&medium_event, size_t(n) * sizeof(uint32_t));
     if (!e) e = hipMalloc(&medium_p, size_t(n) * 3 * sizeof(float));
@@ -1785,6 +1919,15 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.ray_path[0] = state.ray_path[0];
     w.ray_path[1] = state.ray_path[1];
     w.hits        = state.hits;
+    w.ihits       = state.inst_hits;
+    if (scene.instanced) {
+        w.instanced        = 1;
+        w.emitter_instance = scene.emitter_instance;
+        w.tlas             = tlas_view(scene.tlas);
+        w.mesh_geo         = scene.mesh_geo;
+        w.inst_o2w         = scene.inst_o2w;
+        w.inst_material    = scene.inst_material;
+    }
     w.shadow_rays = state.shadow_rays;
     w.shadows     = static_cast<Shadow*>(state.shadows);
     w.occluded    = state.occluded;
@@ -1812,15 +1955,32 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     uint32_t const    grid_path = std::min<uint32_t>(blocks(num_paths, 256), trace.num_cus * 8);
     auto const        start     = std::chrono::steady_clock::now();
 
+    // traversals with their ray counts on the device: the scene's tree, or
+    // the top-level tree over instances
+    auto trace_closest = [&](mi_ray const* rays, uint32_t const* count) -> int32_t {
+        if (scene.instanced) {
+            return trace_instances_launch(scene.tlas, trace.num_cus, rays, num_paths, count,
+                                          state.inst_hits, false, s);
+        }
+        return launch_trace_device_count(scene.bvh, trace, rays, count, num_paths, state.hits,
+                                         false, s);
+    };
+    auto trace_any = [&]() -> int32_t {
+        if (scene.instanced) {
+            return trace_instances_launch(scene.tlas, trace.num_cus, state.shadow_rays, num_shadow,
+                                          &state.counters[C_shadow], state.occluded, true, s);
+        }
+        return launch_trace_device_count(scene.bvh, trace, state.shadow_rays,
+                                         &state.counters[C_shadow], num_shadow, state.occluded,
+                                         true, s);
+    };
+
     // one wave: every kernel reads its counts and sample base on the device
     auto enqueue_wave = [&]() -> int32_t {
         hipLaunchKernelGGL(raygen, dim3(blocks(num_paths, 256)), dim3(256), 0, s, w);
         uint32_t q = 0;
         for (uint32_t d = 0; d <= set.max_bounces; ++d) {
-            if (int32_t const st = launch_trace_device_count(scene.bvh, trace, state.rays[q],
-                                                             &state.counters[C_queue0 + q],
-                                                             num_paths, state.hits, false, s);
-                st) {
+            if (int32_t const st = trace_closest(state.rays[q], &state.counters[C_queue0 + q]); st) {
                 return st;
             }
             if (scene.medium.present) {
@@ -1840,10 +2000,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
                 hipLaunchKernelGGL(shade<false>, dim3(grid_path), dim3(256), 0, s, w, q, d);
             }
             hipLaunchKernelGGL(tally, dim3(1), dim3(1), 0, s, state.counters, q);
-            if (int32_t const st = launch_trace_device_count(scene.bvh, trace, state.shadow_rays,
-                                                             &state.counters[C_shadow], num_shadow,
-                                                             state.occluded, true, s);
-                st) {
+            if (int32_t const st = trace_any(); st) {
                 return st;
             }
             hipLaunchKernelGGL(shadow_resolve, dim3(grid_path), dim3(256), 0, s, w, q, d);
@@ -1854,8 +2011,6 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         return MI_OK;
     };
 
-    // online training on the wave's records: train_steps Adam steps over
-    // consecutive slices of the records
     // online training on the wave's records, all on the device (capturable):
     // training weights, a shuffled copy, train_steps Adam steps over slices
     auto enqueue_training = [&]() -> int32_t {

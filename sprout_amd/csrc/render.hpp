@@ -26,6 +26,12 @@ struct Render_material {
     uint32_t pad[2];
 };
 
+// Vertex data of an instanced mesh on the device.
+struct Mesh_geometry {
+    float const*    positions;  // 3 floats per vertex
+    uint32_t const* indices;    // 3 per triangle
+};
+
 struct RenderScene {
     DeviceBvh        bvh;
     float*           positions    = nullptr;  // 3 floats per vertex
@@ -45,11 +51,14 @@ struct RenderScene {
     dev::Medium_view medium{};
     // instanced scenes: the mesh instances and, last, the triangles above
     // (emitters and other world-space triangles) as one more instance
-    Tlas      tlas;
-    float*    inst_o2w         = nullptr;  // 16 per instance
-    uint32_t* inst_material    = nullptr;  // per instance; Null for the world triangles
-    uint32_t  emitter_instance = 0xFFFFFFFFu;
-    bool      instanced        = false;
+    Tlas                   tlas;
+    std::vector<DeviceBvh> mesh_trees;
+    std::vector<void*>     mesh_buffers;             // device positions / indices per mesh
+    Mesh_geometry*         mesh_geo         = nullptr;  // per mesh
+    float*                 inst_o2w         = nullptr;  // 16 per instance
+    uint32_t*              inst_material    = nullptr;  // per instance; Null for the world triangles
+    uint32_t               emitter_instance = 0xFFFFFFFFu;
+    bool                   instanced        = false;
 
     uint32_t num_vertices  = 0;
     uint32_t num_triangles = 0;
@@ -63,11 +72,12 @@ struct RenderScene {
     void release_medium();
 };
 
-// An instanced scene: num_instances placements of mesh trees (row-vector
+// An instanced scene: num_instances placements of the meshes (row-vector
 // 4x4 matrices, one material each, not emissive) plus the world-space
 // triangles of the arrays (emitters among them), which form the light tree
-// as in create_scene and are traced as one more instance.
-int32_t create_scene_instanced(DeviceBvh const* const* meshes, float const* world_to_object,
+// as in create_scene and are traced as one more instance.  Host arrays.
+int32_t create_scene_instanced(mi_mesh const* meshes, uint32_t num_meshes,
+                               uint32_t const* instance_mesh, float const* world_to_object,
                                float const* object_to_world, uint32_t const* instance_materials,
                                uint32_t num_instances, float const* positions,
                                uint32_t num_vertices, uint32_t const* indices,

@@ -49,6 +49,47 @@ class Scene:
             self.handle = _lib.MI_NULL
 
 
+class InstancedScene(Scene):
+    """Mesh instances beside world-space triangles: meshes [(positions,
+    indices)], instance i showing meshes[mesh_of[i]] with the row-vector 4x4
+    matrices world_to_object[i] / object_to_world[i] and the non-emissive
+    material instance_materials[i]; the triangles (positions, indices,
+    triangle_materials) carry the emitters, as in Scene."""
+
+    def __init__(self, meshes, mesh_of, world_to_object, object_to_world, instance_materials,
+                 positions, indices, triangle_materials, materials, settings=(16, 64, 4)):
+        lib = _lib.load()
+        keep = []
+        arr = (_lib.MiMesh * len(meshes))()
+        for k, (mp, mi) in enumerate(meshes):
+            mp = np.ascontiguousarray(mp, np.float32).reshape(-1, 3)
+            mi = np.ascontiguousarray(mi, np.uint32).reshape(-1, 3)
+            keep += [mp, mi]
+            arr[k] = _lib.MiMesh(mp.ctypes.data, len(mp), mi.ctypes.data, len(mi))
+        mesh_of = np.ascontiguousarray(mesh_of, np.uint32)
+        w2o = np.ascontiguousarray(world_to_object, np.float32).reshape(-1, 16)
+        o2w = np.ascontiguousarray(object_to_world, np.float32).reshape(-1, 16)
+        imat = np.ascontiguousarray(instance_materials, np.uint32)
+        if not (len(mesh_of) == len(w2o) == len(o2w) == len(imat)):
+            raise ValueError("one mesh, two matrices and one material per instance")
+        pos = np.ascontiguousarray(positions, np.float32)
+        idx = np.ascontiguousarray(indices, np.uint32)
+        tmat = np.ascontiguousarray(triangle_materials, np.uint32)
+        mats = np.ascontiguousarray(materials, MATERIAL_DTYPE)
+        handle = ctypes.c_uint32(_lib.MI_NULL)
+        s = _lib.MiBvhSettings(*settings)
+        check(lib.mi_scene_create_instanced(arr, len(meshes), mesh_of.ctypes.data, w2o.ctypes.data,
+                                            o2w.ctypes.data, imat.ctypes.data, len(mesh_of),
+                                            pos.ctypes.data, len(pos), idx.ctypes.data, len(idx),
+                                            tmat.ctypes.data, mats.ctypes.data, len(mats),
+                                            ctypes.byref(s), ctypes.byref(handle)),
+              "mi_scene_create_instanced")
+        del keep
+        self.handle = handle.value
+        self.num_triangles = len(idx) + sum(len(np.asarray(m[1]).reshape(-1, 3))
+                                            for m in (meshes[j] for j in mesh_of))
+
+
 def camera(width: int, height: int, eye=ROOM_CAMERA["eye"], target=ROOM_CAMERA["target"],
            up=ROOM_CAMERA["up"], fov_deg=ROOM_CAMERA["fov_deg"]) -> _lib.MiCamera:
     return _lib.MiCamera(width, height, (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*target),

@@ -139,31 +139,24 @@ BATCH_CAMERA = {"eye": (0.0, 6.0, 26.0), "target": (0.0, 0.0, 0.0), "up": (0.0, 
                 "fov_deg": 45.0}
 
 
-def batch_scene(num_instances: int = 64, num_lights: int = 10000, seed: int = 1):
-    """The batched path-tracing workload baked into one triangle soup:
-    num_instances placements (seeded similarity transforms in a 20-unit cube)
-    of 4 distinct 20 480-triangle displaced spheres, each with its own
-    Substitute material (roughness U[0.05, 1], metallic Bernoulli(0.3), base
-    colour U[0.1, 0.9]^3); a grey ground plane under them; and num_lights
-    small emissive triangles above them with log-normal (sigma 1) radiance.
-    Returns positions, indices, per-triangle material and materials."""
+def batch_scene_instanced(num_instances: int = 64, num_lights: int = 10000, seed: int = 1):
+    """The batched path-tracing workload as sprout's props: num_instances
+    placements (seeded similarity transforms in a 20-unit cube) of 4 distinct
+    20 480-triangle displaced spheres, each with its own Substitute material
+    (roughness U[0.05, 1], metallic Bernoulli(0.3), base colour U[0.1, 0.9]^3);
+    a grey ground plane under them; and num_lights small emissive triangles
+    above them with log-normal (sigma 1) radiance.  Returns a dict: meshes
+    [(positions, indices)], mesh_of, world_to_object, object_to_world,
+    instance_materials, and the world triangles positions, indices,
+    triangle_materials with the materials of all."""
     from .instances import random_transforms
 
     rng = np.random.default_rng(seed)
     meshes = [displaced_icosphere(5, octaves=4, amplitude=0.15, noise_seed=seed + 10 + k,
                                   num_random=0) for k in range(4)]
-    _, o2w = random_transforms(num_instances, seed + 100, extent=10.0)
-    pos, idx, tmat = [], [], []
-    base = 0
-    mats = np.zeros(num_instances + 2, MATERIAL_DTYPE)
+    w2o, o2w = random_transforms(num_instances, seed + 100, extent=10.0)
+    mats = np.zeros(num_instances + 1, MATERIAL_DTYPE)
     for i in range(num_instances):
-        p, t = meshes[i % 4]
-        m = o2w[i].reshape(4, 4).astype(np.float64)
-        wp = (p.astype(np.float64) @ m[:3, :3] + m[3, :3]).astype(np.float32)
-        pos.append(wp)
-        idx.append(t + base)
-        tmat.append(np.full(len(t), i, np.uint32))
-        base += len(wp)
         mats[i]["kind"] = SUBSTITUTE
         mats[i]["color"] = rng.uniform(0.1, 0.9, 3)
         mats[i]["roughness"] = rng.uniform(0.05, 1.0)
@@ -175,10 +168,7 @@ def batch_scene(num_instances: int = 64, num_lights: int = 10000, seed: int = 1)
     mats[g]["color"] = (0.5, 0.5, 0.5)
     mats[g]["ior"] = 1.46
     gp = np.array([[-40, -12, -40], [40, -12, -40], [40, -12, 40], [-40, -12, 40]], np.float32)
-    pos.append(gp)
-    idx.append(np.array([[base, base + 2, base + 1], [base, base + 3, base + 2]], np.uint32))
-    tmat.append(np.full(2, g, np.uint32))
-    base += 4
+    gi = np.array([[0, 2, 1], [0, 3, 2]], np.uint32)
     # lights: small triangles facing down at y in [12, 16], one material each
     lm = np.zeros(num_lights, MATERIAL_DTYPE)
     lm["kind"] = LIGHT
@@ -187,12 +177,42 @@ def batch_scene(num_instances: int = 64, num_lights: int = 10000, seed: int = 1)
                       * rng.uniform(0.5, 1.0, (num_lights, 3)))
     c = np.stack([rng.uniform(-20, 20, num_lights), rng.uniform(12, 16, num_lights),
                   rng.uniform(-20, 20, num_lights)], axis=1)
-    s = rng.uniform(0.1, 0.4, (num_lights, 1))
-    lp = np.stack([c, c + s * [1, 0, 0], c + s * [0, 0, 1]], axis=1).astype(np.float32)
-    pos.append(lp.reshape(-1, 3))
-    li = base + np.arange(3 * num_lights, dtype=np.uint32).reshape(-1, 3)
-    idx.append(li)  # cross(b - a, c - a) = (1,0,0) x (0,0,1) points down
-    tmat.append(np.arange(num_lights, dtype=np.uint32) + num_instances + 1)
-    mats = np.concatenate([mats[:num_instances + 1], lm])
+    sz = rng.uniform(0.1, 0.4, (num_lights, 1))
+    lp = np.stack([c, c + sz * [1, 0, 0], c + sz * [0, 0, 1]], axis=1).astype(np.float32)
+    li = 4 + np.arange(3 * num_lights, dtype=np.uint32).reshape(-1, 3)
+    # cross(b - a, c - a) = (1,0,0) x (0,0,1) points down
+    return {
+        "meshes": meshes,
+        "mesh_of": (np.arange(num_instances) % 4).astype(np.uint32),
+        "world_to_object": w2o,
+        "object_to_world": o2w,
+        "instance_materials": np.arange(num_instances, dtype=np.uint32),
+        "positions": np.concatenate([gp, lp.reshape(-1, 3)]).astype(np.float32),
+        "indices": np.concatenate([gi, li]).astype(np.uint32),
+        "triangle_materials": np.concatenate(
+            [np.full(2, g, np.uint32), np.arange(num_lights, dtype=np.uint32) + num_instances + 1]),
+        "materials": np.concatenate([mats, lm]),
+    }
+
+
+def batch_scene(num_instances: int = 64, num_lights: int = 10000, seed: int = 1):
+    """batch_scene_instanced baked into one triangle soup: every instance's
+    mesh transformed to world space (in double, rounded to float), then the
+    ground and the lights.  Returns positions, indices, per-triangle material
+    and materials."""
+    b = batch_scene_instanced(num_instances, num_lights, seed)
+    pos, idx, tmat = [], [], []
+    base = 0
+    for i in range(num_instances):
+        p, t = b["meshes"][b["mesh_of"][i]]
+        m = b["object_to_world"][i].reshape(4, 4).astype(np.float64)
+        wp = (p.astype(np.float64) @ m[:3, :3] + m[3, :3]).astype(np.float32)
+        pos.append(wp)
+        idx.append(t + base)
+        tmat.append(np.full(len(t), b["instance_materials"][i], np.uint32))
+        base += len(wp)
+    pos.append(b["positions"])
+    idx.append(b["indices"] + base)
+    tmat.append(b["triangle_materials"])
     return (np.concatenate(pos).astype(np.float32), np.concatenate(idx).astype(np.uint32),
-            np.concatenate(tmat).astype(np.uint32), mats)
+            np.concatenate(tmat).astype(np.uint32), b["materials"])

@@ -102,8 +102,10 @@ def test_training_lowers_error_in_the_slit_room(gpu):
 
 
 def test_guided_graph_equals_launches(gpu):
-    """The captured guided wave (render + on-device training) replays the
-    kernel-by-kernel run bit for bit: image and trained parameters."""
+    """The captured guided wave replays the kernel-by-kernel run: bit for bit
+    with a fixed guide; with on-device training both train and give the same
+    image up to noise.  (Training records are appended in queue order, which
+    varies from run to run, so trained guides are not bitwise reproducible.)"""
     import torch
 
     from sprout_amd import scenes
@@ -111,16 +113,30 @@ def test_guided_graph_equals_launches(gpu):
     render, guide = gpu
     sc = scene(gpu, 0.05)
     cam = render.camera(32, 32, **scenes.ROOM_CAMERA)
+    g = guide.Guide(3)
+    render.render_guided(sc, cam, 64, g, fraction=0.5, train=True, train_steps=3, batch=4096,
+                         samples_per_wave=8, seed=4)
+    out = []
+    for graph in (True, False):
+        img, _ = render.render_guided(sc, cam, 32, g, fraction=0.5, train=False,
+                                      samples_per_wave=8, seed=5, graph=graph)
+        torch.cuda.synchronize()
+        out.append(img.cpu().numpy())
+    assert np.array_equal(out[0], out[1])
+    g.release()
+
     out = []
     for graph in (True, False):
         g = guide.Guide(3)
-        img, st = render.render_guided(sc, cam, 64, g, fraction=0.5, train=True, train_steps=3,
-                                       batch=4096, learning_rate=2e-3, samples_per_wave=8, seed=5,
-                                       graph=graph, stats=True)
+        img, st = render.render_guided(sc, cam, 4096, g, fraction=0.5, train=True, train_steps=3,
+                                       batch=4096, learning_rate=2e-3, samples_per_wave=128,
+                                       seed=5 + graph, graph=graph, stats=True)
         torch.cuda.synchronize()
-        out.append((img.cpu().numpy(), g.params(), st.trained))
+        out.append((blocks(img.cpu().numpy() / 4096), st.trained))
         g.release()
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1], out[1][1])
-    assert out[0][2] == out[1][2] > 0
-
+    e = rel_mse(out[0][0], out[1][0])
+    log(f"trained guided, graph vs launches: relMSE {e:.3g}, trained {out[0][1]} / {out[1][1]}")
+    assert out[0][1] > 0 and out[1][1] > 0
+    assert e < 2e-2
+    assert abs(out[0][0].mean() / out[1][0].mean() - 1) < 5e-2
+    sc.release()
