@@ -451,6 +451,12 @@ int32_t mi_guide_release(uint32_t guide);
 /* HOST arrays of MI_GUIDE_NUM_PARAMS floats; set also resets Adam's moments */
 int32_t mi_guide_set_params(uint32_t guide, float const* params);
 int32_t mi_guide_get_params(uint32_t guide, float* params);
+/* The input encoding: feature pair (i, j) of input i is sin / cos of
+ * pi (freq[i][j] x_i) (fp32 product), 9 x 12 HOST floats, finite, |f| <= 4096.
+ * Default: f = 2^(j/2) for the three position inputs, 2^(j/4) for the normal
+ * and direction inputs, j = 0..11.  No counterpart in sprout (see above). */
+#define MI_GUIDE_NUM_FREQ 108u
+int32_t mi_guide_set_frequencies(uint32_t guide, float const* freq);
 /* DEVICE arrays: x n x 9 inputs, out n x 32 raw outputs */
 int32_t mi_guide_forward(uint32_t guide, float const* x, uint32_t n, float* out, void* stream);
 /* DEVICE arrays: x n x 9, dirs n x 3 unit directions, weights n (>= 0);
@@ -481,6 +487,8 @@ typedef struct mi_guide_settings {
     uint32_t batch;
     float    learning_rate;
     uint32_t record_depth;
+    uint32_t guide_depth; /* vertices of depth below this sample the guide (0: all); deeper
+                             ones sample the BSDF alone and record nothing */
 } mi_guide_settings;
 
 /* As mi_render, guided.  With training, every wave is followed on the

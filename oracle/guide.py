@@ -10,8 +10,10 @@ forms of the von Mises-Fisher distribution (tests/test_oracle_guide.py).
 Model
   inputs   x (9): position normalized to the scene box [0,1]^3, normal and
            outgoing direction mapped from [-1,1] to [0,1]
-  features 12 frequencies per input: sin(2^j pi x), cos(2^j pi x),
-           feature 24 i + 2 j (+1 for cos); 216 features, zero-padded to 224
+  features 12 multipliers f per input: sin(pi f x), cos(pi f x), f x in
+           float32; feature 24 i + 2 j (+1 for cos); 216 features,
+           zero-padded to 224; by default f = 2^(j/2) for the position and
+           2^(j/4) for the normal and direction
   network  224 -> 64 -> 64 -> 64 (ReLU) -> 32, biases everywhere
   outputs  8 lobes x (a, b, c, d):
              theta = pi sigmoid(a), phi = 2 pi sigmoid(b)
@@ -63,12 +65,22 @@ def inputs(p, n, wo, box_lo, box_hi) -> np.ndarray:
     return np.concatenate([p, (np.asarray(n) + 1) * 0.5, (np.asarray(wo) + 1) * 0.5], axis=-1)
 
 
-def encode(x: np.ndarray) -> np.ndarray:
-    """(B, 9) in [0,1] -> (B, 224)"""
+def default_frequencies() -> np.ndarray:
+    """(9, 12) float32 multipliers: 2^(j/2) for the position inputs, 2^(j/4)
+    for the normal and direction inputs"""
+    j = np.arange(NUM_FREQ, dtype=np.float64)
+    step = np.array([0.5] * 3 + [0.25] * 6)[:, None]
+    return np.exp2(step * j[None, :]).astype(np.float32)
+
+
+def encode(x: np.ndarray, freq: np.ndarray | None = None) -> np.ndarray:
+    """(B, 9) in [0,1] -> (B, 224): sin / cos of pi f x per input and
+    multiplier, the product f x rounded to float32 as the kernels compute it"""
     b = x.shape[0]
     f = np.zeros((b, FEAT_PAD))
-    freq = (2.0 ** np.arange(NUM_FREQ)) * np.pi
-    arg = x[:, :, None] * freq[None, None, :]  # (B, 9, 12)
+    freq = default_frequencies() if freq is None else np.asarray(freq, np.float32).reshape(9, NUM_FREQ)
+    prod = (np.asarray(x, np.float32)[:, :, None] * freq[None, :, :]).astype(np.float64)
+    arg = prod * np.pi  # (B, 9, 12)
     enc = np.stack([np.sin(arg), np.cos(arg)], axis=-1)  # (B, 9, 12, 2)
     f[:, :NUM_FEAT] = enc.reshape(b, NUM_FEAT)
     return f

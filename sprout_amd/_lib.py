@@ -93,7 +93,7 @@ class MiGuideSettings(ctypes.Structure):
     _fields_ = [("guide", ctypes.c_uint32), ("fraction", ctypes.c_float),
                 ("train", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
                 ("batch", ctypes.c_uint32), ("learning_rate", ctypes.c_float),
-                ("record_depth", ctypes.c_uint32)]
+                ("record_depth", ctypes.c_uint32), ("guide_depth", ctypes.c_uint32)]
 
 
 MI_MATERIAL_SUBSTITUTE = 0
@@ -161,6 +161,7 @@ SIGNATURES = {
     "mi_guide_release": (_i32, [_u32]),
     "mi_guide_set_params": (_i32, [_u32, _vp]),
     "mi_guide_get_params": (_i32, [_u32, _vp]),
+    "mi_guide_set_frequencies": (_i32, [_u32, _vp]),
     "mi_guide_forward": (_i32, [_u32, _vp, _u32, _vp, _vp]),
     "mi_guide_gradients": (_i32, [_u32, _vp, _vp, _vp, _u32, _vp, _vp]),
     "mi_guide_train_step": (_i32, [_u32, _vp, _vp, _vp, _u32, _f32, _vp]),

@@ -635,6 +635,16 @@ int32_t mi_guide_set_params(uint32_t guide, float const* params) {
     return guide_set_params(*g, params);
 }
 
+int32_t mi_guide_set_frequencies(uint32_t guide, float const* freq) {
+    clear_error();
+    GuideNet* g = lookup_guide(guide);
+    if (!g || !freq) {
+        return fail(g ? MI_ERR_ARGS : MI_ERR_HANDLE, "mi_guide_set_frequencies: bad arguments");
+    }
+    (void)hipDeviceSynchronize();
+    return guide_set_frequencies(*g, freq);
+}
+
 int32_t mi_guide_get_params(uint32_t guide, float* params) {
     clear_error();
     GuideNet* g = lookup_guide(guide);

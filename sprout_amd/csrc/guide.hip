@@ -105,22 +105,25 @@ __device__ void load_weights(Weights& s, uint16_t const* wt, float const* params
 }
 
 // Features of 16 queries into X [16][S_feat]; row m = query row0 + m, rows
-// past n are zero.  x: 9 inputs per query in [0,1].
-__device__ void features(float const* x, uint32_t row0, uint32_t n, uint16_t* X) {
+// past n are zero.  x: 9 inputs per query in [0,1]; feature (i, j) is
+// sin / cos of pi (freq[i][j] x_i), the product rounded to fp32.
+__device__ void features(float const* x, float const* freq, uint32_t row0, uint32_t n, uint16_t* X) {
     int const      lane = threadIdx.x & 63;
     int const      m    = lane & 15;
     int const      g    = lane >> 4;
     uint32_t const row  = row0 + m;
     for (int i = g; i < Num_in; i += 4) {
         float const v = row < n ? x[size_t(row) * Num_in + i] : 0.f;
-        float       f = v;
         for (int j = 0; j < Num_freq; ++j) {
-            // 2^j v is exact; sinpi/cospi reduce it exactly
-            float const s = row < n ? sinpif(f) : 0.f;
-            float const c = row < n ? cospif(f) : 0.f;
+            // sin(pi f) = sin(2 pi (f / 2)): the hardware sine of the
+            // fraction of f / 2 turns (f / 2 and its fraction are exact)
+            float const f   = freq[i * Num_freq + j] * v;
+            float const h   = 0.5f * f;
+            float const rev = h - floorf(h);
+            float const s   = row < n ? __builtin_amdgcn_sinf(rev) : 0.f;
+            float const c   = row < n ? __builtin_amdgcn_cosf(rev) : 0.f;
             X[m * S_feat + 24 * i + 2 * j]     = to_bf16(s);
             X[m * S_feat + 24 * i + 2 * j + 1] = to_bf16(c);
-            f *= 2.f;
         }
     }
     if (3 == g) {
@@ -174,8 +177,8 @@ static_assert(16 * S_out * sizeof(float) <= 16 * S_feat * sizeof(uint16_t),
 static_assert(sizeof(Fwd_lds) <= 160 * 1024, "LDS of a workgroup");
 
 __global__ void __launch_bounds__(Fwd_waves * 64)
-    forward_kernel(uint16_t const* wt, float const* params, float const* x, uint32_t n_host,
-                   uint32_t const* n_dev, float* out) {
+    forward_kernel(uint16_t const* wt, float const* params, float const* freq, float const* x,
+                   uint32_t n_host, uint32_t const* n_dev, float* out) {
     __shared__ Fwd_lds s;
     uint32_t const n = n_dev ? min(*n_dev, n_host) : n_host;
     load_weights(s.w, wt, params);
@@ -192,7 +195,7 @@ __global__ void __launch_bounds__(Fwd_waves * 64)
     for (uint32_t tile = blockIdx.x * Fwd_waves + wave; uint64_t(tile) * 16 < n;
          tile += gridDim.x * Fwd_waves) {
         uint32_t const row0 = tile * 16;
-        features(x, row0, n, X);
+        features(x, freq, row0, n, X);
         wave_sync();
         layer<Feat_pad, Hidden, true>(X, S_feat, s.w.w1, S_feat, bb, H0, S_hid, nullptr, 0);
         wave_sync();
@@ -234,57 +237,72 @@ struct Grad_lds {
 static_assert(sizeof(Grad_lds) <= 160 * 1024, "LDS of a workgroup");
 constexpr int Num_bias = 3 * Hidden + Num_out;  // 224
 
-// Loss gradient at the 32 outputs of one sample (row of Y), scaled by
-// g = -(t / sum t) / q: the derivative of -(t / sum t) log q.
-__device__ void output_gradient(float const* y, float dirx, float diry, float dirz, float tw,
-                                int lobe, float out[4]) {
-    // mixture density q
-    float dmax = y[3];
-    for (int k = 1; k < Num_lobes; ++k) {
-        dmax = fmaxf(dmax, y[4 * k + 3]);
-    }
-    float esum = 0.f;
-    for (int k = 0; k < Num_lobes; ++k) {
-        esum += expf(y[4 * k + 3] - dmax);
-    }
-    float q  = 0.f;
-    float vk = 0.f, wk = 0.f, kap = 0.f, cosk = 0.f, sa = 0.f, sb = 0.f, theta = 0.f, phi = 0.f;
-    for (int k = 0; k < Num_lobes; ++k) {
-        float const* r  = y + 4 * k;
-        float const  a  = 1.f / (1.f + expf(-r[0]));
-        float const  b  = 1.f / (1.f + expf(-r[1]));
-        float const  th = dev::Pi * a;
-        float const  ph = 2.f * dev::Pi * b;
-        float        st, ct, sp, cp;
-        sincosf(th, &st, &ct);
-        sincosf(ph, &sp, &cp);
-        float const c  = (st * cp) * dirx + (st * sp) * diry + ct * dirz;
-        float const ka = expf(fminf(fmaxf(r[2], Kappa_lo), Kappa_hi));
-        float const v  = ka / (2.f * dev::Pi * -expm1f(-2.f * ka)) * expf(ka * (c - 1.f));
-        float const w  = expf(r[3] - dmax) / esum;
-        q += w * v;
-        if (k == lobe) {
-            vk = v, wk = w, kap = ka, cosk = c, sa = a, sb = b, theta = th, phi = ph;
-        }
-    }
-    float const g    = -tw / fmaxf(q, 1e-30f);
-    float const resp = wk * vk;
-    float const* r   = y + 4 * lobe;
+// Loss gradient at the outputs of lobes q and q + 4 of one sample (row of
+// Y), for g = -t / q: the derivative of -t log q (the reduction divides by
+// the batch's weight sum).  The sample's four lanes (q = 0..3, lanes r + 16 q)
+// hold two lobes each and share the mixture's sums by lane exchange, so
+// every lobe is evaluated once; all 64 lanes must call it.
+struct Lobe_terms {
+    float v, kap, cosk, sa, sb, theta, phi, d;
+};
+
+__device__ Lobe_terms lobe_terms(float const* r, float dirx, float diry, float dirz) {
+    Lobe_terms t;
+    t.sa  = 1.f / (1.f + expf(-r[0]));
+    t.sb  = 1.f / (1.f + expf(-r[1]));
+    t.theta = dev::Pi * t.sa;
+    t.phi   = 2.f * dev::Pi * t.sb;
+    float st, ct, sp, cp;
+    sincosf(t.theta, &st, &ct);
+    sincosf(t.phi, &sp, &cp);
+    t.cosk = (st * cp) * dirx + (st * sp) * diry + ct * dirz;
+    t.kap  = expf(fminf(fmaxf(r[2], Kappa_lo), Kappa_hi));
+    t.v    = t.kap / (2.f * dev::Pi * -expm1f(-2.f * t.kap)) * expf(t.kap * (t.cosk - 1.f));
+    t.d    = r[3];
+    return t;
+}
+
+__device__ void lobe_gradient(Lobe_terms const& t, float const* r, float g, float q, float wk,
+                              float dirx, float diry, float dirz, float out[4]) {
+    float const resp = wk * t.v;
     // softmax: dq/dd_k = w_k (v_k - q)
-    out[3] = g * wk * (vk - q);
+    out[3] = g * wk * (t.v - q);
     // kappa: d log v / d kappa = 1/kappa - 2 e^(-2k) / (1 - e^(-2k)) + (cos - 1)
-    float const e2   = expf(-2.f * kap);
-    float const dlk  = 1.f / kap - 2.f * e2 / (-expm1f(-2.f * kap)) + (cosk - 1.f);
-    bool const  in   = r[2] > Kappa_lo && r[2] < Kappa_hi;
-    out[2]           = in ? g * resp * dlk * kap : 0.f;
+    float const e2  = expf(-2.f * t.kap);
+    float const dlk = 1.f / t.kap - 2.f * e2 / (-expm1f(-2.f * t.kap)) + (t.cosk - 1.f);
+    bool const  in  = r[2] > Kappa_lo && r[2] < Kappa_hi;
+    out[2]          = in ? g * resp * dlk * t.kap : 0.f;
     // direction: d log v / d mu = kappa w
     float st, ct, sp, cp;
-    sincosf(theta, &st, &ct);
-    sincosf(phi, &sp, &cp);
-    float const dth = kap * ((ct * cp) * dirx + (ct * sp) * diry - st * dirz);
-    float const dph = kap * ((-st * sp) * dirx + (st * cp) * diry);
-    out[0]          = g * resp * dth * dev::Pi * sa * (1.f - sa);
-    out[1]          = g * resp * dph * 2.f * dev::Pi * sb * (1.f - sb);
+    sincosf(t.theta, &st, &ct);
+    sincosf(t.phi, &sp, &cp);
+    float const dth = t.kap * ((ct * cp) * dirx + (ct * sp) * diry - st * dirz);
+    float const dph = t.kap * ((-st * sp) * dirx + (st * cp) * diry);
+    out[0]          = g * resp * dth * dev::Pi * t.sa * (1.f - t.sa);
+    out[1]          = g * resp * dph * 2.f * dev::Pi * t.sb * (1.f - t.sb);
+}
+
+__device__ float sum4(float v) {
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+}
+
+__device__ void output_gradients(float const* y, float dirx, float diry, float dirz, float tw,
+                                 int q, float g0[4], float g1[4]) {
+    Lobe_terms const a = lobe_terms(y + 4 * q, dirx, diry, dirz);
+    Lobe_terms const b = lobe_terms(y + 4 * (q + 4), dirx, diry, dirz);
+    float            dmax = fmaxf(a.d, b.d);
+    dmax                  = fmaxf(dmax, __shfl_xor(dmax, 16));
+    dmax                  = fmaxf(dmax, __shfl_xor(dmax, 32));
+    float const ea        = expf(a.d - dmax);
+    float const eb        = expf(b.d - dmax);
+    float const esum      = sum4(ea + eb);
+    float const wa        = ea / esum;
+    float const wb        = eb / esum;
+    float const qm        = sum4(wa * a.v + wb * b.v);  // mixture density q
+    float const g         = -tw / fmaxf(qm, 1e-30f);
+    lobe_gradient(a, y + 4 * q, g, qm, wa, dirx, diry, dirz, g0);
+    lobe_gradient(b, y + 4 * (q + 4), g, qm, wb, dirx, diry, dirz, g1);
 }
 
 // dst[16][N] = (src[16][K] W^T) * (act > 0): backward through one layer;
@@ -315,9 +333,9 @@ __device__ void back_layer(uint16_t const* src, int ss, uint16_t const* Wt, int 
 }
 
 __global__ void __launch_bounds__(Grad_waves * 64)
-    gradient_kernel(uint16_t const* wt, float const* params, float const* x, float const* dirs,
-                    float const* weights, uint32_t n, uint32_t const* window, float const* t_sum,
-                    float* partial) {
+    gradient_kernel(uint16_t const* wt, float const* params, float const* freq, float const* x,
+                    float const* dirs, float const* weights, uint32_t n, uint32_t const* window,
+                    float* partial, float* wsum) {
     __shared__ Grad_lds s;
     if (window) {
         // the batch is records [window[0], window[0] + window[1]) of the arrays
@@ -333,11 +351,11 @@ __global__ void __launch_bounds__(Grad_waves * 64)
     // thread b < 224 sums the gradient of bias b over the samples, in order
     float bias_acc = 0.f;
 
-    int const   wave  = threadIdx.x >> 6;
-    int const   lane  = threadIdx.x & 63;
-    int const   r     = lane & 15;
-    int const   q     = lane >> 4;
-    float const scale = 1.f / *t_sum;
+    int const wave = threadIdx.x >> 6;
+    int const lane = threadIdx.x & 63;
+    int const r    = lane & 15;
+    int const q    = lane >> 4;
+    float     wacc = 0.f;  // weights of this lane's samples (q == 0), in order
 
     f32x4 acc[Tiles_per_wave];
 #pragma unroll
@@ -358,7 +376,7 @@ __global__ void __launch_bounds__(Grad_waves * 64)
         uint16_t*      DY   = s.dy + wave * 16 * S_out;
         float*         Y    = s.y + wave * 16 * S_out;
 
-        features(x, row0, n, X);
+        features(x, freq, row0, n, X);
         wave_sync();
         layer<Feat_pad, Hidden, true>(X, S_feat, s.w.w1, S_feat, s.w.b, H1, S_hid, nullptr, 0);
         wave_sync();
@@ -372,16 +390,17 @@ __global__ void __launch_bounds__(Grad_waves * 64)
         // output gradients: lane handles sample r, lobes q and q + 4
         {
             uint32_t const row = row0 + r;
-            float          g0[4] = {0.f, 0.f, 0.f, 0.f};
-            float          g1[4] = {0.f, 0.f, 0.f, 0.f};
-            if (row < n) {
-                float const tw = weights[row] * scale;
-                float const dx = dirs[3 * size_t(row) + 0];
-                float const dyv = dirs[3 * size_t(row) + 1];
-                float const dz = dirs[3 * size_t(row) + 2];
-                output_gradient(Y + r * S_out, dx, dyv, dz, tw, q, g0);
-                output_gradient(Y + r * S_out, dx, dyv, dz, tw, q + 4, g1);
+            float          g0[4];
+            float          g1[4];
+            // unnormalized weights: the reduction divides by the batch's weight sum
+            float const tw  = row < n ? weights[row] : 0.f;
+            float const dx  = row < n ? dirs[3 * size_t(row) + 0] : 0.f;
+            float const dyv = row < n ? dirs[3 * size_t(row) + 1] : 0.f;
+            float const dz  = row < n ? dirs[3 * size_t(row) + 2] : 1.f;
+            if (0 == q) {
+                wacc += tw;
             }
+            output_gradients(Y + r * S_out, dx, dyv, dz, tw, q, g0, g1);
             wave_sync();  // Y is read by all lanes before DY and DZ1 are written
             for (int p = 0; p < 4; ++p) {
                 DY[r * S_out + 4 * q + p]       = to_bf16(g0[p]);
@@ -433,6 +452,17 @@ __global__ void __launch_bounds__(Grad_waves * 64)
         __syncthreads();
     }
 
+    // the workgroup's weight sum, lanes in a fixed order (reusing s.y)
+    s.y[threadIdx.x] = wacc;
+    __syncthreads();
+    if (0 == threadIdx.x) {
+        float t = 0.f;
+        for (int k = 0; k < Grad_waves * 64; ++k) {
+            t += s.y[k];
+        }
+        wsum[blockIdx.x] = t;
+    }
+
     // partial sums of this workgroup, in the parameter layout
     float* out = partial + size_t(blockIdx.x) * Num_params;
 #pragma unroll
@@ -463,39 +493,21 @@ __global__ void __launch_bounds__(Grad_waves * 64)
     }
 }
 
-__global__ void sum_weights(float const* w, uint32_t n, uint32_t const* window, float* out) {
-    __shared__ float part[256];
-    float            s = 0.f;
-    if (window) {
-        w += window[0];
-        n = window[1];
-    }
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        s += w[i];
-    }
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (int(threadIdx.x) < h) {
-            part[threadIdx.x] += part[threadIdx.x + h];
-        }
-        __syncthreads();
-    }
-    if (0 == threadIdx.x) {
-        out[0] = part[0] > 0.f ? part[0] : 1.f;
-    }
-}
-
-__global__ void reduce_partials(float const* partial, uint32_t blocks, float* grad) {
+// grad = (sum of the workgroups' partials) / (sum of the batch weights),
+// both summed in workgroup order; the gradient of -(t / sum t) log q.
+__global__ void reduce_partials(float const* partial, float const* wsum, uint32_t blocks,
+                                float* grad) {
     uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= uint32_t(Num_params)) {
         return;
     }
     float s = 0.f;
+    float t = 0.f;
     for (uint32_t b = 0; b < blocks; ++b) {
         s += partial[size_t(b) * Num_params + p];
+        t += wsum[b];
     }
-    grad[p] = s;
+    grad[p] = s / (t > 0.f ? t : 1.f);
 }
 
 // W^T bf16 position of parameter p, or -1 for a bias
@@ -609,7 +621,8 @@ void GuideNet::release() {
     (void)hipFree(adam_v);
     (void)hipFree(grad);
     (void)hipFree(partial);
-    (void)hipFree(t_sum);
+    (void)hipFree(wsum);
+    (void)hipFree(freq);
     (void)hipFree(wt);
     (void)hipFree(window);
     *this = GuideNet();
@@ -623,7 +636,13 @@ int32_t guide_create(uint64_t seed, uint32_t num_cus, GuideNet& out) {
     if (!e) e = hipMalloc(&n.adam_v, Num_params * sizeof(float));
     if (!e) e = hipMalloc(&n.grad, Num_params * sizeof(float));
     if (!e) e = hipMalloc(&n.partial, size_t(n.partial_blocks) * Num_params * sizeof(float));
-    if (!e) e = hipMalloc(&n.t_sum, sizeof(float));
+    if (!e) e = hipMalloc(&n.wsum, size_t(n.partial_blocks) * sizeof(float));
+    if (!e) e = hipMalloc(&n.freq, Num_in * Num_freq * sizeof(float));
+    if (!e) {
+        float f[Num_in * Num_freq];
+        guide_default_frequencies(f);
+        e = hipMemcpy(n.freq, f, sizeof(f), hipMemcpyHostToDevice);
+    }
     if (!e) e = hipMalloc(&n.wt, Num_t * sizeof(uint16_t));
     if (!e) e = hipMalloc(&n.window, 8 * sizeof(uint32_t));
     if (e) {
@@ -657,6 +676,29 @@ int32_t guide_create(uint64_t seed, uint32_t num_cus, GuideNet& out) {
     return guide_set_params(out, p.data());
 }
 
+void guide_default_frequencies(float* out) {
+    for (int i = 0; i < Num_in; ++i) {
+        double const step = i < 3 ? 0.5 : 0.25;  // position: half octaves, directions: quarter
+        for (int j = 0; j < Num_freq; ++j) {
+            out[i * Num_freq + j] = float(std::exp2(step * j));
+        }
+    }
+}
+
+int32_t guide_set_frequencies(GuideNet& net, float const* host_freq) {
+    for (int k = 0; k < Num_in * Num_freq; ++k) {
+        if (!std::isfinite(host_freq[k]) || std::fabs(host_freq[k]) > 4096.f) {
+            return fail(MI_ERR_ARGS, "guide frequencies: entry %d is %g", k, double(host_freq[k]));
+        }
+    }
+    hipError_t const e = hipMemcpy(net.freq, host_freq, Num_in * Num_freq * sizeof(float),
+                                   hipMemcpyHostToDevice);
+    if (e) {
+        return fail(MI_ERR_HIP, "guide frequencies: %s", hipGetErrorString(e));
+    }
+    return MI_OK;
+}
+
 int32_t guide_set_params(GuideNet& net, float const* host_params) {
     hipError_t e = hipMemcpy(net.params, host_params, Num_params * sizeof(float), hipMemcpyHostToDevice);
     if (!e) e = hipMemset(net.adam_m, 0, Num_params * sizeof(float));
@@ -687,7 +729,7 @@ int32_t guide_forward(GuideNet const& net, float const* x, uint32_t n, float* ou
     uint32_t const tiles  = blocks_for(n, 16);
     uint32_t const blocks = std::min<uint32_t>(blocks_for(tiles, Fwd_waves), 1024);
     hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(Fwd_waves * 64), 0, stream, net.wt,
-                       net.params, x, n, static_cast<uint32_t const*>(nullptr), out);
+                       net.params, net.freq, x, n, static_cast<uint32_t const*>(nullptr), out);
     if (hipError_t const e = hipGetLastError(); e) {
         return fail(MI_ERR_HIP, "guide forward: %s", hipGetErrorString(e));
     }
@@ -702,7 +744,7 @@ int32_t guide_forward_device_count(GuideNet const& net, float const* x, uint32_t
     uint32_t const tiles  = blocks_for(max_n, 16);
     uint32_t const blocks = std::min<uint32_t>(blocks_for(tiles, Fwd_waves), 1024);
     hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(Fwd_waves * 64), 0, stream, net.wt,
-                       net.params, x, max_n, n, out);
+                       net.params, net.freq, x, max_n, n, out);
     if (hipError_t const e = hipGetLastError(); e) {
         return fail(MI_ERR_HIP, "guide forward: %s", hipGetErrorString(e));
     }
@@ -715,13 +757,11 @@ namespace {
 void enqueue_gradients(GuideNet& net, float const* x, float const* dirs, float const* weights,
                        uint32_t max_n, hipStream_t stream) {
     uint32_t const blocks = std::min(blocks_for(max_n, Chunk), net.partial_blocks);
-    hipLaunchKernelGGL(sum_weights, dim3(1), dim3(256), 0, stream, weights, 0u,
-                       static_cast<uint32_t const*>(net.window), net.t_sum);
     hipLaunchKernelGGL(gradient_kernel, dim3(blocks), dim3(Grad_waves * 64), 0, stream, net.wt,
-                       net.params, x, dirs, weights, 0u, static_cast<uint32_t const*>(net.window),
-                       net.t_sum, net.partial);
+                       net.params, net.freq, x, dirs, weights, 0u,
+                       static_cast<uint32_t const*>(net.window), net.partial, net.wsum);
     hipLaunchKernelGGL(reduce_partials, dim3(blocks_for(Num_params, 256)), dim3(256), 0, stream,
-                       net.partial, blocks, net.grad);
+                       net.partial, net.wsum, blocks, net.grad);
 }
 
 void enqueue_adam(GuideNet& net, float lr, hipStream_t stream) {

@@ -3,8 +3,10 @@
 // BF16 MFMA inference and training kernels (guide.hip).
 //
 // Network: 9 inputs (position in the scene box, normal and outgoing
-// direction mapped to [0,1]) -> 12-frequency sin/cos encoding (216
-// features, zero-padded to 224) -> 64 -> 64 -> 64 (ReLU) -> 32 = 8 lobes x
+// direction mapped to [0,1]) -> 12-frequency sin/cos encoding, sin(pi f x)
+// and cos(pi f x) for 12 multipliers f per input (216 features, zero-padded
+// to 224; by default f = 2^(j/2) for the position and 2^(j/4) for normal and
+// direction, j = 0..11) -> 64 -> 64 -> 64 (ReLU) -> 32 = 8 lobes x
 // (a, b, c, d): theta = pi sigmoid(a), phi = 2 pi sigmoid(b),
 // kappa = exp(clamp(c, -4, 8)), weight = softmax(d).
 #pragma once
@@ -56,7 +58,8 @@ struct GuideNet {
     float*    adam_v  = nullptr;
     float*    grad    = nullptr;  // reduced gradient of the last batch
     float*    partial = nullptr;  // per-block gradient partial sums
-    float*    t_sum   = nullptr;  // batch weight sum (one float)
+    float*    wsum    = nullptr;  // per-block partial sums of the batch weights
+    float*    freq    = nullptr;  // encoding multipliers, Num_in x Num_freq
     uint16_t* wt      = nullptr;  // bf16 W^T of every layer
     uint32_t* window  = nullptr;  // {first record, count, Adam step} of the current batch
     uint32_t* step_dev     = nullptr;  // Adam steps taken (device word)
@@ -68,6 +71,10 @@ struct GuideNet {
 
 int32_t guide_create(uint64_t seed, uint32_t num_cus, GuideNet& out);
 int32_t guide_set_params(GuideNet& net, float const* host_params);
+// Encoding multipliers (host, Num_in x Num_freq, finite, |f| <= 4096).
+int32_t guide_set_frequencies(GuideNet& net, float const* host_freq);
+// The default multipliers (Num_in x Num_freq) into out.
+void guide_default_frequencies(float* out);
 int32_t guide_get_params(GuideNet const& net, float* host_params);
 // x: n x 9 inputs in [0,1]; out: n x 32 raw outputs (device arrays)
 int32_t guide_forward(GuideNet const& net, float const* x, uint32_t n, float* out,

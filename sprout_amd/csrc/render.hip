@@ -184,6 +184,7 @@ struct Wave {
     float*       guide_x;       // 9 network inputs per queue entry
     float        box_lo[3];
     float        box_scale[3];  // 1 / extent of the scene box
+    uint32_t     guide_depth;   // vertices of depth below this sample the guide
     uint32_t     record_depth;  // vertices of depth below this record samples
     uint32_t     record_capacity;
     float*       train_x;       // 9 per record
@@ -564,6 +565,8 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             path           = w.paths[p];
             Hit const h = load_hit(w, i);
 
+            // the training record of the previous vertex's direction sample
+            uint32_t const prev_record = Guided ? path.record : dev::Null;
             if (Guided && dev::Null != path.record) {
                 // radiance before this vertex: what follows is the previous sample's
                 float* rr   = w.rec_r + 3 * size_t(path.record);
@@ -642,6 +645,15 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                         }
                         for (int k = 0; k < 3; ++k) {
                             path.radiance[k] += path.throughput[k] * (weight * m.emission[k]);
+                        }
+                        if (Guided && dev::Null != prev_record) {
+                            // the guide learns the whole incident radiance: the
+                            // emission found by the sampled direction counts
+                            // without its MIS weight
+                            float* rr = w.rec_r + 3 * size_t(prev_record);
+                            for (int k = 0; k < 3; ++k) {
+                                rr[k] -= path.throughput[k] * ((1.f - weight) * m.emission[k]);
+                            }
                         }
                     }
                     live = false;
@@ -722,7 +734,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                     s.evaluate(wi, f);
                     float const light_pdf = sl / (cl * L.area) * pick.pdf;
                     float       dir_pdf   = f[3];
-                    if (Guided && !in_medium) {
+                    if (Guided && !in_medium && depth < w.guide_depth) {
                         dev::Vmf_mixture mix;
                         mix.set(w.guide_raw + 32 * size_t(i));
                         dir_pdf = w.fraction * mix.pdf(wi) + (1.f - w.fraction) * f[3];
@@ -763,7 +775,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             // BSDF importance sample; guided: one-sample mixture of the guide
             // (probability fraction) and the BSDF
             dev::Bxdf_sample bs;
-            if (Guided && !in_medium) {
+            if (Guided && !in_medium && depth < w.guide_depth) {
                 dev::Vmf_mixture mix;
                 mix.set(w.guide_raw + 32 * size_t(i));
                 float const pick = path.rng.uniform();
@@ -913,8 +925,9 @@ __global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q, uin
 }
 
 // Training weight of every record: average over rgb of (radiance the path
-// gathered after the record's vertex) / (throughput before its direction
-// sample) = incident radiance x reflection / sampling pdf.
+// gathered after the record's vertex, the emitter its direction hit counted
+// without MIS weight) / (throughput before its direction sample) = incident
+// radiance x reflection / sampling pdf.
 __global__ void __launch_bounds__(256) finish_records(Wave w) {
     uint32_t const count  = min(w.counters[C_records], w.record_capacity);
     uint32_t const stride = gridDim.x * blockDim.x;
@@ -1870,10 +1883,14 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     if (int32_t const st = prepare_trace(scene.bvh, trace, std::max(num_paths, num_shadow)); st) {
         return st;
     }
+    uint32_t guide_depth  = 0;
     uint32_t record_depth = 0;
     uint32_t records      = 0;
     if (guided) {
-        record_depth = (gs->train && gs->train_steps) ? std::min(gs->record_depth, set.max_bounces + 1) : 0;
+        guide_depth  = gs->guide_depth ? std::min(gs->guide_depth, set.max_bounces + 1) : set.max_bounces + 1;
+        record_depth = (gs->train && gs->train_steps)
+                           ? std::min(std::min(gs->record_depth, set.max_bounces + 1), guide_depth)
+                           : 0;
         records      = uint32_t(std::min<uint64_t>(uint64_t(num_paths) * record_depth, 1ull << 26));
         if (int32_t const st = state.reserve_guiding(num_paths, std::max<uint32_t>(records, 1)); st) {
             return st;
@@ -1936,6 +1953,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.fraction        = guided ? gs->fraction : 0.f;
     w.guide_raw       = state.guide_raw;
     w.guide_x         = state.guide_x;
+    w.guide_depth     = guide_depth;
     w.record_depth    = record_depth;
     w.record_capacity = records;
     w.train_x         = state.train_x;
@@ -1987,7 +2005,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
                 hipLaunchKernelGGL(medium_track, dim3(grid_path), dim3(256), 0, s, w, q, d);
             }
             hipLaunchKernelGGL(reset_queues, dim3(1), dim3(1), 0, s, state.counters, q ^ 1);
-            if (guided) {
+            if (guided && d < guide_depth) {
                 hipLaunchKernelGGL(guide_inputs, dim3(grid_path), dim3(256), 0, s, w, q);
                 if (int32_t const st = guide_forward_device_count(
                         *net, state.guide_x, &state.counters[C_queue0 + q], num_paths,
@@ -1995,6 +2013,9 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
                     st) {
                     return st;
                 }
+                hipLaunchKernelGGL(shade<true>, dim3(grid_path), dim3(256), 0, s, w, q, d);
+            } else if (guided && d <= record_depth) {
+                // BSDF sampling; closes the records of the vertex before
                 hipLaunchKernelGGL(shade<true>, dim3(grid_path), dim3(256), 0, s, w, q, d);
             } else {
                 hipLaunchKernelGGL(shade<false>, dim3(grid_path), dim3(256), 0, s, w, q, d);

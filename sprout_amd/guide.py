@@ -46,6 +46,14 @@ class Guide:
         assert p.size == NUM_PARAMS
         check(_lib.load().mi_guide_set_params(self.handle, p.ctypes.data), "mi_guide_set_params")
 
+    def set_frequencies(self, freq) -> None:
+        """The encoding multipliers, (9, 12): feature pair (i, j) is sin / cos
+        of pi freq[i][j] x_i."""
+        f = np.ascontiguousarray(freq, np.float32)
+        assert f.size == 9 * 12
+        check(_lib.load().mi_guide_set_frequencies(self.handle, f.ctypes.data),
+              "mi_guide_set_frequencies")
+
     def params(self) -> np.ndarray:
         p = np.zeros(NUM_PARAMS, np.float32)
         check(_lib.load().mi_guide_get_params(self.handle, p.ctypes.data), "mi_guide_get_params")

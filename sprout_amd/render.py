@@ -147,10 +147,11 @@ def render_guided(scene: Scene, cam: _lib.MiCamera, spp: int, guide, fraction: f
                   learning_rate: float = 1e-3, record_depth: int = 4, samples_per_wave: int = 1,
                   min_bounces: int = 4, max_bounces: int = 8, seed: int = 1,
                   first_sample: int = 0, accum: torch.Tensor | None = None, stats: bool = False,
-                  graph: bool = True):
+                  graph: bool = True, guide_depth: int = 0):
     """mi_render_guided: as render(), sampling directions from a one-sample
-    mixture of the guide (probability fraction) and the BSDF, and training
-    the guide online after every wave when train is set."""
+    mixture of the guide (probability fraction) and the BSDF at vertices of
+    depth below guide_depth (0: all), and training the guide online after
+    every wave when train is set."""
     if accum is None:
         accum = torch.zeros((cam.height, cam.width, 3), dtype=torch.float64, device="cuda")
     if accum.dtype != torch.float64 or not accum.is_contiguous() or \
@@ -159,7 +160,7 @@ def render_guided(scene: Scene, cam: _lib.MiCamera, spp: int, guide, fraction: f
     st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
                                0 if graph else _lib.MI_RENDER_NO_GRAPH, seed)
     gs = _lib.MiGuideSettings(guide.handle, fraction, 1 if train else 0, train_steps, batch,
-                              learning_rate, record_depth)
+                              learning_rate, record_depth, guide_depth)
     out = _lib.MiRenderStats() if stats else None
     check(_lib.load().mi_render_guided(scene.handle, ctypes.byref(cam), ctypes.byref(st),
                                        ctypes.byref(gs), ctypes.c_void_p(accum.data_ptr()),
