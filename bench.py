@@ -320,7 +320,8 @@ def instance_trace(steps: int, warmup: int, cpu: bool = False):
     return out
 
 
-SLIT_REF = os.path.join(ROOT, "tests", "golden", "slit_room_ref_256.npz")
+SLIT_REF = os.path.join(ROOT, "tests", "golden", "slit_room_ref_1024.npz")
+SLIT_REF_256 = os.path.join(ROOT, "tests", "golden", "slit_room_ref_256.npz")
 
 
 def rel_mse(x, r, eps=1e-2):
@@ -329,37 +330,45 @@ def rel_mse(x, r, eps=1e-2):
 
 
 def slit_equal_time(seconds: float):
-    """Slit room (light behind walls open through a 0.05-wide slit), 256 x 256,
-    max depth 8: plain MIS and guided MIS (trained online from scratch inside
-    the same time budget) each render for `seconds`; relMSE of each against
-    the 65536-spp FP64 reference image."""
+    """Slit room (light behind walls open through a 0.05-wide slit), max depth
+    8, at 1024 x 1024 (256 x 256 when only that reference exists): plain MIS
+    and guided MIS (trained online from scratch inside the same time budget:
+    one-spp waves with 4 Adam steps of 65 536 records for the first fifth of
+    the time, then 4-spp waves with the frozen guide at the first vertex)
+    each render for `seconds`; relMSE of each against the 65536-spp FP64
+    reference image."""
     import torch
 
     from sprout_amd import guide, render, scenes
 
-    if not os.path.exists(SLIT_REF):
+    path = SLIT_REF if os.path.exists(SLIT_REF) else SLIT_REF_256
+    if not os.path.exists(path):
         return {"skipped": "reference image missing (tests/golden/make_slit_reference.py)"}
-    with np.load(SLIT_REF, allow_pickle=False) as z:
+    with np.load(path, allow_pickle=False) as z:
         ref = z["image"].astype(np.float64)
         ref_spp = int(z["spp"])
+    size = ref.shape[0]
+    spw = 4 if size >= 1024 else 16
     pos, idx, tmat, mats = scenes.room(0.05)
     sc = render.Scene(pos, idx, tmat, mats)
-    cam = render.camera(256, 256, **scenes.ROOM_CAMERA)
+    cam = render.camera(size, size, **scenes.ROOM_CAMERA)
 
-    # warm up both paths (kernels, buffers)
+    # warm up both paths (kernels, buffers, graphs)
     g = guide.Guide(7)
-    render.render(sc, cam, 4, samples_per_wave=4, seed=2)
-    render.render_guided(sc, cam, 4, g, samples_per_wave=4, seed=3)
+    render.render(sc, cam, spw, samples_per_wave=spw, seed=2)
+    render.render_guided(sc, cam, 1, g, train=True, batch=65536, samples_per_wave=1, seed=3,
+                         guide_depth=1, record_depth=1)
+    render.render_guided(sc, cam, spw, g, train=False, samples_per_wave=spw, seed=3, guide_depth=1)
     torch.cuda.synchronize()
     g.release()
 
-    acc = torch.zeros((256, 256, 3), dtype=torch.float64, device="cuda")
+    acc = torch.zeros((size, size, 3), dtype=torch.float64, device="cuda")
     spp = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        render.render(sc, cam, 16, samples_per_wave=16, seed=11, first_sample=spp, accum=acc)
+        render.render(sc, cam, spw, samples_per_wave=spw, seed=11, first_sample=spp, accum=acc)
         torch.cuda.synchronize()
-        spp += 16
+        spp += spw
     t_plain = time.perf_counter() - t0
     e_plain = rel_mse(acc.cpu().numpy() / spp, ref)
 
@@ -369,26 +378,30 @@ def slit_equal_time(seconds: float):
     trained = 0.0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        _, st = render.render_guided(sc, cam, 4, g, fraction=0.5, train=True, train_steps=4,
-                                     learning_rate=2e-3, record_depth=4, samples_per_wave=4,
-                                     seed=12, first_sample=gspp, accum=acc, stats=True)
-        gspp += 4
+        train = time.perf_counter() - t0 < 0.2 * seconds
+        n = 1 if train else spw
+        _, st = render.render_guided(sc, cam, n, g, fraction=0.5, train=train, train_steps=4,
+                                     batch=65536, learning_rate=2e-3, record_depth=1, guide_depth=1,
+                                     samples_per_wave=n, seed=12, first_sample=gspp, accum=acc,
+                                     stats=True)
+        torch.cuda.synchronize()
+        gspp += n
         trained += st.trained
-    torch.cuda.synchronize()
     t_guided = time.perf_counter() - t0
     e_guided = rel_mse(acc.cpu().numpy() / gspp, ref)
     g.release()
     sc.release()
     return {
-        "workload": "slit room 256x256, max depth 8, RR from depth 4; reference 65536 spp FP64",
+        "workload": f"slit room {size}x{size}, max depth 8, RR from depth 4; reference {ref_spp} spp FP64",
         "seconds_plain": round(t_plain, 3),
         "seconds_guided": round(t_guided, 3),
         "spp_plain": spp,
         "spp_guided": gspp,
-        "relmse_plain": round(e_plain, 6),
-        "relmse_guided": round(e_guided, 6),
+        "relmse_plain": round(e_plain, 8),
+        "relmse_guided": round(e_guided, 8),
         "relmse_ratio_plain_over_guided": round(e_plain / e_guided, 3),
         "relmse_eps": 1e-2,
+        "guide": "fraction 0.5 at the first vertex; trained the first 20 % of the time",
         "guide_samples_trained": trained,
         "reference_spp": ref_spp,
     }
@@ -401,7 +414,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--width", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--guide-seconds", type=float, default=2.0,
+    ap.add_argument("--guide-seconds", type=float, default=6.0,
                     help="time budget of each side of the guided vs plain comparison")
     ap.add_argument("--part", choices=["all", "room", "volume", "guiding", "instances", "batch",
                                        "batch_instanced"],

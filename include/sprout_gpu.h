@@ -154,10 +154,13 @@ typedef struct mi_trace_config {
     uint32_t block_size;       /* threads per workgroup, multiple of 64 (default 256) */
     uint32_t blocks_per_cu;    /* workgroups per CU of the persistent grid (default 8) */
     uint32_t kernel;           /* 0: per-lane ray refill, 1: static grid-stride,
-                                  2: refill with inner/leaf phases (while-while, default) */
+                                  2: refill with inner/leaf phases (while-while),
+                                  3: as 2 with stack entries that carry the child's node word
+                                  and box entry distance (default; trees of < 2^22 nodes and
+                                  slots, else 2) */
     uint32_t refill_lanes;     /* kernels 0, 2: hand out rays once this many lanes are idle (24) */
     uint32_t claim_batch;      /* kernels 0, 2: rays a wave claims per atomic (64) */
-    uint32_t lds_entries;      /* stack entries per lane kept in LDS, <= 32 (default 16;
+    uint32_t lds_entries;      /* stack entries per lane kept in LDS, <= 32 (default 10;
                                   0: up to 32); deeper entries spill to global memory */
 } mi_trace_config;
 

@@ -354,7 +354,7 @@ int32_t mi_set_trace_config(mi_trace_config const* config) {
     uint32_t const bpc   = config->blocks_per_cu ? config->blocks_per_cu : defaults.blocks_per_cu;
     uint32_t const lanes = config->refill_lanes ? config->refill_lanes : defaults.refill_lanes;
     uint32_t const batch = config->claim_batch ? config->claim_batch : defaults.claim_batch;
-    if (block % 64 || block > 1024 || bpc > 64 || lanes > 64 || config->kernel > 2 ||
+    if (block % 64 || block > 1024 || bpc > 64 || lanes > 64 || config->kernel > 3 ||
         batch > 65536 || config->lds_entries > 32) {
         return fail(MI_ERR_ARGS,
                     "mi_set_trace_config: block %u, blocks per CU %u, kernel %u, refill lanes %u, "

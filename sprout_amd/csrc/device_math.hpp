@@ -84,6 +84,31 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, F3 o, F3 inv, float m
     return (tmax >= min_t) & (max_t >= tmin) & (tmax >= tmin);
 }
 
+// slab with the box's entry distance tmin: while the ray's min_t and
+// direction stay, the test at a smaller max_t' is exactly max_t' >= tmin
+// (the other two terms do not involve max_t).  tmin is never NaN.
+__device__ __forceinline__ bool slab_entry(float4 lo, float4 hi, F3 o, F3 inv, float min_t,
+                                           float max_t, float& tmin_out) {
+    float const l1x = (lo.x - o.x) * inv.x;
+    float const l1y = (lo.y - o.y) * inv.y;
+    float const l1z = (lo.z - o.z) * inv.z;
+    float const l2x = (hi.x - o.x) * inv.x;
+    float const l2y = (hi.y - o.y) * inv.y;
+    float const l2z = (hi.z - o.z) * inv.z;
+
+    float const tmax_x = fmaxf(fminf(l1x, inf()), fminf(l2x, inf()));
+    float const tmax_y = fmaxf(fminf(l1y, inf()), fminf(l2y, inf()));
+    float const tmax_z = fmaxf(fminf(l1z, inf()), fminf(l2z, inf()));
+    float const tmin_x = fminf(fmaxf(l1x, -inf()), fmaxf(l2x, -inf()));
+    float const tmin_y = fminf(fmaxf(l1y, -inf()), fmaxf(l2y, -inf()));
+    float const tmin_z = fminf(fmaxf(l1z, -inf()), fmaxf(l2z, -inf()));
+
+    float const tmax = fminf(fminf(tmax_x, tmax_y), tmax_z);
+    float const tmin = fmaxf(fmaxf(tmin_x, tmin_y), tmin_z);
+    tmin_out         = tmin;
+    return (tmax >= min_t) & (max_t >= tmin) & (tmax >= tmin);
+}
+
 // triangle::intersect (core/scene/shape/triangle/triangle_primitive_mt.inl:61-93)
 // with e1 = b - a and e2 = c - a precomputed (the same rounded subtraction).
 // On a hit returns true and sets t, u, v.

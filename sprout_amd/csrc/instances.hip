@@ -239,6 +239,269 @@ __global__ void __launch_bounds__(Block) trace_instances(Tlas_view t, mi_ray con
     }
 }
 
+// ---- two-word stack traversal ----
+//
+// The same visiting order with 64-byte sibling-pair fetches: at an inner
+// node both children are box-tested, the far one (when both are hit) is
+// pushed as {its node word, its box entry distance} and the near one taken.
+// A popped child is skipped when its entry distance exceeds the current
+// max_t, which is exactly the reference's box test of it at that max_t
+// (slab_entry).  A node word packs data (< 2^22), axis and count, so popped
+// children need no node fetch.  The mesh walk runs above a sentinel word on
+// the same stack.  The first Lds_entries entries of a lane live in LDS, the
+// rest in a per-thread global area.
+
+constexpr uint32_t Lds_entries   = 12;
+constexpr uint32_t Block2        = 64;
+constexpr uint32_t Blocks_per_cu = 24;
+constexpr uint32_t Data_bits     = 22;
+constexpr uint32_t Sentinel_word = 0xFFFFFFFFu;  // no valid node word (its slot range ends past 2^22)
+
+__device__ __forceinline__ uint32_t pack_node(uint32_t data, uint32_t meta) {
+    return data | ((node_axis(meta) & 3u) << Data_bits) | (node_num(meta) << 24);
+}
+
+__device__ __forceinline__ void unpack_node(uint32_t e, uint32_t& data, uint32_t& meta) {
+    data = e & ((1u << Data_bits) - 1u);
+    meta = ((e >> Data_bits) & 3u) | ((e >> 24) << 8);
+}
+
+struct Pair_stack {
+    uint32_t* lds;    // 2 * Lds_entries * Block2 words, entry-major
+    uint32_t* spill;  // 2 * spill entries * total words, entry-major
+    uint32_t  gid;
+    uint32_t  total;
+    uint32_t  sp;
+
+    __device__ void push(uint32_t node, float tmin) {
+        if (sp < Lds_entries) {
+            lds[(2 * sp) * Block2 + threadIdx.x]     = node;
+            lds[(2 * sp + 1) * Block2 + threadIdx.x] = __float_as_uint(tmin);
+        } else {
+            size_t const k                  = 2 * size_t(sp - Lds_entries);
+            spill[k * total + gid]       = node;
+            spill[(k + 1) * total + gid] = __float_as_uint(tmin);
+        }
+        ++sp;
+    }
+
+    __device__ void pop(uint32_t& node, float& tmin) {
+        --sp;
+        if (sp < Lds_entries) {
+            node = lds[(2 * sp) * Block2 + threadIdx.x];
+            tmin = __uint_as_float(lds[(2 * sp + 1) * Block2 + threadIdx.x]);
+        } else {
+            size_t const k = 2 * size_t(sp - Lds_entries);
+            node           = spill[k * total + gid];
+            tmin           = __uint_as_float(spill[(k + 1) * total + gid]);
+        }
+    }
+};
+
+// Pops until a pending child passes its box test; false at a sentinel (which
+// is consumed) or an empty stack.
+__device__ __forceinline__ bool pop_pending(Pair_stack& stack, Iray const& r, uint32_t& data,
+                                            uint32_t& meta) {
+    while (stack.sp > 0) {
+        uint32_t e;
+        float    tmin;
+        stack.pop(e, tmin);
+        if (Sentinel_word == e) {
+            return false;
+        }
+        if (r.max_t >= tmin) {
+            unpack_node(e, data, meta);
+            return true;
+        }
+    }
+    return false;
+}
+
+// One inner node of either level (node data = first of the sibling pair).
+__device__ __forceinline__ bool pair_step(float4 const* __restrict__ nodes, Pair_stack& stack,
+                                          Iray const& r, uint32_t& data, uint32_t& meta) {
+    uint32_t const a   = data;
+    float4 const   alo = nodes[2 * size_t(a) + 0];
+    float4 const   ahi = nodes[2 * size_t(a) + 1];
+    float4 const   blo = nodes[2 * size_t(a) + 2];
+    float4 const   bhi = nodes[2 * size_t(a) + 3];
+    float          ta, tb;
+    bool const     ha   = dev::slab_entry(alo, ahi, r.o, r.inv, r.min_t, r.max_t, ta);
+    bool const     hb   = dev::slab_entry(blo, bhi, r.o, r.inv, r.min_t, r.max_t, tb);
+    bool const     flip = (r.sign_bits >> node_axis(meta)) & 1u;
+    bool const     hn   = flip ? hb : ha;
+    bool const     hf   = flip ? ha : hb;
+    if (hn | hf) {
+        if (hn & hf) {
+            stack.push(pack_node(__float_as_uint(flip ? alo.w : blo.w),
+                                 __float_as_uint(flip ? ahi.w : bhi.w)),
+                       flip ? ta : tb);
+        }
+        bool const take_b = hn ? flip : !flip;
+        data              = __float_as_uint(take_b ? blo.w : alo.w);
+        meta              = __float_as_uint(take_b ? bhi.w : ahi.w);
+        return true;
+    }
+    return pop_pending(stack, r, data, meta);
+}
+
+// Tree::intersect / intersect_p of one mesh above a sentinel; returns with
+// the sentinel popped.  Closest: shrinks r.max_t on every hit.
+template <bool Any>
+__device__ bool mesh_pairs(float4 const* __restrict__ nodes, float4 const* __restrict__ tris,
+                           Iray& r, Pair_stack& stack, uint32_t& index, uint32_t& slot, float& u,
+                           float& v) {
+    stack.push(Sentinel_word, 0.f);
+    bool         hit  = false;
+    float4 const rlo  = nodes[0];
+    float4 const rhi  = nodes[1];
+    uint32_t     data = __float_as_uint(rlo.w);
+    uint32_t     meta = __float_as_uint(rhi.w);
+    bool         live = dev::slab(rlo, rhi, r.o, r.inv, r.min_t, r.max_t);
+    if (!live) {
+        uint32_t e;
+        float    t;
+        stack.pop(e, t);  // the sentinel
+        return false;
+    }
+    while (live) {
+        uint32_t const num = node_num(meta);
+        if (0 == num) {
+            live = pair_step(nodes, stack, r, data, meta);
+            continue;
+        }
+        for (uint32_t i = data, end = data + num; i < end; ++i) {
+            float4 const ta = tris[3 * size_t(i) + 0];
+            float4 const t1 = tris[3 * size_t(i) + 1];
+            float4 const t2 = tris[3 * size_t(i) + 2];
+            float        t, tu, tv;
+            if (dev::triangle({ta.x, ta.y, ta.z}, {t1.x, t1.y, t1.z}, {t2.x, t2.y, t2.z}, r.o, r.d,
+                              r.min_t, r.max_t, t, tu, tv)) {
+                hit = true;
+                if (Any) {
+                    // unwind to the sentinel
+                    uint32_t e;
+                    float    tm;
+                    do {
+                        stack.pop(e, tm);
+                    } while (Sentinel_word != e);
+                    return true;
+                }
+                r.max_t = t;
+                u       = tu;
+                v       = tv;
+                index   = __float_as_uint(ta.w);
+                slot    = i;
+            }
+        }
+        live = pop_pending(stack, r, data, meta);
+    }
+    return hit;
+}
+
+template <bool Any>
+__global__ void __launch_bounds__(Block2) trace_instances_pairs(Tlas_view t, uint32_t* spill,
+                                                                uint32_t total, mi_ray const* rays,
+                                                                uint32_t num_rays,
+                                                                uint32_t const* num_rays_dev,
+                                                                void* out) {
+    __shared__ uint32_t lds[2 * Lds_entries * Block2];
+    Pair_stack          stack;
+    stack.lds   = lds;
+    stack.spill = spill;
+    stack.gid   = blockIdx.x * blockDim.x + threadIdx.x;
+    stack.total = total;
+
+    if (num_rays_dev) {
+        num_rays = min(num_rays, *num_rays_dev);
+    }
+    uint32_t const stride = gridDim.x * blockDim.x;
+    for (uint32_t ri = stack.gid; ri < num_rays; ri += stride) {
+        float4 const* rp = reinterpret_cast<float4 const*>(rays) + 2 * size_t(ri);
+        float4 const  ro = rp[0];
+        float4 const  rd = rp[1];
+        Iray          w;
+        w.o     = {ro.x, ro.y, ro.z};
+        w.min_t = ro.w;
+        w.d     = {rd.x, rd.y, rd.z};
+        w.max_t = rd.w;
+        set_inverse(w);
+
+        uint32_t instance = Null;
+        uint32_t index    = Null;
+        uint32_t slot     = Null;
+        float    u = 0.f, v = 0.f;
+        bool     any = false;
+
+        stack.sp      = 0;
+        uint32_t data = 0, meta = 0;
+        bool     live = false;
+        if (t.num_nodes) {
+            float4 const rlo = t.nodes[0];
+            float4 const rhi = t.nodes[1];
+            data             = __float_as_uint(rlo.w);
+            meta             = __float_as_uint(rhi.w);
+            live             = dev::slab(rlo, rhi, w.o, w.inv, w.min_t, w.max_t);
+        }
+        while (live) {
+            uint32_t const num = node_num(meta);
+            if (0 == num) {
+                live = pair_step(t.nodes, stack, w, data, meta);
+                continue;
+            }
+            for (uint32_t i = data, end = data + num; i < end; ++i) {
+                uint32_t const        p   = __float_as_uint(t.slots[3 * size_t(i)].w);
+                Instance_record const& ir = t.instances[p];
+                float4 const blo = make_float4(ir.lo[0], ir.lo[1], ir.lo[2], 0.f);
+                float4 const bhi = make_float4(ir.hi[0], ir.hi[1], ir.hi[2], 0.f);
+                // Prop::intersect: the prop's world box first
+                if (!dev::slab(blo, bhi, w.o, w.inv, w.min_t, w.max_t)) {
+                    continue;
+                }
+                Iray o;
+                o.o     = xform_point(ir.world_to_object, w.o);
+                o.d     = xform_vector(ir.world_to_object, w.d);
+                o.min_t = w.min_t;
+                o.max_t = w.max_t;
+                set_inverse(o);
+                Mesh_view const mv  = t.meshes[ir.mesh];
+                uint32_t        idx = Null, hslot = Null;
+                float           hu = 0.f, hv = 0.f;
+                if (mesh_pairs<Any>(mv.nodes, mv.tris, o, stack, idx, hslot, hu, hv)) {
+                    if (Any) {
+                        any = true;
+                        break;
+                    }
+                    w.max_t  = o.max_t;
+                    instance = p;
+                    index    = idx;
+                    slot     = hslot;
+                    u        = hu;
+                    v        = hv;
+                }
+            }
+            if (Any && any) {
+                break;
+            }
+            live = pop_pending(stack, w, data, meta);
+        }
+
+        if (Any) {
+            static_cast<uint32_t*>(out)[ri] = any ? 1u : 0u;
+        } else {
+            mi_instance_hit h;
+            h.t        = w.max_t;
+            h.u        = Null != index ? u : 0.f;
+            h.v        = Null != index ? v : 0.f;
+            h.triangle = index;
+            h.instance = instance;
+            h.slot     = slot;
+            h.pad[0] = h.pad[1] = 0;
+            static_cast<mi_instance_hit*>(out)[ri] = h;
+        }
+    }
+}
+
 // AABB::transform (aabb.inl:263-280): per axis the products of the box's
 // bounds with the matrix row, min / max, then (x + y) + (z + translation)
 void transform_box(float const lo[3], float const hi[3], float const m[16], float out_lo[3],
@@ -257,6 +520,9 @@ void transform_box(float const lo[3], float const hi[3], float const m[16], floa
 
 void Tlas::release() {
     tree.release();
+    (void)hipFree(spill);
+    spill       = nullptr;
+    spill_total = 0;
     (void)hipFree(instances);
     (void)hipFree(meshes);
     instances = nullptr;
@@ -347,6 +613,27 @@ int32_t create_tlas(DeviceBvh const* const* meshes, uint32_t num_meshes, uint32_
     }
     out.num_instances = num_instances;
     out.num_meshes    = num_meshes;
+
+    // the two-word stack traversal: every tree's node words must pack, and
+    // the deepest stack beyond its LDS part spills to a per-thread area
+    bool packs = out.tree.num_nodes < (1u << Data_bits) && out.tree.num_slots < (1u << Data_bits);
+    for (uint32_t k = 0; k < num_meshes; ++k) {
+        packs = packs && meshes[k]->num_nodes < (1u << Data_bits) &&
+                meshes[k]->num_slots < (1u << Data_bits);
+    }
+    if (packs) {
+        int device = 0, cus = 0;
+        e = hipGetDevice(&device);
+        if (!e) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        uint32_t const total  = uint32_t(std::max(cus, 1)) * Blocks_per_cu * Block2;
+        uint32_t const deeper = out.max_stack > Lds_entries ? out.max_stack - Lds_entries : 0;
+        if (!e) e = hipMalloc(&out.spill, std::max<size_t>(1, size_t(2) * deeper * total) * sizeof(uint32_t));
+        if (e) {
+            out.release();
+            return fail(MI_ERR_HIP, "instances: stack area: %s", hipGetErrorString(e));
+        }
+        out.spill_total = total;
+    }
     return MI_OK;
 }
 
@@ -355,7 +642,22 @@ int32_t trace_instances_launch(Tlas const& t, uint32_t num_cus, mi_ray const* ra
     if (0 == n) {
         return MI_OK;
     }
-    Tlas_view const v      = tlas_view(t);
+    Tlas_view const v = tlas_view(t);
+    if (t.spill) {
+        uint64_t const wanted = (uint64_t(n) + Block2 - 1) / Block2;
+        uint32_t const blocks = uint32_t(std::min<uint64_t>(wanted, t.spill_total / Block2));
+        if (any_hit) {
+            hipLaunchKernelGGL(trace_instances_pairs<true>, dim3(blocks), dim3(Block2), 0, stream, v,
+                               t.spill, t.spill_total, rays, n, n_dev, out);
+        } else {
+            hipLaunchKernelGGL(trace_instances_pairs<false>, dim3(blocks), dim3(Block2), 0, stream, v,
+                               t.spill, t.spill_total, rays, n, n_dev, out);
+        }
+        if (hipError_t const e = hipGetLastError(); e) {
+            return fail(MI_ERR_HIP, "instance trace: %s", hipGetErrorString(e));
+        }
+        return MI_OK;
+    }
     uint64_t const  wanted = (uint64_t(n) + Block - 1) / Block;
     uint32_t const blocks = uint32_t(std::min<uint64_t>(wanted, uint64_t(num_cus) * 8));
     size_t const   lds    = size_t(Stack_entries) * Block * sizeof(uint32_t);

@@ -68,6 +68,8 @@ struct Tlas {
     uint32_t                     num_instances = 0;
     uint32_t                     num_meshes    = 0;
     uint32_t                     max_stack     = 0;
+    uint32_t*                    spill         = nullptr;  // stack area of the pair traversal
+    uint32_t                     spill_total   = 0;        // its threads (0: not used)
 
     void release();
 };

@@ -23,6 +23,9 @@
 //   * trace_reference: the reference's loop verbatim (fetch, test, push far,
 //     descend near), counting node fetches and triangle tests per ray; this
 //     gives the algorithmic bytes per ray.
+//   * trace_refill(_packed): persistent waves that refill idle lanes with new
+//     rays; the packed variant's stack entries carry the far child's node
+//     word and box entry distance, so a pop needs no node fetch.
 // The traversal stack lives in LDS (Lds_entries per lane, laid out
 // entry-major so a wave's 64 lanes hit 64 different banks) and spills to a
 // per-thread global area beyond that, sized from the tree's depth.
@@ -522,6 +525,234 @@ __global__ void trace_refill(Launch p) {
     }
 }
 
+// Two-word stack entries {far child's node word, its box entry distance}:
+// the node word packs the child's data (< 2^22) with its axis and primitive
+// count, so a popped child continues without fetching its node again, and a
+// popped child whose entry distance exceeds the current max_t is skipped: the
+// reference's box test of that child at this max_t is exactly max_t >= tmin
+// (slab_entry), the rest of it held when the child was pushed.
+constexpr uint32_t Packed_data_bits = 22;
+
+__device__ __forceinline__ uint32_t pack_node(uint32_t data, uint32_t meta) {
+    // the axis of an inner node is 0..2; a leaf's is never read
+    return data | ((node_axis(meta) & 3u) << Packed_data_bits) | (node_num(meta) << 24);
+}
+
+__device__ __forceinline__ void unpack_node(uint32_t e, uint32_t& data, uint32_t& meta) {
+    data = e & ((1u << Packed_data_bits) - 1u);
+    meta = ((e >> Packed_data_bits) & 3u) | ((e >> 24) << 8);
+}
+
+struct Stack2 {
+    uint32_t* lds;          // 2 * lds_entries * block_size words, entry-major
+    uint32_t* spill;        // 2 * spill entries * total_threads words, entry-major
+    uint32_t  lds_entries;
+    uint32_t  lane;
+    uint32_t  block_size;
+    uint32_t  gid;
+    uint32_t  total_threads;
+    uint32_t  sp;
+
+    __device__ void init(uint32_t* lds_base, uint32_t* spill_base, uint32_t entries,
+                         uint32_t threads) {
+        lds           = lds_base;
+        spill         = spill_base;
+        lds_entries   = entries;
+        lane          = threadIdx.x;
+        block_size    = blockDim.x;
+        gid           = blockIdx.x * blockDim.x + threadIdx.x;
+        total_threads = threads;
+        sp            = 0;
+    }
+
+    __device__ void push(uint32_t node, float tmin) {
+        if (sp < lds_entries) {
+            lds[(2 * sp) * block_size + lane]     = node;
+            lds[(2 * sp + 1) * block_size + lane] = __float_as_uint(tmin);
+        } else {
+            size_t const k                                 = 2 * size_t(sp - lds_entries);
+            spill[k * total_threads + gid]       = node;
+            spill[(k + 1) * total_threads + gid] = __float_as_uint(tmin);
+        }
+        ++sp;
+    }
+
+    __device__ void pop(uint32_t& node, float& tmin) {
+        --sp;
+        if (sp < lds_entries) {
+            node = lds[(2 * sp) * block_size + lane];
+            tmin = __uint_as_float(lds[(2 * sp + 1) * block_size + lane]);
+        } else {
+            size_t const k = 2 * size_t(sp - lds_entries);
+            node           = spill[k * total_threads + gid];
+            tmin           = __uint_as_float(spill[(k + 1) * total_threads + gid]);
+        }
+    }
+};
+
+__device__ __forceinline__ bool pop_next2(Stack2& stack, Ray const& ray, uint32_t& data,
+                                          uint32_t& meta) {
+    while (stack.sp > 0) {
+        uint32_t e;
+        float    tmin;
+        stack.pop(e, tmin);
+        if (ray.max_t >= tmin) {
+            unpack_node(e, data, meta);
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool inner_step2(float4 const* __restrict__ nodes, Stack2& stack,
+                                            Ray const& ray, uint32_t& data, uint32_t& meta) {
+    uint32_t const a = data;
+
+    float4 const alo = nodes[2 * size_t(a) + 0];
+    float4 const ahi = nodes[2 * size_t(a) + 1];
+    float4 const blo = nodes[2 * size_t(a) + 2];
+    float4 const bhi = nodes[2 * size_t(a) + 3];
+
+    float      ta, tb;
+    bool const ha = dev::slab_entry(alo, ahi, ray.o, ray.inv, ray.min_t, ray.max_t, ta);
+    bool const hb = dev::slab_entry(blo, bhi, ray.o, ray.inv, ray.min_t, ray.max_t, tb);
+
+    bool const flip = (ray.sign_bits >> node_axis(meta)) & 1u;
+    bool const hn   = flip ? hb : ha;
+    bool const hf   = flip ? ha : hb;
+
+    if (hn | hf) {
+        if (hn & hf) {
+            uint32_t const fd = __float_as_uint(flip ? alo.w : blo.w);
+            uint32_t const fm = __float_as_uint(flip ? ahi.w : bhi.w);
+            stack.push(pack_node(fd, fm), flip ? ta : tb);
+        }
+        bool const take_b = hn ? flip : !flip;
+        data              = __float_as_uint(take_b ? blo.w : alo.w);
+        meta              = __float_as_uint(take_b ? bhi.w : ahi.w);
+        return true;
+    }
+    return pop_next2(stack, ray, data, meta);
+}
+
+// trace_refill<Any, true> with the two-word stack (trees of < 2^22 nodes and
+// slots); the same rays in the same order, the same results.
+template <bool Any>
+__global__ void trace_refill_packed(Launch p) {
+    extern __shared__ uint32_t lds_stack[];
+
+    float4 const* __restrict__ nodes = p.nodes;
+    float4 const* __restrict__ tris  = p.tris;
+
+    Stack2 stack;
+    stack.init(lds_stack, p.spill, p.lds_entries, p.total_threads);
+
+    uint32_t const lane     = threadIdx.x & 63;
+    uint64_t const lt_mask  = (uint64_t(1) << lane) - 1;
+    uint32_t const num_rays = p.num_rays_dev ? *p.num_rays_dev : p.num_rays;
+
+    bool     active    = false;
+    bool     exhausted = false;
+    uint32_t pool_next = 0;
+    uint32_t pool_end  = 0;
+    uint32_t r         = 0;
+    Ray      ray;
+    uint32_t data  = 0;
+    uint32_t meta  = 0;
+    uint32_t index = Null;
+    float    u     = 0.f;
+    float    v     = 0.f;
+
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        uint32_t const n_idle = uint32_t(__popcll(idle));
+
+        if ((n_idle >= p.refill_lanes || 64 == n_idle) && !(exhausted && pool_next >= pool_end)) {
+            while (idle) {
+                if (pool_next >= pool_end) {
+                    if (exhausted) {
+                        break;
+                    }
+                    uint32_t const leader =
+                        uint32_t(__ffsll(static_cast<unsigned long long>(idle))) - 1;
+                    uint32_t base = 0;
+                    if (lane == leader) {
+                        base = atomicAdd(p.counter, p.claim_batch);
+                    }
+                    base = __shfl(base, leader);
+                    if (base >= num_rays) {
+                        exhausted = true;
+                        break;
+                    }
+                    pool_next = base;
+                    pool_end  = num_rays - base <= p.claim_batch ? num_rays : base + p.claim_batch;
+                    if (pool_end == num_rays) {
+                        exhausted = true;
+                    }
+                }
+                uint32_t const avail = pool_end - pool_next;
+                uint32_t const want  = uint32_t(__popcll(idle));
+                uint32_t const take  = avail < want ? avail : want;
+                bool const     mine  = (idle >> lane) & 1u;
+                uint32_t const rank  = uint32_t(__popcll(idle & lt_mask));
+                if (mine && rank < take) {
+                    r        = pool_next + rank;
+                    ray      = load_ray(p.rays, r);
+                    index    = Null;
+                    u        = 0.f;
+                    v        = 0.f;
+                    stack.sp = 0;
+
+                    float4 const rlo = nodes[0];
+                    float4 const rhi = nodes[1];
+                    if (dev::slab(rlo, rhi, ray.o, ray.inv, ray.min_t, ray.max_t)) {
+                        data   = __float_as_uint(rlo.w);
+                        meta   = __float_as_uint(rhi.w);
+                        active = true;
+                    } else {
+                        write_result(p, r, Any, false, ray.max_t, Null, 0.f, 0.f);
+                    }
+                }
+                pool_next += take;
+                idle = __ballot(mine && rank >= take);
+            }
+        }
+
+        if (0 == __ballot(active)) {
+            if (exhausted && pool_next >= pool_end) {
+                break;
+            }
+            continue;
+        }
+
+        while (__ballot(active && 0 == node_num(meta))) {
+            if (active && 0 == node_num(meta)) {
+                if (!inner_step2(nodes, stack, ray, data, meta)) {
+                    write_result(p, r, Any, false, ray.max_t, index, u, v);
+                    active = false;
+                }
+            }
+        }
+
+        if (active) {
+            uint32_t const num = node_num(meta);
+            bool           live;
+            bool           hit_p = false;
+            if (0 == num) {
+                live = inner_step2(nodes, stack, ray, data, meta);
+            } else {
+                uint32_t tests = 0;
+                hit_p          = leaf<Any>(tris, data, num, ray, index, u, v, tests);
+                live           = !hit_p && pop_next2(stack, ray, data, meta);
+            }
+            if (!live) {
+                write_result(p, r, Any, hit_p, ray.max_t, index, u, v);
+                active = false;
+            }
+        }
+    }
+}
+
 __global__ void reset_counter(uint32_t* counter) {
     *counter = 0;
 }
@@ -529,6 +760,31 @@ __global__ void reset_counter(uint32_t* counter) {
 }  // namespace
 
 namespace {
+
+// The two-word stack holds node words of trees below 2^22 nodes and slots.
+bool packable(DeviceBvh const& bvh) {
+    return bvh.num_nodes < (1u << Packed_data_bits) && bvh.num_slots < (1u << Packed_data_bits);
+}
+
+// Stack entries per lane in LDS (at most what a workgroup's 160 KB hold) and
+// in the global spill area, and words per entry.
+struct Stack_layout {
+    uint32_t words;
+    uint32_t lds_entries;
+    uint32_t spill_entries;
+};
+
+Stack_layout stack_layout(DeviceBvh const& bvh, TraceResources const& res, bool counts) {
+    uint32_t const entries = stack_entries(bvh.max_depth);
+    Stack_layout   l;
+    l.words                = (!counts && 3 == res.kernel && packable(bvh)) ? 2 : 1;
+    uint32_t const lds_cap = res.lds_entries ? res.lds_entries : Max_lds_entries;
+    uint32_t const fit     = (160u * 1024u) / (l.words * res.block_size * 4u);
+    uint32_t const cap     = lds_cap < fit ? lds_cap : fit;
+    l.lds_entries          = entries < cap ? entries : cap;
+    l.spill_entries        = entries - l.lds_entries;
+    return l;
+}
 
 // num_rays_dev set: the refill kernel reads the ray count on the device and
 // num_rays is only its upper bound (it sizes the grid and the stack spill).
@@ -550,14 +806,14 @@ int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, ui
     uint32_t const blocks  = res.grid_blocks(num_rays);
     uint32_t const threads = blocks * block;
 
-    uint32_t const lds_cap     = res.lds_entries ? res.lds_entries : Max_lds_entries;
-    uint32_t const lds_entries = entries < lds_cap ? entries : lds_cap;
-    uint32_t const spill_entries = entries - lds_entries;
-    if (int32_t const st = res.reserve_spill(size_t(spill_entries) * threads); st) {
+    Stack_layout const sl          = stack_layout(bvh, res, counts != nullptr);
+    uint32_t const     lds_entries = sl.lds_entries;
+    bool const         packed      = 2 == sl.words;
+    if (int32_t const st = res.reserve_spill(size_t(sl.words) * sl.spill_entries * threads); st) {
         return st;
     }
 
-    size_t const lds_bytes = size_t(lds_entries) * block * sizeof(uint32_t);
+    size_t const lds_bytes = size_t(sl.words) * lds_entries * block * sizeof(uint32_t);
 
     Launch p;
     p.nodes         = bvh.nodes;
@@ -580,10 +836,17 @@ int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, ui
         } else {
             hipLaunchKernelGGL(trace_reference<false>, dim3(blocks), dim3(block), lds_bytes, stream, p);
         }
+    } else if (packed) {
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, res.counter);
+        if (any_hit) {
+            hipLaunchKernelGGL(trace_refill_packed<true>, dim3(blocks), dim3(block), lds_bytes, stream, p);
+        } else {
+            hipLaunchKernelGGL(trace_refill_packed<false>, dim3(blocks), dim3(block), lds_bytes, stream, p);
+        }
     } else if (res.kernel != 1 || num_rays_dev) {
         hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, res.counter);
         using Kernel    = void (*)(Launch);
-        bool const ww   = 2 == res.kernel;
+        bool const ww   = 2 <= res.kernel;
         Kernel     kern = any_hit ? (ww ? trace_refill<true, true> : trace_refill<true, false>)
                                   : (ww ? trace_refill<false, true> : trace_refill<false, false>);
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(block), lds_bytes, stream, p);
@@ -608,10 +871,9 @@ int32_t prepare_trace(DeviceBvh const& bvh, TraceResources& res, uint32_t max_ra
                     "tree depth %u needs %u stack entries, more than the reference's %u",
                     bvh.max_depth, entries, Reference_stack_size);
     }
-    uint32_t const threads     = res.grid_blocks(max_rays) * res.block_size;
-    uint32_t const lds_cap     = res.lds_entries ? res.lds_entries : Max_lds_entries;
-    uint32_t const lds_entries = entries < lds_cap ? entries : lds_cap;
-    return res.reserve_spill(size_t(entries - lds_entries) * threads);
+    uint32_t const     threads = res.grid_blocks(max_rays) * res.block_size;
+    Stack_layout const sl      = stack_layout(bvh, res, false);
+    return res.reserve_spill(size_t(sl.words) * sl.spill_entries * threads);
 }
 
 int32_t launch_trace(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays,

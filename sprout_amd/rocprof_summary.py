@@ -29,7 +29,7 @@ import glob
 import os
 import statistics
 
-KERNEL = "trace_refill<false, true>"
+KERNEL = "trace_refill_packed<false>"
 COPY_BYTES = 256 << 20
 
 

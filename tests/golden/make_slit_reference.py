@@ -2,12 +2,13 @@
 comparison is measured against (needs a GPU).
 
 Slit room (the box room with its light enclosed behind walls open only
-through a 0.05-wide slit), 256 x 256, plain MIS (mi_render), max depth 8,
-Russian roulette from depth 4, 65536 samples per pixel accumulated in FP64,
-seed 1000 (independent of every render compared with it).  The image is
-stored as float32 radiance.
+through a 0.05-wide slit), 256 x 256 (or --size), plain MIS (mi_render), max
+depth 8, Russian roulette from depth 4, 65536 samples per pixel accumulated
+in FP64, seed 1000 (independent of every render compared with it).  The
+image is stored as float32 radiance (float16 with --half: 2^-11 relative,
+far below the errors it measures).
 
-    python tests/golden/make_slit_reference.py [--spp 65536] [--out PATH]
+    python tests/golden/make_slit_reference.py [--spp 65536] [--size 256] [--half] [--out PATH]
 """
 
 from __future__ import annotations
@@ -22,20 +23,22 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-SIZE = 256
-
 
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--spp", type=int, default=65536)
     ap.add_argument("--chunk", type=int, default=1024)
-    ap.add_argument("--out", default=os.path.join(HERE, "slit_room_ref_256.npz"))
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--half", action="store_true")
+    ap.add_argument("--out", default=None, help="default: slit_room_ref_<size>.npz beside this file")
     ap.add_argument("--partial", default=None,
                     help="FP64 running sum kept here after every chunk; an existing one is resumed")
     ap.add_argument("--resume", default=None,
                     help="start from this running sum (written by --partial) instead")
     ap.add_argument("--seconds", type=float, default=1e9, help="stop (resumable) after this long")
     args = ap.parse_args()
+    size = args.size
+    out = args.out or os.path.join(HERE, f"slit_room_ref_{size}.npz")
 
     import torch
 
@@ -44,8 +47,8 @@ def main() -> None:
     bvh.init(0)
     pos, idx, tmat, mats = scenes.room(0.05)
     sc = render.Scene(pos, idx, tmat, mats)
-    cam = render.camera(SIZE, SIZE, **scenes.ROOM_CAMERA)
-    acc = torch.zeros((SIZE, SIZE, 3), dtype=torch.float64, device="cuda")
+    cam = render.camera(size, size, **scenes.ROOM_CAMERA)
+    acc = torch.zeros((size, size, 3), dtype=torch.float64, device="cuda")
     done = 0
     start = args.resume or args.partial
     if start and os.path.exists(start):
@@ -67,9 +70,9 @@ def main() -> None:
     if done < args.spp:
         print(f"stopped at {done} of {args.spp} spp", flush=True)
         return
-    img = (acc / args.spp).cpu().numpy().astype(np.float32)
-    np.savez_compressed(args.out, image=img, spp=np.int64(args.spp), seed=np.int64(1000))
-    print(f"wrote {args.out}: mean {img.mean():.5f}", flush=True)
+    img = (acc / args.spp).cpu().numpy().astype(np.float16 if args.half else np.float32)
+    np.savez_compressed(out, image=img, spp=np.int64(args.spp), seed=np.int64(1000))
+    print(f"wrote {out}: mean {img.astype(np.float64).mean():.5f}", flush=True)
 
 
 if __name__ == "__main__":

@@ -125,7 +125,9 @@ def test_medium_mesh_closest_and_any(gpu, medium):
 @pytest.mark.parametrize("kernel,block,per_cu,lanes,batch,lds", [
     (0, 64, 1, 1, 1, 32), (0, 128, 3, 8, 7, 4), (0, 256, 8, 16, 256, 16), (0, 512, 2, 64, 64, 1),
     (0, 1024, 1, 32, 1000, 8), (1, 64, 1, 0, 0, 32), (1, 256, 8, 0, 0, 2), (1, 1024, 1, 0, 0, 16),
-    (2, 64, 2, 24, 64, 3), (2, 256, 8, 24, 64, 16), (2, 512, 4, 1, 5, 32)])
+    (2, 64, 2, 24, 64, 3), (2, 256, 8, 24, 64, 16), (2, 512, 4, 1, 5, 32),
+    (3, 64, 2, 24, 64, 3), (3, 256, 8, 24, 64, 16), (3, 256, 6, 24, 64, 10), (3, 512, 4, 1, 5, 32),
+    (3, 1024, 1, 32, 1000, 32)])
 def test_launch_shapes_do_not_change_results(gpu, medium, kernel, block, per_cu, lanes, batch, lds):
     """Every kernel variant and launch shape gives the same bits; small LDS
     stacks force the global spill path on this depth-20 tree."""
