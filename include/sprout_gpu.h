@@ -294,6 +294,40 @@ int32_t mi_scene_release(uint32_t scene);
 int32_t mi_render(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
                   double* accum, void* stream, mi_render_stats* stats);
 
+/* ---- filtered sensor ---- */
+
+enum mi_filter_kind {
+    MI_FILTER_BOX      = 0, /* the unfiltered sensor: a sample counts for its own pixel */
+    MI_FILTER_GAUSSIAN = 1, /* filter::Gaussian(radius, alpha) */
+    MI_FILTER_BLACKMAN = 2, /* filter::Blackman(radius) */
+    MI_FILTER_MITCHELL = 3  /* filter::Mitchell(radius, b, c) */
+};
+
+/* sprout's sensor (take "sensor": {"filter": ..., "clamp": ...}, take_loader.cpp:
+ * 331-436): filter radius in (0, 2] (Filtered_1p0 up to 1, Filtered_2p0 up to 2;
+ * take defaults radius 2, alpha 1.8, b = c = 1/3).  clamp_max[0] < 0: no clamp;
+ * clamp_max[1] < 0: clamp::Luminance(clamp_max[0]); else clamp::Clamp(clamp_max). */
+typedef struct mi_sensor {
+    uint32_t filter; /* mi_filter_kind */
+    float    radius;
+    float    alpha;
+    float    b;
+    float    c;
+    float    clamp_max[3];
+} mi_sensor;
+
+/* As mi_render through sprout's filtered sensor: the camera samples pixels of
+ * the image grown by the filter's reach on every side (the worker's extended
+ * tiles, rendering_worker.cpp:93-104; the generator of grid pixel (gx, gy) is
+ * started at sample * grid pixels + gy * grid width + gx) and every sample
+ * adds weight * clamped colour and weight to the image pixels within reach
+ * (Filtered_1p0 / Filtered_2p0::add_sample, filtered.inl:121-300).  ADDS to
+ * accum, a DEVICE buffer of width * height * 4 doubles (weighted rgb,
+ * weight); the image is rgb / weight (Opaque::resolve, opaque.cpp:61-69). */
+int32_t mi_render_filtered(uint32_t scene, mi_camera const* camera,
+                           mi_render_settings const* settings, mi_sensor const* sensor,
+                           double* accum, void* stream, mi_render_stats* stats);
+
 /* ---- instanced meshes (sprout's props) ---- */
 
 /* A top-level tree over num_instances instances of mesh trees: instance i

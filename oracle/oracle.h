@@ -263,6 +263,31 @@ typedef struct {
 
 enum { ORACLE_PT_MIS = 0, ORACLE_PT_BSDF = 1, ORACLE_PT_NEE = 2 };
 
+/* ---- sensor filters (sensor.c) ---- */
+
+enum {
+    ORACLE_FILTER_BOX      = 0,
+    ORACLE_FILTER_GAUSSIAN = 1,
+    ORACLE_FILTER_BLACKMAN = 2,
+    ORACLE_FILTER_MITCHELL = 3
+};
+
+typedef struct {
+    int32_t kind;
+    float   radius;
+    float   alpha;
+    float   b, c;
+    float   table[30]; /* Gaussian over d^2 in [0, r^2], Blackman over |d| in [0, r] */
+    float   range_end;
+    float   inverse_interval;
+    float   radius_inv;
+} oracle_filter;
+
+void  oracle_filter_init(oracle_filter* f, int32_t kind, float radius, float alpha, float b,
+                         float c);
+float oracle_filter_eval(oracle_filter const* f, float d);
+void  oracle_filter_eval_many(oracle_filter const* f, float const* d, uint32_t n, float* out);
+
 /* An instanced scene for oracle_render: tlas places the meshes and, as its
  * instance `world` (identity matrices), the tree over the render's triangles,
  * whose materials and lights apply to it; every other instance shows its
@@ -285,9 +310,13 @@ typedef struct {
     int32_t  split_lights; /* split the light tree below depth 3 (sprout's Adaptive) */
     oracle_medium const* medium; /* NULL: none */
     oracle_pt_instances const* instances; /* NULL: the triangles only */
+    uint32_t border;  /* pixels rendered beyond each image edge (a filter's radius) */
+    float*   samples; /* non-NULL: per sample and grid pixel r, g, b, jitter x, y of the
+                         (W + 2 border) x (H + 2 border) grid, sample-major; accum unused */
 } oracle_pt_settings;
 
-/* Adds each pixel's radiance sum over the samples into accum (W * H * 3).
+/* Adds each pixel's radiance sum over the samples into accum (W * H * 3), or
+ * with set->samples writes every sample of the border-extended grid there.
  * tree: the light tree over the scene's emissive triangles. */
 int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t const* indices,
                       uint32_t num_triangles, uint32_t const* tri_material,

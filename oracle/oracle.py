@@ -56,7 +56,8 @@ class _OraclePtSettings(ctypes.Structure):
                 ("min_bounces", ctypes.c_uint32), ("max_bounces", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64), ("mode", ctypes.c_int32),
                 ("num_threads", ctypes.c_uint32), ("split_lights", ctypes.c_int32),
-                ("medium", ctypes.c_void_p), ("instances", ctypes.c_void_p)]
+                ("medium", ctypes.c_void_p), ("instances", ctypes.c_void_p),
+                ("border", ctypes.c_uint32), ("samples", ctypes.c_void_p)]
 
 
 class _OracleGridtree(ctypes.Structure):
@@ -137,6 +138,12 @@ def lib():
         _lib.oracle_schlick_f0.restype = ctypes.c_float
         _lib.oracle_schlick_f0.argtypes = [ctypes.c_float, ctypes.c_float]
         _lib.oracle_orthonormal_basis.argtypes = [ctypes.c_void_p] * 3
+        _lib.oracle_filter_init.restype = None
+        _lib.oracle_filter_init.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float,
+                                            ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        _lib.oracle_filter_eval_many.restype = None
+        _lib.oracle_filter_eval_many.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                 ctypes.c_void_p]
         _lib.oracle_render.restype = ctypes.c_int32
         _lib.oracle_render.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                        ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
@@ -421,6 +428,27 @@ def ref_prop_lib(arch: str = "native"):
     l.sref_build_props.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     return l
+
+
+def ref_filter_lib(arch: str = "native"):
+    """The reference's sensor filters (oracle/_ref), or None."""
+    path = ref_lib_path(arch)
+    if not os.path.exists(path):
+        return None
+    l = ctypes.CDLL(path)
+    if not hasattr(l, "sref_filter_eval"):
+        return None
+    l.sref_filter_eval.restype = None
+    l.sref_filter_eval.argtypes = [ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+    return l
+
+
+def ref_filter_eval(l, kind, radius, alpha, b, c, d):
+    d = np.ascontiguousarray(d, np.float32).reshape(-1)
+    out = np.zeros_like(d)
+    l.sref_filter_eval(kind, radius, alpha, b, c, _ptr(d), len(d), _ptr(out))
+    return out
 
 
 def ref_build_props(l, boxes, threads=4):
@@ -722,11 +750,13 @@ def similarity_scale(g, depth) -> float:
 def render(positions, indices, triangle_materials, materials, width, height, spp, camera,
            tree: LightTree, first_sample=0, min_bounces=4, max_bounces=8, seed=1, mode=PT_MIS,
            threads=None, split_lights=True, medium: Medium | None = None,
-           instances: PtInstances | None = None):
+           instances: PtInstances | None = None, border: int = 0, samples: bool = False):
     """Radiance sums per pixel, (height, width, 3) float64, of the CPU path
     tracer (pt.c).  camera: dict with eye, target, up, fov_deg; tree: the
     scene's light tree; medium: a participating medium or None; instances:
-    mesh instances beside the triangles, or None."""
+    mesh instances beside the triangles, or None.  samples=True returns every
+    sample instead: (spp, height + 2 border, width + 2 border, 5) float32 of
+    r, g, b and the pixel jitter x, y, over the grid extended by border."""
     pos = np.ascontiguousarray(positions, np.float32)
     idx = np.ascontiguousarray(indices, np.uint32)
     tmat = np.ascontiguousarray(triangle_materials, np.uint32)
@@ -735,15 +765,93 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
                         (ctypes.c_float * 3)(*camera["target"]),
                         (ctypes.c_float * 3)(*camera["up"]), camera["fov_deg"])
     n = threads or min(os.cpu_count() or 1, 16)
+    out = (np.zeros((spp, height + 2 * border, width + 2 * border, 5), np.float32)
+           if samples else None)
     st = _OraclePtSettings(spp, first_sample, min_bounces, max_bounces, seed, mode, n,
                            1 if split_lights else 0,
                            ctypes.cast(ctypes.pointer(medium.c), ctypes.c_void_p) if medium else None,
-                           ctypes.cast(ctypes.pointer(instances.c), ctypes.c_void_p) if instances else None)
-    accum = np.zeros((height, width, 3), np.float64)
+                           ctypes.cast(ctypes.pointer(instances.c), ctypes.c_void_p) if instances else None,
+                           border, _ptr(out) if samples else None)
+    accum = None if samples else np.zeros((height, width, 3), np.float64)
     status = lib().oracle_render(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(tmat),
                                  _ptr(mats), len(mats), ctypes.byref(tree.c), ctypes.byref(cam),
-                                 ctypes.byref(st), _ptr(accum))
+                                 ctypes.byref(st), _ptr(accum) if accum is not None else None)
     if status:
         raise RuntimeError(f"oracle_render: status {status}")
-    return accum
+    return out if samples else accum
+
+
+FILTER_BOX, FILTER_GAUSSIAN, FILTER_BLACKMAN, FILTER_MITCHELL = 0, 1, 2, 3
+
+
+class _OracleFilter(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("radius", ctypes.c_float), ("alpha", ctypes.c_float),
+                ("b", ctypes.c_float), ("c", ctypes.c_float), ("table", ctypes.c_float * 30),
+                ("range_end", ctypes.c_float), ("inverse_interval", ctypes.c_float),
+                ("radius_inv", ctypes.c_float)]
+
+
+class Filter:
+    """A sensor filter of sensor.c: kind FILTER_*, with the reference's
+    defaults radius 2, alpha 1.8 (Gaussian), b = c = 1/3 (Mitchell)."""
+
+    def __init__(self, kind, radius=2.0, alpha=1.8, b=1.0 / 3.0, c=1.0 / 3.0):
+        self.kind = kind
+        self.radius = float(radius)
+        self.c = _OracleFilter()
+        lib().oracle_filter_init(ctypes.byref(self.c), kind, radius, alpha, b, c)
+
+    def __call__(self, d) -> np.ndarray:
+        d = np.ascontiguousarray(d, np.float32).reshape(-1)
+        out = np.zeros_like(d)
+        lib().oracle_filter_eval_many(ctypes.byref(self.c), _ptr(d), len(d), _ptr(out))
+        return out
+
+    def border(self) -> int:
+        """pixels the splat reaches: 1 up to radius 1 (Filtered_1p0), else 2
+        (Filtered_2p0); 0 for the box (the unfiltered sensor)"""
+        if FILTER_BOX == self.kind:
+            return 0
+        return 1 if self.radius <= 1.0 else 2
+
+
+def clamp_colors(rgb, clamp_max):
+    """clamp::Identity / Clamp / Luminance (clamp.inl:9-35) of float32 colours:
+    clamp_max[0] < 0 none; clamp_max[1] < 0 luminance with max clamp_max[0]"""
+    rgb = np.asarray(rgb, np.float32)
+    if clamp_max is None or clamp_max[0] < 0:
+        return rgb
+    if clamp_max[1] < 0:
+        m = np.float32(clamp_max[0])
+        mc = rgb.max(axis=-1, keepdims=True)
+        scale = np.where(mc > m, m / np.where(mc > 0, mc, 1).astype(np.float32), np.float32(1))
+        return np.where(mc > m, scale.astype(np.float32) * rgb, rgb).astype(np.float32)
+    return np.minimum(rgb, np.asarray(clamp_max, np.float32)).astype(np.float32)
+
+
+def filtered_sums(samples, width, height, filt: Filter, clamp_max=None):
+    """Filtered_1p0 / Filtered_2p0::add_sample (filtered.inl:121-300) over the
+    samples of render(samples=True): every sample of grid pixel (x, y) (image
+    pixel x - border, y - border) adds weight * clamped colour and weight to
+    image pixel (x + k, y + l) for k, l in [-border, border], with weight
+    f(ox - k) f(oy - l), o = jitter - 0.5; pixels outside the image take
+    nothing.  Returns (height, width, 4) float64: weighted rgb, weight."""
+    bd = filt.border()
+    spp, hg, wg, _ = samples.shape
+    assert hg == height + 2 * bd and wg == width + 2 * bd
+    col = clamp_colors(samples[..., 0:3], clamp_max)
+    ox = samples[..., 3] - np.float32(0.5)
+    oy = samples[..., 4] - np.float32(0.5)
+    acc = np.zeros((height, width, 4), np.float64)
+    for l in range(-bd, bd + 1):
+        wy = filt(oy - np.float32(l)).reshape(oy.shape) if bd else np.ones_like(oy)
+        for k in range(-bd, bd + 1):
+            wx = filt(ox - np.float32(k)).reshape(ox.shape) if bd else np.ones_like(ox)
+            w = (wx * wy).astype(np.float32)
+            # grid pixel (gx, gy) lands on image pixel (gx - bd + k, gy - bd + l)
+            ys = slice(bd - l, bd - l + height)
+            xs = slice(bd - k, bd - k + width)
+            acc[..., 0:3] += (w[:, ys, xs, None] * col[:, ys, xs, :]).astype(np.float64).sum(0)
+            acc[..., 3] += w[:, ys, xs].astype(np.float64).sum(0)
+    return acc
 

@@ -320,17 +320,29 @@ typedef struct {
  * order their lights were picked.  The generator order per vertex is then:
  * tracking of the ray that reached it, [roulette 1], [light select 1, 2 per
  * picked light], direction 3, transmittance of its shadow rays. */
-static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, double rgb[3]) {
+/* One camera sample of grid pixel (gx, gy) of the (W + 2 border) x
+ * (H + 2 border) grid, image pixel (gx - border, gy - border); its generator
+ * is started at sample * (grid pixels) + gy * (grid width) + gx, as the
+ * reference's worker numbers the pixels of its filter-extended tiles
+ * (rendering_worker.cpp:93-104).  rgb += radiance; jitter = the pixel offsets. */
+static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample, double rgb[3],
+                    float jitter[2]) {
     oracle_pt_settings const* set    = s->set;
     oracle_medium const*      medium = set->medium;
     uint32_t const W = s->cam->width;
     uint32_t const H = s->cam->height;
+    uint32_t const Wg = W + 2 * set->border;
+    uint32_t const Hg = H + 2 * set->border;
     int32_t const  mode = set->mode;
+    int32_t const  x    = (int32_t)gx - (int32_t)set->border;
+    int32_t const  y    = (int32_t)gy - (int32_t)set->border;
 
     pcg rng;
-    pcg_start(&rng, set->seed, (uint64_t)sample * ((uint64_t)W * H) + (uint64_t)y * W + x);
+    pcg_start(&rng, set->seed, (uint64_t)sample * ((uint64_t)Wg * Hg) + (uint64_t)gy * Wg + gx);
     float const jx = pcg_float(&rng);
     float const jy = pcg_float(&rng);
+    jitter[0]      = jx;
+    jitter[1]      = jy;
 
     float const sx = ((2.f * ((float)x + jx)) / (float)W - 1.f) * s->tan_half * s->aspect;
     float const sy = (1.f - (2.f * ((float)y + jy)) / (float)H) * s->tan_half;
@@ -580,25 +592,40 @@ static int32_t path(scene_t const* s, uint32_t x, uint32_t y, uint32_t sample, d
 
 static void* render_rows(void* arg) {
     scene_t* s = (scene_t*)arg;
-    uint32_t const W = s->cam->width;
-    uint32_t const H = s->cam->height;
+    uint32_t const Wg = s->cam->width + 2 * s->set->border;
+    uint32_t const Hg = s->cam->height + 2 * s->set->border;
+    float* const   out = s->set->samples;
     for (;;) {
         pthread_mutex_lock(&s->lock);
         uint32_t const y = s->next_row++;
         pthread_mutex_unlock(&s->lock);
-        if (y >= H) {
+        if (y >= Hg) {
             break;
         }
-        for (uint32_t x = 0; x < W; ++x) {
+        for (uint32_t x = 0; x < Wg; ++x) {
             double rgb[3] = {0.0, 0.0, 0.0};
             for (uint32_t k = 0; k < s->set->samples_per_pixel; ++k) {
-                if (path(s, x, y, s->set->first_sample + k, rgb)) {
+                float jitter[2];
+                if (out) {
+                    double one[3] = {0.0, 0.0, 0.0};
+                    if (path(s, x, y, s->set->first_sample + k, one, jitter)) {
+                        s->status = ORACLE_ERR_STACK;
+                    }
+                    float* o = out + 5 * (((size_t)k * Hg + y) * Wg + x);
+                    o[0]     = (float)one[0];
+                    o[1]     = (float)one[1];
+                    o[2]     = (float)one[2];
+                    o[3]     = jitter[0];
+                    o[4]     = jitter[1];
+                } else if (path(s, x, y, s->set->first_sample + k, rgb, jitter)) {
                     s->status = ORACLE_ERR_STACK;
                 }
             }
-            double* a = s->accum + 3 * ((size_t)y * W + x);
-            for (int c = 0; c < 3; ++c) {
-                a[c] += rgb[c];
+            if (!out) {
+                double* a = s->accum + 3 * ((size_t)y * Wg + x);
+                for (int c = 0; c < 3; ++c) {
+                    a[c] += rgb[c];
+                }
             }
         }
     }
@@ -610,7 +637,8 @@ int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t co
                       oracle_material const* materials, uint32_t num_materials,
                       oracle_light_tree const* tree, oracle_camera const* cam,
                       oracle_pt_settings const* set, double* accum) {
-    if (!positions || !indices || !tri_material || !materials || !tree || !cam || !set || !accum ||
+    if (!positions || !indices || !tri_material || !materials || !tree || !cam || !set ||
+        (!accum && !set->samples) || (set->border && !set->samples) || set->border > 8 ||
         0 == cam->width || 0 == cam->height || 0 == num_triangles) {
         return ORACLE_ERR_ARGS;
     }

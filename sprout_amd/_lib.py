@@ -70,6 +70,17 @@ class MiMesh(ctypes.Structure):
                 ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint32)]
 
 
+class MiSensor(ctypes.Structure):
+    _fields_ = [("filter", ctypes.c_uint32), ("radius", ctypes.c_float), ("alpha", ctypes.c_float),
+                ("b", ctypes.c_float), ("c", ctypes.c_float), ("clamp_max", ctypes.c_float * 3)]
+
+
+MI_FILTER_BOX = 0
+MI_FILTER_GAUSSIAN = 1
+MI_FILTER_BLACKMAN = 2
+MI_FILTER_MITCHELL = 3
+
+
 class MiCamera(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("eye", ctypes.c_float * 3), ("target", ctypes.c_float * 3),
@@ -134,6 +145,8 @@ SIGNATURES = {
                                          _vp, _u32, _vp, _u32, _vp, _vp, _u32,
                                          ctypes.POINTER(MiBvhSettings), _pu32]),
     "mi_scene_release": (_i32, [_u32]),
+    "mi_render_filtered": (_i32, [_u32, ctypes.POINTER(MiCamera), ctypes.POINTER(MiRenderSettings),
+                                  ctypes.POINTER(MiSensor), _vp, _vp, ctypes.POINTER(MiRenderStats)]),
     "mi_render": (_i32, [_u32, ctypes.POINTER(MiCamera), ctypes.POINTER(MiRenderSettings), _vp,
                          _vp, ctypes.POINTER(MiRenderStats)]),
     "mi_render_guided": (_i32, [_u32, ctypes.POINTER(MiCamera), ctypes.POINTER(MiRenderSettings),

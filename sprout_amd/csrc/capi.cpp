@@ -479,7 +479,7 @@ namespace {
 
 int32_t render_call(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
                     mi_guide_settings const* gs, double* accum, void* stream,
-                    mi_render_stats* stats) {
+                    mi_render_stats* stats, Sensor_filter const* sensor = nullptr) {
     if (!context) {
         return fail(MI_ERR_NO_CONTEXT, "mi_render: call mi_init first");
     }
@@ -516,10 +516,24 @@ int32_t render_call(uint32_t scene, mi_camera const* camera, mi_render_settings 
     rt.claim_batch   = context->trace.claim_batch;
     rt.lds_entries   = context->trace.lds_entries;
     return render(*sc, rt, table, context->render_state, *camera, *settings, net, gs, accum,
-                  as_stream(stream), stats);
+                  as_stream(stream), stats, sensor);
 }
 
 }  // namespace
+
+int32_t mi_render_filtered(uint32_t scene, mi_camera const* camera,
+                           mi_render_settings const* settings, mi_sensor const* sensor,
+                           double* accum, void* stream, mi_render_stats* stats) {
+    clear_error();
+    if (!sensor) {
+        return fail(MI_ERR_ARGS, "mi_render_filtered: null sensor");
+    }
+    Sensor_filter f;
+    if (int32_t const st = make_sensor_filter(*sensor, f); st) {
+        return st;
+    }
+    return render_call(scene, camera, settings, nullptr, accum, stream, stats, &f);
+}
 
 int32_t mi_render(uint32_t scene, mi_camera const* camera, mi_render_settings const* settings,
                   double* accum, void* stream, mi_render_stats* stats) {

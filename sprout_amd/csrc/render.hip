@@ -125,6 +125,7 @@ struct Path {
     Rng      rng;
     uint32_t record;     // training sample of the previous vertex, or Null
     float    prev_n[3];  // geometric normal of the previous vertex
+    float    jitter[2];  // the camera sample's position in its pixel
 };
 
 struct Shadow {
@@ -143,6 +144,10 @@ struct Wave {
     uint32_t               split_lights;
     dev::Medium_view       medium;
     // instanced scenes
+    // camera grid: the image grown by border pixels on every side (filtered sensor)
+    uint32_t               grid_w;
+    int32_t                border;
+    Sensor_filter          filter;
     uint32_t               instanced;
     uint32_t               emitter_instance;  // the instance of the world triangles
     mi_instance_hit const* ihits;
@@ -404,16 +409,18 @@ __global__ void __launch_bounds__(256) raygen(Wave w) {
     if (p >= w.num_paths) {
         return;
     }
-    uint32_t const pixel  = p % w.num_pixels;
+    uint32_t const pixel  = p % w.num_pixels;  // of the camera grid
     uint32_t const plane  = p / w.num_pixels;
     uint32_t const sample = w.counters[C_base] + plane;
-    uint32_t const x      = pixel % w.width;
-    uint32_t const y      = pixel / w.width;
+    int32_t const  x      = int32_t(pixel % w.grid_w) - w.border;
+    int32_t const  y      = int32_t(pixel / w.grid_w) - w.border;
 
     Path path;
     path.rng.start(w.seed, uint64_t(sample) * w.num_pixels + pixel);
     float const jx = path.rng.uniform();
     float const jy = path.rng.uniform();
+    path.jitter[0] = jx;
+    path.jitter[1] = jy;
 
     float const sx = ((2.f * (float(x) + jx)) / float(w.width) - 1.f) * w.tan_half * w.aspect;
     float const sy = (1.f - (2.f * (float(y) + jy)) / float(w.height)) * w.tan_half;
@@ -988,6 +995,83 @@ __global__ void __launch_bounds__(256) resolve(Wave w, uint32_t planes, double* 
     }
 }
 
+// ---- filtered sensor ----
+
+__device__ float filter_eval(Sensor_filter const& f, float d) {
+    if (MI_FILTER_MITCHELL == f.kind) {
+        // Mitchell::mitchell (sensor_mitchell.cpp), unclipped as there
+        float const x  = fabsf(2.f * (d * f.radius_inv));
+        float const xx = x * x;
+        float const b = f.b, c = f.c;
+        if (x > 1.f) {
+            return ((((-b - 6.f * c) * xx) * x + (6.f * b + 30.f * c) * xx) + (-12.f * b - 48.f * c) * x +
+                    (8.f * b + 24.f * c)) *
+                   (1.f / 6.f);
+        }
+        return ((((12.f - 9.f * b) - 6.f * c) * xx) * x + ((-18.f + 12.f * b) + 6.f * c) * xx +
+                (6.f - 2.f * b)) *
+               (1.f / 6.f);
+    }
+    // Interpolated_function_1D_N<float, 30>::operator() (interpolated_function_1d.inl:168-178)
+    float          x      = MI_FILTER_GAUSSIAN == f.kind ? d * d : fabsf(d);
+    x                     = fminf(x, f.range_end);
+    float const    o      = x * f.inverse_interval;
+    uint32_t const offset = uint32_t(o);
+    float const    t      = o - float(offset);
+    uint32_t const next   = offset + 1 < 29u ? offset + 1 : 29u;
+    return (1.f - t) * f.table[offset] + t * f.table[next];
+}
+
+// Gather form of Filtered_*::add_sample: image pixel (x, y) takes from every
+// grid pixel within reach and every sample of the wave, in a fixed order,
+// weight f(ox - k) f(oy - l) * clamped colour and the weight, where (k, l) is
+// its offset from the sample's pixel and o = jitter - 0.5.
+__global__ void __launch_bounds__(256) resolve_filtered(Wave w, uint32_t planes, double* accum) {
+    uint32_t const pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= uint32_t(w.width) * w.height) {
+        return;
+    }
+    Sensor_filter const& f  = w.filter;
+    int32_t const        bd = w.border;
+    int32_t const        x  = int32_t(pixel % w.width);
+    int32_t const        y  = int32_t(pixel / w.width);
+    double               s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k = 0; k < planes; ++k) {
+        for (int32_t l = -bd; l <= bd; ++l) {
+            // the sample pixel lies l rows above: grid row y - l + bd
+            uint32_t const gy = uint32_t(y - l + bd);
+            for (int32_t kx = -bd; kx <= bd; ++kx) {
+                uint32_t const gx = uint32_t(x - kx + bd);
+                Path const&    p  = w.paths[size_t(k) * w.num_pixels + size_t(gy) * w.grid_w + gx];
+                float const    wx = MI_FILTER_BOX == f.kind ? 1.f : filter_eval(f, (p.jitter[0] - 0.5f) - float(kx));
+                float const    wy = MI_FILTER_BOX == f.kind ? 1.f : filter_eval(f, (p.jitter[1] - 0.5f) - float(l));
+                float const    wt = wx * wy;
+                float          c[3] = {p.radiance[0], p.radiance[1], p.radiance[2]};
+                if (1 == f.clamp_mode) {
+                    for (int i = 0; i < 3; ++i) {
+                        c[i] = fminf(c[i], f.clamp_max[i]);
+                    }
+                } else if (2 == f.clamp_mode) {
+                    float const mc = fmaxf(fmaxf(c[0], c[1]), c[2]);
+                    if (mc > f.clamp_max[0]) {
+                        float const r = f.clamp_max[0] / mc;
+                        for (int i = 0; i < 3; ++i) {
+                            c[i] = r * c[i];
+                        }
+                    }
+                }
+                for (int i = 0; i < 3; ++i) {
+                    s[i] += double(wt * c[i]);
+                }
+                s[3] += double(wt);
+            }
+        }
+    }
+    for (int i = 0; i < 4; ++i) {
+        accum[4 * size_t(pixel) + i] += s[i];
+    }
+}
+
 // ---- energy table: sprout's procedure (ggx_integrate.cpp:17-45, 202-240) ----
 
 __device__ float radical_inverse_vdc(uint32_t bits) {
@@ -1554,6 +1638,82 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
     return MI_OK;
 }
 
+namespace {
+
+// Interpolated_function_1D_N<float, 30>::operator() on the host
+float table_eval(Sensor_filter const& f, float x) {
+    x                     = std::min(x, f.range_end);
+    float const    o      = x * f.inverse_interval;
+    uint32_t const offset = uint32_t(o);
+    float const    t      = o - float(offset);
+    uint32_t const next   = std::min(offset + 1, 29u);
+    return (1.f - t) * f.table[offset] + t * f.table[next];
+}
+
+}  // namespace
+
+// The tables as sprout's constructors fill and normalize them:
+// Interpolated_function_1D_N (interpolated_function_1d.inl:124-137) over
+// [0, r^2] (Gaussian, sensor_gaussian.cpp) or [0, r] (Blackman,
+// sensor_blackman.cpp), scaled by 1 / the 64-sample midpoint integral of the
+// unscaled filter over [-r, r].
+int32_t make_sensor_filter(mi_sensor const& in, Sensor_filter& f) {
+    std::memset(&f, 0, sizeof(f));
+    if (in.filter > MI_FILTER_MITCHELL) {
+        return fail(MI_ERR_ARGS, "sensor: filter %u", in.filter);
+    }
+    if (MI_FILTER_BOX != in.filter && !(in.radius > 0.f && in.radius <= 2.f)) {
+        return fail(MI_ERR_ARGS, "sensor: radius %g outside (0, 2]", double(in.radius));
+    }
+    if (MI_FILTER_GAUSSIAN == in.filter && !(in.alpha > 0.f && std::isfinite(in.alpha))) {
+        return fail(MI_ERR_ARGS, "sensor: Gaussian alpha %g", double(in.alpha));
+    }
+    if (MI_FILTER_MITCHELL == in.filter && !(std::isfinite(in.b) && std::isfinite(in.c))) {
+        return fail(MI_ERR_ARGS, "sensor: Mitchell b %g, c %g", double(in.b), double(in.c));
+    }
+    f.kind       = in.filter;
+    f.border     = MI_FILTER_BOX == in.filter ? 0 : (in.radius <= 1.f ? 1 : 2);
+    f.radius_inv = MI_FILTER_BOX == in.filter ? 0.f : 1.f / in.radius;
+    f.b          = in.b;
+    f.c          = in.c;
+    if (in.clamp_max[0] >= 0.f) {
+        f.clamp_mode = in.clamp_max[1] < 0.f ? 2u : 1u;
+        for (int i = 0; i < 3; ++i) {
+            f.clamp_max[i] = in.clamp_max[i];
+        }
+    }
+    if (MI_FILTER_GAUSSIAN == in.filter || MI_FILTER_BLACKMAN == in.filter) {
+        bool const  gauss    = MI_FILTER_GAUSSIAN == in.filter;
+        float const r        = in.radius;
+        float const end      = gauss ? r * r : r;
+        float const interval = end / 29.f;
+        f.range_end          = end;
+        f.inverse_interval   = 1.f / interval;
+        float s              = 0.f;
+        for (int i = 0; i < 30; ++i, s += interval) {
+            if (gauss) {
+                float const e = std::exp(-in.alpha * (r * r));
+                f.table[i]    = std::max(0.f, std::exp(-in.alpha * s) - e);
+            } else {
+                float const a0 = 0.35875f, a1 = 0.48829f, a2 = 0.14128f, a3 = 0.01168f;
+                float const x  = (3.14159265358979323846f * (s + r)) / r;
+                f.table[i]     = ((a0 - a1 * std::cos(x)) + a2 * std::cos(2.f * x)) - a3 * std::cos(3.f * x);
+            }
+        }
+        float const step = r / 64.f;
+        float       m    = 0.5f * step;
+        float       sum  = 0.f;
+        for (int i = 0; i < 64; ++i, m += step) {
+            sum += table_eval(f, gauss ? m * m : m) * step;
+        }
+        float const scale = 1.f / (sum + sum);
+        for (float& v : f.table) {
+            v *= scale;
+        }
+    }
+    return MI_OK;
+}
+
 int32_t create_scene_instanced(mi_mesh const* meshes, uint32_t num_meshes,
                                uint32_t const* instance_mesh, float const* world_to_object,
                                float const* object_to_world, uint32_t const* instance_materials,
@@ -1823,13 +1983,16 @@ int32_t RenderState::reserve_guiding(uint32_t n, uint32_t records) {
 int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_table,
                RenderState& state, mi_camera const& cam, mi_render_settings const& set,
                GuideNet* net, mi_guide_settings const* gs, double* accum, hipStream_t stream,
-               mi_render_stats* stats) {
+               mi_render_stats* stats, Sensor_filter const* sensor) {
     bool const guided = nullptr != net;
     if (guided && (!gs || !(gs->fraction >= 0.f) || !(gs->fraction <= 1.f) ||
                    (gs->train && (0 == gs->train_steps || !(gs->learning_rate > 0.f))))) {
         return fail(MI_ERR_ARGS, "mi_render_guided: bad guide settings");
     }
-    uint64_t const num_pixels64 = uint64_t(cam.width) * cam.height;
+    // the camera grid: the image grown by the filter's reach on every side
+    uint32_t const border       = sensor ? uint32_t(sensor->border) : 0u;
+    uint32_t const grid_w       = cam.width + 2 * border;
+    uint64_t const num_pixels64 = uint64_t(grid_w) * (cam.height + 2 * border);
     uint32_t const spw          = set.samples_per_wave;
     uint64_t const paths64      = num_pixels64 * spw;
     if (0 == num_pixels64 || 0 == spw || 0 == set.samples_per_pixel || paths64 > (1ull << 27) ||
@@ -1925,6 +2088,11 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.aspect      = float(double(cam.width) / double(cam.height));
     w.width       = cam.width;
     w.height      = cam.height;
+    w.grid_w      = grid_w;
+    w.border      = int32_t(border);
+    if (sensor) {
+        w.filter = *sensor;
+    }
     w.min_bounces = set.min_bounces;
     w.max_bounces = set.max_bounces;
     w.seed        = set.seed;
@@ -2027,7 +2195,12 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
             hipLaunchKernelGGL(shadow_resolve, dim3(grid_path), dim3(256), 0, s, w, q, d);
             q ^= 1;
         }
-        hipLaunchKernelGGL(resolve, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, w, spw, accum);
+        if (sensor) {
+            hipLaunchKernelGGL(resolve_filtered, dim3(blocks(uint64_t(cam.width) * cam.height, 256)),
+                               dim3(256), 0, s, w, spw, accum);
+        } else {
+            hipLaunchKernelGGL(resolve, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, w, spw, accum);
+        }
         hipLaunchKernelGGL(advance_base, dim3(1), dim3(1), 0, s, state.counters, spw);
         return MI_OK;
     };
@@ -2084,6 +2257,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
             key.insert(key.end(), b, b + sizeof(v));
         };
         put(accum);
+        put(sensor != nullptr);  // which resolve ends the wave
         put(grid_path);
         put(num_shadow);
         put(spw);

@@ -26,6 +26,23 @@ struct Render_material {
     uint32_t pad[2];
 };
 
+// A sensor filter as the kernels evaluate it: sprout's 30-sample tables
+// (Gaussian over d^2, Blackman over |d|) or Mitchell's polynomial, and the clamp.
+struct Sensor_filter {
+    uint32_t kind;              // mi_filter_kind
+    int32_t  border;            // pixels a sample reaches: 0 (box), 1 or 2
+    float    table[30];
+    float    range_end;
+    float    inverse_interval;
+    float    radius_inv;
+    float    b, c;
+    uint32_t clamp_mode;        // 0 none, 1 per channel, 2 luminance
+    float    clamp_max[3];
+};
+
+// Builds the tables of a sensor as sprout's filter constructors do.
+int32_t make_sensor_filter(mi_sensor const& sensor, Sensor_filter& out);
+
 // Vertex data of an instanced mesh on the device.
 struct Mesh_geometry {
     float const*    positions;  // 3 floats per vertex
@@ -164,6 +181,6 @@ int32_t download_light_tree(RenderScene const& scene, mi_light_node* nodes, uint
 int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_table,
                RenderState& state, mi_camera const& camera, mi_render_settings const& settings,
                GuideNet* net, mi_guide_settings const* gs, double* accum, hipStream_t stream,
-               mi_render_stats* stats);
+               mi_render_stats* stats, Sensor_filter const* sensor = nullptr);
 
 }  // namespace mi

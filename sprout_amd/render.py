@@ -142,6 +142,36 @@ def substitute_sample(params, frames, wo, xi) -> np.ndarray:
     return _batch("mi_substitute_sample", 8, params, frames, wo, xi)
 
 
+def sensor(kind: int = _lib.MI_FILTER_GAUSSIAN, radius: float = 2.0, alpha: float = 1.8,
+           b: float = 1.0 / 3.0, c: float = 1.0 / 3.0, clamp_max=(-1.0, 0.0, 0.0)) -> _lib.MiSensor:
+    """sprout's sensor: a filter (take defaults: Gaussian, radius 2, alpha 1.8;
+    Mitchell b = c = 1/3) and a clamp (clamp_max[0] < 0: none; clamp_max[1]
+    < 0: luminance to clamp_max[0]; else per channel)."""
+    return _lib.MiSensor(kind, radius, alpha, b, c, (ctypes.c_float * 3)(*clamp_max))
+
+
+def render_filtered(scene: Scene, cam: _lib.MiCamera, spp: int, sens: _lib.MiSensor,
+                    samples_per_wave: int = 1, min_bounces: int = 4, max_bounces: int = 8,
+                    seed: int = 1, first_sample: int = 0, graph: bool = True,
+                    accum: torch.Tensor | None = None, stats: bool = False):
+    """mi_render_filtered: adds the filtered samples into accum ((height,
+    width, 4) float64 on the device: weighted rgb and weight; made zero if
+    None) and returns (accum, stats or None); the image is rgb / weight."""
+    if accum is None:
+        accum = torch.zeros((cam.height, cam.width, 4), dtype=torch.float64, device="cuda")
+    if accum.dtype != torch.float64 or not accum.is_contiguous() or \
+            tuple(accum.shape) != (cam.height, cam.width, 4):
+        raise ValueError("accum must be a contiguous (height, width, 4) float64 device tensor")
+    st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
+                               0 if graph else _lib.MI_RENDER_NO_GRAPH, seed)
+    out = _lib.MiRenderStats() if stats else None
+    check(_lib.load().mi_render_filtered(scene.handle, ctypes.byref(cam), ctypes.byref(st),
+                                         ctypes.byref(sens), ctypes.c_void_p(accum.data_ptr()),
+                                         _stream(), ctypes.byref(out) if out is not None else None),
+          "mi_render_filtered")
+    return accum, out
+
+
 def render_guided(scene: Scene, cam: _lib.MiCamera, spp: int, guide, fraction: float = 0.5,
                   train: bool = True, train_steps: int = 4, batch: int = 0,
                   learning_rate: float = 1e-3, record_depth: int = 4, samples_per_wave: int = 1,
