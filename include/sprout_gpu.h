@@ -384,6 +384,19 @@ int32_t mi_light_tree_build(float const* box_min, float const* box_max, float co
                             mi_light_node* nodes, uint32_t* middles, uint32_t* mapping,
                             uint32_t* orders, uint32_t* num_nodes);
 
+/* Builds the Primitive_tree of one emissive part on the host, as
+ * Tree_builder builds it (light_tree_builder.cpp:539-795): its num_lights
+ * triangles as lights (box, cone, power = area; one-sided), the part's cone
+ * (4 floats: dominant axis, cos of the widest angle), leaves of up to
+ * max(num_lights / 64, 4) triangles.  Arrays as mi_light_tree_build; cdf_offset
+ * (one per node: the offset of the leaf's distribution in cdfs, or 0xFFFFFFFF)
+ * and cdfs (*num_cdfs floats, NULL: query) for leaves of more than 4 lights. */
+int32_t mi_light_part_tree_build(float const* box_min, float const* box_max, float const* cones,
+                                 float const* powers, uint32_t num_lights, float const* part_cone,
+                                 mi_light_node* nodes, uint32_t* middles, uint32_t* mapping,
+                                 uint32_t* orders, uint32_t* cdf_offset, float* cdfs,
+                                 uint32_t* num_nodes, uint32_t* num_cdfs);
+
 enum mi_light_flags {
     MI_LIGHT_SPLIT        = 1, /* split the tree (Adaptive light sampling) */
     MI_LIGHT_TOTAL_SPHERE = 2, /* the point is in a medium: no horizon test (translucent) */

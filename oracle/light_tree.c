@@ -239,3 +239,125 @@ float oracle_light_pdf(oracle_light_tree const* t, float const p[3], float const
         }
     }
 }
+
+/* ---- Primitive_tree (light_tree.cpp:777-862) ---- */
+
+static float leaf_pick_pdf(float const* cdf, uint32_t n, float random, uint32_t* off) {
+    uint32_t it = n;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (cdf[i] >= random) {
+            it = i;
+            break;
+        }
+    }
+    *off = 0 != it ? it - 1 : 0;
+    return cdf[*off + 1] - cdf[*off];
+}
+
+void oracle_part_random(oracle_part const* part, float const p[3], float const n[3], int32_t ts,
+                        float random, uint32_t* local, float* out_pdf) {
+    oracle_light_tree const* t   = &part->tree;
+    float                    pdf = 1.f;
+    for (uint32_t nid = 0;;) {
+        oracle_light_node const* nd = &t->nodes[nid];
+        if (has_children(nd)) {
+            uint32_t const c0 = children_or_light(nd);
+            float          p0 = node_weight(&t->nodes[c0], p, n, ts);
+            float          p1 = node_weight(&t->nodes[c0 + 1], p, n, ts);
+            float const    pt = p0 + p1;
+            p0 /= pt;
+            p1 /= pt;
+            if (random < p0) {
+                nid = c0;
+                pdf *= p0;
+                random /= p0;
+            } else {
+                nid = c0 + 1;
+                pdf *= p1;
+                random = (random - p0) / p1;
+            }
+            continue;
+        }
+        uint32_t const begin = children_or_light(nd);
+        uint32_t       off   = 0;
+        if (nd->num_lights <= 4) {
+            if (1 == nd->num_lights) {
+                *local   = t->mapping[begin];
+                *out_pdf = pdf;
+                return;
+            }
+            float cdf[5];
+            leaf_cdf(t, nd, p, n, ts, cdf);
+            float const lp = leaf_pick_pdf(cdf, nd->num_lights, random, &off);
+            *local         = t->mapping[begin + off];
+            *out_pdf       = lp * pdf;
+            return;
+        }
+        float const lp = leaf_pick_pdf(part->cdfs + part->cdf_offset[nid], nd->num_lights, random, &off);
+        *local         = t->mapping[begin + off];
+        *out_pdf       = lp * pdf;
+        return;
+    }
+}
+
+float oracle_part_pdf(oracle_part const* part, float const p[3], float const n[3], int32_t ts,
+                      uint32_t local) {
+    oracle_light_tree const* t   = &part->tree;
+    uint32_t const           lo  = t->orders[local];
+    float                    pdf = 1.f;
+    for (uint32_t nid = 0;;) {
+        oracle_light_node const* nd     = &t->nodes[nid];
+        uint32_t const           middle = t->middles[nid];
+        if (middle > 0) {
+            uint32_t const c0 = children_or_light(nd);
+            float const    p0 = node_weight(&t->nodes[c0], p, n, ts);
+            float const    p1 = node_weight(&t->nodes[c0 + 1], p, n, ts);
+            float const    pt = p0 + p1;
+            if (lo < middle) {
+                nid = c0;
+                pdf *= p0 / pt;
+            } else {
+                nid = c0 + 1;
+                pdf *= p1 / pt;
+            }
+            continue;
+        }
+        uint32_t const idx = lo - children_or_light(nd);
+        if (nd->num_lights <= 4) {
+            if (1 == nd->num_lights) {
+                return pdf;
+            }
+            float cdf[5];
+            leaf_cdf(t, nd, p, n, ts, cdf);
+            return pdf * (cdf[idx + 1] - cdf[idx]);
+        }
+        float const* cdf = part->cdfs + part->cdf_offset[nid];
+        return pdf * (cdf[idx + 1] - cdf[idx]);
+    }
+}
+
+void oracle_part_cone(float const* normals, float const* areas, uint32_t n, float out[4]) {
+    float axis[3] = {0.f, 0.f, 0.f};
+    float total   = 0.f;
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            axis[k] += areas[i] * normals[3 * i + k];
+        }
+        total += areas[i];
+    }
+    float ax[3] = {axis[0] / total, axis[1] / total, axis[2] / total};
+    float const al = sqrtf((ax[0] * ax[0] + ax[1] * ax[1]) + ax[2] * ax[2]);
+    for (int k = 0; k < 3; ++k) {
+        ax[k] /= al;
+    }
+    float angle = 0.f;
+    for (uint32_t i = 0; i < n; ++i) {
+        float const* nn = normals + 3 * i;
+        float const  c  = (ax[0] * nn[0] + ax[1] * nn[1]) + ax[2] * nn[2];
+        angle           = fmaxf(angle, acosf(fminf(fmaxf(c, -1.f), 1.f)));
+    }
+    out[0] = ax[0];
+    out[1] = ax[1];
+    out[2] = ax[2];
+    out[3] = cosf(angle);
+}

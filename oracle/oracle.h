@@ -157,6 +157,8 @@ typedef struct {
     uint32_t num_lights;
 } oracle_light_node;
 
+struct oracle_part;
+
 typedef struct {
     oracle_light_node const* nodes;
     uint32_t const*          middles;
@@ -165,7 +167,25 @@ typedef struct {
     oracle_light const*      lights;
     uint32_t                 num_nodes;
     uint32_t                 num_lights;
+    struct oracle_part const* parts; /* scene trees: the Primitive_tree of each light (a part) */
 } oracle_light_tree;
+
+/* The Primitive_tree of one emissive part over its triangles (power = area):
+ * light_tree.cpp:777-862, with Distribution_1D leaves of more than 4 lights
+ * (distribution_1d.inl:30-44). */
+typedef struct oracle_part {
+    oracle_light_tree tree;
+    uint32_t const*   cdf_offset; /* per node: its distribution in cdfs, or 0xFFFFFFFF */
+    float const*      cdfs;
+} oracle_part;
+
+void  oracle_part_random(oracle_part const* part, float const p[3], float const n[3], int32_t ts,
+                         float random, uint32_t* local, float* pdf);
+float oracle_part_pdf(oracle_part const* part, float const p[3], float const n[3], int32_t ts,
+                      uint32_t local);
+/* Part::init's cone (triangle_mesh.cpp:177-239): the dominant axis of the
+ * triangle normals weighted by area and the cosine of the widest angle */
+void  oracle_part_cone(float const* normals, float const* areas, uint32_t n, float out[4]);
 
 /* total_sphere: the point is in a medium (translucent), lights behind its
  * normal count fully */

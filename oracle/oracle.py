@@ -77,7 +77,12 @@ class _OracleLightTree(ctypes.Structure):
     _fields_ = [("nodes", ctypes.c_void_p), ("middles", ctypes.c_void_p),
                 ("mapping", ctypes.c_void_p), ("orders", ctypes.c_void_p),
                 ("lights", ctypes.c_void_p), ("num_nodes", ctypes.c_uint32),
-                ("num_lights", ctypes.c_uint32)]
+                ("num_lights", ctypes.c_uint32), ("parts", ctypes.c_void_p)]
+
+
+class _OraclePart(ctypes.Structure):
+    _fields_ = [("tree", _OracleLightTree), ("cdf_offset", ctypes.c_void_p),
+                ("cdfs", ctypes.c_void_p)]
 
 
 # oracle_light / the library's light records (48 bytes)
@@ -144,6 +149,16 @@ def lib():
         _lib.oracle_filter_eval_many.restype = None
         _lib.oracle_filter_eval_many.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                  ctypes.c_void_p]
+        _lib.oracle_part_random.restype = None
+        _lib.oracle_part_random.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int32, ctypes.c_float, ctypes.c_void_p,
+                                            ctypes.c_void_p]
+        _lib.oracle_part_pdf.restype = ctypes.c_float
+        _lib.oracle_part_pdf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int32, ctypes.c_uint32]
+        _lib.oracle_part_cone.restype = None
+        _lib.oracle_part_cone.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                          ctypes.c_void_p]
         _lib.oracle_render.restype = ctypes.c_int32
         _lib.oracle_render.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                        ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
@@ -195,8 +210,9 @@ def lib():
     return _lib
 
 
-def _ptr(a: np.ndarray) -> int:
-    return a.ctypes.data
+def _ptr(a: np.ndarray):
+    """a pointer to a's data that keeps a alive while it is passed to a call"""
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 class Tree:
@@ -605,18 +621,64 @@ def light_records(positions, indices, triangle_materials, materials) -> np.ndarr
     return np.array(out, LIGHT_DTYPE)
 
 
-class LightTree:
-    """A built light tree (nodes, middles, mapping, orders) with its lights."""
+class PartTree:
+    """The Primitive_tree of one emissive part: (nodes, middles, mapping,
+    orders, cdf_offset, cdfs) over its triangle records (power = area)."""
 
-    def __init__(self, nodes, middles, mapping, orders, lights):
+    def __init__(self, nodes, middles, mapping, orders, cdf_offset, cdfs, lights):
+        self.nodes = np.ascontiguousarray(nodes).view(LIGHT_NODE_DTYPE)
+        self.middles = np.ascontiguousarray(middles, np.uint32)
+        self.mapping = np.ascontiguousarray(mapping, np.uint32)
+        self.orders = np.ascontiguousarray(orders, np.uint32)
+        self.cdf_offset = np.ascontiguousarray(cdf_offset, np.uint32)
+        self.cdfs = np.ascontiguousarray(cdfs, np.float32)
+        self.lights = np.ascontiguousarray(lights).view(LIGHT_DTYPE)
+
+    def fill(self, c: _OraclePart) -> None:
+        c.tree = _OracleLightTree(_ptr(self.nodes), _ptr(self.middles), _ptr(self.mapping),
+                                  _ptr(self.orders), _ptr(self.lights), len(self.nodes),
+                                  len(self.lights), None)
+        c.cdf_offset = _ptr(self.cdf_offset)
+        c.cdfs = _ptr(self.cdfs) if len(self.cdfs) else None
+
+    def random(self, p, n, random, total_sphere=False):
+        """(part-local triangle, pdf) of Primitive_tree::random_light"""
+        c = _OraclePart()
+        self.fill(c)
+        local = ctypes.c_uint32(0)
+        pdf = ctypes.c_float(0)
+        lib().oracle_part_random(ctypes.byref(c), _ptr(np.ascontiguousarray(p, np.float32)),
+                                 _ptr(np.ascontiguousarray(n, np.float32)),
+                                 1 if total_sphere else 0, random, ctypes.byref(local),
+                                 ctypes.byref(pdf))
+        return local.value, pdf.value
+
+    def pdf(self, p, n, local, total_sphere=False) -> float:
+        c = _OraclePart()
+        self.fill(c)
+        return float(lib().oracle_part_pdf(ctypes.byref(c), _ptr(np.ascontiguousarray(p, np.float32)),
+                                           _ptr(np.ascontiguousarray(n, np.float32)),
+                                           1 if total_sphere else 0, int(local)))
+
+
+class LightTree:
+    """A built light tree (nodes, middles, mapping, orders) with its lights;
+    a scene's tree also holds the Primitive_tree of each light (a part)."""
+
+    def __init__(self, nodes, middles, mapping, orders, lights, parts=None):
         self.nodes = np.ascontiguousarray(nodes).view(LIGHT_NODE_DTYPE)
         self.middles = np.ascontiguousarray(middles, np.uint32)
         self.mapping = np.ascontiguousarray(mapping, np.uint32)
         self.orders = np.ascontiguousarray(orders, np.uint32)
         self.lights = np.ascontiguousarray(lights).view(LIGHT_DTYPE)
+        self.parts = parts or []
+        self._cparts = (_OraclePart * max(1, len(self.parts)))()
+        for k, pt in enumerate(self.parts):
+            pt.fill(self._cparts[k])
         self.c = _OracleLightTree(_ptr(self.nodes), _ptr(self.middles), _ptr(self.mapping),
                                   _ptr(self.orders), _ptr(self.lights),
-                                  len(self.nodes) if len(self.lights) else 0, len(self.lights))
+                                  len(self.nodes) if len(self.lights) else 0, len(self.lights),
+                                  ctypes.cast(self._cparts, ctypes.c_void_p) if self.parts else None)
 
     def random_light(self, p, n, random, split, total_sphere=False):
         pv = np.ascontiguousarray(p, np.float32)
@@ -636,19 +698,91 @@ class LightTree:
                                             int(light)))
 
 
-def scene_light_tree(positions, indices, triangle_materials, materials, build) -> LightTree:
-    """The light tree of a scene: lights from light_records, the tree over
-    them from build(box_min, box_max, cones, powers, two_sided) -> (nodes,
-    middles, mapping, orders), the caller's tree builder."""
+def part_cone(normals, areas):
+    out = np.zeros(4, np.float32)
+    lib().oracle_part_cone(_ptr(np.ascontiguousarray(normals, np.float32)),
+                           _ptr(np.ascontiguousarray(areas, np.float32)), len(areas), _ptr(out))
+    return out
+
+
+def part_records(positions, indices, triangle_materials, materials):
+    """Emissive parts: the emissive triangles (light_records) grouped by
+    material in the order the material first appears (a mesh part,
+    triangle_mesh.cpp:67-247).  Returns (scene records, one per part: box
+    center and half diagonal, the part's cone (part_cone), power = area *
+    max_component(radiance), area = the summed triangle areas, triangle = the
+    part's index; per part (triangle records with power = area, box minima,
+    box maxima, cone))."""
     rec = light_records(positions, indices, triangle_materials, materials)
-    if len(rec) == 0:
-        empty = np.zeros(0, np.uint32)
-        return LightTree(np.zeros(1, LIGHT_NODE_DTYPE), empty, empty, empty, rec)
     pos = np.asarray(positions, np.float32)
-    tris = pos[np.asarray(indices)[rec["triangle"]]]
-    nodes, middles, mapping, orders = build(tris.min(axis=1), tris.max(axis=1), rec["cone"],
-                                            rec["power"], rec["two_sided"])
-    return LightTree(nodes, middles, mapping, orders, rec)
+    idx = np.asarray(indices, np.uint32)
+    mats = np.asarray(materials).view(MATERIAL_DTYPE)
+    tmat = np.asarray(triangle_materials, np.uint32)
+    order = []
+    groups = {}
+    for k, t in enumerate(rec["triangle"]):
+        m = int(tmat[t])
+        if m not in groups:
+            groups[m] = []
+            order.append(m)
+        groups[m].append(k)
+    f32 = np.float32
+    scene = np.zeros(len(order), LIGHT_DTYPE)
+    parts = []
+    for p, m in enumerate(order):
+        tri = rec[groups[m]].copy()
+        radiance = f32(max(mats[m]["emission"]))
+        tri["power"] = tri["area"]
+        corners = pos[idx[tri["triangle"]]]
+        lo = corners.min(axis=1).astype(f32)
+        hi = corners.max(axis=1).astype(f32)
+        cone = part_cone(tri["cone"][:, 0:3], tri["area"])
+        total = f32(0)
+        for a in tri["area"]:
+            total = f32(total + f32(a))
+        blo = lo.min(axis=0)
+        bhi = hi.max(axis=0)
+        ext = (bhi - blo).astype(f32)
+        scene[p]["center"] = f32(0.5) * (blo + bhi)
+        scene[p]["radius"] = f32(0.5) * f32(np.sqrt(f32(f32(ext[0] * ext[0] + ext[1] * ext[1]) +
+                                                        ext[2] * ext[2])))
+        scene[p]["cone"] = cone
+        scene[p]["power"] = total * radiance
+        scene[p]["triangle"] = p
+        scene[p]["area"] = total
+        parts.append((tri, lo, hi, cone))
+    return scene, parts
+
+
+def scene_light_tree(positions, indices, triangle_materials, materials, build,
+                     build_part=None) -> LightTree:
+    """The light trees of a scene: the parts of part_records, the scene tree
+    over them from build(box_min, box_max, cones, powers, two_sided) ->
+    (nodes, middles, mapping, orders), and each part's Primitive_tree from
+    build_part(box_min, box_max, cones, powers, part_cone) -> (nodes,
+    middles, mapping, orders, cdf_offset, cdfs): the caller's builders
+    (default: the library's, sprout_amd.lights)."""
+    if build_part is None:
+        from sprout_amd import lights as _lights
+
+        build_part = _lights.build_part_tree
+    scene, parts = part_records(positions, indices, triangle_materials, materials)
+    if len(scene) == 0:
+        empty = np.zeros(0, np.uint32)
+        return LightTree(np.zeros(1, LIGHT_NODE_DTYPE), empty, empty, empty, scene)
+    pos = np.asarray(positions, np.float32)
+    idx = np.asarray(indices, np.uint32)
+    ptrees = []
+    boxes_lo, boxes_hi = [], []
+    for tri, lo, hi, cone in parts:
+        ptrees.append(PartTree(*build_part(lo, hi, tri["cone"], tri["power"], cone), tri))
+        boxes_lo.append(lo.min(axis=0))
+        boxes_hi.append(hi.max(axis=0))
+    nodes, middles, mapping, orders = build(np.array(boxes_lo, np.float32),
+                                            np.array(boxes_hi, np.float32), scene["cone"],
+                                            scene["power"], scene["two_sided"])
+    del pos, idx
+    return LightTree(nodes, middles, mapping, orders, scene, ptrees)
 
 
 class Medium:

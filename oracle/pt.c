@@ -130,6 +130,7 @@ typedef struct {
     oracle_bvh              bvh;
     oracle_light_tree const* tree;
     uint32_t*               tri_light;
+    uint32_t*               tri_local;
     oracle_camera const*    cam;
     oracle_pt_settings const* set;
     v3                      fwd, right, up;
@@ -433,7 +434,13 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
                             float const origin[3] = {o.x, o.y, o.z};
                             float const tree_pdf =
                                 oracle_light_pdf(s->tree, origin, prev_n, translucent, split, l);
-                            float const ls_pdf = (t * t) / (cos_l * s->tree->lights[l].area);
+                            /* Part::pdf: the part tree's pdf times the relative area */
+                            oracle_part const*  part  = &s->tree->parts[l];
+                            uint32_t const      local = s->tri_local[tri];
+                            float const         ppdf  = oracle_part_pdf(part, origin, prev_n, translucent, local);
+                            oracle_light const* L     = &s->tree->lights[l];
+                            oracle_light const* T     = &part->tree.lights[local];
+                            float const ls_pdf = (t * t) / (cos_l * L->area) * (ppdf * (L->area / T->area));
                             weight             = power_heuristic(bsdf_pdf, ls_pdf * tree_pdf);
                         } else if (ORACLE_PT_NEE == mode) {
                             weight = 0.f;
@@ -487,8 +494,16 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
             uint32_t const num_picks =
                 oracle_light_random(s->tree, po, gn, scatter, select, split, picks, ppdf);
             for (uint32_t k = 0; k < num_picks; ++k) {
-                oracle_light const* L  = &s->tree->lights[picks[k]];
-                uint32_t const      lt = L->triangle;
+                oracle_light const* L  = &s->tree->lights[picks[k]]; /* the part */
+                /* Mesh::sample (triangle_mesh.cpp:462-516): the part's tree
+                 * picks a triangle, then a point on it */
+                oracle_part const*  part = &s->tree->parts[picks[k]];
+                float const         rp   = pcg_float(&rng);
+                uint32_t            local;
+                float               tpdf;
+                oracle_part_random(part, po, gn, scatter, rp, &local, &tpdf);
+                oracle_light const* T  = &part->tree.lights[local];
+                uint32_t const      lt = T->triangle;
                 float const         u1 = pcg_float(&rng);
                 float const         u2 = pcg_float(&rng);
                 if (ORACLE_PT_BSDF == mode) {
@@ -511,7 +526,7 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
                 if (cl >= DOT_MIN) {
                     float f[4];
                     surface_evaluate(&sf, wi, f);
-                    float const light_pdf = sl / (cl * L->area) * ppdf[k];
+                    float const light_pdf = (sl / (cl * L->area)) * (tpdf * (L->area / T->area)) * ppdf[k];
                     float const weight    = ORACLE_PT_MIS == mode
                                                 ? predivided_power_heuristic(light_pdf, f[3])
                                                 : 1.f / light_pdf;
@@ -670,9 +685,19 @@ int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t co
     for (uint32_t t = 0; t < num_triangles; ++t) {
         s.tri_light[t] = SENTINEL;
     }
-    for (uint32_t l = 0; l < tree->num_lights; ++l) {
-        if (tree->lights[l].triangle < num_triangles) {
-            s.tri_light[tree->lights[l].triangle] = l;
+    s.tri_local = (uint32_t*)malloc(sizeof(uint32_t) * num_triangles);
+    for (uint32_t t = 0; t < num_triangles; ++t) {
+        s.tri_local[t] = SENTINEL;
+    }
+    /* scene light l is an emissive part; its triangles are its part tree's lights */
+    for (uint32_t l = 0; l < tree->num_lights && tree->parts; ++l) {
+        oracle_part const* part = &tree->parts[l];
+        for (uint32_t i = 0; i < part->tree.num_lights; ++i) {
+            uint32_t const t = part->tree.lights[i].triangle;
+            if (t < num_triangles) {
+                s.tri_light[t] = l;
+                s.tri_local[t] = i;
+            }
         }
     }
 
@@ -713,6 +738,7 @@ int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t co
     pthread_mutex_destroy(&s.lock);
 
     free(s.tri_light);
+    free(s.tri_local);
     oracle_bvh_free(&s.bvh);
     return s.status;
 }

@@ -153,6 +153,8 @@ SIGNATURES = {
                                 ctypes.POINTER(MiGuideSettings), _vp, _vp,
                                 ctypes.POINTER(MiRenderStats)]),
     "mi_light_tree_build": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _pu32]),
+    "mi_light_part_tree_build": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp,
+                                        _vp, _pu32, _pu32]),
     "mi_light_tree_sample": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp]),
     "mi_light_tree_pdf": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp]),
     "mi_scene_light_tree": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _pu32, _pu32]),

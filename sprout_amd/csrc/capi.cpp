@@ -730,6 +730,36 @@ int32_t mi_light_tree_build(float const* box_min, float const* box_max, float co
     return MI_OK;
 }
 
+int32_t mi_light_part_tree_build(float const* box_min, float const* box_max, float const* cones,
+                                 float const* powers, uint32_t num_lights, float const* part_cone,
+                                 mi_light_node* nodes, uint32_t* middles, uint32_t* mapping,
+                                 uint32_t* orders, uint32_t* cdf_offset, float* cdfs,
+                                 uint32_t* num_nodes, uint32_t* num_cdfs) {
+    clear_error();
+    if (!num_nodes || !num_cdfs || !part_cone ||
+        (num_lights && (!box_min || !box_max || !cones || !powers))) {
+        return fail(MI_ERR_ARGS, "mi_light_part_tree_build: bad arguments");
+    }
+    HostLightTree t;
+    build_part_tree(box_min, box_max, cones, powers, num_lights, part_cone, t);
+    *num_nodes = uint32_t(t.nodes.size());
+    *num_cdfs  = uint32_t(t.cdfs.size());
+    if (nodes) {
+        if (!middles || !mapping || !orders || !cdf_offset || (!cdfs && !t.cdfs.empty())) {
+            return fail(MI_ERR_ARGS, "mi_light_part_tree_build: bad arguments");
+        }
+        std::memcpy(nodes, t.nodes.data(), t.nodes.size() * sizeof(mi_light_node));
+        std::memcpy(middles, t.middles.data(), t.middles.size() * sizeof(uint32_t));
+        std::memcpy(mapping, t.light_mapping.data(), t.light_mapping.size() * sizeof(uint32_t));
+        std::memcpy(orders, t.light_orders.data(), t.light_orders.size() * sizeof(uint32_t));
+        std::memcpy(cdf_offset, t.cdf_offset.data(), t.cdf_offset.size() * sizeof(uint32_t));
+        if (!t.cdfs.empty()) {
+            std::memcpy(cdfs, t.cdfs.data(), t.cdfs.size() * sizeof(float));
+        }
+    }
+    return MI_OK;
+}
+
 int32_t mi_light_tree_sample(uint32_t scene, float const* points, float const* normals,
                              float const* randoms, uint32_t split, uint32_t n, uint32_t* picks,
                              uint32_t* counts, void* stream) {

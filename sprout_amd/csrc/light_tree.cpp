@@ -24,6 +24,7 @@ namespace {
 
 constexpr float    Pi                    = 3.14159265358979323846f;
 constexpr uint32_t Scene_sweep_threshold = 128;
+constexpr uint32_t Part_sweep_threshold  = 32;
 constexpr uint32_t Num_slices            = 16;
 
 struct F3 {
@@ -234,11 +235,16 @@ struct Builder {
     std::vector<Candidate>  candidates;
     uint32_t                current = 1;
     uint32_t                order   = 0;
+    // Primitive_tree of one part (light_tree_builder.cpp:539-758): leaves of
+    // up to max_primitives lights, 32-light sweeps, exhausted nodes become
+    // leaves, one-sided throughout
+    bool     part           = false;
+    uint32_t max_primitives = 4;
 
     Candidate evaluate_splits(uint32_t begin, uint32_t end, Box const& bounds, float cone_weight) {
         uint32_t const len = end - begin;
         candidates.clear();
-        if (len < Scene_sweep_threshold) {
+        if (len < (part ? Part_sweep_threshold : Scene_sweep_threshold)) {
             for (uint32_t i = begin; i < end; ++i) {
                 F3 const& mx = set.boxes[mapping[i]].hi;
                 for (int a = 0; a < 3; ++a) {
@@ -281,6 +287,56 @@ struct Builder {
             }
         }
         return candidates[sc];
+    }
+
+    // Tree_builder::assign (light_tree_builder.cpp:737-758)
+    uint32_t assign(uint32_t node_id, uint32_t begin, uint32_t end, Box const& bounds,
+                    Cone const& cone, float total_power) {
+        for (uint32_t i = begin; i < end; ++i) {
+            orders[mapping[i]] = order++;
+        }
+        Build_node& node = nodes[node_id];
+        node.bounds      = bounds;
+        node.cone        = cone;
+        node.power       = total_power;
+        node.var         = variance(mapping.data(), begin, end, set);
+        node.middle      = 0;
+        node.children    = begin;
+        node.num_lights  = end - begin;
+        node.two_sided   = false;
+        return end;
+    }
+
+    // Tree_builder::split of a part (light_tree_builder.cpp:661-735)
+    uint32_t split_part(uint32_t node_id, uint32_t begin, uint32_t end, Box const& bounds,
+                        Cone const& cone, float total_power) {
+        uint32_t const len = end - begin;
+        if (len <= max_primitives) {
+            return assign(node_id, begin, end, bounds, cone, total_power);
+        }
+        uint32_t const child0      = current;
+        float const    cone_weight = cone_cost(cone.v[3]);
+        Candidate      sc          = evaluate_splits(begin, end, bounds, cone_weight);
+        if (sc.exhausted) {
+            return assign(node_id, begin, end, bounds, cone, total_power);
+        }
+        uint32_t const split_node = uint32_t(
+            std::partition(mapping.data() + begin, mapping.data() + end,
+                           [&](uint32_t l) { return sc.behind(set.boxes[l].hi); }) -
+            mapping.data());
+        current += 2;
+        uint32_t const c0_end = split_part(child0, begin, split_node, sc.box0, sc.cone0, sc.power0);
+        uint32_t const c1_end = split_part(child0 + 1, split_node, end, sc.box1, sc.cone1, sc.power1);
+        Build_node& node = nodes[node_id];
+        node.bounds      = bounds;
+        node.cone        = cone;
+        node.power       = total_power;
+        node.var         = variance(mapping.data(), begin, end, set);
+        node.middle      = c0_end;
+        node.children    = child0;
+        node.num_lights  = len;
+        node.two_sided   = false;
+        return c1_end;
     }
 
     uint32_t split(uint32_t node_id, uint32_t begin, uint32_t end, Box const& bounds,
@@ -353,6 +409,52 @@ struct Builder {
     }
 };
 
+void load_lights(float const* box_min, float const* box_max, float const* cones,
+                 float const* powers, uint8_t const* two_sided, uint32_t num_lights, Lights& set) {
+    set.boxes.resize(num_lights);
+    set.cones.resize(num_lights);
+    set.powers.assign(powers, powers + num_lights);
+    if (two_sided) {
+        set.two_sided.assign(two_sided, two_sided + num_lights);
+    } else {
+        set.two_sided.assign(num_lights, 0);
+    }
+    for (uint32_t l = 0; l < num_lights; ++l) {
+        for (int k = 0; k < 3; ++k) {
+            set.boxes[l].lo[k] = box_min[3 * l + k];
+            set.boxes[l].hi[k] = box_max[3 * l + k];
+        }
+        for (int k = 0; k < 4; ++k) {
+            set.cones[l].v[k] = cones[4 * l + k];
+        }
+    }
+}
+
+void serialize(Builder const& b, HostLightTree& out) {
+    // serialize, light_tree_builder.cpp:712-733
+    out.nodes.resize(b.current);
+    out.middles.resize(b.current);
+    for (uint32_t i = 0; i < b.current; ++i) {
+        Build_node const& s = b.nodes[i];
+        Light_node&       d = out.nodes[i];
+        F3 const          p = s.bounds.position();
+        F3 const          e = s.bounds.extent();
+        d.center[0]         = p[0];
+        d.center[1]         = p[1];
+        d.center[2]         = p[2];
+        d.radius            = 0.5f * std::sqrt(dot3(e.v, e.v));
+        for (int k = 0; k < 4; ++k) {
+            d.cone[k] = s.cone.v[k];
+        }
+        d.power      = s.power;
+        d.variance   = s.var;
+        d.bits       = (s.middle > 0 ? 1u : 0u) | (s.two_sided ? 2u : 0u) | (s.children << 2);
+        d.num_lights = s.num_lights;
+        out.middles[i] = s.middle;
+    }
+}
+
+
 }  // namespace
 
 void build_light_tree(float const* box_min, float const* box_max, float const* cones,
@@ -363,19 +465,7 @@ void build_light_tree(float const* box_min, float const* box_max, float const* c
         return;
     }
     Lights set;
-    set.boxes.resize(num_lights);
-    set.cones.resize(num_lights);
-    set.powers.assign(powers, powers + num_lights);
-    set.two_sided.assign(two_sided, two_sided + num_lights);
-    for (uint32_t l = 0; l < num_lights; ++l) {
-        for (int k = 0; k < 3; ++k) {
-            set.boxes[l].lo[k] = box_min[3 * l + k];
-            set.boxes[l].hi[k] = box_max[3 * l + k];
-        }
-        for (int k = 0; k < 4; ++k) {
-            set.cones[l].v[k] = cones[4 * l + k];
-        }
-    }
+    load_lights(box_min, box_max, cones, powers, two_sided, num_lights, set);
 
     out.light_mapping.resize(num_lights);
     out.light_orders.resize(num_lights);
@@ -397,27 +487,71 @@ void build_light_tree(float const* box_min, float const* box_max, float const* c
         total += set.powers[l];
     }
     b.split(0, 0, num_lights, bounds, cone, ts, total);
+    serialize(b, out);
+}
 
-    // serialize, light_tree_builder.cpp:712-733
-    out.nodes.resize(b.current);
-    out.middles.resize(b.current);
-    for (uint32_t i = 0; i < b.current; ++i) {
-        Build_node const& s = b.nodes[i];
-        Light_node&       d = out.nodes[i];
-        F3 const          p = s.bounds.position();
-        F3 const          e = s.bounds.extent();
-        d.center[0]         = p[0];
-        d.center[1]         = p[1];
-        d.center[2]         = p[2];
-        d.radius            = 0.5f * std::sqrt(dot3(e.v, e.v));
-        for (int k = 0; k < 4; ++k) {
-            d.cone[k] = s.cone.v[k];
+void build_part_tree(float const* box_min, float const* box_max, float const* cones,
+                     float const* powers, uint32_t num_lights, float const part_cone[4],
+                     HostLightTree& out) {
+    out = HostLightTree();
+    if (0 == num_lights) {
+        return;
+    }
+    Lights set;
+    load_lights(box_min, box_max, cones, powers, nullptr, num_lights, set);
+    out.light_mapping.resize(num_lights);
+    out.light_orders.resize(num_lights);
+    for (uint32_t l = 0; l < num_lights; ++l) {
+        out.light_mapping[l] = l;
+    }
+    Builder b{set, out.light_mapping, out.light_orders, {}, {}, 1, 0};
+    b.part           = true;
+    b.max_primitives = std::max(num_lights / 64, 4u);
+    b.nodes.resize(2 * size_t(num_lights) - 1);
+
+    // Part::init (triangle_mesh.cpp:177-239): the box of the lights with
+    // power, the part's cone, the total power
+    Box   bounds;
+    float total = 0.f;
+    for (uint32_t l = 0; l < num_lights; ++l) {
+        if (set.powers[l] > 0.f) {
+            bounds.merge(set.boxes[l]);
+            total += set.powers[l];
         }
-        d.power      = s.power;
-        d.variance   = s.var;
-        d.bits       = (s.middle > 0 ? 1u : 0u) | (s.two_sided ? 2u : 0u) | (s.children << 2);
-        d.num_lights = s.num_lights;
-        out.middles[i] = s.middle;
+    }
+    Cone cone;
+    for (int k = 0; k < 4; ++k) {
+        cone.v[k] = part_cone[k];
+    }
+    b.split_part(0, 0, num_lights, bounds, cone, total);
+    serialize(b, out);
+
+    // leaves of more than 4 lights sample Distribution_1D over the powers
+    // (light_tree_builder.cpp:760-795, distribution_1d.cpp:42-82)
+    out.cdf_offset.assign(out.nodes.size(), 0xFFFFFFFFu);
+    for (uint32_t i = 0; i < out.nodes.size(); ++i) {
+        Light_node const& nd = out.nodes[i];
+        if ((nd.bits & 1u) || nd.num_lights <= 4) {
+            continue;
+        }
+        uint32_t const first = nd.bits >> 2;
+        uint32_t const num   = nd.num_lights;
+        float          integral = 0.f;
+        for (uint32_t t = 0; t < num; ++t) {
+            integral += set.powers[out.light_mapping[first + t]];
+        }
+        out.cdf_offset[i] = uint32_t(out.cdfs.size());
+        if (0.f == integral) {
+            out.cdfs.push_back(1.f);  // size 1: {1, 1}
+            out.cdfs.push_back(1.f);
+            continue;
+        }
+        float const ii = 1.f / integral;
+        out.cdfs.push_back(0.f);
+        for (uint32_t t = 1; t < num; ++t) {
+            out.cdfs.push_back(std::fma(set.powers[out.light_mapping[first + t - 1]], ii, out.cdfs.back()));
+        }
+        out.cdfs.push_back(1.f);
     }
 }
 

@@ -142,6 +142,8 @@ struct Wave {
     dev::Light_tree_view   tree;
     uint32_t const*        tri_light;  // light of an emissive triangle
     uint32_t               split_lights;
+    dev::Part_pools        parts;      // the Primitive_tree of every emissive part
+    uint32_t const*        tri_local;  // an emissive triangle's index in its part
     dev::Medium_view       medium;
     // instanced scenes
     // camera grid: the image grown by border pixels on every side (filtered sensor)
@@ -647,8 +649,17 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                                 w.tree, v3(r.origin[0], r.origin[1], r.origin[2]),
                                 v3(path.prev_n[0], path.prev_n[1], path.prev_n[2]),
                                 0 != (path.flags & Flag_translucent), split, l);
-                            float const ls_pdf = (h.t * h.t) / (cos_l * w.tree.lights[l].area);
-                            weight             = power_heuristic(path.bsdf_pdf, ls_pdf * tree_pdf);
+                            // the triangle's pdf in its part's Primitive_tree
+                            // times its relative area (Part::pdf, triangle_mesh.cpp:264-270)
+                            uint32_t const    local = w.tri_local[h.triangle];
+                            float const       ppdf  = dev::part_pdf(
+                                w.parts, l, v3(r.origin[0], r.origin[1], r.origin[2]),
+                                v3(path.prev_n[0], path.prev_n[1], path.prev_n[2]),
+                                0 != (path.flags & Flag_translucent), local);
+                            Light_info const& L = w.tree.lights[l];
+                            Light_info const& T = w.parts.lights[w.parts.parts[l].light_offset + local];
+                            float const ls_pdf  = (h.t * h.t) / (cos_l * L.area) * (ppdf * (L.area / T.area));
+                            weight              = power_heuristic(path.bsdf_pdf, ls_pdf * tree_pdf);
                         }
                         for (int k = 0; k < 3; ++k) {
                             path.radiance[k] += path.throughput[k] * (weight * m.emission[k]);
@@ -715,8 +726,13 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             Shadow sh;
             {
                 dev::Light_pick const pick = picks[k];
-                Light_info const      L    = w.tree.lights[pick.light];
-                uint32_t const        lt   = L.triangle;
+                Light_info const      L    = w.tree.lights[pick.light];  // the part
+                // Mesh::sample (triangle_mesh.cpp:462-516): the part's tree
+                // picks a triangle, then a point on it
+                float const           rp   = path.rng.uniform();
+                dev::Light_pick const tp   = dev::part_random(w.parts, pick.light, p_o, geo_n, in_medium, rp);
+                Light_info const      T    = w.parts.lights[w.parts.parts[pick.light].light_offset + tp.light];
+                uint32_t const        lt   = T.triangle;
                 float const           u1   = path.rng.uniform();
                 float const           u2   = path.rng.uniform();
 
@@ -739,7 +755,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                 if (cl >= dev::Dot_min) {
                     float f[4];
                     s.evaluate(wi, f);
-                    float const light_pdf = sl / (cl * L.area) * pick.pdf;
+                    float const light_pdf = (sl / (cl * L.area)) * (tp.pdf * (L.area / T.area)) * pick.pdf;
                     float       dir_pdf   = f[3];
                     if (Guided && !in_medium && depth < w.guide_depth) {
                         dev::Vmf_mixture mix;
@@ -1387,6 +1403,15 @@ void RenderScene::release() {
     (void)hipFree(light_orders);
     (void)hipFree(light_infos);
     (void)hipFree(tri_light);
+    (void)hipFree(tri_local);
+    (void)hipFree(part_infos);
+    (void)hipFree(part_nodes);
+    (void)hipFree(part_middles);
+    (void)hipFree(part_cdf_offset);
+    (void)hipFree(part_cdfs);
+    (void)hipFree(part_mapping);
+    (void)hipFree(part_orders);
+    (void)hipFree(part_lights);
     release_medium();
     tlas.release();
     for (DeviceBvh& t : mesh_trees) {
@@ -1526,13 +1551,17 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
         r.kind      = in.kind;
     }
 
-    // emissive triangles: one light each (a single-triangle part), the
-    // box, cone and power Scene::prop_prepare_sampling gives it
-    // (scene.cpp:440-506, light.cpp:38-48)
-    std::vector<Light_info> infos;
-    std::vector<float>      bmin, bmax, cones, powers;
-    std::vector<uint8_t>    two_sided;
-    std::vector<uint32_t>   tri_light(num_triangles, dev::Null);
+    // Emissive parts: the triangles of one emissive material, in the order
+    // the material first appears (a mesh part, triangle_mesh.cpp:67-247).
+    // Each part is one light of the scene tree, with the box, cone and
+    // power Scene::prop_prepare_sampling gives it (scene.cpp:440-506,
+    // light.cpp:38-48), and has a Primitive_tree over its triangles
+    // (power = area, one-sided).
+    std::vector<int32_t>               part_of(num_materials, -1);
+    std::vector<std::vector<uint32_t>> part_tris;
+    std::vector<uint32_t>              tri_light(num_triangles, dev::Null);
+    std::vector<uint32_t>              tri_local(num_triangles, dev::Null);
+    std::vector<Light_info>            tri_info(num_triangles);
     for (uint32_t t = 0; t < num_triangles; ++t) {
         mi_material const& mat = materials[tri_material[t]];
         if (!is_light(mat)) {
@@ -1554,34 +1583,125 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
         for (float& x : n) {
             x /= len;
         }
-        Light_info li;
+        Light_info& li = tri_info[t];
         float lo[3], hi[3];
         for (int k = 0; k < 3; ++k) {
-            lo[k] = std::min(std::min(a[k], b[k]), c[k]);
-            hi[k] = std::max(std::max(a[k], b[k]), c[k]);
-            bmin.push_back(lo[k]);
-            bmax.push_back(hi[k]);
+            lo[k]        = std::min(std::min(a[k], b[k]), c[k]);
+            hi[k]        = std::max(std::max(a[k], b[k]), c[k]);
             li.center[k] = 0.5f * (lo[k] + hi[k]);
             li.cone[k]   = n[k];
         }
         float const ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
-        li.radius   = 0.5f * std::sqrt((ext[0] * ext[0] + ext[1] * ext[1]) + ext[2] * ext[2]);
-        li.cone[3]  = 1.f;
-        li.power    = area * radiance;
+        li.radius    = 0.5f * std::sqrt((ext[0] * ext[0] + ext[1] * ext[1]) + ext[2] * ext[2]);
+        li.cone[3]   = 1.f;
+        li.power     = area;  // the part tree's power (no emission map)
         li.two_sided = 0;
-        li.triangle = t;
-        li.area     = area;
-        for (int k = 0; k < 4; ++k) {
-            cones.push_back(li.cone[k]);
+        li.triangle  = t;
+        li.area      = area;
+        uint32_t const m = tri_material[t];
+        if (part_of[m] < 0) {
+            part_of[m] = int32_t(part_tris.size());
+            part_tris.emplace_back();
         }
-        powers.push_back(li.power);
-        two_sided.push_back(0);
-        tri_light[t] = uint32_t(infos.size());
-        infos.push_back(li);
+        tri_light[t] = uint32_t(part_of[m]);
+        tri_local[t] = uint32_t(part_tris[part_of[m]].size());
+        part_tris[part_of[m]].push_back(t);
+    }
+
+    uint32_t const          num_parts = uint32_t(part_tris.size());
+    std::vector<Light_info> infos(num_parts);
+    std::vector<float>      bmin, bmax, cones, powers;
+    std::vector<uint8_t>    two_sided(num_parts, 0);
+    // the part pools
+    std::vector<dev::Part_info> pinfo(num_parts);
+    std::vector<Light_node>     pnodes;
+    std::vector<uint32_t>       pmiddles, pcdf_off, pmapping, porders;
+    std::vector<float>          pcdfs;
+    std::vector<Light_info>     plights;
+    for (uint32_t p = 0; p < num_parts; ++p) {
+        std::vector<uint32_t> const& tris = part_tris[p];
+        uint32_t const               n    = uint32_t(tris.size());
+        std::vector<float>           tmin(3 * size_t(n)), tmax(3 * size_t(n)), tcone(4 * size_t(n)), tpow(n);
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float axis[3] = {0.f, 0.f, 0.f};
+        float total   = 0.f;
+        for (uint32_t i = 0; i < n; ++i) {
+            Light_info const& li = tri_info[tris[i]];
+            float const*      a  = positions + 3 * size_t(indices[3 * size_t(tris[i]) + 0]);
+            float const*      b  = positions + 3 * size_t(indices[3 * size_t(tris[i]) + 1]);
+            float const*      c  = positions + 3 * size_t(indices[3 * size_t(tris[i]) + 2]);
+            for (int k = 0; k < 3; ++k) {
+                tmin[3 * i + k] = std::min(std::min(a[k], b[k]), c[k]);
+                tmax[3 * i + k] = std::max(std::max(a[k], b[k]), c[k]);
+                lo[k]           = std::min(lo[k], tmin[3 * i + k]);
+                hi[k]           = std::max(hi[k], tmax[3 * i + k]);
+                axis[k] += li.power * li.cone[k];
+            }
+            for (int k = 0; k < 4; ++k) {
+                tcone[4 * i + k] = li.cone[k];
+            }
+            tpow[i] = li.power;
+            total += li.power;
+        }
+        // Part::init: the dominant axis and the widest angle from it
+        float ax[3] = {axis[0] / total, axis[1] / total, axis[2] / total};
+        float const al = std::sqrt((ax[0] * ax[0] + ax[1] * ax[1]) + ax[2] * ax[2]);
+        for (float& x : ax) {
+            x /= al;
+        }
+        float angle = 0.f;
+        for (uint32_t i = 0; i < n; ++i) {
+            float const* nn = tri_info[tris[i]].cone;
+            float const  cc = (ax[0] * nn[0] + ax[1] * nn[1]) + ax[2] * nn[2];
+            angle           = std::max(angle, std::acos(std::min(std::max(cc, -1.f), 1.f)));
+        }
+        float const part_cone[4] = {ax[0], ax[1], ax[2], std::cos(angle)};
+
+        HostLightTree tree;
+        build_part_tree(tmin.data(), tmax.data(), tcone.data(), tpow.data(), n, part_cone, tree);
+        dev::Part_info& pi = pinfo[p];
+        pi.node_offset     = uint32_t(pnodes.size());
+        pi.light_offset    = uint32_t(plights.size());
+        pi.num_nodes       = uint32_t(tree.nodes.size());
+        pi.num_lights      = n;
+        uint32_t const cdf_base = uint32_t(pcdfs.size());
+        for (uint32_t i = 0; i < tree.nodes.size(); ++i) {
+            pnodes.push_back(tree.nodes[i]);
+            pmiddles.push_back(tree.middles[i]);
+            pcdf_off.push_back(tree.cdf_offset[i] == 0xFFFFFFFFu ? dev::Null : cdf_base + tree.cdf_offset[i]);
+        }
+        pcdfs.insert(pcdfs.end(), tree.cdfs.begin(), tree.cdfs.end());
+        for (uint32_t i = 0; i < n; ++i) {
+            pmapping.push_back(tree.light_mapping[i]);
+            porders.push_back(tree.light_orders[i]);
+            plights.push_back(tri_info[tris[i]]);
+        }
+
+        // the scene light of the part
+        float const radiance = std::max(std::max(materials[tri_material[tris[0]]].emission[0],
+                                                 materials[tri_material[tris[0]]].emission[1]),
+                                        materials[tri_material[tris[0]]].emission[2]);
+        Light_info& L = infos[p];
+        for (int k = 0; k < 3; ++k) {
+            bmin.push_back(lo[k]);
+            bmax.push_back(hi[k]);
+            L.center[k] = 0.5f * (lo[k] + hi[k]);
+        }
+        float const ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        L.radius    = 0.5f * std::sqrt((ext[0] * ext[0] + ext[1] * ext[1]) + ext[2] * ext[2]);
+        for (int k = 0; k < 4; ++k) {
+            L.cone[k] = part_cone[k];
+            cones.push_back(part_cone[k]);
+        }
+        L.power     = total * radiance;
+        L.two_sided = 0;
+        L.triangle  = p;
+        L.area      = total;
+        powers.push_back(L.power);
     }
     HostLightTree tree;
     build_light_tree(bmin.data(), bmax.data(), cones.data(), powers.data(), two_sided.data(),
-                     uint32_t(infos.size()), tree);
+                     num_parts, tree);
 
     RenderScene s;
     s.num_vertices  = num_vertices;
@@ -1624,6 +1744,22 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
         if (!e) e = hipMemcpy(s.light_infos, infos.data(), infos.size() * sizeof(Light_info), hipMemcpyHostToDevice);
     }
     if (!e) e = hipMemcpy(s.tri_light, tri_light.data(), size_t(num_triangles) * sizeof(uint32_t), hipMemcpyHostToDevice);
+    // the part pools
+    auto upload = [&e](auto*& dst, auto const& v) {
+        using T = std::remove_reference_t<decltype(v[0])>;
+        size_t const bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+        if (!e) e = hipMalloc(reinterpret_cast<void**>(&dst), bytes);
+        if (!e && !v.empty()) e = hipMemcpy(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    };
+    upload(s.part_infos, pinfo);
+    upload(s.part_nodes, pnodes);
+    upload(s.part_middles, pmiddles);
+    upload(s.part_cdf_offset, pcdf_off);
+    upload(s.part_cdfs, pcdfs);
+    upload(s.part_mapping, pmapping);
+    upload(s.part_orders, porders);
+    upload(s.part_lights, plights);
+    upload(s.tri_local, tri_local);
     if (e) {
         s.release();
         return fail(MI_ERR_HIP, "mi_scene_create: %s", hipGetErrorString(e));
@@ -2074,6 +2210,15 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.tree.num_nodes = scene.num_lights ? scene.num_light_nodes : 0;
     w.tri_light      = scene.tri_light;
     w.split_lights   = split_lights ? 1u : 0u;
+    w.parts.parts      = scene.part_infos;
+    w.parts.nodes      = scene.part_nodes;
+    w.parts.middles    = scene.part_middles;
+    w.parts.cdf_offset = scene.part_cdf_offset;
+    w.parts.cdfs       = scene.part_cdfs;
+    w.parts.mapping    = scene.part_mapping;
+    w.parts.orders     = scene.part_orders;
+    w.parts.lights     = scene.part_lights;
+    w.tri_local        = scene.tri_local;
     w.medium         = scene.medium;
     w.medium_event   = state.medium_event;
     w.medium_p       = state.medium_p;

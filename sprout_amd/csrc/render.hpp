@@ -60,7 +60,17 @@ struct RenderScene {
     uint32_t*        light_mapping = nullptr;
     uint32_t*        light_orders  = nullptr;
     Light_info*      light_infos   = nullptr;
-    uint32_t*        tri_light     = nullptr;  // light of each triangle, or Null
+    uint32_t*        tri_light     = nullptr;  // emissive part of each triangle, or Null
+    uint32_t*        tri_local     = nullptr;  // index of an emissive triangle in its part
+    // Primitive_tree pools of the emissive parts (see dev::Part_pools)
+    dev::Part_info*  part_infos      = nullptr;
+    Light_node*      part_nodes      = nullptr;
+    uint32_t*        part_middles    = nullptr;
+    uint32_t*        part_cdf_offset = nullptr;
+    float*           part_cdfs       = nullptr;
+    uint32_t*        part_mapping    = nullptr;
+    uint32_t*        part_orders     = nullptr;
+    Light_info*      part_lights     = nullptr;
     // participating medium in an axis-aligned box (none while density is null)
     float*           medium_density = nullptr;  // nx * ny * nz, x fastest
     uint32_t*        medium_nodes   = nullptr;  // majorant gridtree

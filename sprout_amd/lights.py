@@ -47,6 +47,36 @@ def build_tree(box_min, box_max, cones, powers, two_sided):
     return nodes[:nn.value], middles[:nn.value], mapping[:n], orders[:n]
 
 
+def build_part_tree(box_min, box_max, cones, powers, part_cone):
+    """Primitive_tree of one part's n triangles -> (nodes, middles, mapping,
+    orders, cdf_offset, cdfs)."""
+    bmin = np.ascontiguousarray(box_min, np.float32)
+    bmax = np.ascontiguousarray(box_max, np.float32)
+    cn = np.ascontiguousarray(cones, np.float32)
+    pw = np.ascontiguousarray(powers, np.float32)
+    pc = np.ascontiguousarray(part_cone, np.float32)
+    n = len(pw)
+    nn, nc = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    lib = _lib.load()
+    check(lib.mi_light_part_tree_build(bmin.ctypes.data, bmax.ctypes.data, cn.ctypes.data,
+                                       pw.ctypes.data, n, pc.ctypes.data, None, None, None, None,
+                                       None, None, ctypes.byref(nn), ctypes.byref(nc)),
+          "mi_light_part_tree_build")
+    nodes = np.zeros(max(nn.value, 1), NODE_DTYPE)
+    middles = np.zeros(max(nn.value, 1), np.uint32)
+    mapping = np.zeros(max(n, 1), np.uint32)
+    orders = np.zeros(max(n, 1), np.uint32)
+    offs = np.zeros(max(nn.value, 1), np.uint32)
+    cdfs = np.zeros(max(nc.value, 1), np.float32)
+    check(lib.mi_light_part_tree_build(bmin.ctypes.data, bmax.ctypes.data, cn.ctypes.data,
+                                       pw.ctypes.data, n, pc.ctypes.data, nodes.ctypes.data,
+                                       middles.ctypes.data, mapping.ctypes.data, orders.ctypes.data,
+                                       offs.ctypes.data, cdfs.ctypes.data, ctypes.byref(nn),
+                                       ctypes.byref(nc)), "mi_light_part_tree_build")
+    return (nodes[:nn.value], middles[:nn.value], mapping[:n], orders[:n], offs[:nn.value],
+            cdfs[:nc.value])
+
+
 def scene_tree(scene):
     """(nodes, middles, mapping, orders, lights) of a render.Scene."""
     lib = _lib.load()

@@ -42,7 +42,8 @@ def test_scene_tree_matches_oracle_lights(gpu, lit):
     sc, scene = lit
     nodes, middles, mapping, orders, infos = lights.scene_tree(sc)
     ref = light_tree(*scene)
-    assert len(infos) == len(ref.lights) == 50  # 48 + the two triangles of the ceiling light
+    # one light per emissive part: the 48 small lights and the ceiling light's two triangles
+    assert len(infos) == len(ref.lights) == 49
     for f in ("center", "radius", "cone", "power", "triangle", "area"):
         assert np.array_equal(infos[f], ref.lights[f]), f
     assert np.array_equal(nodes.view(np.uint8), ref.nodes.view(np.uint8))
@@ -69,7 +70,7 @@ def test_device_sampling_matches_oracle(gpu, bright, split):
     rng = np.random.default_rng(4)
     m = 2000
     p = np.concatenate([rng.uniform([-1, 0, -1], [1, 2, 1], (m - 200, 3)),
-                        tree.lights["center"][rng.integers(0, 50, 200)] - [0, 0.02, 0]]).astype(np.float32)
+                        tree.lights["center"][rng.integers(0, len(tree.lights), 200)] - [0, 0.02, 0]]).astype(np.float32)
     nrm = rng.normal(size=(m, 3))
     nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
     nrm = nrm.astype(np.float32)
