@@ -131,6 +131,23 @@ def room_render(frames: int, cpu: bool):
         "grays_all_rays": round(rays / wall / 1e9, 4),
         "rays_per_sample": round(rays / paths, 3),
     }
+    # the same frames through the filtered sensor (Gaussian, radius 2, alpha
+    # 1.8: the take default), on the 260 x 260 grid it samples
+    sens = render.sensor()
+    acc4 = torch.zeros((256, 256, 4), dtype=torch.float64, device="cuda")
+    render.render_filtered(sc, cam, 16, sens, samples_per_wave=16, seed=1, accum=acc4)
+    torch.cuda.synchronize()
+    paths = 0.0
+    t0 = time.perf_counter()
+    for f in range(frames):
+        _, st = render.render_filtered(sc, cam, 16, sens, samples_per_wave=16, seed=1,
+                                       first_sample=16 * (f + 1), accum=acc4, stats=True)
+        paths += st.paths
+    torch.cuda.synchronize()
+    fwall = time.perf_counter() - t0
+    out["filtered_ms_per_frame"] = round(fwall / frames * 1e3, 3)
+    out["filtered_samples_per_s"] = round(paths / fwall, 1)
+    out["filtered_sensor"] = "Gaussian radius 2 alpha 1.8; 260x260 camera grid for the 256x256 image"
     sc.release()
     if cpu:
         from oracle import oracle as O

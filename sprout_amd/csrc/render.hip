@@ -714,6 +714,14 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             path.flags        = linear ? (path.flags & ~Flag_nearest) : (path.flags | Flag_nearest);
         }
 
+        // the guide's mixture at a guided vertex, for the MIS weights of its
+        // shadow rays and its direction sample
+        bool const       guided_vertex = Guided && live && !in_medium && depth < w.guide_depth;
+        dev::Vmf_mixture mix;
+        if (guided_vertex) {
+            mix.set(w.guide_raw + 32 * size_t(i));
+        }
+
         // one shadow ray per picked light (evaluate_light); a vertex's rays
         // take consecutive queue slots, reserved with one atomic per wave
         uint32_t const sbase = queue_range(num_picks, &w.counters[C_shadow]);
@@ -757,9 +765,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                     s.evaluate(wi, f);
                     float const light_pdf = (sl / (cl * L.area)) * (tp.pdf * (L.area / T.area)) * pick.pdf;
                     float       dir_pdf   = f[3];
-                    if (Guided && !in_medium && depth < w.guide_depth) {
-                        dev::Vmf_mixture mix;
-                        mix.set(w.guide_raw + 32 * size_t(i));
+                    if (guided_vertex) {
                         dir_pdf = w.fraction * mix.pdf(wi) + (1.f - w.fraction) * f[3];
                     }
                     float const weight = predivided_power_heuristic(light_pdf, dir_pdf);
@@ -798,9 +804,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             // BSDF importance sample; guided: one-sample mixture of the guide
             // (probability fraction) and the BSDF
             dev::Bxdf_sample bs;
-            if (Guided && !in_medium && depth < w.guide_depth) {
-                dev::Vmf_mixture mix;
-                mix.set(w.guide_raw + 32 * size_t(i));
+            if (guided_vertex) {
                 float const pick = path.rng.uniform();
                 float const xl   = path.rng.uniform();
                 float const x0   = path.rng.uniform();
