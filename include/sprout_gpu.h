@@ -158,8 +158,8 @@ typedef struct mi_trace_config {
                                   3: as 2 with stack entries that carry the child's node word
                                   and box entry distance (default; trees of < 2^22 nodes and
                                   slots, else 2) */
-    uint32_t refill_lanes;     /* kernels 0, 2: hand out rays once this many lanes are idle (24) */
-    uint32_t claim_batch;      /* kernels 0, 2: rays a wave claims per atomic (64) */
+    uint32_t refill_lanes;     /* kernels 0, 2, 3: hand out rays once this many lanes are idle (8) */
+    uint32_t claim_batch;      /* kernels 0, 2, 3: rays a wave claims per atomic (64) */
     uint32_t lds_entries;      /* stack entries per lane kept in LDS, <= 32 (default 10;
                                   0: up to 32); deeper entries spill to global memory */
 } mi_trace_config;

@@ -36,7 +36,7 @@ struct TraceResources {
     uint32_t  num_cus       = 256;
     uint32_t  kernel        = 3;     // 0 refill, 1 static grid-stride, 2 refill + while-while,
                                      // 3 as 2 with two-word stack entries (small enough trees)
-    uint32_t  refill_lanes  = 24;    // hand out rays once this many lanes of a wave are idle
+    uint32_t  refill_lanes  = 8;     // hand out rays once this many lanes of a wave are idle
     uint32_t  claim_batch   = 64;    // rays a wave claims from the global counter at a time
     uint32_t  lds_entries   = 10;    // stack entries kept in LDS (0: up to 32); deeper ones spill
     uint32_t* counter       = nullptr;  // ray counter of trace_refill (one word)
