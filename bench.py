@@ -499,6 +499,10 @@ def main() -> int:
     d_bounce = torch.from_numpy(bounce).cuda()
     n = len(bounce)
 
+    log("autotuning the traversal's launch shape")
+    shape, tuned_ms = bvh.autotune(tree, d_bounce, 5)
+    log(f"  {shape}: {tuned_ms:.3f} ms")
+
     hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
     occ = torch.empty((n,), dtype=torch.int32, device="cuda")
     hits_cam = torch.empty((n, 4), dtype=torch.int32, device="cuda")
@@ -552,6 +556,7 @@ def main() -> int:
             "tree_nodes": int(info.num_nodes),
             "tree_depth": int(info.max_depth),
             "parallelism": "single GPU",
+            "launch_shape": shape,
         },
         "roofline": {
             "bound": "hbm",

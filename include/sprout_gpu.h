@@ -166,6 +166,18 @@ typedef struct mi_trace_config {
 
 int32_t mi_set_trace_config(mi_trace_config const* config);
 
+/* Autotuner of the traversal's launch shape: times closest-hit traversals of
+ * the given DEVICE rays through the tree over candidate shapes (kernel 3;
+ * LDS stack entries, refill threshold, claim batch and workgroups per CU
+ * around the defaults), `repeats` timed launches each after one warm-up,
+ * with HIP events on `stream`; makes the fastest the context's shape (as
+ * mi_set_trace_config) and returns it in *best (HOST, may be NULL) and its
+ * mean launch time in *best_ms.  Every shape gives the same results, so
+ * tuning changes only the speed.  Waits for the stream.  No counterpart in
+ * sprout (its traversal has no launch shape). */
+int32_t mi_trace_autotune(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, uint32_t repeats,
+                          void* stream, mi_trace_config* best, float* best_ms);
+
 /* Waits for the stream and reports errors of earlier calls on it.
  * Traversals share the context's ray counter and stack spill area: calls on
  * different streams must not run at the same time. */

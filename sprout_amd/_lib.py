@@ -134,6 +134,8 @@ SIGNATURES = {
     "mi_trace_any": (_i32, [_u32, _vp, _u32, _vp, _vp]),
     "mi_trace_counts": (_i32, [_u32, _vp, _u32, _i32, _vp, _vp, _vp]),
     "mi_set_trace_config": (_i32, [ctypes.POINTER(MiTraceConfig)]),
+    "mi_trace_autotune": (_i32, [_u32, _vp, _u32, _u32, _vp, ctypes.POINTER(MiTraceConfig),
+                                 ctypes.POINTER(ctypes.c_float)]),
     "mi_sync": (_i32, [_vp]),
     "mi_gen_displaced_icosphere": (_i32, [_u32, _u32, _f32, _u64, _u32, _f32, _f32, _u64, _vp,
                                           _pu32, _vp, _pu32]),

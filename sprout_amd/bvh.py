@@ -163,5 +163,19 @@ def set_trace_config(block_size: int = 0, blocks_per_cu: int = 0, kernel: int = 
     check(_lib.load().mi_set_trace_config(ctypes.byref(cfg)), "mi_set_trace_config")
 
 
+def autotune(tree, rays, repeats: int = 5):
+    """mi_trace_autotune over device rays (n, 8) float32: makes the fastest
+    launch shape the context's; returns (shape as a dict, mean ms)."""
+    import torch
+
+    r = rays if isinstance(rays, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(rays, np.float32)).cuda()
+    cfg = _lib.MiTraceConfig()
+    ms = ctypes.c_float(0)
+    check(_lib.load().mi_trace_autotune(tree.handle, ctypes.c_void_p(r.data_ptr()), r.shape[0], repeats,
+                                        _stream(None), ctypes.byref(cfg), ctypes.byref(ms)),
+          "mi_trace_autotune")
+    return {f: getattr(cfg, f) for f, _ in cfg._fields_}, ms.value
+
+
 def sync(stream=None) -> None:
     check(_lib.load().mi_sync(_stream(stream)), "mi_sync")

@@ -370,6 +370,92 @@ int32_t mi_set_trace_config(mi_trace_config const* config) {
     return MI_OK;
 }
 
+int32_t mi_trace_autotune(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, uint32_t repeats,
+                          void* stream, mi_trace_config* best, float* best_ms) {
+    clear_error();
+    DeviceBvh const* b = lookup(bvh);
+    if (!b) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT, "mi_trace_autotune: bad handle %u",
+                    bvh);
+    }
+    if (0 == num_rays || !rays || 0 == repeats || repeats > 1000) {
+        return fail(MI_ERR_ARGS, "mi_trace_autotune: bad arguments");
+    }
+    hipStream_t const s    = as_stream(stream);
+    mi_hit*           hits = nullptr;
+    hipEvent_t        e0 = nullptr, e1 = nullptr;
+    hipError_t        e  = hipMalloc(&hits, size_t(num_rays) * sizeof(mi_hit));
+    if (!e) e = hipEventCreate(&e0);
+    if (!e) e = hipEventCreate(&e1);
+    if (e) {
+        (void)hipFree(hits);
+        return fail(MI_ERR_HIP, "mi_trace_autotune: %s", hipGetErrorString(e));
+    }
+    // candidate shapes around the measured defaults (block 256, claim 64)
+    struct Shape {
+        uint32_t lds, lanes, batch, per_cu;
+    };
+    static Shape const shapes[] = {{10, 8, 64, 8},  {8, 8, 64, 8},   {12, 8, 64, 8},  {10, 4, 64, 8},
+                                   {10, 16, 64, 8}, {10, 24, 64, 8}, {10, 8, 32, 8},  {10, 8, 128, 8},
+                                   {10, 8, 64, 6},  {10, 8, 64, 10}, {16, 8, 64, 5},  {9, 8, 64, 8}};
+    // only the shape fields change; the counter and spill buffers stay the
+    // context's own (a launch may grow the spill area)
+    auto set_shape = [](TraceResources& t, Shape const& sh, uint32_t kernel, uint32_t block) {
+        t.kernel        = kernel;
+        t.lds_entries   = sh.lds;
+        t.refill_lanes  = sh.lanes;
+        t.claim_batch   = sh.batch;
+        t.blocks_per_cu = sh.per_cu;
+        t.block_size    = block;
+    };
+    Shape const    old_shape{context->trace.lds_entries, context->trace.refill_lanes,
+                          context->trace.claim_batch, context->trace.blocks_per_cu};
+    uint32_t const old_kernel = context->trace.kernel;
+    uint32_t const old_block  = context->trace.block_size;
+    float          best_t     = 0.f;
+    Shape          best_shape = shapes[0];
+    int32_t        st         = MI_OK;
+    for (Shape const& sh : shapes) {
+        TraceResources& t = context->trace;
+        set_shape(t, sh, 3, 256);
+        st = launch_trace(*b, t, rays, num_rays, hits, nullptr, false, s);  // warm-up
+        if (!st) e = hipEventRecord(e0, s);
+        for (uint32_t k = 0; k < repeats && !st && !e; ++k) {
+            st = launch_trace(*b, t, rays, num_rays, hits, nullptr, false, s);
+        }
+        if (!st && !e) e = hipEventRecord(e1, s);
+        if (!st && !e) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (!st && !e) e = hipEventElapsedTime(&ms, e0, e1);
+        if (st || e) {
+            break;
+        }
+        ms /= float(repeats);
+        if (0.f == best_t || ms < best_t) {
+            best_t     = ms;
+            best_shape = sh;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(hits);
+    if (st || e) {
+        set_shape(context->trace, old_shape, old_kernel, old_block);
+        return st ? st : fail(MI_ERR_HIP, "mi_trace_autotune: %s", hipGetErrorString(e));
+    }
+    TraceResources& t = context->trace;
+    set_shape(t, best_shape, 3, 256);
+    if (best) {
+        *best = mi_trace_config{t.block_size, t.blocks_per_cu, t.kernel, t.refill_lanes,
+                                t.claim_batch, t.lds_entries};
+    }
+    if (best_ms) {
+        *best_ms = best_t;
+    }
+    return MI_OK;
+}
+
 int32_t mi_sync(void* stream) {
     clear_error();
     if (!context) {

@@ -235,3 +235,19 @@ def test_upload_rejects_malformed_trees(gpu):
         gpu.upload(bad, np.zeros(1, np.uint32), pos, idx)
     with pytest.raises(MiError):
         gpu.upload(nodes, np.array([7], np.uint32), pos, idx)  # triangle out of range
+
+
+def test_autotune_picks_a_shape_that_keeps_the_results(gpu, medium):
+    from sprout_amd import bvh
+
+    pos, idx, otree, tree, rays = medium
+    log("autotune on the medium mesh")
+    try:
+        shape, ms = bvh.autotune(tree, rays, 2)
+        assert ms > 0
+        assert shape["kernel"] == 3 and shape["block_size"] == 256
+        assert 1 <= shape["lds_entries"] <= 16 and 4 <= shape["refill_lanes"] <= 24
+        assert np.array_equal(run_closest(gpu, tree, rays), O.trace(otree, pos, idx, rays))
+        assert np.array_equal(run_any(gpu, tree, rays), O.trace(otree, pos, idx, rays, any_hit=True))
+    finally:
+        gpu.set_trace_config()
