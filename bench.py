@@ -341,6 +341,44 @@ SLIT_REF = os.path.join(ROOT, "tests", "golden", "slit_room_ref_1024.npz")
 SLIT_REF_256 = os.path.join(ROOT, "tests", "golden", "slit_room_ref_256.npz")
 
 
+def guide_kernels():
+    """The guide's BF16 MFMA kernels on device-resident batches: forward of
+    1 M queries and the loss gradient of 256 k records, mean kernel time from
+    HIP events over 20 launches each.  MFMA work per sample: forward
+    2 (224*64 + 64*64 + 64*64 + 64*32) = 49 152 flops; gradient = forward +
+    back-propagation through three layers + the four weight gradients
+    = 3 * 49 152 - 2 * 224 * 64 = 118 784 flops (the first layer's input
+    gradient is not formed).  Utilisation = achieved / 2.5 PFLOP/s (dense
+    BF16 on MI355X)."""
+    import torch
+
+    from sprout_amd import guide
+
+    g = guide.Guide(5)
+    n_fwd, n_grad = 1 << 20, 1 << 18
+    x = torch.rand((n_fwd, 9), dtype=torch.float32, device="cuda")
+    d = torch.nn.functional.normalize(torch.randn((n_grad, 3), device="cuda"), dim=1)
+    w = torch.rand((n_grad, 1), dtype=torch.float32, device="cuda")
+    xg = x[:n_grad].contiguous()
+    _, ms_f = timed_launches(lambda: g.forward(x), 20, 3)
+    _, ms_g = timed_launches(lambda: g.gradients(xg, d, w), 20, 3)
+    g.release()
+    flops_f = 2 * (224 * 64 + 64 * 64 + 64 * 64 + 64 * 32)
+    flops_g = 3 * flops_f - 2 * 224 * 64
+    tf_f = n_fwd * flops_f / (ms_f * 1e-3) / 1e12
+    tf_g = n_grad * flops_g / (ms_g * 1e-3) / 1e12
+    return {
+        "forward_ms_per_1M_queries": round(ms_f, 4),
+        "forward_tflops": round(tf_f, 1),
+        "forward_mfma_utilisation": round(tf_f / 2500.0, 4),
+        "gradient_ms_per_256k_records": round(ms_g, 4),
+        "gradient_tflops": round(tf_g, 1),
+        "gradient_mfma_utilisation": round(tf_g / 2500.0, 4),
+        "note": "utilisation = MFMA flops of the layers / kernel time / 2.5 PFLOP/s dense BF16; "
+                "the kernel time also holds the sin/cos features, the vMF terms and LDS staging",
+    }
+
+
 def rel_mse(x, r, eps=1e-2):
     """mean over pixels and channels of (x - r)^2 / (r^2 + eps), eps = 1e-2"""
     return float(np.mean((x - r) ** 2 / (r * r + eps)))
@@ -434,7 +472,7 @@ def main() -> int:
     ap.add_argument("--guide-seconds", type=float, default=6.0,
                     help="time budget of each side of the guided vs plain comparison")
     ap.add_argument("--part", choices=["all", "room", "volume", "guiding", "instances", "batch",
-                                       "batch_instanced"],
+                                       "batch_instanced", "guide_kernels"],
                     default="all",
                     help="run one workload only and print its figures (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -458,6 +496,7 @@ def main() -> int:
         parts = {"room": lambda: room_render(5, False), "volume": lambda: cloud_render(3, False),
                  "batch": lambda: batch_render(2, False),
                  "batch_instanced": lambda: batch_render(2, True),
+                 "guide_kernels": guide_kernels,
                  "guiding": lambda: slit_equal_time(args.guide_seconds),
                  "instances": lambda: instance_trace(args.steps, args.warmup)}
         print(json.dumps({"part": args.part, "figures": parts[args.part]()}), flush=True)
@@ -590,6 +629,8 @@ def main() -> int:
     line["extra"]["instances"] = instance_trace(args.steps, args.warmup, not args.no_cpu_baseline)
     log("fBm cloud path tracing")
     line["extra"]["volume_render"] = cloud_render(3, not args.no_cpu_baseline)
+    log("guide kernels")
+    line["extra"]["guide_kernels"] = guide_kernels()
     log("slit room: guided vs plain at equal time")
     line["extra"]["guiding"] = slit_equal_time(args.guide_seconds)
     print(json.dumps(line), flush=True)

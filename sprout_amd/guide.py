@@ -30,6 +30,10 @@ def _dev(a, cols=None) -> torch.Tensor:
     return t
 
 
+def _column(w):
+    return w.reshape(-1, 1) if isinstance(w, torch.Tensor) else np.asarray(w).reshape(-1, 1)
+
+
 class Guide:
     def __init__(self, seed: int = 1):
         h = ctypes.c_uint32(_lib.MI_NULL)
@@ -67,7 +71,7 @@ class Guide:
         return out
 
     def gradients(self, x, dirs, weights) -> torch.Tensor:
-        xd, dd, wd = _dev(x, 9), _dev(dirs, 3), _dev(np.asarray(weights).reshape(-1, 1), 1)
+        xd, dd, wd = _dev(x, 9), _dev(dirs, 3), _dev(_column(weights), 1)
         g = torch.empty(NUM_PARAMS, dtype=torch.float32, device="cuda")
         check(_lib.load().mi_guide_gradients(self.handle, _ptr(xd), _ptr(dd), _ptr(wd),
                                              xd.shape[0], _ptr(g), _stream()),
@@ -75,9 +79,7 @@ class Guide:
         return g
 
     def train_step(self, x, dirs, weights, lr: float = 1e-3) -> None:
-        xd, dd, wd = _dev(x, 9), _dev(dirs, 3), _dev(np.asarray(weights).reshape(-1, 1)
-                                                     if not isinstance(weights, torch.Tensor)
-                                                     else weights.reshape(-1, 1), 1)
+        xd, dd, wd = _dev(x, 9), _dev(dirs, 3), _dev(_column(weights), 1)
         check(_lib.load().mi_guide_train_step(self.handle, _ptr(xd), _ptr(dd), _ptr(wd),
                                               xd.shape[0], lr, _stream()), "mi_guide_train_step")
 
