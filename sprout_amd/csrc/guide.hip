@@ -9,9 +9,10 @@
 // 8-byte read; activations as bf16 [row][feature] so an A operand is one
 // 8-byte read.
 //
-// forward   a wave computes 16 queries: features (sinpi/cospi of 2^j x), then
-//           four layers through LDS; weights are loaded once per workgroup
-//           and the grid strides over query tiles.
+// forward   a wave computes 16 queries: the first layer takes its features
+//           (sinpi / cospi of f x) straight from registers, two hidden layers
+//           pass through LDS, the output layer stores to memory; weights
+//           are loaded once per workgroup and the grid strides over tiles.
 // gradient  a workgroup of 4 waves takes 64 samples at a time: each wave runs
 //           forward and backward for its 16 (activations and pre-activation
 //           gradients kept in LDS), then the 4 waves split the 96 16x16 tiles
@@ -26,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
@@ -133,87 +135,161 @@ __device__ void features(float const* x, float const* freq, uint32_t row0, uint3
     }
 }
 
-// out[16][N] = act(A[16][K] W + b): A bf16 in LDS (stride sa), W^T in LDS
-// (stride sw); result bf16 to H (stride sh) or, for the last layer, fp32.
-template <int K, int N, bool Relu>
-__device__ void layer(uint16_t const* A, int sa, uint16_t const* Wt, int sw, float const* bias,
-                      uint16_t* H, int sh, float* F, int sf) {
-    int const lane = threadIdx.x & 63;
-    int const r    = lane & 15;
-    int const q    = lane >> 4;
+// Epilogue of a layer: acc[nt][t] is output 16 nt + 4q + t of row r; adds
+// the bias and writes, per Mode, ReLU'd bf16 to H (one 8-byte store per
+// tile), fp32 to F (Mode_f32), or fp32 rows r < rows to global F (Mode_out;
+// float4 stores where F is 16-byte aligned, sf a multiple of 4).
+enum { Mode_relu = 0, Mode_f32 = 1, Mode_out = 2 };
+
+template <int N, int Mode>
+__device__ __forceinline__ void store_layer(f32x4 const (&acc)[N / 16], float const* bias, int r,
+                                            int q, uint16_t* H, int sh, float* F, int sf,
+                                            int rows) {
 #pragma unroll
     for (int nt = 0; nt < N / 16; ++nt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < K / 16; ++ks) {
-            bf16x4 const a = load4(A + r * sa + ks * 16 + 4 * q);
-            bf16x4 const b = load4(Wt + (nt * 16 + r) * sw + ks * 16 + 4 * q);
-            acc            = mfma(a, b, acc);
-        }
-        int const j = nt * 16 + r;
+        int const j0 = nt * 16 + 4 * q;
+        float     v[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            int const i = 4 * q + t;
-            float     v = acc[t] + bias[j];
-            if (Relu) {
-                v                = fmaxf(v, 0.f);
-                H[i * sh + j]    = to_bf16(v);
+            v[t] = acc[nt][t] + bias[j0 + t];
+        }
+        if (Mode_relu == Mode) {
+            uint2 pk;
+            pk.x = uint32_t(to_bf16(fmaxf(v[0], 0.f))) | (uint32_t(to_bf16(fmaxf(v[1], 0.f))) << 16);
+            pk.y = uint32_t(to_bf16(fmaxf(v[2], 0.f))) | (uint32_t(to_bf16(fmaxf(v[3], 0.f))) << 16);
+            *reinterpret_cast<uint2*>(H + r * sh + j0) = pk;
+        } else if (Mode_f32 == Mode) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                F[r * sf + j0 + t] = v[t];
+            }
+        } else if (r < rows) {
+            float* const o = F + size_t(r) * sf + j0;
+            if (0 == (reinterpret_cast<uintptr_t>(o) & 15u)) {
+                *reinterpret_cast<float4*>(o) = make_float4(v[0], v[1], v[2], v[3]);
             } else {
-                F[i * sf + j] = v;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    o[t] = v[t];
+                }
             }
         }
     }
 }
 
-constexpr int Fwd_waves = 8;
+// out[16][N] = act(A[16][K] W + b), formed as its transpose W^T A^T so that a
+// lane's four results are four consecutive outputs of one row: MFMA A
+// operand = rows of W^T in LDS (stride sw), B operand = activation rows
+// (bf16 in LDS, stride sa), D[4q + t][r] = output 16 nt + 4q + t of row r.
+// The depth steps run outermost, so each activation operand is read once and
+// the N / 16 accumulation chains interleave.
+template <int K, int N, int Mode>
+__device__ void layer(uint16_t const* A, int sa, uint16_t const* Wt, int sw, float const* bias,
+                      uint16_t* H, int sh, float* F, int sf, int rows = 16) {
+    int const lane = threadIdx.x & 63;
+    int const r    = lane & 15;
+    int const q    = lane >> 4;
+    f32x4     acc[N / 16];
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt) {
+        acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int ks = 0; ks < K / 16; ++ks) {
+        bf16x4 const b = load4(A + r * sa + ks * 16 + 4 * q);
+#pragma unroll
+        for (int nt = 0; nt < N / 16; ++nt) {
+            acc[nt] = mfma(load4(Wt + (nt * 16 + r) * sw + ks * 16 + 4 * q), b, acc[nt]);
+        }
+    }
+    store_layer<N, Mode>(acc, bias, r, q, H, sh, F, sf, rows);
+}
 
-struct Fwd_lds {
+// First layer of the forward pass with its features formed in registers: the
+// B operand of lane (r, q) at depth step ks is features k0 .. k0 + 3, k0 =
+// 16 ks + 4q, of query row0 + r, i.e. sin, cos, sin, cos of pi f x_i for
+// i = k0 / 24 and the multipliers j0, j0 + 1 with j0 = (k0 % 24) / 2 (zero
+// past the 216 features and for rows past n): the values features() writes.
+__device__ void layer1_direct(float const* x, float const* freq, uint32_t row0, uint32_t n,
+                              uint16_t const* Wt, float const* bias, uint16_t* H) {
+    int const      lane = threadIdx.x & 63;
+    int const      r    = lane & 15;
+    int const      q    = lane >> 4;
+    uint32_t const row  = row0 + r;
+    bool const     live = row < n;
+    f32x4          acc[Hidden / 16];
+#pragma unroll
+    for (int nt = 0; nt < Hidden / 16; ++nt) {
+        acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int ks = 0; ks < Feat_pad / 16; ++ks) {
+        int const k0 = ks * 16 + 4 * q;
+        bf16x4    b  = {0, 0, 0, 0};
+        if (k0 < Num_feat) {
+            int const   i  = k0 / 24;
+            int const   j0 = (k0 % 24) / 2;
+            float const v  = live ? x[size_t(row) * Num_in + i] : 0.f;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                float const f   = freq[i * Num_freq + j0 + u] * v;
+                float const h   = 0.5f * f;
+                float const rev = h - floorf(h);
+                float const sn  = live ? __builtin_amdgcn_sinf(rev) : 0.f;
+                float const cs  = live ? __builtin_amdgcn_cosf(rev) : 0.f;
+                b[2 * u]        = short(to_bf16(sn));
+                b[2 * u + 1]    = short(to_bf16(cs));
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < Hidden / 16; ++nt) {
+            acc[nt] = mfma(load4(Wt + (nt * 16 + r) * S_feat + k0), b, acc[nt]);
+        }
+    }
+    store_layer<Hidden, Mode_relu>(acc, bias, r, q, H, S_hid, nullptr, 0, 16);
+}
+
+constexpr int Fwd_waves = 16;
+
+struct alignas(16) Fwd_lds {
     Weights  w;
-    uint16_t x[Fwd_waves][16 * S_feat];  // features, then the fp32 outputs of the tile
+    float    freq[Num_in * Num_freq];
     uint16_t h[Fwd_waves][2][16 * S_hid];
 };
-static_assert(16 * S_out * sizeof(float) <= 16 * S_feat * sizeof(uint16_t),
-              "a tile's outputs fit in the space of its features");
 static_assert(sizeof(Fwd_lds) <= 160 * 1024, "LDS of a workgroup");
+static_assert(offsetof(Fwd_lds, h) % 16 == 0 && (S_hid * sizeof(uint16_t)) % 8 == 0,
+              "8-byte aligned activation rows");
 
+// A wave computes 16 queries per tile: layer 1 from register features, two
+// hidden layers through LDS, the output layer stored straight to out.
 __global__ void __launch_bounds__(Fwd_waves * 64)
     forward_kernel(uint16_t const* wt, float const* params, float const* freq, float const* x,
                    uint32_t n_host, uint32_t const* n_dev, float* out) {
     __shared__ Fwd_lds s;
     uint32_t const n = n_dev ? min(*n_dev, n_host) : n_host;
     load_weights(s.w, wt, params);
+    for (int i = threadIdx.x; i < Num_in * Num_freq; i += blockDim.x) {
+        s.freq[i] = freq[i];
+    }
     __syncthreads();
 
     int const wave  = threadIdx.x >> 6;
-    int const lane  = threadIdx.x & 63;
-    uint16_t* X     = s.x[wave];
     uint16_t* H0    = s.h[wave][0];
     uint16_t* H1    = s.h[wave][1];
-    float*    Y     = reinterpret_cast<float*>(X);
     float const* bb = s.w.b;
 
     for (uint32_t tile = blockIdx.x * Fwd_waves + wave; uint64_t(tile) * 16 < n;
          tile += gridDim.x * Fwd_waves) {
         uint32_t const row0 = tile * 16;
-        features(x, freq, row0, n, X);
+        layer1_direct(x, s.freq, row0, n, s.w.w1, bb, H0);
         wave_sync();
-        layer<Feat_pad, Hidden, true>(X, S_feat, s.w.w1, S_feat, bb, H0, S_hid, nullptr, 0);
+        layer<Hidden, Hidden, Mode_relu>(H0, S_hid, s.w.w2, S_hid, bb + Hidden, H1, S_hid, nullptr, 0);
         wave_sync();
-        layer<Hidden, Hidden, true>(H0, S_hid, s.w.w2, S_hid, bb + Hidden, H1, S_hid, nullptr, 0);
+        layer<Hidden, Hidden, Mode_relu>(H1, S_hid, s.w.w3, S_hid, bb + 2 * Hidden, H0, S_hid, nullptr, 0);
         wave_sync();
-        layer<Hidden, Hidden, true>(H1, S_hid, s.w.w3, S_hid, bb + 2 * Hidden, H0, S_hid, nullptr, 0);
-        wave_sync();
-        layer<Hidden, Num_out, false>(H0, S_hid, s.w.w4, S_hid, bb + 3 * Hidden, nullptr, 0, Y, S_out);
-        wave_sync();
-        // coalesced store of the 16 x 32 outputs
-        for (int e = lane; e < 16 * Num_out; e += 64) {
-            int const      m   = e / Num_out;
-            int const      j   = e % Num_out;
-            uint32_t const row = row0 + m;
-            if (row < n) {
-                out[size_t(row) * Num_out + j] = Y[m * S_out + j];
-            }
-        }
+        layer<Hidden, Num_out, Mode_out>(H0, S_hid, s.w.w4, S_hid, bb + 3 * Hidden, nullptr, 0,
+                                         out + size_t(row0) * Num_out, Num_out,
+                                         int(min(n - row0, 16u)));
         wave_sync();
     }
 }
@@ -226,7 +302,7 @@ constexpr int Grad_tiles = (Feat_pad / 16) * (Hidden / 16) + 2 * (Hidden / 16) *
                            (Hidden / 16) * (Num_out / 16);  // 56 + 32 + 8 = 96
 constexpr int Tiles_per_wave = Grad_tiles / Grad_waves;      // 24
 
-struct Grad_lds {
+struct alignas(16) Grad_lds {
     Weights  w;
     uint16_t x[Chunk * S_feat];
     uint16_t h[3][Chunk * S_hid];   // post-ReLU activations of the hidden layers
@@ -378,13 +454,13 @@ __global__ void __launch_bounds__(Grad_waves * 64)
 
         features(x, freq, row0, n, X);
         wave_sync();
-        layer<Feat_pad, Hidden, true>(X, S_feat, s.w.w1, S_feat, s.w.b, H1, S_hid, nullptr, 0);
+        layer<Feat_pad, Hidden, Mode_relu>(X, S_feat, s.w.w1, S_feat, s.w.b, H1, S_hid, nullptr, 0);
         wave_sync();
-        layer<Hidden, Hidden, true>(H1, S_hid, s.w.w2, S_hid, s.w.b + Hidden, H2, S_hid, nullptr, 0);
+        layer<Hidden, Hidden, Mode_relu>(H1, S_hid, s.w.w2, S_hid, s.w.b + Hidden, H2, S_hid, nullptr, 0);
         wave_sync();
-        layer<Hidden, Hidden, true>(H2, S_hid, s.w.w3, S_hid, s.w.b + 2 * Hidden, H3, S_hid, nullptr, 0);
+        layer<Hidden, Hidden, Mode_relu>(H2, S_hid, s.w.w3, S_hid, s.w.b + 2 * Hidden, H3, S_hid, nullptr, 0);
         wave_sync();
-        layer<Hidden, Num_out, false>(H3, S_hid, s.w.w4, S_hid, s.w.b + 3 * Hidden, nullptr, 0, Y, S_out);
+        layer<Hidden, Num_out, Mode_f32>(H3, S_hid, s.w.w4, S_hid, s.w.b + 3 * Hidden, nullptr, 0, Y, S_out);
         wave_sync();
 
         // output gradients: lane handles sample r, lobes q and q + 4
@@ -493,21 +569,42 @@ __global__ void __launch_bounds__(Grad_waves * 64)
     }
 }
 
-// grad = (sum of the workgroups' partials) / (sum of the batch weights),
-// both summed in workgroup order; the gradient of -(t / sum t) log q.
-__global__ void reduce_partials(float const* partial, float const* wsum, uint32_t blocks,
-                                float* grad) {
-    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= uint32_t(Num_params)) {
-        return;
+// grad = (sum of the workgroups' partials) / (sum of the batch weights).  A
+// workgroup covers 64 parameters: wave w sums the partials of workgroups
+// w, w + 8, w + 16, ... in order, then the eight sums are added in wave
+// order (the weights alike), so gradients are deterministic.
+constexpr int Reduce_waves = 8;
+
+__global__ void __launch_bounds__(Reduce_waves * 64)
+    reduce_partials(float const* partial, float const* wsum, uint32_t blocks, float* grad) {
+    __shared__ float sum_s[Reduce_waves][64];
+    __shared__ float sum_t[Reduce_waves];
+    uint32_t const lane = threadIdx.x & 63;
+    uint32_t const wave = threadIdx.x >> 6;
+    uint32_t const p    = blockIdx.x * 64 + lane;
+    float          s    = 0.f;
+    float          t    = 0.f;
+    if (p < uint32_t(Num_params)) {
+        for (uint32_t b = wave; b < blocks; b += Reduce_waves) {
+            s += partial[size_t(b) * Num_params + p];
+            t += wsum[b];
+        }
     }
-    float s = 0.f;
-    float t = 0.f;
-    for (uint32_t b = 0; b < blocks; ++b) {
-        s += partial[size_t(b) * Num_params + p];
-        t += wsum[b];
+    sum_s[wave][lane] = s;
+    if (0 == lane) {
+        sum_t[wave] = t;
     }
-    grad[p] = s / (t > 0.f ? t : 1.f);
+    __syncthreads();
+    if (0 == wave && p < uint32_t(Num_params)) {
+        float ss = 0.f;
+        float tt = 0.f;
+#pragma unroll
+        for (int w = 0; w < Reduce_waves; ++w) {
+            ss += sum_s[w][lane];
+            tt += sum_t[w];
+        }
+        grad[p] = ss / (tt > 0.f ? tt : 1.f);
+    }
 }
 
 // W^T bf16 position of parameter p, or -1 for a bias
@@ -760,8 +857,8 @@ void enqueue_gradients(GuideNet& net, float const* x, float const* dirs, float c
     hipLaunchKernelGGL(gradient_kernel, dim3(blocks), dim3(Grad_waves * 64), 0, stream, net.wt,
                        net.params, net.freq, x, dirs, weights, 0u,
                        static_cast<uint32_t const*>(net.window), net.partial, net.wsum);
-    hipLaunchKernelGGL(reduce_partials, dim3(blocks_for(Num_params, 256)), dim3(256), 0, stream,
-                       net.partial, net.wsum, blocks, net.grad);
+    hipLaunchKernelGGL(reduce_partials, dim3(blocks_for(Num_params, 64)), dim3(Reduce_waves * 64),
+                       0, stream, net.partial, net.wsum, blocks, net.grad);
 }
 
 void enqueue_adam(GuideNet& net, float lr, hipStream_t stream) {

@@ -73,7 +73,10 @@ def test_guided_converges_to_plain(gpu, fraction, train):
 
 
 def test_training_lowers_error_in_the_slit_room(gpu):
-    """Equal sample count after training: guided per-pixel error below plain."""
+    """Equal sample count after training: guided per-pixel error below plain.
+    The ratio of one 64 spp pair scatters between 0.81 and 1.05 (measured on
+    MI355X, 16 pairs over 4 trainings), its mean over four pairs between 0.88
+    and 0.91, so the mean of four pairs is held to 0.95."""
     import torch
 
     from sprout_amd import scenes
@@ -89,14 +92,17 @@ def test_training_lowers_error_in_the_slit_room(gpu):
     render.render_guided(sc, cam, 512, g, fraction=0.5, train=True, train_steps=8,
                          learning_rate=2e-3, samples_per_wave=16, seed=32)
     torch.cuda.synchronize()
-    log("equal-spp comparison, 64 spp")
-    plain, _ = render.render(sc, cam, 64, samples_per_wave=64, seed=33)
-    guided, _ = render.render_guided(sc, cam, 64, g, fraction=0.5, train=False,
-                                     samples_per_wave=64, seed=34)
-    e_plain = rel_mse(plain.cpu().numpy() / 64, ref)
-    e_guided = rel_mse(guided.cpu().numpy() / 64, ref)
-    log(f"relMSE plain {e_plain:.4g} guided {e_guided:.4g}")
-    assert e_guided < 0.9 * e_plain, (e_plain, e_guided)
+    log("equal-spp comparison, 4 x 64 spp")
+    e_plain, e_guided = [], []
+    for k in range(4):
+        plain, _ = render.render(sc, cam, 64, samples_per_wave=64, seed=33 + 2 * k)
+        guided, _ = render.render_guided(sc, cam, 64, g, fraction=0.5, train=False,
+                                         samples_per_wave=64, seed=34 + 2 * k)
+        e_plain.append(rel_mse(plain.cpu().numpy() / 64, ref))
+        e_guided.append(rel_mse(guided.cpu().numpy() / 64, ref))
+    ratio = float(np.mean(e_guided) / np.mean(e_plain))
+    log(f"relMSE plain {np.mean(e_plain):.4g} guided {np.mean(e_guided):.4g} ratio {ratio:.3f}")
+    assert ratio < 0.95, (e_plain, e_guided)
     g.release()
     sc.release()
 
