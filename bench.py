@@ -257,9 +257,25 @@ def batch_render(frames: int, instanced: bool):
         rays += st.extend_rays + st.shadow_rays
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    extra = {}
+    if instanced:
+        # an animated frame's top-level rebuild: every instance moved 0.01 up
+        # per step (mi_scene_update_instances, host matrices in, tree on the device)
+        o2w = b["object_to_world"].reshape(-1, 4, 4).astype(np.float64)
+        times = []
+        for k in range(1, 6):
+            m = o2w.copy()
+            m[:, 3, 1] += 0.01 * k
+            w2o, o2w_k = np.linalg.inv(m).astype(np.float32), m.astype(np.float32)
+            t1 = time.perf_counter()
+            sc.update_instances(w2o, o2w_k)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t1)
+        extra["top_level_rebuild_ms"] = round(float(np.median(times)) * 1e3, 3)
     sc.release()
     form = "instanced: 4 mesh trees under a top-level tree" if instanced else "baked, one tree"
     return {
+        **extra,
         "workload": f"64 instances x 20480-tri spheres ({form}, 1.32M tris), 10000 lights, "
                     "1920x1080, 4 spp per frame, RR from depth 3, max depth 16",
         "scene_setup_s": round(setup, 2),

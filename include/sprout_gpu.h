@@ -293,6 +293,19 @@ int32_t mi_scene_create_instanced(mi_mesh const* meshes, uint32_t num_meshes,
                                   uint32_t num_materials, mi_bvh_settings const* bvh_settings,
                                   uint32_t* scene);
 
+/* Moves the instances of an instanced scene: new world_to_object /
+ * object_to_world matrices (HOST, 16 floats each, one pair per instance in
+ * creation order, num_instances as at creation).  Rebuilds the top-level tree
+ * on the device from the instance boxes and updates the transforms the hit
+ * points use; meshes, their trees, materials and the light tree (its
+ * emitters are the world triangles) are kept.  A render afterwards equals
+ * one of a scene created with the new matrices.  Waits for work in flight;
+ * on failure the scene keeps its old placement.  Mirrors
+ * su_entity_set_transformation (capi.cpp:521-541) followed by the per-frame
+ * prop-tree rebuild of Scene::compile (scene.cpp:216-267). */
+int32_t mi_scene_update_instances(uint32_t scene, float const* world_to_object,
+                                  float const* object_to_world, uint32_t num_instances);
+
 int32_t mi_scene_release(uint32_t scene);
 
 /* Renders and ADDS each pixel's radiance sum over the samples into accum,

@@ -146,6 +146,7 @@ SIGNATURES = {
     "mi_scene_create_instanced": (_i32, [ctypes.POINTER(MiMesh), _u32, _vp, _vp, _vp, _vp, _u32,
                                          _vp, _u32, _vp, _u32, _vp, _vp, _u32,
                                          ctypes.POINTER(MiBvhSettings), _pu32]),
+    "mi_scene_update_instances": (_i32, [_u32, _vp, _vp, _u32]),
     "mi_scene_release": (_i32, [_u32]),
     "mi_render_filtered": (_i32, [_u32, ctypes.POINTER(MiCamera), ctypes.POINTER(MiRenderSettings),
                                   ctypes.POINTER(MiSensor), _vp, _vp, ctypes.POINTER(MiRenderStats)]),

@@ -547,6 +547,21 @@ int32_t mi_scene_create_instanced(mi_mesh const* meshes, uint32_t num_meshes,
     return MI_OK;
 }
 
+int32_t mi_scene_update_instances(uint32_t scene, float const* world_to_object,
+                                  float const* object_to_world, uint32_t num_instances) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT,
+                    "mi_scene_update_instances: bad handle %u", scene);
+    }
+    if (!world_to_object || !object_to_world || 0 == num_instances) {
+        return fail(MI_ERR_ARGS, "mi_scene_update_instances: bad arguments");
+    }
+    (void)hipDeviceSynchronize();  // renders in flight read the old top level
+    return update_instances(*sc, world_to_object, object_to_world, num_instances);
+}
+
 int32_t mi_scene_release(uint32_t scene) {
     clear_error();
     RenderScene* sc = lookup_scene(scene);

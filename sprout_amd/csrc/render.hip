@@ -1941,6 +1941,11 @@ int32_t create_scene_instanced(mi_mesh const* meshes, uint32_t num_meshes,
         w2o.push_back(id);
         o2w.push_back(id);
     }
+    for (int k = 0; k < 3; ++k) {
+        s.world_box_lo[k] = s.box_lo[k];
+        s.world_box_hi[k] = s.box_hi[k];
+    }
+    s.tlas_settings = settings;
     if (int32_t const st = create_tlas(trees.data(), num_meshes + 1, mesh_of.data(), w2o.data(),
                                        o2w.data(), n_all, settings, nullptr, s.tlas);
         st) {
@@ -1967,6 +1972,61 @@ int32_t create_scene_instanced(mi_mesh const* meshes, uint32_t num_meshes,
     s.instanced        = true;
     s.emitter_instance = num_instances;
     out                = s;
+    return MI_OK;
+}
+
+int32_t update_instances(RenderScene& s, float const* world_to_object, float const* object_to_world,
+                         uint32_t num_instances) {
+    if (!s.instanced) {
+        return fail(MI_ERR_ARGS, "mi_scene_update_instances: the scene has no instances");
+    }
+    uint32_t const n_all = s.tlas.num_instances;
+    if (num_instances + 1 != n_all) {
+        return fail(MI_ERR_ARGS, "mi_scene_update_instances: %u instances given, the scene has %u",
+                    num_instances, n_all - 1);
+    }
+    uint32_t const                num_meshes = uint32_t(s.mesh_trees.size());
+    std::vector<DeviceBvh const*> trees(num_meshes + 1);
+    for (uint32_t k = 0; k < num_meshes; ++k) {
+        trees[k] = &s.mesh_trees[k];
+    }
+    trees[num_meshes] = &s.bvh;
+    std::vector<uint32_t> mesh_of(n_all);
+    for (uint32_t i = 0; i < n_all; ++i) {
+        mesh_of[i] = s.tlas.records[i].mesh;
+    }
+    std::vector<float> w2o(world_to_object, world_to_object + 16 * size_t(num_instances));
+    std::vector<float> o2w(object_to_world, object_to_world + 16 * size_t(num_instances));
+    for (int k = 0; k < 16; ++k) {
+        float const id = (k % 5 == 0) ? 1.f : 0.f;
+        w2o.push_back(id);
+        o2w.push_back(id);
+    }
+    // build the new top level beside the old one, which stays in use on failure
+    Tlas fresh;
+    if (int32_t const st = create_tlas(trees.data(), num_meshes + 1, mesh_of.data(), w2o.data(),
+                                       o2w.data(), n_all, s.tlas_settings, nullptr, fresh);
+        st) {
+        return st;
+    }
+    if (hipError_t const e = hipMemcpy(s.inst_o2w, o2w.data(), o2w.size() * sizeof(float),
+                                       hipMemcpyHostToDevice);
+        e) {
+        fresh.release();
+        return fail(MI_ERR_HIP, "mi_scene_update_instances: %s", hipGetErrorString(e));
+    }
+    s.tlas.release();
+    s.tlas = std::move(fresh);
+    for (int k = 0; k < 3; ++k) {
+        s.box_lo[k] = s.world_box_lo[k];
+        s.box_hi[k] = s.world_box_hi[k];
+    }
+    for (Instance_record const& r : s.tlas.records) {
+        for (int k = 0; k < 3; ++k) {
+            s.box_lo[k] = std::min(s.box_lo[k], r.lo[k]);
+            s.box_hi[k] = std::max(s.box_hi[k], r.hi[k]);
+        }
+    }
     return MI_OK;
 }
 

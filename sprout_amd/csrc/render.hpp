@@ -86,6 +86,9 @@ struct RenderScene {
     uint32_t*              inst_material    = nullptr;  // per instance; Null for the world triangles
     uint32_t               emitter_instance = 0xFFFFFFFFu;
     bool                   instanced        = false;
+    mi_bvh_settings        tlas_settings{16, 64, 4};
+    float                  world_box_lo[3] = {0.f, 0.f, 0.f};  // bounds of the world triangles
+    float                  world_box_hi[3] = {0.f, 0.f, 0.f};
 
     uint32_t num_vertices  = 0;
     uint32_t num_triangles = 0;
@@ -111,6 +114,14 @@ int32_t create_scene_instanced(mi_mesh const* meshes, uint32_t num_meshes,
                                uint32_t num_triangles, uint32_t const* tri_material,
                                mi_material const* materials, uint32_t num_materials,
                                mi_bvh_settings const& settings, RenderScene& out);
+
+// New matrices for the instances of an instanced scene (host arrays, one
+// pair per instance, as at creation): rebuilds the top-level tree on the
+// device and updates the instance transforms; meshes, materials and the
+// light tree (whose emitters are world triangles) stay.  On failure the
+// scene keeps its old placement.  No render may be in flight.
+int32_t update_instances(RenderScene& scene, float const* world_to_object,
+                         float const* object_to_world, uint32_t num_instances);
 
 // Density grid (nx * ny * nz floats, x fastest, host memory) filling the box
 // [box_lo, box_hi]; cc_a / cc_s: absorption and scattering coefficients per

@@ -86,8 +86,19 @@ class InstancedScene(Scene):
               "mi_scene_create_instanced")
         del keep
         self.handle = handle.value
+        self.num_instances = len(mesh_of)
         self.num_triangles = len(idx) + sum(len(np.asarray(m[1]).reshape(-1, 3))
                                             for m in (meshes[j] for j in mesh_of))
+
+    def update_instances(self, world_to_object, object_to_world) -> None:
+        """mi_scene_update_instances: new matrices for every instance, in
+        creation order; the top-level tree is rebuilt on the device."""
+        w2o = np.ascontiguousarray(world_to_object, np.float32).reshape(-1, 16)
+        o2w = np.ascontiguousarray(object_to_world, np.float32).reshape(-1, 16)
+        if len(w2o) != len(o2w):
+            raise ValueError("two matrices per instance")
+        check(_lib.load().mi_scene_update_instances(self.handle, w2o.ctypes.data, o2w.ctypes.data,
+                                                    len(w2o)), "mi_scene_update_instances")
 
 
 def camera(width: int, height: int, eye=ROOM_CAMERA["eye"], target=ROOM_CAMERA["target"],

@@ -107,3 +107,47 @@ def test_rejects_bad_scenes(gpu):
     bad["instance_materials"] = np.array([0, 99], np.uint32)
     with pytest.raises(_lib.MiError):
         make(gpu, bad)
+
+
+def test_update_instances_equals_a_fresh_scene(gpu):
+    """mi_scene_update_instances rebuilds the top level on the device: a
+    render after moving the instances equals, bit for bit, the render of a
+    scene created with the new matrices; moving them back gives the first
+    image again, and a rejected update keeps the placement."""
+    from sprout_amd import _lib
+
+    sc = instanced_room(6, seed=4)
+    moved = instanced_room(6, seed=11)  # other placements (its meshes unused)
+    s = make(gpu, sc)
+    kw = dict(samples_per_wave=4, seed=13)
+    log("instanced room 64x48 16 spp, then the instances moved")
+    first = image(gpu, s, 64, 48, 16, **kw)
+    s.update_instances(moved["world_to_object"], moved["object_to_world"])
+    after = image(gpu, s, 64, 48, 16, **kw)
+    fresh_sc = dict(sc, world_to_object=moved["world_to_object"],
+                    object_to_world=moved["object_to_world"])
+    fresh = make(gpu, fresh_sc)
+    want = image(gpu, fresh, 64, 48, 16, **kw)
+    fresh.release()
+    assert not np.array_equal(after, first)
+    assert np.array_equal(after, want)
+
+    s.update_instances(sc["world_to_object"], sc["object_to_world"])
+    assert np.array_equal(image(gpu, s, 64, 48, 16, **kw), first)
+
+    with pytest.raises(_lib.MiError):
+        s.update_instances(sc["world_to_object"][:5], sc["object_to_world"][:5])
+    bad = sc["object_to_world"].copy()
+    bad[2, 0] = np.nan
+    with pytest.raises(_lib.MiError):
+        s.update_instances(sc["world_to_object"], bad)
+    assert np.array_equal(image(gpu, s, 64, 48, 16, **kw), first)
+    s.release()
+
+    pos, idx, tm, mats = baked(sc)
+    soup = gpu.Scene(pos, idx, tm, mats)
+    with pytest.raises(_lib.MiError):
+        _lib.check(_lib.load().mi_scene_update_instances(
+            soup.handle, sc["world_to_object"].ctypes.data, sc["object_to_world"].ctypes.data, 6),
+            "mi_scene_update_instances")
+    soup.release()
