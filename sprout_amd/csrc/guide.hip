@@ -45,9 +45,18 @@ constexpr int S_feat = Feat_pad + 8;  // 232
 constexpr int S_hid  = Hidden + 8;    // 72
 constexpr int S_out  = Num_out + 8;   // 40
 
+typedef float  f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// fp32 -> bf16, rounded to nearest even (v_cvt_pk_bf16_f32)
 __device__ __forceinline__ uint16_t to_bf16(float f) {
-    uint32_t const u = __float_as_uint(f);
-    return uint16_t((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
+}
+
+// two bf16 in one word, lo in the low half (one v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    f32x2_t const f = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2_t));
 }
 
 __device__ __forceinline__ float from_bf16(uint16_t h) {
@@ -155,8 +164,8 @@ __device__ __forceinline__ void store_layer(f32x4 const (&acc)[N / 16], float co
         }
         if (Mode_relu == Mode) {
             uint2 pk;
-            pk.x = uint32_t(to_bf16(fmaxf(v[0], 0.f))) | (uint32_t(to_bf16(fmaxf(v[1], 0.f))) << 16);
-            pk.y = uint32_t(to_bf16(fmaxf(v[2], 0.f))) | (uint32_t(to_bf16(fmaxf(v[3], 0.f))) << 16);
+            pk.x = pack_bf16(fmaxf(v[0], 0.f), fmaxf(v[1], 0.f));
+            pk.y = pack_bf16(fmaxf(v[2], 0.f), fmaxf(v[3], 0.f));
             *reinterpret_cast<uint2*>(H + r * sh + j0) = pk;
         } else if (Mode_f32 == Mode) {
 #pragma unroll
@@ -225,7 +234,7 @@ __device__ void layer1_direct(float const* x, float const* freq, uint32_t row0, 
 #pragma unroll
     for (int ks = 0; ks < Feat_pad / 16; ++ks) {
         int const k0 = ks * 16 + 4 * q;
-        bf16x4    b  = {0, 0, 0, 0};
+        uint32_t  pk[2] = {0u, 0u};
         if (k0 < Num_feat) {
             int const   i  = k0 / 24;
             int const   j0 = (k0 % 24) / 2;
@@ -237,10 +246,10 @@ __device__ void layer1_direct(float const* x, float const* freq, uint32_t row0, 
                 float const rev = h - floorf(h);
                 float const sn  = live ? __builtin_amdgcn_sinf(rev) : 0.f;
                 float const cs  = live ? __builtin_amdgcn_cosf(rev) : 0.f;
-                b[2 * u]        = short(to_bf16(sn));
-                b[2 * u + 1]    = short(to_bf16(cs));
+                pk[u]           = pack_bf16(sn, cs);
             }
         }
+        bf16x4 const b = __builtin_bit_cast(bf16x4, make_uint2(pk[0], pk[1]));
 #pragma unroll
         for (int nt = 0; nt < Hidden / 16; ++nt) {
             acc[nt] = mfma(load4(Wt + (nt * 16 + r) * S_feat + k0), b, acc[nt]);
