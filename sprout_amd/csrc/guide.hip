@@ -832,8 +832,9 @@ int32_t guide_forward(GuideNet const& net, float const* x, uint32_t n, float* ou
     if (0 == n) {
         return MI_OK;
     }
+    // one workgroup per CU fits (LDS): more would reload the weights per round
     uint32_t const tiles  = blocks_for(n, 16);
-    uint32_t const blocks = std::min<uint32_t>(blocks_for(tiles, Fwd_waves), 1024);
+    uint32_t const blocks = std::min<uint32_t>(blocks_for(tiles, Fwd_waves), net.partial_blocks);
     hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(Fwd_waves * 64), 0, stream, net.wt,
                        net.params, net.freq, x, n, static_cast<uint32_t const*>(nullptr), out);
     if (hipError_t const e = hipGetLastError(); e) {
@@ -848,7 +849,7 @@ int32_t guide_forward_device_count(GuideNet const& net, float const* x, uint32_t
         return MI_OK;
     }
     uint32_t const tiles  = blocks_for(max_n, 16);
-    uint32_t const blocks = std::min<uint32_t>(blocks_for(tiles, Fwd_waves), 1024);
+    uint32_t const blocks = std::min<uint32_t>(blocks_for(tiles, Fwd_waves), net.partial_blocks);
     hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(Fwd_waves * 64), 0, stream, net.wt,
                        net.params, net.freq, x, max_n, n, out);
     if (hipError_t const e = hipGetLastError(); e) {
