@@ -133,8 +133,8 @@ __device__ void features(float const* x, float const* freq, uint32_t row0, uint3
             float const rev = h - floorf(h);
             float const s   = row < n ? __builtin_amdgcn_sinf(rev) : 0.f;
             float const c   = row < n ? __builtin_amdgcn_cosf(rev) : 0.f;
-            X[m * S_feat + 24 * i + 2 * j]     = to_bf16(s);
-            X[m * S_feat + 24 * i + 2 * j + 1] = to_bf16(c);
+            // the pair (sin, cos) as one 4-byte store (row and pair offsets are even)
+            *reinterpret_cast<uint32_t*>(X + m * S_feat + 24 * i + 2 * j) = pack_bf16(s, c);
         }
     }
     if (3 == g) {
@@ -320,6 +320,9 @@ struct alignas(16) Grad_lds {
     float    y[Chunk * S_out];      // outputs
 };
 static_assert(sizeof(Grad_lds) <= 160 * 1024, "LDS of a workgroup");
+static_assert(offsetof(Grad_lds, x) % 8 == 0 && offsetof(Grad_lds, dy) % 8 == 0 &&
+                  (S_feat * sizeof(uint16_t)) % 4 == 0 && (S_out * sizeof(uint16_t)) % 8 == 0,
+              "packed feature and output-gradient stores are aligned");
 constexpr int Num_bias = 3 * Hidden + Num_out;  // 224
 
 // Loss gradient at the outputs of lobes q and q + 4 of one sample (row of
@@ -487,10 +490,11 @@ __global__ void __launch_bounds__(Grad_waves * 64)
             }
             output_gradients(Y + r * S_out, dx, dyv, dz, tw, q, g0, g1);
             wave_sync();  // Y is read by all lanes before DY and DZ1 are written
-            for (int p = 0; p < 4; ++p) {
-                DY[r * S_out + 4 * q + p]       = to_bf16(g0[p]);
-                DY[r * S_out + 4 * (q + 4) + p] = to_bf16(g1[p]);
-            }
+            // four consecutive outputs per store (offsets are multiples of 4 elements)
+            *reinterpret_cast<uint2*>(DY + r * S_out + 4 * q) =
+                make_uint2(pack_bf16(g0[0], g0[1]), pack_bf16(g0[2], g0[3]));
+            *reinterpret_cast<uint2*>(DY + r * S_out + 4 * (q + 4)) =
+                make_uint2(pack_bf16(g1[0], g1[1]), pack_bf16(g1[2], g1[3]));
         }
         wave_sync();
         back_layer<Num_out, Hidden>(DY, S_out, s.w.w4, S_hid, H3, S_hid, DZ3, S_hid);
