@@ -393,30 +393,41 @@ __device__ void output_gradients(float const* y, float dirx, float diry, float d
     lobe_gradient(b, y + 4 * (q + 4), g, qm, wb, dirx, diry, dirz, g1);
 }
 
-// dst[16][N] = (src[16][K] W^T) * (act > 0): backward through one layer;
-// B[k][j] = W[j][k] = Wt[k][j] is gathered from 4 rows of Wt.
+// dst[16][N] = (src[16][K] W^T) * (act > 0): backward through one layer,
+// W[j][k] = Wt[k][j].  Formed as its transpose like layer(): MFMA A operand
+// A[i][k] = W[16 nt + i][k] = Wt[k][16 nt + i], gathered from 4 rows of Wt;
+// B operand = rows of src; so a lane holds outputs 16 nt + 4q + t of row r,
+// read against the ReLU mask with one 8-byte load and stored with one.
 template <int K, int N>
 __device__ void back_layer(uint16_t const* src, int ss, uint16_t const* Wt, int sw,
                            uint16_t const* act, int sact, uint16_t* dst, int sd) {
     int const lane = threadIdx.x & 63;
     int const r    = lane & 15;
     int const q    = lane >> 4;
+    f32x4     acc[N / 16];
 #pragma unroll
     for (int nt = 0; nt < N / 16; ++nt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
 #pragma unroll
-        for (int ks = 0; ks < K / 16; ++ks) {
-            bf16x4 const a = load4(src + r * ss + ks * 16 + 4 * q);
-            bf16x4 const b = gather4(Wt + (ks * 16 + 4 * q) * sw + nt * 16 + r, sw);
-            acc            = mfma(a, b, acc);
+    for (int ks = 0; ks < K / 16; ++ks) {
+        bf16x4 const b = load4(src + r * ss + ks * 16 + 4 * q);
+#pragma unroll
+        for (int nt = 0; nt < N / 16; ++nt) {
+            acc[nt] = mfma(gather4(Wt + (ks * 16 + 4 * q) * sw + nt * 16 + r, sw), b, acc[nt]);
         }
-        int const j = nt * 16 + r;
+    }
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt) {
+        int const    j0 = nt * 16 + 4 * q;
+        bf16x4 const m  = load4(act + r * sact + j0);
+        float        v[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            int const   i = 4 * q + t;
-            float const v = from_bf16(act[i * sact + j]) > 0.f ? acc[t] : 0.f;
-            dst[i * sd + j] = to_bf16(v);
+            v[t] = from_bf16(uint16_t(m[t])) > 0.f ? acc[nt][t] : 0.f;
         }
+        *reinterpret_cast<uint2*>(dst + r * sd + j0) =
+            make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
     }
 }
 

