@@ -218,8 +218,10 @@ __device__ void layer(uint16_t const* A, int sa, uint16_t const* Wt, int sw, flo
 // B operand of lane (r, q) at depth step ks is features k0 .. k0 + 3, k0 =
 // 16 ks + 4q, of query row0 + r, i.e. sin, cos, sin, cos of pi f x_i for
 // i = k0 / 24 and the multipliers j0, j0 + 1 with j0 = (k0 % 24) / 2 (zero
-// past the 216 features and for rows past n): the values features() writes.
-__device__ void layer1_direct(float const* x, float const* freq, uint32_t row0, uint32_t n,
+// past the 216 features): the values features() writes.  half_freq holds
+// f / 2, and (f / 2) x equals (f x) / 2 for every normal product.  Rows past
+// n run on zero inputs; rows do not mix and theirs are never stored.
+__device__ void layer1_direct(float const* x, float const* half_freq, uint32_t row0, uint32_t n,
                               uint16_t const* Wt, float const* bias, uint16_t* H) {
     int const      lane = threadIdx.x & 63;
     int const      r    = lane & 15;
@@ -241,12 +243,9 @@ __device__ void layer1_direct(float const* x, float const* freq, uint32_t row0, 
             float const v  = live ? x[size_t(row) * Num_in + i] : 0.f;
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
-                float const f   = freq[i * Num_freq + j0 + u] * v;
-                float const h   = 0.5f * f;
+                float const h   = half_freq[i * Num_freq + j0 + u] * v;
                 float const rev = h - floorf(h);
-                float const sn  = live ? __builtin_amdgcn_sinf(rev) : 0.f;
-                float const cs  = live ? __builtin_amdgcn_cosf(rev) : 0.f;
-                pk[u]           = pack_bf16(sn, cs);
+                pk[u] = pack_bf16(__builtin_amdgcn_sinf(rev), __builtin_amdgcn_cosf(rev));
             }
         }
         bf16x4 const b = __builtin_bit_cast(bf16x4, make_uint2(pk[0], pk[1]));
@@ -262,7 +261,7 @@ constexpr int Fwd_waves = 16;
 
 struct alignas(16) Fwd_lds {
     Weights  w;
-    float    freq[Num_in * Num_freq];
+    float    half_freq[Num_in * Num_freq];
     uint16_t h[Fwd_waves][2][16 * S_hid];
 };
 static_assert(sizeof(Fwd_lds) <= 160 * 1024, "LDS of a workgroup");
@@ -278,7 +277,7 @@ __global__ void __launch_bounds__(Fwd_waves * 64)
     uint32_t const n = n_dev ? min(*n_dev, n_host) : n_host;
     load_weights(s.w, wt, params);
     for (int i = threadIdx.x; i < Num_in * Num_freq; i += blockDim.x) {
-        s.freq[i] = freq[i];
+        s.half_freq[i] = 0.5f * freq[i];
     }
     __syncthreads();
 
@@ -290,7 +289,7 @@ __global__ void __launch_bounds__(Fwd_waves * 64)
     for (uint32_t tile = blockIdx.x * Fwd_waves + wave; uint64_t(tile) * 16 < n;
          tile += gridDim.x * Fwd_waves) {
         uint32_t const row0 = tile * 16;
-        layer1_direct(x, s.freq, row0, n, s.w.w1, bb, H0);
+        layer1_direct(x, s.half_freq, row0, n, s.w.w1, bb, H0);
         wave_sync();
         layer<Hidden, Hidden, Mode_relu>(H0, S_hid, s.w.w2, S_hid, bb + Hidden, H1, S_hid, nullptr, 0);
         wave_sync();
