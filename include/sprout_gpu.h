@@ -204,16 +204,21 @@ typedef struct mi_material {
     uint32_t kind;
 } mi_material;
 
-/* Pinhole camera: vertical field of view in degrees; pixel (x, y) row-major
- * from the top-left; each sample jitters uniformly inside its pixel (box
- * filter). */
+/* sprout's pinhole Perspective camera (camera_perspective.cpp:47-80,
+ * 204-226): the camera entity's world position and the rows of its rotation
+ * (r0 the image's right, r1 its up, r2 the view direction; sprout's
+ * Composed_transformation rotation with unit scale) and the HORIZONTAL field
+ * of view in degrees (the take's "fov").  Pixel (x, y), row-major from the
+ * top-left, with pixel sample (u, v) looks along
+ *   normalize(left_top + (x + u) d_x + (y + v) d_y) . [r0; r1; r2],
+ * left_top = (-1, h / w, 1 / tan(fov / 2)), d_x = (2 / w, 0, 0),
+ * d_y = (0, -2 (h / w) / h, 0), computed in fp32 as sprout's on_update does. */
 typedef struct mi_camera {
     uint32_t width;
     uint32_t height;
-    float    eye[3];
-    float    target[3];
-    float    up[3];
-    float    fov_deg;
+    float    position[3];
+    float    rotation[9]; /* rows r0, r1, r2 */
+    float    fov_deg;     /* horizontal */
 } mi_camera;
 
 /* Pathtracer_MIS settings (pathtracer_mis.hpp Settings; take_loader.cpp:617-739).
@@ -240,6 +245,15 @@ enum mi_render_flags {
     MI_RENDER_NO_LIGHT_SPLIT = 2, /* one light per vertex (sprout's Light_sampling::Single); by
                                      default the light tree splits at depths 0-2 (Adaptive,
                                      take_loader.cpp:624, pathtracer_mis.cpp:423-426) */
+    MI_RENDER_GOLDEN_RATIO   = 4, /* pixel samples from sprout's Golden_ratio sampler
+                                     (sampler_golden_ratio.cpp:21-73): per pixel, the generator
+                                     started as Worker::render starts it (seed 0, sequence
+                                     (y + r) W' + x + r over the grid grown by the filter reach
+                                     r, rendering_worker.cpp:97-104), a golden-ratio sequence
+                                     of samples_per_pixel points biased-shuffled, sample s
+                                     takes point s; otherwise two draws of the path's own
+                                     generator.  The frame is the one call: first_sample must
+                                     be 0 */
 };
 
 typedef struct mi_render_stats {

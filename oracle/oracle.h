@@ -251,8 +251,39 @@ typedef struct {
     uint64_t state;
     uint64_t inc;
 } oracle_pcg;
-void  oracle_pcg_start(oracle_pcg* g, uint64_t seed, uint64_t sequence);
-float oracle_pcg_float(void* g);
+void     oracle_pcg_start(oracle_pcg* g, uint64_t seed, uint64_t sequence);
+float    oracle_pcg_float(void* g);
+uint32_t oracle_pcg_uint(oracle_pcg* g);
+
+/* ---- camera samples and the perspective camera (sampler.c) ---- */
+
+/* The generator sequence of pixel (x, y) of a frame: rendering_worker.cpp:97-104
+ * over the border-extended grid of grid_width x grid_height pixels. */
+uint64_t oracle_pixel_sequence(int32_t x, int32_t y, uint32_t grid_width, uint32_t grid_height,
+                               int32_t border, uint32_t iteration);
+
+/* The spp camera samples of one pixel from sprout's Golden_ratio sampler: the
+ * generator started at (0, sequence), then image (2 spp floats), lens (2 spp)
+ * and time (spp) dimensions as Sampler::camera_sample generates them on the
+ * first sample.  lens and time may be NULL. */
+int32_t oracle_golden_ratio_camera_samples(uint32_t spp, uint64_t sequence, float* image,
+                                           float* lens, float* time);
+
+typedef struct {
+    float left_top[3];
+    float d_x[3];
+    float d_y[3];
+} oracle_perspective;
+
+/* Perspective::on_update for a horizontal field of view in degrees (no lens tilt or shift) */
+void oracle_perspective_setup(uint32_t width, uint32_t height, float fov_deg,
+                              oracle_perspective* cam);
+
+/* Perspective::generate_ray of pixel (x, y) with the pixel sample (u, v), for a
+ * camera at position with rotation rows r0, r1, r2 (9 floats, unit scale). */
+void oracle_perspective_ray(oracle_perspective const* cam, float const position[3],
+                            float const rotation[9], int32_t x, int32_t y, float u, float v,
+                            float origin[3], float direction[3]);
 
 /* Batch form of oracle_medium_track / _transmittance over n rays (8 floats
  * each: origin, min_t, direction, max_t), generator (seed, index) per ray;
@@ -272,12 +303,13 @@ typedef struct {
     uint32_t kind; /* 0 Substitute, 1 Lambert, 2 light */
 } oracle_material;
 
+/* sprout's Perspective camera (no lens): position, rotation rows r0 (right),
+ * r1 (up), r2 (view direction), horizontal field of view in degrees. */
 typedef struct {
     uint32_t width;
     uint32_t height;
-    float    eye[3];
-    float    target[3];
-    float    up[3];
+    float    position[3];
+    float    rotation[9];
     float    fov_deg;
 } oracle_camera;
 
@@ -331,6 +363,7 @@ typedef struct {
     oracle_medium const* medium; /* NULL: none */
     oracle_pt_instances const* instances; /* NULL: the triangles only */
     uint32_t border;  /* pixels rendered beyond each image edge (a filter's radius) */
+    int32_t  golden_ratio; /* pixel samples from the Golden_ratio sampler (first_sample 0) */
     float*   samples; /* non-NULL: per sample and grid pixel r, g, b, jitter x, y of the
                          (W + 2 border) x (H + 2 border) grid, sample-major; accum unused */
 } oracle_pt_settings;

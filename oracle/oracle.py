@@ -10,6 +10,7 @@ leg may import this module; the product never does.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -47,8 +48,17 @@ class _OracleBvh(ctypes.Structure):
 
 class _OracleCamera(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
-                ("eye", ctypes.c_float * 3), ("target", ctypes.c_float * 3),
-                ("up", ctypes.c_float * 3), ("fov_deg", ctypes.c_float)]
+                ("position", ctypes.c_float * 3), ("rotation", ctypes.c_float * 9),
+                ("fov_deg", ctypes.c_float)]
+
+
+class _OraclePcg(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_uint64), ("inc", ctypes.c_uint64)]
+
+
+class _OraclePerspective(ctypes.Structure):
+    _fields_ = [("left_top", ctypes.c_float * 3), ("d_x", ctypes.c_float * 3),
+                ("d_y", ctypes.c_float * 3)]
 
 
 class _OraclePtSettings(ctypes.Structure):
@@ -57,7 +67,8 @@ class _OraclePtSettings(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("mode", ctypes.c_int32),
                 ("num_threads", ctypes.c_uint32), ("split_lights", ctypes.c_int32),
                 ("medium", ctypes.c_void_p), ("instances", ctypes.c_void_p),
-                ("border", ctypes.c_uint32), ("samples", ctypes.c_void_p)]
+                ("border", ctypes.c_uint32), ("golden_ratio", ctypes.c_int32),
+                ("samples", ctypes.c_void_p)]
 
 
 class _OracleGridtree(ctypes.Structure):
@@ -159,6 +170,27 @@ def lib():
         _lib.oracle_part_cone.restype = None
         _lib.oracle_part_cone.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                           ctypes.c_void_p]
+        _lib.oracle_pcg_start.restype = None
+        _lib.oracle_pcg_start.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
+        _lib.oracle_pcg_float.restype = ctypes.c_float
+        _lib.oracle_pcg_float.argtypes = [ctypes.c_void_p]
+        _lib.oracle_pcg_uint.restype = ctypes.c_uint32
+        _lib.oracle_pcg_uint.argtypes = [ctypes.c_void_p]
+        _lib.oracle_pixel_sequence.restype = ctypes.c_uint64
+        _lib.oracle_pixel_sequence.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
+                                               ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32]
+        _lib.oracle_golden_ratio_camera_samples.restype = ctypes.c_int32
+        _lib.oracle_golden_ratio_camera_samples.argtypes = [ctypes.c_uint32, ctypes.c_uint64,
+                                                            ctypes.c_void_p, ctypes.c_void_p,
+                                                            ctypes.c_void_p]
+        _lib.oracle_perspective_setup.restype = None
+        _lib.oracle_perspective_setup.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                                  ctypes.POINTER(_OraclePerspective)]
+        _lib.oracle_perspective_ray.restype = None
+        _lib.oracle_perspective_ray.argtypes = [ctypes.POINTER(_OraclePerspective), ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.c_float, ctypes.c_float, ctypes.c_void_p,
+                                                ctypes.c_void_p]
         _lib.oracle_render.restype = ctypes.c_int32
         _lib.oracle_render.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                        ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
@@ -881,23 +913,125 @@ def similarity_scale(g, depth) -> float:
     return float(lib().oracle_similarity_scale(g, depth))
 
 
+def look_at(width, height, eye, target, up, fov_deg):
+    """A camera at eye looking at target with a VERTICAL field of view, in
+    sprout's camera form: (position, rotation rows r0 r1 r2 as 9 float32,
+    horizontal fov in degrees); rows in double rounded to float: r2 the view
+    direction, r0 = normalize(up x r2) (image right: sprout's camera frame is
+    left-handed), r1 = r2 x r0."""
+    e = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - e
+    f = f / np.sqrt(f @ f)
+    u = np.asarray(up, np.float64)
+    r = np.array([u[1] * f[2] - u[2] * f[1], u[2] * f[0] - u[0] * f[2], u[0] * f[1] - u[1] * f[0]])
+    r = r / np.sqrt(r @ r)
+    v = np.array([f[1] * r[2] - f[2] * r[1], f[2] * r[0] - f[0] * r[2], f[0] * r[1] - f[1] * r[0]])
+    fov_h = math.degrees(2.0 * math.atan(math.tan(0.5 * math.radians(float(fov_deg)))
+                                         * width / height))
+    return e.astype(np.float32), np.concatenate([r, v, f]).astype(np.float32), np.float32(fov_h)
+
+
+def _camera(width, height, camera) -> _OracleCamera:
+    """camera: {position, rotation (9), fov_deg horizontal} or {eye, target, up, fov_deg vertical}"""
+    if "rotation" in camera:
+        pos = np.asarray(camera["position"], np.float32)
+        rot = np.asarray(camera["rotation"], np.float32).reshape(9)
+        fov = np.float32(camera["fov_deg"])
+    else:
+        pos, rot, fov = look_at(width, height, camera["eye"], camera["target"], camera["up"],
+                                camera["fov_deg"])
+    return _OracleCamera(width, height, (ctypes.c_float * 3)(*pos.tolist()),
+                         (ctypes.c_float * 9)(*rot.tolist()), float(fov))
+
+
+def pixel_sequence(x, y, grid_width, grid_height, border=0, iteration=0) -> int:
+    """The generator sequence sprout's worker starts for pixel (x, y)."""
+    return int(lib().oracle_pixel_sequence(x, y, grid_width, grid_height, border, iteration))
+
+
+def golden_ratio_camera_samples(spp, sequence):
+    """(image (spp, 2), lens (spp, 2), time (spp,)) float32 of one pixel's
+    Golden_ratio sampler, its generator started at (0, sequence)."""
+    image = np.zeros((spp, 2), np.float32)
+    lens = np.zeros((spp, 2), np.float32)
+    time = np.zeros(spp, np.float32)
+    st = lib().oracle_golden_ratio_camera_samples(spp, sequence, _ptr(image), _ptr(lens), _ptr(time))
+    if st:
+        raise ValueError("oracle_golden_ratio_camera_samples")
+    return image, lens, time
+
+
+def pcg_uint(seed, sequence, n):
+    g = _OraclePcg()
+    lib().oracle_pcg_start(ctypes.byref(g), seed, sequence)
+    return np.array([lib().oracle_pcg_uint(ctypes.byref(g)) for _ in range(n)], np.uint32)
+
+
+def pcg_float(seed, sequence, n):
+    g = _OraclePcg()
+    lib().oracle_pcg_start(ctypes.byref(g), seed, sequence)
+    return np.array([lib().oracle_pcg_float(ctypes.byref(g)) for _ in range(n)], np.float32)
+
+
+def camera_rays(width, height, camera, pixels, uv):
+    """Perspective::generate_ray for pixels (n, 2) int (x, y; may lie in a
+    filter border) with pixel samples uv (n, 2): (n, 8) float32 rays."""
+    cam = _camera(width, height, camera)
+    p = _OraclePerspective()
+    lib().oracle_perspective_setup(width, height, cam.fov_deg, ctypes.byref(p))
+    pos = np.array(cam.position[:], np.float32)
+    rot = np.array(cam.rotation[:], np.float32)
+    out = np.zeros((len(pixels), 8), np.float32)
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    for i, ((x, y), (u, v)) in enumerate(zip(np.asarray(pixels), np.asarray(uv, np.float32))):
+        lib().oracle_perspective_ray(ctypes.byref(p), _ptr(pos), _ptr(rot), int(x), int(y),
+                                     float(u), float(v), _ptr(o), _ptr(d))
+        out[i, 0:3] = o
+        out[i, 4:7] = d
+        out[i, 7] = np.float32(3.4028234663852886e38)
+    return out
+
+
+def ref_sampler_lib(arch: str = "native"):
+    """The reference's generator, Golden_ratio sampler and sRGB -> AP1 (oracle/_ref), or None."""
+    path = ref_lib_path(arch)
+    if not os.path.exists(path):
+        return None
+    l = ctypes.CDLL(path)
+    if not hasattr(l, "sref_golden_ratio_camera_samples"):
+        return None
+    l.sref_pcg_uint.restype = None
+    l.sref_pcg_uint.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p]
+    l.sref_pcg_float.restype = None
+    l.sref_pcg_float.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p]
+    l.sref_golden_ratio_camera_samples.restype = None
+    l.sref_golden_ratio_camera_samples.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
+                                                   ctypes.c_int32, ctypes.c_void_p,
+                                                   ctypes.c_void_p, ctypes.c_void_p]
+    l.sref_srgb_to_ap1.restype = None
+    l.sref_srgb_to_ap1.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+    return l
+
+
 def render(positions, indices, triangle_materials, materials, width, height, spp, camera,
            tree: LightTree, first_sample=0, min_bounces=4, max_bounces=8, seed=1, mode=PT_MIS,
            threads=None, split_lights=True, medium: Medium | None = None,
-           instances: PtInstances | None = None, border: int = 0, samples: bool = False):
+           instances: PtInstances | None = None, border: int = 0, samples: bool = False,
+           golden_ratio: bool = False):
     """Radiance sums per pixel, (height, width, 3) float64, of the CPU path
-    tracer (pt.c).  camera: dict with eye, target, up, fov_deg; tree: the
-    scene's light tree; medium: a participating medium or None; instances:
-    mesh instances beside the triangles, or None.  samples=True returns every
-    sample instead: (spp, height + 2 border, width + 2 border, 5) float32 of
-    r, g, b and the pixel jitter x, y, over the grid extended by border."""
+    tracer (pt.c).  camera: {eye, target, up, fov_deg (vertical)} or sprout's
+    {position, rotation, fov_deg (horizontal)}; tree: the scene's light tree;
+    medium: a participating medium or None; instances: mesh instances beside
+    the triangles, or None; golden_ratio: pixel samples from sprout's
+    Golden_ratio sampler.  samples=True returns every sample instead: (spp,
+    height + 2 border, width + 2 border, 5) float32 of r, g, b and the pixel
+    sample x, y, over the grid extended by border."""
     pos = np.ascontiguousarray(positions, np.float32)
     idx = np.ascontiguousarray(indices, np.uint32)
     tmat = np.ascontiguousarray(triangle_materials, np.uint32)
     mats = np.ascontiguousarray(materials).view(MATERIAL_DTYPE)
-    cam = _OracleCamera(width, height, (ctypes.c_float * 3)(*camera["eye"]),
-                        (ctypes.c_float * 3)(*camera["target"]),
-                        (ctypes.c_float * 3)(*camera["up"]), camera["fov_deg"])
+    cam = _camera(width, height, camera)
     n = threads or min(os.cpu_count() or 1, 16)
     out = (np.zeros((spp, height + 2 * border, width + 2 * border, 5), np.float32)
            if samples else None)
@@ -905,7 +1039,7 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
                            1 if split_lights else 0,
                            ctypes.cast(ctypes.pointer(medium.c), ctypes.c_void_p) if medium else None,
                            ctypes.cast(ctypes.pointer(instances.c), ctypes.c_void_p) if instances else None,
-                           border, _ptr(out) if samples else None)
+                           border, 1 if golden_ratio else 0, _ptr(out) if samples else None)
     accum = None if samples else np.zeros((height, width, 3), np.float64)
     status = lib().oracle_render(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(tmat),
                                  _ptr(mats), len(mats), ctypes.byref(tree.c), ctypes.byref(cam),

@@ -83,6 +83,10 @@ float oracle_pcg_float(void* g) {
     return pcg_float((pcg*)g);
 }
 
+uint32_t oracle_pcg_uint(oracle_pcg* g) {
+    return pcg_next((pcg*)g);
+}
+
 typedef struct {
     float x, y, z;
 } v3;
@@ -133,8 +137,7 @@ typedef struct {
     uint32_t*               tri_local;
     oracle_camera const*    cam;
     oracle_pt_settings const* set;
-    v3                      fwd, right, up;
-    float                   tan_half, aspect;
+    oracle_perspective      persp;
     double*                 accum;
     uint32_t                next_row;
     pthread_mutex_t         lock;
@@ -326,8 +329,8 @@ typedef struct {
  * is started at sample * (grid pixels) + gy * (grid width) + gx, as the
  * reference's worker numbers the pixels of its filter-extended tiles
  * (rendering_worker.cpp:93-104).  rgb += radiance; jitter = the pixel offsets. */
-static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample, double rgb[3],
-                    float jitter[2]) {
+static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
+                    float const* pixel_uv, double rgb[3], float jitter[2]) {
     oracle_pt_settings const* set    = s->set;
     oracle_medium const*      medium = set->medium;
     uint32_t const W = s->cam->width;
@@ -340,17 +343,21 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
 
     pcg rng;
     pcg_start(&rng, set->seed, (uint64_t)sample * ((uint64_t)Wg * Hg) + (uint64_t)gy * Wg + gx);
-    float const jx = pcg_float(&rng);
-    float const jy = pcg_float(&rng);
-    jitter[0]      = jx;
-    jitter[1]      = jy;
+    float jx = pcg_float(&rng);
+    float jy = pcg_float(&rng);
+    if (pixel_uv) {
+        /* the pixel's Golden_ratio sample; the draws above are spent as with
+         * the random sampler */
+        jx = pixel_uv[0];
+        jy = pixel_uv[1];
+    }
+    jitter[0] = jx;
+    jitter[1] = jy;
 
-    float const sx = ((2.f * ((float)x + jx)) / (float)W - 1.f) * s->tan_half * s->aspect;
-    float const sy = (1.f - (2.f * ((float)y + jy)) / (float)H) * s->tan_half;
-    v3          o  = mk(s->cam->eye[0], s->cam->eye[1], s->cam->eye[2]);
-    v3          d  = normalize(mk((s->fwd.x + sx * s->right.x) + sy * s->up.x,
-                                  (s->fwd.y + sx * s->right.y) + sy * s->up.y,
-                                  (s->fwd.z + sx * s->right.z) + sy * s->up.z));
+    float oo[3], dd[3];
+    oracle_perspective_ray(&s->persp, s->cam->position, s->cam->rotation, x, y, jx, jy, oo, dd);
+    v3 o = mk(oo[0], oo[1], oo[2]);
+    v3 d = mk(dd[0], dd[1], dd[2]);
 
     float    throughput[3] = {1.f, 1.f, 1.f};
     float    radiance[3]   = {0.f, 0.f, 0.f};
@@ -610,6 +617,9 @@ static void* render_rows(void* arg) {
     uint32_t const Wg = s->cam->width + 2 * s->set->border;
     uint32_t const Hg = s->cam->height + 2 * s->set->border;
     float* const   out = s->set->samples;
+    float* const   uv  = s->set->golden_ratio
+                             ? (float*)malloc(sizeof(float) * 2 * s->set->samples_per_pixel)
+                             : NULL;
     for (;;) {
         pthread_mutex_lock(&s->lock);
         uint32_t const y = s->next_row++;
@@ -619,11 +629,17 @@ static void* render_rows(void* arg) {
         }
         for (uint32_t x = 0; x < Wg; ++x) {
             double rgb[3] = {0.0, 0.0, 0.0};
+            if (uv) {
+                /* the grid pixel's generator sequence, iteration 0 */
+                oracle_golden_ratio_camera_samples(s->set->samples_per_pixel,
+                                                   (uint64_t)y * Wg + x, uv, NULL, NULL);
+            }
             for (uint32_t k = 0; k < s->set->samples_per_pixel; ++k) {
-                float jitter[2];
+                float        jitter[2];
+                float const* puv = uv ? uv + 2 * k : NULL;
                 if (out) {
                     double one[3] = {0.0, 0.0, 0.0};
-                    if (path(s, x, y, s->set->first_sample + k, one, jitter)) {
+                    if (path(s, x, y, s->set->first_sample + k, puv, one, jitter)) {
                         s->status = ORACLE_ERR_STACK;
                     }
                     float* o = out + 5 * (((size_t)k * Hg + y) * Wg + x);
@@ -632,7 +648,7 @@ static void* render_rows(void* arg) {
                     o[2]     = (float)one[2];
                     o[3]     = jitter[0];
                     o[4]     = jitter[1];
-                } else if (path(s, x, y, s->set->first_sample + k, rgb, jitter)) {
+                } else if (path(s, x, y, s->set->first_sample + k, puv, rgb, jitter)) {
                     s->status = ORACLE_ERR_STACK;
                 }
             }
@@ -644,6 +660,7 @@ static void* render_rows(void* arg) {
             }
         }
     }
+    free(uv);
     return NULL;
 }
 
@@ -654,6 +671,7 @@ int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t co
                       oracle_pt_settings const* set, double* accum) {
     if (!positions || !indices || !tri_material || !materials || !tree || !cam || !set ||
         (!accum && !set->samples) || (set->border && !set->samples) || set->border > 8 ||
+        (set->golden_ratio && set->first_sample) ||
         0 == cam->width || 0 == cam->height || 0 == num_triangles) {
         return ORACLE_ERR_ARGS;
     }
@@ -701,27 +719,7 @@ int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t co
         }
     }
 
-    /* camera frame in double, rounded to float: right = fwd x up, up' = right x fwd */
-    double f[3] = {(double)cam->target[0] - cam->eye[0], (double)cam->target[1] - cam->eye[1],
-                   (double)cam->target[2] - cam->eye[2]};
-    double const fl = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-    for (int k = 0; k < 3; ++k) {
-        f[k] /= fl;
-    }
-    double const u0[3] = {cam->up[0], cam->up[1], cam->up[2]};
-    double r[3] = {f[1] * u0[2] - f[2] * u0[1], f[2] * u0[0] - f[0] * u0[2],
-                   f[0] * u0[1] - f[1] * u0[0]};
-    double const rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-    for (int k = 0; k < 3; ++k) {
-        r[k] /= rl;
-    }
-    double const up[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2],
-                          r[0] * f[1] - r[1] * f[0]};
-    s.fwd      = mk((float)f[0], (float)f[1], (float)f[2]);
-    s.right    = mk((float)r[0], (float)r[1], (float)r[2]);
-    s.up       = mk((float)up[0], (float)up[1], (float)up[2]);
-    s.tan_half = (float)tan(0.5 * (double)cam->fov_deg * 3.14159265358979323846 / 180.0);
-    s.aspect   = (float)((double)cam->width / (double)cam->height);
+    oracle_perspective_setup(cam->width, cam->height, cam->fov_deg, &s.persp);
 
     pthread_mutex_init(&s.lock, NULL);
     uint32_t nt = set->num_threads ? set->num_threads : 1;

@@ -1,5 +1,6 @@
 # Builds the reference's own triangle SBVH builder and traversal, its
-# Substitute (Disney + GGX) material sample and its sensor filters, from the
+# Substitute (Disney + GGX) material sample, its sensor filters, its random
+# generator, Golden_ratio camera sampler and sRGB -> AP1 mapping, from the
 # reference sources where they lie, into oracle/_ref/ (never committed).
 # Flags follow the reference's CMake configuration for a Release build on
 # Linux (source/CMakeLists.txt: C++20, -O3, -march=native, no RTTI, no
@@ -38,13 +39,15 @@ REF_SOURCES := \
   $(SRC)/core/logging/log.cpp \
   $(SRC)/core/scene/material/substitute/substitute_sample.cpp \
   $(SRC)/core/sampler/sampler.cpp \
+  $(SRC)/core/sampler/sampler_golden_ratio.cpp \
   $(SRC)/core/rendering/sensor/filter/sensor_gaussian.cpp \
   $(SRC)/core/rendering/sensor/filter/sensor_blackman.cpp \
   $(SRC)/core/rendering/sensor/filter/sensor_mitchell.cpp
 
 all: $(LIB)
 
-DRIVERS := $(HERE)ref_driver.cpp $(HERE)ref_bsdf_driver.cpp $(HERE)ref_filter_driver.cpp
+DRIVERS := $(HERE)ref_driver.cpp $(HERE)ref_bsdf_driver.cpp $(HERE)ref_filter_driver.cpp \
+           $(HERE)ref_sampler_driver.cpp
 
 $(LIB): $(REF_SOURCES) $(DRIVERS)
 	@mkdir -p $(OUT)

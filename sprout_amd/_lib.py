@@ -83,8 +83,8 @@ MI_FILTER_MITCHELL = 3
 
 class MiCamera(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
-                ("eye", ctypes.c_float * 3), ("target", ctypes.c_float * 3),
-                ("up", ctypes.c_float * 3), ("fov_deg", ctypes.c_float)]
+                ("position", ctypes.c_float * 3), ("rotation", ctypes.c_float * 9),
+                ("fov_deg", ctypes.c_float)]
 
 
 class MiRenderSettings(ctypes.Structure):
@@ -112,6 +112,7 @@ MI_MATERIAL_LAMBERT = 1
 MI_MATERIAL_LIGHT = 2
 MI_RENDER_NO_GRAPH = 1
 MI_RENDER_NO_LIGHT_SPLIT = 2
+MI_RENDER_GOLDEN_RATIO = 4
 
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int32

@@ -160,15 +160,16 @@ struct Wave {
     uint32_t*              medium_event;  // per queue entry: Medium_*
     float*                 medium_p;      // scattering point per queue entry
     float const*           e_table;
-    // camera
-    float    eye[3];
-    float    fwd[3];
-    float    right[3];
-    float    up[3];
-    float    tan_half;
-    float    aspect;
-    uint32_t width;
-    uint32_t height;
+    // camera: Perspective's position, rotation rows and on_update frame
+    float         position[3];
+    float         rotation[9];  // rows r0 (right), r1 (up), r2 (view direction)
+    float         left_top[3];
+    float         d_x[3];
+    float         d_y[3];
+    uint32_t      width;
+    uint32_t      height;
+    float2 const* pixel_samples;  // Golden_ratio: table_spp samples per grid pixel, or null
+    uint32_t      table_spp;
     // settings
     uint32_t min_bounces;
     uint32_t max_bounces;
@@ -199,7 +200,8 @@ struct Wave {
     float*       train_w;       // 1 per record
     float*       rec_t;         // throughput before the direction sample
     float*       rec_r;         // path radiance before the next vertex
-    uint32_t*    rec_path;
+    uint32_t*    rec_path;      // path of each record slot, Null: no record this wave
+    uint32_t*    rec_block;     // per 256 record slots: valid records, then their first rank
 };
 
 __device__ __forceinline__ float offset_bits(float p, int32_t o) {
@@ -395,6 +397,87 @@ struct Surface {
     }
 };
 
+// The image dimension of sprout's Golden_ratio sampler for every pixel of
+// the camera grid: the generator started as Worker::render starts it per
+// pixel (seed 0, sequence = the grid pixel (y + r) W' + x + r, iteration 0;
+// rendering_worker.cpp:97-104), then Golden_ratio::generate_2D
+// (sampler_golden_ratio.cpp:51-61): two draws, math::golden_ratio over spp
+// points (sample_distribution.inl:83-147) and rnd::biased_shuffle
+// (shuffle.hpp:12-32).  Nothing before the first camera sample draws from
+// the pixel's generator, so these are the pixel_uv of sprout's samples.
+// table is sample-major (spp rows of num_pixels), so lanes of neighbouring
+// pixels touch neighbouring words.
+__global__ void __launch_bounds__(256) golden_ratio_samples(float2* table, uint32_t num_pixels,
+                                                            uint32_t spp) {
+    uint32_t const pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= num_pixels) {
+        return;
+    }
+    auto at = [=](uint32_t i) -> float2& { return table[size_t(i) * num_pixels + pixel]; };
+    constexpr float Step = 0.618033988749894f;
+
+    // the start pair float2(random_float(), random_float()): g++, which builds
+    // the reference, evaluates the two arguments right to left
+    Rng rng;
+    rng.start(0, pixel);
+    float const r1 = rng.uniform();
+    float const r0 = rng.uniform();
+
+    // the r0 sequence in .y, keeping the index of its smallest value
+    float    x     = r0;
+    float    least = x;
+    uint32_t idx   = 0;
+    for (uint32_t i = 0; i < spp; ++i) {
+        at(i).y = x;
+        if (x < least) {
+            least = x;
+            idx   = i;
+        }
+        x += Step;
+        if (x >= 1.f) {
+            x -= 1.f;
+        }
+    }
+    // read in the Fibonacci order from the smallest into .x
+    uint32_t f = 1, fp = 1, parity = 0;
+    while (f + fp < spp) {
+        uint32_t const t = f;
+        f += fp;
+        fp = t;
+        ++parity;
+    }
+    uint32_t const inc = (parity & 1u) ? f : fp;
+    uint32_t const dec = (parity & 1u) ? fp : f;
+    at(0).x            = at(idx).y;
+    for (uint32_t i = 1; i < spp; ++i) {
+        if (idx < dec) {
+            idx += inc;
+            if (idx >= spp) {
+                idx -= dec;
+            }
+        } else {
+            idx -= dec;
+        }
+        at(i).x = at(idx).y;
+    }
+    // the r1 sequence in .y
+    float y = r1;
+    for (uint32_t i = 0; i < spp; ++i) {
+        at(i).y = y;
+        y += Step;
+        if (y >= 1.f) {
+            y -= 1.f;
+        }
+    }
+    // biased_shuffle: from the back, partner = high word of random * (i + 1)
+    for (uint32_t i = spp - 1; i > 0; --i) {
+        uint64_t const o = (uint64_t(rng.next()) * (uint64_t(i) + 1)) >> 32;
+        float2 const   t = at(i);
+        at(i)            = at(uint32_t(o));
+        at(uint32_t(o))  = t;
+    }
+}
+
 __global__ void set_base(uint32_t* counters, uint32_t first_sample) {
     counters[C_base] = first_sample;
     for (uint32_t k = 0; k < 6; ++k) {
@@ -419,16 +502,40 @@ __global__ void __launch_bounds__(256) raygen(Wave w) {
 
     Path path;
     path.rng.start(w.seed, uint64_t(sample) * w.num_pixels + pixel);
-    float const jx = path.rng.uniform();
-    float const jy = path.rng.uniform();
+    float jx = path.rng.uniform();
+    float jy = path.rng.uniform();
+    if (w.pixel_samples) {
+        // the pixel's Golden_ratio sequence (the two draws above keep the
+        // path's generator where the random sampler leaves it)
+        float2 const ps = w.pixel_samples[size_t(sample) * w.num_pixels + pixel];
+        jx              = ps.x;
+        jy              = ps.y;
+    }
     path.jitter[0] = jx;
     path.jitter[1] = jy;
 
-    float const sx = ((2.f * (float(x) + jx)) / float(w.width) - 1.f) * w.tan_half * w.aspect;
-    float const sy = (1.f - (2.f * (float(y) + jy)) / float(w.height)) * w.tan_half;
-    V3 const    d  = dev::normalize(v3((w.fwd[0] + sx * w.right[0]) + sy * w.up[0],
-                                       (w.fwd[1] + sx * w.right[1]) + sy * w.up[1],
-                                       (w.fwd[2] + sx * w.right[2]) + sy * w.up[2]));
+    // Perspective::generate_ray (camera_perspective.cpp:47-80) with sprout's
+    // contraction into fused multiply-adds, normalize's Newton step on the
+    // correctly rounded 1/sqrt, then object_to_world_vector
+    // (composed_transformation.inl:84-93)
+    float const c0 = float(x) + jx;
+    float const c1 = float(y) + jy;
+    float       dc[3];
+    for (int k = 0; k < 3; ++k) {
+        dc[k] = __builtin_fmaf(c1, w.d_y[k], __builtin_fmaf(c0, w.d_x[k], w.left_top[k]));
+    }
+    float const dd = __builtin_fmaf(dc[2], dc[2], __builtin_fmaf(dc[0], dc[0], dc[1] * dc[1]));
+    float const ri = 1.f / sqrtf(dd);
+    float const ns = (0.5f * ri) * (3.f - (dd * ri) * ri);
+    float const nx = dc[0] * ns;
+    float const ny = dc[1] * ns;
+    float const nz = dc[2] * ns;
+    float const* r0 = w.rotation;
+    float const* r1 = w.rotation + 3;
+    float const* r2 = w.rotation + 6;
+    V3 const     d  = v3(__builtin_fmaf(nz, r2[0], __builtin_fmaf(ny, r1[0], nx * r0[0])),
+                         __builtin_fmaf(nz, r2[1], __builtin_fmaf(ny, r1[1], nx * r0[1])),
+                         __builtin_fmaf(nz, r2[2], __builtin_fmaf(nx, r0[2], ny * r1[2])));
 
     for (int c = 0; c < 3; ++c) {
         path.throughput[c] = 1.f;
@@ -441,9 +548,9 @@ __global__ void __launch_bounds__(256) raygen(Wave w) {
     w.paths[p]    = path;
 
     mi_ray r;
-    r.origin[0]      = w.eye[0];
-    r.origin[1]      = w.eye[1];
-    r.origin[2]      = w.eye[2];
+    r.origin[0]      = w.position[0];
+    r.origin[1]      = w.position[1];
+    r.origin[2]      = w.position[2];
     r.min_t          = 0.f;
     r.direction[0]   = d.x;
     r.direction[1]   = d.y;
@@ -874,9 +981,12 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
         }
 
         if (Guided) {
-            record              = record && spawn;
-            uint32_t const slot = queue_slot(record, &w.counters[C_records]);
-            if (record && slot < w.record_capacity) {
+            // a fixed slot per (path, depth): the records, and so the
+            // training, do not depend on the order the queue was filled in
+            record                = record && spawn;
+            uint64_t const slot64 = uint64_t(p) * w.record_depth + depth;
+            if (record && slot64 < w.record_capacity) {
+                uint32_t const slot = uint32_t(slot64);
                 float const* xi = w.guide_x + 9 * size_t(i);
                 float*       xo = w.train_x + 9 * size_t(slot);
                 for (int k = 0; k < 9; ++k) {
@@ -951,15 +1061,27 @@ __global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q, uin
     }
 }
 
+// Marks every record slot of the wave empty; the shading kernels fill the
+// slot path * record_depth + depth of each vertex that records.
+__global__ void __launch_bounds__(256) clear_records(Wave w) {
+    uint32_t const stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.record_capacity; i += stride) {
+        w.rec_path[i] = dev::Null;
+    }
+}
+
 // Training weight of every record: average over rgb of (radiance the path
 // gathered after the record's vertex, the emitter its direction hit counted
 // without MIS weight) / (throughput before its direction sample) = incident
 // radiance x reflection / sampling pdf.
 __global__ void __launch_bounds__(256) finish_records(Wave w) {
-    uint32_t const count  = min(w.counters[C_records], w.record_capacity);
     uint32_t const stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        Path const& p   = w.paths[w.rec_path[i]];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.record_capacity; i += stride) {
+        uint32_t const path = w.rec_path[i];
+        if (dev::Null == path) {
+            continue;
+        }
+        Path const& p   = w.paths[path];
         float       sum = 0.f;
         for (int k = 0; k < 3; ++k) {
             float const t = w.rec_t[3 * size_t(i) + k];
@@ -971,25 +1093,93 @@ __global__ void __launch_bounds__(256) finish_records(Wave w) {
     }
 }
 
-// Moves record j of the wave to slot (j P + salt) mod count, P = 2^31 - 1
-// (prime, so a bijection for every count below it): the records arrive in
-// queue order, which follows the pixels; consecutive training slices of the
-// shuffled copy mix the whole image.
+// Rank of a valid record slot among the valid slots of its block of 256
+// (slot order), and the block's count.  Every thread of the block calls it.
+__device__ __forceinline__ uint32_t block_rank(bool valid, uint32_t* block_total) {
+    __shared__ uint32_t wave_counts[4];
+    uint32_t const lane  = threadIdx.x & 63;
+    uint32_t const wave  = threadIdx.x >> 6;
+    uint64_t const mask  = __ballot(valid);
+    if (0 == lane) {
+        wave_counts[wave] = uint32_t(__popcll(mask));
+    }
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t k = 0; k < wave; ++k) {
+        before += wave_counts[k];
+    }
+    *block_total = (wave_counts[0] + wave_counts[1]) + (wave_counts[2] + wave_counts[3]);
+    return before + uint32_t(__popcll(mask & ((uint64_t(1) << lane) - 1)));
+}
+
+// Valid records per block of 256 slots (blocks of 256 threads, one slot each).
+__global__ void __launch_bounds__(256) count_records(Wave w) {
+    uint32_t const i     = blockIdx.x * 256 + threadIdx.x;
+    bool const     valid = i < w.record_capacity && dev::Null != w.rec_path[i];
+    uint32_t       total = 0;
+    block_rank(valid, &total);
+    if (0 == threadIdx.x) {
+        w.rec_block[blockIdx.x] = total;
+    }
+}
+
+// Exclusive prefix sums of the block counts, in block order (one block of
+// 1024 threads, each over a contiguous run); the total is the wave's record
+// count.
+__global__ void __launch_bounds__(1024) scan_record_blocks(Wave w, uint32_t num_blocks) {
+    __shared__ uint32_t sums[1024];
+    uint32_t const t     = threadIdx.x;
+    uint32_t const run   = (num_blocks + 1023) / 1024;
+    uint32_t const begin = min(t * run, num_blocks);
+    uint32_t const end   = min(begin + run, num_blocks);
+    uint32_t       local = 0;
+    for (uint32_t b = begin; b < end; ++b) {
+        local += w.rec_block[b];
+    }
+    sums[t] = local;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        uint32_t const add = t >= d ? sums[t - d] : 0u;
+        __syncthreads();
+        sums[t] += add;
+        __syncthreads();
+    }
+    uint32_t offset = sums[t] - local;
+    for (uint32_t b = begin; b < end; ++b) {
+        uint32_t const c = w.rec_block[b];
+        w.rec_block[b]   = offset;
+        offset += c;
+    }
+    if (1023 == t) {
+        w.counters[C_records] = sums[1023];
+    }
+}
+
+// Moves the valid records, in slot order, to the shuffled copy: record j
+// (its rank among the valid slots) goes to (j P + salt) mod count,
+// P = 2^31 - 1 (prime, so a bijection for every count below it).  The slots
+// follow the pixels; consecutive training slices of the shuffled copy mix
+// the whole image.  Blocks of 256 threads, one slot each, as count_records.
 __global__ void __launch_bounds__(256) shuffle_records(Wave w, uint32_t const* step, float* sx,
                                                        float* sd, float* sw) {
-    uint32_t const count  = min(w.counters[C_records], w.record_capacity);
-    uint32_t const salt   = *step * 2654435761u;  // a new permutation after every Adam step
-    uint32_t const stride = gridDim.x * blockDim.x;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
-        uint32_t const k = uint32_t((uint64_t(j) * 2147483647ull + salt) % count);
-        for (int c = 0; c < 9; ++c) {
-            sx[9 * size_t(k) + c] = w.train_x[9 * size_t(j) + c];
-        }
-        for (int c = 0; c < 3; ++c) {
-            sd[3 * size_t(k) + c] = w.train_dir[3 * size_t(j) + c];
-        }
-        sw[k] = w.train_w[j];
+    uint32_t const i     = blockIdx.x * 256 + threadIdx.x;
+    bool const     valid = i < w.record_capacity && dev::Null != w.rec_path[i];
+    uint32_t       total = 0;
+    uint32_t const rank  = block_rank(valid, &total);
+    if (!valid) {
+        return;
     }
+    uint32_t const count = w.counters[C_records];
+    uint32_t const salt  = *step * 2654435761u;  // a new permutation after every Adam step
+    uint32_t const j     = w.rec_block[blockIdx.x] + rank;
+    uint32_t const k     = uint32_t((uint64_t(j) * 2147483647ull + salt) % count);
+    for (int c = 0; c < 9; ++c) {
+        sx[9 * size_t(k) + c] = w.train_x[9 * size_t(i) + c];
+    }
+    for (int c = 0; c < 3; ++c) {
+        sd[3 * size_t(k) + c] = w.train_dir[3 * size_t(i) + c];
+    }
+    sw[k] = w.train_w[i];
 }
 
 __global__ void reset_records(uint32_t* counters) {
@@ -2055,6 +2245,8 @@ void RenderState::release() {
     (void)hipFree(rec_t);
     (void)hipFree(rec_r);
     (void)hipFree(rec_path);
+    (void)hipFree(rec_block);
+    (void)hipFree(pixel_samples);
     if (graph) {
         (void)hipGraphExecDestroy(graph);
     }
@@ -2133,6 +2325,24 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
     return MI_OK;
 }
 
+int32_t RenderState::reserve_pixel_samples(uint64_t entries) {
+    if (entries <= pixel_sample_capacity) {
+        return MI_OK;
+    }
+    if (stream) {
+        (void)hipStreamSynchronize(stream);  // earlier frames may still read the table
+    }
+    (void)hipFree(pixel_samples);
+    pixel_samples         = nullptr;
+    pixel_sample_capacity = 0;
+    if (hipError_t const e = hipMalloc(&pixel_samples, entries * sizeof(float2)); e) {
+        return fail(MI_ERR_HIP, "pixel sample table (%llu): %s", (unsigned long long)entries,
+                    hipGetErrorString(e));
+    }
+    pixel_sample_capacity = entries;
+    return MI_OK;
+}
+
 int32_t RenderState::reserve_guiding(uint32_t n, uint32_t records) {
     if ((n > guide_capacity || records > record_capacity) && stream) {
         (void)hipStreamSynchronize(stream);  // earlier waves may still use the buffers
@@ -2156,12 +2366,14 @@ int32_t RenderState::reserve_guiding(uint32_t n, uint32_t records) {
         (void)hipFree(rec_t);
         (void)hipFree(rec_r);
         (void)hipFree(rec_path);
+        (void)hipFree(rec_block);
         (void)hipFree(shuf_x);
         (void)hipFree(shuf_dir);
         (void)hipFree(shuf_w);
         train_x = train_dir = train_w = rec_t = rec_r = nullptr;
         shuf_x = shuf_dir = shuf_w = nullptr;
         rec_path        = nullptr;
+        rec_block       = nullptr;
         record_capacity = 0;
         hipError_t e    = hipMalloc(&train_x, size_t(records) * 9 * sizeof(float));
         if (!e) e = hipMalloc(&shuf_x, size_t(records) * 9 * sizeof(float));
@@ -2172,6 +2384,7 @@ int32_t RenderState::reserve_guiding(uint32_t n, uint32_t records) {
         if (!e) e = hipMalloc(&rec_t, size_t(records) * 3 * sizeof(float));
         if (!e) e = hipMalloc(&rec_r, size_t(records) * 3 * sizeof(float));
         if (!e) e = hipMalloc(&rec_path, size_t(records) * sizeof(uint32_t));
+        if (!e) e = hipMalloc(&rec_block, size_t((records + 255) / 256) * sizeof(uint32_t));
         if (e) {
             return fail(MI_ERR_HIP, "training records (%u): %s", records, hipGetErrorString(e));
         }
@@ -2206,28 +2419,41 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     uint32_t const num_pixels = uint32_t(num_pixels64);
     uint32_t const num_paths  = uint32_t(paths64);
 
-    // camera frame: right = fwd x up, true up = right x fwd
-    double       f[3]  = {double(cam.target[0]) - cam.eye[0], double(cam.target[1]) - cam.eye[1],
-                          double(cam.target[2]) - cam.eye[2]};
-    double const fl    = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-    double const u0[3] = {cam.up[0], cam.up[1], cam.up[2]};
-    if (!(fl > 0.0)) {
-        return fail(MI_ERR_ARGS, "mi_render: eye and target coincide");
+    for (float const v : cam.position) {
+        if (!std::isfinite(v)) {
+            return fail(MI_ERR_ARGS, "mi_render: camera position not finite");
+        }
     }
-    for (double& x : f) {
-        x /= fl;
+    for (float const v : cam.rotation) {
+        if (!std::isfinite(v)) {
+            return fail(MI_ERR_ARGS, "mi_render: camera rotation not finite");
+        }
     }
-    double r[3] = {f[1] * u0[2] - f[2] * u0[1], f[2] * u0[0] - f[0] * u0[2],
-                   f[0] * u0[1] - f[1] * u0[0]};
-    double const rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-    if (!(rl > 0.0)) {
-        return fail(MI_ERR_ARGS, "mi_render: up is parallel to the view direction");
+    // Perspective::on_update (camera_perspective.cpp:204-226) without lens
+    // tilt or shift, in fp32: fov in radians as degrees_to_radians forms it
+    float const fov_rad = cam.fov_deg * (3.14159265358979323846f / 180.f);
+    float const fw      = float(cam.width);
+    float const fh      = float(cam.height);
+    float const ratio   = fh / fw;
+    float const cz      = 1.f / std::tan(0.5f * fov_rad);
+    float const lt[3]   = {-1.f, ratio, cz};
+    float const rt[3]   = {1.f, ratio, cz};
+    float const lb[3]   = {-1.f, -ratio, cz};
+
+    // Golden_ratio pixel samples: one frame of samples_per_pixel per call
+    bool const golden = 0 != (set.flags & MI_RENDER_GOLDEN_RATIO);
+    if (golden) {
+        uint64_t const entries = num_pixels64 * set.samples_per_pixel;
+        if (0 != set.first_sample || entries > (1ull << 30)) {
+            return fail(MI_ERR_ARGS,
+                        "mi_render: Golden_ratio pixel samples need first_sample 0 and at most "
+                        "2^30 samples per frame (%llu)",
+                        (unsigned long long)entries);
+        }
+        if (int32_t const st = state.reserve_pixel_samples(entries); st) {
+            return st;
+        }
     }
-    for (double& x : r) {
-        x /= rl;
-    }
-    double const up[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2],
-                          r[0] * f[1] - r[1] * f[0]};
 
     // a vertex sends one shadow ray per picked light: up to Max_lights where
     // the tree splits
@@ -2288,13 +2514,16 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.medium_p       = state.medium_p;
     w.e_table      = e_table;
     for (int k = 0; k < 3; ++k) {
-        w.eye[k]   = cam.eye[k];
-        w.fwd[k]   = float(f[k]);
-        w.right[k] = float(r[k]);
-        w.up[k]    = float(up[k]);
+        w.position[k] = cam.position[k];
+        w.left_top[k] = lt[k];
+        w.d_x[k]      = (rt[k] - lt[k]) / fw;
+        w.d_y[k]      = (lb[k] - lt[k]) / fh;
     }
-    w.tan_half    = float(std::tan(0.5 * double(cam.fov_deg) * M_PI / 180.0));
-    w.aspect      = float(double(cam.width) / double(cam.height));
+    for (int k = 0; k < 9; ++k) {
+        w.rotation[k] = cam.rotation[k];
+    }
+    w.pixel_samples = golden ? state.pixel_samples : nullptr;
+    w.table_spp     = golden ? set.samples_per_pixel : 0u;
     w.width       = cam.width;
     w.height      = cam.height;
     w.grid_w      = grid_w;
@@ -2339,6 +2568,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.rec_t           = state.rec_t;
     w.rec_r           = state.rec_r;
     w.rec_path        = state.rec_path;
+    w.rec_block       = state.rec_block;
     for (int k = 0; k < 3; ++k) {
         float const ext = scene.box_hi[k] - scene.box_lo[k];
         w.box_lo[k]     = scene.box_lo[k];
@@ -2371,7 +2601,11 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     };
 
     // one wave: every kernel reads its counts and sample base on the device
+    bool const training = guided && gs->train && gs->train_steps > 0;
     auto enqueue_wave = [&]() -> int32_t {
+        if (training) {
+            hipLaunchKernelGGL(clear_records, dim3(grid_path), dim3(256), 0, s, w);
+        }
         hipLaunchKernelGGL(raygen, dim3(blocks(num_paths, 256)), dim3(256), 0, s, w);
         uint32_t q = 0;
         for (uint32_t d = 0; d <= set.max_bounces; ++d) {
@@ -2417,8 +2651,11 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     // online training on the wave's records, all on the device (capturable):
     // training weights, a shuffled copy, train_steps Adam steps over slices
     auto enqueue_training = [&]() -> int32_t {
+        uint32_t const record_blocks = (records + 255) / 256;
         hipLaunchKernelGGL(finish_records, dim3(grid_path), dim3(256), 0, s, w);
-        hipLaunchKernelGGL(shuffle_records, dim3(grid_path), dim3(256), 0, s, w,
+        hipLaunchKernelGGL(count_records, dim3(record_blocks), dim3(256), 0, s, w);
+        hipLaunchKernelGGL(scan_record_blocks, dim3(1), dim3(1024), 0, s, w, record_blocks);
+        hipLaunchKernelGGL(shuffle_records, dim3(record_blocks), dim3(256), 0, s, w,
                            static_cast<uint32_t const*>(net->step_dev), state.shuf_x, state.shuf_dir,
                            state.shuf_w);
         if (int32_t const st = guide_train_records(
@@ -2431,7 +2668,6 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
         return MI_OK;
     };
-    bool const training = guided && gs->train && gs->train_steps > 0;
 
     // order after the caller's earlier work on its stream
     hipError_t e = hipEventRecord(state.begin, stream);
@@ -2440,6 +2676,10 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         return fail(MI_ERR_HIP, "mi_render: %s", hipGetErrorString(e));
     }
     hipLaunchKernelGGL(set_base, dim3(1), dim3(1), 0, s, state.counters, set.first_sample);
+    if (golden) {
+        hipLaunchKernelGGL(golden_ratio_samples, dim3(blocks(num_pixels, 256)), dim3(256), 0, s,
+                           state.pixel_samples, num_pixels, set.samples_per_pixel);
+    }
     if (guided) {
         hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
     }

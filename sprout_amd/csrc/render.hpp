@@ -162,6 +162,9 @@ struct RenderState {
     float*    rec_t           = nullptr;
     float*    rec_r           = nullptr;
     uint32_t* rec_path        = nullptr;
+    uint32_t* rec_block       = nullptr;  // valid records per 256 slots, then their offsets
+    float2*   pixel_samples   = nullptr;  // Golden_ratio pixel samples of a frame
+    uint64_t  pixel_sample_capacity = 0;
     float*    shuf_x          = nullptr;  // the wave's records, shuffled for training
     float*    shuf_dir        = nullptr;
     float*    shuf_w          = nullptr;
@@ -170,6 +173,7 @@ struct RenderState {
 
     int32_t reserve(uint32_t num_paths, uint32_t num_shadow);
     int32_t reserve_guiding(uint32_t num_paths, uint32_t records);
+    int32_t reserve_pixel_samples(uint64_t entries);
     void    release();
 };
 

@@ -101,25 +101,63 @@ class InstancedScene(Scene):
                                                     len(w2o)), "mi_scene_update_instances")
 
 
+def perspective(width: int, height: int, position, rotation, fov_deg: float) -> _lib.MiCamera:
+    """sprout's Perspective camera: world position, rotation rows r0 (image
+    right), r1 (image up), r2 (view direction) as 9 floats, horizontal field
+    of view in degrees (the take's "fov")."""
+    rot = np.ascontiguousarray(rotation, np.float64).reshape(9)
+    return _lib.MiCamera(width, height, (ctypes.c_float * 3)(*position),
+                         (ctypes.c_float * 9)(*rot), float(fov_deg))
+
+
+def look_at(width: int, height: int, eye, target, up, fov_deg: float):
+    """A camera at eye looking at target, given a VERTICAL field of view in
+    degrees, in sprout's camera form: (position, rotation rows r0 r1 r2 as 9
+    float32, horizontal fov in degrees).  Rows (computed in double, rounded
+    to float): r2 = normalize(target - eye), r0 = normalize(up x r2) (image
+    right in sprout's left-handed camera frame), r1 = r2 x r0."""
+    e = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - e
+    if not np.linalg.norm(f) > 0:
+        raise ValueError("eye and target coincide")
+    f /= np.linalg.norm(f)
+    r = np.cross(np.asarray(up, np.float64), f)
+    if not np.linalg.norm(r) > 0:
+        raise ValueError("up is parallel to the view direction")
+    r /= np.linalg.norm(r)
+    u = np.cross(f, r)
+    rot = np.concatenate([r, u, f]).astype(np.float32)
+    fov_h = np.degrees(2.0 * np.arctan(np.tan(0.5 * np.radians(float(fov_deg))) * width / height))
+    return e.astype(np.float32), rot, np.float32(fov_h)
+
+
 def camera(width: int, height: int, eye=ROOM_CAMERA["eye"], target=ROOM_CAMERA["target"],
            up=ROOM_CAMERA["up"], fov_deg=ROOM_CAMERA["fov_deg"]) -> _lib.MiCamera:
-    return _lib.MiCamera(width, height, (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*target),
-                         (ctypes.c_float * 3)(*up), fov_deg)
+    """A Perspective camera at eye looking at target with a vertical field of
+    view of fov_deg degrees (look_at)."""
+    pos, rot, fov_h = look_at(width, height, eye, target, up, fov_deg)
+    return perspective(width, height, pos, rot, fov_h)
+
+
+def _flags(graph: bool, golden_ratio: bool) -> int:
+    return (0 if graph else _lib.MI_RENDER_NO_GRAPH) | (_lib.MI_RENDER_GOLDEN_RATIO if golden_ratio else 0)
 
 
 def render(scene: Scene, cam: _lib.MiCamera, spp: int, samples_per_wave: int = 1,
            min_bounces: int = 4, max_bounces: int = 8, seed: int = 1, first_sample: int = 0,
-           graph: bool = True, accum: torch.Tensor | None = None, stats: bool = False):
+           graph: bool = True, accum: torch.Tensor | None = None, stats: bool = False,
+           golden_ratio: bool = False):
     """Adds the radiance sums of samples first_sample .. first_sample+spp-1 into
     accum ((height, width, 3) float64 on the device, made zero if None) and
-    returns (accum, stats or None)."""
+    returns (accum, stats or None).  golden_ratio: pixel samples from sprout's
+    Golden_ratio sampler (one frame per call, first_sample 0)."""
     if accum is None:
         accum = torch.zeros((cam.height, cam.width, 3), dtype=torch.float64, device="cuda")
     if accum.dtype != torch.float64 or not accum.is_contiguous() or \
             tuple(accum.shape) != (cam.height, cam.width, 3):
         raise ValueError("accum must be a contiguous (height, width, 3) float64 device tensor")
     st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
-                               0 if graph else _lib.MI_RENDER_NO_GRAPH, seed)
+                               _flags(graph, golden_ratio), seed)
     out = _lib.MiRenderStats() if stats else None
     check(_lib.load().mi_render(scene.handle, ctypes.byref(cam), ctypes.byref(st),
                                 ctypes.c_void_p(accum.data_ptr()), _stream(),
@@ -164,7 +202,8 @@ def sensor(kind: int = _lib.MI_FILTER_GAUSSIAN, radius: float = 2.0, alpha: floa
 def render_filtered(scene: Scene, cam: _lib.MiCamera, spp: int, sens: _lib.MiSensor,
                     samples_per_wave: int = 1, min_bounces: int = 4, max_bounces: int = 8,
                     seed: int = 1, first_sample: int = 0, graph: bool = True,
-                    accum: torch.Tensor | None = None, stats: bool = False):
+                    accum: torch.Tensor | None = None, stats: bool = False,
+                    golden_ratio: bool = False):
     """mi_render_filtered: adds the filtered samples into accum ((height,
     width, 4) float64 on the device: weighted rgb and weight; made zero if
     None) and returns (accum, stats or None); the image is rgb / weight."""
@@ -174,7 +213,7 @@ def render_filtered(scene: Scene, cam: _lib.MiCamera, spp: int, sens: _lib.MiSen
             tuple(accum.shape) != (cam.height, cam.width, 4):
         raise ValueError("accum must be a contiguous (height, width, 4) float64 device tensor")
     st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
-                               0 if graph else _lib.MI_RENDER_NO_GRAPH, seed)
+                               _flags(graph, golden_ratio), seed)
     out = _lib.MiRenderStats() if stats else None
     check(_lib.load().mi_render_filtered(scene.handle, ctypes.byref(cam), ctypes.byref(st),
                                          ctypes.byref(sens), ctypes.c_void_p(accum.data_ptr()),
@@ -188,7 +227,7 @@ def render_guided(scene: Scene, cam: _lib.MiCamera, spp: int, guide, fraction: f
                   learning_rate: float = 1e-3, record_depth: int = 4, samples_per_wave: int = 1,
                   min_bounces: int = 4, max_bounces: int = 8, seed: int = 1,
                   first_sample: int = 0, accum: torch.Tensor | None = None, stats: bool = False,
-                  graph: bool = True, guide_depth: int = 0):
+                  graph: bool = True, guide_depth: int = 0, golden_ratio: bool = False):
     """mi_render_guided: as render(), sampling directions from a one-sample
     mixture of the guide (probability fraction) and the BSDF at vertices of
     depth below guide_depth (0: all), and training the guide online after
@@ -199,7 +238,7 @@ def render_guided(scene: Scene, cam: _lib.MiCamera, spp: int, guide, fraction: f
             tuple(accum.shape) != (cam.height, cam.width, 3):
         raise ValueError("accum must be a contiguous (height, width, 3) float64 device tensor")
     st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
-                               0 if graph else _lib.MI_RENDER_NO_GRAPH, seed)
+                               _flags(graph, golden_ratio), seed)
     gs = _lib.MiGuideSettings(guide.handle, fraction, 1 if train else 0, train_steps, batch,
                               learning_rate, record_depth, guide_depth)
     out = _lib.MiRenderStats() if stats else None

@@ -3,23 +3,30 @@ GPU path renders: triangle meshes in sprout's JSON mesh format
 (triangle_json_handler.cpp: "geometry": "parts" {material_index,
 start_index, num_indices}, "vertices": "positions", "indices"), scenes
 (scene_loader.cpp: "materials" with Substitute {color, roughness, metallic,
-ior} or Light {emission [r, g, b] | emittance {value, spectrum {rgb}}},
+ior} or Light {emission | emittance {quantity Radiance, value, spectrum}},
 "entities" of type Prop / Light with a mesh file shape, materials and a
 transformation {position, scale, rotation in degrees}) and takes
 (take_loader.cpp: "scene", a Perspective camera {parameters.fov,
-transformation, sensor.resolution, sensor.filter}, sampler
-samples_per_pixel, integrator surface PTMIS min / max bounces).
+transformation, sensor.resolution, sensor.filter}, sampler Random or
+Golden_ratio with samples_per_pixel, integrator surface PTMIS min / max
+bounces).
 
-Rotations follow json::create_rotation_matrix (base/json/json.cpp:216-227:
-Rz Rx Ry from degrees) and transforms sprout's row-vector convention (a
-point's world position is x r0 + y r1 + z r2 + r3 with rows r_i = scale_i
-R_i).  sprout's camera looks along its +z with +x to the right
-(camera_perspective.cpp:204-226) and its fov is horizontal; this library's
-camera takes a vertical fov and puts fwd x up to the right, so a take's
-image comes out mirrored left to right (`Take.mirror_x`): flip its columns.
+Colours are read as sprout's read_color reads them (material_provider.cpp:
+1030-1066) and mapped from sRGB to ACEScg (AP1) with spectrum::sRGB_to_AP1
+(base/spectrum/aces.hpp:14-18), as sprout's default build does (SU_ACESCG,
+material_provider.cpp:1022-1028); the mapping's fused multiply-adds follow
+the reference as compiled.  Rotations follow json::create_rotation_matrix
+(base/json/json.cpp:216-227: Rz Rx Ry from degrees) and transforms sprout's
+row-vector convention (a point's world position is x r0 + y r1 + z r2 + r3
+with rows r_i = scale_i R_i).  The camera is sprout's Perspective: its
+entity's position and rotation rows and the take's horizontal fov
+(camera_perspective.cpp:47-80, 204-226).
 
-Built-in shapes (Sphere, Rectangle, ...) and spectra given as temperatures
-are not read: they raise NotImplementedError naming what is missing.
+Anything this path does not render raises NotImplementedError naming it:
+built-in shapes (Sphere, Rectangle, ...), spectra given as temperatures,
+textures, emissive or coated Substitutes and their other optional keys,
+light quantities other than Radiance, and materials other than Substitute
+and Light (sprout has no Lambert material).
 """
 
 from __future__ import annotations
@@ -30,7 +37,7 @@ import os
 
 import numpy as np
 
-from .scenes import LAMBERT, LIGHT, MATERIAL_DTYPE, SUBSTITUTE
+from .scenes import LIGHT, MATERIAL_DTYPE, SUBSTITUTE
 
 
 def _read(path):
@@ -74,36 +81,162 @@ def object_to_world(trafo):
     return m
 
 
+# spectrum::sRGB_to_AP1 coefficients (base/spectrum/aces.hpp:14-18), rows = output channel
+_SRGB_TO_AP1 = np.array([[0.61309732, 0.33952285, 0.04737928],
+                         [0.07019422, 0.91635557, 0.01345259],
+                         [0.02061560, 0.10956983, 0.86981512]], np.float32)
+
+
+def _fma32(a, b, c) -> np.float32:
+    """a * b + c rounded once to float32 (nearest, ties to even), exactly."""
+    from fractions import Fraction
+
+    a, b, c = np.float32(a), np.float32(b), np.float32(c)
+    exact = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    f = np.float32(float(exact))
+    best = None
+    for cand in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        if not np.isfinite(cand):
+            continue
+        err = abs(Fraction(float(cand)) - exact)
+        key = (err, int(np.asarray(cand).view(np.uint32)) & 1)
+        if best is None or key < best[0]:
+            best = (key, cand)
+    return best[1]
+
+
+def srgb_to_ap1(rgb) -> np.ndarray:
+    """spectrum::sRGB_to_AP1 in float32, as the reference build evaluates
+    c0 x + c1 y + c2 z: fma(c2, z, fma(c0, x, c1 y))."""
+    x, y, z = (np.float32(v) for v in rgb)
+    out = np.zeros(3, np.float32)
+    for k in range(3):
+        c0, c1, c2 = _SRGB_TO_AP1[k]
+        out[k] = _fma32(c2, z, _fma32(c0, x, np.float32(c1 * y)))
+    return out
+
+
+def _gamma_to_linear(c: np.float32) -> np.float32:
+    """spectrum::gamma_to_linear_sRGB of one channel (rgb.hpp:34-44)"""
+    c = np.float32(c)
+    if c <= 0:
+        return np.float32(0)
+    if c < np.float32(0.04045):
+        return np.float32(c / np.float32(12.92))
+    if c < 1:
+        return np.float32(np.power(np.float32((c + np.float32(0.055)) / np.float32(1.055)),
+                                   np.float32(2.4)))
+    return np.float32(1)
+
+
+def read_color(value) -> np.ndarray:
+    """read_color (material_provider.cpp:1030-1066), ACEScg build: a number
+    is a grey whose mapped green channel fills all three; an array an sRGB
+    colour; an object {"sRGB": colour, "linear": false} a colour read
+    recursively (so mapped), then gamma-decoded as gamma_to_linear_sRGB(float3)
+    does (its blue channel taken from green, rgb.hpp:54-57) and mapped again;
+    a string "#rrggbb" a hex colour.  Temperatures are not read."""
+    if isinstance(value, bool):
+        raise NotImplementedError(f"colour {value!r}")
+    if isinstance(value, (int, float)):
+        g = srgb_to_ap1([value, value, value])[1]
+        return np.array([g, g, g], np.float32)
+    if isinstance(value, list):
+        if len(value) != 3:
+            raise ValueError(f"colour {value!r}")
+        return srgb_to_ap1(value)
+    if isinstance(value, dict):
+        rgb = np.zeros(3, np.float32)
+        linear = True
+        for k, v in value.items():
+            if k == "sRGB":
+                rgb = read_color(v)
+            elif k == "temperature":
+                raise NotImplementedError("colour given as a black-body temperature")
+            elif k == "linear":
+                linear = bool(v)
+        if not linear:
+            rgb = np.array([_gamma_to_linear(rgb[0]), _gamma_to_linear(rgb[1]),
+                            _gamma_to_linear(rgb[1])], np.float32)
+        return srgb_to_ap1(rgb)
+    if isinstance(value, str) and len(value) == 7 and value.startswith("#"):
+        rgb = [np.float32(float(int(value[1 + 2 * i:3 + 2 * i], 16))) / np.float32(255)
+               for i in range(3)]
+        return srgb_to_ap1(rgb)
+    raise NotImplementedError(f"colour {value!r}")
+
+
+def _mapped_color(value, default) -> np.ndarray:
+    """read_mapped_value for a colour (material_provider.cpp:1097-1111): an
+    object is a texture description whose "value" is the colour (else the
+    default stays); textures are not read"""
+    if isinstance(value, dict):
+        if value.get("file"):
+            raise NotImplementedError(f"texture {value['file']}")
+        if "value" in value:
+            return read_color(value["value"])
+        return np.asarray(default, np.float32)
+    return read_color(value)
+
+
+def _mapped_float(value, default) -> float:
+    if isinstance(value, dict):
+        if "file" in value:
+            raise NotImplementedError(f"texture {value['file']}")
+        return float(value.get("value", default))
+    return float(value)
+
+
+_SUBSTITUTE_KEYS = {"color", "roughness", "metallic", "ior", "sampler"}
+
+
 def _material(node):
     rec = np.zeros(1, MATERIAL_DTYPE)[0]
     rec["ior"] = 1.46
     r = node.get("rendering", node)
     if "Substitute" in r:
         m = r["Substitute"]
+        extra = sorted(k for k in m if k not in _SUBSTITUTE_KEYS and not (
+            (k == "two_sided" and not m[k]) or (k == "anisotropy" and not m[k]) or
+            (k == "emission" and not np.any(_mapped_color(m[k], np.zeros(3)))) or
+            (k == "emission_factor")))
+        if extra:
+            raise NotImplementedError(f"Substitute keys {extra}")
         rec["kind"] = SUBSTITUTE
-        rec["color"] = m.get("color", [0.5, 0.5, 0.5])
-        rec["roughness"] = m.get("roughness", 0.8)
-        rec["metallic"] = m.get("metallic", 0.0)
-        rec["ior"] = m.get("ior", 1.46)
-    elif "Lambert" in r:
-        rec["kind"] = LAMBERT
-        rec["color"] = r["Lambert"].get("color", [0.5, 0.5, 0.5])
+        rec["color"] = _mapped_color(m.get("color", {}), np.full(3, 0.5))
+        rec["roughness"] = _mapped_float(m.get("roughness", 0.8), 0.8)
+        rec["metallic"] = float(m.get("metallic", 0.0))
+        rec["ior"] = float(m.get("ior", 1.46))
     elif "Light" in r:
         m = r["Light"]
+        extra = sorted(k for k in m if k not in ("emission", "emittance", "sampler") and not (
+            k == "two_sided" and not m[k]) and not (k == "emission_factor"))
+        if extra:
+            raise NotImplementedError(f"Light keys {extra}")
         rec["kind"] = LIGHT
         rec["ior"] = 1.0
-        if "emission" in m:
-            rec["emission"] = m["emission"]
+        # Provider::load_light (material_provider.cpp:299-363): an emittance
+        # of quantity Radiance gives value x colour; without a quantity the
+        # emission (default 10, not mapped) is the radiance
+        emission = _mapped_color(m.get("emission", {}), np.full(3, 10.0))
+        quantity = ""
+        color = np.ones(3, np.float32)
+        value = np.float32(1)
+        if "emittance" in m:
+            e = m["emittance"]
+            quantity = e.get("quantity", "")
+            if "spectrum" in e:
+                color = read_color(e["spectrum"])
+            value = np.float32(e.get("value", 1.0))
+        if quantity == "Radiance":
+            rec["emission"] = (value * color).astype(np.float32)
+        elif quantity == "":
+            rec["emission"] = emission
         else:
-            e = m.get("emittance", {})
-            if e.get("quantity", "Radiance") != "Radiance":
-                raise NotImplementedError(f"light emittance quantity {e.get('quantity')}")
-            spec = e.get("spectrum", {})
-            if "rgb" not in spec:
-                raise NotImplementedError(f"light spectrum {sorted(spec)}")
-            rec["emission"] = np.asarray(spec["rgb"], np.float32) * float(e.get("value", 1.0))
+            raise NotImplementedError(f"light emittance quantity {quantity}")
     else:
-        raise NotImplementedError(f"material {sorted(r)}")
+        raise NotImplementedError(f"material {sorted(r)} (the GPU path renders Substitute and "
+                                  "Light; sprout has no Lambert material)")
     return rec
 
 
@@ -143,8 +276,9 @@ def load_scene(path):
 
 
 class Take:
-    """A take file: the scene arrays, the camera (for render.camera), the
-    samples per pixel, the bounces and the sensor's filter."""
+    """A take file: the scene arrays, the camera (position, rotation rows and
+    horizontal fov: render.perspective(width, height, **camera)), the
+    sampler, the samples per pixel, the bounces and the sensor's filter."""
 
     def __init__(self, path):
         doc = _read(path)
@@ -153,21 +287,31 @@ class Take:
         cam = doc.get("camera", {}).get("Perspective")
         if cam is None:
             raise NotImplementedError(f"camera {sorted(doc.get('camera', {}))}")
+        params = cam.get("parameters", {})
+        unsupported = sorted(k for k in params if k not in ("fov",) and not (
+            k == "lens_radius" and not params[k]))
+        if unsupported:
+            raise NotImplementedError(f"camera parameters {unsupported}")
         sensor = cam.get("sensor", {})
         self.width, self.height = (int(v) for v in sensor.get("resolution", [1280, 720]))
         m = object_to_world(cam.get("transformation"))
-        eye = m[3, :3]
-        fov_h = math.radians(float(cam.get("parameters", {}).get("fov", 60)))
-        fov_v = 2.0 * math.atan(math.tan(0.5 * fov_h) * self.height / self.width)
-        self.camera = {"eye": tuple(eye), "target": tuple(eye + m[2, :3]), "up": tuple(m[1, :3]),
-                       "fov_deg": math.degrees(fov_v)}
-        self.mirror_x = True
+        self.camera = {"position": m[3, :3].astype(np.float32),
+                       "rotation": m[:3, :3].reshape(9).astype(np.float32),
+                       "fov_deg": float(params.get("fov", 60))}
         filt = sensor.get("filter", {})
         self.filter = next(iter(filt)) if filt else None
         self.filter_params = filt.get(self.filter, {}) if filt else {}
         spl = doc.get("sampler", {})
-        self.samples_per_pixel = int(next(iter(spl.values())).get("samples_per_pixel", 1)) if spl else 1
+        self.sampler = next(iter(spl)) if spl else "Random"
+        if self.sampler not in ("Random", "Golden_ratio"):
+            raise NotImplementedError(f"sampler {self.sampler}")
+        self.samples_per_pixel = int(spl[self.sampler].get("samples_per_pixel", 1)) if spl else 1
         surf = doc.get("integrator", {}).get("surface", {})
         pt = next(iter(surf.values())) if surf else {}
         self.min_bounces = int(pt.get("min_bounces", 4))
         self.max_bounces = int(pt.get("max_bounces", 8))
+
+    @property
+    def golden_ratio(self) -> bool:
+        """the take's sampler is Golden_ratio (render(..., golden_ratio=True))"""
+        return self.sampler == "Golden_ratio"

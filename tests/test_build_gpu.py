@@ -137,13 +137,20 @@ def test_builds_the_oracle_tree(gpu, case):
     assert_same_tree(nodes, slots, otree.nodes, otree.slots)
 
 
-def test_full_microbenchmark_mesh_matches_reference_digest(gpu):
+@pytest.fixture(scope="module")
+def c1_tree(gpu):
+    """The full 1.41 M-triangle micro-benchmark mesh and its GPU-built tree."""
     from sprout_amd import scenes
 
-    stats = json.load(open(os.path.join(ROOT, "tests", "golden", "bvh_c1_stats.json")))
     pos, idx = scenes.displaced_icosphere()
     log(f"build the {len(idx)}-triangle mesh")
     tree = gpu_build(gpu, pos, idx)
+    return pos, idx, tree
+
+
+def test_full_microbenchmark_mesh_matches_reference_digest(c1_tree):
+    pos, idx, tree = c1_tree
+    stats = json.load(open(os.path.join(ROOT, "tests", "golden", "bvh_c1_stats.json")))
     info = tree.info()
     log(f"  {info.num_nodes} nodes, depth {info.max_depth}, {info.build_ms:.1f} ms")
     nodes, slots = tree.download()
@@ -152,6 +159,36 @@ def test_full_microbenchmark_mesh_matches_reference_digest(gpu):
     assert info.max_depth == stats["max_depth"]
     assert hashlib.sha256(canonical(nodes).tobytes()).hexdigest() == stats["nodes_sha256"]
     assert hashlib.sha256(slots.astype("<u4").tobytes()).hexdigest() == stats["slots_sha256"]
+
+
+def test_full_microbenchmark_traversal_matches_the_oracle(c1_tree):
+    """Closest and any hit on the full 1.41 M-triangle tree (depth 24) for
+    every 64th of the benchmark's 2048 x 2048 camera rays and of its 4 M
+    cosine bounce rays (65 536 each), bit for bit against the oracle's
+    traversal of the same tree, which equals the reference's tree by the
+    digest test above."""
+    import torch
+
+    from sprout_amd import scenes
+
+    pos, idx, tree = c1_tree
+    cam = scenes.camera_rays(2048, 2048)
+    cam_hits = tree.intersect(torch.from_numpy(cam).cuda()).cpu().numpy()
+    bounce = scenes.bounce_rays(cam, cam_hits, pos, idx, 2048 * 2048, seed=3)
+    nodes, slots = tree.download()
+    otree = O.Tree(nodes.view(O.NODE_DTYPE), slots)
+    for name, rays in (("camera", cam[::64]), ("bounce", bounce[::64])):
+        rays = np.ascontiguousarray(rays)
+        log(f"  {name}: {len(rays)} rays, closest and any hit against the oracle")
+        d = torch.from_numpy(rays).cuda()
+        got = tree.intersect(d).cpu().numpy().view(np.uint32)
+        occ = tree.intersect_p(d).cpu().numpy().view(np.uint32)
+        want = O.trace(otree, pos, idx, rays, threads=8)
+        want_occ = O.trace(otree, pos, idx, rays, any_hit=True, threads=8)
+        bad = np.nonzero(np.any(got != want, axis=1))[0]
+        assert len(bad) == 0, f"{name}: {len(bad)} rays differ, first {bad[:4]}"
+        assert np.array_equal(occ, want_occ)
+        assert np.mean(want[:, 3] != 0xFFFFFFFF) > 0.2
 
 
 def test_duplicates_up_to_255_become_one_leaf_and_more_are_rejected(gpu):

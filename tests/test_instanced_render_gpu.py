@@ -151,3 +151,73 @@ def test_update_instances_equals_a_fresh_scene(gpu):
             soup.handle, sc["world_to_object"].ctypes.data, sc["object_to_world"].ctypes.data, 6),
             "mi_scene_update_instances")
     soup.release()
+
+
+@pytest.fixture(scope="module")
+def batch(gpu):
+    """The batched path-tracing scene at full size: 64 placements of four
+    20 480-triangle displaced spheres with per-instance Substitute materials
+    (roughness U[0.05, 1], metallic Bernoulli(0.3), colour U[0.1, 0.9]^3),
+    a ground quad and 10 000 emissive triangles of log-normal power."""
+    from sprout_amd import scenes
+
+    sc = scenes.batch_scene_instanced()
+    s = make(gpu, sc)
+    yield s, sc
+    s.release()
+
+
+def batch_image(gpu, s, w, h, spp, **kw):
+    import torch
+
+    from sprout_amd import scenes
+
+    cam = gpu.camera(w, h, **scenes.BATCH_CAMERA)
+    acc, _ = gpu.render(s, cam, spp, min_bounces=3, max_bounces=16, **kw)
+    torch.cuda.synchronize()
+    return acc.cpu().numpy() / spp
+
+
+def test_batch_scene_same_seeds_as_oracle(gpu, batch):
+    """The batched workload at reduced resolution (240 x 135, 16 spp, Russian
+    roulette from depth 3, max depth 16, adaptive light-tree splitting over
+    the 10 000 emitters) against the oracle's instanced path tracer on the
+    same seeds."""
+    from sprout_amd import scenes
+
+    s, sc = batch
+    w, h, spp = 240, 135, 16
+    log(f"batch scene {w}x{h} {spp} spp against the oracle")
+    img = batch_image(gpu, s, w, h, spp, samples_per_wave=4, seed=5)
+    tree = light_tree(sc["positions"], sc["indices"], sc["triangle_materials"], sc["materials"])
+    ref = O.render(sc["positions"], sc["indices"], sc["triangle_materials"], sc["materials"], w, h,
+                   spp, scenes.BATCH_CAMERA, tree, seed=5, min_bounces=3, max_bounces=16,
+                   instances=oracle_instances(sc)) / spp
+    assert np.all(np.isfinite(img))
+    diff = np.abs(img - ref) / np.maximum(ref, 1e-3)
+    close = float(np.mean(diff < 1e-4))
+    e = rel_mse(img, ref)
+    log(f"  pixels within 1e-4: {close:.4f}, relMSE {e:.3g}, means {img.mean():.5g} {ref.mean():.5g}")
+    assert close > 0.95
+    assert e < 1e-5
+
+
+def test_batch_scene_equals_the_baked_soup(gpu, batch):
+    """The same scene baked into one world-space soup (1.32 M triangles under
+    one tree) renders the same image on the same seeds up to the rounding of
+    the baked vertices."""
+    from sprout_amd import scenes
+
+    s, _ = batch
+    pos, idx, tm, mats = scenes.batch_scene()
+    soup = gpu.Scene(pos, idx, tm, mats)
+    log("batch scene against its baked soup, 240x135 16 spp")
+    a = batch_image(gpu, s, 240, 135, 16, samples_per_wave=4, seed=9)
+    b = batch_image(gpu, soup, 240, 135, 16, samples_per_wave=4, seed=9)
+    soup.release()
+    diff = np.abs(a - b) / np.maximum(b, 1e-3)
+    close = float(np.mean(diff < 1e-3))
+    e = rel_mse(a, b)
+    log(f"  pixels within 1e-3: {close:.4f}, relMSE {e:.3g}")
+    assert close > 0.9
+    assert e < 1e-3

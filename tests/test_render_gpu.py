@@ -124,6 +124,28 @@ def test_room_same_seeds_as_oracle(gpu, room, spw, graph):
     assert rel_mse(img, ref) < 1e-5
 
 
+@pytest.mark.parametrize("spp,spw", [(16, 4), (13, 1)])
+def test_room_golden_ratio_same_seeds_as_oracle(gpu, room, spp, spw):
+    """Pixel samples from sprout's Golden_ratio sampler, started per pixel as
+    sprout's worker starts it: same image as the oracle's on the same seeds
+    (so the device's table equals the reference-pinned oracle sequences),
+    and a different one from the random sampler's."""
+    from sprout_amd import scenes
+
+    sc, (pos, idx, tmat, mats) = room
+    w, h = 64, 48
+    log(f"room {w}x{h} {spp} spp golden ratio, waves of {spw}")
+    img = gpu_image(gpu, sc, w, h, spp, samples_per_wave=spw, seed=5, golden_ratio=True) / spp
+    ref = O.render(pos, idx, tmat, mats, w, h, spp, scenes.ROOM_CAMERA,
+                   light_tree(pos, idx, tmat, mats), seed=5, golden_ratio=True) / spp
+    rnd = gpu_image(gpu, sc, w, h, spp, samples_per_wave=spw, seed=5) / spp
+    assert np.all(np.isfinite(img))
+    diff = np.abs(img - ref) / np.maximum(ref, 1e-3)
+    assert np.mean(diff < 1e-4) > 0.97, np.mean(diff < 1e-4)
+    assert rel_mse(img, ref) < 1e-5
+    assert not np.array_equal(img, rnd)
+
+
 def test_graph_and_launches_agree_bitwise(gpu, room):
     sc, _ = room
     a = gpu_image(gpu, sc, 40, 40, 8, samples_per_wave=2, graph=True, seed=3)
@@ -185,7 +207,17 @@ def test_bad_render_arguments(gpu, room):
     with pytest.raises(_lib.MiError) as e:
         gpu.render(sc, cam, 4, max_bounces=65)
     assert e.value.status == -2
-    bad = gpu.camera(16, 16, eye=(0, 1, 0), target=(0, 2, 0), up=(0, 1, 0), fov_deg=40)
+    with pytest.raises(ValueError):
+        gpu.camera(16, 16, eye=(0, 1, 0), target=(0, 2, 0), up=(0, 1, 0), fov_deg=40)
+    bad = gpu.perspective(16, 16, (0, 1, 3), [1, 0, 0, 0, 1, 0, 0, 0, float("nan")], 40)
     with pytest.raises(_lib.MiError) as e:
         gpu.render(sc, bad, 4)
+    assert e.value.status == -2
+    wide = gpu.perspective(16, 16, (0, 1, 3), [1, 0, 0, 0, 1, 0, 0, 0, 1], 180)
+    with pytest.raises(_lib.MiError) as e:
+        gpu.render(sc, wide, 4)
+    assert e.value.status == -2
+    # Golden_ratio samples are one frame per call
+    with pytest.raises(_lib.MiError) as e:
+        gpu.render(sc, cam, 4, first_sample=4, golden_ratio=True)
     assert e.value.status == -2

@@ -16,7 +16,10 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def test_take_renders_as_the_oracle():
+@pytest.mark.parametrize("golden_ratio", [False, True])
+def test_take_renders_as_the_oracle(golden_ratio):
+    """The take's Perspective camera, colours mapped to ACEScg, the Gaussian
+    sensor; pixel samples from the random or the Golden_ratio sampler."""
     import torch
 
     from sprout_amd import bvh, render, take
@@ -25,20 +28,21 @@ def test_take_renders_as_the_oracle():
     bvh.init(0)
     t = take.Take(os.path.join(HERE, "golden", "take_fixture", "small.take"))
     pos, idx, tmat, mats = t.scene
-    print(f"take fixture: {len(idx)} triangles, {t.width}x{t.height}, {t.samples_per_pixel} spp",
-          flush=True)
+    print(f"take fixture: {len(idx)} triangles, {t.width}x{t.height}, {t.samples_per_pixel} spp, "
+          f"golden ratio {golden_ratio}", flush=True)
     sc = render.Scene(pos, idx, tmat, mats)
-    cam = render.camera(t.width, t.height, **t.camera)
+    cam = render.perspective(t.width, t.height, **t.camera)
     spp = t.samples_per_pixel
     acc, _ = render.render_filtered(sc, cam, spp, render.sensor(), samples_per_wave=8, seed=3,
-                                    min_bounces=t.min_bounces, max_bounces=t.max_bounces)
+                                    min_bounces=t.min_bounces, max_bounces=t.max_bounces,
+                                    golden_ratio=golden_ratio)
     torch.cuda.synchronize()
     acc = acc.cpu().numpy()
     sc.release()
     f = O.Filter(O.FILTER_GAUSSIAN, 2.0)
     s = O.render(pos, idx, tmat, mats, t.width, t.height, spp, t.camera,
                  light_tree(pos, idx, tmat, mats), seed=3, min_bounces=t.min_bounces,
-                 max_bounces=t.max_bounces, border=2, samples=True)
+                 max_bounces=t.max_bounces, border=2, samples=True, golden_ratio=golden_ratio)
     ref = O.filtered_sums(s, t.width, t.height, f)
     img = acc[..., 0:3] / acc[..., 3:4]
     rimg = ref[..., 0:3] / ref[..., 3:4]
