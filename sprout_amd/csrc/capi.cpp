@@ -133,7 +133,7 @@ int32_t mi_init(int32_t device) {
     }
 
     uint32_t* counter = nullptr;
-    if (hipError_t const e = hipMalloc(&counter, sizeof(uint32_t)); e != hipSuccess) {
+    if (hipError_t const e = hipMalloc(&counter, Counter_words * sizeof(uint32_t)); e != hipSuccess) {
         return fail(MI_ERR_HIP, "mi_init: %s", hipGetErrorString(e));
     }
 
@@ -604,7 +604,7 @@ int32_t render_call(uint32_t scene, mi_camera const* camera, mi_render_settings 
     }
     TraceResources& rt = context->render_trace;
     if (!rt.counter) {
-        if (hipError_t const e = hipMalloc(&rt.counter, sizeof(uint32_t)); e) {
+        if (hipError_t const e = hipMalloc(&rt.counter, Counter_words * sizeof(uint32_t)); e) {
             return fail(MI_ERR_HIP, "mi_render: %s", hipGetErrorString(e));
         }
     }

@@ -98,19 +98,63 @@ int32_t guide_train_records(GuideNet& net, float const* x, float const* dirs, fl
 
 namespace dev {
 
+// A mixture as Vmf_mixture::store writes it: per lobe mu (3), kappa, the
+// amplitude (weight * kappa / (2 pi (1 - exp(-2 kappa)))) and the weight.
+inline constexpr int Vmf_stride = 6;
+
+// Its density: the sum over the lobes of amplitude * exp(kappa (mu.w - 1)).
+__device__ __forceinline__ float vmf_mixture_pdf(float const* lobes, V3 w) {
+    float q = 0.f;
+#pragma unroll
+    for (int k = 0; k < guide::Num_lobes; ++k) {
+        float const* l = lobes + Vmf_stride * k;
+        float const  c = l[0] * w.x + l[1] * w.y + l[2] * w.z;
+        q += l[4] * expf(l[3] * (c - 1.f));
+    }
+    return q;
+}
+
+// A direction from it: the lobe by u0 (the last unless the running sum of
+// the weights passes u0 earlier), then w.mu = 1 + log(u1 + (1 - u1)
+// e^(-2 kappa)) / kappa and the azimuth 2 pi u2.
+__device__ inline V3 vmf_mixture_sample(float const* lobes, float u0, float u1, float u2) {
+    int   lobe = guide::Num_lobes - 1;
+    float acc  = 0.f;
+    for (int k = 0; k < guide::Num_lobes - 1; ++k) {
+        acc += lobes[Vmf_stride * k + 5];
+        if (u0 < acc) {
+            lobe = k;
+            break;
+        }
+    }
+    float const* l   = lobes + Vmf_stride * lobe;
+    V3 const     m   = v3(l[0], l[1], l[2]);
+    float const  kap = l[3];
+    float const  wz  = fminf(1.f, fmaxf(-1.f, 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kap)) / kap));
+    float const  r   = sqrtf(fmaxf(0.f, 1.f - wz * wz));
+    float        sp, cp;
+    sincosf(2.f * Pi * u2, &sp, &cp);
+    Layer const fr = frame_from_normal(m);
+    return normalize(fr.tangent_to_world(v3(r * cp, r * sp, wz)));
+}
+
 // The mixture of one query, ready for sampling and density evaluation.
 struct Vmf_mixture {
+    static constexpr int Stored = Vmf_stride * guide::Num_lobes;  // floats store writes
+
     float mu[guide::Num_lobes][3];
     float kappa[guide::Num_lobes];
-    float norm[guide::Num_lobes];  // kappa / (2 pi (1 - exp(-2 kappa)))
+    float amp[guide::Num_lobes];     // weight * kappa / (2 pi (1 - exp(-2 kappa)))
     float weight[guide::Num_lobes];
 
     __device__ void set(float const* raw) {
         float dmax = raw[3];
+#pragma unroll
         for (int k = 1; k < guide::Num_lobes; ++k) {
             dmax = fmaxf(dmax, raw[4 * k + 3]);
         }
         float wsum = 0.f;
+#pragma unroll
         for (int k = 0; k < guide::Num_lobes; ++k) {
             float const* r     = raw + 4 * k;
             float const  sa    = 1.f / (1.f + expf(-r[0]));
@@ -120,52 +164,44 @@ struct Vmf_mixture {
             float        st, ct, sp, cp;
             sincosf(theta, &st, &ct);
             sincosf(phi, &sp, &cp);
-            mu[k][0]          = st * cp;
-            mu[k][1]          = st * sp;
-            mu[k][2]          = ct;
-            float const kap   = expf(fminf(fmaxf(r[2], guide::Kappa_lo), guide::Kappa_hi));
-            kappa[k]          = kap;
-            norm[k]           = kap / (2.f * Pi * -expm1f(-2.f * kap));
-            weight[k]         = expf(r[3] - dmax);
+            mu[k][0]        = st * cp;
+            mu[k][1]        = st * sp;
+            mu[k][2]        = ct;
+            float const kap = expf(fminf(fmaxf(r[2], guide::Kappa_lo), guide::Kappa_hi));
+            kappa[k]        = kap;
+            weight[k]       = expf(r[3] - dmax);
             wsum += weight[k];
         }
+#pragma unroll
         for (int k = 0; k < guide::Num_lobes; ++k) {
             weight[k] /= wsum;
+            amp[k] = weight[k] * (kappa[k] / (2.f * Pi * -expm1f(-2.f * kappa[k])));
+        }
+    }
+
+    // Stored floats per lobe: mu (3), kappa, amplitude, weight;
+    // vmf_mixture_pdf of them equals pdf() bit for bit, and
+    // vmf_mixture_sample samples them.
+    __device__ void store(float* out) const {
+#pragma unroll
+        for (int k = 0; k < guide::Num_lobes; ++k) {
+            out[Vmf_stride * k + 0] = mu[k][0];
+            out[Vmf_stride * k + 1] = mu[k][1];
+            out[Vmf_stride * k + 2] = mu[k][2];
+            out[Vmf_stride * k + 3] = kappa[k];
+            out[Vmf_stride * k + 4] = amp[k];
+            out[Vmf_stride * k + 5] = weight[k];
         }
     }
 
     __device__ float pdf(V3 w) const {
         float q = 0.f;
+#pragma unroll
         for (int k = 0; k < guide::Num_lobes; ++k) {
             float const c = mu[k][0] * w.x + mu[k][1] * w.y + mu[k][2] * w.z;
-            q += weight[k] * norm[k] * expf(kappa[k] * (c - 1.f));
+            q += amp[k] * expf(kappa[k] * (c - 1.f));
         }
         return q;
-    }
-
-    // lobe by u0, direction by (u1, u2): w.mu = 1 + log(u1 + (1 - u1) e^(-2 kappa)) / kappa
-    __device__ V3 sample(float u0, float u1, float u2) const {
-        // the last lobe unless the running sum passes u0 earlier (static
-        // register indices: every lobe is visited)
-        V3    m     = v3(mu[guide::Num_lobes - 1][0], mu[guide::Num_lobes - 1][1],
-                         mu[guide::Num_lobes - 1][2]);
-        float kap   = kappa[guide::Num_lobes - 1];
-        float acc   = 0.f;
-        bool  found = false;
-#pragma unroll
-        for (int k = 0; k < guide::Num_lobes - 1; ++k) {
-            acc += weight[k];
-            bool const take = !found && u0 < acc;
-            m               = take ? v3(mu[k][0], mu[k][1], mu[k][2]) : m;
-            kap             = take ? kappa[k] : kap;
-            found           = found || take;
-        }
-        float const wz = fminf(1.f, fmaxf(-1.f, 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kap)) / kap));
-        float const r  = sqrtf(fmaxf(0.f, 1.f - wz * wz));
-        float       sp, cp;
-        sincosf(2.f * Pi * u2, &sp, &cp);
-        Layer const l = frame_from_normal(m);
-        return normalize(l.tangent_to_world(v3(r * cp, r * sp, wz)));
     }
 };
 

@@ -64,14 +64,17 @@ using dev::v3;
 
 constexpr float Ray_max_t = 3.40282347e+38f;
 
-// Counter words of RenderState::counters
-constexpr uint32_t C_queue0     = 0;  // live rays, two ping-pong queues
-constexpr uint32_t C_shadow     = 2;  // shadow rays
-constexpr uint32_t C_base       = 4;  // first sample index of the current wave
-constexpr uint32_t C_records    = 6;  // training samples recorded in the current wave
-constexpr uint32_t C_totals     = 8;  // two uint64: extension and shadow rays traced,
-                                      // then one uint64: guiding records trained on
-constexpr uint32_t Num_counters = 16;
+// Counter words of RenderState::counters, each on its own 256-byte line:
+// atomics on one line serialize (about 12 ns each), so the two queues the
+// shading kernel appends to must not share one.
+constexpr uint32_t Counter_line = 64;
+constexpr uint32_t C_queue0     = 0;                 // live rays, two ping-pong queues (lines 0, 1)
+constexpr uint32_t C_shadow     = 2 * Counter_line;  // shadow rays
+constexpr uint32_t C_base       = 3 * Counter_line;  // first sample index of the current wave
+constexpr uint32_t C_records    = 4 * Counter_line;  // training samples recorded in the current wave
+constexpr uint32_t C_totals     = 5 * Counter_line;  // two uint64: extension and shadow rays traced,
+                                                     // then one uint64: guiding records trained on
+constexpr uint32_t Num_counters = 6 * Counter_line;
 
 // rnd::Generator, base/random/generator.inl:16-49
 struct Rng {
@@ -97,6 +100,25 @@ struct Rng {
     __device__ float uniform() {
         uint32_t const bits = (next() & 0x007FFFFFu) | 0x3F800000u;
         return __uint_as_float(bits) - 1.f;
+    }
+
+    // The state n steps ahead in O(log n) steps: the n-fold composition of
+    // the LCG (Brown, "Random Number Generation with Arbitrary Strides", 1994)
+    __device__ void advance(uint64_t n) {
+        uint64_t mul     = 6364136223846793005ull;
+        uint64_t add     = inc | 1;
+        uint64_t acc_mul = 1;
+        uint64_t acc_add = 0;
+        while (n) {
+            if (n & 1) {
+                acc_mul *= mul;
+                acc_add = acc_add * mul + add;
+            }
+            add = (mul + 1) * add;
+            mul *= mul;
+            n >>= 1;
+        }
+        state = acc_mul * state + acc_add;
     }
 };
 
@@ -131,6 +153,31 @@ struct Path {
 struct Shadow {
     float    contrib[3];
     uint32_t path;
+};
+
+// What the shadow rays of a vertex need (one per queue entry with picks).
+struct Nee_vertex {
+    uint64_t rng_state;  // the path's generator after the light-selection draw
+    uint64_t rng_inc;
+    float    p[3];       // origin of the shadow rays
+    uint32_t path;
+    float    n[3];       // geometric normal; (0, 0, 1) in the medium
+    uint32_t material;   // the surface's material, or Kind_phase
+    float    wo[3];
+    uint32_t flags;      // Nee_guided, Nee_medium
+    float    throughput[3];
+    uint32_t pad;
+};
+
+constexpr uint32_t Nee_guided = 1;  // the guide's mixture (guide_mix) is in the MIS weight
+constexpr uint32_t Nee_medium = 2;  // a scattering vertex of the medium
+
+// One light pick in its shadow-queue slot: pick k of a queue entry's vertex.
+struct Pick_slot {
+    uint32_t light;  // the part, or Null: no ray
+    float    pdf;
+    uint32_t vertex;
+    uint32_t k;
 };
 
 struct Wave {
@@ -185,10 +232,13 @@ struct Wave {
     Shadow*   shadows;
     uint32_t* occluded;
     uint2*    shadow_range;  // first shadow entry and count of each queue entry's vertex
+    Nee_vertex* nee;         // per queue entry whose vertex picked lights
+    Pick_slot*  picks;       // per shadow entry: the light it goes to
     uint32_t* counters;
     // guiding
     float        fraction;      // probability of sampling the guide
     float const* guide_raw;     // 32 network outputs per queue entry
+    float*       guide_mix;     // the mixture of a guided vertex with light picks (Vmf_mixture::store)
     float*       guide_x;       // 9 network inputs per queue entry
     float        box_lo[3];
     float        box_scale[3];  // 1 / extent of the scene box
@@ -308,38 +358,38 @@ __device__ __forceinline__ float predivided_power_heuristic(float f, float g) {
     return f / (f * f + g * g);
 }
 
-// Appends one entry per requesting lane to a queue: one atomic per wave.
-// Every lane of the wave must call it.
-__device__ __forceinline__ uint32_t queue_slot(bool want, uint32_t* counter) {
-    uint64_t const mask = __ballot(want);
+// Reserves n consecutive entries per thread of the block in a queue with one
+// atomic per block; the threads' ranges follow in thread order.  Every thread
+// of the block calls it (blocks of up to 1024 threads).  Same-address atomics
+// serialize, so one per block instead of one per wave keeps them off the
+// critical path.
+__device__ __forceinline__ uint32_t block_queue_range(uint32_t n, uint32_t* counter) {
+    __shared__ uint32_t wave_total[16];
+    __shared__ uint32_t block_base;
     uint32_t const lane = threadIdx.x & 63;
-    if (0 == mask) {
-        return 0;
-    }
-    uint32_t const leader = uint32_t(__ffsll(static_cast<unsigned long long>(mask))) - 1;
-    uint32_t       base   = 0;
-    if (lane == leader) {
-        base = atomicAdd(counter, uint32_t(__popcll(mask)));
-    }
-    base = __shfl(base, leader);
-    return base + uint32_t(__popcll(mask & ((uint64_t(1) << lane) - 1)));
-}
-
-// Reserves n consecutive queue entries per lane: one atomic per wave, the
-// lanes' ranges in lane order.  Every lane of the wave must call it.
-__device__ __forceinline__ uint32_t queue_range(uint32_t n, uint32_t* counter) {
-    uint32_t const lane = threadIdx.x & 63;
-    uint32_t       inc  = n;  // inclusive prefix sum over the lanes
+    uint32_t const wave = threadIdx.x >> 6;
+    uint32_t       inc  = n;  // inclusive prefix sum over the wave's lanes
     for (uint32_t d = 1; d < 64; d <<= 1) {
         uint32_t const up = __shfl_up(inc, d, 64);
         inc += lane >= d ? up : 0u;
     }
-    uint32_t const total = __shfl(inc, 63);
-    uint32_t       base  = 0;
-    if (63 == lane && total) {
-        base = atomicAdd(counter, total);
+    if (63 == lane) {
+        wave_total[wave] = inc;
     }
-    base = __shfl(base, 63);
+    __syncthreads();
+    if (0 == threadIdx.x) {
+        uint32_t total = 0;
+        for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) {
+            total += wave_total[k];
+        }
+        block_base = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = block_base;
+    for (uint32_t k = 0; k < wave; ++k) {
+        base += wave_total[k];
+    }
+    __syncthreads();  // the next call reuses wave_total and block_base
     return base + inc - n;
 }
 
@@ -561,21 +611,21 @@ __global__ void __launch_bounds__(256) raygen(Wave w) {
 
     if (0 == p) {
         w.counters[C_queue0]     = w.num_paths;
-        w.counters[C_queue0 + 1] = 0;
+        w.counters[C_queue0 + Counter_line] = 0;
         w.counters[C_shadow]     = 0;
     }
 }
 
 // Clears the queue the next shade fills and the shadow queue.
 __global__ void reset_queues(uint32_t* counters, uint32_t out_q) {
-    counters[C_queue0 + out_q] = 0;
+    counters[C_queue0 + Counter_line * out_q] = 0;
     counters[C_shadow]         = 0;
 }
 
 // Ray totals for the statistics: the queue just traced and the shadow queue.
 __global__ void tally(uint32_t* counters, uint32_t in_q) {
     uint64_t* totals = reinterpret_cast<uint64_t*>(counters + C_totals);
-    totals[0] += counters[C_queue0 + in_q];
+    totals[0] += counters[C_queue0 + Counter_line * in_q];
     totals[1] += counters[C_shadow];
 }
 
@@ -583,7 +633,7 @@ __global__ void tally(uint32_t* counters, uint32_t in_q) {
 // its surface hit that lies in the medium box is tracked; the tracking weight
 // multiplies the path throughput, and a scattering point replaces the hit.
 __global__ void __launch_bounds__(256) medium_track(Wave w, uint32_t in_q, uint32_t depth) {
-    uint32_t const count  = w.counters[C_queue0 + in_q];
+    uint32_t const count  = w.counters[C_queue0 + Counter_line * in_q];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         mi_ray const r     = w.rays[in_q][i];
@@ -619,7 +669,7 @@ __global__ void __launch_bounds__(256) medium_track(Wave w, uint32_t in_q, uint3
 // Network inputs of every queued hit: position in the scene box, geometric
 // normal and outgoing direction, each mapped to [0,1].
 __global__ void __launch_bounds__(256) guide_inputs(Wave w, uint32_t in_q) {
-    uint32_t const count  = w.counters[C_queue0 + in_q];
+    uint32_t const count  = w.counters[C_queue0 + Counter_line * in_q];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         Hit const h  = load_hit(w, i);
@@ -646,9 +696,15 @@ __global__ void __launch_bounds__(256) guide_inputs(Wave w, uint32_t in_q) {
     }
 }
 
+// One bounce of Pathtracer_MIS::integrate (pathtracer_mis.cpp:95-279) for
+// every queued ray: emission with its MIS weight, Russian roulette, the light
+// picks of next-event estimation (their rays are made by the nee kernel), the
+// direction sample (the BSDF's, or with Guided the one-sample mixture of the
+// guide and the BSDF) and the next ray.  The path's fields are read and
+// written where they are used, so few stay in registers.
 template <bool Guided>
-__global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t depth) {
-    uint32_t const count  = w.counters[C_queue0 + in_q];
+__device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32_t depth) {
+    uint32_t const count  = w.counters[C_queue0 + Counter_line * in_q];
     uint32_t const out_q  = in_q ^ 1;
     uint32_t const stride = gridDim.x * blockDim.x;
     // sample_lights splits the light tree at the first bounces
@@ -659,117 +715,97 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         uint32_t const i = base + threadIdx.x;
 
-        bool     spawn  = false;
-        bool     record = false;
-        bool     live   = false;  // a shading vertex that continues the path
-        uint32_t p      = 0;
-        Path     path;
-        mi_ray   next_ray;
-        float    rec_t[3] = {0.f, 0.f, 0.f};
-        V3       hit_p    = v3(0.f, 0.f, 0.f);
-        V3       geo_n    = v3(0.f, 0.f, 1.f);
-        V3       p_o      = v3(0.f, 0.f, 0.f);
+        bool     spawn     = false;
+        bool     live      = false;  // a shading vertex that continues the path
         bool     in_medium = false;  // a scattering vertex of the medium
-        Surface  s;
-        s.kind = MI_MATERIAL_LAMBERT;
-
-        dev::Light_pick picks[dev::Max_lights];
-        uint32_t        num_picks = 0;
+        uint32_t p         = 0;
+        float    thr[3]    = {0.f, 0.f, 0.f};
+        uint32_t flags     = 0;
+        Rng      rng;
+        rng.state = rng.inc = 0;
+        uint32_t material   = 0;  // of a surface vertex
+        V3       hit_p      = v3(0.f, 0.f, 0.f);
+        V3       geo_n      = v3(0.f, 0.f, 1.f);
+        V3       wo         = v3(0.f, 0.f, 1.f);
+        V3       p_o        = v3(0.f, 0.f, 0.f);
+        // next-event estimation: the lights the tree picks at p_o
+        // (sample_lights); counted here, picked once their queue slots are known
+        uint32_t num_picks = 0;
+        float    select    = 0.f;
+        uint32_t prev_record = dev::Null;  // training record of the previous vertex's sample
 
         if (i < count) {
-            p              = w.ray_path[in_q][i];
-            path           = w.paths[p];
-            Hit const h = load_hit(w, i);
+            p                = w.ray_path[in_q][i];
+            Path const* pp   = &w.paths[p];
+            thr[0]           = pp->throughput[0];
+            thr[1]           = pp->throughput[1];
+            thr[2]           = pp->throughput[2];
+            flags            = pp->flags;
+            rng              = pp->rng;
+            Hit const h      = load_hit(w, i);
+            mi_ray const r   = w.rays[in_q][i];
+            wo               = v3(-r.direction[0], -r.direction[1], -r.direction[2]);
 
-            // the training record of the previous vertex's direction sample
-            uint32_t const prev_record = Guided ? path.record : dev::Null;
-            if (Guided && dev::Null != path.record) {
-                // radiance before this vertex: what follows is the previous sample's
-                float* rr   = w.rec_r + 3 * size_t(path.record);
-                rr[0]       = path.radiance[0];
-                rr[1]       = path.radiance[1];
-                rr[2]       = path.radiance[2];
-                path.record = dev::Null;
+            if (Guided) {
+                prev_record = pp->record;
+                if (dev::Null != prev_record) {
+                    // radiance before this vertex: what follows is the previous sample's
+                    float* rr = w.rec_r + 3 * size_t(prev_record);
+                    rr[0]     = pp->radiance[0];
+                    rr[1]     = pp->radiance[1];
+                    rr[2]     = pp->radiance[2];
+                }
             }
 
             uint32_t const event = w.medium.present ? w.medium_event[i] : Medium_none;
+            bool           roulette = false;
 
             if (Medium_scatter == event) {
                 // Event::Scatter: the vertex is the scattering point
                 // (pathtracer_mis.cpp:215-250); no emission, the same depth
                 // and roulette tests as a surface
-                mi_ray const r = w.rays[in_q][i];
                 hit_p = v3(w.medium_p[3 * size_t(i) + 0], w.medium_p[3 * size_t(i) + 1],
                            w.medium_p[3 * size_t(i) + 2]);
                 in_medium = true;
                 live      = true;
-                if (0 != depth) {
-                    if (depth >= w.max_bounces) {
-                        live = false;
-                    } else if (depth >= w.min_bounces) {
-                        float const q = max_component(path.throughput);
-                        if (path.rng.uniform() > q) {
-                            live = false;
-                        } else {
-                            for (int k = 0; k < 3; ++k) {
-                                path.throughput[k] /= q;
-                            }
-                        }
-                    }
-                }
-                if (live) {
-                    s.kind   = Kind_phase;
-                    s.g      = w.medium.g;
-                    s.sub.wo = v3(-r.direction[0], -r.direction[1], -r.direction[2]);
-                    // offset_p(n, translucent) of a subsurface vertex: the point itself;
-                    // the tree weighs lights over the whole sphere
-                    p_o = hit_p;
-                    if (w.tree.num_nodes) {
-                        float const select = path.rng.uniform();
-                        num_picks = dev::random_light(w.tree, p_o, geo_n, true, select, split, picks);
-                    }
-                }
+                roulette  = 0 != depth;
             } else if (Medium_absorbed == event) {
                 // Event::Abort: the tracking weight fell below Abort_epsilon
             } else if (dev::Null != h.triangle) {
-                mi_ray const        r  = w.rays[in_q][i];
                 Surface_point const sp = surface_point(w, h);
                 hit_p                  = sp.p;
                 geo_n                  = sp.n;
-                V3 const wo            = v3(-r.direction[0], -r.direction[1], -r.direction[2]);
+                material               = sp.material;
+                uint32_t const kind    = w.materials[sp.material].kind;
+                live                   = true;
 
-                Render_material const m = w.materials[sp.material];
-
-                live = true;
-
-                if (MI_MATERIAL_LIGHT == m.kind) {
+                if (MI_MATERIAL_LIGHT == kind) {
                     // first vertex: mat_sample.radiance(); later: connect_light
                     float const cos_l = dev::dot(geo_n, wo);
                     if (cos_l > 0.f) {
-                        float weight = 1.f;
-                        uint32_t const l = sp.light;
+                        float          weight = 1.f;
+                        uint32_t const l      = sp.light;
                         // an emitter light sampling cannot pick (zero power)
                         // is found by BSDF sampling alone: weight 1
-                        if (0 != depth && !(path.flags & Flag_singular) && dev::Null != l) {
+                        if (0 != depth && !(flags & Flag_singular) && dev::Null != l) {
                             // Mesh::pdf times Tree::pdf at the previous vertex
-                            float const tree_pdf = dev::light_pdf(
-                                w.tree, v3(r.origin[0], r.origin[1], r.origin[2]),
-                                v3(path.prev_n[0], path.prev_n[1], path.prev_n[2]),
-                                0 != (path.flags & Flag_translucent), split, l);
+                            V3 const prev_p = v3(r.origin[0], r.origin[1], r.origin[2]);
+                            V3 const prev_n = v3(pp->prev_n[0], pp->prev_n[1], pp->prev_n[2]);
+                            bool const translucent = 0 != (flags & Flag_translucent);
+                            float const tree_pdf   = dev::light_pdf(w.tree, prev_p, prev_n, translucent, split, l);
                             // the triangle's pdf in its part's Primitive_tree
                             // times its relative area (Part::pdf, triangle_mesh.cpp:264-270)
                             uint32_t const    local = w.tri_local[h.triangle];
-                            float const       ppdf  = dev::part_pdf(
-                                w.parts, l, v3(r.origin[0], r.origin[1], r.origin[2]),
-                                v3(path.prev_n[0], path.prev_n[1], path.prev_n[2]),
-                                0 != (path.flags & Flag_translucent), local);
+                            float const       ppdf  = dev::part_pdf(w.parts, l, prev_p, prev_n, translucent, local);
                             Light_info const& L = w.tree.lights[l];
                             Light_info const& T = w.parts.lights[w.parts.parts[l].light_offset + local];
                             float const ls_pdf  = (h.t * h.t) / (cos_l * L.area) * (ppdf * (L.area / T.area));
-                            weight              = power_heuristic(path.bsdf_pdf, ls_pdf * tree_pdf);
+                            weight              = power_heuristic(pp->bsdf_pdf, ls_pdf * tree_pdf);
                         }
+                        float const* e  = w.materials[sp.material].emission;
+                        float*       rad = w.paths[p].radiance;
                         for (int k = 0; k < 3; ++k) {
-                            path.radiance[k] += path.throughput[k] * (weight * m.emission[k]);
+                            rad[k] = rad[k] + thr[k] * (weight * e[k]);
                         }
                         if (Guided && dev::Null != prev_record) {
                             // the guide learns the whole incident radiance: the
@@ -777,147 +813,126 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                             // without its MIS weight
                             float* rr = w.rec_r + 3 * size_t(prev_record);
                             for (int k = 0; k < 3; ++k) {
-                                rr[k] -= path.throughput[k] * ((1.f - weight) * m.emission[k]);
+                                rr[k] -= thr[k] * ((1.f - weight) * e[k]);
                             }
                         }
                     }
                     live = false;
-                } else if (0 != depth) {
-                    if (depth >= w.max_bounces) {
+                } else {
+                    roulette = 0 != depth;
+                }
+            }
+
+            if (roulette) {
+                if (depth >= w.max_bounces) {
+                    live = false;
+                } else if (depth >= w.min_bounces) {
+                    // russian_roulette, integrator_helper.hpp:42-52
+                    float const q = max_component(thr);
+                    if (rng.uniform() > q) {
                         live = false;
-                    } else if (depth >= w.min_bounces) {
-                        // russian_roulette, integrator_helper.hpp:42-52
-                        float const q = max_component(path.throughput);
-                        if (path.rng.uniform() > q) {
-                            live = false;
-                        } else {
-                            for (int k = 0; k < 3; ++k) {
-                                path.throughput[k] /= q;
-                            }
+                    } else {
+                        for (int k = 0; k < 3; ++k) {
+                            thr[k] /= q;
                         }
                     }
                 }
-
-                if (live && !(dev::dot(geo_n, wo) > 0.f)) {
-                    live = false;  // back face: evaluate and sample give nothing
+            }
+            if (live && !in_medium && !(dev::dot(geo_n, wo) > 0.f)) {
+                live = false;  // back face: evaluate and sample give nothing
+            }
+            if (live) {
+                // offset_p: a surface vertex moves off the surface; a
+                // scattering vertex uses the point itself and its tree
+                // weighs lights over the whole sphere
+                p_o = in_medium ? hit_p : offset_ray(hit_p, geo_n);
+                if (w.tree.num_nodes) {
+                    select    = rng.uniform();
+                    num_picks = dev::count_lights(w.tree, p_o, split);
                 }
-
-                if (live) {
-                    s.kind = m.kind;
-                    s.sub.set(dev::frame_from_normal(geo_n), geo_n, wo, m.color, m.roughness,
-                              m.metallic, m.ior, 1.f, w.e_table);
-                    // next-event estimation: the lights the tree picks (sample_lights)
-                    p_o = offset_ray(hit_p, geo_n);
-                    if (w.tree.num_nodes) {
-                        float const select = path.rng.uniform();
-                        num_picks = dev::random_light(w.tree, p_o, geo_n, false, select, split, picks);
-                    }
-                }
+                bool const linear = depth <= 1 || 0 != (flags & Flag_primary);
+                flags             = linear ? (flags & ~Flag_nearest) : (flags | Flag_nearest);
             }
         }
 
-        if (live) {
-            bool const linear = depth <= 1 || 0 != (path.flags & Flag_primary);
-            path.flags        = linear ? (path.flags & ~Flag_nearest) : (path.flags | Flag_nearest);
-        }
+        bool const guided_vertex = Guided && live && !in_medium && depth < w.guide_depth;
 
-        // the guide's mixture at a guided vertex, for the MIS weights of its
-        // shadow rays and its direction sample
-        bool const       guided_vertex = Guided && live && !in_medium && depth < w.guide_depth;
-        dev::Vmf_mixture mix;
-        if (guided_vertex) {
-            mix.set(w.guide_raw + 32 * size_t(i));
-        }
-
-        // one shadow ray per picked light (evaluate_light); a vertex's rays
-        // take consecutive queue slots, reserved with one atomic per wave
-        uint32_t const sbase = queue_range(num_picks, &w.counters[C_shadow]);
+        // next-event estimation (sample_lights / evaluate_light): a vertex's
+        // shadow rays take consecutive queue slots, reserved with one atomic
+        // per block.  The tree's picks go to those slots and the nee kernel
+        // makes their rays from the random numbers that follow the selection
+        // draw; the path's generator skips them here.
+        uint32_t const sbase = block_queue_range(num_picks, &w.counters[C_shadow]);
         if (i < count) {
             w.shadow_range[i] = make_uint2(sbase, num_picks);
         }
-        for (uint32_t k = 0; k < num_picks; ++k) {
-            bool   want = false;
-            mi_ray shadow_ray;
-            Shadow sh;
-            {
-                dev::Light_pick const pick = picks[k];
-                Light_info const      L    = w.tree.lights[pick.light];  // the part
-                // Mesh::sample (triangle_mesh.cpp:462-516): the part's tree
-                // picks a triangle, then a point on it
-                float const           rp   = path.rng.uniform();
-                dev::Light_pick const tp   = dev::part_random(w.parts, pick.light, p_o, geo_n, in_medium, rp);
-                Light_info const      T    = w.parts.lights[w.parts.parts[pick.light].light_offset + tp.light];
-                uint32_t const        lt   = T.triangle;
-                float const           u1   = path.rng.uniform();
-                float const           u2   = path.rng.uniform();
-
-                V3 const la = vertex(w, w.indices[3 * size_t(lt) + 0]);
-                V3 const lb = vertex(w, w.indices[3 * size_t(lt) + 1]);
-                V3 const lc = vertex(w, w.indices[3 * size_t(lt) + 2]);
-                // sample_triangle_uniform, base/math/sampling.inl:39-43
-                float const su = sqrtf(u1);
-                float const lu = 1.f - su;
-                float const lv = u2 * su;
-                float const lw = (1.f - lu) - lv;
-                V3 const    lp = (lw * la + lu * lb) + lv * lc;
-                V3 const    ln = dev::normalize(dev::cross(lb - la, lc - la));
-
-                V3 const    axis = offset_ray(lp, ln) - p_o;
-                float const sl   = dev::dot(axis, axis);
-                float const dist = sqrtf(sl);
-                V3 const    wi   = v3(axis.x / dist, axis.y / dist, axis.z / dist);
-                float const cl   = -dev::dot(ln, wi);
-                if (cl >= dev::Dot_min) {
-                    float f[4];
-                    s.evaluate(wi, f);
-                    float const light_pdf = (sl / (cl * L.area)) * (tp.pdf * (L.area / T.area)) * pick.pdf;
-                    float       dir_pdf   = f[3];
-                    if (guided_vertex) {
-                        dir_pdf = w.fraction * mix.pdf(wi) + (1.f - w.fraction) * f[3];
-                    }
-                    float const weight = predivided_power_heuristic(light_pdf, dir_pdf);
-                    Render_material const lm = w.materials[w.tri_material[lt]];
-                    for (int c = 0; c < 3; ++c) {
-                        sh.contrib[c] = path.throughput[c] * (weight * (lm.emission[c] * f[c]));
-                    }
-                    sh.path                 = p;
-                    shadow_ray.origin[0]    = p_o.x;
-                    shadow_ray.origin[1]    = p_o.y;
-                    shadow_ray.origin[2]    = p_o.z;
-                    shadow_ray.min_t        = 0.f;
-                    shadow_ray.direction[0] = wi.x;
-                    shadow_ray.direction[1] = wi.y;
-                    shadow_ray.direction[2] = wi.z;
-                    shadow_ray.max_t        = dist;
-                    want = sh.contrib[0] > 0.f || sh.contrib[1] > 0.f || sh.contrib[2] > 0.f;
-                }
+        if (num_picks) {
+            Nee_vertex v;
+            v.rng_state = rng.state;
+            v.rng_inc   = rng.inc;
+            v.p[0]      = p_o.x;
+            v.p[1]      = p_o.y;
+            v.p[2]      = p_o.z;
+            v.path      = p;
+            v.n[0]      = geo_n.x;
+            v.n[1]      = geo_n.y;
+            v.n[2]      = geo_n.z;
+            v.material  = in_medium ? Kind_phase : material;
+            v.wo[0]     = wo.x;
+            v.wo[1]     = wo.y;
+            v.wo[2]     = wo.z;
+            v.flags     = (guided_vertex ? Nee_guided : 0u) | (in_medium ? Nee_medium : 0u);
+            for (int c = 0; c < 3; ++c) {
+                v.throughput[c] = thr[c];
             }
-            if (!want) {
-                // nothing to add: an empty interval, resolved as not occluded
-                shadow_ray.origin[0] = shadow_ray.origin[1] = shadow_ray.origin[2] = 0.f;
-                shadow_ray.direction[0] = 0.f;
-                shadow_ray.direction[1] = 0.f;
-                shadow_ray.direction[2] = 1.f;
-                shadow_ray.min_t = 1.f;
-                shadow_ray.max_t = 0.f;
-                sh.contrib[0] = sh.contrib[1] = sh.contrib[2] = 0.f;
-                sh.path = p;
+            v.pad    = 0;
+            w.nee[i] = v;
+            uint32_t const made = dev::for_each_light(
+                w.tree, p_o, geo_n, in_medium, select, split, [&](uint32_t k, dev::Light_pick const pick) {
+                    if (k < num_picks) {
+                        w.picks[sbase + k] = Pick_slot{pick.light, pick.pdf, i, k};
+                    }
+                });
+            for (uint32_t k = made; k < num_picks; ++k) {
+                // unreachable while both descents agree: a slot without a ray
+                w.picks[sbase + k] = Pick_slot{dev::Null, 0.f, i, k};
             }
-            w.shadow_rays[sbase + k] = shadow_ray;
-            w.shadows[sbase + k]     = sh;
+            rng.advance(3 * uint64_t(num_picks));  // rp, u1, u2 per pick
         }
 
+        mi_ray next_ray;
+        bool   record   = false;
+        float  rec_t[3] = {0.f, 0.f, 0.f};
         if (live) {
+            Surface s;
+            if (in_medium) {
+                s.kind   = Kind_phase;
+                s.g      = w.medium.g;
+                s.sub.wo = wo;
+            } else {
+                Render_material const m = w.materials[material];
+                s.kind                  = m.kind;
+                s.sub.set(dev::frame_from_normal(geo_n), geo_n, wo, m.color, m.roughness, m.metallic,
+                          m.ior, 1.f, w.e_table);
+            }
             // BSDF importance sample; guided: one-sample mixture of the guide
             // (probability fraction) and the BSDF
             dev::Bxdf_sample bs;
             if (guided_vertex) {
-                float const pick = path.rng.uniform();
-                float const xl   = path.rng.uniform();
-                float const x0   = path.rng.uniform();
-                float const x1   = path.rng.uniform();
+                // the mixture goes to memory: this sample and the MIS weights
+                // of the vertex's shadow rays read it there
+                float* const mix = w.guide_mix + dev::Vmf_mixture::Stored * size_t(i);
+                {
+                    dev::Vmf_mixture m;
+                    m.set(w.guide_raw + 32 * size_t(i));
+                    m.store(mix);
+                }
+                float const pick = rng.uniform();
+                float const xl   = rng.uniform();
+                float const x0   = rng.uniform();
+                float const x1   = rng.uniform();
                 if (pick < w.fraction) {
-                    V3 const gw = mix.sample(xl, x0, x1);
+                    V3 const gw = dev::vmf_mixture_sample(mix, xl, x0, x1);
                     float    f[4];
                     s.evaluate(gw, f);
                     bs.wi      = gw;
@@ -933,38 +948,39 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
                     s.sample(xl, x0, x1, bs);
                 }
                 if (bs.pdf > 0.f) {
-                    bs.pdf = w.fraction * mix.pdf(bs.wi) + (1.f - w.fraction) * bs.pdf;
+                    bs.pdf = w.fraction * dev::vmf_mixture_pdf(mix, bs.wi) + (1.f - w.fraction) * bs.pdf;
                 }
                 if (bs.pdf > 0.f && depth < w.record_depth) {
                     record   = true;
-                    rec_t[0] = path.throughput[0];
-                    rec_t[1] = path.throughput[1];
-                    rec_t[2] = path.throughput[2];
+                    rec_t[0] = thr[0];
+                    rec_t[1] = thr[1];
+                    rec_t[2] = thr[2];
                 }
             } else {
-                float const xl = path.rng.uniform();
-                float const x0 = path.rng.uniform();
-                float const x1 = path.rng.uniform();
+                float const xl = rng.uniform();
+                float const x0 = rng.uniform();
+                float const x1 = rng.uniform();
                 s.sample(xl, x0, x1, bs);
             }
             if (bs.pdf > 0.f) {
                 if (dev::Type_specular == bs.type) {
-                    path.flags |= Flag_singular;
+                    flags |= Flag_singular;
                 } else {
-                    path.flags &= ~(Flag_singular | Flag_primary);
+                    flags &= ~(Flag_singular | Flag_primary);
                 }
                 if (in_medium) {
-                    path.flags |= Flag_translucent;
+                    flags |= Flag_translucent;
                 } else {
-                    path.flags &= ~Flag_translucent;
+                    flags &= ~Flag_translucent;
                 }
-                path.bsdf_pdf = bs.pdf;
                 for (int k = 0; k < 3; ++k) {
-                    path.throughput[k] *= bs.refl[k] / bs.pdf;
+                    thr[k] *= bs.refl[k] / bs.pdf;
                 }
-                path.prev_n[0]        = geo_n.x;
-                path.prev_n[1]        = geo_n.y;
-                path.prev_n[2]        = geo_n.z;
+                Path* const pp = &w.paths[p];
+                pp->bsdf_pdf   = bs.pdf;
+                pp->prev_n[0]  = geo_n.x;
+                pp->prev_n[1]  = geo_n.y;
+                pp->prev_n[2]  = geo_n.z;
                 // a scattering vertex spawns from the point itself (subsurface offset_p)
                 V3 const o = in_medium ? hit_p
                                        : offset_ray(hit_p, dev::dot(geo_n, bs.wi) > 0.f ? geo_n : -geo_n);
@@ -980,38 +996,158 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
             }
         }
 
-        if (Guided) {
-            // a fixed slot per (path, depth): the records, and so the
-            // training, do not depend on the order the queue was filled in
-            record                = record && spawn;
-            uint64_t const slot64 = uint64_t(p) * w.record_depth + depth;
-            if (record && slot64 < w.record_capacity) {
-                uint32_t const slot = uint32_t(slot64);
-                float const* xi = w.guide_x + 9 * size_t(i);
-                float*       xo = w.train_x + 9 * size_t(slot);
-                for (int k = 0; k < 9; ++k) {
-                    xo[k] = xi[k];
+        if (i < count) {
+            Path* const pp    = &w.paths[p];
+            pp->throughput[0] = thr[0];
+            pp->throughput[1] = thr[1];
+            pp->throughput[2] = thr[2];
+            pp->flags         = flags;
+            pp->rng           = rng;
+            if (Guided) {
+                // a fixed slot per (path, depth): the records, and so the
+                // training, do not depend on the order the queue was filled in
+                uint32_t       rec    = dev::Null;
+                uint64_t const slot64 = uint64_t(p) * w.record_depth + depth;
+                if (record && spawn && slot64 < w.record_capacity) {
+                    uint32_t const slot = uint32_t(slot64);
+                    float const*   xi   = w.guide_x + 9 * size_t(i);
+                    float*         xo   = w.train_x + 9 * size_t(slot);
+                    for (int k = 0; k < 9; ++k) {
+                        xo[k] = xi[k];
+                    }
+                    w.train_dir[3 * size_t(slot) + 0] = next_ray.direction[0];
+                    w.train_dir[3 * size_t(slot) + 1] = next_ray.direction[1];
+                    w.train_dir[3 * size_t(slot) + 2] = next_ray.direction[2];
+                    for (int k = 0; k < 3; ++k) {
+                        w.rec_t[3 * size_t(slot) + k] = rec_t[k];
+                        w.rec_r[3 * size_t(slot) + k] = pp->radiance[k];
+                    }
+                    w.rec_path[slot] = p;
+                    rec              = slot;
                 }
-                w.train_dir[3 * size_t(slot) + 0] = next_ray.direction[0];
-                w.train_dir[3 * size_t(slot) + 1] = next_ray.direction[1];
-                w.train_dir[3 * size_t(slot) + 2] = next_ray.direction[2];
-                for (int k = 0; k < 3; ++k) {
-                    w.rec_t[3 * size_t(slot) + k] = rec_t[k];
-                    w.rec_r[3 * size_t(slot) + k] = path.radiance[k];
-                }
-                w.rec_path[slot] = p;
-                path.record      = slot;
+                pp->record = rec;
             }
         }
-        if (i < count) {
-            w.paths[p] = path;
-        }
 
-        uint32_t const ray_slot = queue_slot(spawn, &w.counters[C_queue0 + out_q]);
+        uint32_t const ray_slot = block_queue_range(spawn ? 1u : 0u, &w.counters[C_queue0 + Counter_line * out_q]);
         if (spawn) {
             w.rays[out_q][ray_slot]     = next_ray;
             w.ray_path[out_q][ray_slot] = p;
         }
+    }
+}
+
+// 127 VGPRs: four waves per SIMD
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) shade(Wave w, uint32_t in_q,
+                                                                                   uint32_t depth) {
+    shade_queue<false>(w, in_q, depth);
+}
+
+__global__ void __launch_bounds__(256) shade_guided(Wave w, uint32_t in_q, uint32_t depth) {
+    shade_queue<true>(w, in_q, depth);
+}
+
+// The shadow ray of every light pick of the bounce (evaluate_light,
+// pathtracer_mis.cpp:323-358), one thread per pick: Mesh::sample picks a
+// triangle of the part and a point on it (3 random numbers of the path's
+// generator, pick k taking numbers 3k..3k+2 after the vertex's selection
+// draw), the vertex's reflection toward it and the MIS weight against the
+// vertex's direction sampling (with the guide's mixture at a guided vertex).
+// A ray with nothing to add gets an empty interval.
+__global__ void __launch_bounds__(256) nee(Wave w) {
+    uint32_t const count  = w.counters[C_shadow];
+    uint32_t const stride = gridDim.x * blockDim.x;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
+        Pick_slot const  pick = w.picks[j];
+        Nee_vertex const v    = w.nee[pick.vertex];
+        bool             want = false;
+        mi_ray           shadow_ray;
+        Shadow           sh;
+        if (dev::Null != pick.light) {
+            Rng rng;
+            rng.state = v.rng_state;
+            rng.inc   = v.rng_inc;
+            rng.advance(3 * uint64_t(pick.k));
+            bool const in_medium = 0 != (v.flags & Nee_medium);
+            V3 const   p_o       = v3(v.p[0], v.p[1], v.p[2]);
+            V3 const   geo_n     = v3(v.n[0], v.n[1], v.n[2]);
+            Surface    s;
+            if (in_medium) {
+                s.kind   = Kind_phase;
+                s.g      = w.medium.g;
+                s.sub.wo = v3(v.wo[0], v.wo[1], v.wo[2]);
+            } else {
+                Render_material const m = w.materials[v.material];
+                s.kind                  = m.kind;
+                s.sub.set(dev::frame_from_normal(geo_n), geo_n, v3(v.wo[0], v.wo[1], v.wo[2]),
+                          m.color, m.roughness, m.metallic, m.ior, 1.f, w.e_table);
+            }
+
+            Light_info const L = w.tree.lights[pick.light];  // the part
+            // Mesh::sample (triangle_mesh.cpp:462-516): the part's tree
+            // picks a triangle, then a point on it
+            float const           rp = rng.uniform();
+            dev::Light_pick const tp = dev::part_random(w.parts, pick.light, p_o, geo_n, in_medium, rp);
+            Light_info const      T  = w.parts.lights[w.parts.parts[pick.light].light_offset + tp.light];
+            uint32_t const        lt = T.triangle;
+            float const           u1 = rng.uniform();
+            float const           u2 = rng.uniform();
+
+            V3 const la = vertex(w, w.indices[3 * size_t(lt) + 0]);
+            V3 const lb = vertex(w, w.indices[3 * size_t(lt) + 1]);
+            V3 const lc = vertex(w, w.indices[3 * size_t(lt) + 2]);
+            // sample_triangle_uniform, base/math/sampling.inl:39-43
+            float const su = sqrtf(u1);
+            float const lu = 1.f - su;
+            float const lv = u2 * su;
+            float const lw = (1.f - lu) - lv;
+            V3 const    lp = (lw * la + lu * lb) + lv * lc;
+            V3 const    ln = dev::normalize(dev::cross(lb - la, lc - la));
+
+            V3 const    axis = offset_ray(lp, ln) - p_o;
+            float const sl   = dev::dot(axis, axis);
+            float const dist = sqrtf(sl);
+            V3 const    wi   = v3(axis.x / dist, axis.y / dist, axis.z / dist);
+            float const cl   = -dev::dot(ln, wi);
+            if (cl >= dev::Dot_min) {
+                float f[4];
+                s.evaluate(wi, f);
+                float const light_pdf = (sl / (cl * L.area)) * (tp.pdf * (L.area / T.area)) * pick.pdf;
+                float       dir_pdf   = f[3];
+                if (v.flags & Nee_guided) {
+                    float const* mix = w.guide_mix + dev::Vmf_mixture::Stored * size_t(pick.vertex);
+                    dir_pdf = w.fraction * dev::vmf_mixture_pdf(mix, wi) + (1.f - w.fraction) * f[3];
+                }
+                float const           weight = predivided_power_heuristic(light_pdf, dir_pdf);
+                Render_material const lm     = w.materials[w.tri_material[lt]];
+                for (int c = 0; c < 3; ++c) {
+                    sh.contrib[c] = v.throughput[c] * (weight * (lm.emission[c] * f[c]));
+                }
+                sh.path                 = v.path;
+                shadow_ray.origin[0]    = p_o.x;
+                shadow_ray.origin[1]    = p_o.y;
+                shadow_ray.origin[2]    = p_o.z;
+                shadow_ray.min_t        = 0.f;
+                shadow_ray.direction[0] = wi.x;
+                shadow_ray.direction[1] = wi.y;
+                shadow_ray.direction[2] = wi.z;
+                shadow_ray.max_t        = dist;
+                want = sh.contrib[0] > 0.f || sh.contrib[1] > 0.f || sh.contrib[2] > 0.f;
+            }
+        }
+        if (!want) {
+            // nothing to add: an empty interval, resolved as not occluded
+            shadow_ray.origin[0] = shadow_ray.origin[1] = shadow_ray.origin[2] = 0.f;
+            shadow_ray.direction[0] = 0.f;
+            shadow_ray.direction[1] = 0.f;
+            shadow_ray.direction[2] = 1.f;
+            shadow_ray.min_t        = 1.f;
+            shadow_ray.max_t        = 0.f;
+            sh.contrib[0] = sh.contrib[1] = sh.contrib[2] = 0.f;
+            sh.path                 = v.path;
+        }
+        w.shadow_rays[j] = shadow_ray;
+        w.shadows[j]     = sh;
     }
 }
 
@@ -1022,7 +1158,7 @@ __global__ void __launch_bounds__(256) shade(Wave w, uint32_t in_q, uint32_t dep
 // transmittance, rendering_worker.cpp:143-153), drawing from the path's
 // generator after the vertex's direction sample.
 __global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q, uint32_t depth) {
-    uint32_t const count  = w.counters[C_queue0 + in_q];
+    uint32_t const count  = w.counters[C_queue0 + Counter_line * in_q];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         uint2 const range = w.shadow_range[i];
@@ -1446,17 +1582,17 @@ __global__ void light_tree_sample_batch(dev::Light_tree_view t, float const* poi
     if (i >= n) {
         return;
     }
-    dev::Light_pick out[dev::Max_lights];
-    V3 const        p = v3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
-    V3 const        nn = v3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
-    uint32_t const  c = dev::random_light(t, p, nn, 0 != (split & MI_LIGHT_TOTAL_SPHERE), randoms[i],
-                                          0 != (split & MI_LIGHT_SPLIT), out);
-    counts[i]         = c;
-    uint32_t* o       = picks + 2 * size_t(i) * dev::Max_lights;
-    for (uint32_t k = 0; k < c; ++k) {
-        o[2 * k]     = out[k].light;
-        o[2 * k + 1] = __float_as_uint(out[k].pdf);
-    }
+    V3 const       p  = v3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
+    V3 const       nn = v3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+    uint32_t*      o  = picks + 2 * size_t(i) * dev::Max_lights;
+    uint32_t const c  = dev::for_each_light(t, p, nn, 0 != (split & MI_LIGHT_TOTAL_SPHERE), randoms[i],
+                                            0 != (split & MI_LIGHT_SPLIT),
+                                            [o](uint32_t k, dev::Light_pick const pick) {
+                                                o[2 * k]     = pick.light;
+                                                o[2 * k + 1] = __float_as_uint(pick.pdf);
+                                            });
+    // the count the shading kernels reserve shadow rays by
+    counts[i] = c == dev::count_lights(t, p, 0 != (split & MI_LIGHT_SPLIT)) ? c : dev::Null;
 }
 
 __global__ void light_tree_pdf_batch(dev::Light_tree_view t, float const* points,
@@ -2233,9 +2369,12 @@ void RenderState::release() {
     (void)hipFree(shadow_range);
     (void)hipFree(medium_event);
     (void)hipFree(medium_p);
+    (void)hipFree(nee);
+    (void)hipFree(picks);
     (void)hipFree(counters);
     (void)hipFree(guide_x);
     (void)hipFree(guide_raw);
+    (void)hipFree(guide_mix);
     (void)hipFree(train_x);
     (void)hipFree(train_dir);
     (void)hipFree(train_w);
@@ -2280,13 +2419,16 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
         (void)hipFree(shadow_rays);
         (void)hipFree(shadows);
         (void)hipFree(occluded);
+        (void)hipFree(picks);
         shadow_rays     = nullptr;
         shadows         = nullptr;
         occluded        = nullptr;
+        picks           = nullptr;
         shadow_capacity = 0;
         hipError_t e    = hipMalloc(&shadow_rays, size_t(num_shadow) * sizeof(mi_ray));
         if (!e) e = hipMalloc(&shadows, size_t(num_shadow) * sizeof(Shadow));
         if (!e) e = hipMalloc(&occluded, size_t(num_shadow) * sizeof(uint32_t));
+        if (!e) e = hipMalloc(&picks, size_t(num_shadow) * sizeof(Pick_slot));
         if (e) {
             return fail(MI_ERR_HIP, "shadow queue of %u rays: %s", num_shadow, hipGetErrorString(e));
         }
@@ -2304,6 +2446,7 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
     (void)hipFree(shadow_range);
     (void)hipFree(medium_event);
     (void)hipFree(medium_p);
+    (void)hipFree(nee);
     capacity     = 0;
     hipError_t e = hipMalloc(&paths, size_t(n) * sizeof(Path));
     for (int q = 0; q < 2 && !e; ++q) {
@@ -2317,6 +2460,7 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
     if (!e) e = hipMalloc(&shadow_range, size_t(n) * sizeof(uint2));
     if (!e) e = hipMalloc(&medium_event, size_t(n) * sizeof(uint32_t));
     if (!e) e = hipMalloc(&medium_p, size_t(n) * 3 * sizeof(float));
+    if (!e) e = hipMalloc(&nee, size_t(n) * sizeof(Nee_vertex));
     if (e) {
         release();
         return fail(MI_ERR_HIP, "render state of %u paths: %s", n, hipGetErrorString(e));
@@ -2350,10 +2494,12 @@ int32_t RenderState::reserve_guiding(uint32_t n, uint32_t records) {
     if (n > guide_capacity) {
         (void)hipFree(guide_x);
         (void)hipFree(guide_raw);
-        guide_x = guide_raw = nullptr;
+        (void)hipFree(guide_mix);
+        guide_x = guide_raw = guide_mix = nullptr;
         guide_capacity      = 0;
         hipError_t e        = hipMalloc(&guide_x, size_t(n) * 9 * sizeof(float));
         if (!e) e = hipMalloc(&guide_raw, size_t(n) * 32 * sizeof(float));
+        if (!e) e = hipMalloc(&guide_mix, size_t(n) * dev::Vmf_mixture::Stored * sizeof(float));
         if (e) {
             return fail(MI_ERR_HIP, "guiding buffers of %u paths: %s", n, hipGetErrorString(e));
         }
@@ -2555,6 +2701,9 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.shadows     = static_cast<Shadow*>(state.shadows);
     w.occluded    = state.occluded;
     w.shadow_range = state.shadow_range;
+    w.nee          = static_cast<Nee_vertex*>(state.nee);
+    w.picks        = static_cast<Pick_slot*>(state.picks);
+    w.guide_mix    = state.guide_mix;
     w.counters    = state.counters;
     w.fraction        = guided ? gs->fraction : 0.f;
     w.guide_raw       = state.guide_raw;
@@ -2578,6 +2727,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     hipStream_t const s         = state.stream;
     uint32_t const    waves     = set.samples_per_pixel / spw;
     uint32_t const    grid_path = std::min<uint32_t>(blocks(num_paths, 256), trace.num_cus * 8);
+    uint32_t const    grid_shadow = std::min<uint32_t>(blocks(num_shadow, 256), trace.num_cus * 8);
     auto const        start     = std::chrono::steady_clock::now();
 
     // traversals with their ray counts on the device: the scene's tree, or
@@ -2609,7 +2759,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         hipLaunchKernelGGL(raygen, dim3(blocks(num_paths, 256)), dim3(256), 0, s, w);
         uint32_t q = 0;
         for (uint32_t d = 0; d <= set.max_bounces; ++d) {
-            if (int32_t const st = trace_closest(state.rays[q], &state.counters[C_queue0 + q]); st) {
+            if (int32_t const st = trace_closest(state.rays[q], &state.counters[C_queue0 + Counter_line * q]); st) {
                 return st;
             }
             if (scene.medium.present) {
@@ -2619,18 +2769,19 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
             if (guided && d < guide_depth) {
                 hipLaunchKernelGGL(guide_inputs, dim3(grid_path), dim3(256), 0, s, w, q);
                 if (int32_t const st = guide_forward_device_count(
-                        *net, state.guide_x, &state.counters[C_queue0 + q], num_paths,
+                        *net, state.guide_x, &state.counters[C_queue0 + Counter_line * q], num_paths,
                         state.guide_raw, s);
                     st) {
                     return st;
                 }
-                hipLaunchKernelGGL(shade<true>, dim3(grid_path), dim3(256), 0, s, w, q, d);
+                hipLaunchKernelGGL(shade_guided, dim3(grid_path), dim3(256), 0, s, w, q, d);
             } else if (guided && d <= record_depth) {
                 // BSDF sampling; closes the records of the vertex before
-                hipLaunchKernelGGL(shade<true>, dim3(grid_path), dim3(256), 0, s, w, q, d);
+                hipLaunchKernelGGL(shade_guided, dim3(grid_path), dim3(256), 0, s, w, q, d);
             } else {
-                hipLaunchKernelGGL(shade<false>, dim3(grid_path), dim3(256), 0, s, w, q, d);
+                hipLaunchKernelGGL(shade, dim3(grid_path), dim3(256), 0, s, w, q, d);
             }
+            hipLaunchKernelGGL(nee, dim3(grid_shadow), dim3(256), 0, s, w);
             hipLaunchKernelGGL(tally, dim3(1), dim3(1), 0, s, state.counters, q);
             if (int32_t const st = trace_any(); st) {
                 return st;

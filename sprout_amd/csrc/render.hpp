@@ -143,6 +143,8 @@ struct RenderState {
     void*        shadows     = nullptr;
     uint32_t*    occluded    = nullptr;
     uint2*       shadow_range = nullptr;
+    void*        nee          = nullptr;  // per queue entry: what its shadow rays need
+    void*        picks        = nullptr;  // per shadow ray: its light pick
     uint32_t*    medium_event = nullptr;  // per queue entry
     float*       medium_p     = nullptr;  // 3 per queue entry
     uint32_t*    counters    = nullptr;  // queue counters, wave base, ray totals
@@ -155,6 +157,7 @@ struct RenderState {
     uint32_t  guide_capacity  = 0;
     float*    guide_x         = nullptr;  // network inputs per queue entry
     float*    guide_raw       = nullptr;  // network outputs per queue entry
+    float*    guide_mix       = nullptr;  // prepared mixtures of guided vertices with light picks
     uint32_t  record_capacity = 0;
     float*    train_x         = nullptr;
     float*    train_dir       = nullptr;

@@ -36,6 +36,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace mi {
 
 namespace {
@@ -96,7 +98,7 @@ struct Launch {
     uint32_t*     spill;
     uint32_t      total_threads;
     uint32_t      lds_entries;
-    uint32_t*     counter;     // next unclaimed ray (trace_refill)
+    uint32_t*     counter;     // claim counters of trace_refill: Claim_parts words, Claim_stride apart
     uint32_t      refill_lanes;
     uint32_t      claim_batch;
 };
@@ -399,6 +401,52 @@ __device__ __forceinline__ bool inner_step(float4 const* __restrict__ nodes, Sta
     return pop_next(nodes, stack, ray, data, meta);
 }
 
+// The ray claims of the refill kernels.  The rays are split into
+// Claim_parts equal parts, each with its own counter word on its own 256-byte
+// line (Claim_stride words apart).  A wave starts on part (wave mod
+// Claim_parts) and moves to the next part once its part is used up.
+// Atomics on one address serialize (about 12 ns each), so spreading the
+// claims over the parts keeps them off the critical path.
+struct Claims {
+    uint32_t part;   // wave-uniform: the part claimed from next
+    uint32_t tried;  // parts found used up (they stay used up)
+
+    __device__ void init() {
+        part  = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % Claim_parts;
+        tried = 0;
+    }
+
+    // Claims up to batch rays [begin, end) for the wave; the leader lane
+    // issues the atomic.  False once every part is used up.
+    __device__ bool next(uint32_t* counters, uint32_t num_rays, uint32_t batch, uint32_t lane,
+                         uint32_t leader, uint32_t& begin, uint32_t& end) {
+        uint64_t const size = (uint64_t(num_rays) + Claim_parts - 1) / Claim_parts;
+        while (tried < Claim_parts) {
+            uint64_t const  lo_part = uint64_t(part) * size;
+            uint64_t const  lo      = lo_part < num_rays ? lo_part : num_rays;
+            uint64_t const  hi      = lo + size < num_rays ? lo + size : num_rays;
+            uint32_t* const c       = counters + part * Claim_stride;
+            uint32_t        taken   = 0;
+            if (lane == leader) {
+                // a part seen used up costs no atomic
+                taken = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (lo + taken < hi) {
+                    taken = atomicAdd(c, batch);
+                }
+            }
+            taken = __shfl(taken, leader);
+            if (lo + taken < hi) {
+                begin = uint32_t(lo + taken);
+                end   = uint32_t(lo + taken + batch < hi ? lo + taken + batch : hi);
+                return true;
+            }
+            part = part + 1 == Claim_parts ? 0u : part + 1;
+            ++tried;
+        }
+        return false;
+    }
+};
+
 // Persistent traversal with per-lane ray refill: a lane whose ray is done
 // takes the next unclaimed ray, so the 64 lanes of a wave stay busy while
 // rays of very different lengths finish.  Each wave claims rays in batches
@@ -421,6 +469,8 @@ __global__ void trace_refill(Launch p) {
     uint64_t const lt_mask  = (uint64_t(1) << lane) - 1;
     uint32_t const num_rays = p.num_rays_dev ? *p.num_rays_dev : p.num_rays;
 
+    Claims   claims;
+    claims.init();
     bool     active    = false;
     bool     exhausted = false;  // the global counter is past the last ray (wave-uniform)
     uint32_t pool_next = 0;      // wave-uniform pool of claimed rays [pool_next, pool_end)
@@ -445,19 +495,10 @@ __global__ void trace_refill(Launch p) {
                     }
                     uint32_t const leader =
                         uint32_t(__ffsll(static_cast<unsigned long long>(idle))) - 1;
-                    uint32_t base = 0;
-                    if (lane == leader) {
-                        base = atomicAdd(p.counter, p.claim_batch);
-                    }
-                    base = __shfl(base, leader);
-                    if (base >= num_rays) {
+                    if (!claims.next(p.counter, num_rays, p.claim_batch, lane, leader, pool_next,
+                                     pool_end)) {
                         exhausted = true;
                         break;
-                    }
-                    pool_next = base;
-                    pool_end  = num_rays - base <= p.claim_batch ? num_rays : base + p.claim_batch;
-                    if (pool_end == num_rays) {
-                        exhausted = true;
                     }
                 }
                 uint32_t const avail = pool_end - pool_next;
@@ -651,6 +692,8 @@ __global__ void trace_refill_packed(Launch p) {
     uint64_t const lt_mask  = (uint64_t(1) << lane) - 1;
     uint32_t const num_rays = p.num_rays_dev ? *p.num_rays_dev : p.num_rays;
 
+    Claims   claims;
+    claims.init();
     bool     active    = false;
     bool     exhausted = false;
     uint32_t pool_next = 0;
@@ -675,19 +718,10 @@ __global__ void trace_refill_packed(Launch p) {
                     }
                     uint32_t const leader =
                         uint32_t(__ffsll(static_cast<unsigned long long>(idle))) - 1;
-                    uint32_t base = 0;
-                    if (lane == leader) {
-                        base = atomicAdd(p.counter, p.claim_batch);
-                    }
-                    base = __shfl(base, leader);
-                    if (base >= num_rays) {
+                    if (!claims.next(p.counter, num_rays, p.claim_batch, lane, leader, pool_next,
+                                     pool_end)) {
                         exhausted = true;
                         break;
-                    }
-                    pool_next = base;
-                    pool_end  = num_rays - base <= p.claim_batch ? num_rays : base + p.claim_batch;
-                    if (pool_end == num_rays) {
-                        exhausted = true;
                     }
                 }
                 uint32_t const avail = pool_end - pool_next;
@@ -754,7 +788,9 @@ __global__ void trace_refill_packed(Launch p) {
 }
 
 __global__ void reset_counter(uint32_t* counter) {
-    *counter = 0;
+    if (threadIdx.x < Claim_parts) {
+        counter[threadIdx.x * Claim_stride] = 0;
+    }
 }
 
 }  // namespace
@@ -812,6 +848,8 @@ int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, ui
     if (int32_t const st = res.reserve_spill(size_t(sl.words) * sl.spill_entries * threads); st) {
         return st;
     }
+    uint32_t* const spill   = res.spill;
+    uint32_t* const counter = res.counter;
 
     size_t const lds_bytes = size_t(sl.words) * lds_entries * block * sizeof(uint32_t);
 
@@ -823,10 +861,10 @@ int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, ui
     p.num_rays_dev  = num_rays_dev;
     p.out           = out;
     p.counts        = counts;
-    p.spill         = res.spill;
+    p.spill         = spill;
     p.total_threads = threads;
     p.lds_entries   = lds_entries;
-    p.counter       = res.counter;
+    p.counter       = counter;
     p.refill_lanes  = res.refill_lanes;
     p.claim_batch   = res.claim_batch;
 
@@ -837,14 +875,14 @@ int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, ui
             hipLaunchKernelGGL(trace_reference<false>, dim3(blocks), dim3(block), lds_bytes, stream, p);
         }
     } else if (packed) {
-        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, res.counter);
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, stream, counter);
         if (any_hit) {
             hipLaunchKernelGGL(trace_refill_packed<true>, dim3(blocks), dim3(block), lds_bytes, stream, p);
         } else {
             hipLaunchKernelGGL(trace_refill_packed<false>, dim3(blocks), dim3(block), lds_bytes, stream, p);
         }
     } else if (res.kernel != 1 || num_rays_dev) {
-        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, res.counter);
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, stream, counter);
         using Kernel    = void (*)(Launch);
         bool const ww   = 2 <= res.kernel;
         Kernel     kern = any_hit ? (ww ? trace_refill<true, true> : trace_refill<true, false>)

@@ -29,6 +29,12 @@ struct DeviceBvh {
     void release();
 };
 
+// The refill traversals claim rays from Claim_parts counter words, each on
+// its own 256-byte line: a ray counter buffer holds Counter_words words.
+inline constexpr uint32_t Claim_parts   = 32;
+inline constexpr uint32_t Claim_stride  = 64;
+inline constexpr uint32_t Counter_words = Claim_parts * Claim_stride;
+
 // Persistent-grid launch state shared by all traversals of a context.
 struct TraceResources {
     uint32_t  block_size    = 256;
@@ -39,7 +45,7 @@ struct TraceResources {
     uint32_t  refill_lanes  = 8;     // hand out rays once this many lanes of a wave are idle
     uint32_t  claim_batch   = 64;    // rays a wave claims from the global counter at a time
     uint32_t  lds_entries   = 10;    // stack entries kept in LDS (0: up to 32); deeper ones spill
-    uint32_t* counter       = nullptr;  // ray counter of trace_refill (one word)
+    uint32_t* counter       = nullptr;  // ray claim counters of trace_refill (Counter_words)
     uint32_t* spill         = nullptr;
     size_t    spill_words   = 0;
 
