@@ -135,6 +135,21 @@ int32_t mi_bvh_download(uint32_t bvh, void* nodes, uint32_t* slots);
 int32_t mi_trace_closest(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, mi_hit* hits,
                          void* stream);
 
+/*
+ * mi_trace_closest without device allocation.  No counterpart in sprout: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The ray counter and the traversal stack's spill area come from a fixed
+ * 32 MiB device array of the library (each GPU holds its own until the
+ * process ends; mi_init looks up its address).  The results equal
+ * mi_trace_closest's; the persistent grid shrinks to what the array holds.
+ * MI_ERR_ARGS when the tree's stack (max_depth + 2 entries, those beyond
+ * the LDS part spilled) needs more than the array for one workgroup.
+ * Calls that overlap on one GPU (other streams or threads) race on the array.
+ */
+int32_t mi_trace_closest_noalloc(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, mi_hit* hits,
+                                 void* stream);
+
 /* Any hit: occluded[i] = 1 if ray i hits anything in [min_t, max_t], else 0.
  * Replaces bvh::Tree::intersect_p (triangle_bvh_tree.cpp:145-185). */
 int32_t mi_trace_any(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, uint32_t* occluded,

@@ -132,6 +132,7 @@ SIGNATURES = {
     "mi_bvh_get_info": (_i32, [_u32, ctypes.POINTER(MiBvhInfo)]),
     "mi_bvh_download": (_i32, [_u32, _vp, _vp]),
     "mi_trace_closest": (_i32, [_u32, _vp, _u32, _vp, _vp]),
+    "mi_trace_closest_noalloc": (_i32, [_u32, _vp, _u32, _vp, _vp]),
     "mi_trace_any": (_i32, [_u32, _vp, _u32, _vp, _vp]),
     "mi_trace_counts": (_i32, [_u32, _vp, _u32, _i32, _vp, _vp, _vp]),
     "mi_set_trace_config": (_i32, [ctypes.POINTER(MiTraceConfig)]),

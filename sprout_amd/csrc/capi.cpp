@@ -132,6 +132,11 @@ int32_t mi_init(int32_t device) {
         return fail(MI_ERR_HIP, "mi_init: %s", hipGetErrorString(e));
     }
 
+    // the fixed scratch of mi_trace_closest_noalloc on this (current) device
+    if (int32_t const st = noalloc_scratch_setup(device); st) {
+        return st;
+    }
+
     uint32_t* counter = nullptr;
     if (hipError_t const e = hipMalloc(&counter, Counter_words * sizeof(uint32_t)); e != hipSuccess) {
         return fail(MI_ERR_HIP, "mi_init: %s", hipGetErrorString(e));
@@ -309,6 +314,21 @@ int32_t mi_trace_closest(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, mi
     }
     return launch_trace(*b, context->trace, rays, num_rays, hits, nullptr, false,
                         as_stream(stream));
+}
+
+int32_t mi_trace_closest_noalloc(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, mi_hit* hits,
+                                 void* stream) {
+    clear_error();
+    DeviceBvh const* b = lookup(bvh);
+    if (!b) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT,
+                    "mi_trace_closest_noalloc: bad handle %u", bvh);
+    }
+    if (num_rays && (!rays || !hits)) {
+        return fail(MI_ERR_ARGS, "mi_trace_closest_noalloc: null buffers");
+    }
+    return launch_trace_noalloc(*b, context->trace, context->device, rays, num_rays, hits,
+                                as_stream(stream));
 }
 
 int32_t mi_trace_any(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, uint32_t* occluded,

@@ -822,11 +822,21 @@ Stack_layout stack_layout(DeviceBvh const& bvh, TraceResources const& res, bool 
     return l;
 }
 
+// A fixed scratch area: the ray counter and a stack spill area of a given
+// capacity, which the launch fits its persistent grid into.
+struct Fixed_scratch {
+    uint32_t* counter;
+    uint32_t* spill;
+    size_t    spill_words;
+};
+
 // num_rays_dev set: the refill kernel reads the ray count on the device and
 // num_rays is only its upper bound (it sizes the grid and the stack spill).
+// fixed set: the launch allocates nothing and uses fixed's counter and spill
+// area, with as many workgroups (at most the usual grid) as the area holds.
 int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, uint32_t num_rays,
                uint32_t const* num_rays_dev, void* out, uint32_t* counts, bool any_hit,
-               hipStream_t stream) {
+               hipStream_t stream, Fixed_scratch const* fixed = nullptr) {
     if (0 == num_rays) {
         return MI_OK;
     }
@@ -838,18 +848,37 @@ int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, ui
                     bvh.max_depth, entries, Reference_stack_size);
     }
 
-    uint32_t const block   = res.block_size;
-    uint32_t const blocks  = res.grid_blocks(num_rays);
-    uint32_t const threads = blocks * block;
+    uint32_t const block  = res.block_size;
+    uint32_t       blocks = res.grid_blocks(num_rays);
 
     Stack_layout const sl          = stack_layout(bvh, res, counts != nullptr);
     uint32_t const     lds_entries = sl.lds_entries;
     bool const         packed      = 2 == sl.words;
-    if (int32_t const st = res.reserve_spill(size_t(sl.words) * sl.spill_entries * threads); st) {
-        return st;
+    size_t const       lane_words  = size_t(sl.words) * sl.spill_entries;
+    uint32_t*          spill       = nullptr;
+    uint32_t*          counter     = nullptr;
+    if (fixed) {
+        if (lane_words) {
+            size_t const fit = fixed->spill_words / (lane_words * block);
+            if (0 == fit) {
+                return fail(MI_ERR_ARGS,
+                            "tree depth %u spills %zu bytes of traversal stack per workgroup of %u, "
+                            "more than the fixed scratch's %zu bytes",
+                            bvh.max_depth, lane_words * block * sizeof(uint32_t), block,
+                            fixed->spill_words * sizeof(uint32_t));
+            }
+            blocks = uint32_t(std::min<size_t>(blocks, fit));
+        }
+        spill   = fixed->spill;
+        counter = fixed->counter;
+    } else {
+        if (int32_t const st = res.reserve_spill(lane_words * blocks * block); st) {
+            return st;
+        }
+        spill   = res.spill;
+        counter = res.counter;
     }
-    uint32_t* const spill   = res.spill;
-    uint32_t* const counter = res.counter;
+    uint32_t const threads = blocks * block;
 
     size_t const lds_bytes = size_t(sl.words) * lds_entries * block * sizeof(uint32_t);
 
@@ -924,6 +953,53 @@ int32_t launch_trace_device_count(DeviceBvh const& bvh, TraceResources& res, mi_
                                   uint32_t const* num_rays, uint32_t max_rays, void* out,
                                   bool any_hit, hipStream_t stream) {
     return launch(bvh, res, rays, max_rays, num_rays, out, nullptr, any_hit, stream);
+}
+
+// ---- closest hit without allocation ----
+
+namespace {
+
+// Scratch of the traversal that allocates nothing: the ray claim counters
+// (Claim_parts lines of 256 bytes), then the stack spill area.  Uninitialised device data:
+// every GPU the library runs on holds its own copy until the process ends.
+// In host code `scratch` names a host-side stand-in; kernels get its device
+// address, looked up per device at set-up.
+constexpr size_t Noalloc_scratch_bytes = size_t(32) << 20;
+constexpr size_t Noalloc_counter_words = size_t(Claim_parts) * Claim_stride;
+static __device__ __align__(256) uint32_t scratch[Noalloc_scratch_bytes / sizeof(uint32_t)];
+
+constexpr int Noalloc_max_devices = 64;
+uint32_t*     scratch_address[Noalloc_max_devices] = {};
+
+}  // namespace
+
+int32_t noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= Noalloc_max_devices) {
+        return fail(MI_ERR_ARGS, "fixed traversal scratch: device %d out of range", device);
+    }
+    if (scratch_address[device]) {
+        return MI_OK;
+    }
+    void* ptr = nullptr;
+    if (hipError_t const e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)); e != hipSuccess) {
+        return fail(MI_ERR_HIP, "fixed traversal scratch on device %d: %s", device,
+                    hipGetErrorString(e));
+    }
+    scratch_address[device] = static_cast<uint32_t*>(ptr);
+    return MI_OK;
+}
+
+int32_t launch_trace_noalloc(DeviceBvh const& bvh, TraceResources& res, int device,
+                             mi_ray const* rays, uint32_t num_rays, mi_hit* hits,
+                             hipStream_t stream) {
+    uint32_t* const base =
+        (device >= 0 && device < Noalloc_max_devices) ? scratch_address[device] : nullptr;
+    if (!base) {
+        return fail(MI_ERR_NO_CONTEXT, "fixed traversal scratch not set up on device %d", device);
+    }
+    Fixed_scratch const fixed{base, base + Noalloc_counter_words,
+                              Noalloc_scratch_bytes / sizeof(uint32_t) - Noalloc_counter_words};
+    return launch(bvh, res, rays, num_rays, nullptr, hits, nullptr, false, stream, &fixed);
 }
 
 }  // namespace mi

@@ -88,4 +88,15 @@ int32_t launch_trace_device_count(DeviceBvh const& bvh, TraceResources& res, mi_
                                   uint32_t const* num_rays, uint32_t max_rays, void* out,
                                   bool any_hit, hipStream_t stream);
 
+// Sets up the fixed scratch of launch_trace_noalloc on `device`, which must
+// be the current device (once per device; later calls return at once).
+int32_t noalloc_scratch_setup(int device);
+
+// Closest hit as launch_trace, allocating nothing: the ray counters and stack
+// spill live in the library's fixed 32 MiB device array (the persistent grid
+// shrinks to fit it).  Calls that overlap on one device race on that array.
+int32_t launch_trace_noalloc(DeviceBvh const& bvh, TraceResources& res, int device,
+                             mi_ray const* rays, uint32_t num_rays, mi_hit* hits,
+                             hipStream_t stream);
+
 }  // namespace mi
