@@ -24,6 +24,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -38,6 +39,22 @@ PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "trace_closest_c1.md")
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+def host_threads() -> int:
+    """Threads for the CPU baselines: what `nproc` reports, i.e. the
+    processing units this process may use (its CPU affinity, limited by
+    OMP_NUM_THREADS where that is set)."""
+    try:
+        return max(1, int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                         check=True).stdout.strip()))
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        return len(os.sched_getaffinity(0))
+
+
+def host_info() -> dict:
+    return {"nproc": host_threads(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "os_cpu_count": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def timed_launches(fn, steps: int, warmup: int):
@@ -152,7 +169,7 @@ def room_render(frames: int, cpu: bool):
     if cpu:
         from oracle import oracle as O
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         t0 = time.perf_counter()
         from sprout_amd import lights
 
@@ -209,7 +226,7 @@ def cloud_render(frames: int, cpu: bool):
         from oracle import oracle as O
         from sprout_amd import lights
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         om = O.Medium(grid, *scenes.CLOUD_BOX, scenes.CLOUD_CC_A, scenes.CLOUD_CC_S, scenes.CLOUD_G)
         tree = O.scene_light_tree(pos, idx, tmat, mats, lights.build_tree)
         t0 = time.perf_counter()
@@ -491,6 +508,8 @@ def main() -> int:
                                        "batch_instanced", "guide_kernels"],
                     default="all",
                     help="run one workload only and print its figures (for profiling)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the traversal benchmark's line, without the other workloads' figures")
     ap.add_argument("--copy-probe", action="store_true",
                     help="launch one 256 MiB device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -508,6 +527,13 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     bvh.init(0)
 
+    if args.copy_probe:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     if args.part != "all":
         parts = {"room": lambda: room_render(5, False), "volume": lambda: cloud_render(3, False),
                  "batch": lambda: batch_render(2, False),
@@ -517,13 +543,6 @@ def main() -> int:
                  "instances": lambda: instance_trace(args.steps, args.warmup)}
         print(json.dumps({"part": args.part, "figures": parts[args.part]()}), flush=True)
         return 0
-
-    if args.copy_probe:
-        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
 
     log("generating mesh")
     pos, idx = scenes.displaced_icosphere()
@@ -586,7 +605,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         log(f"CPU baseline on {threads} threads")
         cpu = cpu_baseline(pos, idx, bounce, threads)
 
@@ -636,6 +655,10 @@ def main() -> int:
     }
     hit_frac = float((hits.cpu().numpy().view(np.uint32)[:, 3] != 0xFFFFFFFF).mean())
     line["extra"]["bounce_hit_fraction"] = round(hit_frac, 4)
+    line["extra"]["host"] = host_info()
+    if args.no_extras:
+        print(json.dumps(line), flush=True)
+        return 0
     log("box room path tracing")
     line["extra"]["room_render"] = room_render(5, not args.no_cpu_baseline)
     log("batched path tracing")

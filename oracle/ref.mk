@@ -1,7 +1,8 @@
 # Builds the reference's own triangle SBVH builder and traversal, its
 # Substitute (Disney + GGX) material sample, its sensor filters, its random
-# generator, Golden_ratio camera sampler and sRGB -> AP1 mapping, from the
-# reference sources where they lie, into oracle/_ref/ (never committed).
+# generator, Golden_ratio camera sampler and sRGB -> AP1 mapping, and its
+# JSON mesh handler and binary mesh writer, from the reference sources where
+# they lie, into oracle/_ref/ (never committed).
 # Flags follow the reference's CMake configuration for a Release build on
 # Linux (source/CMakeLists.txt: C++20, -O3, -march=native, no RTTI, no
 # exceptions, SU_ACESCG / SU_IBL_MIS_COMPENSATION / SU_ZSTD definitions).
@@ -42,12 +43,16 @@ REF_SOURCES := \
   $(SRC)/core/sampler/sampler_golden_ratio.cpp \
   $(SRC)/core/rendering/sensor/filter/sensor_gaussian.cpp \
   $(SRC)/core/rendering/sensor/filter/sensor_blackman.cpp \
-  $(SRC)/core/rendering/sensor/filter/sensor_mitchell.cpp
+  $(SRC)/core/rendering/sensor/filter/sensor_mitchell.cpp \
+  $(SRC)/core/scene/shape/triangle/triangle_json_handler.cpp \
+  $(SRC)/core/scene/shape/triangle/triangle_mesh_exporter.cpp \
+  $(SRC)/core/scene/shape/triangle/vertex_layout_description.cpp \
+  $(SRC)/base/string/string.cpp
 
 all: $(LIB)
 
 DRIVERS := $(HERE)ref_driver.cpp $(HERE)ref_bsdf_driver.cpp $(HERE)ref_filter_driver.cpp \
-           $(HERE)ref_sampler_driver.cpp
+           $(HERE)ref_sampler_driver.cpp $(HERE)ref_mesh_driver.cpp
 
 $(LIB): $(REF_SOURCES) $(DRIVERS)
 	@mkdir -p $(OUT)

@@ -146,11 +146,107 @@ def stats_only(argv):
     print(text)
 
 
+def _pmc_means(directory, kernel):
+    """Mean per dispatch of every counter the run collected for kernels whose
+    name contains `kernel` (function name match, ignoring templates)."""
+    rows = _rows(directory, "counter_collection.csv")
+    values = {}
+    for r in rows:
+        name = r["Kernel_Name"].replace("mi::(anonymous namespace)::", "").replace("void ", "")
+        if name.split("(")[0] != kernel:
+            continue
+        values.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    return {k: statistics.mean(v) for k, v in values.items()}, {k: len(v) for k, v in values.items()}
+
+
+def _copy_factor(directory, counter):
+    try:
+        return (COPY_BYTES / 1024) / _copy_kib(_rows(directory, "counter_collection.csv"), counter)
+    except (ValueError, FileNotFoundError):
+        return None
+
+
+def counters(argv):
+    """Per-kernel counters of one workload:
+    python -m sprout_amd.rocprof_summary --counters <kt> <pmc dir>... --kernels K... --simds N
+        --title T --out F --command C
+    Derived: HBM bytes (FETCH_SIZE / WRITE_SIZE scaled by the 256 MiB copy
+    probe of the same run), L2 hit rate, MFMA busy fraction
+    (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x SIMDs)), VALU busy
+    (SQ_ACTIVE_INST_VALU x 4 / (GRBM_GUI_ACTIVE x CUs)) and lane utilisation
+    (SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64))."""
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--counters", dest="dirs", nargs="+", required=True)
+    ap.add_argument("--kernels", nargs="+", required=True)
+    ap.add_argument("--simds", type=int, default=1024)
+    ap.add_argument("--title", required=True)
+    ap.add_argument("--out", default="")
+    ap.add_argument("--command", default="")
+    ap.add_argument("--note", default="")
+    args = ap.parse_args(argv)
+    kt, pmc_dirs = args.dirs[0], args.dirs[1:]
+    stats = {r["Name"].replace("mi::(anonymous namespace)::", "").replace("void ", "").split("(")[0]: r
+             for r in _rows(kt, "kernel_stats.csv")}
+    factors = {}
+    for d in pmc_dirs:
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            f = _copy_factor(d, c)
+            if f:
+                factors[c] = f
+    lines = []
+    for k in args.kernels:
+        means = {}
+        for d in pmc_dirs:
+            m, _ = _pmc_means(d, k)
+            means.update(m)
+        st = stats.get(k)
+        avg_us = float(st["AverageNs"]) / 1e3 if st else float("nan")
+        lines.append(f"### `{k}`\n")
+        lines.append(f"calls: {st['Calls'] if st else 0}; mean duration {avg_us:.1f} us\n")
+        lines.append("| counter | mean per dispatch |\n|---|---|")
+        for c in sorted(means):
+            lines.append(f"| {c} | {means[c]:.6g} |")
+        derived = []
+        if "FETCH_SIZE" in means and "FETCH_SIZE" in factors:
+            b = means["FETCH_SIZE"] * 1024 * factors["FETCH_SIZE"]
+            derived.append(f"read beyond L2 {b / 1e6:.2f} MB per dispatch ({b / (avg_us * 1e-6) / 1e9:.0f} GB/s)")
+        if "WRITE_SIZE" in means and "WRITE_SIZE" in factors:
+            b = means["WRITE_SIZE"] * 1024 * factors["WRITE_SIZE"]
+            derived.append(f"written {b / 1e6:.2f} MB per dispatch ({b / (avg_us * 1e-6) / 1e9:.0f} GB/s)")
+        if "TCC_HIT_sum" in means and "TCC_MISS_sum" in means:
+            t = means["TCC_HIT_sum"] + means["TCC_MISS_sum"]
+            if t > 0:
+                derived.append(f"L2 hit rate {means['TCC_HIT_sum'] / t:.3f}")
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in means and means.get("GRBM_GUI_ACTIVE"):
+            derived.append(f"MFMA busy {means['SQ_VALU_MFMA_BUSY_CYCLES'] / (means['GRBM_GUI_ACTIVE'] * args.simds):.3f}")
+        if "SQ_ACTIVE_INST_VALU" in means and means.get("GRBM_GUI_ACTIVE"):
+            derived.append(f"VALU busy {4 * means['SQ_ACTIVE_INST_VALU'] / (means['GRBM_GUI_ACTIVE'] * args.simds):.3f}")
+        if "SQ_THREAD_CYCLES_VALU" in means and means.get("SQ_ACTIVE_INST_VALU"):
+            derived.append(f"VALU lane utilisation {means['SQ_THREAD_CYCLES_VALU'] / (64 * means['SQ_ACTIVE_INST_VALU']):.3f}")
+        if derived:
+            lines.append("\n" + "; ".join(derived) + "\n")
+        else:
+            lines.append("")
+    text = (f"# rocprofv3: {args.title}\n\n"
+            f"Commands (run from the repository root on the GPU box, after `cd /tmp && export TMPDIR=/tmp`):\n\n"
+            f"```\n{args.command}\n```\n\n"
+            f"Kernel statistics of the `--kernel-trace --stats` run:\n\n{kernel_table(kt)}\n\n"
+            f"Counters per dispatch (means over all dispatches of the kernel; one `--pmc` run per "
+            f"group):\n\n" + "\n".join(lines) + (f"\n{args.note}\n" if args.note else ""))
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
+    print(text)
+
+
 def main():
     import sys
 
     if "--stats-only" in sys.argv:
         stats_only(sys.argv[1:])
+        return
+    if "--counters" in sys.argv:
+        counters(sys.argv[1:])
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("kt")

@@ -1,7 +1,8 @@
 """sprout's scene formats at the boundary, for the shapes and materials the
 GPU path renders: triangle meshes in sprout's JSON mesh format
 (triangle_json_handler.cpp: "geometry": "parts" {material_index,
-start_index, num_indices}, "vertices": "positions", "indices"), scenes
+start_index, num_indices}, "vertices": "positions", "indices") or its
+binary mesh format (.sub, triangle_mesh_provider.cpp:435), scenes
 (scene_loader.cpp: "materials" with Substitute {color, roughness, metallic,
 ior} or Light {emission | emittance {quantity Radiance, value, spectrum}},
 "entities" of type Prop / Light with a mesh file shape, materials and a
@@ -45,9 +46,94 @@ def _read(path):
         return json.load(f)
 
 
+SUB_MAGIC = b"SUB\0"
+# serialize's Vertex: position, normal, tangent (packed float3), uv (float2),
+# bitangent sign (uint8), padding: 48 bytes in a file's interleaved stream,
+# 45 per vertex over its separate streams (shape_vertex.cpp:7-9)
+_VERTEX_BYTES = 48
+_VERTEX_UNPADDED = 3 * 4 + 3 * 4 + 3 * 4 + 2 * 4 + 1
+_INDEX_TYPES = {"Int16": (np.int16, True), "UInt16": (np.uint16, False),
+                "Int32": (np.int32, True), "UInt32": (np.uint32, False)}
+
+
+def load_binary_mesh(path):
+    """sprout's binary mesh (file::Type::SUB) as Provider::load_binary reads
+    it (triangle_mesh_provider.cpp:435-666) and serialize::Exporter::write
+    writes it (triangle_mesh_exporter.cpp): the magic "SUB\\0", a uint64 JSON
+    size, the JSON header ("geometry": "parts", "vertices" {binary {offset,
+    size}, num_vertices, layout}, "indices" {binary, num_indices, encoding}),
+    then the binary block.  Positions come first in the vertex data (one
+    48-byte vertex each when the layout's Bitangent_sign is in stream 0,
+    else the separate position stream); indices are UInt16 / UInt32, or
+    Int16 / Int32 deltas chained over the parts in order
+    (fill_triangles_delta).  Returns what load_mesh returns."""
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:4] != SUB_MAGIC:
+        raise ValueError(f"{path}: not a binary mesh")
+    json_size = int(np.frombuffer(data, np.uint64, 1, 4)[0])
+    head = json.loads(data[12:12 + json_size].decode())
+    g = head.get("geometry")
+    if g is None:
+        raise ValueError(f"{path}: no geometry node")
+    parts = [(int(p["start_index"]), int(p["num_indices"]), int(p["material_index"]))
+             for p in g.get("parts", [])]
+    v = g.get("vertices", {})
+    vb = v.get("binary", {})
+    v_off, v_size = int(vb.get("offset", 0)), int(vb.get("size", 0))
+    num_vertices = int(v.get("num_vertices", 0))
+    interleaved = any(e.get("semantic_name") == "Bitangent_sign" and int(e.get("stream", 0)) == 0
+                      for e in v.get("layout", []))
+    ib = g.get("indices", {})
+    i_off, i_size = int(ib.get("binary", {}).get("offset", 0)), int(ib.get("binary", {}).get("size", 0))
+    dtype, delta = _INDEX_TYPES.get(ib.get("encoding"), (np.uint32, False))
+    num_indices = int(ib.get("num_indices", 0))
+    if 0 == num_vertices:
+        # files written before the count was stored
+        num_vertices = v_size // _VERTEX_BYTES
+        if not interleaved:
+            i_off = num_vertices * _VERTEX_UNPADDED
+    start = 12 + json_size
+    if interleaved:
+        rec = np.frombuffer(data, np.uint8, num_vertices * _VERTEX_BYTES, start + v_off)
+        pos = rec.reshape(-1, _VERTEX_BYTES)[:, :12].copy().view(np.float32).reshape(-1, 3)
+    else:
+        pos = np.frombuffer(data, np.float32, 3 * num_vertices, start + v_off).reshape(-1, 3).copy()
+    if 0 == num_indices:
+        num_indices = i_size // np.dtype(dtype).itemsize
+    if start + i_off + i_size > len(data):
+        raise ValueError(f"{path}: could not read all index data")
+    raw = np.frombuffer(data, dtype, num_indices, start + i_off).astype(np.int64)
+    idx = np.zeros(num_indices, np.int64)
+    part = np.zeros(num_indices // 3, np.uint32)
+    covered = np.zeros(num_indices // 3, bool)
+    previous = 0
+    for first, count, material in parts:
+        if first + count > num_indices:
+            raise ValueError(f"{path}: part indices out of bounds")
+        t0, t1 = first // 3, (first + count) // 3
+        seg = raw[3 * t0:3 * t1]
+        if delta:
+            seg = previous + np.cumsum(seg)
+            if len(seg):
+                previous = int(seg[-1])
+        idx[3 * t0:3 * t1] = seg
+        part[t0:t1] = material
+        covered[t0:t1] = True
+    if not covered.all():
+        raise ValueError(f"{path}: triangles outside every part")
+    if len(idx) and (idx.min() < 0 or idx.max() >= len(pos)):
+        raise ValueError(f"{path}: index out of range of {len(pos)} vertices")
+    return pos, idx.astype(np.uint32).reshape(-1, 3), part
+
+
 def load_mesh(path):
-    """A triangle mesh file -> (positions (v, 3) float32, indices (t, 3)
-    uint32, part of each triangle (t,) uint32)."""
+    """A triangle mesh file, sprout's JSON mesh format or its binary format
+    (load_binary_mesh) -> (positions (v, 3) float32, indices (t, 3) uint32,
+    part of each triangle (t,) uint32)."""
+    with open(path, "rb") as f:
+        if f.read(4) == SUB_MAGIC:
+            return load_binary_mesh(path)
     g = _read(path)["geometry"]
     pos = np.asarray(g["vertices"]["positions"], np.float32).reshape(-1, 3)
     idx = np.asarray(g["indices"], np.uint32).reshape(-1, 3)

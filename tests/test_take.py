@@ -155,3 +155,60 @@ def test_reference_sample():
     # its lights are spheres with black-body spectra: named, not guessed
     with pytest.raises(NotImplementedError):
         take.Take(os.path.join(SAMPLE, "mis.take"))
+
+
+def test_binary_mesh_written_by_the_reference():
+    """tests/golden/mesh_parts.sub was written by the reference's own
+    Exporter::write from mesh_parts.json (tests/golden/make_mesh_sub.py):
+    both load to the same positions, triangles and parts."""
+    from sprout_amd import take
+
+    sub = take.load_mesh(os.path.join(HERE, "golden", "mesh_parts.sub"))
+    src = take.load_mesh(os.path.join(HERE, "golden", "mesh_parts.json"))
+    for a, b in zip(sub, src):
+        assert a.dtype == b.dtype and np.array_equal(a, b)
+    assert sub[2].tolist() == [1, 1, 1, 1, 0, 0, 0]
+
+
+@pytest.mark.parametrize("case", ["int16_delta", "uint16", "int32_delta"])
+def test_binary_mesh_index_encodings(tmp_path, case):
+    """Each index encoding the exporter chooses (Int16 deltas, UInt16 when a
+    delta needs 16 bits but every index fits, Int32 deltas above 65535),
+    written by the reference's exporter at test time and read back."""
+    import sys
+
+    from oracle import oracle as O
+    from sprout_amd import take
+
+    lib = O.ref_lib_path()
+    if not os.path.exists(lib):
+        pytest.skip("oracle/_ref not built (the reference is absent)")
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_mesh_sub as M
+
+    n = {"int16_delta": 40, "uint16": 40000, "int32_delta": 70000}[case]
+    rng = np.random.default_rng(n)
+    pos = rng.uniform(-2, 2, (n, 3)).astype(np.float32)
+    idx = rng.integers(0, 40, (30, 3))
+    if case != "int16_delta":
+        idx[3] = [0, n - 1, 1]  # a delta of n - 1
+    parts = [(0, 45, 2), (45, 45, 0)]
+    path = M.export(lib, M.mesh_json(pos, idx, parts), str(tmp_path), "m")
+    with open(path, "rb") as f:
+        data = f.read()
+    head = data[12:12 + int(np.frombuffer(data, np.uint64, 1, 4)[0])].decode()
+    want = {"int16_delta": '"Int16"', "uint16": '"UInt16"', "int32_delta": '"Int32"'}[case]
+    assert want in head
+    p, i, part = take.load_mesh(path)
+    assert np.array_equal(p, pos) and np.array_equal(i, idx.astype(np.uint32))
+    assert part.tolist() == [2] * 15 + [0] * 15
+
+
+def test_binary_mesh_rejects_bad_files(tmp_path):
+    from sprout_amd import take
+
+    src = open(os.path.join(HERE, "golden", "mesh_parts.sub"), "rb").read()
+    bad = tmp_path / "short.sub"
+    bad.write_bytes(src[:-4])  # index data cut short
+    with pytest.raises(ValueError):
+        take.load_mesh(str(bad))
