@@ -172,13 +172,16 @@ def counters(argv):
         --title T --out F --command C
     Derived: HBM bytes (FETCH_SIZE / WRITE_SIZE scaled by the 256 MiB copy
     probe of the same run), L2 hit rate, MFMA busy fraction
-    (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x SIMDs)), VALU busy
-    (SQ_ACTIVE_INST_VALU x 4 / (GRBM_GUI_ACTIVE x CUs)) and lane utilisation
-    (SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64))."""
+    (SQ_VALU_MFMA_BUSY_CYCLES / (GPU cycles x SIMDs)), VALU busy
+    (SQ_ACTIVE_INST_VALU x 4 / (GPU cycles x SIMDs)) and lane utilisation
+    (SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64)).  The GPU cycles
+    are GRBM_GUI_ACTIVE / XCCs: the counter reads the sum over the XCCs
+    (8 on MI355X), where rocprof's own MfmaUtil takes their maximum."""
     ap = argparse.ArgumentParser()
     ap.add_argument("--counters", dest="dirs", nargs="+", required=True)
     ap.add_argument("--kernels", nargs="+", required=True)
     ap.add_argument("--simds", type=int, default=1024)
+    ap.add_argument("--xccs", type=int, default=8)
     ap.add_argument("--title", required=True)
     ap.add_argument("--out", default="")
     ap.add_argument("--command", default="")
@@ -217,10 +220,13 @@ def counters(argv):
             t = means["TCC_HIT_sum"] + means["TCC_MISS_sum"]
             if t > 0:
                 derived.append(f"L2 hit rate {means['TCC_HIT_sum'] / t:.3f}")
-        if "SQ_VALU_MFMA_BUSY_CYCLES" in means and means.get("GRBM_GUI_ACTIVE"):
-            derived.append(f"MFMA busy {means['SQ_VALU_MFMA_BUSY_CYCLES'] / (means['GRBM_GUI_ACTIVE'] * args.simds):.3f}")
-        if "SQ_ACTIVE_INST_VALU" in means and means.get("GRBM_GUI_ACTIVE"):
-            derived.append(f"VALU busy {4 * means['SQ_ACTIVE_INST_VALU'] / (means['GRBM_GUI_ACTIVE'] * args.simds):.3f}")
+        cycles = means.get("GRBM_GUI_ACTIVE", 0.0) / args.xccs
+        if cycles:
+            derived.append(f"GPU cycles {cycles:.4g} ({cycles / (avg_us * 1e-6) / 1e9:.2f} GHz over the mean duration)")
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in means and cycles:
+            derived.append(f"MFMA busy {means['SQ_VALU_MFMA_BUSY_CYCLES'] / (cycles * args.simds):.3f}")
+        if "SQ_ACTIVE_INST_VALU" in means and cycles:
+            derived.append(f"VALU busy {4 * means['SQ_ACTIVE_INST_VALU'] / (cycles * args.simds):.3f}")
         if "SQ_THREAD_CYCLES_VALU" in means and means.get("SQ_ACTIVE_INST_VALU"):
             derived.append(f"VALU lane utilisation {means['SQ_THREAD_CYCLES_VALU'] / (64 * means['SQ_ACTIVE_INST_VALU']):.3f}")
         if derived:
