@@ -69,6 +69,7 @@ struct Tlas {
     uint32_t                     num_meshes    = 0;
     uint32_t                     max_stack     = 0;
     uint32_t*                    spill         = nullptr;  // stack area of the pair traversal
+    uint32_t*                    claims        = nullptr;  // its ray claim counters (Counter_words)
     uint32_t                     spill_total   = 0;        // its threads (0: not used)
 
     void release();
