@@ -1,9 +1,9 @@
 // BF16 MFMA inference and training kernels of the path-guiding network.
 //
-// Every matrix product runs on v_mfma_f32_16x16x16_bf16 (64-wide wave, FP32
-// accumulation).  Operand lanes, for lane l, r = l % 16, q = l / 16:
-//   A (16 x 16, rows i, depth k): A[r][4q + t], t = 0..3
-//   B (16 x 16, depth k, cols j): B[4q + t][r]
+// Matrix products run on gfx950's v_mfma_f32_16x16x32_bf16 (64-wide wave,
+// FP32 accumulation).  Operand lanes, for lane l, r = l % 16, q = l / 16:
+//   A (16 x 16, rows i, depth k): A[r][4q + t], t = 0..3  (x32: A[r][8q + t], t = 0..7)
+//   B (16 x 16, depth k, cols j): B[4q + t][r]            (x32: B[8q + t][r])
 //   D (16 x 16):                   D[4q + t][r]
 // Weights live in LDS as bf16 W^T [out][in] so a forward B operand is one
 // 8-byte read; activations as bf16 [row][feature] so an A operand is one
@@ -73,17 +73,30 @@ __device__ __forceinline__ bf16x4 load4(uint16_t const* p) {
     return *reinterpret_cast<bf16x4 const*>(p);
 }
 
-__device__ __forceinline__ bf16x4 gather4(uint16_t const* p, int stride) {
-    bf16x4 v;
-    v[0] = short(p[0]);
-    v[1] = short(p[stride]);
-    v[2] = short(p[2 * stride]);
-    v[3] = short(p[3 * stride]);
-    return v;
+
+
+// gfx950's double-depth product: 16 x 16 x 32, each lane supplying eight
+// consecutive depth elements.  Lane (r, q) gives A[r][8q + t] and
+// B[8q + t][r], t = 0..7; whatever order the unit adds the 32 depth terms
+// in, D is the sum over all of them, so the operands only need the same
+// depth index at the same lane and slot.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 load8(uint16_t const* p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<uint4 const*>(p));
 }
 
-__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+__device__ __forceinline__ bf16x8 gather8(uint16_t const* p, int stride) {
+    uint32_t w[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        w[t] = uint32_t(p[2 * t * stride]) | (uint32_t(p[(2 * t + 1) * stride]) << 16);
+    }
+    return __builtin_bit_cast(bf16x8, make_uint4(w[0], w[1], w[2], w[3]));
+}
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 struct Weights {
@@ -190,6 +203,7 @@ __device__ __forceinline__ void store_layer(f32x4 const (&acc)[N / 16], float co
 // lane's four results are four consecutive outputs of one row: MFMA A
 // operand = rows of W^T in LDS (stride sw), B operand = activation rows
 // (bf16 in LDS, stride sa), D[4q + t][r] = output 16 nt + 4q + t of row r.
+// Double-depth products: one 16-byte operand read per lane and step.
 // The depth steps run outermost, so each activation operand is read once and
 // the N / 16 accumulation chains interleave.
 template <int K, int N, int Mode>
@@ -203,22 +217,23 @@ __device__ void layer(uint16_t const* A, int sa, uint16_t const* Wt, int sw, flo
     for (int nt = 0; nt < N / 16; ++nt) {
         acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
+    static_assert(K % 32 == 0, "depth in steps of 32");
 #pragma unroll
-    for (int ks = 0; ks < K / 16; ++ks) {
-        bf16x4 const b = load4(A + r * sa + ks * 16 + 4 * q);
+    for (int ks = 0; ks < K / 32; ++ks) {
+        bf16x8 const b = load8(A + r * sa + ks * 32 + 8 * q);
 #pragma unroll
         for (int nt = 0; nt < N / 16; ++nt) {
-            acc[nt] = mfma(load4(Wt + (nt * 16 + r) * sw + ks * 16 + 4 * q), b, acc[nt]);
+            acc[nt] = mfma32(load8(Wt + (nt * 16 + r) * sw + ks * 32 + 8 * q), b, acc[nt]);
         }
     }
     store_layer<N, Mode>(acc, bias, r, q, H, sh, F, sf, rows);
 }
 
 // First layer of the forward pass with its features formed in registers: the
-// B operand of lane (r, q) at depth step ks is features k0 .. k0 + 3, k0 =
-// 16 ks + 4q, of query row0 + r, i.e. sin, cos, sin, cos of pi f x_i for
-// i = k0 / 24 and the multipliers j0, j0 + 1 with j0 = (k0 % 24) / 2 (zero
-// past the 216 features): the values features() writes.  half_freq holds
+// B operand of lane (r, q) at depth step ks is features k0 .. k0 + 7, k0 =
+// 32 ks + 8q, of query row0 + r, i.e. (sin, cos) of pi f x_i for i = k0 / 24
+// and the multipliers j0 .. j0 + 3 with j0 = (k0 % 24) / 2 (zero past the
+// 216 features): the values features() writes.  half_freq holds
 // f / 2, and (f / 2) x equals (f x) / 2 for every normal product.  Rows past
 // n run on zero inputs; rows do not mix and theirs are never stored.
 __device__ void layer1_direct(float const* x, float const* half_freq, uint32_t row0, uint32_t n,
@@ -233,25 +248,28 @@ __device__ void layer1_direct(float const* x, float const* half_freq, uint32_t r
     for (int nt = 0; nt < Hidden / 16; ++nt) {
         acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
+    static_assert(Feat_pad % 32 == 0 && 24 % 8 == 0, "eight features of one input per lane");
 #pragma unroll
-    for (int ks = 0; ks < Feat_pad / 16; ++ks) {
-        int const k0 = ks * 16 + 4 * q;
-        uint32_t  pk[2] = {0u, 0u};
+    for (int ks = 0; ks < Feat_pad / 32; ++ks) {
+        int const k0 = ks * 32 + 8 * q;
+        uint4     pk = make_uint4(0u, 0u, 0u, 0u);
         if (k0 < Num_feat) {
             int const   i  = k0 / 24;
             int const   j0 = (k0 % 24) / 2;
             float const v  = live ? x[size_t(row) * Num_in + i] : 0.f;
+            uint32_t    w[4];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < 4; ++u) {
                 float const h   = half_freq[i * Num_freq + j0 + u] * v;
                 float const rev = h - floorf(h);
-                pk[u] = pack_bf16(__builtin_amdgcn_sinf(rev), __builtin_amdgcn_cosf(rev));
+                w[u] = pack_bf16(__builtin_amdgcn_sinf(rev), __builtin_amdgcn_cosf(rev));
             }
+            pk = make_uint4(w[0], w[1], w[2], w[3]);
         }
-        bf16x4 const b = __builtin_bit_cast(bf16x4, make_uint2(pk[0], pk[1]));
+        bf16x8 const b = __builtin_bit_cast(bf16x8, pk);
 #pragma unroll
         for (int nt = 0; nt < Hidden / 16; ++nt) {
-            acc[nt] = mfma(load4(Wt + (nt * 16 + r) * S_feat + k0), b, acc[nt]);
+            acc[nt] = mfma32(load8(Wt + (nt * 16 + r) * S_feat + k0), b, acc[nt]);
         }
     }
     store_layer<Hidden, Mode_relu>(acc, bias, r, q, H, S_hid, nullptr, 0, 16);
@@ -408,12 +426,13 @@ __device__ void back_layer(uint16_t const* src, int ss, uint16_t const* Wt, int 
     for (int nt = 0; nt < N / 16; ++nt) {
         acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
+    static_assert(K % 32 == 0, "depth in steps of 32");
 #pragma unroll
-    for (int ks = 0; ks < K / 16; ++ks) {
-        bf16x4 const b = load4(src + r * ss + ks * 16 + 4 * q);
+    for (int ks = 0; ks < K / 32; ++ks) {
+        bf16x8 const b = load8(src + r * ss + ks * 32 + 8 * q);
 #pragma unroll
         for (int nt = 0; nt < N / 16; ++nt) {
-            acc[nt] = mfma(gather4(Wt + (ks * 16 + 4 * q) * sw + nt * 16 + r, sw), b, acc[nt]);
+            acc[nt] = mfma32(gather8(Wt + (ks * 32 + 8 * q) * sw + nt * 16 + r, sw), b, acc[nt]);
         }
     }
 #pragma unroll
@@ -541,11 +560,11 @@ __global__ void __launch_bounds__(Grad_waves * 64)
                 act = s.h[2], sa = S_hid, dz = s.dy, sd = S_out, mt = (tile - 88) / 2, nt = (tile - 88) % 2;
             }
 #pragma unroll
-            for (int ks = 0; ks < Chunk / 16; ++ks) {
+            for (int ks = 0; ks < Chunk / 32; ++ks) {
                 // A[i][k] = act[sample k][feature i]; B[k][j] = dz[sample k][j]
-                bf16x4 const a = gather4(act + (ks * 16 + 4 * q) * sa + mt * 16 + r, sa);
-                bf16x4 const b = gather4(dz + (ks * 16 + 4 * q) * sd + nt * 16 + r, sd);
-                acc[k]         = mfma(a, b, acc[k]);
+                bf16x8 const a = gather8(act + (ks * 32 + 8 * q) * sa + mt * 16 + r, sa);
+                bf16x8 const b = gather8(dz + (ks * 32 + 8 * q) * sd + nt * 16 + r, sd);
+                acc[k]         = mfma32(a, b, acc[k]);
             }
         }
         __syncthreads();

@@ -179,6 +179,10 @@ def test_guided_slit_room_1024_converges_to_the_reference(gpu):
         f"guided {b_guided:.4g}; means {plain.mean():.5g} {guided.mean():.5g} ref {ref.mean():.5g}")
     assert e_guided < 1.25 * e_plain
     assert b_guided < 1.5 * b_plain
+    # the reference is the same image: over 16 x 16 blocks the 16-spp plain
+    # render is within its noise of it (a mirrored or shifted reference
+    # leaves a bias of about 3e-3 here)
+    assert b_plain < 3e-4
     assert abs(guided.mean() / ref.mean() - 1) < 1e-2
     assert abs(plain.mean() / ref.mean() - 1) < 1e-2
 
