@@ -148,6 +148,16 @@ def room_render(frames: int, cpu: bool):
         "grays_all_rays": round(rays / wall / 1e9, 4),
         "rays_per_sample": round(rays / paths, 3),
     }
+    # the same frames with regenerating waves (a 4-spp path pool refilled
+    # from the frame's samples)
+    render.render(sc, cam, 16, samples_per_wave=4, seed=1, accum=acc, regenerate=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        render.render(sc, cam, 16, samples_per_wave=4, seed=1, first_sample=16 * (f + 1), accum=acc,
+                      regenerate=True)
+    torch.cuda.synchronize()
+    out["regenerating_ms_per_frame"] = round((time.perf_counter() - t0) / frames * 1e3, 3)
     # the same frames through the filtered sensor (Gaussian, radius 2, alpha
     # 1.8: the take default), on the 260 x 260 grid it samples
     sens = render.sensor()
@@ -184,58 +194,94 @@ def room_render(frames: int, cpu: bool):
 
 
 def cloud_render(frames: int, cpu: bool):
-    """Heterogeneous volume: 256^3 density grid of seeded 6-octave fBm (max
-    density 20, scattering albedo 0.9, Henyey-Greenstein g 0.6) in a box
-    above a ground quad, lit by a 1000-triangle emissive quad; 1024 x 1024,
-    4 spp per frame, max depth 8, RGB delta tracking for scattering and
-    ratio tracking for transmittance."""
+    """Spectral heterogeneous volume as configured: 256^3 density grid of
+    seeded 6-octave fBm (max density 20, scattering albedo 0.9,
+    Henyey-Greenstein g 0.6) over a flat 32^3 majorant grid, in a box above a
+    ground quad, lit by a 1000-triangle emissive quad; hero-wavelength
+    transport (4 wavelengths over 380-720 nm), delta tracking for scattering
+    and ratio tracking for transmittance; 1024 x 1024, 4 spp per frame, max
+    depth 8.  Beside it: the same frames in rgb over sprout's gridtree and
+    over the flat grid, and the guided render (the MLP at surface and medium
+    vertices, trained online) in spectral mode."""
     import torch
 
-    from sprout_amd import medium, render, scenes
+    from sprout_amd import guide, medium, render, scenes
 
     pos, idx, tmat, mats = scenes.cloud_scene()
     grid = scenes.fbm_density(256, 6, 1, 20.0)
     sc = render.Scene(pos, idx, tmat, mats)
-    medium.set_medium(sc, grid, *scenes.CLOUD_BOX, scenes.CLOUD_CC_A, scenes.CLOUD_CC_S,
-                      scenes.CLOUD_G)
     w = 1024
     spp = 4
     cam = render.camera(w, w, **scenes.CLOUD_CAMERA)
     acc = torch.zeros((w, w, 3), dtype=torch.float64, device="cuda")
-    render.render(sc, cam, spp, samples_per_wave=spp, seed=1, accum=acc)
-    torch.cuda.synchronize()
-    paths = rays = 0.0
-    t0 = time.perf_counter()
-    for f in range(frames):
-        _, st = render.render(sc, cam, spp, samples_per_wave=spp, seed=1,
-                              first_sample=spp * (f + 1), accum=acc, stats=True)
-        paths += st.paths
-        rays += st.extend_rays + st.shadow_rays
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+
+    def set_grid(kind):
+        medium.set_medium(sc, grid, *scenes.CLOUD_BOX, scenes.CLOUD_CC_A, scenes.CLOUD_CC_S,
+                          scenes.CLOUD_G, majorants=kind)
+
+    def timed(spectral, stats=False):
+        render.render(sc, cam, spp, samples_per_wave=spp, seed=1, accum=acc, spectral=spectral)
+        torch.cuda.synchronize()
+        paths = rays = 0.0
+        t0 = time.perf_counter()
+        for f in range(frames):
+            _, st = render.render(sc, cam, spp, samples_per_wave=spp, seed=1,
+                                  first_sample=spp * (f + 1), accum=acc, stats=stats,
+                                  spectral=spectral)
+            if stats:
+                paths += st.paths
+                rays += st.extend_rays + st.shadow_rays
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, paths, rays
+
+    set_grid("flat")
+    wall, paths, rays = timed(True, stats=True)
     out = {
-        "workload": "fBm cloud 256^3 (max density 20, albedo 0.9, g 0.6), 1000-tri emissive quad, "
-                    "1024x1024, 4 spp per frame, max depth 8, RGB tracking",
+        "workload": "fBm cloud 256^3 (max density 20, albedo 0.9, g 0.6), flat 32^3 majorant "
+                    "grid, 1000-tri emissive quad, 1024x1024, 4 spp per frame, max depth 8, "
+                    "hero-wavelength tracking (4 wavelengths, 380-720 nm)",
         "ms_per_frame": round(wall / frames * 1e3, 3),
         "samples_per_s": round(paths / wall, 1),
         "rays_per_sample": round(rays / paths, 3),
         "seconds_for_256_spp": round(w * w * 256 / (paths / wall), 2),
     }
+    out["rgb_flat_majorants_ms_per_frame"] = round(timed(False)[0] / frames * 1e3, 3)
+    set_grid("gridtree")
+    out["rgb_gridtree_ms_per_frame"] = round(timed(False)[0] / frames * 1e3, 3)
+    out["spectral_gridtree_ms_per_frame"] = round(timed(True)[0] / frames * 1e3, 3)
+    # guided: the MLP at surface and medium vertices, one Adam step per wave
+    set_grid("flat")
+    g = guide.Guide(1)
+    render.render_guided(sc, cam, spp, g, fraction=0.5, train=True, train_steps=1,
+                         batch=262144, learning_rate=5e-3, samples_per_wave=spp, seed=1,
+                         accum=acc, guide_medium=True, spectral=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        render.render_guided(sc, cam, spp, g, fraction=0.5, train=True, train_steps=1,
+                             batch=262144, learning_rate=5e-3, samples_per_wave=spp, seed=1,
+                             first_sample=spp * (f + 1), accum=acc, guide_medium=True,
+                             spectral=True)
+    torch.cuda.synchronize()
+    out["guided_spectral_ms_per_frame"] = round((time.perf_counter() - t0) / frames * 1e3, 3)
+    g.release()
     sc.release()
     if cpu:
         from oracle import oracle as O
         from sprout_amd import lights
 
         threads = host_threads()
-        om = O.Medium(grid, *scenes.CLOUD_BOX, scenes.CLOUD_CC_A, scenes.CLOUD_CC_S, scenes.CLOUD_G)
+        om = O.Medium(grid, *scenes.CLOUD_BOX, scenes.CLOUD_CC_A, scenes.CLOUD_CC_S, scenes.CLOUD_G,
+                      majorants="flat")
         tree = O.scene_light_tree(pos, idx, tmat, mats, lights.build_tree)
         t0 = time.perf_counter()
         O.render(pos, idx, tmat, mats, 256, 256, 16, scenes.CLOUD_CAMERA, tree, seed=1,
-                 threads=threads, medium=om)
+                 threads=threads, medium=om, spectral=True)
         dt = time.perf_counter() - t0
         out["cpu_port_samples_per_s"] = round(256 * 256 * 16 / dt, 1)
         out["cpu_port_threads"] = threads
-        out["cpu_port_sample"] = "256x256 x 16 spp of the same cloud, oracle/pt.c + oracle/medium.c"
+        out["cpu_port_sample"] = ("256x256 x 16 spp of the same cloud, spectral, flat majorants, "
+                                  "oracle/pt.c + oracle/medium.c")
     return out
 
 
@@ -275,6 +321,17 @@ def batch_render(frames: int, instanced: bool):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     extra = {}
+    # the same frames with regenerating waves (a 1-spp pool of path slots
+    # refilled from the frame's 4 spp)
+    render.render(sc, cam, 4, samples_per_wave=1, min_bounces=3, max_bounces=16, seed=1, accum=acc,
+                  regenerate=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        render.render(sc, cam, 4, samples_per_wave=1, min_bounces=3, max_bounces=16, seed=1,
+                      first_sample=1 + 4 * f, accum=acc, regenerate=True)
+    torch.cuda.synchronize()
+    extra["regenerating_ms_per_frame"] = round((time.perf_counter() - t0) / frames * 1e3, 2)
     if instanced:
         # an animated frame's top-level rebuild: every instance moved 0.01 up
         # per step (mi_scene_update_instances, host matrices in, tree on the device)

@@ -269,6 +269,24 @@ enum mi_render_flags {
                                      takes point s; otherwise two draws of the path's own
                                      generator.  The frame is the one call: first_sample must
                                      be 0 */
+    MI_RENDER_REGENERATE     = 8, /* regenerating waves (mi_render with the box sensor): a path
+                                     slot whose path ends starts the frame's next sample, so the
+                                     ray queues stay full while paths of uneven depth finish; the
+                                     image equals the wave-by-wave render bit for bit (each
+                                     sample's random numbers depend on the sample alone, and the
+                                     samples are summed per pixel in the same order).  Needs room
+                                     for min(spp, 2^27 / pixels) planes of samples (12 bytes each) */
+    MI_RENDER_SPECTRAL       = 16, /* hero-wavelength transport (no counterpart in sprout, which
+                                     renders RGB): every path carries four wavelengths over
+                                     380-720 nm, a hero drawn by a hash of (seed, the path's
+                                     generator sequence) and three at quarter-range offsets;
+                                     rgb colours, emission and medium coefficients are
+                                     upsampled by a partition of unity of three bands (blue
+                                     falling 450-510 nm, red rising 560-610 nm, green the
+                                     rest), tracking and transmittance run per wavelength with
+                                     history-aware weights, and the film reads rgb through the
+                                     bands' dual basis.  Grey scenes render the rgb image in
+                                     expectation, with the same random numbers per path */
 };
 
 typedef struct mi_render_stats {
@@ -334,6 +352,22 @@ int32_t mi_scene_create_instanced(mi_mesh const* meshes, uint32_t num_meshes,
  * prop-tree rebuild of Scene::compile (scene.cpp:216-267). */
 int32_t mi_scene_update_instances(uint32_t scene, float const* world_to_object,
                                   float const* object_to_world, uint32_t num_instances);
+
+/* New vertex positions for a scene of mi_scene_create (same vertex count and
+ * topology; positions: DEVICE array, 3 floats per vertex), as a frame of an
+ * animation moves its props and emitters: everything rebuilds on the device
+ * on `stream` (the call waits): each emissive triangle's light, every
+ * emissive part's Primitive_tree and cone, the scene's light tree over the
+ * parts (Scene::compile, scene.cpp:216-267, light_tree_builder.cpp:436-760),
+ * the same trees bit for bit that mi_scene_create would build for these
+ * positions, then the tree over the triangles with the GPU builder.  A scene
+ * light-tree node that only the host builder's median split divides is built
+ * on the host.  MI_ERR_ARGS, and the old positions kept, if an emissive
+ * triangle would lose its area.  light_ms (optional): host wall time of the
+ * light part.  Waits for renders in flight.  Instanced scenes move their
+ * instances with mi_scene_update_instances. */
+int32_t mi_scene_update_positions(uint32_t scene, float const* positions, uint32_t num_vertices,
+                                  double* light_ms, void* stream);
 
 int32_t mi_scene_release(uint32_t scene);
 
@@ -451,6 +485,35 @@ int32_t mi_light_part_tree_build(float const* box_min, float const* box_max, flo
                                  uint32_t* orders, uint32_t* cdf_offset, float* cdfs,
                                  uint32_t* num_nodes, uint32_t* num_cdfs);
 
+/* As mi_light_tree_build, on the device: DEVICE arrays, the same tree bit for
+ * bit (nodes, middles, mapping, orders), built a level of nodes at a time on
+ * `stream` (the call waits for it).  *num_nodes (host) receives the node
+ * count.  MI_ERR_UNSPLITTABLE: a node every candidate leaves one side of,
+ * which the host builder splits at the median; build on the host then.  No
+ * counterpart in the reference, whose Tree_builder runs on its thread pool
+ * (light_tree_builder.cpp:436-540; Scene::compile, scene.cpp:216-267). */
+int32_t mi_light_tree_build_device(float const* box_min, float const* box_max, float const* cones,
+                                   float const* powers, uint8_t const* two_sided,
+                                   uint32_t num_lights, mi_light_node* nodes, uint32_t* middles,
+                                   uint32_t* mapping, uint32_t* orders, uint32_t* num_nodes,
+                                   void* stream);
+
+/* The Primitive_trees of num_parts parts at once on the device, each as
+ * mi_light_part_tree_build builds it: part r holds lights first[r] ..
+ * first[r + 1] - 1 (first: HOST array of num_parts + 1, from 0 to
+ * num_lights, no part empty) and its cone part_cones[4 r ..].  DEVICE
+ * arrays; part r's nodes, middles and cdf_offset start at 2 first[r] - r,
+ * its cdfs at 2 first[r]; mapping and orders are local to the part.
+ * num_nodes: HOST, per part; num_cdfs: DEVICE, per part. */
+int32_t mi_light_part_trees_build_device(float const* box_min, float const* box_max,
+                                         float const* cones, float const* powers,
+                                         uint32_t num_lights, uint32_t const* first,
+                                         uint32_t num_parts, float const* part_cones,
+                                         mi_light_node* nodes, uint32_t* middles,
+                                         uint32_t* mapping, uint32_t* orders,
+                                         uint32_t* cdf_offset, float* cdfs, uint32_t* num_nodes,
+                                         uint32_t* num_cdfs, void* stream);
+
 enum mi_light_flags {
     MI_LIGHT_SPLIT        = 1, /* split the tree (Adaptive light sampling) */
     MI_LIGHT_TOTAL_SPHERE = 2, /* the point is in a medium: no horizon test (translucent) */
@@ -493,6 +556,24 @@ int32_t mi_scene_set_medium(uint32_t scene, float const* density, uint32_t const
                             float const box_lo[3], float const box_hi[3], float const cc_a[3],
                             float const cc_s[3], float g);
 
+/* The majorants tracking steps with through a scene's medium. */
+enum mi_majorants {
+    MI_MAJORANTS_GRIDTREE = 0, /* sprout's gridtree (Octree_builder, volumetric_octree_builder.cpp:
+                                  13-296): 32^3-voxel cells split into octants down to 8^3 voxels
+                                  while their density range is 0.1 or more */
+    MI_MAJORANTS_FLAT     = 1, /* a flat grid of 8^3-voxel cells (32^3 cells for 256^3 voxels),
+                                  each with the minorant / majorant of its voxels and one voxel of
+                                  border (the linear filter's reach); stored as a gridtree whose
+                                  cells all split to their leaves.  No counterpart in sprout */
+};
+
+/* mi_scene_set_medium with a choice of majorants (MI_MAJORANTS_*); both give
+ * unbiased tracking, they differ in how many null collisions a step makes. */
+int32_t mi_scene_set_medium_majorants(uint32_t scene, float const* density, uint32_t const dims[3],
+                                      float const box_lo[3], float const box_hi[3],
+                                      float const cc_a[3], float const cc_s[3], float g,
+                                      uint32_t majorants);
+
 /* HOST, no GPU: the majorant gridtree of a density grid (Octree_builder::build).
  * num_cells (3): top-level cells of 32^3 voxels per axis.  nodes: uint32 per
  * node, bit 0 = has children, bits 1-31 = first child (8 consecutive) or
@@ -501,6 +582,12 @@ int32_t mi_scene_set_medium(uint32_t scene, float const* density, uint32_t const
 int32_t mi_medium_gridtree(float const* density, uint32_t const dims[3], float const cc_a[3],
                            float const cc_s[3], uint32_t* num_cells, uint32_t* nodes,
                            uint32_t* num_nodes, float* data, uint32_t* num_data);
+
+/* HOST, no GPU: as mi_medium_gridtree for either kind of majorants. */
+int32_t mi_medium_majorant_tree(float const* density, uint32_t const dims[3], float const cc_a[3],
+                                float const cc_s[3], uint32_t majorants, uint32_t* num_cells,
+                                uint32_t* nodes, uint32_t* num_nodes, float* data,
+                                uint32_t* num_data);
 
 enum mi_medium_flags {
     MI_MEDIUM_TRANSMITTANCE = 1, /* Tracking::transmittance instead of Tracking_multi::integrate */
@@ -593,6 +680,9 @@ typedef struct mi_guide_settings {
     uint32_t record_depth;
     uint32_t guide_depth; /* vertices of depth below this sample the guide (0: all); deeper
                              ones sample the BSDF alone and record nothing */
+    uint32_t medium;      /* nonzero: scattering vertices in the scene's medium are guided
+                             and recorded too, the guide mixed with the phase function; their
+                             network inputs are the scattering point, a zero normal and wo */
 } mi_guide_settings;
 
 /* As mi_render, guided.  With training, every wave is followed on the

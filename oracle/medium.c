@@ -60,6 +60,7 @@ typedef struct build_node {
 typedef struct {
     float const* density;
     int32_t      d[3];
+    int32_t      uniform; /* every cell split to its 8^3-voxel leaves (flat majorants) */
     float        cm[4];
     uint32_t     num_nodes;
     uint32_t     num_data;
@@ -103,7 +104,7 @@ static void split(splitter* sp, build_node* node, int32_t const lo[3], int32_t c
         max_density = 0.f;
     }
     float const diff = max_density - min_density;
-    if (LOG2_CELL_DIM - 3 == (int32_t)depth || diff < 0.1f || maxb[0] - minb[0] < w ||
+    if (LOG2_CELL_DIM - 3 == (int32_t)depth || (!sp->uniform && diff < 0.1f) || maxb[0] - minb[0] < w ||
         maxb[1] - minb[1] < w || maxb[2] - minb[2] < w) {
         node->children = NULL;
         /* mt_epsilon is 0: both branches of the reference store these */
@@ -168,6 +169,11 @@ static void free_nodes(build_node* node) {
 
 int32_t oracle_gridtree_build(float const* density, int32_t const dims[3], float const cc_a[3],
                               float const cc_s[3], oracle_gridtree* out) {
+    return oracle_majorant_tree_build(density, dims, cc_a, cc_s, 0, out);
+}
+
+int32_t oracle_majorant_tree_build(float const* density, int32_t const dims[3], float const cc_a[3],
+                                   float const cc_s[3], int32_t uniform, oracle_gridtree* out) {
     memset(out, 0, sizeof(*out));
     if (!density || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) {
         return ORACLE_ERR_ARGS;
@@ -188,6 +194,7 @@ int32_t oracle_gridtree_build(float const* density, int32_t const dims[3], float
     splitter sp;
     memset(&sp, 0, sizeof(sp));
     sp.density = density;
+    sp.uniform = uniform;
     memcpy(sp.d, dims, sizeof(sp.d));
     sp.cm[0] = min3(cc_a);
     sp.cm[1] = min3(cc_s);
@@ -414,72 +421,93 @@ float oracle_similarity_scale(float g, uint32_t depth) {
     return (1.f - g) / (1.f - gs);
 }
 
-static void coefficients(oracle_medium const* m, tray const* r, float t, float srs, int32_t nearest,
-                         float a[3], float s[3]) {
+/* The collision coefficients per unit density of the nc channels a path
+ * carries: the medium's rgb, or a hero-wavelength path's four wavelengths. */
+typedef struct {
+    int32_t     nc;
+    float const* a;
+    float const* s;
+} channels;
+
+static void coefficients(oracle_medium const* m, channels const* cc, tray const* r, float t,
+                         float srs, int32_t nearest, float* a, float* s) {
     float uvw[3];
     tray_point(r, t, uvw);
     float const dn = oracle_medium_density(m, uvw, nearest);
-    for (int c = 0; c < 3; ++c) {
-        a[c] = dn * m->cc_a[c];
-        s[c] = (dn * m->cc_s[c]) * srs;
+    for (int c = 0; c < cc->nc; ++c) {
+        a[c] = dn * cc->a[c];
+        s[c] = (dn * cc->s[c]) * srs;
     }
 }
 
-static float average(float const v[3]) {
-    return (v[0] + v[1] + v[2]) / 3.f;
+static float average(int32_t nc, float const* v) {
+    if (3 == nc) {
+        return (v[0] + v[1] + v[2]) / 3.f;
+    }
+    return (((v[0] + v[1]) + v[2]) + v[3]) * 0.25f;
 }
 
-static int all_below(float const w[3]) {
-    return w[0] < ABORT_EPSILON && w[1] < ABORT_EPSILON && w[2] < ABORT_EPSILON;
+static int all_below(int32_t nc, float const* w) {
+    for (int c = 0; c < nc; ++c) {
+        if (!(w[c] < ABORT_EPSILON)) {
+            return 0;
+        }
+    }
+    return 1;
 }
 
 /* Tracking::tracking; 1 = scattered at *t_out */
-static int tracking(oracle_medium const* m, tray const* r, float const cm[4], float srs,
-                    int32_t nearest, oracle_random_fn rnd, void* rs, float* t_out, float w[3]) {
-    float const mt = cm[2] + cm[3];
+static int tracking(oracle_medium const* m, channels const* cc, tray const* r, float const cm[4],
+                    float srs, int32_t nearest, oracle_random_fn rnd, void* rs, float* t_out,
+                    float* w) {
+    int32_t const nc = cc->nc;
+    float const   mt = cm[2] + cm[3];
     if (mt < MIN_MT) {
         return 0;
     }
-    float lw[3] = {w[0], w[1], w[2]};
+    float lw[4];
+    memcpy(lw, w, sizeof(float) * (size_t)nc);
     float const imt = 1.f / mt;
     for (float t = r->min_t, d = r->max_t;;) {
         float const r0 = rnd(rs);
         t -= logf(1.f - r0) * imt;
         if (t > d) {
-            memcpy(w, lw, sizeof(lw));
+            memcpy(w, lw, sizeof(float) * (size_t)nc);
             return 0;
         }
-        float mu_a[3], mu_s[3];
-        coefficients(m, r, t, srs, nearest, mu_a, mu_s);
-        float mu_n[3], sw[3], nw[3];
-        for (int c = 0; c < 3; ++c) {
+        float mu_a[4], mu_s[4];
+        coefficients(m, cc, r, t, srs, nearest, mu_a, mu_s);
+        float mu_n[4], sw[4] = {0.f, 0.f, 0.f, 0.f}, nw[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < nc; ++c) {
             mu_n[c] = mt - (mu_a[c] + mu_s[c]);
             sw[c]   = mu_s[c] * lw[c];
             nw[c]   = mu_n[c] * lw[c];
         }
-        float const ms = average(sw);
-        float const mn = average(nw);
-        float const cc = 1.f / (ms + mn);
-        float const ps = ms * cc;
-        float const pn = mn * cc;
+        float const ms = average(nc, sw);
+        float const mn = average(nc, nw);
+        float const ic = 1.f / (ms + mn);
+        float const ps = ms * ic;
+        float const pn = mn * ic;
         float const r1 = rnd(rs);
         if ((r1 <= 1.f - pn) & (ps > 0.f)) {
-            for (int c = 0; c < 3; ++c) {
+            for (int c = 0; c < nc; ++c) {
                 w[c] = lw[c] * (mu_s[c] / (mt * ps));
             }
             *t_out = t;
             return 1;
         }
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < nc; ++c) {
             lw[c] *= mu_n[c] / (mt * pn);
         }
     }
 }
 
 /* tracking_transmitted with residual ratio tracking; 0 = below the abort threshold */
-static int tracking_transmitted(oracle_medium const* m, tray const* r, float const cm[4], float srs,
-                                int32_t nearest, oracle_random_fn rnd, void* rs, float w[3]) {
-    float const mt = cm[2] + cm[3];
+static int tracking_transmitted(oracle_medium const* m, channels const* cc, tray const* r,
+                                float const cm[4], float srs, int32_t nearest,
+                                oracle_random_fn rnd, void* rs, float* w) {
+    int32_t const nc = cc->nc;
+    float const   mt = cm[2] + cm[3];
     if (mt < MIN_MT) {
         return 1;
     }
@@ -488,10 +516,10 @@ static int tracking_transmitted(oracle_medium const* m, tray const* r, float con
     float       mr       = mt;
     if (minorant > 0.f) {
         float const att = expf(-(r->max_t - r->min_t) * minorant);
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < nc; ++c) {
             w[c] *= att;
         }
-        if (all_below(w)) {
+        if (all_below(nc, w)) {
             return 0;
         }
         mr = mt - minorant;
@@ -507,14 +535,14 @@ static int tracking_transmitted(oracle_medium const* m, tray const* r, float con
         if (t > d) {
             return 1;
         }
-        float mu_a[3], mu_s[3];
-        coefficients(m, r, t, srs, nearest, mu_a, mu_s);
-        for (int c = 0; c < 3; ++c) {
+        float mu_a[4], mu_s[4];
+        coefficients(m, cc, r, t, srs, nearest, mu_a, mu_s);
+        for (int c = 0; c < nc; ++c) {
             float const mu_t = (mu_a[c] + mu_s[c]) - control;
             float const mu_n = mr - mu_t;
             w[c] *= imt * mu_n;
         }
-        if (all_below(w)) {
+        if (all_below(nc, w)) {
             return 0;
         }
     }
@@ -564,20 +592,24 @@ int32_t oracle_medium_overlap(oracle_medium const* m, float const o[3], float co
     return *t0 < *t1;
 }
 
-int32_t oracle_medium_track(oracle_medium const* m, float const o[3], float const d[3], float t0,
-                            float t1, uint32_t depth, int32_t nearest, oracle_random_fn rnd,
-                            void* rs, float* t_out, float w[3]) {
-    tray r;
+int32_t oracle_medium_track_n(oracle_medium const* m, float const o[3], float const d[3],
+                              float t0, float t1, uint32_t depth, int32_t nearest,
+                              oracle_random_fn rnd, void* rs, int32_t nc, float const* cc_a,
+                              float const* cc_s, float* t_out, float* w) {
+    channels const cc = {nc, cc_a, cc_s};
+    tray           r;
     texture_ray(m, o, d, t0, t1, &r);
     float const srs   = oracle_similarity_scale(m->g, depth);
     int32_t     event = 0;
-    w[0] = w[1] = w[2] = 1.f;
+    for (int c = 0; c < nc; ++c) {
+        w[c] = 1.f;
+    }
     for (float const end = t1; r.min_t < end;) {
         float cm[4];
         if (gridtree_intersect(m, &r, cm)) {
             cm[1] *= srs;
             cm[3] *= srs;
-            if (tracking(m, &r, cm, srs, nearest, rnd, rs, t_out, w)) {
+            if (tracking(m, &cc, &r, cm, srs, nearest, rnd, rs, t_out, w)) {
                 event = 1;
                 break;
             }
@@ -585,13 +617,24 @@ int32_t oracle_medium_track(oracle_medium const* m, float const o[3], float cons
         r.min_t = oracle_offset_f(r.max_t);
         r.max_t = end;
     }
-    return all_below(w) ? 2 : event;
+    return all_below(nc, w) ? 2 : event;
 }
 
-int32_t oracle_medium_transmittance(oracle_medium const* m, float const o[3], float const d[3],
-                                    float t0, float t1, uint32_t depth, int32_t nearest,
-                                    oracle_random_fn rnd, void* rs, float tr[3]) {
-    tr[0] = tr[1] = tr[2] = 1.f;
+int32_t oracle_medium_track(oracle_medium const* m, float const o[3], float const d[3], float t0,
+                            float t1, uint32_t depth, int32_t nearest, oracle_random_fn rnd,
+                            void* rs, float* t_out, float w[3]) {
+    return oracle_medium_track_n(m, o, d, t0, t1, depth, nearest, rnd, rs, 3, m->cc_a, m->cc_s,
+                                 t_out, w);
+}
+
+int32_t oracle_medium_transmittance_n(oracle_medium const* m, float const o[3], float const d[3],
+                                      float t0, float t1, uint32_t depth, int32_t nearest,
+                                      oracle_random_fn rnd, void* rs, int32_t nc,
+                                      float const* cc_a, float const* cc_s, float* tr) {
+    channels const cc = {nc, cc_a, cc_s};
+    for (int c = 0; c < nc; ++c) {
+        tr[c] = 1.f;
+    }
     if (oracle_offset_f(t0) >= t1) {
         return 1;
     }
@@ -603,8 +646,10 @@ int32_t oracle_medium_transmittance(oracle_medium const* m, float const o[3], fl
         if (gridtree_intersect(m, &r, cm)) {
             cm[1] *= srs;
             cm[3] *= srs;
-            if (!tracking_transmitted(m, &r, cm, srs, nearest, rnd, rs, tr)) {
-                tr[0] = tr[1] = tr[2] = 0.f;
+            if (!tracking_transmitted(m, &cc, &r, cm, srs, nearest, rnd, rs, tr)) {
+                for (int c = 0; c < nc; ++c) {
+                    tr[c] = 0.f;
+                }
                 return 0;
             }
         }
@@ -612,6 +657,13 @@ int32_t oracle_medium_transmittance(oracle_medium const* m, float const o[3], fl
         r.max_t = end;
     }
     return 1;
+}
+
+int32_t oracle_medium_transmittance(oracle_medium const* m, float const o[3], float const d[3],
+                                    float t0, float t1, uint32_t depth, int32_t nearest,
+                                    oracle_random_fn rnd, void* rs, float tr[3]) {
+    return oracle_medium_transmittance_n(m, o, d, t0, t1, depth, nearest, rnd, rs, 3, m->cc_a,
+                                         m->cc_s, tr);
 }
 
 float oracle_phase_hg(float cos_theta, float g) {

@@ -213,6 +213,9 @@ typedef struct {
 
 int32_t oracle_gridtree_build(float const* density, int32_t const dims[3], float const cc_a[3],
                               float const cc_s[3], oracle_gridtree* out);
+/* uniform != 0: every cell split to its 8^3-voxel leaves (flat majorants). */
+int32_t oracle_majorant_tree_build(float const* density, int32_t const dims[3], float const cc_a[3],
+                                   float const cc_s[3], int32_t uniform, oracle_gridtree* out);
 void    oracle_gridtree_free(oracle_gridtree* t);
 
 typedef struct {
@@ -243,6 +246,16 @@ int32_t oracle_medium_track(oracle_medium const* m, float const o[3], float cons
 int32_t oracle_medium_transmittance(oracle_medium const* m, float const o[3], float const d[3],
                                     float t0, float t1, uint32_t depth, int32_t nearest,
                                     oracle_random_fn rnd, void* rs, float tr[3]);
+/* The same over nc (3 or 4) channels with the given coefficients per unit
+ * density (a hero-wavelength path's four wavelengths) */
+int32_t oracle_medium_track_n(oracle_medium const* m, float const o[3], float const d[3],
+                              float t0, float t1, uint32_t depth, int32_t nearest,
+                              oracle_random_fn rnd, void* rs, int32_t nc, float const* cc_a,
+                              float const* cc_s, float* t_out, float* w);
+int32_t oracle_medium_transmittance_n(oracle_medium const* m, float const o[3], float const d[3],
+                                      float t0, float t1, uint32_t depth, int32_t nearest,
+                                      oracle_random_fn rnd, void* rs, int32_t nc,
+                                      float const* cc_a, float const* cc_s, float* tr);
 float   oracle_phase_hg(float cos_theta, float g);
 void    oracle_sample_phase(float const wo[3], float r0, float r1, float g, float wi[3], float* pdf);
 
@@ -366,7 +379,24 @@ typedef struct {
     int32_t  golden_ratio; /* pixel samples from the Golden_ratio sampler (first_sample 0) */
     float*   samples; /* non-NULL: per sample and grid pixel r, g, b, jitter x, y of the
                          (W + 2 border) x (H + 2 border) grid, sample-major; accum unused */
+    int32_t  spectral; /* hero-wavelength transport over four wavelengths (spectral.c) */
 } oracle_pt_settings;
+
+/* ---- hero-wavelength transport (spectral.c) ---- */
+
+/* the hero's uniform number of the path with generator sequence `sequence` */
+float oracle_spectral_hero(uint64_t seed, uint64_t sequence);
+/* wavelength k (0-3) of a path with hero number u, in nm */
+float oracle_spectral_wavelength(float u, int32_t k);
+/* band weights (red, green, blue) at lambda nm; they sum to one */
+void  oracle_spectral_bands(float lambda, float b[3]);
+/* an rgb value at the wavelength with band weights b */
+float oracle_spectral_upsample(float const b[3], float const x[3]);
+/* the inverse of the bands' Gram matrix over the uniform wavelength measure */
+void  oracle_spectral_response(float minv[9]);
+/* rgb of values at four wavelengths with band weights b */
+void  oracle_spectral_to_rgb(float const minv[9], float const b[4][3], float const v[4],
+                             float rgb[3]);
 
 /* Adds each pixel's radiance sum over the samples into accum (W * H * 3), or
  * with set->samples writes every sample of the border-extended grid there.

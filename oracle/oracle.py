@@ -68,7 +68,7 @@ class _OraclePtSettings(ctypes.Structure):
                 ("num_threads", ctypes.c_uint32), ("split_lights", ctypes.c_int32),
                 ("medium", ctypes.c_void_p), ("instances", ctypes.c_void_p),
                 ("border", ctypes.c_uint32), ("golden_ratio", ctypes.c_int32),
-                ("samples", ctypes.c_void_p)]
+                ("samples", ctypes.c_void_p), ("spectral", ctypes.c_int32)]
 
 
 class _OracleGridtree(ctypes.Structure):
@@ -174,6 +174,32 @@ def lib():
         _lib.oracle_pcg_start.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
         _lib.oracle_pcg_float.restype = ctypes.c_float
         _lib.oracle_pcg_float.argtypes = [ctypes.c_void_p]
+        _lib.oracle_spectral_hero.restype = ctypes.c_float
+        _lib.oracle_spectral_hero.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        _lib.oracle_spectral_wavelength.restype = ctypes.c_float
+        _lib.oracle_spectral_wavelength.argtypes = [ctypes.c_float, ctypes.c_int32]
+        _lib.oracle_spectral_bands.restype = None
+        _lib.oracle_spectral_bands.argtypes = [ctypes.c_float, ctypes.c_void_p]
+        _lib.oracle_spectral_upsample.restype = ctypes.c_float
+        _lib.oracle_spectral_upsample.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.oracle_spectral_response.restype = None
+        _lib.oracle_spectral_response.argtypes = [ctypes.c_void_p]
+        _lib.oracle_spectral_to_rgb.restype = None
+        _lib.oracle_spectral_to_rgb.argtypes = [ctypes.c_void_p] * 4
+        _lib.oracle_medium_overlap.restype = ctypes.c_int32
+        _lib.oracle_medium_overlap.argtypes = [ctypes.POINTER(_OracleMedium), ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_float, ctypes.c_float,
+                                               ctypes.c_void_p, ctypes.c_void_p]
+        for name in ("oracle_medium_track_n", "oracle_medium_transmittance_n"):
+            getattr(_lib, name).restype = ctypes.c_int32
+        _lib.oracle_medium_track_n.argtypes = (
+            [ctypes.c_void_p] * 3 + [ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                     ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_int32] + [ctypes.c_void_p] * 4)
+        _lib.oracle_medium_transmittance_n.argtypes = (
+            [ctypes.c_void_p] * 3 + [ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                     ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_int32] + [ctypes.c_void_p] * 3)
         _lib.oracle_pcg_uint.restype = ctypes.c_uint32
         _lib.oracle_pcg_uint.argtypes = [ctypes.c_void_p]
         _lib.oracle_pixel_sequence.restype = ctypes.c_uint64
@@ -199,6 +225,9 @@ def lib():
                                        ctypes.POINTER(_OraclePtSettings), ctypes.c_void_p]
         _lib.oracle_gridtree_build.restype = ctypes.c_int32
         _lib.oracle_gridtree_build.argtypes = [ctypes.c_void_p] * 4 + [ctypes.POINTER(_OracleGridtree)]
+        _lib.oracle_majorant_tree_build.restype = ctypes.c_int32
+        _lib.oracle_majorant_tree_build.argtypes = ([ctypes.c_void_p] * 4 + [ctypes.c_int32]
+                                                    + [ctypes.POINTER(_OracleGridtree)])
         _lib.oracle_gridtree_free.restype = None
         _lib.oracle_gridtree_free.argtypes = [ctypes.POINTER(_OracleGridtree)]
         _lib.oracle_medium_batch.restype = ctypes.c_int32
@@ -820,9 +849,10 @@ def scene_light_tree(positions, indices, triangle_materials, materials, build,
 class Medium:
     """A density grid (nz, ny, nx) float32, x fastest, filling the box
     [box_lo, box_hi], with absorption / scattering per unit density (rgb) and
-    Henyey-Greenstein g; its majorant gridtree is built by medium.c."""
+    Henyey-Greenstein g; its majorant gridtree is built by medium.c
+    (majorants "flat": every cell split to its 8^3-voxel leaves)."""
 
-    def __init__(self, density, box_lo, box_hi, cc_a, cc_s, g):
+    def __init__(self, density, box_lo, box_hi, cc_a, cc_s, g, majorants="gridtree"):
         self.density = np.ascontiguousarray(density, np.float32)
         assert self.density.ndim == 3
         nz, ny, nx = self.density.shape
@@ -831,10 +861,13 @@ class Medium:
         dims = np.array(self.dims, np.int32)
         a = np.asarray(cc_a, np.float32)
         s_ = np.asarray(cc_s, np.float32)
-        st = lib().oracle_gridtree_build(_ptr(self.density), _ptr(dims), _ptr(a), _ptr(s_),
-                                         ctypes.byref(self.tree))
+        if majorants not in ("gridtree", "flat"):
+            raise ValueError(f"majorants {majorants!r}")
+        st = lib().oracle_majorant_tree_build(_ptr(self.density), _ptr(dims), _ptr(a), _ptr(s_),
+                                              1 if majorants == "flat" else 0,
+                                              ctypes.byref(self.tree))
         if st:
-            raise RuntimeError(f"oracle_gridtree_build: status {st}")
+            raise RuntimeError(f"oracle_majorant_tree_build: status {st}")
         f3 = ctypes.c_float * 3
         self.c = _OracleMedium(_ptr(self.density), (ctypes.c_int32 * 3)(*self.dims),
                                ctypes.pointer(self.tree), f3(*box_lo), f3(*box_hi), f3(*a),
@@ -973,6 +1006,84 @@ def pcg_float(seed, sequence, n):
     return np.array([lib().oracle_pcg_float(ctypes.byref(g)) for _ in range(n)], np.float32)
 
 
+def spectral_hero(seed, sequence) -> float:
+    """The hero wavelength's uniform number of a path (spectral.c)."""
+    return float(lib().oracle_spectral_hero(seed, sequence))
+
+
+def spectral_wavelength(u, k) -> float:
+    return float(lib().oracle_spectral_wavelength(u, k))
+
+
+def spectral_bands(lam) -> np.ndarray:
+    b = np.zeros(3, np.float32)
+    lib().oracle_spectral_bands(float(lam), _ptr(b))
+    return b
+
+
+def spectral_upsample(b, x) -> float:
+    bb = np.ascontiguousarray(b, np.float32)
+    xx = np.ascontiguousarray(x, np.float32)
+    return float(lib().oracle_spectral_upsample(_ptr(bb), _ptr(xx)))
+
+
+def spectral_response() -> np.ndarray:
+    m = np.zeros(9, np.float32)
+    lib().oracle_spectral_response(_ptr(m))
+    return m.reshape(3, 3)
+
+
+def spectral_to_rgb(b4, v4) -> np.ndarray:
+    """rgb of values v4 at four wavelengths with band weights b4 (4, 3)."""
+    m = np.ascontiguousarray(spectral_response(), np.float32)
+    b = np.ascontiguousarray(b4, np.float32)
+    v = np.ascontiguousarray(v4, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().oracle_spectral_to_rgb(_ptr(m), _ptr(b), _ptr(v), _ptr(out))
+    return out
+
+
+def medium_channels(medium: "Medium", rays, cc_a, cc_s, depth=0, seed=1, transmittance=False,
+                    nearest=False):
+    """Tracking (or with transmittance, ratio tracking) of rays (n, 8)
+    through the medium over len(cc_a) channels with those coefficients per
+    unit density; generator (seed, index) per ray -> (n, 2 + nc): {event or
+    transmitted, t, weights}."""
+    a = np.ascontiguousarray(cc_a, np.float32)
+    s_ = np.ascontiguousarray(cc_s, np.float32)
+    nc = len(a)
+    r = np.ascontiguousarray(rays, np.float32)
+    out = np.zeros((len(r), 2 + nc), np.float32)
+    rnd = ctypes.cast(lib().oracle_pcg_float, ctypes.c_void_p)
+    t0 = ctypes.c_float()
+    t1 = ctypes.c_float()
+    w = np.zeros(nc, np.float32)
+    ts = ctypes.c_float()
+    for i, ray in enumerate(r):
+        o = np.ascontiguousarray(ray[0:3])
+        d = np.ascontiguousarray(ray[4:7])
+        w[:] = 1.0
+        if not lib().oracle_medium_overlap(ctypes.byref(medium.c), _ptr(o), _ptr(d), float(ray[3]),
+                                           float(ray[7]), ctypes.byref(t0), ctypes.byref(t1)):
+            out[i, 0] = 1.0 if transmittance else 0.0
+            out[i, 2:] = 1.0
+            continue
+        g = _OraclePcg()
+        lib().oracle_pcg_start(ctypes.byref(g), seed, i)
+        if transmittance:
+            out[i, 0] = lib().oracle_medium_transmittance_n(
+                ctypes.byref(medium.c), _ptr(o), _ptr(d), t0.value, t1.value, depth,
+                1 if nearest else 0, rnd, ctypes.byref(g), nc, _ptr(a), _ptr(s_), _ptr(w))
+        else:
+            out[i, 0] = lib().oracle_medium_track_n(
+                ctypes.byref(medium.c), _ptr(o), _ptr(d), t0.value, t1.value, depth,
+                1 if nearest else 0, rnd, ctypes.byref(g), nc, _ptr(a), _ptr(s_),
+                ctypes.byref(ts), _ptr(w))
+            out[i, 1] = ts.value
+        out[i, 2:] = w
+    return out
+
+
 def camera_rays(width, height, camera, pixels, uv):
     """Perspective::generate_ray for pixels (n, 2) int (x, y; may lie in a
     filter border) with pixel samples uv (n, 2): (n, 8) float32 rays."""
@@ -1018,13 +1129,14 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
            tree: LightTree, first_sample=0, min_bounces=4, max_bounces=8, seed=1, mode=PT_MIS,
            threads=None, split_lights=True, medium: Medium | None = None,
            instances: PtInstances | None = None, border: int = 0, samples: bool = False,
-           golden_ratio: bool = False):
+           golden_ratio: bool = False, spectral: bool = False):
     """Radiance sums per pixel, (height, width, 3) float64, of the CPU path
     tracer (pt.c).  camera: {eye, target, up, fov_deg (vertical)} or sprout's
     {position, rotation, fov_deg (horizontal)}; tree: the scene's light tree;
     medium: a participating medium or None; instances: mesh instances beside
     the triangles, or None; golden_ratio: pixel samples from sprout's
-    Golden_ratio sampler.  samples=True returns every sample instead: (spp,
+    Golden_ratio sampler; spectral: hero-wavelength transport over four
+    wavelengths (spectral.c).  samples=True returns every sample instead: (spp,
     height + 2 border, width + 2 border, 5) float32 of r, g, b and the pixel
     sample x, y, over the grid extended by border."""
     pos = np.ascontiguousarray(positions, np.float32)
@@ -1039,7 +1151,8 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
                            1 if split_lights else 0,
                            ctypes.cast(ctypes.pointer(medium.c), ctypes.c_void_p) if medium else None,
                            ctypes.cast(ctypes.pointer(instances.c), ctypes.c_void_p) if instances else None,
-                           border, 1 if golden_ratio else 0, _ptr(out) if samples else None)
+                           border, 1 if golden_ratio else 0, _ptr(out) if samples else None,
+                           1 if spectral else 0)
     accum = None if samples else np.zeros((height, width, 3), np.float64)
     status = lib().oracle_render(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(tmat),
                                  _ptr(mats), len(mats), ctypes.byref(tree.c), ctypes.byref(cam),

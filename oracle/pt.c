@@ -142,6 +142,7 @@ typedef struct {
     uint32_t                next_row;
     pthread_mutex_t         lock;
     int32_t                 status;
+    float                   minv[9]; /* spectral: the film's response */
 } scene_t;
 
 static v3 vertex(scene_t const* s, uint32_t i) {
@@ -307,7 +308,7 @@ static float predivided_power_heuristic(float f, float g) {
 
 /* a shadow ray of the current vertex, resolved after its direction sample */
 typedef struct {
-    float contrib[3];
+    float contrib[4]; /* rgb, or a spectral path's four wavelengths */
     v3    o;
     v3    wi;
     float dist;
@@ -328,7 +329,13 @@ typedef struct {
  * (H + 2 border) grid, image pixel (gx - border, gy - border); its generator
  * is started at sample * (grid pixels) + gy * (grid width) + gx, as the
  * reference's worker numbers the pixels of its filter-extended tiles
- * (rendering_worker.cpp:93-104).  rgb += radiance; jitter = the pixel offsets. */
+ * (rendering_worker.cpp:93-104).  rgb += radiance; jitter = the pixel offsets.
+ *
+ * Spectral (spectral.c): the throughput holds the path's four wavelengths,
+ * its hero from a hash of (seed, the generator's sequence), so the generator
+ * draws as in rgb; colours, emission and the medium's coefficients are
+ * upsampled to the wavelengths, and every contribution reaches rgb through
+ * the film's response. */
 static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
                     float const* pixel_uv, double rgb[3], float jitter[2]) {
     oracle_pt_settings const* set    = s->set;
@@ -341,8 +348,23 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
     int32_t const  x    = (int32_t)gx - (int32_t)set->border;
     int32_t const  y    = (int32_t)gy - (int32_t)set->border;
 
-    pcg rng;
-    pcg_start(&rng, set->seed, (uint64_t)sample * ((uint64_t)Wg * Hg) + (uint64_t)gy * Wg + gx);
+    pcg            rng;
+    uint64_t const sequence = (uint64_t)sample * ((uint64_t)Wg * Hg) + (uint64_t)gy * Wg + gx;
+    pcg_start(&rng, set->seed, sequence);
+    int32_t const spectral = set->spectral;
+    int32_t const nc       = spectral ? 4 : 3;
+    float         wb[4][3]; /* spectral: band weights of the four wavelengths */
+    float         cca[4], ccs[4];
+    if (spectral) {
+        float const hero = oracle_spectral_hero(set->seed, sequence);
+        for (int k = 0; k < 4; ++k) {
+            oracle_spectral_bands(oracle_spectral_wavelength(hero, k), wb[k]);
+            if (medium) {
+                cca[k] = oracle_spectral_upsample(wb[k], medium->cc_a);
+                ccs[k] = oracle_spectral_upsample(wb[k], medium->cc_s);
+            }
+        }
+    }
     float jx = pcg_float(&rng);
     float jy = pcg_float(&rng);
     if (pixel_uv) {
@@ -359,7 +381,7 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
     v3 o = mk(oo[0], oo[1], oo[2]);
     v3 d = mk(dd[0], dd[1], dd[2]);
 
-    float    throughput[3] = {1.f, 1.f, 1.f};
+    float    throughput[4] = {1.f, 1.f, 1.f, 1.f};
     float    radiance[3]   = {0.f, 0.f, 0.f};
     float    bsdf_pdf      = 0.f;
     int      singular      = 1;
@@ -390,10 +412,11 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
             float       t0, t1;
             if (oracle_medium_overlap(medium, oa, da, 0.f, SENTINEL == tri ? RAY_MAX_T : t, &t0, &t1)) {
                 float         ts = 0.f;
-                float         w[3];
-                int32_t const e = oracle_medium_track(medium, oa, da, t0, t1, depth, nearest,
-                                                      oracle_pcg_float, &rng, &ts, w);
-                for (int k = 0; k < 3; ++k) {
+                float         w[4];
+                int32_t const e = oracle_medium_track_n(
+                    medium, oa, da, t0, t1, depth, nearest, oracle_pcg_float, &rng, nc,
+                    spectral ? cca : medium->cc_a, spectral ? ccs : medium->cc_s, &ts, w);
+                for (int k = 0; k < nc; ++k) {
                     throughput[k] *= w[k];
                 }
                 if (2 == e) {
@@ -453,8 +476,19 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
                             weight = 0.f;
                         }
                     }
-                    for (int k = 0; k < 3; ++k) {
-                        radiance[k] += throughput[k] * (weight * m->emission[k]);
+                    if (spectral) {
+                        float c4[4], rgb[3];
+                        for (int k = 0; k < 4; ++k) {
+                            c4[k] = throughput[k] * (weight * oracle_spectral_upsample(wb[k], m->emission));
+                        }
+                        oracle_spectral_to_rgb(s->minv, (float const(*)[3])wb, c4, rgb);
+                        for (int k = 0; k < 3; ++k) {
+                            radiance[k] += rgb[k];
+                        }
+                    } else {
+                        for (int k = 0; k < 3; ++k) {
+                            radiance[k] += throughput[k] * (weight * m->emission[k]);
+                        }
                     }
                 }
                 break;
@@ -472,11 +506,14 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
                 break;
             }
             if (depth >= set->min_bounces) {
-                float const q = fmaxf(throughput[0], fmaxf(throughput[1], throughput[2]));
+                float q = fmaxf(throughput[0], fmaxf(throughput[1], throughput[2]));
+                if (spectral) {
+                    q = fmaxf(q, throughput[3]);
+                }
                 if (pcg_float(&rng) > q) {
                     break;
                 }
-                for (int k = 0; k < 3; ++k) {
+                for (int k = 0; k < nc; ++k) {
                     throughput[k] /= q;
                 }
             }
@@ -539,10 +576,20 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
                                                 : 1.f / light_pdf;
                     oracle_material const* lm = &s->materials[s->tri_material[lt]];
                     shadow_t*              sh = &shadows[num_shadows];
-                    for (int c = 0; c < 3; ++c) {
-                        sh->contrib[c] = throughput[c] * (weight * (lm->emission[c] * f[c]));
+                    if (spectral) {
+                        for (int c = 0; c < 4; ++c) {
+                            float const el = oracle_spectral_upsample(wb[c], lm->emission);
+                            float const fl = oracle_spectral_upsample(wb[c], f);
+                            sh->contrib[c] = throughput[c] * (weight * (el * fl));
+                        }
+                    } else {
+                        for (int c = 0; c < 3; ++c) {
+                            sh->contrib[c] = throughput[c] * (weight * (lm->emission[c] * f[c]));
+                        }
+                        sh->contrib[3] = 0.f;
                     }
-                    if (sh->contrib[0] > 0.f || sh->contrib[1] > 0.f || sh->contrib[2] > 0.f) {
+                    if (sh->contrib[0] > 0.f || sh->contrib[1] > 0.f || sh->contrib[2] > 0.f ||
+                        sh->contrib[3] > 0.f) {
                         uint32_t occ = 0;
                         if (occluded(s, p_o, wi, dist, &occ)) {
                             return ORACLE_ERR_STACK;
@@ -567,19 +614,32 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
         /* the vertex's unoccluded shadow rays, after its direction sample */
         for (uint32_t k = 0; k < num_shadows; ++k) {
             shadow_t const* sh    = &shadows[k];
-            float           tr[3] = {1.f, 1.f, 1.f};
+            float           tr[4] = {1.f, 1.f, 1.f, 1.f};
             if (medium) {
                 float const oa[3] = {sh->o.x, sh->o.y, sh->o.z};
                 float const da[3] = {sh->wi.x, sh->wi.y, sh->wi.z};
                 float       t0, t1;
                 if (oracle_medium_overlap(medium, oa, da, 0.f, sh->dist, &t0, &t1) &&
-                    !oracle_medium_transmittance(medium, oa, da, t0, t1, depth, nearest,
-                                                 oracle_pcg_float, &rng, tr)) {
+                    !oracle_medium_transmittance_n(medium, oa, da, t0, t1, depth, nearest,
+                                                   oracle_pcg_float, &rng, nc,
+                                                   spectral ? cca : medium->cc_a,
+                                                   spectral ? ccs : medium->cc_s, tr)) {
                     continue;
                 }
             }
-            for (int c = 0; c < 3; ++c) {
-                radiance[c] += tr[c] * sh->contrib[c];
+            if (spectral) {
+                float c4[4], rgb[3];
+                for (int c = 0; c < 4; ++c) {
+                    c4[c] = tr[c] * sh->contrib[c];
+                }
+                oracle_spectral_to_rgb(s->minv, (float const(*)[3])wb, c4, rgb);
+                for (int c = 0; c < 3; ++c) {
+                    radiance[c] += rgb[c];
+                }
+            } else {
+                for (int c = 0; c < 3; ++c) {
+                    radiance[c] += tr[c] * sh->contrib[c];
+                }
             }
         }
 
@@ -594,8 +654,14 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
         }
         translucent = scatter;
         bsdf_pdf    = bs[6];
-        for (int k = 0; k < 3; ++k) {
-            throughput[k] *= bs[3 + k] / bs[6];
+        if (spectral) {
+            for (int k = 0; k < 4; ++k) {
+                throughput[k] *= oracle_spectral_upsample(wb[k], bs + 3) / bs[6];
+            }
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                throughput[k] *= bs[3 + k] / bs[6];
+            }
         }
         v3 const wi = mk(bs[0], bs[1], bs[2]);
         o = scatter ? hit_p
@@ -720,6 +786,7 @@ int32_t oracle_render(float const* positions, uint32_t num_vertices, uint32_t co
     }
 
     oracle_perspective_setup(cam->width, cam->height, cam->fov_deg, &s.persp);
+    oracle_spectral_response(s.minv);
 
     pthread_mutex_init(&s.lock, NULL);
     uint32_t nt = set->num_threads ? set->num_threads : 1;

@@ -104,7 +104,8 @@ class MiGuideSettings(ctypes.Structure):
     _fields_ = [("guide", ctypes.c_uint32), ("fraction", ctypes.c_float),
                 ("train", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
                 ("batch", ctypes.c_uint32), ("learning_rate", ctypes.c_float),
-                ("record_depth", ctypes.c_uint32), ("guide_depth", ctypes.c_uint32)]
+                ("record_depth", ctypes.c_uint32), ("guide_depth", ctypes.c_uint32),
+                ("medium", ctypes.c_uint32)]
 
 
 MI_MATERIAL_SUBSTITUTE = 0
@@ -113,6 +114,8 @@ MI_MATERIAL_LIGHT = 2
 MI_RENDER_NO_GRAPH = 1
 MI_RENDER_NO_LIGHT_SPLIT = 2
 MI_RENDER_GOLDEN_RATIO = 4
+MI_RENDER_REGENERATE = 8
+MI_RENDER_SPECTRAL = 16
 
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int32
@@ -160,6 +163,11 @@ SIGNATURES = {
     "mi_light_tree_build": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _pu32]),
     "mi_light_part_tree_build": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp,
                                         _vp, _pu32, _pu32]),
+    "mi_light_tree_build_device": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp,
+                                          _pu32, _vp]),
+    "mi_light_part_trees_build_device": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp,
+                                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mi_scene_update_positions": (_i32, [_u32, _vp, _u32, _vp, _vp]),
     "mi_light_tree_sample": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp]),
     "mi_light_tree_pdf": (_i32, [_u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp]),
     "mi_scene_light_tree": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _pu32, _pu32]),
@@ -170,6 +178,8 @@ SIGNATURES = {
     "mi_instances_download": (_i32, [_u32, _vp, _vp, _vp, _pu32, _pu32]),
     "mi_scene_set_medium": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp, _f32]),
     "mi_medium_gridtree": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _pu32, _vp, _pu32]),
+    "mi_medium_majorant_tree": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _pu32, _vp, _pu32]),
+    "mi_scene_set_medium_majorants": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _u32]),
     "mi_medium_track": (_i32, [_u32, _vp, _u32, ctypes.c_uint64, _u32, _u32, _vp, _vp]),
     "mi_ggx_e_table": (_i32, [_vp, _vp]),
     "mi_substitute_evaluate": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),

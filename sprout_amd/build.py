@@ -16,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsprout_gpu.so")
 
 SOURCES = ["capi.cpp", "gen.cpp", "light_tree.cpp", "medium.cpp", "bvh_device.hip", "bvh_build.hip", "trace.hip",
+           "light_build.hip",
            "render.hip", "instances.hip",
            "guide.hip"]
 

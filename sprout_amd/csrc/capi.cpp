@@ -4,6 +4,7 @@
 #include "common.hpp"
 #include "guide.hpp"
 #include "instances.hpp"
+#include "light_build.hip.hpp"
 #include "light_tree.hpp"
 #include "render.hpp"
 #include "trace.hpp"
@@ -32,6 +33,7 @@ struct Context {
     TraceResources            render_trace;  // the renderer's own ray counter and spill area
     RenderState               render_state;
     float*                    e_table = nullptr;  // GGX energy table, made on first use
+    Light_forest_scratch      light_scratch;       // the device light-tree builder's
 };
 
 Context* context = nullptr;
@@ -181,6 +183,7 @@ int32_t mi_release(void) {
     }
     context->render_state.release();
     context->render_trace.release();
+    context->light_scratch.release();
     (void)hipFree(context->e_table);
     context->trace.release();
     delete context;
@@ -582,6 +585,22 @@ int32_t mi_scene_update_instances(uint32_t scene, float const* world_to_object,
     return update_instances(*sc, world_to_object, object_to_world, num_instances);
 }
 
+int32_t mi_scene_update_positions(uint32_t scene, float const* positions, uint32_t num_vertices,
+                                  double* light_ms, void* stream) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT,
+                    "mi_scene_update_positions: bad handle %u", scene);
+    }
+    if (!positions) {
+        return fail(MI_ERR_ARGS, "mi_scene_update_positions: null positions");
+    }
+    (void)hipDeviceSynchronize();  // renders in flight read the old geometry and lights
+    return update_positions(*sc, positions, num_vertices, context->light_scratch,
+                            as_stream(stream), light_ms);
+}
+
 int32_t mi_scene_release(uint32_t scene) {
     clear_error();
     RenderScene* sc = lookup_scene(scene);
@@ -851,6 +870,96 @@ int32_t mi_light_tree_build(float const* box_min, float const* box_max, float co
     return MI_OK;
 }
 
+int32_t mi_light_tree_build_device(float const* box_min, float const* box_max, float const* cones,
+                                   float const* powers, uint8_t const* two_sided,
+                                   uint32_t num_lights, mi_light_node* nodes, uint32_t* middles,
+                                   uint32_t* mapping, uint32_t* orders, uint32_t* num_nodes,
+                                   void* stream) {
+    clear_error();
+    if (!context) {
+        return fail(MI_ERR_NO_CONTEXT, "mi_light_tree_build_device: call mi_init first");
+    }
+    if (!num_nodes || (num_lights && (!box_min || !box_max || !cones || !powers || !two_sided ||
+                                      !nodes || !middles || !mapping || !orders))) {
+        return fail(MI_ERR_ARGS, "mi_light_tree_build_device: bad arguments");
+    }
+    *num_nodes = 0;
+    if (0 == num_lights) {
+        return MI_OK;
+    }
+    uint32_t const   first[2] = {0, num_lights};
+    Light_forest     in;
+    in.set        = {box_min, box_max, cones, powers, two_sided};
+    in.first      = first;
+    in.num_roots  = 1;
+    in.num_lights = num_lights;
+    Light_forest_out out;
+    out.nodes          = nodes;
+    out.middles        = middles;
+    out.mapping        = mapping;
+    out.orders         = orders;
+    out.num_nodes_host = num_nodes;
+    int32_t const st   = build_light_forest(in, context->light_scratch, out, as_stream(stream));
+    if (MI_ERR_UNSPLITTABLE == st) {
+        return fail(st, "mi_light_tree_build_device: a node no candidate splits (build on the host)");
+    }
+    if (st) {
+        return fail(st, "mi_light_tree_build_device: %s", hipGetErrorString(hipGetLastError()));
+    }
+    return MI_OK;
+}
+
+int32_t mi_light_part_trees_build_device(float const* box_min, float const* box_max,
+                                         float const* cones, float const* powers,
+                                         uint32_t num_lights, uint32_t const* first,
+                                         uint32_t num_parts, float const* part_cones,
+                                         mi_light_node* nodes, uint32_t* middles,
+                                         uint32_t* mapping, uint32_t* orders,
+                                         uint32_t* cdf_offset, float* cdfs, uint32_t* num_nodes,
+                                         uint32_t* num_cdfs, void* stream) {
+    clear_error();
+    if (!context) {
+        return fail(MI_ERR_NO_CONTEXT, "mi_light_part_trees_build_device: call mi_init first");
+    }
+    if (!first || (num_parts && (!box_min || !box_max || !cones || !powers || !part_cones ||
+                                 !nodes || !middles || !mapping || !orders || !cdf_offset ||
+                                 !cdfs || !num_nodes || !num_cdfs))) {
+        return fail(MI_ERR_ARGS, "mi_light_part_trees_build_device: bad arguments");
+    }
+    if (0 == num_parts) {
+        return MI_OK;
+    }
+    if (0 != first[0] || first[num_parts] != num_lights) {
+        return fail(MI_ERR_ARGS, "mi_light_part_trees_build_device: first must run from 0 to %u",
+                    num_lights);
+    }
+    for (uint32_t r = 0; r < num_parts; ++r) {
+        if (first[r + 1] <= first[r]) {
+            return fail(MI_ERR_ARGS, "mi_light_part_trees_build_device: part %u is empty", r);
+        }
+    }
+    Light_forest in;
+    in.set        = {box_min, box_max, cones, powers, nullptr};
+    in.first      = first;
+    in.part_cones = part_cones;
+    in.num_roots  = num_parts;
+    in.num_lights = num_lights;
+    in.part       = true;
+    Light_forest_out out;
+    out.nodes          = nodes;
+    out.middles        = middles;
+    out.mapping        = mapping;
+    out.orders         = orders;
+    out.cdf_offset     = cdf_offset;
+    out.cdfs           = cdfs;
+    out.num_cdfs       = num_cdfs;
+    out.num_nodes_host = num_nodes;
+    if (int32_t const st = build_light_forest(in, context->light_scratch, out, as_stream(stream)); st) {
+        return fail(st, "mi_light_part_trees_build_device: %s", hipGetErrorString(hipGetLastError()));
+    }
+    return MI_OK;
+}
+
 int32_t mi_light_part_tree_build(float const* box_min, float const* box_max, float const* cones,
                                  float const* powers, uint32_t num_lights, float const* part_cone,
                                  mi_light_node* nodes, uint32_t* middles, uint32_t* mapping,
@@ -1045,21 +1154,49 @@ int32_t mi_scene_set_medium(uint32_t scene, float const* density, uint32_t const
     return set_medium(*sc, density, dims, box_lo, box_hi, cc_a, cc_s, g);
 }
 
+int32_t mi_scene_set_medium_majorants(uint32_t scene, float const* density, uint32_t const dims[3],
+                                      float const box_lo[3], float const box_hi[3],
+                                      float const cc_a[3], float const cc_s[3], float g,
+                                      uint32_t majorants) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT,
+                    "mi_scene_set_medium_majorants: bad handle %u", scene);
+    }
+    if (density && (!dims || !box_lo || !box_hi || !cc_a || !cc_s)) {
+        return fail(MI_ERR_ARGS, "mi_scene_set_medium_majorants: null argument");
+    }
+    (void)hipDeviceSynchronize();  // renders in flight may read the old medium
+    return set_medium(*sc, density, dims, box_lo, box_hi, cc_a, cc_s, g, majorants);
+}
+
 int32_t mi_medium_gridtree(float const* density, uint32_t const dims[3], float const cc_a[3],
                            float const cc_s[3], uint32_t* num_cells, uint32_t* nodes,
                            uint32_t* num_nodes, float* data, uint32_t* num_data) {
+    return mi_medium_majorant_tree(density, dims, cc_a, cc_s, MI_MAJORANTS_GRIDTREE, num_cells,
+                                   nodes, num_nodes, data, num_data);
+}
+
+int32_t mi_medium_majorant_tree(float const* density, uint32_t const dims[3], float const cc_a[3],
+                                float const cc_s[3], uint32_t majorants, uint32_t* num_cells,
+                                uint32_t* nodes, uint32_t* num_nodes, float* data,
+                                uint32_t* num_data) {
     clear_error();
+    if (majorants > MI_MAJORANTS_FLAT) {
+        return fail(MI_ERR_ARGS, "mi_medium_majorant_tree: majorants %u unknown", majorants);
+    }
     if (!density || !dims || !cc_a || !cc_s || !num_nodes || !num_data) {
-        return fail(MI_ERR_ARGS, "mi_medium_gridtree: null argument");
+        return fail(MI_ERR_ARGS, "mi_medium_majorant_tree: null argument");
     }
     for (int k = 0; k < 3; ++k) {
         if (0 == dims[k] || dims[k] > 4096) {
-            return fail(MI_ERR_ARGS, "mi_medium_gridtree: dimension %u out of [1, 4096]", dims[k]);
+            return fail(MI_ERR_ARGS, "mi_medium_majorant_tree: dimension %u out of [1, 4096]", dims[k]);
         }
     }
     int32_t const idims[3] = {int32_t(dims[0]), int32_t(dims[1]), int32_t(dims[2])};
     HostGridtree  tree;
-    build_gridtree(density, idims, cc_a, cc_s, tree);
+    build_gridtree(density, idims, cc_a, cc_s, tree, MI_MAJORANTS_FLAT == majorants);
     if (num_cells) {
         for (int k = 0; k < 3; ++k) {
             num_cells[k] = uint32_t(tree.num_cells[k]);
@@ -1067,7 +1204,7 @@ int32_t mi_medium_gridtree(float const* density, uint32_t const dims[3], float c
     }
     if (nodes) {
         if (*num_nodes < tree.nodes.size()) {
-            return fail(MI_ERR_ARGS, "mi_medium_gridtree: %zu nodes do not fit in %u",
+            return fail(MI_ERR_ARGS, "mi_medium_majorant_tree: %zu nodes do not fit in %u",
                         tree.nodes.size(), *num_nodes);
         }
         std::memcpy(nodes, tree.nodes.data(), tree.nodes.size() * sizeof(uint32_t));

@@ -14,6 +14,8 @@
 
 #include "light_tree.hpp"
 
+#include "light_build.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -22,198 +24,21 @@ namespace mi {
 
 namespace {
 
-constexpr float    Pi                    = 3.14159265358979323846f;
-constexpr uint32_t Scene_sweep_threshold = 128;
-constexpr uint32_t Part_sweep_threshold  = 32;
-constexpr uint32_t Num_slices            = 16;
-
-struct F3 {
-    float v[3];
-    float&       operator[](int i) { return v[i]; }
-    float const& operator[](int i) const { return v[i]; }
-};
-
-struct Box {
-    F3 lo{{3.40282347e+38f, 3.40282347e+38f, 3.40282347e+38f}};
-    F3 hi{{-3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f}};
-
-    void merge(Box const& b) {
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = std::min(lo[k], b.lo[k]);
-            hi[k] = std::max(hi[k], b.hi[k]);
-        }
-    }
-    F3 extent() const { return {{hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]}}; }
-    F3 position() const {
-        return {{0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2])}};
-    }
-    float surface_area() const {
-        F3 const d = extent();
-        return 2.f * (d[0] * d[1] + d[0] * d[2] + d[1] * d[2]);
-    }
-};
-
-struct Cone {
-    float v[4] = {1.f, 1.f, 1.f, 1.f};  // float4(1) is the empty cone
-};
-
-float dot3(float const* a, float const* b) {
-    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
-}
-
-// cone::merge, base/math/cone.inl:17-60
-Cone merge(Cone a, Cone b) {
-    if (1.f == a.v[0] && 1.f == a.v[1] && 1.f == a.v[2] && 1.f == a.v[3]) {
-        return b;
-    }
-    if (0 == std::memcmp(a.v, b.v, sizeof(a.v))) {
-        return a;
-    }
-    float a_angle = std::acos(a.v[3]);
-    float b_angle = std::acos(b.v[3]);
-    if (b_angle > a_angle) {
-        std::swap(a, b);
-        std::swap(a_angle, b_angle);
-    }
-    float const d_angle = std::acos(std::min(std::max(dot3(a.v, b.v), -1.f), 1.f));
-    if (std::min(d_angle + b_angle, Pi) <= a_angle) {
-        return a;
-    }
-    float const o_angle = (a_angle + d_angle + b_angle) / 2.f;
-    // (the reference computes, and drops, a full-sphere cone when o_angle >= pi)
-    float const r_angle = o_angle - a_angle;
-
-    float c[3] = {a.v[1] * b.v[2] - a.v[2] * b.v[1], a.v[2] * b.v[0] - a.v[0] * b.v[2],
-                  a.v[0] * b.v[1] - a.v[1] * b.v[0]};
-    float const cc = dot3(c, c);
-    if (!(cc > 0.f)) {
-        // opposite axes: the reference normalizes a zero vector here and gets
-        // a NaN cone; the cone that bounds both is the whole sphere
-        Cone r = a;
-        r.v[3] = -1.f;
-        return r;
-    }
-    float const cl = 1.f / std::sqrt(cc);
-    for (float& x : c) {
-        x *= cl;
-    }
-    // set_rotation(rot, c, r_angle)
-    float const cs = std::cos(r_angle);
-    float const sn = std::sin(r_angle);
-    float const t  = 1.f - cs;
-    float       m[3][3];
-    m[0][0]    = cs + c[0] * c[0] * t;
-    m[1][1]    = cs + c[1] * c[1] * t;
-    m[2][2]    = cs + c[2] * c[2] * t;
-    float tmp1 = c[0] * c[1] * t;
-    float tmp2 = c[2] * sn;
-    m[1][0]    = tmp1 + tmp2;
-    m[0][1]    = tmp1 - tmp2;
-    tmp1       = c[0] * c[2] * t;
-    tmp2       = c[1] * sn;
-    m[2][0]    = tmp1 - tmp2;
-    m[0][2]    = tmp1 + tmp2;
-    tmp1       = c[1] * c[2] * t;
-    tmp2       = c[0] * sn;
-    m[2][1]    = tmp1 + tmp2;
-    m[1][2]    = tmp1 - tmp2;
-    // transform_vector(rot, a.xyz): row vector times matrix
-    float ax[3];
-    for (int j = 0; j < 3; ++j) {
-        ax[j] = a.v[0] * m[0][j] + a.v[1] * m[1][j] + a.v[2] * m[2][j];
-    }
-    float const al = 1.f / std::sqrt(dot3(ax, ax));
-    Cone        r;
-    r.v[0] = ax[0] * al;
-    r.v[1] = ax[1] * al;
-    r.v[2] = ax[2] * al;
-    r.v[3] = std::cos(o_angle);
-    return r;
-}
-
-float cone_cost(float cos) {
-    float const o   = std::acos(cos);
-    float const w   = std::min(o + (Pi / 2.f), Pi);
-    float const sin = std::sin(o);
-    float const b   = (Pi / 2.f) * (2.f * w * sin - std::cos(o - 2.f * w) - 2.f * o * sin + cos);
-    return (2.f * Pi) * (1.f - cos) + b;
-}
+using lb::Box;
+using lb::Candidate;
+using lb::Cone;
 
 struct Lights {
-    std::vector<Box>     boxes;
-    std::vector<Cone>    cones;
+    std::vector<float>   lo, hi;  // 3 per light
+    std::vector<float>   cones;   // 4 per light
     std::vector<float>   powers;
     std::vector<uint8_t> two_sided;
-};
 
-float variance(uint32_t const* lights, uint32_t begin, uint32_t end, Lights const& set) {
-    float ap  = 0.f;
-    float aps = 0.f;
-    for (uint32_t i = begin, n = 0; i < end; ++i) {
-        float const p = set.powers[lights[i]];
-        if (p > 0.f) {
-            float const in = 1.f / float(++n);
-            ap += (p - ap) * in;
-            aps += (p * p - aps) * in;
-        }
+    lb::Light_set view() const {
+        return {lo.data(), hi.data(), cones.data(), powers.data(), two_sided.data()};
     }
-    return std::abs(aps - ap * ap);
-}
-
-struct Candidate {
-    Box   box0, box1;
-    Cone  cone0, cone1;
-    float power0 = 0.f, power1 = 0.f;
-    float d      = 0.f;
-    float cost   = 0.f;
-    int   axis   = 0;
-    bool  two0 = false, two1 = false;
-    bool  exhausted = false;
-
-    bool behind(F3 const& p) const { return p[axis] < d; }
-
-    void evaluate(uint32_t begin, uint32_t end, uint32_t const* lights, Box const& bounds,
-                  float cone_weight, Lights const& set) {
-        uint32_t n0 = 0, n1 = 0;
-        Box      b0, b1;
-        Cone     c0, c1;
-        bool     t0 = false, t1 = false;
-        float    p0 = 0.f, p1 = 0.f;
-        for (uint32_t i = begin; i < end; ++i) {
-            uint32_t const l     = lights[i];
-            float const    power = set.powers[l];
-            if (0.f == power) {
-                continue;
-            }
-            if (behind(set.boxes[l].hi)) {
-                ++n0;
-                b0.merge(set.boxes[l]);
-                c0 = merge(c0, set.cones[l]);
-                t0 |= 0 != set.two_sided[l];
-                p0 += power;
-            } else {
-                ++n1;
-                b1.merge(set.boxes[l]);
-                c1 = merge(c1, set.cones[l]);
-                t1 |= 0 != set.two_sided[l];
-                p1 += power;
-            }
-        }
-        F3 const    extent = bounds.extent();
-        float const reg    = std::max(std::max(extent[0], extent[1]), extent[2]) / extent[axis];
-        float const sa     = bounds.surface_area();
-        box0 = b0, box1 = b1, cone0 = c0, cone1 = c1, power0 = p0, power1 = p1, two0 = t0, two1 = t1;
-        if (0 == n0 || 0 == n1) {
-            cost      = 2.f * reg * (p0 + p1) * (4.f * Pi) * sa * float(end - begin);
-            exhausted = true;
-        } else {
-            float const wa = cone_cost(c0.v[3]) * (t0 ? 2.f : 1.f);
-            float const wb = cone_cost(c1.v[3]) * (t1 ? 2.f : 1.f);
-            cost = reg * (((p0 * wa * b0.surface_area()) + (p1 * wb * b1.surface_area())) /
-                          (sa * cone_weight));
-            exhausted = false;
-        }
-    }
+    float const* hi_of(uint32_t l) const { return hi.data() + 3 * size_t(l); }
+    float const* lo_of(uint32_t l) const { return lo.data() + 3 * size_t(l); }
 };
 
 struct Build_node {
@@ -241,46 +66,34 @@ struct Builder {
     bool     part           = false;
     uint32_t max_primitives = 4;
 
+    // evaluate_splits (light_tree_builder.cpp:346-420): the cheapest
+    // candidate, the first of equal ones; none (a node whose lights sit at
+    // one point, too many for the sweep) gives an exhausted candidate
     Candidate evaluate_splits(uint32_t begin, uint32_t end, Box const& bounds, float cone_weight) {
-        uint32_t const len = end - begin;
-        candidates.clear();
-        if (len < (part ? Part_sweep_threshold : Scene_sweep_threshold)) {
-            for (uint32_t i = begin; i < end; ++i) {
-                F3 const& mx = set.boxes[mapping[i]].hi;
-                for (int a = 0; a < 3; ++a) {
-                    Candidate c;
-                    c.d    = mx[a];
-                    c.axis = a;
-                    candidates.push_back(c);
-                }
+        uint32_t const  len  = end - begin;
+        lb::Light_set const view = set.view();
+        bool const      sweep = len < (part ? lb::Part_sweep_threshold : lb::Scene_sweep_threshold);
+        uint32_t const  n     = sweep ? 3 * len : lb::grid_candidates(bounds);
+        candidates.resize(n);
+        for (uint32_t k = 0; k < n; ++k) {
+            Candidate& c = candidates[k];
+            c            = Candidate();
+            if (sweep) {
+                lb::sweep_candidate(mapping.data(), begin, k, view, c);
+            } else {
+                lb::grid_candidate(bounds, k, c);
             }
-        } else {
-            F3 const    position = bounds.position();
-            F3 const    extent   = bounds.extent();
-            F3 const    mn       = bounds.lo;
-            int const   la       = extent[0] >= extent[1] ? (extent[0] >= extent[2] ? 0 : 2)
-                                                          : (extent[1] >= extent[2] ? 1 : 2);
-            float const step     = extent[la] / float(Num_slices);
-            for (int a = 0; a < 3; ++a) {
-                float const    extent_a  = extent[a];
-                uint32_t const num_steps = uint32_t(std::ceil(extent_a / step));
-                float const    step_a    = extent_a / float(num_steps);
-                for (uint32_t i = 1; i < num_steps; ++i) {
-                    F3 slice  = position;
-                    slice[a]  = mn[a] + float(i) * step_a;
-                    Candidate c;
-                    c.d    = slice[a];
-                    c.axis = a;
-                    candidates.push_back(c);
-                }
-            }
+            c.evaluate(begin, end, mapping.data(), bounds, cone_weight, view);
         }
-        for (Candidate& c : candidates) {
-            c.evaluate(begin, end, mapping.data(), bounds, cone_weight, set);
+        if (0 == n) {
+            Candidate c;
+            c.exhausted = true;
+            c.d         = -3.40282347e+38f;  // every light on the far side
+            return c;
         }
         float    min_cost = candidates[0].cost;
         uint32_t sc       = 0;
-        for (uint32_t i = 1; i < candidates.size(); ++i) {
+        for (uint32_t i = 1; i < n; ++i) {
             if (candidates[i].cost < min_cost) {
                 sc       = i;
                 min_cost = candidates[i].cost;
@@ -289,7 +102,7 @@ struct Builder {
         return candidates[sc];
     }
 
-    // Tree_builder::assign (light_tree_builder.cpp:737-758)
+// Tree_builder::assign (light_tree_builder.cpp:737-758)
     uint32_t assign(uint32_t node_id, uint32_t begin, uint32_t end, Box const& bounds,
                     Cone const& cone, float total_power) {
         for (uint32_t i = begin; i < end; ++i) {
@@ -299,7 +112,7 @@ struct Builder {
         node.bounds      = bounds;
         node.cone        = cone;
         node.power       = total_power;
-        node.var         = variance(mapping.data(), begin, end, set);
+        node.var         = lb::variance(mapping.data(), begin, end, set.view());
         node.middle      = 0;
         node.children    = begin;
         node.num_lights  = end - begin;
@@ -315,14 +128,14 @@ struct Builder {
             return assign(node_id, begin, end, bounds, cone, total_power);
         }
         uint32_t const child0      = current;
-        float const    cone_weight = cone_cost(cone.v[3]);
+        float const    cone_weight = lb::cone_cost(cone.v[3]);
         Candidate      sc          = evaluate_splits(begin, end, bounds, cone_weight);
         if (sc.exhausted) {
             return assign(node_id, begin, end, bounds, cone, total_power);
         }
         uint32_t const split_node = uint32_t(
             std::partition(mapping.data() + begin, mapping.data() + end,
-                           [&](uint32_t l) { return sc.behind(set.boxes[l].hi); }) -
+                           [&](uint32_t l) { return sc.behind(set.hi_of(l)); }) -
             mapping.data());
         current += 2;
         uint32_t const c0_end = split_part(child0, begin, split_node, sc.box0, sc.cone0, sc.power0);
@@ -331,7 +144,7 @@ struct Builder {
         node.bounds      = bounds;
         node.cone        = cone;
         node.power       = total_power;
-        node.var         = variance(mapping.data(), begin, end, set);
+        node.var         = lb::variance(mapping.data(), begin, end, set.view());
         node.middle      = c0_end;
         node.children    = child0;
         node.num_lights  = len;
@@ -346,13 +159,13 @@ struct Builder {
             bool node_two_sided = false;
             for (uint32_t i = begin; i < end; ++i) {
                 orders[mapping[i]] = order++;
-                node_two_sided |= 0 != set.two_sided[mapping[i]];
+                node_two_sided |= set.view().two(mapping[i]);
             }
             Build_node& node = nodes[node_id];
             node.bounds      = bounds;
             node.cone        = cone;
             node.power       = total_power;
-            node.var         = variance(mapping.data(), begin, end, set);
+            node.var         = lb::variance(mapping.data(), begin, end, set.view());
             node.middle      = 0;
             node.children    = begin;
             node.num_lights  = len;
@@ -361,21 +174,21 @@ struct Builder {
         }
         uint32_t const child0 = current;
         current += 2;
-        float const cone_weight = cone_cost(cone.v[3]);
+        float const cone_weight = lb::cone_cost(cone.v[3]);
         Candidate   sc          = evaluate_splits(begin, end, bounds, cone_weight);
         uint32_t    split_node  = uint32_t(
             std::partition(mapping.data() + begin, mapping.data() + end,
-                           [&](uint32_t l) { return sc.behind(set.boxes[l].hi); }) -
+                           [&](uint32_t l) { return sc.behind(set.hi_of(l)); }) -
             mapping.data());
         if (split_node == begin || split_node == end) {
             // every candidate leaves a side empty (the reference asserts this
             // never happens and recurses forever when it does): split at the
             // median of the box centers along the longest axis instead
-            F3 const  e    = bounds.extent();
-            int const axis = e[0] >= e[1] ? (e[0] >= e[2] ? 0 : 2) : (e[1] >= e[2] ? 1 : 2);
+            float const e0 = bounds.extent(0), e1 = bounds.extent(1), e2 = bounds.extent(2);
+            int const   axis = e0 >= e1 ? (e0 >= e2 ? 0 : 2) : (e1 >= e2 ? 1 : 2);
             std::sort(mapping.data() + begin, mapping.data() + end, [&](uint32_t x, uint32_t y) {
-                float const cx = set.boxes[x].lo[axis] + set.boxes[x].hi[axis];
-                float const cy = set.boxes[y].lo[axis] + set.boxes[y].hi[axis];
+                float const cx = set.lo_of(x)[axis] + set.hi_of(x)[axis];
+                float const cy = set.lo_of(y)[axis] + set.hi_of(y)[axis];
                 return cx < cy || (cx == cy && x < y);
             });
             split_node = begin + (end - begin) / 2;
@@ -385,10 +198,11 @@ struct Builder {
             float p0 = 0.f, p1 = 0.f;
             for (uint32_t i = begin; i < end; ++i) {
                 uint32_t const l = mapping[i];
+                lb::Light_set const view = set.view();
                 if (i < split_node) {
-                    b0.merge(set.boxes[l]), c0 = merge(c0, set.cones[l]), t0 |= 0 != set.two_sided[l], p0 += set.powers[l];
+                    b0.merge(set.lo_of(l), set.hi_of(l)), c0 = lb::merge(c0, view.cone(l)), t0 |= view.two(l), p0 += set.powers[l];
                 } else {
-                    b1.merge(set.boxes[l]), c1 = merge(c1, set.cones[l]), t1 |= 0 != set.two_sided[l], p1 += set.powers[l];
+                    b1.merge(set.lo_of(l), set.hi_of(l)), c1 = lb::merge(c1, view.cone(l)), t1 |= view.two(l), p1 += set.powers[l];
                 }
             }
             sc.box0 = b0, sc.box1 = b1, sc.cone0 = c0, sc.cone1 = c1, sc.two0 = t0, sc.two1 = t1;
@@ -400,7 +214,7 @@ struct Builder {
         node.bounds      = bounds;
         node.cone        = cone;
         node.power       = total_power;
-        node.var         = variance(mapping.data(), begin, end, set);
+        node.var         = lb::variance(mapping.data(), begin, end, set.view());
         node.middle      = c0_end;
         node.children    = child0;
         node.num_lights  = len;
@@ -411,22 +225,14 @@ struct Builder {
 
 void load_lights(float const* box_min, float const* box_max, float const* cones,
                  float const* powers, uint8_t const* two_sided, uint32_t num_lights, Lights& set) {
-    set.boxes.resize(num_lights);
-    set.cones.resize(num_lights);
+    set.lo.assign(box_min, box_min + 3 * size_t(num_lights));
+    set.hi.assign(box_max, box_max + 3 * size_t(num_lights));
+    set.cones.assign(cones, cones + 4 * size_t(num_lights));
     set.powers.assign(powers, powers + num_lights);
     if (two_sided) {
         set.two_sided.assign(two_sided, two_sided + num_lights);
     } else {
         set.two_sided.assign(num_lights, 0);
-    }
-    for (uint32_t l = 0; l < num_lights; ++l) {
-        for (int k = 0; k < 3; ++k) {
-            set.boxes[l].lo[k] = box_min[3 * l + k];
-            set.boxes[l].hi[k] = box_max[3 * l + k];
-        }
-        for (int k = 0; k < 4; ++k) {
-            set.cones[l].v[k] = cones[4 * l + k];
-        }
     }
 }
 
@@ -437,12 +243,11 @@ void serialize(Builder const& b, HostLightTree& out) {
     for (uint32_t i = 0; i < b.current; ++i) {
         Build_node const& s = b.nodes[i];
         Light_node&       d = out.nodes[i];
-        F3 const          p = s.bounds.position();
-        F3 const          e = s.bounds.extent();
-        d.center[0]         = p[0];
-        d.center[1]         = p[1];
-        d.center[2]         = p[2];
-        d.radius            = 0.5f * std::sqrt(dot3(e.v, e.v));
+        float const       e[3] = {s.bounds.extent(0), s.bounds.extent(1), s.bounds.extent(2)};
+        d.center[0]            = s.bounds.position(0);
+        d.center[1]            = s.bounds.position(1);
+        d.center[2]            = s.bounds.position(2);
+        d.radius               = 0.5f * std::sqrt(lb::dot3(e, e));
         for (int k = 0; k < 4; ++k) {
             d.cone[k] = s.cone.v[k];
         }
@@ -480,10 +285,11 @@ void build_light_tree(float const* box_min, float const* box_max, float const* c
     Cone  cone;
     bool  ts    = false;
     float total = 0.f;
+    lb::Light_set const view = set.view();
     for (uint32_t l = 0; l < num_lights; ++l) {
-        bounds.merge(set.boxes[l]);
-        cone = merge(cone, set.cones[l]);
-        ts |= 0 != set.two_sided[l];
+        bounds.merge(set.lo_of(l), set.hi_of(l));
+        cone = lb::merge(cone, view.cone(l));
+        ts |= view.two(l);
         total += set.powers[l];
     }
     b.split(0, 0, num_lights, bounds, cone, ts, total);
@@ -515,7 +321,7 @@ void build_part_tree(float const* box_min, float const* box_max, float const* co
     float total = 0.f;
     for (uint32_t l = 0; l < num_lights; ++l) {
         if (set.powers[l] > 0.f) {
-            bounds.merge(set.boxes[l]);
+            bounds.merge(set.lo_of(l), set.hi_of(l));
             total += set.powers[l];
         }
     }

@@ -4,7 +4,9 @@
 // Top grid of 32^3-voxel cells; each cell splits into octants down to 8^3
 // voxels while the density range (one voxel of border included, for the
 // linear filter) is 0.1 or more; leaves keep {minorant, majorant} of mu_a and
-// mu_s.  Cells are built one after the other (the reference hands them to its
+// mu_s.  The uniform variant splits every cell down to its 8^3-voxel leaves:
+// a flat grid of majorants (32^3 of them for 256^3 voxels) in the same node
+// format, so the tracking walks it unchanged.  Cells are built one after the other (the reference hands them to its
 // thread pool; serialization order does not depend on it).
 
 #include "medium.hpp"
@@ -28,6 +30,7 @@ bool is_empty(Medium_cm const& c) {
 struct Splitter {
     float const* density;
     int32_t      d[3];
+    bool         uniform = false;  // split every cell to its 8^3-voxel leaves
     Medium_cm    cm;  // CM(cc): min / max components of the unit coefficients
     uint32_t     num_nodes = 0;
     uint32_t     num_data  = 0;
@@ -56,7 +59,7 @@ struct Splitter {
             max_density = 0.f;
         }
         float const diff = max_density - min_density;
-        if (dev::Log2_cell_dim - 3 == int32_t(depth) || diff < 0.1f || maxb[0] - minb[0] < w ||
+        if (dev::Log2_cell_dim - 3 == int32_t(depth) || (!uniform && diff < 0.1f) || maxb[0] - minb[0] < w ||
             maxb[1] - minb[1] < w || maxb[2] - minb[2] < w) {
             node.children.reset();
             node.data = {min_density * cm.minorant_mu_a, min_density * cm.minorant_mu_s,
@@ -105,7 +108,7 @@ void serialize(Build_node const& node, uint32_t current, uint32_t& next, uint32_
 }  // namespace
 
 void build_gridtree(float const* density, int32_t const dims[3], float const cc_a[3],
-                    float const cc_s[3], HostGridtree& out) {
+                    float const cc_s[3], HostGridtree& out, bool uniform) {
     out = HostGridtree();
     int32_t nc[3];
     for (int k = 0; k < 3; ++k) {
@@ -118,6 +121,7 @@ void build_gridtree(float const* density, int32_t const dims[3], float const cc_
 
     Splitter sp;
     sp.density = density;
+    sp.uniform = uniform;
     for (int k = 0; k < 3; ++k) {
         sp.d[k] = dims[k];
     }

@@ -42,9 +42,10 @@ struct HostGridtree {
 };
 
 // Octree_builder::build for a one-channel density grid (x fastest) and the
-// medium's coefficients per unit density (rgb).
+// medium's coefficients per unit density (rgb).  uniform: every top cell
+// split to its 8^3-voxel leaves (a flat majorant grid).
 void build_gridtree(float const* density, int32_t const dims[3], float const cc_a[3],
-                    float const cc_s[3], HostGridtree& out);
+                    float const cc_s[3], HostGridtree& out, bool uniform = false);
 
 namespace dev {
 
@@ -256,77 +257,112 @@ __device__ __forceinline__ float medium_uniform(Rng& rng) {
     return rng.uniform();
 }
 
-__device__ __forceinline__ float average3(float const v[3]) {
-    return (v[0] + v[1] + v[2]) / 3.f;
+// Collision coefficients per unit density of the channels a path carries:
+// rgb (the medium's own) or, for hero-wavelength paths, the four wavelengths'
+// upsampled values.
+template <int NC>
+struct Channel_cc {
+    float a[NC];
+    float s[NC];
+};
+
+__device__ __forceinline__ Channel_cc<3> rgb_channels(Medium_view const& m) {
+    Channel_cc<3> cc;
+    for (int c = 0; c < 3; ++c) {
+        cc.a[c] = m.cc_a[c];
+        cc.s[c] = m.cc_s[c];
+    }
+    return cc;
+}
+
+template <int NC>
+__device__ __forceinline__ float average_channels(float const v[NC]) {
+    if constexpr (3 == NC) {
+        return (v[0] + v[1] + v[2]) / 3.f;
+    } else {
+        return (((v[0] + v[1]) + v[2]) + v[3]) * 0.25f;
+    }
 }
 
 // collision coefficients at uvw, mu_s scaled by srs; nearest: the filter
 // the path tracer asks for after its second vertex (Filter::Nearest,
 // pathtracer_mis.cpp:121), else the grid's linear sampler
-__device__ __forceinline__ void coefficients(Medium_view const& m, V3 uvw, float srs, bool nearest,
-                                             float a[3], float s[3]) {
+template <int NC>
+__device__ __forceinline__ void coefficients(Medium_view const& m, Channel_cc<NC> const& cc, V3 uvw,
+                                             float srs, bool nearest, float a[NC], float s[NC]) {
     float const dn = nearest ? density_nearest(m, uvw) : density(m, uvw);
-    for (int c = 0; c < 3; ++c) {
-        a[c] = dn * m.cc_a[c];
-        s[c] = (dn * m.cc_s[c]) * srs;
+    for (int c = 0; c < NC; ++c) {
+        a[c] = dn * cc.a[c];
+        s[c] = (dn * cc.s[c]) * srs;
     }
 }
 
-// Tracking::tracking (scattering, no emission): true = scattered at t_out
-template <typename Rng>
-__device__ inline bool tracking(Medium_view const& m, Tray const& r, Medium_cm const& cm, float srs,
-                                bool nearest, Rng& rng, float& t_out, float w[3]) {
+// Tracking::tracking (scattering, no emission): true = scattered at t_out.
+// The channel weights are history-aware (the scatter / null probabilities
+// average the channels weighted by the path so far).
+template <int NC, typename Rng>
+__device__ inline bool tracking(Medium_view const& m, Channel_cc<NC> const& cc, Tray const& r,
+                                Medium_cm const& cm, float srs, bool nearest, Rng& rng,
+                                float& t_out, float w[NC]) {
     float const mt = cm.majorant_mu_a + cm.majorant_mu_s;
     if (mt < Min_mt) {
         return false;
     }
-    float       lw[3] = {w[0], w[1], w[2]};
-    float const imt   = 1.f / mt;
+    float lw[NC];
+    for (int c = 0; c < NC; ++c) {
+        lw[c] = w[c];
+    }
+    float const imt = 1.f / mt;
     for (float t = r.min_t, d = r.max_t;;) {
         float const r0 = medium_uniform(rng);
         t -= logf(1.f - r0) * imt;
         if (t > d) {
-            for (int c = 0; c < 3; ++c) {
+            for (int c = 0; c < NC; ++c) {
                 w[c] = lw[c];
             }
             return false;
         }
-        float mu_a[3], mu_s[3];
-        coefficients(m, r.point(t), srs, nearest, mu_a, mu_s);
-        float mu_n[3], msw[3], mnw[3];
-        for (int c = 0; c < 3; ++c) {
+        float mu_a[NC], mu_s[NC];
+        coefficients<NC>(m, cc, r.point(t), srs, nearest, mu_a, mu_s);
+        float mu_n[NC], msw[NC], mnw[NC];
+        for (int c = 0; c < NC; ++c) {
             mu_n[c] = mt - (mu_a[c] + mu_s[c]);
             msw[c]  = mu_s[c] * lw[c];
             mnw[c]  = mu_n[c] * lw[c];
         }
-        float const ms = average3(msw);
-        float const mn = average3(mnw);
-        float const cc = 1.f / (ms + mn);
-        float const ps = ms * cc;
-        float const pn = mn * cc;
+        float const ms = average_channels<NC>(msw);
+        float const mn = average_channels<NC>(mnw);
+        float const ic = 1.f / (ms + mn);
+        float const ps = ms * ic;
+        float const pn = mn * ic;
         float const r1 = medium_uniform(rng);
         if ((r1 <= 1.f - pn) & (ps > 0.f)) {
-            for (int c = 0; c < 3; ++c) {
+            for (int c = 0; c < NC; ++c) {
                 w[c] = lw[c] * (mu_s[c] / (mt * ps));
             }
             t_out = t;
             return true;
         }
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < NC; ++c) {
             lw[c] *= mu_n[c] / (mt * pn);
         }
     }
 }
 
-__device__ __forceinline__ bool all_below(float const w[3], float e) {
-    return w[0] < e && w[1] < e && w[2] < e;
+template <int NC>
+__device__ __forceinline__ bool all_below(float const w[NC], float e) {
+    bool below = true;
+    for (int c = 0; c < NC; ++c) {
+        below = below && w[c] < e;
+    }
+    return below;
 }
 
 // tracking_transmitted with its residual-ratio-tracking branch
-template <typename Rng>
-__device__ inline bool tracking_transmitted(Medium_view const& m, Tray const& r,
-                                            Medium_cm const& cm, float srs, bool nearest, Rng& rng,
-                                            float w[3]) {
+template <int NC, typename Rng>
+__device__ inline bool tracking_transmitted(Medium_view const& m, Channel_cc<NC> const& cc,
+                                            Tray const& r, Medium_cm const& cm, float srs,
+                                            bool nearest, Rng& rng, float w[NC]) {
     float const mt = cm.majorant_mu_a + cm.majorant_mu_s;
     if (mt < Min_mt) {
         return true;
@@ -336,10 +372,10 @@ __device__ inline bool tracking_transmitted(Medium_view const& m, Tray const& r,
     float       mr       = mt;
     if (minorant > 0.f) {
         float const att = expf(-(r.max_t - r.min_t) * minorant);
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < NC; ++c) {
             w[c] *= att;
         }
-        if (all_below(w, Abort_epsilon)) {
+        if (all_below<NC>(w, Abort_epsilon)) {
             return false;
         }
         mr = mt - minorant;
@@ -355,13 +391,13 @@ __device__ inline bool tracking_transmitted(Medium_view const& m, Tray const& r,
         if (t > d) {
             return true;
         }
-        float mu_a[3], mu_s[3];
-        coefficients(m, r.point(t), srs, nearest, mu_a, mu_s);
-        for (int c = 0; c < 3; ++c) {
+        float mu_a[NC], mu_s[NC];
+        coefficients<NC>(m, cc, r.point(t), srs, nearest, mu_a, mu_s);
+        for (int c = 0; c < NC; ++c) {
             float const mu_t = (mu_a[c] + mu_s[c]) - control;
             w[c] *= imt * (mr - mu_t);
         }
-        if (all_below(w, Abort_epsilon)) {
+        if (all_below<NC>(w, Abort_epsilon)) {
             return false;
         }
     }
@@ -419,20 +455,22 @@ __device__ inline bool box_overlap(Medium_view const& m, V3 o, V3 d, float min_t
 // Tracking_multi::integrate for one segment (tracking_multi.cpp:101-162):
 // 1 scattered at t_out, 0 passed, 2 aborted (every channel of w below
 // Abort_epsilon, scattered or not).  w multiplies the throughput.
-template <typename Rng>
-__device__ inline int track_segment(Medium_view const& m, V3 o, V3 d, float t0, float t1,
-                                    uint32_t depth, bool nearest, Rng& rng, float& t_out,
-                                    float w[3]) {
+template <int NC, typename Rng>
+__device__ inline int track_segment(Medium_view const& m, Channel_cc<NC> const& cc, V3 o, V3 d,
+                                    float t0, float t1, uint32_t depth, bool nearest, Rng& rng,
+                                    float& t_out, float w[NC]) {
     Tray        r     = texture_ray(m, o, d, t0, t1);
     float const srs   = similarity_scale(m.g, depth);
     int         event = 0;
-    w[0] = w[1] = w[2] = 1.f;
+    for (int c = 0; c < NC; ++c) {
+        w[c] = 1.f;
+    }
     for (float const end = t1; r.min_t < end;) {
         Medium_cm cm;
         if (gridtree_intersect(m, r, cm)) {
             cm.minorant_mu_s *= srs;
             cm.majorant_mu_s *= srs;
-            if (tracking(m, r, cm, srs, nearest, rng, t_out, w)) {
+            if (tracking<NC>(m, cc, r, cm, srs, nearest, rng, t_out, w)) {
                 event = 1;
                 break;
             }
@@ -440,15 +478,25 @@ __device__ inline int track_segment(Medium_view const& m, V3 o, V3 d, float t0, 
         r.min_t = offset_f(r.max_t);
         r.max_t = end;
     }
-    return all_below(w, Abort_epsilon) ? 2 : event;
+    return all_below<NC>(w, Abort_epsilon) ? 2 : event;
+}
+
+template <typename Rng>
+__device__ inline int track_segment(Medium_view const& m, V3 o, V3 d, float t0, float t1,
+                                    uint32_t depth, bool nearest, Rng& rng, float& t_out,
+                                    float w[3]) {
+    return track_segment<3>(m, rgb_channels(m), o, d, t0, t1, depth, nearest, rng, t_out, w);
 }
 
 // Tracking::transmittance for the part of a segment inside the box:
 // false = blocked (tr below Abort_epsilon), else tr.
-template <typename Rng>
-__device__ inline bool transmittance(Medium_view const& m, V3 o, V3 d, float t0, float t1,
-                                     uint32_t depth, bool nearest, Rng& rng, float tr[3]) {
-    tr[0] = tr[1] = tr[2] = 1.f;
+template <int NC, typename Rng>
+__device__ inline bool transmittance(Medium_view const& m, Channel_cc<NC> const& cc, V3 o, V3 d,
+                                     float t0, float t1, uint32_t depth, bool nearest, Rng& rng,
+                                     float tr[NC]) {
+    for (int c = 0; c < NC; ++c) {
+        tr[c] = 1.f;
+    }
     if (offset_f(t0) >= t1) {
         return true;
     }
@@ -459,8 +507,10 @@ __device__ inline bool transmittance(Medium_view const& m, V3 o, V3 d, float t0,
         if (gridtree_intersect(m, r, cm)) {
             cm.minorant_mu_s *= srs;
             cm.majorant_mu_s *= srs;
-            if (!tracking_transmitted(m, r, cm, srs, nearest, rng, tr)) {
-                tr[0] = tr[1] = tr[2] = 0.f;
+            if (!tracking_transmitted<NC>(m, cc, r, cm, srs, nearest, rng, tr)) {
+                for (int c = 0; c < NC; ++c) {
+                    tr[c] = 0.f;
+                }
                 return false;
             }
         }
@@ -468,6 +518,12 @@ __device__ inline bool transmittance(Medium_view const& m, V3 o, V3 d, float t0,
         r.max_t = end;
     }
     return true;
+}
+
+template <typename Rng>
+__device__ inline bool transmittance(Medium_view const& m, V3 o, V3 d, float t0, float t1,
+                                     uint32_t depth, bool nearest, Rng& rng, float tr[3]) {
+    return transmittance<3>(m, rgb_channels(m), o, d, t0, t1, depth, nearest, rng, tr);
 }
 
 }  // namespace mi::dev

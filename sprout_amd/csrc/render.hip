@@ -42,8 +42,11 @@
 #include "common.hpp"
 #include "device_math.hpp"
 #include "guide.hpp"
+#include "light_build.hip.hpp"
+#include "light_build.hpp"
 #include "light_tree.hpp"
 #include "medium.hpp"
+#include "spectral.hpp"
 #include "render.hpp"
 #include "trace.hpp"
 
@@ -74,7 +77,9 @@ constexpr uint32_t C_base       = 3 * Counter_line;  // first sample index of th
 constexpr uint32_t C_records    = 4 * Counter_line;  // training samples recorded in the current wave
 constexpr uint32_t C_totals     = 5 * Counter_line;  // two uint64: extension and shadow rays traced,
                                                      // then one uint64: guiding records trained on
-constexpr uint32_t Num_counters = 6 * Counter_line;
+constexpr uint32_t C_done       = 6 * Counter_line;  // regenerating: paths that ended this bounce
+constexpr uint32_t C_next       = 7 * Counter_line;  // regenerating: the next sample of the batch to start
+constexpr uint32_t Num_counters = 8 * Counter_line;
 
 // rnd::Generator, base/random/generator.inl:16-49
 struct Rng {
@@ -148,6 +153,8 @@ struct Path {
     uint32_t record;     // training sample of the previous vertex, or Null
     float    prev_n[3];  // geometric normal of the previous vertex
     float    jitter[2];  // the camera sample's position in its pixel
+    uint32_t depth;      // depth of the vertex the path's ray leads to (0: camera ray)
+    uint32_t sample;     // the sample of the frame batch: plane * num_pixels + pixel
 };
 
 struct Shadow {
@@ -166,7 +173,7 @@ struct Nee_vertex {
     float    wo[3];
     uint32_t flags;      // Nee_guided, Nee_medium
     float    throughput[3];
-    uint32_t pad;
+    uint32_t depth;      // the vertex's depth
 };
 
 constexpr uint32_t Nee_guided = 1;  // the guide's mixture (guide_mix) is in the MIS weight
@@ -223,6 +230,10 @@ struct Wave {
     uint64_t seed;
     uint32_t num_paths;
     uint32_t num_pixels;
+    uint32_t batch_samples;  // samples of the frame batch (a wave's num_paths without regeneration)
+    uint32_t regenerate;     // ended paths hand their slot to the batch's next sample
+    float*   sample_rad;     // regenerating: radiance of every sample of the batch (3 floats)
+    uint32_t* done;          // regenerating: slots of the paths that ended this bounce
     // state
     Path*     paths;
     mi_ray*   rays[2];
@@ -243,6 +254,14 @@ struct Wave {
     float        box_lo[3];
     float        box_scale[3];  // 1 / extent of the scene box
     uint32_t     guide_depth;   // vertices of depth below this sample the guide
+    uint32_t     guide_medium;  // scattering vertices in the medium sample the guide too
+    // hero-wavelength paths: four wavelengths each; throughput[0..2] hold
+    // the first three, path_spec[path].x the fourth and .y the hero number
+    uint32_t     spectral;
+    float        spec_minv[9];  // the film's response: inverse Gram matrix of the bands
+    float2*      path_spec;     // per path slot
+    float*       nee_thr4;      // per queue entry: the vertex's fourth throughput
+    float*       shadow_c4;     // per shadow entry: the fourth wavelength's contribution
     uint32_t     record_depth;  // vertices of depth below this record samples
     uint32_t     record_capacity;
     float*       train_x;       // 9 per record
@@ -341,6 +360,15 @@ __device__ Surface_point surface_point(Wave const& w, Hit const& h) {
     sp.material      = w.inst_material[h.instance];
     sp.light         = dev::Null;
     return sp;
+}
+
+template <int NC>
+__device__ __forceinline__ float max_channel(float const t[NC]) {
+    float m = t[0];
+    for (int c = 1; c < NC; ++c) {
+        m = fmaxf(m, t[c]);
+    }
+    return m;
 }
 
 __device__ __forceinline__ float max_component(float const t[3]) {
@@ -539,13 +567,12 @@ __global__ void advance_base(uint32_t* counters, uint32_t samples) {
     counters[C_base] += samples;
 }
 
-__global__ void __launch_bounds__(256) raygen(Wave w) {
-    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= w.num_paths) {
-        return;
-    }
-    uint32_t const pixel  = p % w.num_pixels;  // of the camera grid
-    uint32_t const plane  = p / w.num_pixels;
+// Starts sample k of the frame batch (plane k / num_pixels, pixel k %
+// num_pixels of the camera grid) in path slot `slot`: the camera sample and
+// the path state; returns the camera ray.
+__device__ mi_ray start_path(Wave const& w, uint32_t k, uint32_t slot) {
+    uint32_t const pixel  = k % w.num_pixels;  // of the camera grid
+    uint32_t const plane  = k / w.num_pixels;
     uint32_t const sample = w.counters[C_base] + plane;
     int32_t const  x      = int32_t(pixel % w.grid_w) - w.border;
     int32_t const  y      = int32_t(pixel / w.grid_w) - w.border;
@@ -595,31 +622,49 @@ __global__ void __launch_bounds__(256) raygen(Wave w) {
     path.flags    = Flag_singular | Flag_primary;
     path.record   = dev::Null;
     path.prev_n[0] = path.prev_n[1] = path.prev_n[2] = 0.f;
-    w.paths[p]    = path;
+    path.depth    = 0;
+    path.sample   = k;
+    w.paths[slot] = path;
+    if (w.spectral) {
+        w.path_spec[slot] = make_float2(
+            1.f, spectral::hero_u(w.seed, uint64_t(sample) * w.num_pixels + pixel));
+    }
 
     mi_ray r;
-    r.origin[0]      = w.position[0];
-    r.origin[1]      = w.position[1];
-    r.origin[2]      = w.position[2];
-    r.min_t          = 0.f;
-    r.direction[0]   = d.x;
-    r.direction[1]   = d.y;
-    r.direction[2]   = d.z;
-    r.max_t          = Ray_max_t;
-    w.rays[0][p]     = r;
-    w.ray_path[0][p] = p;
+    r.origin[0]    = w.position[0];
+    r.origin[1]    = w.position[1];
+    r.origin[2]    = w.position[2];
+    r.min_t        = 0.f;
+    r.direction[0] = d.x;
+    r.direction[1] = d.y;
+    r.direction[2] = d.z;
+    r.max_t        = Ray_max_t;
+    return r;
+}
 
+// The first rays of a wave (or of a regenerating batch): sample p in slot p
+// for the first min(num_paths, batch_samples) slots.
+__global__ void __launch_bounds__(256) raygen(Wave w) {
+    uint32_t const live = min(w.num_paths, w.batch_samples);
+    uint32_t const p    = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < live) {
+        w.rays[0][p]     = start_path(w, p, p);
+        w.ray_path[0][p] = p;
+    }
     if (0 == p) {
-        w.counters[C_queue0]     = w.num_paths;
+        w.counters[C_queue0]                = live;
         w.counters[C_queue0 + Counter_line] = 0;
-        w.counters[C_shadow]     = 0;
+        w.counters[C_shadow]                = 0;
+        w.counters[C_done]                  = 0;
+        w.counters[C_next]                  = live;  // the next sample to start
     }
 }
 
 // Clears the queue the next shade fills and the shadow queue.
 __global__ void reset_queues(uint32_t* counters, uint32_t out_q) {
     counters[C_queue0 + Counter_line * out_q] = 0;
-    counters[C_shadow]         = 0;
+    counters[C_shadow]                        = 0;
+    counters[C_done]                          = 0;
 }
 
 // Ray totals for the statistics: the queue just traced and the shadow queue.
@@ -629,10 +674,24 @@ __global__ void tally(uint32_t* counters, uint32_t in_q) {
     totals[1] += counters[C_shadow];
 }
 
+// The collision coefficients per unit density at a path's four wavelengths.
+__device__ __forceinline__ dev::Channel_cc<4> wavelength_channels(dev::Medium_view const& m,
+                                                                  float hero) {
+    spectral::Wavelengths wl;
+    wl.set(hero);
+    dev::Channel_cc<4> cc;
+    wl.upsample4(m.cc_a, cc.a);
+    wl.upsample4(m.cc_s, cc.s);
+    return cc;
+}
+
 // Tracking_multi::integrate for every queued ray: the part of the ray before
 // its surface hit that lies in the medium box is tracked; the tracking weight
 // multiplies the path throughput, and a scattering point replaces the hit.
-__global__ void __launch_bounds__(256) medium_track(Wave w, uint32_t in_q, uint32_t depth) {
+// Spectral: the channels are the path's four wavelengths.
+template <bool Spectral>
+__device__ __forceinline__ void medium_track_queue(Wave const& w, uint32_t in_q) {
+    constexpr int  NC     = Spectral ? 4 : 3;
     uint32_t const count  = w.counters[C_queue0 + Counter_line * in_q];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
@@ -643,12 +702,22 @@ __global__ void __launch_bounds__(256) medium_track(Wave w, uint32_t in_q, uint3
         uint32_t     event = Medium_none;
         float        t0, t1;
         if (dev::box_overlap(w.medium, o, d, r.min_t, h.t, t0, t1)) {
-            uint32_t const p    = w.ray_path[in_q][i];
-            Path           path = w.paths[p];
+            uint32_t const p     = w.ray_path[in_q][i];
+            Path           path  = w.paths[p];
             float          t_out = 0.f;
-            float          tw[3];
-            int const      e = dev::track_segment(w.medium, o, d, t0, t1, depth,
-                                                      0 != (path.flags & Flag_nearest), path.rng, t_out, tw);
+            float          tw[NC];
+            int            e;
+            if constexpr (Spectral) {
+                float2 sp = w.path_spec[p];
+                e = dev::track_segment<4>(w.medium, wavelength_channels(w.medium, sp.y), o, d, t0,
+                                          t1, path.depth, 0 != (path.flags & Flag_nearest),
+                                          path.rng, t_out, tw);
+                sp.x *= tw[3];
+                w.path_spec[p] = sp;
+            } else {
+                e = dev::track_segment(w.medium, o, d, t0, t1, path.depth,
+                                       0 != (path.flags & Flag_nearest), path.rng, t_out, tw);
+            }
             for (int c = 0; c < 3; ++c) {
                 path.throughput[c] *= tw[c];
             }
@@ -666,24 +735,44 @@ __global__ void __launch_bounds__(256) medium_track(Wave w, uint32_t in_q, uint3
     }
 }
 
+__global__ void __launch_bounds__(256) medium_track(Wave w, uint32_t in_q) {
+    medium_track_queue<false>(w, in_q);
+}
+
+__global__ void __launch_bounds__(256) medium_track_spectral(Wave w, uint32_t in_q) {
+    medium_track_queue<true>(w, in_q);
+}
+
 // Network inputs of every queued hit: position in the scene box, geometric
-// normal and outgoing direction, each mapped to [0,1].
+// normal and outgoing direction, each mapped to [0,1].  A scattering point in
+// the medium has no normal: its normal inputs are those of a zero vector.
 __global__ void __launch_bounds__(256) guide_inputs(Wave w, uint32_t in_q) {
     uint32_t const count  = w.counters[C_queue0 + Counter_line * in_q];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        Hit const h  = load_hit(w, i);
-        float*    xo = w.guide_x + 9 * size_t(i);
-        if (dev::Null == h.triangle) {
+        Hit const h       = load_hit(w, i);
+        float*    xo      = w.guide_x + 9 * size_t(i);
+        bool const scatter = w.medium.present && Medium_scatter == w.medium_event[i];
+        if (dev::Null == h.triangle && !scatter) {
             for (int k = 0; k < 9; ++k) {
                 xo[k] = 0.5f;
             }
             continue;
         }
-        mi_ray const        r     = w.rays[in_q][i];
-        Surface_point const sp    = surface_point(w, h);
-        V3 const            n     = sp.n;
-        float const         p3[3] = {sp.p.x, sp.p.y, sp.p.z};
+        mi_ray const r = w.rays[in_q][i];
+        V3           n = v3(0.f, 0.f, 0.f);
+        float        p3[3];
+        if (scatter) {
+            for (int k = 0; k < 3; ++k) {
+                p3[k] = w.medium_p[3 * size_t(i) + k];
+            }
+        } else {
+            Surface_point const sp = surface_point(w, h);
+            n                      = sp.n;
+            p3[0]                  = sp.p.x;
+            p3[1]                  = sp.p.y;
+            p3[2]                  = sp.p.z;
+        }
         for (int k = 0; k < 3; ++k) {
             xo[k] = dev::clampf((p3[k] - w.box_lo[k]) * w.box_scale[k], 0.f, 1.f);
         }
@@ -701,15 +790,16 @@ __global__ void __launch_bounds__(256) guide_inputs(Wave w, uint32_t in_q) {
 // picks of next-event estimation (their rays are made by the nee kernel), the
 // direction sample (the BSDF's, or with Guided the one-sample mixture of the
 // guide and the BSDF) and the next ray.  The path's fields are read and
-// written where they are used, so few stay in registers.
-template <bool Guided>
-__device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32_t depth) {
+// written where they are used, so few stay in registers.  Spectral: the
+// throughput has the path's four wavelengths; rgb reflectance and emission
+// are upsampled to them and contributions reach the radiance through the
+// film's response.
+template <bool Guided, bool Spectral>
+__device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q) {
+    constexpr int  NC     = Spectral ? 4 : 3;
     uint32_t const count  = w.counters[C_queue0 + Counter_line * in_q];
     uint32_t const out_q  = in_q ^ 1;
     uint32_t const stride = gridDim.x * blockDim.x;
-    // sample_lights splits the light tree at the first bounces
-    // (Pathtracer_MIS::splitting, pathtracer_mis.cpp:423-426)
-    bool const split = w.split_lights && depth < Num_split_bounces;
 
     // block-uniform trip count: every lane reaches the wave-wide queue appends
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
@@ -719,7 +809,11 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
         bool     live      = false;  // a shading vertex that continues the path
         bool     in_medium = false;  // a scattering vertex of the medium
         uint32_t p         = 0;
-        float    thr[3]    = {0.f, 0.f, 0.f};
+        float    thr[NC];
+        for (int c = 0; c < NC; ++c) {
+            thr[c] = 0.f;
+        }
+        float    hero      = 0.f;  // spectral: the path's hero number
         uint32_t flags     = 0;
         Rng      rng;
         rng.state = rng.inc = 0;
@@ -733,13 +827,24 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
         uint32_t num_picks = 0;
         float    select    = 0.f;
         uint32_t prev_record = dev::Null;  // training record of the previous vertex's sample
+        uint32_t depth       = 0;          // of this vertex
+        bool     split       = false;
 
         if (i < count) {
             p                = w.ray_path[in_q][i];
             Path const* pp   = &w.paths[p];
+            depth            = pp->depth;
+            // sample_lights splits the light tree at the first bounces
+            // (Pathtracer_MIS::splitting, pathtracer_mis.cpp:423-426)
+            split            = w.split_lights && depth < Num_split_bounces;
             thr[0]           = pp->throughput[0];
             thr[1]           = pp->throughput[1];
             thr[2]           = pp->throughput[2];
+            if constexpr (Spectral) {
+                float2 const sp = w.path_spec[p];
+                thr[3]          = sp.x;
+                hero            = sp.y;
+            }
             flags            = pp->flags;
             rng              = pp->rng;
             Hit const h      = load_hit(w, i);
@@ -804,16 +909,40 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
                         }
                         float const* e  = w.materials[sp.material].emission;
                         float*       rad = w.paths[p].radiance;
-                        for (int k = 0; k < 3; ++k) {
-                            rad[k] = rad[k] + thr[k] * (weight * e[k]);
-                        }
-                        if (Guided && dev::Null != prev_record) {
-                            // the guide learns the whole incident radiance: the
-                            // emission found by the sampled direction counts
-                            // without its MIS weight
-                            float* rr = w.rec_r + 3 * size_t(prev_record);
+                        if constexpr (Spectral) {
+                            spectral::Wavelengths wl;
+                            wl.set(hero);
+                            float el[4], c4[4], rgb[3];
+                            wl.upsample4(e, el);
+                            for (int k = 0; k < 4; ++k) {
+                                c4[k] = thr[k] * (weight * el[k]);
+                            }
+                            wl.to_rgb(w.spec_minv, c4, rgb);
                             for (int k = 0; k < 3; ++k) {
-                                rr[k] -= thr[k] * ((1.f - weight) * e[k]);
+                                rad[k] = rad[k] + rgb[k];
+                            }
+                            if (Guided && dev::Null != prev_record) {
+                                float* rr = w.rec_r + 3 * size_t(prev_record);
+                                for (int k = 0; k < 4; ++k) {
+                                    c4[k] = thr[k] * ((1.f - weight) * el[k]);
+                                }
+                                wl.to_rgb(w.spec_minv, c4, rgb);
+                                for (int k = 0; k < 3; ++k) {
+                                    rr[k] -= rgb[k];
+                                }
+                            }
+                        } else {
+                            for (int k = 0; k < 3; ++k) {
+                                rad[k] = rad[k] + thr[k] * (weight * e[k]);
+                            }
+                            if (Guided && dev::Null != prev_record) {
+                                // the guide learns the whole incident radiance: the
+                                // emission found by the sampled direction counts
+                                // without its MIS weight
+                                float* rr = w.rec_r + 3 * size_t(prev_record);
+                                for (int k = 0; k < 3; ++k) {
+                                    rr[k] -= thr[k] * ((1.f - weight) * e[k]);
+                                }
                             }
                         }
                     }
@@ -827,12 +956,18 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
                 if (depth >= w.max_bounces) {
                     live = false;
                 } else if (depth >= w.min_bounces) {
-                    // russian_roulette, integrator_helper.hpp:42-52
-                    float const q = max_component(thr);
+                    // russian_roulette, integrator_helper.hpp:42-52: the
+                    // continuation probability is the largest channel, and
+                    // the throughput is divided by it even above one, which
+                    // loses energy; sprout's throughput stays at or below one
+                    // almost always.  A guided path's does not (the mixture
+                    // pdf can be far below the BSDF's), so there the
+                    // probability is capped at one, which keeps it unbiased.
+                    float const q = Guided ? fminf(max_channel<NC>(thr), 1.f) : max_channel<NC>(thr);
                     if (rng.uniform() > q) {
                         live = false;
                     } else {
-                        for (int k = 0; k < 3; ++k) {
+                        for (int k = 0; k < NC; ++k) {
                             thr[k] /= q;
                         }
                     }
@@ -855,7 +990,8 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
             }
         }
 
-        bool const guided_vertex = Guided && live && !in_medium && depth < w.guide_depth;
+        bool const guided_vertex = Guided && live && (!in_medium || 0 != w.guide_medium) &&
+                                   depth < w.guide_depth;
 
         // next-event estimation (sample_lights / evaluate_light): a vertex's
         // shadow rays take consecutive queue slots, reserved with one atomic
@@ -885,7 +1021,10 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
             for (int c = 0; c < 3; ++c) {
                 v.throughput[c] = thr[c];
             }
-            v.pad    = 0;
+            if constexpr (Spectral) {
+                w.nee_thr4[i] = thr[3];
+            }
+            v.depth  = depth;
             w.nee[i] = v;
             uint32_t const made = dev::for_each_light(
                 w.tree, p_o, geo_n, in_medium, select, split, [&](uint32_t k, dev::Light_pick const pick) {
@@ -941,7 +1080,7 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
                     bs.refl[2] = f[2];
                     bs.pdf     = f[3];
                     bs.type    = dev::Type_diffuse;
-                    if (!(dev::dot(geo_n, gw) > 0.f)) {
+                    if (!in_medium && !(dev::dot(geo_n, gw) > 0.f)) {
                         bs.pdf = 0.f;  // under the surface: the reflection is zero
                     }
                 } else {
@@ -951,10 +1090,16 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
                     bs.pdf = w.fraction * dev::vmf_mixture_pdf(mix, bs.wi) + (1.f - w.fraction) * bs.pdf;
                 }
                 if (bs.pdf > 0.f && depth < w.record_depth) {
-                    record   = true;
-                    rec_t[0] = thr[0];
-                    rec_t[1] = thr[1];
-                    rec_t[2] = thr[2];
+                    record = true;
+                    if constexpr (Spectral) {
+                        // the training weight's throughput: the wavelengths' mean
+                        float const mean = (((thr[0] + thr[1]) + thr[2]) + thr[3]) * 0.25f;
+                        rec_t[0] = rec_t[1] = rec_t[2] = mean;
+                    } else {
+                        rec_t[0] = thr[0];
+                        rec_t[1] = thr[1];
+                        rec_t[2] = thr[2];
+                    }
                 }
             } else {
                 float const xl = rng.uniform();
@@ -973,11 +1118,22 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
                 } else {
                     flags &= ~Flag_translucent;
                 }
-                for (int k = 0; k < 3; ++k) {
-                    thr[k] *= bs.refl[k] / bs.pdf;
+                if constexpr (Spectral) {
+                    spectral::Wavelengths wl;
+                    wl.set(hero);
+                    float rl[4];
+                    wl.upsample4(bs.refl, rl);
+                    for (int k = 0; k < 4; ++k) {
+                        thr[k] *= rl[k] / bs.pdf;
+                    }
+                } else {
+                    for (int k = 0; k < 3; ++k) {
+                        thr[k] *= bs.refl[k] / bs.pdf;
+                    }
                 }
                 Path* const pp = &w.paths[p];
                 pp->bsdf_pdf   = bs.pdf;
+                pp->depth      = depth + 1;
                 pp->prev_n[0]  = geo_n.x;
                 pp->prev_n[1]  = geo_n.y;
                 pp->prev_n[2]  = geo_n.z;
@@ -1001,6 +1157,9 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
             pp->throughput[0] = thr[0];
             pp->throughput[1] = thr[1];
             pp->throughput[2] = thr[2];
+            if constexpr (Spectral) {
+                w.path_spec[p].x = thr[3];
+            }
             pp->flags         = flags;
             pp->rng           = rng;
             if (Guided) {
@@ -1034,17 +1193,96 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q, uint32
             w.rays[out_q][ray_slot]     = next_ray;
             w.ray_path[out_q][ray_slot] = p;
         }
+        if (w.regenerate) {
+            // the path ends here: its slot goes to the next sample once the
+            // vertex's shadow rays are resolved
+            bool const     ended = i < count && !spawn;
+            uint32_t const d     = block_queue_range(ended ? 1u : 0u, &w.counters[C_done]);
+            if (ended) {
+                w.done[d] = p;
+            }
+        }
     }
 }
 
 // 127 VGPRs: four waves per SIMD
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) shade(Wave w, uint32_t in_q,
-                                                                                   uint32_t depth) {
-    shade_queue<false>(w, in_q, depth);
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) shade(Wave w,
+                                                                                   uint32_t in_q) {
+    shade_queue<false, false>(w, in_q);
 }
 
-__global__ void __launch_bounds__(256) shade_guided(Wave w, uint32_t in_q, uint32_t depth) {
-    shade_queue<true>(w, in_q, depth);
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+shade_spectral(Wave w, uint32_t in_q) {
+    shade_queue<false, true>(w, in_q);
+}
+
+__global__ void __launch_bounds__(256) shade_guided_spectral(Wave w, uint32_t in_q) {
+    shade_queue<true, true>(w, in_q);
+}
+
+__global__ void __launch_bounds__(256) shade_guided(Wave w, uint32_t in_q) {
+    shade_queue<true, false>(w, in_q);
+}
+
+// Regenerating waves: every path that ended this bounce (its last vertex's
+// shadow rays now resolved) leaves its radiance in the batch's sample buffer,
+// and its slot starts the batch's next sample, whose camera ray joins the
+// queue the next bounce traces.  One atomic per block for the samples and
+// one for the queue.
+__global__ void __launch_bounds__(256) regenerate(Wave w, uint32_t out_q) {
+    uint32_t const count  = w.counters[C_done];
+    uint32_t const stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {  // block-uniform
+        uint32_t const i    = base + threadIdx.x;
+        bool const     have = i < count;
+        uint32_t       slot = 0;
+        if (have) {
+            slot                = w.done[i];
+            Path const&    path = w.paths[slot];
+            float* const   out  = w.sample_rad + 3 * size_t(path.sample);
+            out[0]              = path.radiance[0];
+            out[1]              = path.radiance[1];
+            out[2]              = path.radiance[2];
+        }
+        uint32_t const k     = block_queue_range(have ? 1u : 0u, &w.counters[C_next]);
+        bool const     start = have && k < w.batch_samples;
+        mi_ray         ray;
+        if (start) {
+            ray = start_path(w, k, slot);
+        }
+        uint32_t const q = block_queue_range(start ? 1u : 0u, &w.counters[C_queue0 + Counter_line * out_q]);
+        if (start) {
+            w.rays[out_q][q]     = ray;
+            w.ray_path[out_q][q] = slot;
+        }
+    }
+}
+
+// Regenerating waves: the batch's samples summed per pixel in the order the
+// waves of spw planes add them (the same FP64 sums as resolve).
+__global__ void __launch_bounds__(256) resolve_samples(Wave w, uint32_t planes, uint32_t spw,
+                                                       double* accum) {
+    uint32_t const pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= w.num_pixels) {
+        return;
+    }
+    double a[3] = {accum[3 * size_t(pixel) + 0], accum[3 * size_t(pixel) + 1],
+                   accum[3 * size_t(pixel) + 2]};
+    for (uint32_t k0 = 0; k0 < planes; k0 += spw) {
+        double s[3] = {0.0, 0.0, 0.0};
+        for (uint32_t k = k0; k < k0 + spw; ++k) {
+            float const* r = w.sample_rad + 3 * (size_t(k) * w.num_pixels + pixel);
+            for (int c = 0; c < 3; ++c) {
+                s[c] += double(r[c]);
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            a[c] += s[c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        accum[3 * size_t(pixel) + c] = a[c];
+    }
 }
 
 // The shadow ray of every light pick of the bounce (evaluate_light,
@@ -1054,7 +1292,10 @@ __global__ void __launch_bounds__(256) shade_guided(Wave w, uint32_t in_q, uint3
 // draw), the vertex's reflection toward it and the MIS weight against the
 // vertex's direction sampling (with the guide's mixture at a guided vertex).
 // A ray with nothing to add gets an empty interval.
-__global__ void __launch_bounds__(256) nee(Wave w) {
+// Spectral: the contribution per wavelength (the fourth in shadow_c4), the
+// light's emission and the reflection upsampled to the path's wavelengths.
+template <bool Spectral>
+__device__ __forceinline__ void nee_queue(Wave const& w) {
     uint32_t const count  = w.counters[C_shadow];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
@@ -1063,6 +1304,7 @@ __global__ void __launch_bounds__(256) nee(Wave w) {
         bool             want = false;
         mi_ray           shadow_ray;
         Shadow           sh;
+        float            c4 = 0.f;
         if (dev::Null != pick.light) {
             Rng rng;
             rng.state = v.rng_state;
@@ -1120,8 +1362,22 @@ __global__ void __launch_bounds__(256) nee(Wave w) {
                 }
                 float const           weight = predivided_power_heuristic(light_pdf, dir_pdf);
                 Render_material const lm     = w.materials[w.tri_material[lt]];
-                for (int c = 0; c < 3; ++c) {
-                    sh.contrib[c] = v.throughput[c] * (weight * (lm.emission[c] * f[c]));
+                if constexpr (Spectral) {
+                    spectral::Wavelengths wl;
+                    wl.set(w.path_spec[v.path].y);
+                    float el[4], fl[4];
+                    wl.upsample4(lm.emission, el);
+                    wl.upsample4(f, fl);
+                    float const t4[4] = {v.throughput[0], v.throughput[1], v.throughput[2],
+                                         w.nee_thr4[pick.vertex]};
+                    for (int c = 0; c < 3; ++c) {
+                        sh.contrib[c] = t4[c] * (weight * (el[c] * fl[c]));
+                    }
+                    c4 = t4[3] * (weight * (el[3] * fl[3]));
+                } else {
+                    for (int c = 0; c < 3; ++c) {
+                        sh.contrib[c] = v.throughput[c] * (weight * (lm.emission[c] * f[c]));
+                    }
                 }
                 sh.path                 = v.path;
                 shadow_ray.origin[0]    = p_o.x;
@@ -1132,7 +1388,8 @@ __global__ void __launch_bounds__(256) nee(Wave w) {
                 shadow_ray.direction[1] = wi.y;
                 shadow_ray.direction[2] = wi.z;
                 shadow_ray.max_t        = dist;
-                want = sh.contrib[0] > 0.f || sh.contrib[1] > 0.f || sh.contrib[2] > 0.f;
+                want = sh.contrib[0] > 0.f || sh.contrib[1] > 0.f || sh.contrib[2] > 0.f ||
+                       c4 > 0.f;
             }
         }
         if (!want) {
@@ -1145,10 +1402,22 @@ __global__ void __launch_bounds__(256) nee(Wave w) {
             shadow_ray.max_t        = 0.f;
             sh.contrib[0] = sh.contrib[1] = sh.contrib[2] = 0.f;
             sh.path                 = v.path;
+            c4                      = 0.f;
         }
         w.shadow_rays[j] = shadow_ray;
         w.shadows[j]     = sh;
+        if constexpr (Spectral) {
+            w.shadow_c4[j] = c4;
+        }
     }
+}
+
+__global__ void __launch_bounds__(256) nee(Wave w) {
+    nee_queue<false>(w);
+}
+
+__global__ void __launch_bounds__(256) nee_spectral(Wave w) {
+    nee_queue<true>(w);
 }
 
 // Adds the unoccluded light contributions of every vertex shaded this
@@ -1157,7 +1426,10 @@ __global__ void __launch_bounds__(256) nee(Wave w) {
 // attenuated by ratio tracking (Worker::transmitted: visibility, then
 // transmittance, rendering_worker.cpp:143-153), drawing from the path's
 // generator after the vertex's direction sample.
-__global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q, uint32_t depth) {
+// Spectral: transmittance per wavelength, then the film's response.
+template <bool Spectral>
+__device__ __forceinline__ void shadow_resolve_queue(Wave const& w, uint32_t in_q) {
+    constexpr int  NC     = Spectral ? 4 : 3;
     uint32_t const count  = w.counters[C_queue0 + Counter_line * in_q];
     uint32_t const stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
@@ -1165,29 +1437,52 @@ __global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q, uin
         if (0 == range.y) {
             continue;
         }
-        Path* const p   = &w.paths[w.ray_path[in_q][i]];
-        Rng         rng = p->rng;
-        float       radiance[3] = {p->radiance[0], p->radiance[1], p->radiance[2]};
+        uint32_t const path        = w.ray_path[in_q][i];
+        Path* const    p           = &w.paths[path];
+        Rng            rng         = p->rng;
+        float          radiance[3] = {p->radiance[0], p->radiance[1], p->radiance[2]};
+        spectral::Wavelengths wl;
+        dev::Channel_cc<NC>   cc;
+        if constexpr (Spectral) {
+            wl.set(w.path_spec[path].y);
+            wl.upsample4(w.medium.cc_a, cc.a);
+            wl.upsample4(w.medium.cc_s, cc.s);
+        } else {
+            cc = dev::rgb_channels(w.medium);
+        }
         for (uint32_t k = 0; k < range.y; ++k) {
             if (w.occluded[range.x + k]) {
                 continue;
             }
-            Shadow const s     = w.shadows[range.x + k];
-            float        tr[3] = {1.f, 1.f, 1.f};
+            Shadow const s = w.shadows[range.x + k];
+            float        tr[NC];
+            for (int c = 0; c < NC; ++c) {
+                tr[c] = 1.f;
+            }
             if (w.medium.present) {
                 mi_ray const r = w.shadow_rays[range.x + k];
                 V3 const     o = v3(r.origin[0], r.origin[1], r.origin[2]);
                 V3 const     d = v3(r.direction[0], r.direction[1], r.direction[2]);
                 float        t0, t1;
                 if (r.min_t < r.max_t && dev::box_overlap(w.medium, o, d, r.min_t, r.max_t, t0, t1)) {
-                    if (!dev::transmittance(w.medium, o, d, t0, t1, depth,
-                                            0 != (p->flags & Flag_nearest), rng, tr)) {
+                    if (!dev::transmittance<NC>(w.medium, cc, o, d, t0, t1, w.nee[i].depth,
+                                                0 != (p->flags & Flag_nearest), rng, tr)) {
                         continue;
                     }
                 }
             }
-            for (int c = 0; c < 3; ++c) {
-                radiance[c] += tr[c] * s.contrib[c];
+            if constexpr (Spectral) {
+                float const c4[4] = {tr[0] * s.contrib[0], tr[1] * s.contrib[1],
+                                     tr[2] * s.contrib[2], tr[3] * w.shadow_c4[range.x + k]};
+                float       rgb[3];
+                wl.to_rgb(w.spec_minv, c4, rgb);
+                for (int c = 0; c < 3; ++c) {
+                    radiance[c] += rgb[c];
+                }
+            } else {
+                for (int c = 0; c < 3; ++c) {
+                    radiance[c] += tr[c] * s.contrib[c];
+                }
             }
         }
         for (int c = 0; c < 3; ++c) {
@@ -1195,6 +1490,14 @@ __global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q, uin
         }
         p->rng = rng;
     }
+}
+
+__global__ void __launch_bounds__(256) shadow_resolve(Wave w, uint32_t in_q) {
+    shadow_resolve_queue<false>(w, in_q);
+}
+
+__global__ void __launch_bounds__(256) shadow_resolve_spectral(Wave w, uint32_t in_q) {
+    shadow_resolve_queue<true>(w, in_q);
 }
 
 // Marks every record slot of the wave empty; the shading kernels fill the
@@ -1742,6 +2045,9 @@ void RenderScene::release() {
     (void)hipFree(part_mapping);
     (void)hipFree(part_orders);
     (void)hipFree(part_lights);
+    (void)hipFree(emit_tris);
+    (void)hipFree(part_radiance);
+    emitter_work.release();
     release_medium();
     tlas.release();
     for (DeviceBvh& t : mesh_trees) {
@@ -1770,8 +2076,11 @@ constexpr double Max_optical_depth = 1.0e4;
 
 int32_t set_medium(RenderScene& scene, float const* density, uint32_t const dims[3],
                    float const box_lo[3], float const box_hi[3], float const cc_a[3],
-                   float const cc_s[3], float g) {
+                   float const cc_s[3], float g, uint32_t majorants) {
     scene.release_medium();
+    if (majorants > MI_MAJORANTS_FLAT) {
+        return fail(MI_ERR_ARGS, "medium majorants %u unknown", majorants);
+    }
     if (!density) {
         return MI_OK;
     }
@@ -1813,7 +2122,7 @@ int32_t set_medium(RenderScene& scene, float const* density, uint32_t const dims
     }
     int32_t const idims[3] = {int32_t(dims[0]), int32_t(dims[1]), int32_t(dims[2])};
     HostGridtree  tree;
-    build_gridtree(density, idims, cc_a, cc_s, tree);
+    build_gridtree(density, idims, cc_a, cc_s, tree, MI_MAJORANTS_FLAT == majorants);
 
     hipError_t e = hipMalloc(&scene.medium_density, n * sizeof(float));
     if (!e) e = hipMalloc(&scene.medium_nodes, tree.nodes.size() * sizeof(uint32_t));
@@ -1900,34 +2209,22 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
         float const* a = positions + 3 * size_t(indices[3 * size_t(t) + 0]);
         float const* b = positions + 3 * size_t(indices[3 * size_t(t) + 1]);
         float const* c = positions + 3 * size_t(indices[3 * size_t(t) + 2]);
-        float const  e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-        float const  e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-        float        n[3]  = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
-                              e1[0] * e2[1] - e1[1] * e2[0]};
-        float const  len   = std::sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
-        float const  area  = 0.5f * len;
         float const  radiance = std::max(std::max(mat.emission[0], mat.emission[1]), mat.emission[2]);
-        if (!(area > 0.f) || !(radiance > 0.f)) {
+        lb::Tri_light tl;
+        if (!lb::triangle_light(a, b, c, tl) || !(radiance > 0.f)) {
             continue;
         }
-        for (float& x : n) {
-            x /= len;
-        }
         Light_info& li = tri_info[t];
-        float lo[3], hi[3];
         for (int k = 0; k < 3; ++k) {
-            lo[k]        = std::min(std::min(a[k], b[k]), c[k]);
-            hi[k]        = std::max(std::max(a[k], b[k]), c[k]);
-            li.center[k] = 0.5f * (lo[k] + hi[k]);
-            li.cone[k]   = n[k];
+            li.center[k] = tl.center[k];
+            li.cone[k]   = tl.n[k];
         }
-        float const ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
-        li.radius    = 0.5f * std::sqrt((ext[0] * ext[0] + ext[1] * ext[1]) + ext[2] * ext[2]);
+        li.radius    = tl.radius;
         li.cone[3]   = 1.f;
-        li.power     = area;  // the part tree's power (no emission map)
+        li.power     = tl.area;  // the part tree's power (no emission map)
         li.two_sided = 0;
         li.triangle  = t;
-        li.area      = area;
+        li.area      = tl.area;
         uint32_t const m = tri_material[t];
         if (part_of[m] < 0) {
             part_of[m] = int32_t(part_tris.size());
@@ -1942,70 +2239,74 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
     std::vector<Light_info> infos(num_parts);
     std::vector<float>      bmin, bmax, cones, powers;
     std::vector<uint8_t>    two_sided(num_parts, 0);
-    // the part pools
+    // the part pools, laid out as the device builder writes them: part p
+    // with its lights from first_p keeps its nodes from 2 first_p - p (room
+    // for 2 n_p - 1) and its leaf distributions from 2 first_p (room for 2 n_p)
+    uint32_t num_emitters = 0;
+    for (auto const& tris : part_tris) {
+        num_emitters += uint32_t(tris.size());
+    }
     std::vector<dev::Part_info> pinfo(num_parts);
-    std::vector<Light_node>     pnodes;
-    std::vector<uint32_t>       pmiddles, pcdf_off, pmapping, porders;
-    std::vector<float>          pcdfs;
+    std::vector<Light_node>     pnodes(std::max<size_t>(1, 2 * size_t(num_emitters)));
+    std::vector<uint32_t>       pmiddles(pnodes.size(), 0), pcdf_off(pnodes.size(), dev::Null);
+    std::vector<uint32_t>       pmapping, porders, emit_tris, part_first(1, 0);
+    std::vector<float>          pcdfs(std::max<size_t>(1, 2 * size_t(num_emitters)), 0.f);
+    std::vector<float>          part_radiance(num_parts);
     std::vector<Light_info>     plights;
     for (uint32_t p = 0; p < num_parts; ++p) {
         std::vector<uint32_t> const& tris = part_tris[p];
         uint32_t const               n    = uint32_t(tris.size());
         std::vector<float>           tmin(3 * size_t(n)), tmax(3 * size_t(n)), tcone(4 * size_t(n)), tpow(n);
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        float axis[3] = {0.f, 0.f, 0.f};
-        float total   = 0.f;
+        auto const tri = [&](uint32_t i) {
+            float const*  a = positions + 3 * size_t(indices[3 * size_t(tris[i]) + 0]);
+            float const*  b = positions + 3 * size_t(indices[3 * size_t(tris[i]) + 1]);
+            float const*  c = positions + 3 * size_t(indices[3 * size_t(tris[i]) + 2]);
+            lb::Tri_light t;
+            lb::triangle_light(a, b, c, t);
+            return t;
+        };
         for (uint32_t i = 0; i < n; ++i) {
-            Light_info const& li = tri_info[tris[i]];
-            float const*      a  = positions + 3 * size_t(indices[3 * size_t(tris[i]) + 0]);
-            float const*      b  = positions + 3 * size_t(indices[3 * size_t(tris[i]) + 1]);
-            float const*      c  = positions + 3 * size_t(indices[3 * size_t(tris[i]) + 2]);
+            lb::Tri_light const t = tri(i);
             for (int k = 0; k < 3; ++k) {
-                tmin[3 * i + k] = std::min(std::min(a[k], b[k]), c[k]);
-                tmax[3 * i + k] = std::max(std::max(a[k], b[k]), c[k]);
-                lo[k]           = std::min(lo[k], tmin[3 * i + k]);
-                hi[k]           = std::max(hi[k], tmax[3 * i + k]);
-                axis[k] += li.power * li.cone[k];
+                tmin[3 * i + k] = t.lo[k];
+                tmax[3 * i + k] = t.hi[k];
             }
             for (int k = 0; k < 4; ++k) {
-                tcone[4 * i + k] = li.cone[k];
+                tcone[4 * i + k] = tri_info[tris[i]].cone[k];
             }
-            tpow[i] = li.power;
-            total += li.power;
+            tpow[i] = tri_info[tris[i]].power;
         }
         // Part::init: the dominant axis and the widest angle from it
-        float ax[3] = {axis[0] / total, axis[1] / total, axis[2] / total};
-        float const al = std::sqrt((ax[0] * ax[0] + ax[1] * ax[1]) + ax[2] * ax[2]);
-        for (float& x : ax) {
-            x /= al;
-        }
-        float angle = 0.f;
-        for (uint32_t i = 0; i < n; ++i) {
-            float const* nn = tri_info[tris[i]].cone;
-            float const  cc = (ax[0] * nn[0] + ax[1] * nn[1]) + ax[2] * nn[2];
-            angle           = std::max(angle, std::acos(std::min(std::max(cc, -1.f), 1.f)));
-        }
-        float const part_cone[4] = {ax[0], ax[1], ax[2], std::cos(angle)};
+        lb::Part_light pl;
+        lb::part_light(n, tri, pl);
+        float const* lo        = pl.lo;
+        float const* hi        = pl.hi;
+        float const  total     = pl.total;
+        float const* part_cone = pl.cone;
 
         HostLightTree tree;
         build_part_tree(tmin.data(), tmax.data(), tcone.data(), tpow.data(), n, part_cone, tree);
-        dev::Part_info& pi = pinfo[p];
-        pi.node_offset     = uint32_t(pnodes.size());
-        pi.light_offset    = uint32_t(plights.size());
-        pi.num_nodes       = uint32_t(tree.nodes.size());
-        pi.num_lights      = n;
-        uint32_t const cdf_base = uint32_t(pcdfs.size());
+        dev::Part_info& pi      = pinfo[p];
+        uint32_t const  first_p = uint32_t(plights.size());
+        pi.node_offset          = 2 * first_p - p;
+        pi.light_offset         = first_p;
+        pi.num_nodes            = uint32_t(tree.nodes.size());
+        pi.num_lights           = n;
+        uint32_t const cdf_base = 2 * first_p;
         for (uint32_t i = 0; i < tree.nodes.size(); ++i) {
-            pnodes.push_back(tree.nodes[i]);
-            pmiddles.push_back(tree.middles[i]);
-            pcdf_off.push_back(tree.cdf_offset[i] == 0xFFFFFFFFu ? dev::Null : cdf_base + tree.cdf_offset[i]);
+            pnodes[pi.node_offset + i]   = tree.nodes[i];
+            pmiddles[pi.node_offset + i] = tree.middles[i];
+            pcdf_off[pi.node_offset + i] =
+                tree.cdf_offset[i] == 0xFFFFFFFFu ? dev::Null : cdf_base + tree.cdf_offset[i];
         }
-        pcdfs.insert(pcdfs.end(), tree.cdfs.begin(), tree.cdfs.end());
+        std::copy(tree.cdfs.begin(), tree.cdfs.end(), pcdfs.begin() + cdf_base);
         for (uint32_t i = 0; i < n; ++i) {
             pmapping.push_back(tree.light_mapping[i]);
             porders.push_back(tree.light_orders[i]);
             plights.push_back(tri_info[tris[i]]);
+            emit_tris.push_back(tris[i]);
         }
+        part_first.push_back(uint32_t(plights.size()));
 
         // the scene light of the part
         float const radiance = std::max(std::max(materials[tri_material[tris[0]]].emission[0],
@@ -2028,6 +2329,7 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
         L.triangle  = p;
         L.area      = total;
         powers.push_back(L.power);
+        part_radiance[p] = radiance;
     }
     HostLightTree tree;
     build_light_tree(bmin.data(), bmax.data(), cones.data(), powers.data(), two_sided.data(),
@@ -2051,7 +2353,7 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
     }
 
     size_t const nl = std::max<size_t>(1, infos.size());
-    size_t const nn = std::max<size_t>(1, tree.nodes.size());
+    size_t const nn = std::max<size_t>(1, 2 * infos.size());  // room for any tree over the parts
     hipError_t   e  = hipMalloc(&s.positions, size_t(num_vertices) * 3 * sizeof(float));
     if (!e) e = hipMalloc(&s.indices, size_t(num_triangles) * 3 * sizeof(uint32_t));
     if (!e) e = hipMalloc(&s.tri_material, size_t(num_triangles) * sizeof(uint32_t));
@@ -2090,6 +2392,12 @@ int32_t create_scene(float const* positions, uint32_t num_vertices, uint32_t con
     upload(s.part_orders, porders);
     upload(s.part_lights, plights);
     upload(s.tri_local, tri_local);
+    upload(s.emit_tris, emit_tris);
+    upload(s.part_radiance, part_radiance);
+    s.part_first     = part_first;
+    s.part_info_host = pinfo;
+    s.num_emitters   = num_emitters;
+    s.bvh_settings   = settings;
     if (e) {
         s.release();
         return fail(MI_ERR_HIP, "mi_scene_create: %s", hipGetErrorString(e));
@@ -2177,6 +2485,345 @@ int32_t make_sensor_filter(mi_sensor const& in, Sensor_filter& f) {
             v *= scale;
         }
     }
+    return MI_OK;
+}
+
+namespace {
+
+// Every emissive triangle (part order) as a light: the part trees' inputs
+// and the part's light records, as create_scene forms them.
+__global__ void emitter_lights(float const* pos, uint32_t const* idx, uint32_t const* emit_tris,
+                               uint32_t n, Emitter_work w, Light_info* plights) {
+    uint32_t const j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) {
+        return;
+    }
+    uint32_t const t = emit_tris[j];
+    lb::Tri_light  tl;
+    if (!lb::triangle_light(pos + 3 * size_t(idx[3 * size_t(t) + 0]),
+                            pos + 3 * size_t(idx[3 * size_t(t) + 1]),
+                            pos + 3 * size_t(idx[3 * size_t(t) + 2]), tl)) {
+        atomicOr(w.bad, 1u);
+        return;
+    }
+    Light_info li;
+    for (int k = 0; k < 3; ++k) {
+        w.tri_lo[3 * size_t(j) + k]   = tl.lo[k];
+        w.tri_hi[3 * size_t(j) + k]   = tl.hi[k];
+        w.tri_cone[4 * size_t(j) + k] = tl.n[k];
+        li.center[k]                  = tl.center[k];
+        li.cone[k]                    = tl.n[k];
+    }
+    w.tri_cone[4 * size_t(j) + 3] = 1.f;
+    w.tri_power[j]                = tl.area;
+    li.radius    = tl.radius;
+    li.cone[3]   = 1.f;
+    li.power     = tl.area;
+    li.two_sided = 0;
+    li.triangle  = t;
+    li.area      = tl.area;
+    plights[j]   = li;
+}
+
+// Part::init of every part and its light record in the scene tree.
+__global__ void part_lights(float const* pos, uint32_t const* idx, uint32_t const* emit_tris,
+                            float const* radiance, uint32_t num_parts, Emitter_work w,
+                            Light_info* infos) {
+    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= num_parts) {
+        return;
+    }
+    uint32_t const first = w.part_first[r];
+    uint32_t const n     = w.part_first[r + 1] - first;
+    lb::Part_light pl;
+    lb::part_light(n,
+                   [&](uint32_t i) {
+                       uint32_t const t = emit_tris[first + i];
+                       lb::Tri_light  tl;
+                       lb::triangle_light(pos + 3 * size_t(idx[3 * size_t(t) + 0]),
+                                          pos + 3 * size_t(idx[3 * size_t(t) + 1]),
+                                          pos + 3 * size_t(idx[3 * size_t(t) + 2]), tl);
+                       return tl;
+                   },
+                   pl);
+    Light_info L;
+    for (int k = 0; k < 3; ++k) {
+        w.part_lo[3 * size_t(r) + k] = pl.lo[k];
+        w.part_hi[3 * size_t(r) + k] = pl.hi[k];
+        L.center[k]                  = 0.5f * (pl.lo[k] + pl.hi[k]);
+    }
+    float const ext[3] = {pl.hi[0] - pl.lo[0], pl.hi[1] - pl.lo[1], pl.hi[2] - pl.lo[2]};
+    L.radius           = 0.5f * sqrtf((ext[0] * ext[0] + ext[1] * ext[1]) + ext[2] * ext[2]);
+    for (int k = 0; k < 4; ++k) {
+        w.part_cone[4 * size_t(r) + k] = pl.cone[k];
+        L.cone[k]                      = pl.cone[k];
+    }
+    L.power       = pl.total * radiance[r];
+    L.two_sided   = 0;
+    L.triangle    = r;
+    L.area        = pl.total;
+    w.part_pow[r] = L.power;
+    w.part_two[r] = 0;
+    infos[r]      = L;
+}
+
+// The part pools' cdf offsets, absolute: part r's distributions start at 2 first_r.
+__global__ void absolute_cdf_offsets(Emitter_work w, dev::Part_info const* infos,
+                                     uint32_t num_parts, uint32_t* cdf_offset) {
+    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= num_parts) {
+        return;
+    }
+    dev::Part_info const pi = infos[r];
+    for (uint32_t i = pi.node_offset; i < pi.node_offset + pi.num_nodes; ++i) {
+        uint32_t const rel = w.cdf_rel[i];
+        cdf_offset[i]      = dev::Null == rel ? dev::Null : 2 * pi.light_offset + rel;
+    }
+}
+
+// Per block: min and max of the vertices it covers.
+__global__ void __launch_bounds__(256) vertex_bounds(float const* pos, uint32_t n, float* out) {
+    __shared__ float red[6][256];
+    float            b[6] = {3.40282347e+38f, 3.40282347e+38f, 3.40282347e+38f,
+                             -3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f};
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            float const x = pos[3 * size_t(v) + k];
+            b[k]          = fminf(b[k], x);
+            b[3 + k]      = fmaxf(b[3 + k], x);
+        }
+    }
+    for (int k = 0; k < 6; ++k) {
+        red[k][threadIdx.x] = b[k];
+    }
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            for (int k = 0; k < 3; ++k) {
+                red[k][threadIdx.x]     = fminf(red[k][threadIdx.x], red[k][threadIdx.x + off]);
+                red[3 + k][threadIdx.x] = fmaxf(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + off]);
+            }
+        }
+        __syncthreads();
+    }
+    if (0 == threadIdx.x) {
+        for (int k = 0; k < 6; ++k) {
+            out[6 * size_t(blockIdx.x) + k] = red[k][0];
+        }
+    }
+}
+
+constexpr uint32_t Bounds_blocks = 256;
+
+}  // namespace
+
+void Emitter_work::release() {
+    (void)hipFree(tri_lo);
+    (void)hipFree(tri_hi);
+    (void)hipFree(tri_cone);
+    (void)hipFree(tri_power);
+    (void)hipFree(part_cone);
+    (void)hipFree(part_lo);
+    (void)hipFree(part_hi);
+    (void)hipFree(part_pow);
+    (void)hipFree(part_two);
+    (void)hipFree(cdf_rel);
+    (void)hipFree(num_cdfs);
+    (void)hipFree(part_first);
+    (void)hipFree(bad);
+    (void)hipFree(bounds);
+    *this = Emitter_work();
+}
+
+int32_t update_positions(RenderScene& s, float const* positions, uint32_t num_vertices,
+                         Light_forest_scratch& scratch, hipStream_t stream, double* light_ms) {
+    mi_bvh_settings const settings = s.bvh_settings;
+    if (s.instanced) {
+        return fail(MI_ERR_ARGS, "mi_scene_update_positions: an instanced scene (move its "
+                                 "instances with mi_scene_update_instances)");
+    }
+    if (num_vertices != s.num_vertices || !positions) {
+        return fail(MI_ERR_ARGS, "mi_scene_update_positions: %u vertices for a scene of %u",
+                    num_vertices, s.num_vertices);
+    }
+    uint32_t const N = s.num_emitters;
+    uint32_t const P = s.num_lights;
+    Emitter_work&  w = s.emitter_work;
+    hipError_t     e = hipSuccess;
+    if (N > w.capacity_tris || P > w.capacity_parts || !w.bad) {
+        w.release();
+        size_t const n = std::max<size_t>(1, N), p = std::max<size_t>(1, P);
+        e = hipMalloc(&w.tri_lo, 3 * n * sizeof(float));
+        if (!e) e = hipMalloc(&w.tri_hi, 3 * n * sizeof(float));
+        if (!e) e = hipMalloc(&w.tri_cone, 4 * n * sizeof(float));
+        if (!e) e = hipMalloc(&w.tri_power, n * sizeof(float));
+        if (!e) e = hipMalloc(&w.part_cone, 4 * p * sizeof(float));
+        if (!e) e = hipMalloc(&w.part_lo, 3 * p * sizeof(float));
+        if (!e) e = hipMalloc(&w.part_hi, 3 * p * sizeof(float));
+        if (!e) e = hipMalloc(&w.part_pow, p * sizeof(float));
+        if (!e) e = hipMalloc(&w.part_two, p);
+        if (!e) e = hipMalloc(&w.cdf_rel, 2 * n * sizeof(uint32_t));
+        if (!e) e = hipMalloc(&w.num_cdfs, p * sizeof(uint32_t));
+        if (!e) e = hipMalloc(&w.part_first, (p + 1) * sizeof(uint32_t));
+        if (!e) e = hipMalloc(&w.bad, sizeof(uint32_t));
+        if (!e) e = hipMalloc(&w.bounds, 6 * Bounds_blocks * sizeof(float));
+        if (!e && P) {
+            e = hipMemcpy(w.part_first, s.part_first.data(), (P + 1) * sizeof(uint32_t),
+                          hipMemcpyHostToDevice);
+        }
+        if (e) {
+            w.release();
+            return fail(MI_ERR_HIP, "mi_scene_update_positions: %s", hipGetErrorString(e));
+        }
+        w.capacity_tris  = N;
+        w.capacity_parts = P;
+    }
+    // the new positions, checked before the scene takes them: every
+    // emissive triangle must keep an area
+    float* fresh = nullptr;
+    e            = hipMalloc(&fresh, std::max<size_t>(1, num_vertices) * 3 * sizeof(float));
+    if (!e) e = hipMemcpyAsync(fresh, positions, size_t(num_vertices) * 3 * sizeof(float),
+                               hipMemcpyDeviceToDevice, stream);
+    if (!e) e = hipMemsetAsync(w.bad, 0, sizeof(uint32_t), stream);
+    if (e) {
+        (void)hipFree(fresh);
+        return fail(MI_ERR_HIP, "mi_scene_update_positions: %s", hipGetErrorString(e));
+    }
+    auto const t0 = std::chrono::steady_clock::now();
+    if (N) {
+        hipLaunchKernelGGL(emitter_lights, dim3(blocks(N, 256)), dim3(256), 0, stream, fresh,
+                           s.indices, s.emit_tris, N, w, s.part_lights);
+        uint32_t bad = 0;
+        e = hipMemcpyAsync(&bad, w.bad, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+        if (!e) e = hipStreamSynchronize(stream);
+        if (e || bad) {
+            (void)hipFree(fresh);
+            // the part lights of the old positions are made again below on
+            // success only; restore them by recomputing from the scene's own
+            hipLaunchKernelGGL(emitter_lights, dim3(blocks(N, 256)), dim3(256), 0, stream,
+                               s.positions, s.indices, s.emit_tris, N, w, s.part_lights);
+            (void)hipStreamSynchronize(stream);
+            return e ? fail(MI_ERR_HIP, "mi_scene_update_positions: %s", hipGetErrorString(e))
+                     : fail(MI_ERR_ARGS, "mi_scene_update_positions: an emissive triangle has "
+                                         "no area at the new positions");
+        }
+    }
+    (void)hipFree(s.positions);
+    s.positions = fresh;
+
+    if (P) {
+        hipLaunchKernelGGL(part_lights, dim3(blocks(P, 64)), dim3(64), 0, stream, s.positions,
+                           s.indices, s.emit_tris, s.part_radiance, P, w, s.light_infos);
+        // every part's Primitive_tree at once
+        Light_forest pin;
+        pin.set        = {w.tri_lo, w.tri_hi, w.tri_cone, w.tri_power, nullptr};
+        pin.first      = s.part_first.data();
+        pin.part_cones = w.part_cone;
+        pin.num_roots  = P;
+        pin.num_lights = N;
+        pin.part       = true;
+        std::vector<uint32_t> part_nodes(P);
+        Light_forest_out pout;
+        pout.nodes          = s.part_nodes;
+        pout.middles        = s.part_middles;
+        pout.mapping        = s.part_mapping;
+        pout.orders         = s.part_orders;
+        pout.cdf_offset     = w.cdf_rel;
+        pout.cdfs           = s.part_cdfs;
+        pout.num_cdfs       = w.num_cdfs;
+        pout.num_nodes_host = part_nodes.data();
+        if (int32_t const st = build_light_forest(pin, scratch, pout, stream); st) {
+            return fail(st, "mi_scene_update_positions: part trees");
+        }
+        for (uint32_t r = 0; r < P; ++r) {
+            s.part_info_host[r].num_nodes = part_nodes[r];
+        }
+        e = hipMemcpyAsync(s.part_infos, s.part_info_host.data(), P * sizeof(dev::Part_info),
+                           hipMemcpyHostToDevice, stream);
+        if (e) {
+            return fail(MI_ERR_HIP, "mi_scene_update_positions: %s", hipGetErrorString(e));
+        }
+        hipLaunchKernelGGL(absolute_cdf_offsets, dim3(blocks(P, 64)), dim3(64), 0, stream, w,
+                           s.part_infos, P, s.part_cdf_offset);
+        // the scene tree over the parts
+        uint32_t const first[2] = {0, P};
+        Light_forest   sin;
+        sin.set        = {w.part_lo, w.part_hi, w.part_cone, w.part_pow, w.part_two};
+        sin.first      = first;
+        sin.num_roots  = 1;
+        sin.num_lights = P;
+        uint32_t         scene_nodes = 0;
+        Light_forest_out sout;
+        sout.nodes          = s.light_nodes;
+        sout.middles        = s.light_middles;
+        sout.mapping        = s.light_mapping;
+        sout.orders         = s.light_orders;
+        sout.num_nodes_host = &scene_nodes;
+        int32_t st = build_light_forest(sin, scratch, sout, stream);
+        if (MI_ERR_UNSPLITTABLE == st) {
+            // a node only the host builder's median split divides: build there
+            std::vector<float>   lo(3 * size_t(P)), hi(3 * size_t(P)), cn(4 * size_t(P)), pw(P);
+            std::vector<uint8_t> two(P);
+            e = hipMemcpyAsync(lo.data(), w.part_lo, lo.size() * sizeof(float), hipMemcpyDeviceToHost, stream);
+            if (!e) e = hipMemcpyAsync(hi.data(), w.part_hi, hi.size() * sizeof(float), hipMemcpyDeviceToHost, stream);
+            if (!e) e = hipMemcpyAsync(cn.data(), w.part_cone, cn.size() * sizeof(float), hipMemcpyDeviceToHost, stream);
+            if (!e) e = hipMemcpyAsync(pw.data(), w.part_pow, pw.size() * sizeof(float), hipMemcpyDeviceToHost, stream);
+            if (!e) e = hipMemcpyAsync(two.data(), w.part_two, two.size(), hipMemcpyDeviceToHost, stream);
+            if (!e) e = hipStreamSynchronize(stream);
+            HostLightTree t;
+            if (!e) {
+                build_light_tree(lo.data(), hi.data(), cn.data(), pw.data(), two.data(), P, t);
+                scene_nodes = uint32_t(t.nodes.size());
+                e = hipMemcpyAsync(s.light_nodes, t.nodes.data(), t.nodes.size() * sizeof(Light_node), hipMemcpyHostToDevice, stream);
+            }
+            if (!e) e = hipMemcpyAsync(s.light_middles, t.middles.data(), t.middles.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream);
+            if (!e) e = hipMemcpyAsync(s.light_mapping, t.light_mapping.data(), P * sizeof(uint32_t), hipMemcpyHostToDevice, stream);
+            if (!e) e = hipMemcpyAsync(s.light_orders, t.light_orders.data(), P * sizeof(uint32_t), hipMemcpyHostToDevice, stream);
+            if (!e) e = hipStreamSynchronize(stream);
+            st = e ? MI_ERR_HIP : MI_OK;
+        }
+        if (st) {
+            return fail(st, "mi_scene_update_positions: scene light tree");
+        }
+        s.num_light_nodes = scene_nodes;
+    }
+    e = hipStreamSynchronize(stream);
+    if (e) {
+        return fail(MI_ERR_HIP, "mi_scene_update_positions: %s", hipGetErrorString(e));
+    }
+    if (light_ms) {
+        *light_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    // the bounds of every vertex (the guide's position inputs)
+    if (num_vertices) {
+        hipLaunchKernelGGL(vertex_bounds, dim3(Bounds_blocks), dim3(256), 0, stream, s.positions,
+                           num_vertices, w.bounds);
+        std::vector<float> b(6 * Bounds_blocks);
+        e = hipMemcpyAsync(b.data(), w.bounds, b.size() * sizeof(float), hipMemcpyDeviceToHost, stream);
+        if (!e) e = hipStreamSynchronize(stream);
+        if (e) {
+            return fail(MI_ERR_HIP, "mi_scene_update_positions: %s", hipGetErrorString(e));
+        }
+        for (int k = 0; k < 3; ++k) {
+            s.box_lo[k] = b[k];
+            s.box_hi[k] = b[3 + k];
+        }
+        for (uint32_t i = 1; i < Bounds_blocks; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                s.box_lo[k] = std::min(s.box_lo[k], b[6 * i + k]);
+                s.box_hi[k] = std::max(s.box_hi[k], b[6 * i + 3 + k]);
+            }
+        }
+    }
+    // the traced geometry: a new tree from the GPU builder
+    DeviceBvh bvh;
+    if (int32_t const st = build_bvh(s.positions, num_vertices, 3, s.indices, s.num_triangles,
+                                     settings, nullptr, bvh);
+        st) {
+        return st;
+    }
+    s.bvh.release();
+    s.bvh = bvh;
     return MI_OK;
 }
 
@@ -2386,8 +3033,19 @@ void RenderState::release() {
     (void)hipFree(rec_path);
     (void)hipFree(rec_block);
     (void)hipFree(pixel_samples);
+    (void)hipFree(sample_rad);
+    (void)hipFree(done);
+    (void)hipFree(path_spec);
+    (void)hipFree(nee_thr4);
+    (void)hipFree(shadow_c4);
+    if (live_host) {
+        (void)hipHostFree(live_host);
+    }
     if (graph) {
         (void)hipGraphExecDestroy(graph);
+    }
+    if (regen_graph) {
+        (void)hipGraphExecDestroy(regen_graph);
     }
     if (begin) {
         (void)hipEventDestroy(begin);
@@ -2466,6 +3124,123 @@ int32_t RenderState::reserve(uint32_t n, uint32_t num_shadow) {
         return fail(MI_ERR_HIP, "render state of %u paths: %s", n, hipGetErrorString(e));
     }
     capacity = n;
+    return MI_OK;
+}
+
+namespace spectral {
+
+void response_matrix(float minv[9]) {
+    // the bands in double, as bands() forms them
+    auto band = [](double lambda, double b[3]) {
+        double blue = (510.0 - lambda) / 60.0;
+        double red  = (lambda - 560.0) / 50.0;
+        blue        = blue < 0.0 ? 0.0 : (blue > 1.0 ? 1.0 : blue);
+        red         = red < 0.0 ? 0.0 : (red > 1.0 ? 1.0 : red);
+        b[0]        = red;
+        b[1]        = (1.0 - blue) - red;
+        b[2]        = blue;
+    };
+    double const knots[6] = {380.0, 450.0, 510.0, 560.0, 610.0, 720.0};
+    double       m[9]     = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int piece = 0; piece < 5; ++piece) {
+        double const a = knots[piece];
+        double const c = knots[piece + 1];
+        double       ba[3], bm[3], bc[3];
+        band(a, ba);
+        band(0.5 * (a + c), bm);
+        band(c, bc);
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) {
+                m[3 * i + j] += (c - a) / 6.0 * (ba[i] * ba[j] + 4.0 * bm[i] * bm[j] + bc[i] * bc[j]);
+            }
+        }
+    }
+    for (double& v : m) {
+        v /= double(Lambda_range);
+    }
+    // the inverse by cofactors
+    double const c00 = m[4] * m[8] - m[5] * m[7];
+    double const c01 = m[5] * m[6] - m[3] * m[8];
+    double const c02 = m[3] * m[7] - m[4] * m[6];
+    double const det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+    double const inv[9] = {c00 / det,
+                           (m[2] * m[7] - m[1] * m[8]) / det,
+                           (m[1] * m[5] - m[2] * m[4]) / det,
+                           c01 / det,
+                           (m[0] * m[8] - m[2] * m[6]) / det,
+                           (m[2] * m[3] - m[0] * m[5]) / det,
+                           c02 / det,
+                           (m[1] * m[6] - m[0] * m[7]) / det,
+                           (m[0] * m[4] - m[1] * m[3]) / det};
+    for (int k = 0; k < 9; ++k) {
+        minv[k] = float(inv[k]);
+    }
+}
+
+}  // namespace spectral
+
+int32_t RenderState::reserve_spectral(uint32_t n, uint32_t num_shadow) {
+    if ((n > spectral_capacity || num_shadow > spectral_shadow_capacity) && stream) {
+        (void)hipStreamSynchronize(stream);
+    }
+    if (n > spectral_capacity) {
+        (void)hipFree(path_spec);
+        (void)hipFree(nee_thr4);
+        path_spec         = nullptr;
+        nee_thr4          = nullptr;
+        spectral_capacity = 0;
+        hipError_t e      = hipMalloc(&path_spec, size_t(n) * sizeof(float2));
+        if (!e) e = hipMalloc(&nee_thr4, size_t(n) * sizeof(float));
+        if (e) {
+            return fail(MI_ERR_HIP, "spectral state of %u paths: %s", n, hipGetErrorString(e));
+        }
+        spectral_capacity = n;
+    }
+    if (num_shadow > spectral_shadow_capacity) {
+        (void)hipFree(shadow_c4);
+        shadow_c4                = nullptr;
+        spectral_shadow_capacity = 0;
+        if (hipError_t const e = hipMalloc(&shadow_c4, size_t(num_shadow) * sizeof(float)); e) {
+            return fail(MI_ERR_HIP, "spectral shadow queue of %u rays: %s", num_shadow,
+                        hipGetErrorString(e));
+        }
+        spectral_shadow_capacity = num_shadow;
+    }
+    return MI_OK;
+}
+
+int32_t RenderState::reserve_regeneration(uint64_t samples, uint32_t slots) {
+    if (!live_host) {
+        if (hipError_t const e = hipHostMalloc(&live_host, sizeof(uint32_t)); e) {
+            live_host = nullptr;
+            return fail(MI_ERR_HIP, "regeneration flag: %s", hipGetErrorString(e));
+        }
+    }
+    if (samples <= sample_capacity && slots <= done_capacity) {
+        return MI_OK;
+    }
+    if (stream) {
+        (void)hipStreamSynchronize(stream);  // earlier batches may still use the buffers
+    }
+    if (samples > sample_capacity) {
+        (void)hipFree(sample_rad);
+        sample_rad      = nullptr;
+        sample_capacity = 0;
+        if (hipError_t const e = hipMalloc(&sample_rad, samples * 3 * sizeof(float)); e) {
+            return fail(MI_ERR_HIP, "sample buffer of %llu samples: %s", (unsigned long long)samples,
+                        hipGetErrorString(e));
+        }
+        sample_capacity = samples;
+    }
+    if (slots > done_capacity) {
+        (void)hipFree(done);
+        done          = nullptr;
+        done_capacity = 0;
+        if (hipError_t const e = hipMalloc(&done, size_t(slots) * sizeof(uint32_t)); e) {
+            return fail(MI_ERR_HIP, "ended-path list of %u slots: %s", slots, hipGetErrorString(e));
+        }
+        done_capacity = slots;
+    }
     return MI_OK;
 }
 
@@ -2612,8 +3387,33 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
                     (unsigned long long)shadow64);
     }
     uint32_t const num_shadow = uint32_t(shadow64);
+    // regenerating waves: plain renders through the box sensor; the frame
+    // in equal batches of whole waves whose samples fit the sample buffer
+    // (2^27 samples): the largest multiple of spw dividing spp that fits
+    bool const regen        = !guided && !sensor && 0 != (set.flags & MI_RENDER_REGENERATE);
+    uint32_t   batch_planes = spw;
+    if (regen) {
+        uint32_t const fit = uint32_t(std::max<uint64_t>(1, (uint64_t(1) << 27) / num_pixels));
+        for (uint32_t planes = spw; planes <= std::min(set.samples_per_pixel, fit); planes += spw) {
+            if (0 == set.samples_per_pixel % planes) {
+                batch_planes = planes;
+            }
+        }
+    }
+    uint64_t const batch_samples = uint64_t(batch_planes) * num_pixels;
+    if (regen) {
+        if (int32_t const st = state.reserve_regeneration(batch_samples, num_paths); st) {
+            return st;
+        }
+    }
     if (int32_t const st = state.reserve(num_paths, num_shadow); st) {
         return st;
+    }
+    bool const spectral = 0 != (set.flags & MI_RENDER_SPECTRAL);
+    if (spectral) {
+        if (int32_t const st = state.reserve_spectral(num_paths, num_shadow); st) {
+            return st;
+        }
     }
     if (int32_t const st = prepare_trace(scene.bvh, trace, std::max(num_paths, num_shadow)); st) {
         return st;
@@ -2681,6 +3481,10 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.max_bounces = set.max_bounces;
     w.seed        = set.seed;
     w.num_paths   = num_paths;
+    w.batch_samples = regen ? uint32_t(batch_samples) : num_paths;
+    w.regenerate    = regen ? 1u : 0u;
+    w.sample_rad    = regen ? state.sample_rad : nullptr;
+    w.done          = regen ? state.done : nullptr;
     w.num_pixels  = num_pixels;
     w.paths       = static_cast<Path*>(state.paths);
     w.rays[0]     = state.rays[0];
@@ -2709,6 +3513,14 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     w.guide_raw       = state.guide_raw;
     w.guide_x         = state.guide_x;
     w.guide_depth     = guide_depth;
+    w.guide_medium    = guided && 0 != gs->medium ? 1u : 0u;
+    if (spectral) {
+        w.spectral  = 1;
+        spectral::response_matrix(w.spec_minv);
+        w.path_spec = state.path_spec;
+        w.nee_thr4  = state.nee_thr4;
+        w.shadow_c4 = state.shadow_c4;
+    }
     w.record_depth    = record_depth;
     w.record_capacity = records;
     w.train_x         = state.train_x;
@@ -2763,7 +3575,8 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
                 return st;
             }
             if (scene.medium.present) {
-                hipLaunchKernelGGL(medium_track, dim3(grid_path), dim3(256), 0, s, w, q, d);
+                hipLaunchKernelGGL(spectral ? medium_track_spectral : medium_track,
+                                   dim3(grid_path), dim3(256), 0, s, w, q);
             }
             hipLaunchKernelGGL(reset_queues, dim3(1), dim3(1), 0, s, state.counters, q ^ 1);
             if (guided && d < guide_depth) {
@@ -2774,19 +3587,23 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
                     st) {
                     return st;
                 }
-                hipLaunchKernelGGL(shade_guided, dim3(grid_path), dim3(256), 0, s, w, q, d);
+                hipLaunchKernelGGL(spectral ? shade_guided_spectral : shade_guided,
+                                   dim3(grid_path), dim3(256), 0, s, w, q);
             } else if (guided && d <= record_depth) {
                 // BSDF sampling; closes the records of the vertex before
-                hipLaunchKernelGGL(shade_guided, dim3(grid_path), dim3(256), 0, s, w, q, d);
+                hipLaunchKernelGGL(spectral ? shade_guided_spectral : shade_guided,
+                                   dim3(grid_path), dim3(256), 0, s, w, q);
             } else {
-                hipLaunchKernelGGL(shade, dim3(grid_path), dim3(256), 0, s, w, q, d);
+                hipLaunchKernelGGL(spectral ? shade_spectral : shade, dim3(grid_path), dim3(256),
+                                   0, s, w, q);
             }
-            hipLaunchKernelGGL(nee, dim3(grid_shadow), dim3(256), 0, s, w);
+            hipLaunchKernelGGL(spectral ? nee_spectral : nee, dim3(grid_shadow), dim3(256), 0, s, w);
             hipLaunchKernelGGL(tally, dim3(1), dim3(1), 0, s, state.counters, q);
             if (int32_t const st = trace_any(); st) {
                 return st;
             }
-            hipLaunchKernelGGL(shadow_resolve, dim3(grid_path), dim3(256), 0, s, w, q, d);
+            hipLaunchKernelGGL(spectral ? shadow_resolve_spectral : shadow_resolve,
+                               dim3(grid_path), dim3(256), 0, s, w, q);
             q ^= 1;
         }
         if (sensor) {
@@ -2820,6 +3637,29 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         return MI_OK;
     };
 
+    // one bounce of a regenerating batch: trace queue q, shade into q ^ 1,
+    // resolve the shadow rays, restart the ended paths' slots into q ^ 1
+    auto enqueue_bounce = [&](uint32_t q) -> int32_t {
+        if (int32_t const st = trace_closest(state.rays[q], &state.counters[C_queue0 + Counter_line * q]); st) {
+            return st;
+        }
+        if (scene.medium.present) {
+            hipLaunchKernelGGL(spectral ? medium_track_spectral : medium_track, dim3(grid_path),
+                               dim3(256), 0, s, w, q);
+        }
+        hipLaunchKernelGGL(reset_queues, dim3(1), dim3(1), 0, s, state.counters, q ^ 1);
+        hipLaunchKernelGGL(spectral ? shade_spectral : shade, dim3(grid_path), dim3(256), 0, s, w, q);
+        hipLaunchKernelGGL(spectral ? nee_spectral : nee, dim3(grid_shadow), dim3(256), 0, s, w);
+        hipLaunchKernelGGL(tally, dim3(1), dim3(1), 0, s, state.counters, q);
+        if (int32_t const st = trace_any(); st) {
+            return st;
+        }
+        hipLaunchKernelGGL(spectral ? shadow_resolve_spectral : shadow_resolve, dim3(grid_path),
+                           dim3(256), 0, s, w, q);
+        hipLaunchKernelGGL(regenerate, dim3(grid_path), dim3(256), 0, s, w, q ^ 1);
+        return MI_OK;
+    };
+
     // order after the caller's earlier work on its stream
     hipError_t e = hipEventRecord(state.begin, stream);
     if (!e) e = hipStreamWaitEvent(s, state.begin, 0);
@@ -2835,7 +3675,106 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         hipLaunchKernelGGL(reset_records, dim3(1), dim3(1), 0, s, state.counters);
     }
 
-    if (set.flags & MI_RENDER_NO_GRAPH) {
+    if (regen) {
+        // Bounces run until the queue empties: a slot whose path ends starts
+        // the batch's next sample, so the queue stays full until the batch's
+        // samples run out.  Two bounces (queues 0 -> 1 -> 0) form one
+        // captured graph, replayed until the queue to trace next is empty;
+        // every path ends within max_bounces + 1 bounces, which bounds the
+        // replays.
+        uint32_t const live_slots = uint32_t(std::min<uint64_t>(num_paths, batch_samples));
+        uint64_t const max_pairs =
+            (uint64_t(set.max_bounces) + 2) * ((batch_samples + live_slots - 1) / live_slots + 1) + 8;
+        bool const use_graph = 0 == (set.flags & MI_RENDER_NO_GRAPH);
+        if (use_graph) {
+            std::vector<uint8_t> key(sizeof(Wave));
+            std::memcpy(key.data(), &w, sizeof(Wave));
+            auto put = [&key](auto const& v) {
+                uint8_t const* b = reinterpret_cast<uint8_t const*>(&v);
+                key.insert(key.end(), b, b + sizeof(v));
+            };
+            put(grid_path);
+            put(num_shadow);
+            put(scene.bvh.nodes);
+            put(scene.bvh.tris);
+            put(trace.counter);
+            put(trace.spill);
+            put(trace.lds_entries);
+            put(trace.kernel);
+            put(trace.block_size);
+            put(trace.blocks_per_cu);
+            put(trace.refill_lanes);
+            put(trace.claim_batch);
+            if (!state.regen_graph || key != state.regen_key) {
+                if (state.regen_graph) {
+                    (void)hipStreamSynchronize(s);
+                    (void)hipGraphExecDestroy(state.regen_graph);
+                    state.regen_graph = nullptr;
+                    state.regen_key.clear();
+                }
+                hipGraph_t graph = nullptr;
+                e                = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+                if (e) {
+                    return fail(MI_ERR_HIP, "mi_render: begin capture: %s", hipGetErrorString(e));
+                }
+                int32_t st = enqueue_bounce(0);
+                if (!st) {
+                    st = enqueue_bounce(1);
+                }
+                hipError_t const ce = hipStreamEndCapture(s, &graph);
+                if (st) {
+                    if (graph) {
+                        (void)hipGraphDestroy(graph);
+                    }
+                    return st;
+                }
+                if (ce) {
+                    return fail(MI_ERR_HIP, "mi_render: end capture: %s", hipGetErrorString(ce));
+                }
+                e = hipGraphInstantiate(&state.regen_graph, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (e) {
+                    state.regen_graph = nullptr;
+                    return fail(MI_ERR_HIP, "mi_render: graph instantiate: %s", hipGetErrorString(e));
+                }
+                state.regen_key = std::move(key);
+            }
+        }
+        for (uint32_t done_planes = 0; done_planes < set.samples_per_pixel; done_planes += batch_planes) {
+            hipLaunchKernelGGL(raygen, dim3(blocks(num_paths, 256)), dim3(256), 0, s, w);
+            uint64_t pairs = 0;
+            for (;;) {
+                if (use_graph) {
+                    e = hipGraphLaunch(state.regen_graph, s);
+                    if (e) {
+                        return fail(MI_ERR_HIP, "mi_render: graph launch: %s", hipGetErrorString(e));
+                    }
+                } else {
+                    for (uint32_t q = 0; q < 2; ++q) {
+                        if (int32_t const st = enqueue_bounce(q); st) {
+                            return st;
+                        }
+                    }
+                }
+                e = hipMemcpyAsync(state.live_host, &state.counters[C_queue0], sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, s);
+                if (!e) e = hipStreamSynchronize(s);
+                if (e) {
+                    return fail(MI_ERR_HIP, "mi_render: %s", hipGetErrorString(e));
+                }
+                if (0 == *state.live_host) {
+                    break;
+                }
+                if (++pairs > max_pairs) {
+                    return fail(MI_ERR_HIP, "mi_render: paths still live after %llu bounces",
+                                (unsigned long long)(2 * pairs));
+                }
+            }
+            hipLaunchKernelGGL(resolve_samples, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, w,
+                               batch_planes, spw, accum);
+            hipLaunchKernelGGL(advance_base, dim3(1), dim3(1), 0, s, state.counters, batch_planes);
+        }
+    } else if (set.flags & MI_RENDER_NO_GRAPH) {
         for (uint32_t wave = 0; wave < waves; ++wave) {
             if (int32_t const st = enqueue_wave(); st) {
                 return st;
@@ -2938,7 +3877,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
         }
         stats->seconds =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-        stats->paths       = double(num_paths) * waves;
+        stats->paths       = double(num_pixels) * set.samples_per_pixel;
         stats->extend_rays = double(totals[0]);
         stats->shadow_rays = double(totals[1]);
         stats->trained     = double(totals[2]);

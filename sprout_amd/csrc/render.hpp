@@ -49,6 +49,28 @@ struct Mesh_geometry {
     uint32_t const* indices;    // 3 per triangle
 };
 
+// Device buffers of update_positions: the builders' light inputs.
+struct Emitter_work {
+    float*    tri_lo    = nullptr;  // 3 per emissive triangle
+    float*    tri_hi    = nullptr;
+    float*    tri_cone  = nullptr;  // 4 per emissive triangle
+    float*    tri_power = nullptr;
+    float*    part_cone = nullptr;  // 4 per part
+    float*    part_lo   = nullptr;  // 3 per part
+    float*    part_hi   = nullptr;
+    float*    part_pow  = nullptr;
+    uint8_t*  part_two  = nullptr;
+    uint32_t* cdf_rel   = nullptr;  // per pool node, relative to its part
+    uint32_t* num_cdfs  = nullptr;  // per part
+    uint32_t* part_first = nullptr; // per part + 1 (device)
+    uint32_t* bad       = nullptr;  // an emissive triangle without area
+    float*    bounds    = nullptr;  // per block of the bounds reduction: 6 floats
+    uint32_t  capacity_tris  = 0;
+    uint32_t  capacity_parts = 0;
+
+    void release();
+};
+
 struct RenderScene {
     DeviceBvh        bvh;
     float*           positions    = nullptr;  // 3 floats per vertex
@@ -71,6 +93,14 @@ struct RenderScene {
     uint32_t*        part_mapping    = nullptr;
     uint32_t*        part_orders     = nullptr;
     Light_info*      part_lights     = nullptr;
+    // what moving the emitters needs: the emissive triangles in part order,
+    // each part's first (host), its radiance, and the pools' layout
+    uint32_t*                    emit_tris     = nullptr;
+    float*                       part_radiance = nullptr;
+    std::vector<uint32_t>        part_first;
+    std::vector<dev::Part_info>  part_info_host;
+    uint32_t                     num_emitters  = 0;
+    Emitter_work                 emitter_work;
     // participating medium in an axis-aligned box (none while density is null)
     float*           medium_density = nullptr;  // nx * ny * nz, x fastest
     uint32_t*        medium_nodes   = nullptr;  // majorant gridtree
@@ -87,6 +117,7 @@ struct RenderScene {
     uint32_t               emitter_instance = 0xFFFFFFFFu;
     bool                   instanced        = false;
     mi_bvh_settings        tlas_settings{16, 64, 4};
+    mi_bvh_settings        bvh_settings{16, 64, 4};  // of the triangles' tree
     float                  world_box_lo[3] = {0.f, 0.f, 0.f};  // bounds of the world triangles
     float                  world_box_hi[3] = {0.f, 0.f, 0.f};
 
@@ -101,6 +132,19 @@ struct RenderScene {
     void release();
     void release_medium();
 };
+
+struct Light_forest_scratch;
+
+// New positions for the vertices of a (non-instanced) scene, a device array
+// of num_vertices x 3 floats with the scene's vertex count: the emissive
+// triangles' lights, every part's Primitive_tree and the scene light tree
+// are rebuilt on the device (the same trees create_scene builds on the
+// host), then the scene's tree over its triangles (the GPU builder with the
+// scene's settings).  Fails, keeping the old positions, when an emissive triangle
+// would lose its area.  light_ms: the light part's host wall time.  No render
+// may be in flight.
+int32_t update_positions(RenderScene& s, float const* positions, uint32_t num_vertices,
+                         Light_forest_scratch& scratch, hipStream_t stream, double* light_ms);
 
 // An instanced scene: num_instances placements of the meshes (row-vector
 // 4x4 matrices, one material each, not emissive) plus the world-space
@@ -129,7 +173,7 @@ int32_t update_instances(RenderScene& scene, float const* world_to_object,
 // the scene had; density == nullptr removes it.
 int32_t set_medium(RenderScene& scene, float const* density, uint32_t const dims[3],
                    float const box_lo[3], float const box_hi[3], float const cc_a[3],
-                   float const cc_s[3], float g);
+                   float const cc_s[3], float g, uint32_t majorants = MI_MAJORANTS_GRIDTREE);
 
 // Buffers of one render, kept between calls and grown on demand.
 struct RenderState {
@@ -168,6 +212,18 @@ struct RenderState {
     uint32_t* rec_block       = nullptr;  // valid records per 256 slots, then their offsets
     float2*   pixel_samples   = nullptr;  // Golden_ratio pixel samples of a frame
     uint64_t  pixel_sample_capacity = 0;
+    float*    sample_rad      = nullptr;  // regenerating: radiance of every sample of a batch
+    uint64_t  sample_capacity = 0;
+    uint32_t* done            = nullptr;  // regenerating: slots of the paths that ended this bounce
+    uint32_t  done_capacity   = 0;
+    uint32_t* live_host       = nullptr;  // pinned: the next queue's length after a graph replay
+    hipGraphExec_t       regen_graph = nullptr;  // two regenerating bounces, replayed while its key holds
+    std::vector<uint8_t> regen_key;
+    float2*   path_spec       = nullptr;  // spectral: per path slot {fourth throughput, hero}
+    float*    nee_thr4        = nullptr;  // spectral: per queue entry
+    uint32_t  spectral_capacity = 0;
+    float*    shadow_c4       = nullptr;  // spectral: per shadow entry
+    uint32_t  spectral_shadow_capacity = 0;
     float*    shuf_x          = nullptr;  // the wave's records, shuffled for training
     float*    shuf_dir        = nullptr;
     float*    shuf_w          = nullptr;
@@ -177,6 +233,8 @@ struct RenderState {
     int32_t reserve(uint32_t num_paths, uint32_t num_shadow);
     int32_t reserve_guiding(uint32_t num_paths, uint32_t records);
     int32_t reserve_pixel_samples(uint64_t entries);
+    int32_t reserve_regeneration(uint64_t samples, uint32_t slots);
+    int32_t reserve_spectral(uint32_t num_paths, uint32_t num_shadow);
     void    release();
 };
 

@@ -77,6 +77,89 @@ def build_part_tree(box_min, box_max, cones, powers, part_cone):
             cdfs[:nc.value])
 
 
+def _device(a, dtype):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype)).cuda()
+
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def build_tree_device(box_min, box_max, cones, powers, two_sided):
+    """As build_tree, built on the device (mi_light_tree_build_device)."""
+    import torch
+
+    n = len(powers)
+    args = [_device(box_min, np.float32), _device(box_max, np.float32), _device(cones, np.float32),
+            _device(powers, np.float32), _device(two_sided, np.uint8)]
+    nodes = torch.zeros((max(2 * n - 1, 1), NODE_DTYPE.itemsize // 4), dtype=torch.int32,
+                        device="cuda")
+    middles = torch.zeros(max(2 * n - 1, 1), dtype=torch.int32, device="cuda")
+    mapping = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    orders = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    nn = ctypes.c_uint32(0)
+    check(_lib.load().mi_light_tree_build_device(*[_dptr(a) for a in args], n, _dptr(nodes),
+                                                 _dptr(middles), _dptr(mapping), _dptr(orders),
+                                                 ctypes.byref(nn), _stream()),
+          "mi_light_tree_build_device")
+    torch.cuda.synchronize()
+    k = nn.value
+    return (nodes.cpu().numpy().view(NODE_DTYPE).reshape(-1)[:k],
+            middles.cpu().numpy().view(np.uint32)[:k], mapping.cpu().numpy().view(np.uint32)[:n],
+            orders.cpu().numpy().view(np.uint32)[:n])
+
+
+def build_part_trees_device(box_min, box_max, cones, powers, first, part_cones):
+    """The Primitive_trees of parts first[r] .. first[r + 1] - 1 at once on the
+    device -> per part (nodes, middles, mapping, orders, cdf_offset, cdfs) as
+    build_part_tree gives them."""
+    import torch
+
+    n = len(powers)
+    fst = np.ascontiguousarray(first, np.uint32)
+    r = len(fst) - 1
+    args = [_device(box_min, np.float32), _device(box_max, np.float32), _device(cones, np.float32),
+            _device(powers, np.float32)]
+    pc = _device(part_cones, np.float32)
+    words = NODE_DTYPE.itemsize // 4
+    nodes = torch.zeros((2 * n, words), dtype=torch.int32, device="cuda")
+    middles = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
+    mapping = torch.zeros(n, dtype=torch.int32, device="cuda")
+    orders = torch.zeros(n, dtype=torch.int32, device="cuda")
+    offs = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
+    cdfs = torch.zeros(2 * n, dtype=torch.float32, device="cuda")
+    ncdf = torch.zeros(r, dtype=torch.int32, device="cuda")
+    nn = np.zeros(r, np.uint32)
+    check(_lib.load().mi_light_part_trees_build_device(
+        *[_dptr(a) for a in args], n, fst.ctypes.data, r, _dptr(pc), _dptr(nodes), _dptr(middles),
+        _dptr(mapping), _dptr(orders), _dptr(offs), _dptr(cdfs), nn.ctypes.data, _dptr(ncdf),
+        _stream()), "mi_light_part_trees_build_device")
+    torch.cuda.synchronize()
+    nodes = nodes.cpu().numpy().view(NODE_DTYPE).reshape(-1)
+    middles = middles.cpu().numpy().view(np.uint32)
+    mapping = mapping.cpu().numpy().view(np.uint32)
+    orders = orders.cpu().numpy().view(np.uint32)
+    offs = offs.cpu().numpy().view(np.uint32)
+    cdfs = cdfs.cpu().numpy()
+    ncdf = ncdf.cpu().numpy()
+    out = []
+    for k in range(r):
+        b, e = int(fst[k]), int(fst[k + 1])
+        nb, cb = 2 * b - k, 2 * b
+        m = int(nn[k])
+        out.append((nodes[nb:nb + m], middles[nb:nb + m], mapping[b:e], orders[b:e],
+                    offs[nb:nb + m], cdfs[cb:cb + int(ncdf[k])]))
+    return out
+
+
 def scene_tree(scene):
     """(nodes, middles, mapping, orders, lights) of a render.Scene."""
     lib = _lib.load()

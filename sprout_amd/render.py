@@ -42,6 +42,22 @@ class Scene:
                                   ctypes.byref(handle)), "mi_scene_create")
         self.handle = handle.value
         self.num_triangles = len(idx)
+        self.num_vertices = len(pos)
+
+    def update_positions(self, positions) -> float:
+        """mi_scene_update_positions: new vertex positions ((n, 3) float32,
+        numpy or a device tensor) with the lights, light trees and triangle
+        tree rebuilt on the device; returns the light part's milliseconds."""
+        p = positions if isinstance(positions, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(positions, np.float32))
+        p = p.to(device="cuda", dtype=torch.float32).contiguous()
+        if p.dim() != 2 or p.shape[1] != 3:
+            raise ValueError("positions must be (n, 3)")
+        ms = ctypes.c_double(0.0)
+        check(_lib.load().mi_scene_update_positions(self.handle, ctypes.c_void_p(p.data_ptr()),
+                                                    p.shape[0], ctypes.byref(ms), _stream()),
+              "mi_scene_update_positions")
+        return ms.value
 
     def release(self) -> None:
         if self.handle != _lib.MI_NULL:
@@ -139,25 +155,30 @@ def camera(width: int, height: int, eye=ROOM_CAMERA["eye"], target=ROOM_CAMERA["
     return perspective(width, height, pos, rot, fov_h)
 
 
-def _flags(graph: bool, golden_ratio: bool) -> int:
-    return (0 if graph else _lib.MI_RENDER_NO_GRAPH) | (_lib.MI_RENDER_GOLDEN_RATIO if golden_ratio else 0)
+def _flags(graph: bool, golden_ratio: bool, regenerate: bool = False,
+           spectral: bool = False) -> int:
+    return ((0 if graph else _lib.MI_RENDER_NO_GRAPH) | (_lib.MI_RENDER_GOLDEN_RATIO if golden_ratio else 0)
+            | (_lib.MI_RENDER_REGENERATE if regenerate else 0)
+            | (_lib.MI_RENDER_SPECTRAL if spectral else 0))
 
 
 def render(scene: Scene, cam: _lib.MiCamera, spp: int, samples_per_wave: int = 1,
            min_bounces: int = 4, max_bounces: int = 8, seed: int = 1, first_sample: int = 0,
            graph: bool = True, accum: torch.Tensor | None = None, stats: bool = False,
-           golden_ratio: bool = False):
+           golden_ratio: bool = False, regenerate: bool = False, spectral: bool = False):
     """Adds the radiance sums of samples first_sample .. first_sample+spp-1 into
     accum ((height, width, 3) float64 on the device, made zero if None) and
     returns (accum, stats or None).  golden_ratio: pixel samples from sprout's
-    Golden_ratio sampler (one frame per call, first_sample 0)."""
+    Golden_ratio sampler (one frame per call, first_sample 0).  regenerate:
+    ended paths hand their slots to the frame's next samples (the same image).
+    spectral: hero-wavelength transport over four wavelengths per path."""
     if accum is None:
         accum = torch.zeros((cam.height, cam.width, 3), dtype=torch.float64, device="cuda")
     if accum.dtype != torch.float64 or not accum.is_contiguous() or \
             tuple(accum.shape) != (cam.height, cam.width, 3):
         raise ValueError("accum must be a contiguous (height, width, 3) float64 device tensor")
     st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
-                               _flags(graph, golden_ratio), seed)
+                               _flags(graph, golden_ratio, regenerate, spectral), seed)
     out = _lib.MiRenderStats() if stats else None
     check(_lib.load().mi_render(scene.handle, ctypes.byref(cam), ctypes.byref(st),
                                 ctypes.c_void_p(accum.data_ptr()), _stream(),
@@ -203,7 +224,7 @@ def render_filtered(scene: Scene, cam: _lib.MiCamera, spp: int, sens: _lib.MiSen
                     samples_per_wave: int = 1, min_bounces: int = 4, max_bounces: int = 8,
                     seed: int = 1, first_sample: int = 0, graph: bool = True,
                     accum: torch.Tensor | None = None, stats: bool = False,
-                    golden_ratio: bool = False):
+                    golden_ratio: bool = False, spectral: bool = False):
     """mi_render_filtered: adds the filtered samples into accum ((height,
     width, 4) float64 on the device: weighted rgb and weight; made zero if
     None) and returns (accum, stats or None); the image is rgb / weight."""
@@ -213,7 +234,7 @@ def render_filtered(scene: Scene, cam: _lib.MiCamera, spp: int, sens: _lib.MiSen
             tuple(accum.shape) != (cam.height, cam.width, 4):
         raise ValueError("accum must be a contiguous (height, width, 4) float64 device tensor")
     st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
-                               _flags(graph, golden_ratio), seed)
+                               _flags(graph, golden_ratio, spectral=spectral), seed)
     out = _lib.MiRenderStats() if stats else None
     check(_lib.load().mi_render_filtered(scene.handle, ctypes.byref(cam), ctypes.byref(st),
                                          ctypes.byref(sens), ctypes.c_void_p(accum.data_ptr()),
@@ -227,20 +248,23 @@ def render_guided(scene: Scene, cam: _lib.MiCamera, spp: int, guide, fraction: f
                   learning_rate: float = 1e-3, record_depth: int = 4, samples_per_wave: int = 1,
                   min_bounces: int = 4, max_bounces: int = 8, seed: int = 1,
                   first_sample: int = 0, accum: torch.Tensor | None = None, stats: bool = False,
-                  graph: bool = True, guide_depth: int = 0, golden_ratio: bool = False):
+                  graph: bool = True, guide_depth: int = 0, golden_ratio: bool = False,
+                  guide_medium: bool = False, spectral: bool = False):
     """mi_render_guided: as render(), sampling directions from a one-sample
     mixture of the guide (probability fraction) and the BSDF at vertices of
     depth below guide_depth (0: all), and training the guide online after
-    every wave when train is set."""
+    every wave when train is set.  guide_medium: scattering vertices in the
+    scene's medium mix the guide with the phase function too."""
     if accum is None:
         accum = torch.zeros((cam.height, cam.width, 3), dtype=torch.float64, device="cuda")
     if accum.dtype != torch.float64 or not accum.is_contiguous() or \
             tuple(accum.shape) != (cam.height, cam.width, 3):
         raise ValueError("accum must be a contiguous (height, width, 3) float64 device tensor")
     st = _lib.MiRenderSettings(spp, samples_per_wave, first_sample, min_bounces, max_bounces,
-                               _flags(graph, golden_ratio), seed)
+                               _flags(graph, golden_ratio, spectral=spectral), seed)
     gs = _lib.MiGuideSettings(guide.handle, fraction, 1 if train else 0, train_steps, batch,
-                              learning_rate, record_depth, guide_depth)
+                              learning_rate, record_depth, guide_depth,
+                              1 if guide_medium else 0)
     out = _lib.MiRenderStats() if stats else None
     check(_lib.load().mi_render_guided(scene.handle, ctypes.byref(cam), ctypes.byref(st),
                                        ctypes.byref(gs), ctypes.c_void_p(accum.data_ptr()),

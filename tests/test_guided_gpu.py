@@ -74,9 +74,9 @@ def test_guided_converges_to_plain(gpu, fraction, train):
 
 def test_training_lowers_error_in_the_slit_room(gpu):
     """Equal sample count after training: guided per-pixel error below plain.
-    Records sit in fixed slots (path, depth) and are compacted in slot order,
-    so a training is bitwise reproducible; this one is compared over four
-    64 spp pairs against a 16384 spp reference."""
+    The guide samples the first two vertices (fraction 0.8), trained on
+    8192 spp of the 64 x 64 image; the comparison runs over eight 64 spp
+    pairs against a 16384 spp reference."""
     import torch
 
     from sprout_amd import scenes
@@ -89,21 +89,22 @@ def test_training_lowers_error_in_the_slit_room(gpu):
     ref = ref.cpu().numpy() / 16384
     e_plain, e_guided = [], []
     g = guide.Guide(5)
-    log("training the guide, 512 spp")
-    render.render_guided(sc, cam, 512, g, fraction=0.5, train=True, train_steps=8,
-                         learning_rate=2e-3, samples_per_wave=16, seed=32)
+    log("training the guide, 8192 spp")
+    render.render_guided(sc, cam, 8192, g, fraction=0.8, train=True, train_steps=4,
+                         learning_rate=3e-3, samples_per_wave=64, seed=32, guide_depth=2,
+                         record_depth=2)
     torch.cuda.synchronize()
-    log("equal-spp comparison, 4 x 64 spp")
-    for k in range(4):
+    log("equal-spp comparison, 8 x 64 spp")
+    for k in range(8):
         plain, _ = render.render(sc, cam, 64, samples_per_wave=64, seed=33 + 2 * k)
-        guided, _ = render.render_guided(sc, cam, 64, g, fraction=0.5, train=False,
-                                         samples_per_wave=64, seed=34 + 2 * k)
+        guided, _ = render.render_guided(sc, cam, 64, g, fraction=0.8, train=False,
+                                         samples_per_wave=64, seed=34 + 2 * k, guide_depth=2)
         e_plain.append(rel_mse(plain.cpu().numpy() / 64, ref))
         e_guided.append(rel_mse(guided.cpu().numpy() / 64, ref))
     g.release()
     ratio = float(np.mean(e_guided) / np.mean(e_plain))
     log(f"relMSE plain {np.mean(e_plain):.4g} guided {np.mean(e_guided):.4g} ratio {ratio:.3f}")
-    assert ratio < 0.97, (e_plain, e_guided)
+    assert ratio < 0.95, (e_plain, e_guided)
     sc.release()
 
 
@@ -223,3 +224,88 @@ def test_guided_graph_equals_launches(gpu):
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][2], out[1][2])
     sc.release()
+
+
+@pytest.fixture(scope="module")
+def medium_room(gpu):
+    """The box room holding a smooth random 40 x 48 x 56 cloud (density up to
+    3, albedo 0.75-0.55, Henyey-Greenstein g 0.5) around its middle."""
+    from scipy.ndimage import gaussian_filter
+
+    from sprout_amd import medium as M
+
+    render, _ = gpu
+    rng = np.random.default_rng(1)
+    grid = gaussian_filter(rng.random((40, 48, 56)), 2.0)
+    grid = (grid - grid.min()) / (grid.max() - grid.min())
+    grid[grid < 0.3] = 0.0
+    grid = (grid * 3.0).astype(np.float32)
+    sc = scene(gpu, 0.0)
+    M.set_medium(sc, grid, (-0.6, 0.3, -0.6), (0.6, 1.5, 0.6), (0.3, 0.4, 0.5), (1.2, 0.9, 0.6),
+                 0.5)
+    yield sc
+    sc.release()
+
+
+@pytest.mark.parametrize("fraction,train,roulette", [(0.5, False, False), (1.0, False, True),
+                                                    (0.8, True, True)])
+def test_guided_medium_vertices_converge_to_plain(gpu, medium_room, fraction, train, roulette):
+    """Scattering vertices in the medium sample the one-sample mixture of the
+    guide and the phase function (guide_medium): with the mixture pdf in the
+    path weight and in the MIS weights of their shadow rays the render,
+    trained or not, converges to the plain MIS image without Russian
+    roulette.  The plain render is the reference without roulette: sprout's
+    roulette divides by a continuation probability above one when the
+    throughput exceeds one, which a chromatic medium's tracking weights do
+    now and then; guided paths cap that probability at one."""
+    import torch
+
+    from sprout_amd import scenes
+
+    render, guide = gpu
+    g = guide.Guide(5)
+    cam = render.camera(32, 32, **scenes.ROOM_CAMERA)
+    log(f"guided medium vertices, fraction {fraction} train {train} roulette {roulette}: "
+        "4096 spp vs plain")
+    ref, _ = render.render(medium_room, cam, 4096, samples_per_wave=256, seed=31, min_bounces=8,
+                           max_bounces=8)
+    acc, st = render.render_guided(medium_room, cam, 4096, g, fraction=fraction, train=train,
+                                   train_steps=2, samples_per_wave=256, seed=32, stats=True,
+                                   guide_medium=True, min_bounces=4 if roulette else 8,
+                                   max_bounces=8)
+    torch.cuda.synchronize()
+    a = blocks(acc.cpu().numpy() / 4096)
+    b = blocks(ref.cpu().numpy() / 4096)
+    e = rel_mse(a, b)
+    log(f"  relMSE of 4x4 blocks {e:.3g}, means {a.mean():.5g} {b.mean():.5g}")
+    assert np.all(np.isfinite(a))
+    assert e < 2e-3, e
+    assert abs(a.mean() / b.mean() - 1) < 1e-2
+    if train:
+        assert st.trained > 0
+    g.release()
+
+
+def test_guide_medium_switch(gpu, medium_room):
+    """guide_medium changes the samples only where a path scatters in a
+    medium: on the room without a medium the switch gives the same image bit
+    for bit, with one it gives another image."""
+    import torch
+
+    from sprout_amd import scenes
+
+    render, guide = gpu
+    g = guide.Guide(6)
+    cam = render.camera(32, 32, **scenes.ROOM_CAMERA)
+    plain_room = scene(gpu, 0.0)
+    out = {}
+    for name, sc in (("room", plain_room), ("medium", medium_room)):
+        for on in (False, True):
+            img, _ = render.render_guided(sc, cam, 64, g, fraction=0.5, train=False,
+                                          samples_per_wave=16, seed=7, guide_medium=on)
+            torch.cuda.synchronize()
+            out[name, on] = img.cpu().numpy()
+    assert np.array_equal(out["room", False], out["room", True])
+    assert not np.array_equal(out["medium", False], out["medium", True])
+    plain_room.release()
+    g.release()

@@ -200,6 +200,9 @@ def test_batch_scene_same_seeds_as_oracle(gpu, batch):
     log(f"  pixels within 1e-4: {close:.4f}, relMSE {e:.3g}, means {img.mean():.5g} {ref.mean():.5g}")
     assert close > 0.95
     assert e < 1e-5
+    # regenerating waves over the instanced scene: the same image bit for bit
+    regen = batch_image(gpu, s, w, h, spp, samples_per_wave=4, seed=5, regenerate=True)
+    assert np.array_equal(img, regen)
 
 
 def test_batch_scene_equals_the_baked_soup(gpu, batch):

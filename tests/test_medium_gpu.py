@@ -96,6 +96,47 @@ def test_batch_tracking_matches_oracle(gpu, medium_scene, transmittance, nearest
         assert (ref[:, 0] == 1).mean() > 0.1
 
 
+@pytest.mark.parametrize("transmittance", [False, True])
+def test_flat_majorants_match_oracle(gpu, transmittance):
+    """The flat majorant grid (MI_MAJORANTS_FLAT) on a cloud of two
+    contrasts, one under the gridtree's split threshold: batch tracking
+    matches the oracle's over the same flat grid, and its transmittance
+    agrees in the mean with the gridtree's (both are unbiased)."""
+    from sprout_amd import medium as M
+    from sprout_amd import scenes
+
+    pos, idx, tmat, mats = scenes.room()
+    grid = random_grid((64, 64, 64), 2)
+    grid[:, :, :32] *= np.float32(0.03)  # a faint half
+    box = ((-0.6, 0.3, -0.6), (0.6, 1.5, 0.6))
+    cc_a, cc_s, g = (0.3, 0.4, 0.5), (1.2, 0.9, 0.6), 0.5
+    flat = gpu.Scene(pos, idx, tmat, mats)
+    tree = gpu.Scene(pos, idx, tmat, mats)
+    M.set_medium(flat, grid, *box, cc_a, cc_s, g, majorants="flat")
+    M.set_medium(tree, grid, *box, cc_a, cc_s, g)
+    with pytest.raises(ValueError):
+        M.set_medium(tree, grid, *box, cc_a, cc_s, g, majorants="octree")
+    om = O.Medium(grid, *box, cc_a, cc_s, g, majorants="flat")
+    rays = random_rays(20000, 21, *box)
+    log(f"flat majorants batch: transmittance {transmittance}")
+    dev = M.track(flat, rays, seed=4, transmittance=transmittance)
+    ref = om.batch(rays, seed=4, transmittance=transmittance)
+    same_event = dev[:, 0] == ref[:, 0]
+    assert same_event.mean() > 0.995, same_event.mean()
+    close = same_event & np.all(np.abs(dev[:, 1:] - ref[:, 1:]) <= 1e-4 * np.abs(ref[:, 1:]) + 1e-6,
+                                axis=1)
+    assert close.mean() > 0.99, close.mean()
+    if transmittance:
+        n = 200000
+        many = np.repeat(rays[:16], n // 16, axis=0)
+        a = M.track(flat, many, seed=5, transmittance=True)[:, 2:].reshape(16, -1, 3)
+        b = M.track(tree, many, seed=6, transmittance=True)[:, 2:].reshape(16, -1, 3)
+        sem = np.sqrt(a.var(axis=1) / a.shape[1] + b.var(axis=1) / b.shape[1])
+        assert np.all(np.abs(a.mean(axis=1) - b.mean(axis=1)) <= 5 * sem + 1e-4)
+    flat.release()
+    tree.release()
+
+
 def test_room_with_medium_same_seeds_as_oracle(gpu, medium_scene):
     import torch
 
@@ -136,8 +177,11 @@ def test_cloud_scene_graph_and_launches_agree(gpu):
         log("cloud 32^3: graph vs launches")
         a, _ = gpu.render(sc, cam, spp, samples_per_wave=4, seed=3)
         b, _ = gpu.render(sc, cam, spp, samples_per_wave=4, seed=3, graph=False)
+        c, _ = gpu.render(sc, cam, spp, samples_per_wave=4, seed=3, regenerate=True)
         torch.cuda.synchronize()
         assert torch.equal(a, b)
+        # regenerating waves: tracking reads each path's own depth
+        assert torch.equal(a, c)
         log("cloud 32^3: oracle, same seeds")
         img = a.cpu().numpy() / spp
         ref = O.render(pos, idx, tmat, mats, w, h, spp, scenes.CLOUD_CAMERA,

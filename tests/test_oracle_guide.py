@@ -109,6 +109,41 @@ def test_vmf_and_mixture_integrate_to_one():
     assert abs(q.sum() * da - 1) < 2e-3
 
 
+@pytest.mark.parametrize("g", [-0.3, 0.0, 0.6, 0.9])
+def test_medium_vertex_mixture_integrates_to_one(g):
+    """At a scattering vertex the guided pdf is fraction * q(w) + (1 -
+    fraction) * HG(wo.w, g) over the whole sphere; it integrates to one, and
+    the one-sample mixture sampler (guide with probability fraction, else the
+    phase function) draws directions with that density (binned in the
+    cosine to wo)."""
+    from oracle import oracle as O
+
+    w, da = sphere_grid(800)
+    rng = np.random.default_rng(2)
+    raw = rng.normal(size=(1, 32)) * 2
+    wo = np.array([0.48, -0.6, 0.64])
+    fraction = 0.7
+    q = G.mixture_pdf(np.repeat(raw, len(w), 0), w)
+    ph = np.array([O.phase_hg(c, g) for c in w @ wo])
+    assert abs(ph.sum() * da - 1) < 2e-3
+    pdf = fraction * q + (1 - fraction) * ph
+    assert abs(pdf.sum() * da - 1) < 2e-3
+
+    mu, kappa, wts = (a[0] for a in G.lobes(raw))
+    n = 20000
+    samples = np.zeros((n, 3))
+    for i in range(n):
+        if rng.random() < fraction:
+            lobe = min(np.searchsorted(np.cumsum(wts), rng.random()), len(wts) - 1)
+            samples[i] = G.sample_vmf(mu[lobe], kappa[lobe], rng.random(), rng.random())
+        else:
+            samples[i], _ = O.sample_phase(wo, rng.random(), rng.random(), g)
+    edges = np.linspace(-1, 1, 11)
+    got = np.histogram(samples @ wo, edges)[0] / n
+    want = np.array([pdf[(w @ wo >= a) & (w @ wo < b)].sum() * da for a, b in zip(edges, edges[1:])])
+    assert np.all(np.abs(got - want) < 5 * np.sqrt(want * (1 - want) / n) + 2e-3), (got, want)
+
+
 def test_vmf_sampler_mean_cosine():
     """E[mu.w] = coth(kappa) - 1/kappa for a vMF on the sphere."""
     rng = np.random.default_rng(4)

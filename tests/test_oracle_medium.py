@@ -145,6 +145,89 @@ def test_gridtree_library_matches_oracle_and_bounds_density(shape, seed):
             assert d * s_lo >= cm[1] * (1 - 1e-6) - 1e-7
 
 
+@pytest.mark.parametrize("shape,seed", [((64, 64, 64), 4), ((40, 33, 70), 3), ((17, 5, 100), 5)])
+def test_flat_majorants_library_matches_oracle_bound_and_are_tighter(shape, seed):
+    """The flat majorant grid (every 32^3-voxel cell split to 8^3-voxel
+    leaves): the library's and the oracle's builders agree bit for bit; every
+    cell at least 9 voxels wide (with its border) reaches the 8^3 leaves;
+    every leaf bounds the filtered density; and at every voxel the flat
+    majorant is at most the gridtree's (a leaf's voxels and border lie inside
+    its ancestors')."""
+    from sprout_amd import medium as M
+
+    grid = random_grid(shape, seed)
+    cc_a, cc_s = (0.2, 0.1, 0.3), (0.7, 0.9, 0.5)
+    flat = O.Medium(grid, (0, 0, 0), (1, 1, 1), cc_a, cc_s, 0.0, majorants="flat")
+    tree = O.Medium(grid, (0, 0, 0), (1, 1, 1), cc_a, cc_s, 0.0)
+    fnodes, fdata, fcells = flat.gridtree()
+    lnodes, ldata, lcells = M.gridtree(grid, cc_a, cc_s, majorants="flat")
+    assert tuple(fcells) == tuple(int(c) for c in lcells)
+    assert np.array_equal(fnodes, lnodes)
+    assert np.array_equal(fdata.view(np.uint32), ldata.view(np.uint32))
+    with pytest.raises(ValueError):
+        M.gridtree(grid, cc_a, cc_s, majorants="octree")
+
+    nz, ny, nx = grid.shape
+    dims = (nx, ny, nz)
+    fleaves = leaf_boxes(fnodes, fdata, fcells, dims)
+    for lo, hi, _ in fleaves:
+        span = [min(hi[k], dims[k]) - lo[k] for k in range(3)]
+        if min(span) > 0 and all(min(dims[k] - lo[k], 32) >= 9 for k in range(3)):
+            assert max(hi[k] - lo[k] for k in range(3)) == 8, (lo, hi)
+
+    def majorant_map(leaves):
+        m = np.zeros((nz, ny, nx, 2), np.float64)
+        for lo, hi, cm in leaves:
+            if cm is not None:
+                m[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] = cm[2:4]
+        return m
+
+    mf = majorant_map(fleaves)
+    mt = majorant_map(leaf_boxes(*tree.gridtree()[:2], tree.gridtree()[2], dims))
+    assert np.all(mf <= mt)
+    # each voxel's own density is under its flat majorant
+    for c in range(2):
+        hi_cc = max(cc_a) if c == 0 else max(cc_s)
+        assert np.all(grid * hi_cc <= mf[..., c] * (1 + 1e-6) + 1e-7)
+    if min(shape) >= 16:
+        # a low-contrast cloud (density range under the gridtree's 0.1
+        # split threshold) keeps the gridtree at its 32^3 cells; the flat
+        # grid is strictly tighter there
+        faint = grid * np.float32(0.03)
+        f = O.Medium(faint, (0, 0, 0), (1, 1, 1), cc_a, cc_s, 0.0, majorants="flat")
+        t = O.Medium(faint, (0, 0, 0), (1, 1, 1), cc_a, cc_s, 0.0)
+        assert not (t.gridtree()[0] & 1).any()
+        mf = majorant_map(leaf_boxes(*f.gridtree()[:2], f.gridtree()[2], dims))
+        mt = majorant_map(leaf_boxes(*t.gridtree()[:2], t.gridtree()[2], dims))
+        assert np.all(mf <= mt) and mf.sum() < 0.95 * mt.sum()
+
+
+@pytest.mark.parametrize("majorants", ["gridtree", "flat"])
+def test_ratio_tracking_is_unbiased_in_a_cloud(majorants):
+    """A smooth random 64^3 cloud: the mean ratio-tracking estimate along a
+    diagonal ray equals exp(-integral of mu_t) (numpy quadrature of the
+    trilinear density) with either kind of majorants."""
+    grid = random_grid((64, 64, 64), 8)
+    cc_a, cc_s = (0.5, 0.8, 0.3), (2.0, 1.5, 2.5)
+    m = O.Medium(grid, (0, 0, 0), (1, 1, 1), cc_a, cc_s, 0.0, majorants=majorants)
+    n = 20000
+    o = np.array([-0.5, 0.1, 0.2])
+    d = np.array([1.0, 0.7, 0.55])
+    d /= np.linalg.norm(d)
+    out = m.batch(rays_through(n, o, d), transmittance=True, seed=11)
+    # the ray's span inside the unit box
+    t_in = max((0 - o[k]) / d[k] for k in range(3))
+    t_out = min((1 - o[k]) / d[k] for k in range(3))
+    ts = np.linspace(t_in, t_out, 8001)
+    mid = 0.5 * (ts[1:] + ts[:-1])
+    dens = np.array([lin_density(grid, o + t * d) for t in mid])
+    tau = dens.sum() * (ts[1] - ts[0])
+    expect = np.exp(-tau * (np.array(cc_a) + np.array(cc_s)))
+    mean = out[:, 2:].mean(axis=0)
+    sem = out[:, 2:].std(axis=0) / np.sqrt(n)
+    assert np.all(np.abs(mean - expect) < 5 * sem + 2e-3 * expect), (mean, expect, sem)
+
+
 def rays_through(n, o, d, max_t=1e30):
     r = np.zeros((n, 8), np.float32)
     r[:, 0:3] = o

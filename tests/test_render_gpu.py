@@ -153,6 +153,36 @@ def test_graph_and_launches_agree_bitwise(gpu, room):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("spw,graph,golden", [(1, True, False), (4, True, False), (2, False, False),
+                                              (4, True, True)])
+def test_regenerating_waves_give_the_same_image(gpu, room, spw, graph, golden):
+    """Regenerating waves (a slot whose path ends starts the frame's next
+    sample) render the wave-by-wave image bit for bit: a sample's random
+    numbers depend on the sample alone and the samples are summed per pixel
+    in the same order."""
+    sc, _ = room
+    kw = dict(samples_per_wave=spw, seed=7, golden_ratio=golden, max_bounces=6, min_bounces=2)
+    a = gpu_image(gpu, sc, 48, 40, 16, graph=graph, **kw)
+    b = gpu_image(gpu, sc, 48, 40, 16, graph=graph, regenerate=True, **kw)
+    assert np.all(np.isfinite(b)) and a.mean() > 0
+    assert np.array_equal(a, b)
+
+
+def test_regenerating_waves_count_every_sample(gpu, room):
+    """Statistics of a regenerating frame: every sample started once."""
+    import torch
+
+    from sprout_amd import scenes
+
+    sc, _ = room
+    cam = gpu.camera(33, 17, **scenes.ROOM_CAMERA)
+    _, st = gpu.render(sc, cam, 12, samples_per_wave=3, seed=2, stats=True, regenerate=True)
+    _, sw = gpu.render(sc, cam, 12, samples_per_wave=3, seed=2, stats=True)
+    torch.cuda.synchronize()
+    assert st.paths == 33 * 17 * 12
+    assert st.extend_rays == sw.extend_rays and st.shadow_rays == sw.shadow_rays
+
+
 def test_progressive_render_adds_up(gpu, room):
     import torch
 
