@@ -346,6 +346,28 @@ def batch_render(frames: int, instanced: bool):
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t1)
         extra["top_level_rebuild_ms"] = round(float(np.median(times)) * 1e3, 3)
+    else:
+        # an animated frame of the soup: the 10 000 emitters moved (each part
+        # translated), lights, light trees and triangle tree rebuilt on the
+        # device (mi_scene_update_positions, device positions in)
+        rng = np.random.default_rng(5)
+        kinds = mats["kind"][tmat]
+        em = kinds == 2
+        verts = idx[em].reshape(-1)
+        owner = np.repeat(tmat[em], 3)
+        v, first = np.unique(verts, return_index=True)
+        times, light = [], []
+        for k in range(1, 4):
+            moved = pos.copy()
+            moved[v] += rng.uniform(-0.05, 0.05, (len(mats), 3)).astype(np.float32)[owner[first]]
+            dev = torch.from_numpy(moved).cuda()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            light.append(sc.update_positions(dev))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t1)
+        extra["emitter_update_ms"] = round(float(np.median(times)) * 1e3, 2)
+        extra["emitter_update_light_trees_ms"] = round(float(np.median(light)), 2)
     sc.release()
     form = "instanced: 4 mesh trees under a top-level tree" if instanced else "baked, one tree"
     return {
@@ -567,6 +589,10 @@ def main() -> int:
                     help="run one workload only and print its figures (for profiling)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the traversal benchmark's line, without the other workloads' figures")
+    ap.add_argument("--shape", default="",
+                    help="traversal launch shape kernel,lds_entries,refill_lanes,claim_batch,"
+                         "blocks_per_cu instead of autotuning (for profiles: the timed launches "
+                         "then follow the one camera-ray launch)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="launch one 256 MiB device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -630,9 +656,16 @@ def main() -> int:
     d_bounce = torch.from_numpy(bounce).cuda()
     n = len(bounce)
 
-    log("autotuning the traversal's launch shape")
-    shape, tuned_ms = bvh.autotune(tree, d_bounce, 5)
-    log(f"  {shape}: {tuned_ms:.3f} ms")
+    if args.shape:
+        kernel, lds, lanes, batch, per_cu = (int(x) for x in args.shape.split(","))
+        bvh.set_trace_config(256, per_cu, kernel, lanes, batch, lds)
+        shape = {"block_size": 256, "blocks_per_cu": per_cu, "kernel": kernel,
+                 "refill_lanes": lanes, "claim_batch": batch, "lds_entries": lds}
+        log(f"launch shape {shape}")
+    else:
+        log("autotuning the traversal's launch shape")
+        shape, tuned_ms = bvh.autotune(tree, d_bounce, 5)
+        log(f"  {shape}: {tuned_ms:.3f} ms")
 
     hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
     occ = torch.empty((n,), dtype=torch.int32, device="cuda")

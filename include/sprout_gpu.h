@@ -172,7 +172,12 @@ typedef struct mi_trace_config {
                                   2: refill with inner/leaf phases (while-while),
                                   3: as 2 with stack entries that carry the child's node word
                                   and box entry distance (default; trees of < 2^22 nodes and
-                                  slots, else 2) */
+                                  slots, else 2),
+                                  4: as 3 over a compressed copy of the tree: 32 bytes per
+                                  sibling pair (16-bit grid boxes that hold the originals, the
+                                  node words), half the bytes of the 64-byte pair; the same
+                                  nodes visited in the same order, plus any the grown boxes
+                                  admit, so the same hits */
     uint32_t refill_lanes;     /* kernels 0, 2, 3: hand out rays once this many lanes are idle (8) */
     uint32_t claim_batch;      /* kernels 0, 2, 3: rays a wave claims per atomic (64) */
     uint32_t lds_entries;      /* stack entries per lane kept in LDS, <= 32 (default 10;
@@ -182,9 +187,9 @@ typedef struct mi_trace_config {
 int32_t mi_set_trace_config(mi_trace_config const* config);
 
 /* Autotuner of the traversal's launch shape: times closest-hit traversals of
- * the given DEVICE rays through the tree over candidate shapes (kernel 3;
- * LDS stack entries, refill threshold, claim batch and workgroups per CU
- * around the defaults), `repeats` timed launches each after one warm-up,
+ * the given DEVICE rays through the tree over candidate shapes (kernels 3
+ * and 4; LDS stack entries, refill threshold, claim batch and workgroups per
+ * CU around the defaults), `repeats` timed launches each after one warm-up,
  * with HIP events on `stream`; makes the fastest the context's shape (as
  * mi_set_trace_config) and returns it in *best (HOST, may be NULL) and its
  * mean launch time in *best_ms.  Every shape gives the same results, so

@@ -47,6 +47,8 @@ static_assert(sizeof(HostNode) == 32);
 }  // namespace
 
 void DeviceBvh::release() {
+    (void)hipFree(qpairs);
+    qpairs = nullptr;
     (void)hipFree(nodes_alloc);
     (void)hipFree(tris);
     (void)hipFree(slots);

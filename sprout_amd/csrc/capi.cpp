@@ -377,7 +377,7 @@ int32_t mi_set_trace_config(mi_trace_config const* config) {
     uint32_t const bpc   = config->blocks_per_cu ? config->blocks_per_cu : defaults.blocks_per_cu;
     uint32_t const lanes = config->refill_lanes ? config->refill_lanes : defaults.refill_lanes;
     uint32_t const batch = config->claim_batch ? config->claim_batch : defaults.claim_batch;
-    if (block % 64 || block > 1024 || bpc > 64 || lanes > 64 || config->kernel > 3 ||
+    if (block % 64 || block > 1024 || bpc > 64 || lanes > 64 || config->kernel > 4 ||
         batch > 65536 || config->lds_entries > 32) {
         return fail(MI_ERR_ARGS,
                     "mi_set_trace_config: block %u, blocks per CU %u, kernel %u, refill lanes %u, "
@@ -435,12 +435,14 @@ int32_t mi_trace_autotune(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, u
                           context->trace.claim_batch, context->trace.blocks_per_cu};
     uint32_t const old_kernel = context->trace.kernel;
     uint32_t const old_block  = context->trace.block_size;
-    float          best_t     = 0.f;
-    Shape          best_shape = shapes[0];
-    int32_t        st         = MI_OK;
+    float          best_t      = 0.f;
+    Shape          best_shape  = shapes[0];
+    uint32_t       best_kernel = 3;
+    int32_t        st          = MI_OK;
+    for (uint32_t const kernel : {3u, 4u})
     for (Shape const& sh : shapes) {
         TraceResources& t = context->trace;
-        set_shape(t, sh, 3, 256);
+        set_shape(t, sh, kernel, 256);
         st = launch_trace(*b, t, rays, num_rays, hits, nullptr, false, s);  // warm-up
         if (!st) e = hipEventRecord(e0, s);
         for (uint32_t k = 0; k < repeats && !st && !e; ++k) {
@@ -455,8 +457,9 @@ int32_t mi_trace_autotune(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, u
         }
         ms /= float(repeats);
         if (0.f == best_t || ms < best_t) {
-            best_t     = ms;
-            best_shape = sh;
+            best_t      = ms;
+            best_shape  = sh;
+            best_kernel = kernel;
         }
     }
     (void)hipEventDestroy(e0);
@@ -468,7 +471,7 @@ int32_t mi_trace_autotune(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, u
         return st ? st : fail(MI_ERR_HIP, "mi_trace_autotune: %s", hipGetErrorString(e));
     }
     TraceResources& t = context->trace;
-    set_shape(t, best_shape, 3, 256);
+    set_shape(t, best_shape, best_kernel, 256);
     if (best) {
         *best = mi_trace_config{t.block_size, t.blocks_per_cu, t.kernel, t.refill_lanes,
                                 t.claim_batch, t.lds_entries};

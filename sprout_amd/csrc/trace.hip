@@ -90,6 +90,9 @@ struct Stack {
 
 struct Launch {
     float4 const* nodes;
+    uint4 const*  qpairs;     // kernel 4: compressed sibling pairs
+    float         qorigin[3];
+    float         qscale[3];
     float4 const* tris;
     mi_ray const* rays;
     uint32_t      num_rays;
@@ -631,9 +634,57 @@ __device__ __forceinline__ bool inner_step2(float4 const* __restrict__ nodes, St
     return pop_next2(stack, ray, data, meta);
 }
 
+// A sibling pair from its 32-byte compressed record: each box coordinate is
+// origin + q * scale on a 16-bit grid over the root box, the lower corners
+// rounded down and the upper ones up when the record was made, so each box
+// holds the original; the words are pack_node's.
+__device__ __forceinline__ float dequant(uint32_t q, float scale, float origin) {
+    return __builtin_fmaf(float(q), scale, origin);
+}
+
+__device__ __forceinline__ bool inner_step_q(Launch const& p, Stack2& stack, Ray const& ray,
+                                             uint32_t& data, uint32_t& meta) {
+    uint32_t const a   = data;
+    uint4 const*   rec = p.qpairs + 2 * size_t((a - 1) >> 1);
+    uint4 const    r0  = rec[0];  // lo_a.xyz hi_a.xyz lo_b.xy as 16-bit pairs
+    uint4 const    r1  = rec[1];  // lo_b.z hi_b.xyz, word_a, word_b
+    float const    ox = p.qorigin[0], oy = p.qorigin[1], oz = p.qorigin[2];
+    float const    sx = p.qscale[0], sy = p.qscale[1], sz = p.qscale[2];
+    float4 const alo = make_float4(dequant(r0.x & 0xFFFFu, sx, ox), dequant(r0.x >> 16, sy, oy),
+                                   dequant(r0.y & 0xFFFFu, sz, oz), 0.f);
+    float4 const ahi = make_float4(dequant(r0.y >> 16, sx, ox), dequant(r0.z & 0xFFFFu, sy, oy),
+                                   dequant(r0.z >> 16, sz, oz), 0.f);
+    float4 const blo = make_float4(dequant(r0.w & 0xFFFFu, sx, ox), dequant(r0.w >> 16, sy, oy),
+                                   dequant(r1.x & 0xFFFFu, sz, oz), 0.f);
+    float4 const bhi = make_float4(dequant(r1.x >> 16, sx, ox), dequant(r1.y & 0xFFFFu, sy, oy),
+                                   dequant(r1.y >> 16, sz, oz), 0.f);
+
+    float      ta, tb;
+    bool const ha = dev::slab_entry(alo, ahi, ray.o, ray.inv, ray.min_t, ray.max_t, ta);
+    bool const hb = dev::slab_entry(blo, bhi, ray.o, ray.inv, ray.min_t, ray.max_t, tb);
+
+    bool const flip = (ray.sign_bits >> node_axis(meta)) & 1u;
+    bool const hn   = flip ? hb : ha;
+    bool const hf   = flip ? ha : hb;
+
+    if (hn | hf) {
+        if (hn & hf) {
+            stack.push(flip ? r1.z : r1.w, flip ? ta : tb);
+        }
+        bool const take_b = hn ? flip : !flip;
+        unpack_node(take_b ? r1.w : r1.z, data, meta);
+        return true;
+    }
+    return pop_next2(stack, ray, data, meta);
+}
+
 // trace_refill<Any, true> with the two-word stack (trees of < 2^22 nodes and
-// slots); the same rays in the same order, the same results.
-template <bool Any>
+// slots); the same rays in the same order, the same results.  Quant: inner
+// nodes from the compressed pairs (kernel 4); a box of a pair holds the
+// original, so every node the exact boxes admit is visited, in the same
+// near / far order (the axis and the ray's sign decide it), and the
+// triangles are tested in the same relative order.
+template <bool Any, bool Quant>
 __global__ void trace_refill_packed(Launch p) {
     extern __shared__ uint32_t lds_stack[];
 
@@ -716,7 +767,9 @@ __global__ void trace_refill_packed(Launch p) {
 
         while (__ballot(active && 0 == node_num(meta))) {
             if (active && 0 == node_num(meta)) {
-                if (!inner_step2(nodes, stack, ray, data, meta)) {
+                bool const more = Quant ? inner_step_q(p, stack, ray, data, meta)
+                                        : inner_step2(nodes, stack, ray, data, meta);
+                if (!more) {
                     write_result(p, r, Any, false, ray.max_t, index, u, v);
                     active = false;
                 }
@@ -742,6 +795,70 @@ __global__ void trace_refill_packed(Launch p) {
     }
 }
 
+// One compressed sibling pair per thread (children 2j + 1, 2j + 2): 16-bit
+// grid coordinates, lower corners the largest grid value at or below the
+// coordinate, upper ones the smallest at or above (checked with the
+// traversal's own dequantization).
+__device__ uint32_t quant_lo(float x, float scale, float origin) {
+    float    g = floorf((x - origin) / scale);
+    uint32_t q = g <= 0.f ? 0u : (g >= 65535.f ? 65535u : uint32_t(g));
+    while (q > 0 && dequant(q, scale, origin) > x) {
+        --q;
+    }
+    return q;
+}
+
+__device__ uint32_t quant_hi(float x, float scale, float origin) {
+    float    g = ceilf((x - origin) / scale);
+    uint32_t q = g <= 0.f ? 0u : (g >= 65535.f ? 65535u : uint32_t(g));
+    while (q < 65535u && dequant(q, scale, origin) < x) {
+        ++q;
+    }
+    return q;
+}
+
+// A pair record serves the inner node whose children start at 2j + 1; a
+// tree whose children start at an even index cannot use them (*odd_only
+// then gets 0).
+__global__ void quantize_pairs(float4 const* nodes, uint32_t num_pairs, float ox, float oy,
+                               float oz, float sx, float sy, float sz, uint4* out,
+                               uint32_t* odd_only) {
+    uint32_t const j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (0 == j) {
+        float4 const rhi = nodes[1];
+        if (0 == node_num(__float_as_uint(rhi.w)) && 0 == (__float_as_uint(nodes[0].w) & 1u)) {
+            atomicAnd(odd_only, 0u);
+        }
+    }
+    if (j >= num_pairs) {
+        return;
+    }
+    size_t const a   = 2 * size_t(j) + 1;
+    float4 const alo = nodes[2 * a + 0];
+    float4 const ahi = nodes[2 * a + 1];
+    float4 const blo = nodes[2 * a + 2];
+    float4 const bhi = nodes[2 * a + 3];
+    if ((0 == node_num(__float_as_uint(ahi.w)) && 0 == (__float_as_uint(alo.w) & 1u)) ||
+        (0 == node_num(__float_as_uint(bhi.w)) && 0 == (__float_as_uint(blo.w) & 1u))) {
+        atomicAnd(odd_only, 0u);
+    }
+    uint32_t const q[12] = {quant_lo(alo.x, sx, ox), quant_lo(alo.y, sy, oy), quant_lo(alo.z, sz, oz),
+                            quant_hi(ahi.x, sx, ox), quant_hi(ahi.y, sy, oy), quant_hi(ahi.z, sz, oz),
+                            quant_lo(blo.x, sx, ox), quant_lo(blo.y, sy, oy), quant_lo(blo.z, sz, oz),
+                            quant_hi(bhi.x, sx, ox), quant_hi(bhi.y, sy, oy), quant_hi(bhi.z, sz, oz)};
+    uint4 r0, r1;
+    r0.x = q[0] | (q[1] << 16);
+    r0.y = q[2] | (q[3] << 16);
+    r0.z = q[4] | (q[5] << 16);
+    r0.w = q[6] | (q[7] << 16);
+    r1.x = q[8] | (q[9] << 16);
+    r1.y = q[10] | (q[11] << 16);
+    r1.z = pack_node(__float_as_uint(alo.w), __float_as_uint(ahi.w));
+    r1.w = pack_node(__float_as_uint(blo.w), __float_as_uint(bhi.w));
+    out[2 * size_t(j) + 0] = r0;
+    out[2 * size_t(j) + 1] = r1;
+}
+
 __global__ void reset_counter(uint32_t* counter) {
     if (threadIdx.x < Claim_parts) {
         counter[threadIdx.x * Claim_stride] = 0;
@@ -757,6 +874,57 @@ bool packable(DeviceBvh const& bvh) {
     return bvh.num_nodes < (1u << Packed_data_bits) && bvh.num_slots < (1u << Packed_data_bits);
 }
 
+// The compressed pairs of kernel 4 (made once per tree): a grid of 65 535
+// steps per axis over the root box, its scale grown until the last step
+// reaches the box's upper corner.
+int32_t ensure_quantized(DeviceBvh const& bvh, hipStream_t stream) {
+    if (bvh.qpairs || bvh.qfailed || bvh.num_nodes < 3) {
+        return MI_OK;
+    }
+    uint32_t const num_pairs = (bvh.num_nodes - 1) / 2;
+    for (int k = 0; k < 3; ++k) {
+        float const lo = bvh.aabb_min[k];
+        float const hi = bvh.aabb_max[k];
+        float       sc = (hi - lo) / 65535.f;
+        if (!(sc > 0.f)) {
+            sc = 1.17549435e-38f;
+        }
+        while (std::fma(65535.f, sc, lo) < hi) {
+            sc = std::nextafter(sc, 3.40282347e+38f);
+        }
+        bvh.qorigin[k] = lo;
+        bvh.qscale[k]  = sc;
+    }
+    uint4*     q    = nullptr;
+    uint32_t*  flag = nullptr;
+    hipError_t e    = hipMalloc(&q, size_t(num_pairs) * 2 * sizeof(uint4));
+    if (!e) e = hipMalloc(&flag, sizeof(uint32_t));
+    uint32_t odd = 1;
+    if (!e) e = hipMemcpyAsync(flag, &odd, sizeof(uint32_t), hipMemcpyHostToDevice, stream);
+    if (e) {
+        (void)hipFree(q);
+        (void)hipFree(flag);
+        return fail(MI_ERR_HIP, "compressed tree of %u pairs: %s", num_pairs, hipGetErrorString(e));
+    }
+    hipLaunchKernelGGL(quantize_pairs, dim3((num_pairs + 255) / 256), dim3(256), 0, stream,
+                       bvh.nodes, num_pairs, bvh.qorigin[0], bvh.qorigin[1], bvh.qorigin[2],
+                       bvh.qscale[0], bvh.qscale[1], bvh.qscale[2], q, flag);
+    e = hipMemcpyAsync(&odd, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+    if (!e) e = hipStreamSynchronize(stream);
+    (void)hipFree(flag);
+    if (e) {
+        (void)hipFree(q);
+        return fail(MI_ERR_HIP, "compressed tree: %s", hipGetErrorString(e));
+    }
+    if (!odd) {
+        (void)hipFree(q);
+        bvh.qfailed = true;  // kernel 3 serves this tree
+        return MI_OK;
+    }
+    bvh.qpairs = q;
+    return MI_OK;
+}
+
 // Stack entries per lane in LDS (at most what a workgroup's 160 KB hold) and
 // in the global spill area, and words per entry.
 struct Stack_layout {
@@ -768,7 +936,7 @@ struct Stack_layout {
 Stack_layout stack_layout(DeviceBvh const& bvh, TraceResources const& res, bool counts) {
     uint32_t const entries = stack_entries(bvh.max_depth);
     Stack_layout   l;
-    l.words                = (!counts && 3 == res.kernel && packable(bvh)) ? 2 : 1;
+    l.words                = (!counts && 3 <= res.kernel && packable(bvh)) ? 2 : 1;
     uint32_t const lds_cap = res.lds_entries ? res.lds_entries : Max_lds_entries;
     uint32_t const fit     = (160u * 1024u) / (l.words * res.block_size * 4u);
     uint32_t const cap     = lds_cap < fit ? lds_cap : fit;
@@ -837,8 +1005,21 @@ int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, ui
 
     size_t const lds_bytes = size_t(sl.words) * lds_entries * block * sizeof(uint32_t);
 
+    // (a launch that must not allocate uses the compressed pairs only once made)
+    if (packed && 4 == res.kernel && bvh.num_nodes >= 3 && !fixed) {
+        if (int32_t const st = ensure_quantized(bvh, stream); st) {
+            return st;
+        }
+    }
+    bool const quant = packed && 4 == res.kernel && nullptr != bvh.qpairs;
+
     Launch p;
     p.nodes         = bvh.nodes;
+    p.qpairs        = bvh.qpairs;
+    for (int k = 0; k < 3; ++k) {
+        p.qorigin[k] = bvh.qorigin[k];
+        p.qscale[k]  = bvh.qscale[k];
+    }
     p.tris          = bvh.tris;
     p.rays          = rays;
     p.num_rays      = num_rays;
@@ -860,11 +1041,10 @@ int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, ui
         }
     } else if (packed) {
         hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, stream, counter);
-        if (any_hit) {
-            hipLaunchKernelGGL(trace_refill_packed<true>, dim3(blocks), dim3(block), lds_bytes, stream, p);
-        } else {
-            hipLaunchKernelGGL(trace_refill_packed<false>, dim3(blocks), dim3(block), lds_bytes, stream, p);
-        }
+        using Kernel = void (*)(Launch);
+        Kernel const kern = any_hit ? (quant ? trace_refill_packed<true, true> : trace_refill_packed<true, false>)
+                                    : (quant ? trace_refill_packed<false, true> : trace_refill_packed<false, false>);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(block), lds_bytes, stream, p);
     } else if (res.kernel != 1 || num_rays_dev) {
         hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, stream, counter);
         using Kernel    = void (*)(Launch);
@@ -895,6 +1075,12 @@ int32_t prepare_trace(DeviceBvh const& bvh, TraceResources& res, uint32_t max_ra
     }
     uint32_t const     threads = res.grid_blocks(max_rays) * res.block_size;
     Stack_layout const sl      = stack_layout(bvh, res, false);
+    if (4 == res.kernel && 2 == sl.words) {
+        // made before any capture: a launch inside a graph allocates nothing
+        if (int32_t const st = ensure_quantized(bvh, nullptr); st) {
+            return st;
+        }
+    }
     return res.reserve_spill(size_t(sl.words) * sl.spill_entries * threads);
 }
 

@@ -26,6 +26,15 @@ struct DeviceBvh {
     float  aabb_max[3] = {0.f, 0.f, 0.f};
     double build_ms    = 0.0;
 
+    // Compressed sibling pairs of kernel 4, made on first use: per pair of
+    // children (a, a + 1) 32 bytes, the two boxes as 16-bit grid coordinates
+    // (lower corners rounded down, upper ones up, so every box holds the
+    // original) and the two packed node words; the grid spans the root box.
+    mutable uint4* qpairs     = nullptr;
+    mutable float  qorigin[3] = {0.f, 0.f, 0.f};
+    mutable float  qscale[3]  = {0.f, 0.f, 0.f};
+    mutable bool   qfailed    = false;  // a child pair starts at an even index: no compressed copy
+
     void release();
 };
 
@@ -41,7 +50,8 @@ struct TraceResources {
     uint32_t  blocks_per_cu = 8;
     uint32_t  num_cus       = 256;
     uint32_t  kernel        = 3;     // 0 refill, 1 static grid-stride, 2 refill + while-while,
-                                     // 3 as 2 with two-word stack entries (small enough trees)
+                                     // 3 as 2 with two-word stack entries (small enough trees),
+                                     // 4 as 3 over compressed 32-byte sibling pairs
     uint32_t  refill_lanes  = 8;     // hand out rays once this many lanes of a wave are idle
     uint32_t  claim_batch   = 64;    // rays a wave claims from the global counter at a time
     uint32_t  lds_entries   = 10;    // stack entries kept in LDS (0: up to 32); deeper ones spill

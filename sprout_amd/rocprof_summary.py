@@ -29,7 +29,7 @@ import glob
 import os
 import statistics
 
-KERNEL = "trace_refill_packed<false>"
+KERNEL = "trace_refill_packed<false, false>"  # closest hit, kernel 3 (kernel 4: <false, true>)
 COPY_BYTES = 256 << 20
 
 
@@ -56,7 +56,10 @@ def _copy_kib(rows, counter):
     return max(vals)
 
 
-def summarize(kt, fetch, write, l2, steps, warmup):
+def summarize(kt, fetch, write, l2, steps, warmup, kernel=KERNEL):
+    """The timed launches follow one camera-ray launch of the same kernel
+    (bench.py with a fixed --shape)."""
+    KERNEL = kernel
     timed = slice(1, 1 + warmup + steps)
 
     trace = [r for r in _rows(kt, "kernel_trace.csv") if KERNEL in r["Kernel_Name"]]
@@ -266,8 +269,9 @@ def main():
     ap.add_argument("--out", default="")
     ap.add_argument("--command", default="")
     ap.add_argument("--title", default="closest-hit traversal, BVH micro-benchmark")
+    ap.add_argument("--kernel", default=KERNEL)
     args = ap.parse_args()
-    s = summarize(args.kt, args.fetch, args.write, args.l2, args.steps, args.warmup)
+    s = summarize(args.kt, args.fetch, args.write, args.l2, args.steps, args.warmup, args.kernel)
     s["num_rays"] = args.rays
     s["num_nodes"] = args.nodes
     figures = "\n".join(f"{k}: {v:.6g}" if isinstance(v, float) else f"{k}: {v}"

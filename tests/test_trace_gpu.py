@@ -127,7 +127,8 @@ def test_medium_mesh_closest_and_any(gpu, medium):
     (0, 1024, 1, 32, 1000, 8), (1, 64, 1, 0, 0, 32), (1, 256, 8, 0, 0, 2), (1, 1024, 1, 0, 0, 16),
     (2, 64, 2, 24, 64, 3), (2, 256, 8, 24, 64, 16), (2, 512, 4, 1, 5, 32),
     (3, 64, 2, 24, 64, 3), (3, 256, 8, 24, 64, 16), (3, 256, 6, 24, 64, 10), (3, 512, 4, 1, 5, 32),
-    (3, 1024, 1, 32, 1000, 32)])
+    (3, 1024, 1, 32, 1000, 32), (4, 64, 2, 24, 64, 3), (4, 256, 8, 8, 64, 10),
+    (4, 512, 4, 1, 5, 32)])
 def test_launch_shapes_do_not_change_results(gpu, medium, kernel, block, per_cu, lanes, batch, lds):
     """Every kernel variant and launch shape gives the same bits; small LDS
     stacks force the global spill path on this depth-20 tree."""
@@ -251,3 +252,34 @@ def test_autotune_picks_a_shape_that_keeps_the_results(gpu, medium):
         assert np.array_equal(run_any(gpu, tree, rays), O.trace(otree, pos, idx, rays, any_hit=True))
     finally:
         gpu.set_trace_config()
+
+
+def test_compressed_pairs_on_the_full_mesh(gpu):
+    """Kernel 4 (32-byte compressed sibling pairs) against kernel 3 on the
+    micro-benchmark's 1.41 M-triangle mesh, 2^20 camera rays and their
+    cosine bounce rays: the same hits bit for bit, closest and any."""
+    import torch
+
+    from sprout_amd import scenes
+
+    log("displaced icosphere, GPU SBVH")
+    pos, idx = scenes.displaced_icosphere()
+    tree = gpu.Builder_SAH(16, 64, 4).build(torch.from_numpy(pos).cuda(),
+                                            torch.from_numpy(idx.view(np.int32)).cuda())
+    cam = scenes.camera_rays(1024, 1024)
+    cam_hits = run_closest(gpu, tree, cam)
+    bounce = scenes.bounce_rays(cam, cam_hits.view(np.float32).reshape(-1, 4), pos, idx,
+                                1024 * 1024, seed=3)
+    out = {}
+    try:
+        for kernel in (3, 4):
+            gpu.set_trace_config(kernel=kernel)
+            log(f"kernel {kernel}: camera and bounce rays")
+            out[kernel] = [run_closest(gpu, tree, cam), run_closest(gpu, tree, bounce),
+                           run_any(gpu, tree, bounce)]
+    finally:
+        gpu.set_trace_config()
+    for a, b in zip(out[3], out[4]):
+        mism = int(np.count_nonzero(np.any(a.reshape(len(a), -1) != b.reshape(len(b), -1), axis=1)))
+        assert 0 == mism, mism
+    assert (out[3][1].reshape(-1, 4)[:, 3] != 0xFFFFFFFF).mean() > 0.1  # 22 % of bounces hit
