@@ -591,8 +591,8 @@ def main() -> int:
                     help="only the traversal benchmark's line, without the other workloads' figures")
     ap.add_argument("--shape", default="",
                     help="traversal launch shape kernel,lds_entries,refill_lanes,claim_batch,"
-                         "blocks_per_cu instead of autotuning (for profiles: the timed launches "
-                         "then follow the one camera-ray launch)")
+                         "blocks_per_cu[,sort_rays] instead of autotuning (for profiles: the "
+                         "timed launches then follow the one camera-ray launch)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="launch one 256 MiB device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -644,6 +644,13 @@ def main() -> int:
     log(f"  {info.num_nodes} nodes, {info.num_slots} slots, depth {info.max_depth}, "
         f"{build_s * 1e3:.1f} ms")
 
+    if args.shape:
+        kernel, lds, lanes, batch, per_cu, sort = (int(x) for x in (args.shape + ",0").split(",")[:6])
+        bvh.set_trace_config(256, per_cu, kernel, lanes, batch, lds, sort)
+        shape = {"block_size": 256, "blocks_per_cu": per_cu, "kernel": kernel,
+                 "refill_lanes": lanes, "claim_batch": batch, "lds_entries": lds,
+                 "sort_rays": sort}
+        log(f"launch shape {shape}")
     w = args.width
     log(f"camera rays {w}x{w}")
     cam = scenes.camera_rays(w, w)
@@ -656,13 +663,7 @@ def main() -> int:
     d_bounce = torch.from_numpy(bounce).cuda()
     n = len(bounce)
 
-    if args.shape:
-        kernel, lds, lanes, batch, per_cu = (int(x) for x in args.shape.split(","))
-        bvh.set_trace_config(256, per_cu, kernel, lanes, batch, lds)
-        shape = {"block_size": 256, "blocks_per_cu": per_cu, "kernel": kernel,
-                 "refill_lanes": lanes, "claim_batch": batch, "lds_entries": lds}
-        log(f"launch shape {shape}")
-    else:
+    if not args.shape:
         log("autotuning the traversal's launch shape")
         shape, tuned_ms = bvh.autotune(tree, d_bounce, 5)
         log(f"  {shape}: {tuned_ms:.3f} ms")

@@ -182,6 +182,10 @@ typedef struct mi_trace_config {
     uint32_t claim_batch;      /* kernels 0, 2, 3: rays a wave claims per atomic (64) */
     uint32_t lds_entries;      /* stack entries per lane kept in LDS, <= 32 (default 10;
                                   0: up to 32); deeper entries spill to global memory */
+    uint32_t sort_rays;        /* kernels 3 and 4, mi_trace_closest / mi_trace_any: trace the
+                                  rays in the order of (direction octant, Morton code of the
+                                  origin), a radix sort per call; results in the callers'
+                                  order, the same values (0: off) */
 } mi_trace_config;
 
 int32_t mi_set_trace_config(mi_trace_config const* config);
@@ -189,7 +193,7 @@ int32_t mi_set_trace_config(mi_trace_config const* config);
 /* Autotuner of the traversal's launch shape: times closest-hit traversals of
  * the given DEVICE rays through the tree over candidate shapes (kernels 3
  * and 4; LDS stack entries, refill threshold, claim batch and workgroups per
- * CU around the defaults), `repeats` timed launches each after one warm-up,
+ * CU around the defaults, then the best with the rays sorted), `repeats` timed launches each after one warm-up,
  * with HIP events on `stream`; makes the fastest the context's shape (as
  * mi_set_trace_config) and returns it in *best (HOST, may be NULL) and its
  * mean launch time in *best_ms.  Every shape gives the same results, so
@@ -370,7 +374,9 @@ int32_t mi_scene_update_instances(uint32_t scene, float const* world_to_object,
  * on the host.  MI_ERR_ARGS, and the old positions kept, if an emissive
  * triangle would lose its area.  light_ms (optional): host wall time of the
  * light part.  Waits for renders in flight.  Instanced scenes move their
- * instances with mi_scene_update_instances. */
+ * instances with mi_scene_update_instances.  Both updates keep the scene
+ * box of its creation as the guide's position normalization, so a trained
+ * guide sees the same inputs after a move (points beyond it clamp). */
 int32_t mi_scene_update_positions(uint32_t scene, float const* positions, uint32_t num_vertices,
                                   double* light_ms, void* stream);
 

@@ -56,7 +56,8 @@ class MiBvhInfo(ctypes.Structure):
 class MiTraceConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("kernel", ctypes.c_uint32), ("refill_lanes", ctypes.c_uint32),
-                ("claim_batch", ctypes.c_uint32), ("lds_entries", ctypes.c_uint32)]
+                ("claim_batch", ctypes.c_uint32), ("lds_entries", ctypes.c_uint32),
+                ("sort_rays", ctypes.c_uint32)]
 
 
 class MiMaterial(ctypes.Structure):

@@ -156,10 +156,12 @@ def upload(nodes: np.ndarray, slots: np.ndarray, positions: np.ndarray, indices:
 
 
 def set_trace_config(block_size: int = 0, blocks_per_cu: int = 0, kernel: int = 0,
-                     refill_lanes: int = 0, claim_batch: int = 0, lds_entries: int = 0) -> None:
-    """Traversal launch shape; 0 keeps the library default of each field."""
+                     refill_lanes: int = 0, claim_batch: int = 0, lds_entries: int = 0,
+                     sort_rays: int = 0) -> None:
+    """Traversal launch shape; 0 keeps the library default of each field
+    (sort_rays: 0 off)."""
     cfg = _lib.MiTraceConfig(block_size, blocks_per_cu, kernel, refill_lanes, claim_batch,
-                             lds_entries)
+                             lds_entries, sort_rays)
     check(_lib.load().mi_set_trace_config(ctypes.byref(cfg)), "mi_set_trace_config")
 
 

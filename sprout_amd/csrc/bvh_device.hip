@@ -76,9 +76,15 @@ int32_t TraceResources::reserve_spill(size_t words) {
 void TraceResources::release() {
     (void)hipFree(spill);
     (void)hipFree(counter);
-    spill       = nullptr;
-    counter     = nullptr;
-    spill_words = 0;
+    (void)hipFree(sort_keys);
+    (void)hipFree(sort_temp);
+    spill           = nullptr;
+    counter         = nullptr;
+    spill_words     = 0;
+    sort_keys       = nullptr;
+    sort_temp       = nullptr;
+    sort_temp_bytes = 0;
+    sort_capacity   = 0;
 }
 
 int32_t fill_triangles(DeviceBvh& bvh, float const* positions, uint32_t position_stride,

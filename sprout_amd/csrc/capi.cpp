@@ -390,6 +390,7 @@ int32_t mi_set_trace_config(mi_trace_config const* config) {
     context->trace.refill_lanes  = lanes;
     context->trace.claim_batch   = batch;
     context->trace.lds_entries   = config->lds_entries ? config->lds_entries : defaults.lds_entries;
+    context->trace.sort_rays     = config->sort_rays ? 1u : 0u;
     return MI_OK;
 }
 
@@ -443,6 +444,7 @@ int32_t mi_trace_autotune(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, u
     for (Shape const& sh : shapes) {
         TraceResources& t = context->trace;
         set_shape(t, sh, kernel, 256);
+        t.sort_rays = 0;
         st = launch_trace(*b, t, rays, num_rays, hits, nullptr, false, s);  // warm-up
         if (!st) e = hipEventRecord(e0, s);
         for (uint32_t k = 0; k < repeats && !st && !e; ++k) {
@@ -472,9 +474,35 @@ int32_t mi_trace_autotune(uint32_t bvh, mi_ray const* rays, uint32_t num_rays, u
     }
     TraceResources& t = context->trace;
     set_shape(t, best_shape, best_kernel, 256);
+    // the best shape once more with the rays sorted first (the sort counts)
+    {
+        hipEvent_t f0 = nullptr, f1 = nullptr;
+        hipError_t fe = hipEventCreate(&f0);
+        if (!fe) fe = hipEventCreate(&f1);
+        mi_hit* sh = nullptr;
+        if (!fe) fe = hipMalloc(&sh, size_t(num_rays) * sizeof(mi_hit));
+        t.sort_rays = 1;
+        int32_t fs  = fe ? MI_ERR_HIP : launch_trace(*b, t, rays, num_rays, sh, nullptr, false, s);
+        if (!fs && !fe) fe = hipEventRecord(f0, s);
+        for (uint32_t k = 0; k < repeats && !fs && !fe; ++k) {
+            fs = launch_trace(*b, t, rays, num_rays, sh, nullptr, false, s);
+        }
+        if (!fs && !fe) fe = hipEventRecord(f1, s);
+        if (!fs && !fe) fe = hipEventSynchronize(f1);
+        float ms = 0.f;
+        if (!fs && !fe) fe = hipEventElapsedTime(&ms, f0, f1);
+        t.sort_rays = (!fs && !fe && ms / float(repeats) < best_t) ? 1u : 0u;
+        if (t.sort_rays) {
+            best_t = ms / float(repeats);
+        }
+        (void)hipEventDestroy(f0);
+        (void)hipEventDestroy(f1);
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(sh);
+    }
     if (best) {
         *best = mi_trace_config{t.block_size, t.blocks_per_cu, t.kernel, t.refill_lanes,
-                                t.claim_batch, t.lds_entries};
+                                t.claim_batch, t.lds_entries, t.sort_rays};
     }
     if (best_ms) {
         *best_ms = best_t;

@@ -2581,39 +2581,6 @@ __global__ void absolute_cdf_offsets(Emitter_work w, dev::Part_info const* infos
     }
 }
 
-// Per block: min and max of the vertices it covers.
-__global__ void __launch_bounds__(256) vertex_bounds(float const* pos, uint32_t n, float* out) {
-    __shared__ float red[6][256];
-    float            b[6] = {3.40282347e+38f, 3.40282347e+38f, 3.40282347e+38f,
-                             -3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f};
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        for (int k = 0; k < 3; ++k) {
-            float const x = pos[3 * size_t(v) + k];
-            b[k]          = fminf(b[k], x);
-            b[3 + k]      = fmaxf(b[3 + k], x);
-        }
-    }
-    for (int k = 0; k < 6; ++k) {
-        red[k][threadIdx.x] = b[k];
-    }
-    __syncthreads();
-    for (uint32_t off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
-            for (int k = 0; k < 3; ++k) {
-                red[k][threadIdx.x]     = fminf(red[k][threadIdx.x], red[k][threadIdx.x + off]);
-                red[3 + k][threadIdx.x] = fmaxf(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + off]);
-            }
-        }
-        __syncthreads();
-    }
-    if (0 == threadIdx.x) {
-        for (int k = 0; k < 6; ++k) {
-            out[6 * size_t(blockIdx.x) + k] = red[k][0];
-        }
-    }
-}
-
-constexpr uint32_t Bounds_blocks = 256;
 
 }  // namespace
 
@@ -2631,7 +2598,6 @@ void Emitter_work::release() {
     (void)hipFree(num_cdfs);
     (void)hipFree(part_first);
     (void)hipFree(bad);
-    (void)hipFree(bounds);
     *this = Emitter_work();
 }
 
@@ -2666,7 +2632,6 @@ int32_t update_positions(RenderScene& s, float const* positions, uint32_t num_ve
         if (!e) e = hipMalloc(&w.num_cdfs, p * sizeof(uint32_t));
         if (!e) e = hipMalloc(&w.part_first, (p + 1) * sizeof(uint32_t));
         if (!e) e = hipMalloc(&w.bad, sizeof(uint32_t));
-        if (!e) e = hipMalloc(&w.bounds, 6 * Bounds_blocks * sizeof(float));
         if (!e && P) {
             e = hipMemcpy(w.part_first, s.part_first.data(), (P + 1) * sizeof(uint32_t),
                           hipMemcpyHostToDevice);
@@ -2794,27 +2759,8 @@ int32_t update_positions(RenderScene& s, float const* positions, uint32_t num_ve
     if (light_ms) {
         *light_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    // the bounds of every vertex (the guide's position inputs)
-    if (num_vertices) {
-        hipLaunchKernelGGL(vertex_bounds, dim3(Bounds_blocks), dim3(256), 0, stream, s.positions,
-                           num_vertices, w.bounds);
-        std::vector<float> b(6 * Bounds_blocks);
-        e = hipMemcpyAsync(b.data(), w.bounds, b.size() * sizeof(float), hipMemcpyDeviceToHost, stream);
-        if (!e) e = hipStreamSynchronize(stream);
-        if (e) {
-            return fail(MI_ERR_HIP, "mi_scene_update_positions: %s", hipGetErrorString(e));
-        }
-        for (int k = 0; k < 3; ++k) {
-            s.box_lo[k] = b[k];
-            s.box_hi[k] = b[3 + k];
-        }
-        for (uint32_t i = 1; i < Bounds_blocks; ++i) {
-            for (int k = 0; k < 3; ++k) {
-                s.box_lo[k] = std::min(s.box_lo[k], b[6 * i + k]);
-                s.box_hi[k] = std::max(s.box_hi[k], b[6 * i + 3 + k]);
-            }
-        }
-    }
+    // box_lo / box_hi stay the creation's: they map positions to the guide's
+    // inputs, which a trained guide keeps across moves
     // the traced geometry: a new tree from the GPU builder
     DeviceBvh bvh;
     if (int32_t const st = build_bvh(s.positions, num_vertices, 3, s.indices, s.num_triangles,
@@ -2990,16 +2936,8 @@ int32_t update_instances(RenderScene& s, float const* world_to_object, float con
     }
     s.tlas.release();
     s.tlas = std::move(fresh);
-    for (int k = 0; k < 3; ++k) {
-        s.box_lo[k] = s.world_box_lo[k];
-        s.box_hi[k] = s.world_box_hi[k];
-    }
-    for (Instance_record const& r : s.tlas.records) {
-        for (int k = 0; k < 3; ++k) {
-            s.box_lo[k] = std::min(s.box_lo[k], r.lo[k]);
-            s.box_hi[k] = std::max(s.box_hi[k], r.hi[k]);
-        }
-    }
+    // box_lo / box_hi stay the creation's: they map positions to the guide's
+    // inputs, which a trained guide keeps across moves
     return MI_OK;
 }
 

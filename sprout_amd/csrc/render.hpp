@@ -64,7 +64,6 @@ struct Emitter_work {
     uint32_t* num_cdfs  = nullptr;  // per part
     uint32_t* part_first = nullptr; // per part + 1 (device)
     uint32_t* bad       = nullptr;  // an emissive triangle without area
-    float*    bounds    = nullptr;  // per block of the bounds reduction: 6 floats
     uint32_t  capacity_tris  = 0;
     uint32_t  capacity_parts = 0;
 
@@ -126,8 +125,8 @@ struct RenderScene {
     uint32_t num_materials = 0;
     uint32_t num_lights      = 0;
     uint32_t num_light_nodes = 0;
-    float    box_lo[3]     = {0.f, 0.f, 0.f};  // bounds of all vertices
-    float    box_hi[3]     = {0.f, 0.f, 0.f};
+    float    box_lo[3]     = {0.f, 0.f, 0.f};  // bounds of all vertices at creation (the
+    float    box_hi[3]     = {0.f, 0.f, 0.f};  // guide's position inputs; moves keep them)
 
     void release();
     void release_medium();

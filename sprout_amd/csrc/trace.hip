@@ -36,6 +36,7 @@
 #include "trace.hpp"
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -105,6 +106,7 @@ struct Launch {
     uint32_t*     counter;     // claim counters of trace_refill: Claim_parts words, Claim_stride apart
     uint32_t      refill_lanes;
     uint32_t      claim_batch;
+    uint32_t const* order;    // kernels 3 and 4: the k-th claimed ray is order[k] (or k)
 };
 
 struct Ray {
@@ -736,7 +738,7 @@ __global__ void trace_refill_packed(Launch p) {
                 bool const     mine  = (idle >> lane) & 1u;
                 uint32_t const rank  = uint32_t(__popcll(idle & lt_mask));
                 if (mine && rank < take) {
-                    r        = pool_next + rank;
+                    r        = p.order ? p.order[pool_next + rank] : pool_next + rank;
                     ray      = load_ray(p.rays, r);
                     index    = Null;
                     u        = 0.f;
@@ -857,6 +859,36 @@ __global__ void quantize_pairs(float4 const* nodes, uint32_t num_pairs, float ox
     r1.w = pack_node(__float_as_uint(blo.w), __float_as_uint(bhi.w));
     out[2 * size_t(j) + 0] = r0;
     out[2 * size_t(j) + 1] = r1;
+}
+
+// Sort key of each ray: its direction octant over a 27-bit Morton code of
+// its origin on a 512^3 grid over the tree's box (origins outside clamp).
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__global__ void ray_keys(mi_ray const* rays, uint32_t n, float lx, float ly, float lz, float sx,
+                         float sy, float sz, uint32_t* keys, uint32_t* order) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    float4 const* p  = reinterpret_cast<float4 const*>(rays) + 2 * size_t(i);
+    float4 const  ro = p[0];
+    float4 const  rd = p[1];
+    auto cell = [](float x, float lo, float s) {
+        float const c = (x - lo) * s;
+        return c >= 511.f ? 511u : (c > 0.f ? uint32_t(c) : 0u);
+    };
+    uint32_t const oct = (rd.x < 0.f ? 4u : 0u) | (rd.y < 0.f ? 2u : 0u) | (rd.z < 0.f ? 1u : 0u);
+    uint32_t const m   = (spread3(cell(ro.x, lx, sx)) << 2) | (spread3(cell(ro.y, ly, sy)) << 1) |
+                       spread3(cell(ro.z, lz, sz));
+    keys[i]  = (oct << 27) | m;
+    order[i] = i;
 }
 
 __global__ void reset_counter(uint32_t* counter) {
@@ -1013,7 +1045,53 @@ int32_t launch(DeviceBvh const& bvh, TraceResources& res, mi_ray const* rays, ui
     }
     bool const quant = packed && 4 == res.kernel && nullptr != bvh.qpairs;
 
+    // the rays' tracing order (kernels 3 and 4, host count, allocating launches)
+    uint32_t const* order = nullptr;
+    if (res.sort_rays && packed && !counts && !num_rays_dev && !fixed && num_rays > 1) {
+        if (num_rays > res.sort_capacity) {
+            (void)hipFree(res.sort_keys);
+            (void)hipFree(res.sort_temp);
+            res.sort_keys       = nullptr;
+            res.sort_temp       = nullptr;
+            res.sort_capacity   = 0;
+            res.sort_temp_bytes = 0;
+            size_t     bytes = 0;
+            hipError_t e     = hipcub::DeviceRadixSort::SortPairs(
+                nullptr, bytes, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+                static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr), int(num_rays), 0,
+                30, stream);
+            if (!e) e = hipMalloc(&res.sort_keys, 4 * size_t(num_rays) * sizeof(uint32_t));
+            if (!e) e = hipMalloc(&res.sort_temp, std::max<size_t>(bytes, 1));
+            if (e) {
+                return fail(MI_ERR_HIP, "ray order of %u rays: %s", num_rays, hipGetErrorString(e));
+            }
+            res.sort_capacity   = num_rays;
+            res.sort_temp_bytes = std::max<size_t>(bytes, 1);
+        }
+        uint32_t* const keys_in   = res.sort_keys;
+        uint32_t* const keys_out  = keys_in + res.sort_capacity;
+        uint32_t* const order_in  = keys_out + res.sort_capacity;
+        uint32_t* const order_out = order_in + res.sort_capacity;
+        float           scale[3];
+        for (int k = 0; k < 3; ++k) {
+            float const ext = bvh.aabb_max[k] - bvh.aabb_min[k];
+            scale[k]        = ext > 0.f ? 512.f / ext : 0.f;
+        }
+        hipLaunchKernelGGL(ray_keys, dim3((num_rays + 255) / 256), dim3(256), 0, stream, rays,
+                           num_rays, bvh.aabb_min[0], bvh.aabb_min[1], bvh.aabb_min[2], scale[0],
+                           scale[1], scale[2], keys_in, order_in);
+        size_t     bytes = res.sort_temp_bytes;
+        hipError_t e     = hipcub::DeviceRadixSort::SortPairs(res.sort_temp, bytes, keys_in,
+                                                              keys_out, order_in, order_out,
+                                                              int(num_rays), 0, 30, stream);
+        if (e) {
+            return fail(MI_ERR_HIP, "ray order: %s", hipGetErrorString(e));
+        }
+        order = order_out;
+    }
+
     Launch p;
+    p.order         = order;
     p.nodes         = bvh.nodes;
     p.qpairs        = bvh.qpairs;
     for (int k = 0; k < 3; ++k) {

@@ -58,6 +58,15 @@ struct TraceResources {
     uint32_t* counter       = nullptr;  // ray claim counters of trace_refill (Counter_words)
     uint32_t* spill         = nullptr;
     size_t    spill_words   = 0;
+    // kernels 3 and 4 with a host ray count: trace the rays in the order of
+    // a key (direction octant, Morton code of the origin in the tree's box)
+    // so that a wave's rays start close together; results land in the
+    // callers' order, so they are the same
+    uint32_t  sort_rays     = 0;
+    uint32_t* sort_keys     = nullptr;  // 4 arrays of sort_capacity words: keys in / out, order in / out
+    void*     sort_temp     = nullptr;
+    size_t    sort_temp_bytes = 0;
+    uint32_t  sort_capacity   = 0;
 
     uint32_t grid_blocks(uint32_t num_rays) const {
         uint64_t const wanted   = (uint64_t(num_rays) + block_size - 1) / block_size;

@@ -143,6 +143,24 @@ def test_launch_shapes_do_not_change_results(gpu, medium, kernel, block, per_cu,
         gpu.set_trace_config()
 
 
+@pytest.mark.parametrize("kernel", [3, 4])
+def test_sorted_rays_give_the_same_results(gpu, medium, golden_small, kernel):
+    """sort_rays: the rays are traced in the order of their octant and
+    origin Morton code, their results written in the callers' order: the
+    same bits as the oracle, also for one ray and for two."""
+    pos, idx, otree, tree, rays = medium
+    log(f"kernel {kernel}, rays sorted")
+    gpu.set_trace_config(kernel=kernel, sort_rays=1)
+    try:
+        assert np.array_equal(run_closest(gpu, tree, rays), O.trace(otree, pos, idx, rays))
+        assert np.array_equal(run_any(gpu, tree, rays), O.trace(otree, pos, idx, rays, any_hit=True))
+        for n in (1, 2, 3):
+            assert np.array_equal(run_closest(gpu, tree, rays[-n:]),
+                                  O.trace(otree, pos, idx, rays[-n:]))
+    finally:
+        gpu.set_trace_config()
+
+
 def test_empty_and_single_ray_batches(gpu, golden_small):
     import torch
 
