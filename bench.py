@@ -285,6 +285,66 @@ def cloud_render(frames: int, cpu: bool):
     return out
 
 
+def cloud_guided_equal_time(seconds: float = 1.5, ref_spp: int = 65536):
+    """The fBm cloud as configured (flat majorants, hero-wavelength
+    tracking), at 256 x 256: plain spectral MIS against guided spectral MIS
+    with the guide at surface and medium vertices (trained online from
+    scratch in the first fifth of its time budget, then frozen), each for
+    `seconds`; relMSE against a `ref_spp` plain spectral render made here.
+    The plain side reaches about 3 600 spp in 1.5 s, so the reference's own
+    variance is about 5 % of the measured error."""
+    import torch
+
+    from sprout_amd import guide, medium, render, scenes
+
+    pos, idx, tmat, mats = scenes.cloud_scene()
+    grid = scenes.fbm_density(256, 6, 1, 20.0)
+    sc = render.Scene(pos, idx, tmat, mats)
+    medium.set_medium(sc, grid, *scenes.CLOUD_BOX, scenes.CLOUD_CC_A, scenes.CLOUD_CC_S,
+                      scenes.CLOUD_G, majorants="flat")
+    size = 256
+    cam = render.camera(size, size, **scenes.CLOUD_CAMERA)
+    ref, _ = render.render(sc, cam, ref_spp, samples_per_wave=64, seed=99, spectral=True)
+    torch.cuda.synchronize()
+    ref = ref.cpu().numpy() / ref_spp
+    acc = torch.zeros((size, size, 3), dtype=torch.float64, device="cuda")
+    spw = 16
+    render.render(sc, cam, spw, samples_per_wave=spw, seed=2, spectral=True)
+    torch.cuda.synchronize()
+    spp = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        render.render(sc, cam, spw, samples_per_wave=spw, seed=11, first_sample=spp, accum=acc,
+                      spectral=True)
+        torch.cuda.synchronize()
+        spp += spw
+    e_plain = rel_mse(acc.cpu().numpy() / spp, ref)
+    out = {"workload": "fBm cloud 256^3, flat majorants, spectral, 256x256, max depth 8",
+           "seconds": seconds, "reference_spp": ref_spp, "plain_spp": spp,
+           "plain_relmse": e_plain}
+    for depth in (1, 3):
+        g = guide.Guide(8)
+        acc.zero_()
+        gspp = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            train = time.perf_counter() - t0 < 0.2 * seconds
+            n = 4 if train else spw
+            render.render_guided(sc, cam, n, g, fraction=0.5, train=train, train_steps=4,
+                                 batch=65536, learning_rate=3e-3, record_depth=depth,
+                                 guide_depth=depth, samples_per_wave=n, seed=12,
+                                 first_sample=gspp, accum=acc, guide_medium=True, spectral=True)
+            torch.cuda.synchronize()
+            gspp += n
+        e = rel_mse(acc.cpu().numpy() / gspp, ref)
+        out[f"guided_depth{depth}_spp"] = gspp
+        out[f"guided_depth{depth}_relmse"] = e
+        out[f"guided_depth{depth}_equal_time_gain"] = round(e_plain / e, 3)
+        g.release()
+    sc.release()
+    return out
+
+
 def batch_render(frames: int, instanced: bool):
     """Batched path tracing: 64 placements of 4 distinct 20 480-triangle
     displaced spheres with per-instance Substitute materials, a ground plane
@@ -583,7 +643,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--guide-seconds", type=float, default=6.0,
                     help="time budget of each side of the guided vs plain comparison")
-    ap.add_argument("--part", choices=["all", "room", "volume", "guiding", "instances", "batch",
+    ap.add_argument("--part", choices=["all", "room", "volume", "volume_guiding", "guiding",
+                                       "instances", "batch",
                                        "batch_instanced", "guide_kernels"],
                     default="all",
                     help="run one workload only and print its figures (for profiling)")
@@ -619,6 +680,7 @@ def main() -> int:
 
     if args.part != "all":
         parts = {"room": lambda: room_render(5, False), "volume": lambda: cloud_render(3, False),
+                 "volume_guiding": cloud_guided_equal_time,
                  "batch": lambda: batch_render(2, False),
                  "batch_instanced": lambda: batch_render(2, True),
                  "guide_kernels": guide_kernels,
@@ -759,6 +821,8 @@ def main() -> int:
     line["extra"]["instances"] = instance_trace(args.steps, args.warmup, not args.no_cpu_baseline)
     log("fBm cloud path tracing")
     line["extra"]["volume_render"] = cloud_render(3, not args.no_cpu_baseline)
+    log("fBm cloud: guided vs plain at equal time")
+    line["extra"]["volume_guiding"] = cloud_guided_equal_time()
     log("guide kernels")
     line["extra"]["guide_kernels"] = guide_kernels()
     log("slit room: guided vs plain at equal time")
