@@ -549,6 +549,19 @@ int32_t mi_scene_light_tree(uint32_t scene, mi_light_node* nodes, uint32_t* midd
                             uint32_t* mapping, uint32_t* orders, void* infos,
                             uint32_t* num_nodes, uint32_t* num_lights);
 
+/* Shading normals of a scene's triangles: 3 floats per vertex (HOST, any
+ * non-zero length; stored normalized).  A hit shades with its triangle's
+ * vertex normals interpolated at the hit's barycentrics and normalized, as
+ * sprout's Mesh::intersect does (triangle_mesh.cpp:398-414,
+ * triangle_primitive_mt.inl:274-287); the geometric normal still decides
+ * the side, offsets rays and weighs lights.  sprout gives every mesh vertex
+ * a normal (the file's, or for a JSON mesh without them the last geometric
+ * normal written to it, triangle_mesh_provider.cpp:103-118); without this
+ * call a scene shades with the geometric normal.  normals NULL goes back to
+ * that.  Moving the vertices (mi_scene_update_positions) keeps the normals.
+ * Mesh instances shade with their geometric normal. */
+int32_t mi_scene_set_normals(uint32_t scene, float const* normals, uint32_t num_vertices);
+
 /* ---- participating medium ---- */
 
 /* Gives a scene a heterogeneous medium: a density grid of dims[0] x dims[1] x

@@ -380,6 +380,8 @@ typedef struct {
     float*   samples; /* non-NULL: per sample and grid pixel r, g, b, jitter x, y of the
                          (W + 2 border) x (H + 2 border) grid, sample-major; accum unused */
     int32_t  spectral; /* hero-wavelength transport over four wavelengths (spectral.c) */
+    float const* normals; /* per vertex of the triangles, 3 each: the shading normals,
+                             interpolated at a hit; NULL: the geometric normal */
 } oracle_pt_settings;
 
 /* ---- hero-wavelength transport (spectral.c) ---- */

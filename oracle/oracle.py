@@ -68,7 +68,8 @@ class _OraclePtSettings(ctypes.Structure):
                 ("num_threads", ctypes.c_uint32), ("split_lights", ctypes.c_int32),
                 ("medium", ctypes.c_void_p), ("instances", ctypes.c_void_p),
                 ("border", ctypes.c_uint32), ("golden_ratio", ctypes.c_int32),
-                ("samples", ctypes.c_void_p), ("spectral", ctypes.c_int32)]
+                ("samples", ctypes.c_void_p), ("spectral", ctypes.c_int32),
+                ("normals", ctypes.c_void_p)]
 
 
 class _OracleGridtree(ctypes.Structure):
@@ -1129,14 +1130,16 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
            tree: LightTree, first_sample=0, min_bounces=4, max_bounces=8, seed=1, mode=PT_MIS,
            threads=None, split_lights=True, medium: Medium | None = None,
            instances: PtInstances | None = None, border: int = 0, samples: bool = False,
-           golden_ratio: bool = False, spectral: bool = False):
+           golden_ratio: bool = False, spectral: bool = False, normals=None):
     """Radiance sums per pixel, (height, width, 3) float64, of the CPU path
     tracer (pt.c).  camera: {eye, target, up, fov_deg (vertical)} or sprout's
     {position, rotation, fov_deg (horizontal)}; tree: the scene's light tree;
     medium: a participating medium or None; instances: mesh instances beside
     the triangles, or None; golden_ratio: pixel samples from sprout's
     Golden_ratio sampler; spectral: hero-wavelength transport over four
-    wavelengths (spectral.c).  samples=True returns every sample instead: (spp,
+    wavelengths (spectral.c); normals: (num_vertices, 3) shading normals,
+    interpolated at hits on the triangles (None: the geometric normal).
+    samples=True returns every sample instead: (spp,
     height + 2 border, width + 2 border, 5) float32 of r, g, b and the pixel
     sample x, y, over the grid extended by border."""
     pos = np.ascontiguousarray(positions, np.float32)
@@ -1144,6 +1147,11 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
     tmat = np.ascontiguousarray(triangle_materials, np.uint32)
     mats = np.ascontiguousarray(materials).view(MATERIAL_DTYPE)
     cam = _camera(width, height, camera)
+    nrm = None
+    if normals is not None:
+        nrm = np.ascontiguousarray(normals, np.float32)
+        if nrm.shape != pos.shape:
+            raise ValueError("normals: one (x, y, z) per vertex")
     n = threads or min(os.cpu_count() or 1, 16)
     out = (np.zeros((spp, height + 2 * border, width + 2 * border, 5), np.float32)
            if samples else None)
@@ -1152,7 +1160,7 @@ def render(positions, indices, triangle_materials, materials, width, height, spp
                            ctypes.cast(ctypes.pointer(medium.c), ctypes.c_void_p) if medium else None,
                            ctypes.cast(ctypes.pointer(instances.c), ctypes.c_void_p) if instances else None,
                            border, 1 if golden_ratio else 0, _ptr(out) if samples else None,
-                           1 if spectral else 0)
+                           1 if spectral else 0, _ptr(nrm) if nrm is not None else None)
     accum = None if samples else np.zeros((height, width, 3), np.float64)
     status = lib().oracle_render(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(tmat),
                                  _ptr(mats), len(mats), ctypes.byref(tree.c), ctypes.byref(cam),

@@ -216,8 +216,11 @@ typedef struct {
     float   albedo[3];
 } surface;
 
-static void surface_set(surface* sf, oracle_material const* m, v3 geo_n, v3 wo) {
-    float const na[3] = {geo_n.x, geo_n.y, geo_n.z};
+/* ns: the shading normal (the layer's frame), geo_n: the geometric normal
+ * (Sample::same_hemisphere, material_sample.inl:141-143) */
+static void surface_set(surface* sf, oracle_material const* m, v3 ns, v3 geo_n, v3 wo) {
+    float const na[3] = {ns.x, ns.y, ns.z};
+    float const ga[3] = {geo_n.x, geo_n.y, geo_n.z};
     float       t[3], b[3];
     oracle_orthonormal_basis(na, t, b);
     sf->kind = (int32_t)m->kind;
@@ -232,9 +235,9 @@ static void surface_set(surface* sf, oracle_material const* m, v3 geo_n, v3 wo) 
         sf->frame[k]     = t[k];
         sf->frame[3 + k] = b[k];
         sf->frame[6 + k] = na[k];
-        sf->frame[9 + k] = na[k];
+        sf->frame[9 + k] = ga[k];
     }
-    sf->n  = geo_n;
+    sf->n  = ns;
     sf->t  = mk(t[0], t[1], t[2]);
     sf->b  = mk(b[0], b[1], b[2]);
     sf->wo = wo;
@@ -406,6 +409,7 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
         int   scatter = 0;
         v3    hit_p   = mk(0.f, 0.f, 0.f);
         v3    geo_n   = mk(0.f, 0.f, 1.f);
+        v3    shading_n = geo_n;
         if (medium) {
             float const oa[3] = {o.x, o.y, o.z};
             float const da[3] = {d.x, d.y, d.z};
@@ -452,6 +456,18 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
                 m              = &s->materials[s->tri_material[tri]];
                 light          = s->tri_light[tri];
             }
+            shading_n = geo_n;
+            if (set->normals && !(pi && inst != pi->world)) {
+                /* the vertex normals interpolated and normalized
+                 * (triangle::interpolate_data, triangle_primitive_mt.inl:274-287) */
+                float const* nn = set->normals;
+                uint32_t const* ix = s->indices + 3 * (size_t)tri;
+                v3 const na = mk(nn[3 * (size_t)ix[0]], nn[3 * (size_t)ix[0] + 1], nn[3 * (size_t)ix[0] + 2]);
+                v3 const nb = mk(nn[3 * (size_t)ix[1]], nn[3 * (size_t)ix[1] + 1], nn[3 * (size_t)ix[1] + 2]);
+                v3 const nc = mk(nn[3 * (size_t)ix[2]], nn[3 * (size_t)ix[2] + 1], nn[3 * (size_t)ix[2] + 2]);
+                float const bw = (1.f - hu) - hv;
+                shading_n      = normalize(add(add(scale(bw, na), scale(hu, nb)), scale(hv, nc)));
+            }
 
             if (KIND_LIGHT == m->kind) {
                 float const cos_l = dot(geo_n, wo);
@@ -493,7 +509,7 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
                 }
                 break;
             }
-            surface_set(&sf, m, geo_n, wo);
+            surface_set(&sf, m, shading_n, geo_n, wo);
         } else {
             memset(&sf, 0, sizeof(sf));
             sf.kind = KIND_PHASE;

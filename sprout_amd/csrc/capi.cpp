@@ -1169,6 +1169,17 @@ int32_t mi_instances_download(uint32_t instances, void* nodes, uint32_t* slots, 
     return MI_OK;
 }
 
+int32_t mi_scene_set_normals(uint32_t scene, float const* normals, uint32_t num_vertices) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT,
+                    "mi_scene_set_normals: bad handle %u", scene);
+    }
+    (void)hipDeviceSynchronize();  // renders in flight may read the old normals
+    return set_normals(*sc, normals, num_vertices);
+}
+
 int32_t mi_scene_set_medium(uint32_t scene, float const* density, uint32_t const dims[3],
                             float const box_lo[3], float const box_hi[3], float const cc_a[3],
                             float const cc_s[3], float g) {

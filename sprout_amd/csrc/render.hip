@@ -170,6 +170,7 @@ struct Nee_vertex {
     uint32_t path;
     float    n[3];       // geometric normal; (0, 0, 1) in the medium
     uint32_t material;   // the surface's material, or Kind_phase
+    float    ns[3];      // shading normal (the material's frame)
     float    wo[3];
     uint32_t flags;      // Nee_guided, Nee_medium
     float    throughput[3];
@@ -190,6 +191,7 @@ struct Pick_slot {
 struct Wave {
     // scene
     float const*           positions;
+    float const*           normals;  // shading normals per vertex, or null
     uint32_t const*        indices;
     uint32_t const*        tri_material;
     Render_material const* materials;
@@ -317,7 +319,8 @@ __device__ __forceinline__ Hit load_hit(Wave const& w, uint32_t i) {
 // Point and geometric normal of a (non-missed) hit, its material and light.
 struct Surface_point {
     V3       p;
-    V3       n;
+    V3       n;   // geometric normal
+    V3       ns;  // shading normal: the vertex normals interpolated, else n
     uint32_t material;
     uint32_t light;  // Null unless an emitter of the light tree
 };
@@ -333,6 +336,19 @@ __device__ Surface_point surface_point(Wave const& w, Hit const& h) {
         float const    bw  = (1.f - h.u) - h.v;
         sp.p               = (bw * a + h.u * b) + h.v * c;
         sp.n               = dev::normalize(dev::cross(b - a, c - a));
+        sp.ns              = sp.n;
+        if (w.normals) {
+            // triangle::interpolate_data (triangle_primitive_mt.inl:274-287):
+            // (1 - u - v) n_a + u n_b + v n_c, normalized
+            auto const nv = [&w](uint32_t k) {
+                float const* q = w.normals + 3 * size_t(k);
+                return v3(q[0], q[1], q[2]);
+            };
+            V3 const na = nv(w.indices[3 * size_t(tri) + 0]);
+            V3 const nb = nv(w.indices[3 * size_t(tri) + 1]);
+            V3 const nc = nv(w.indices[3 * size_t(tri) + 2]);
+            sp.ns       = dev::normalize((bw * na + h.u * nb) + h.v * nc);
+        }
         sp.material        = w.tri_material[tri];
         sp.light           = w.tri_light[tri];
         return sp;
@@ -357,6 +373,7 @@ __device__ Surface_point surface_point(Wave const& w, Hit const& h) {
     dev::F3 const nw = dev::xform_vector(m, {no.x, no.y, no.z});
     sp.p             = v3(pw.x, pw.y, pw.z);
     sp.n             = dev::normalize(v3(nw.x, nw.y, nw.z));
+    sp.ns            = sp.n;
     sp.material      = w.inst_material[h.instance];
     sp.light         = dev::Null;
     return sp;
@@ -820,6 +837,7 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q) {
         uint32_t material   = 0;  // of a surface vertex
         V3       hit_p      = v3(0.f, 0.f, 0.f);
         V3       geo_n      = v3(0.f, 0.f, 1.f);
+        V3       shading_n  = geo_n;
         V3       wo         = v3(0.f, 0.f, 1.f);
         V3       p_o        = v3(0.f, 0.f, 0.f);
         // next-event estimation: the lights the tree picks at p_o
@@ -880,6 +898,7 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q) {
                 Surface_point const sp = surface_point(w, h);
                 hit_p                  = sp.p;
                 geo_n                  = sp.n;
+                shading_n              = sp.ns;
                 material               = sp.material;
                 uint32_t const kind    = w.materials[sp.material].kind;
                 live                   = true;
@@ -1014,6 +1033,9 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q) {
             v.n[1]      = geo_n.y;
             v.n[2]      = geo_n.z;
             v.material  = in_medium ? Kind_phase : material;
+            v.ns[0]     = shading_n.x;
+            v.ns[1]     = shading_n.y;
+            v.ns[2]     = shading_n.z;
             v.wo[0]     = wo.x;
             v.wo[1]     = wo.y;
             v.wo[2]     = wo.z;
@@ -1051,7 +1073,7 @@ __device__ __forceinline__ void shade_queue(Wave const& w, uint32_t in_q) {
             } else {
                 Render_material const m = w.materials[material];
                 s.kind                  = m.kind;
-                s.sub.set(dev::frame_from_normal(geo_n), geo_n, wo, m.color, m.roughness, m.metallic,
+                s.sub.set(dev::frame_from_normal(shading_n), geo_n, wo, m.color, m.roughness, m.metallic,
                           m.ior, 1.f, w.e_table);
             }
             // BSDF importance sample; guided: one-sample mixture of the guide
@@ -1321,7 +1343,8 @@ __device__ __forceinline__ void nee_queue(Wave const& w) {
             } else {
                 Render_material const m = w.materials[v.material];
                 s.kind                  = m.kind;
-                s.sub.set(dev::frame_from_normal(geo_n), geo_n, v3(v.wo[0], v.wo[1], v.wo[2]),
+                s.sub.set(dev::frame_from_normal(v3(v.ns[0], v.ns[1], v.ns[2])), geo_n,
+                          v3(v.wo[0], v.wo[1], v.wo[2]),
                           m.color, m.roughness, m.metallic, m.ior, 1.f, w.e_table);
             }
 
@@ -2027,6 +2050,7 @@ int32_t download_light_tree(RenderScene const& scene, mi_light_node* nodes, uint
 void RenderScene::release() {
     bvh.release();
     (void)hipFree(positions);
+    (void)hipFree(normals);
     (void)hipFree(indices);
     (void)hipFree(tri_material);
     (void)hipFree(materials);
@@ -2073,6 +2097,46 @@ void RenderScene::release_medium() {
 }
 
 constexpr double Max_optical_depth = 1.0e4;
+
+int32_t set_normals(RenderScene& s, float const* normals, uint32_t num_vertices) {
+    if (!normals) {
+        (void)hipFree(s.normals);
+        s.normals = nullptr;
+        return MI_OK;
+    }
+    if (num_vertices != s.num_vertices) {
+        return fail(MI_ERR_ARGS, "mi_scene_set_normals: %u normals for %u vertices", num_vertices,
+                    s.num_vertices);
+    }
+    // sprout keeps a vertex's tangent frame as a unit quaternion
+    // (triangle_bvh_indexed_data.inl:136-152), so its normals are unit length
+    std::vector<float> unit(size_t(num_vertices) * 3);
+    for (uint32_t v = 0; v < num_vertices; ++v) {
+        float const* n  = normals + 3 * size_t(v);
+        float const  l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        if (!(l2 > 0.f) || !std::isfinite(l2)) {
+            return fail(MI_ERR_ARGS, "mi_scene_set_normals: vertex %u has no direction", v);
+        }
+        float const il = 1.f / std::sqrt(l2);
+        for (int k = 0; k < 3; ++k) {
+            unit[3 * size_t(v) + k] = n[k] * il;
+        }
+    }
+    float*     d = s.normals;
+    hipError_t e = hipSuccess;
+    if (!d) {
+        e = hipMalloc(&d, unit.size() * sizeof(float));
+    }
+    if (!e) e = hipMemcpy(d, unit.data(), unit.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e) {
+        if (d != s.normals) {
+            (void)hipFree(d);
+        }
+        return fail(MI_ERR_HIP, "mi_scene_set_normals: %s", hipGetErrorString(e));
+    }
+    s.normals = d;
+    return MI_OK;
+}
 
 int32_t set_medium(RenderScene& scene, float const* density, uint32_t const dims[3],
                    float const box_lo[3], float const box_hi[3], float const cc_a[3],
@@ -3373,6 +3437,7 @@ int32_t render(RenderScene const& scene, TraceResources& trace, float const* e_t
     Wave w;
     std::memset(&w, 0, sizeof(w));  // padding too: the bytes key the cached graph
     w.positions    = scene.positions;
+    w.normals      = scene.normals;
     w.indices      = scene.indices;
     w.tri_material = scene.tri_material;
     w.materials    = scene.materials;

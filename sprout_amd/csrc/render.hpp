@@ -73,6 +73,7 @@ struct Emitter_work {
 struct RenderScene {
     DeviceBvh        bvh;
     float*           positions    = nullptr;  // 3 floats per vertex
+    float*           normals      = nullptr;  // 3 per vertex: shading normals, or null (geometric)
     uint32_t*        indices      = nullptr;  // 3 per triangle
     uint32_t*        tri_material = nullptr;
     Render_material* materials    = nullptr;
@@ -165,6 +166,12 @@ int32_t create_scene_instanced(mi_mesh const* meshes, uint32_t num_meshes,
 // scene keeps its old placement.  No render may be in flight.
 int32_t update_instances(RenderScene& scene, float const* world_to_object,
                          float const* object_to_world, uint32_t num_instances);
+
+// Shading normals of the scene's triangles: 3 floats per vertex (host),
+// normalized here; a hit shades with the normals of its triangle's vertices
+// interpolated at the hit and normalized.  normals == nullptr goes back to
+// the geometric normal.  Mesh instances keep the geometric normal.
+int32_t set_normals(RenderScene& scene, float const* normals, uint32_t num_vertices);
 
 // Density grid (nx * ny * nz floats, x fastest, host memory) filling the box
 // [box_lo, box_hi]; cc_a / cc_s: absorption and scattering coefficients per

@@ -27,9 +27,11 @@ def _stream():
 
 
 class Scene:
-    """Triangles, per-triangle materials and materials (scenes.MATERIAL_DTYPE)."""
+    """Triangles, per-triangle materials and materials (scenes.MATERIAL_DTYPE);
+    normals: per-vertex shading normals (set_normals) or None."""
 
-    def __init__(self, positions, indices, triangle_materials, materials, settings=(16, 64, 4)):
+    def __init__(self, positions, indices, triangle_materials, materials, settings=(16, 64, 4),
+                 normals=None):
         lib = _lib.load()
         pos = np.ascontiguousarray(positions, np.float32)
         idx = np.ascontiguousarray(indices, np.uint32)
@@ -43,6 +45,21 @@ class Scene:
         self.handle = handle.value
         self.num_triangles = len(idx)
         self.num_vertices = len(pos)
+        if normals is not None:
+            self.set_normals(normals)
+
+    def set_normals(self, normals) -> None:
+        """mi_scene_set_normals: (num_vertices, 3) shading normals, interpolated
+        at each hit as sprout interpolates its vertex normals; None goes back
+        to the geometric normal."""
+        if normals is None:
+            check(_lib.load().mi_scene_set_normals(self.handle, None, 0), "mi_scene_set_normals")
+            return
+        n = np.ascontiguousarray(normals, np.float32)
+        if n.shape != (self.num_vertices, 3):
+            raise ValueError(f"normals must be ({self.num_vertices}, 3)")
+        check(_lib.load().mi_scene_set_normals(self.handle, n.ctypes.data, len(n)),
+              "mi_scene_set_normals")
 
     def update_positions(self, positions) -> float:
         """mi_scene_update_positions: new vertex positions ((n, 3) float32,

@@ -66,7 +66,8 @@ def load_binary_mesh(path):
     48-byte vertex each when the layout's Bitangent_sign is in stream 0,
     else the separate position stream); indices are UInt16 / UInt32, or
     Int16 / Int32 deltas chained over the parts in order
-    (fill_triangles_delta).  Returns what load_mesh returns."""
+    (fill_triangles_delta).  Returns positions, indices, the part of each
+    triangle and the vertex normals (the normal stream)."""
     with open(path, "rb") as f:
         data = f.read()
     if data[:4] != SUB_MAGIC:
@@ -93,12 +94,19 @@ def load_binary_mesh(path):
         num_vertices = v_size // _VERTEX_BYTES
         if not interleaved:
             i_off = num_vertices * _VERTEX_UNPADDED
+    if any(e.get("semantic_name") == "Tangent_space" for e in v.get("layout", [])):
+        raise NotImplementedError(f"{path}: tangent space stored as quaternions")
     start = 12 + json_size
     if interleaved:
         rec = np.frombuffer(data, np.uint8, num_vertices * _VERTEX_BYTES, start + v_off)
-        pos = rec.reshape(-1, _VERTEX_BYTES)[:, :12].copy().view(np.float32).reshape(-1, 3)
+        rec = rec.reshape(-1, _VERTEX_BYTES)
+        pos = rec[:, :12].copy().view(np.float32).reshape(-1, 3)
+        nrm = rec[:, 12:24].copy().view(np.float32).reshape(-1, 3)
     else:
         pos = np.frombuffer(data, np.float32, 3 * num_vertices, start + v_off).reshape(-1, 3).copy()
+        # the normal stream follows the positions (Vertex_stream_separate)
+        nrm = np.frombuffer(data, np.float32, 3 * num_vertices,
+                            start + v_off + 12 * num_vertices).reshape(-1, 3).copy()
     if 0 == num_indices:
         num_indices = i_size // np.dtype(dtype).itemsize
     if start + i_off + i_size > len(data):
@@ -124,16 +132,40 @@ def load_binary_mesh(path):
         raise ValueError(f"{path}: triangles outside every part")
     if len(idx) and (idx.min() < 0 or idx.max() >= len(pos)):
         raise ValueError(f"{path}: index out of range of {len(pos)} vertices")
-    return pos, idx.astype(np.uint32).reshape(-1, 3), part
+    return pos, idx.astype(np.uint32).reshape(-1, 3), part, nrm
 
 
-def load_mesh(path):
+def face_normals_last_writer(pos, idx):
+    """The vertex normals sprout gives a JSON mesh without normals
+    (triangle_mesh_provider.cpp:103-118): each triangle in order writes its
+    normalized geometric normal, normalize(cross(b - a, c - a)) in float32,
+    to its three vertices, so a vertex shared by several triangles keeps the
+    last one's."""
+    p = np.asarray(pos, np.float32)
+    a, b, c = p[idx[:, 0]], p[idx[:, 1]], p[idx[:, 2]]
+    n = np.cross(b - a, c - a).astype(np.float32)
+    n = (n / np.sqrt((n * n).sum(1, keepdims=True))).astype(np.float32)
+    out = np.zeros_like(p)
+    order = np.repeat(np.arange(len(idx)), 3)
+    verts = idx.reshape(-1)
+    last = np.full(len(p), -1, np.int64)
+    np.maximum.at(last, verts, order)
+    written = last >= 0
+    out[written] = n[last[written]]
+    return out
+
+
+def load_mesh(path, with_normals=False):
     """A triangle mesh file, sprout's JSON mesh format or its binary format
     (load_binary_mesh) -> (positions (v, 3) float32, indices (t, 3) uint32,
-    part of each triangle (t,) uint32)."""
+    part of each triangle (t,) uint32), and with_normals: the vertex normals
+    (v, 3) float32 sprout shades with (the file's, or for a JSON mesh
+    without them face_normals_last_writer)."""
     with open(path, "rb") as f:
-        if f.read(4) == SUB_MAGIC:
-            return load_binary_mesh(path)
+        binary = f.read(4) == SUB_MAGIC
+    if binary:
+        pos, idx, part, nrm = load_binary_mesh(path)
+        return (pos, idx, part, nrm) if with_normals else (pos, idx, part)
     g = _read(path)["geometry"]
     pos = np.asarray(g["vertices"]["positions"], np.float32).reshape(-1, 3)
     idx = np.asarray(g["indices"], np.uint32).reshape(-1, 3)
@@ -143,7 +175,15 @@ def load_mesh(path):
         part[first:first + int(p["num_indices"]) // 3] = int(p["material_index"])
     if len(idx) and int(idx.max()) >= len(pos):
         raise ValueError(f"{path}: index {int(idx.max())} of {len(pos)} vertices")
-    return pos, idx, part
+    if not with_normals:
+        return pos, idx, part
+    if "normals" in g["vertices"]:
+        nrm = np.asarray(g["vertices"]["normals"], np.float32).reshape(-1, 3)
+        if nrm.shape != pos.shape:
+            raise ValueError(f"{path}: {len(nrm)} normals for {len(pos)} vertices")
+    else:
+        nrm = face_normals_last_writer(pos, idx)
+    return pos, idx, part, nrm
 
 
 def rotation_matrix(degrees):
@@ -326,10 +366,18 @@ def _material(node):
     return rec
 
 
-def load_scene(path):
+def load_scene(path, with_normals=False):
     """A scene file -> (positions, indices, per-triangle material, materials)
     of all its mesh entities in world space (transformed in float64, rounded
-    to float32)."""
+    to float32); with_normals: and the world shading normal of each vertex.
+
+    sprout intersects a prop in object space and turns its geometric and
+    shading normals to world space with the rotation alone
+    (Mesh::intersect, triangle_mesh.cpp:388-421), so under a non-uniform
+    scale its geometric normal is not the world triangle's.  The shading
+    normals here are rotated alone as sprout's are; a prop whose scale tilts
+    some triangle's geometric normal that way (by more than 1e-5) is refused,
+    since this path takes the geometric normal from the world triangle."""
     doc = _read(path)
     base = os.path.dirname(os.path.abspath(path))
     names = {}
@@ -337,7 +385,7 @@ def load_scene(path):
     for k, node in enumerate(doc.get("materials", [])):
         names[node.get("name", str(k))] = len(mats)
         mats.append(_material(node))
-    pos, idx, tmat = [], [], []
+    pos, idx, tmat, nrms = [], [], [], []
     nv = 0
     for e in doc.get("entities", []):
         if e.get("type") not in ("Prop", "Light"):
@@ -345,31 +393,60 @@ def load_scene(path):
         shape = e.get("shape", {})
         if "file" not in shape:
             raise NotImplementedError(f"shape {shape.get('type', shape)}")
-        p, i, part = load_mesh(os.path.join(base, shape["file"]))
+        p, i, part, n = load_mesh(os.path.join(base, shape["file"]), with_normals=True)
         slots = [names[m] for m in e.get("materials", [])]
         if not slots or int(part.max(initial=0)) >= len(slots):
             raise ValueError(f"entity {e.get('name', '?')}: {len(slots)} materials for its parts")
-        m = object_to_world(e.get("transformation"))
+        t = e.get("transformation") or {}
+        m = object_to_world(t)
         wp = (p.astype(np.float64) @ m[:3, :3] + m[3, :3]).astype(np.float32)
+        r = rotation_matrix(t.get("rotation", [0, 0, 0]))
+        scale = np.asarray(t.get("scale", [1, 1, 1]), np.float64)
+        if len(i) and not np.all(scale == scale[0]):
+            _check_rotated_normals(p, i, wp, r, e.get("name", shape["file"]))
+        wn = n.astype(np.float64) @ r
+        length = np.sqrt((wn * wn).sum(1, keepdims=True))
+        wn = np.where(length > 0, wn / np.where(length > 0, length, 1), 0)
+        nrms.append(wn.astype(np.float32))
         pos.append(wp)
         idx.append(i + nv)
         tmat.append(np.asarray(slots, np.uint32)[part])
         nv += len(wp)
     if not idx:
         raise ValueError(f"{path}: no mesh entities")
-    return (np.concatenate(pos), np.concatenate(idx).astype(np.uint32), np.concatenate(tmat),
-            np.array(mats, MATERIAL_DTYPE))
+    out = (np.concatenate(pos), np.concatenate(idx).astype(np.uint32), np.concatenate(tmat),
+           np.array(mats, MATERIAL_DTYPE))
+    return (*out, np.concatenate(nrms)) if with_normals else out
+
+
+def _check_rotated_normals(p, i, wp, r, name):
+    """refuses a prop whose geometric normals sprout would tilt (the object
+    triangle's normal rotated alone against the world triangle's)"""
+    def unit(v):
+        return v / np.maximum(np.sqrt((v * v).sum(1, keepdims=True)), 1e-30)
+
+    po = p.astype(np.float64)
+    pw = wp.astype(np.float64)
+    no = unit(np.cross(po[i[:, 1]] - po[i[:, 0]], po[i[:, 2]] - po[i[:, 0]])) @ r
+    nw = unit(np.cross(pw[i[:, 1]] - pw[i[:, 0]], pw[i[:, 2]] - pw[i[:, 0]]))
+    if np.abs(no - nw).max() > 1e-5:
+        raise NotImplementedError(
+            f"entity {name}: a non-uniform scale that tilts its normals (sprout rotates "
+            "normals without the scale; bake the scale into the mesh)")
 
 
 class Take:
-    """A take file: the scene arrays, the camera (position, rotation rows and
-    horizontal fov: render.perspective(width, height, **camera)), the
-    sampler, the samples per pixel, the bounces and the sensor's filter."""
+    """A take file: the scene arrays, the vertex shading normals
+    (render.Scene(..., normals=take.normals)), the camera (position,
+    rotation rows and horizontal fov: render.perspective(width, height,
+    **camera)), the sampler, the samples per pixel, the bounces and the
+    sensor's filter."""
 
     def __init__(self, path):
         doc = _read(path)
         base = os.path.dirname(os.path.abspath(path))
-        self.scene = load_scene(os.path.join(base, doc["scene"]))
+        *self.scene, self.normals = load_scene(os.path.join(base, doc["scene"]), with_normals=True)
+        self.scene = tuple(self.scene)
         cam = doc.get("camera", {}).get("Perspective")
         if cam is None:
             raise NotImplementedError(f"camera {sorted(doc.get('camera', {}))}")

@@ -177,3 +177,36 @@ def test_matches_reference_library_on_fresh_cases():
     b = O.substitute_evaluate(params, frames, wo, wi, ref, "sref")
     rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-6)
     assert np.percentile(rel, 99) < 5e-5
+
+
+@pytest.mark.skipif(O.ref_bsdf_lib() is None, reason="reference not built here (oracle/ref.mk)")
+@pytest.mark.parametrize("metallic", [None, 0.0, 1.0])
+def test_matches_reference_library_with_tilted_shading_normals(metallic):
+    """shading normals up to ~35 degrees off the geometric normal, so wo
+    falls below the shading hemisphere in about one case in seven
+    (clamp_abs_n_dot, same_hemisphere on the geometric normal)"""
+    ref = O.ref_bsdf_lib()
+    n = 8192
+    params, frames, wo, normal, rng = random_cases(n, 13, metallic=metallic)
+    ns = normal + 0.35 * rng.normal(size=(n, 3))
+    ns /= np.linalg.norm(ns, axis=1, keepdims=True)
+    for i in range(n):
+        t, b = O.orthonormal_basis(ns[i].astype(np.float32))
+        frames[i, 0:3], frames[i, 3:6] = t, b
+    frames[:, 6:9] = ns
+    assert 0.05 < np.mean((ns * wo).sum(1) < 0) < 0.3
+    wi = rng.normal(size=(n, 3))
+    wi = (wi / np.linalg.norm(wi, axis=1, keepdims=True)).astype(np.float32)
+    a = O.substitute_evaluate(params, frames, wo, wi)
+    b = O.substitute_evaluate(params, frames, wo, wi, ref, "sref")
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-6)
+    assert np.percentile(rel, 99) < 5e-5 and rel.max() < 1e-3
+    xi = rng.random((n, 3)).astype(np.float32)
+    sa = O.substitute_sample(params, frames, wo, xi)
+    sb = O.substitute_sample(params, frames, wo, xi, ref, "sref")
+    valid = sb[:, 6] > 0
+    assert np.array_equal(sa[:, 6] > 0, valid)
+    w = sa[valid, 3:6] / sa[valid, 6:7]
+    wr = sb[valid, 3:6] / sb[valid, 6:7]
+    rel = np.abs(w - wr) / np.maximum(np.abs(wr), 1e-6)
+    assert np.percentile(rel, 99.9) < 1e-3 and np.median(rel) < 1e-5

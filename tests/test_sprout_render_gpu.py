@@ -1,0 +1,115 @@
+"""The GPU path tracer with shading normals (mi_scene_set_normals) against
+the reference's own renderer and against the oracle.
+
+* In expectation: the takes of tests/golden/take_fixture and
+  tests/golden/take_smooth rendered at 16 384 spp against the reference's
+  65 536-spp renders (tests/golden/sprout_renders.npz, made by
+  tests/golden/make_sprout_renders.py): image means within 0.5 %, 16 x 16
+  blocks within 1 % (small) and 2.5 % (smooth; its residual sits on the
+  sphere's silhouette, see tests/test_sprout_render.py).
+* Under the same random numbers: the smooth take against the oracle.
+* The entry point: normals of the wrong count or without direction are
+  refused, and None goes back to the geometric normal.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from scene_util import light_tree
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+TAKES = {"small": os.path.join(GOLDEN, "take_fixture", "small.take"),
+         "smooth": os.path.join(GOLDEN, "take_smooth", "smooth.take")}
+TOLERANCE = {"small": (0.005, 0.01), "smooth": (0.005, 0.025)}
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+
+    from sprout_amd import bvh, render
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    bvh.init(0)
+    yield render
+    torch.cuda.synchronize()
+
+
+def blocks(img, b):
+    h, w, _ = img.shape
+    return img[:h // b * b, :w // b * b].reshape(h // b, b, w // b, b, 3).mean(axis=(1, 3))
+
+
+def gpu_image(render, t, spp, seed, normals=True):
+    import torch
+
+    pos, idx, tmat, mats = t.scene
+    sc = render.Scene(pos, idx, tmat, mats, normals=t.normals if normals else None)
+    cam = render.perspective(t.width, t.height, **t.camera)
+    acc, _ = render.render(sc, cam, spp, samples_per_wave=min(spp, 256), seed=seed,
+                           min_bounces=t.min_bounces, max_bounces=t.max_bounces)
+    torch.cuda.synchronize()
+    sc.release()
+    return acc.cpu().numpy() / spp
+
+
+@pytest.mark.parametrize("name", sorted(TAKES))
+def test_gpu_converges_to_sprouts_render(gpu, name):
+    from sprout_amd import take
+
+    with np.load(os.path.join(GOLDEN, "sprout_renders.npz"), allow_pickle=False) as z:
+        ref = z[name]
+    t = take.Take(TAKES[name])
+    print(f"take {name}: {len(t.scene[1])} triangles, 16384 spp on the GPU", flush=True)
+    img = gpu_image(gpu, t, 16384, seed=19)
+    mean = np.abs(img.mean(axis=(0, 1)) / ref.mean(axis=(0, 1)) - 1).max()
+    rb, ib = blocks(ref, 16), blocks(img, 16)
+    block = (np.abs(ib - rb) / np.maximum(rb, 2e-3)).max()
+    tol_mean, tol_block = TOLERANCE[name]
+    assert mean < tol_mean, mean
+    assert block < tol_block, block
+    # the geometric normal instead: far off
+    flat = gpu_image(gpu, t, 4096, seed=19, normals=False)
+    fb = blocks(flat, 16)
+    assert (np.abs(fb - rb) / np.maximum(rb, 2e-3)).max() > 4 * tol_block
+
+
+def test_smooth_take_equals_the_oracle(gpu):
+    """same seeds: the interpolated normals and the frames built from them
+    agree with the oracle's up to the device's float rounding"""
+    from sprout_amd import take
+
+    t = take.Take(TAKES["smooth"])
+    pos, idx, tmat, mats = t.scene
+    spp = 64
+    img = gpu_image(gpu, t, spp, seed=3)
+    ref = O.render(pos, idx, tmat, mats, t.width, t.height, spp, t.camera,
+                   light_tree(pos, idx, tmat, mats), seed=3, min_bounces=t.min_bounces,
+                   max_bounces=t.max_bounces, normals=t.normals) / spp
+    assert img.mean() > 1e-3
+    diff = np.abs(img - ref) / np.maximum(np.abs(ref), 1e-3)
+    assert np.mean(diff < 1e-4) > 0.95, np.mean(diff < 1e-4)
+    assert np.mean((img - ref) ** 2 / (ref * ref + 1e-2)) < 1e-5
+
+
+def test_set_normals_arguments(gpu):
+    from sprout_amd import _lib, take
+
+    t = take.Take(TAKES["small"])
+    pos, idx, tmat, mats = t.scene
+    sc = gpu.Scene(pos, idx, tmat, mats)
+    with pytest.raises(ValueError):
+        sc.set_normals(t.normals[:-1])
+    bad = t.normals.copy()
+    bad[1] = 0
+    with pytest.raises(_lib.MiError):
+        sc.set_normals(bad)
+    sc.set_normals(t.normals)
+    sc.set_normals(None)
+    sc.release()

@@ -42,7 +42,7 @@ def test_take_fixture():
     assert len(idx) == 5 and len(mats) == 3
     assert mats["kind"].tolist() == [scenes.SUBSTITUTE, scenes.SUBSTITUTE, scenes.LIGHT]
     assert tmat.tolist() == [0, 0, 1, 2, 2]
-    # the floor is scaled by 2 in x and z: area 16 from 4
+    # the floor is 4 x 4
     a, b, c = (pos[idx[:2, k]] for k in range(3))
     area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1).sum()
     assert np.isclose(area, 16)
@@ -212,3 +212,49 @@ def test_binary_mesh_rejects_bad_files(tmp_path):
     bad.write_bytes(src[:-4])  # index data cut short
     with pytest.raises(ValueError):
         take.load_mesh(str(bad))
+
+
+def test_vertex_normals_as_sprout_gives_them():
+    """a JSON mesh without normals: each triangle writes its geometric normal
+    to its vertices in order, the last one stays (the floor's two back
+    corners take the metallic triangle's normal); explicit normals are read
+    and rotated without the scale"""
+    from sprout_amd import take
+
+    pos, idx, part, nrm = take.load_mesh(os.path.join(FIX, "plane.json"), with_normals=True)
+    up = np.array([0, 1, 0], np.float32)
+    assert np.allclose(nrm[[2, 3]], up)
+    a, b, c = pos[idx[2]]
+    g = np.cross(b - a, c - a)
+    g /= np.linalg.norm(g)
+    assert np.allclose(nrm[[0, 1, 4]], g, atol=1e-6)
+    t = take.Take(os.path.join(HERE, "golden", "take_smooth", "smooth.take"))
+    pos, idx, tmat, mats = t.scene
+    assert t.normals.shape == pos.shape
+    assert np.allclose(np.linalg.norm(t.normals, axis=1), 1, atol=1e-5)
+    # the sphere (centre 0, 0.7, 0.4, radius 0.7): normals point away from the centre
+    sphere = np.isin(np.arange(len(pos)), idx[(tmat == 1) | (tmat == 2)])
+    radial = pos[sphere] - np.array([0, 0.7, 0.4], np.float32)
+    radial /= np.linalg.norm(radial, axis=1, keepdims=True)
+    assert np.allclose(t.normals[sphere], radial, atol=1e-4)
+
+
+def test_non_uniform_scale_that_tilts_normals_is_refused(tmp_path):
+    """sprout rotates normals without the scale (triangle_mesh.cpp:388-421):
+    a tilted triangle under a non-uniform scale would shade differently, so
+    it is refused; the lamp quad's scale (0.5, 0.5, 1) keeps its normal"""
+    import json
+    import shutil
+
+    from sprout_amd import take
+
+    for f in os.listdir(FIX):
+        shutil.copy(os.path.join(FIX, f), tmp_path / f)
+    doc = json.loads((tmp_path / "small.scene").read_text())
+    doc["entities"][0]["transformation"]["scale"] = [2, 1, 2]
+    (tmp_path / "small.scene").write_text(json.dumps(doc))
+    with pytest.raises(NotImplementedError, match="non-uniform scale"):
+        take.load_scene(str(tmp_path / "small.scene"))
+    doc["entities"][0]["transformation"]["scale"] = [2, 2, 2]
+    (tmp_path / "small.scene").write_text(json.dumps(doc))
+    take.load_scene(str(tmp_path / "small.scene"))

@@ -18,8 +18,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 @pytest.mark.parametrize("golden_ratio", [False, True])
 def test_take_renders_as_the_oracle(golden_ratio):
-    """The take's Perspective camera, colours mapped to ACEScg, the Gaussian
-    sensor; pixel samples from the random or the Golden_ratio sampler."""
+    """The take's Perspective camera, colours mapped to ACEScg, sprout's
+    vertex normals (the last geometric normal written to each vertex of a
+    JSON mesh without normals), the Gaussian sensor; pixel samples from the
+    random or the Golden_ratio sampler."""
     import torch
 
     from sprout_amd import bvh, render, take
@@ -30,7 +32,7 @@ def test_take_renders_as_the_oracle(golden_ratio):
     pos, idx, tmat, mats = t.scene
     print(f"take fixture: {len(idx)} triangles, {t.width}x{t.height}, {t.samples_per_pixel} spp, "
           f"golden ratio {golden_ratio}", flush=True)
-    sc = render.Scene(pos, idx, tmat, mats)
+    sc = render.Scene(pos, idx, tmat, mats, normals=t.normals)
     cam = render.perspective(t.width, t.height, **t.camera)
     spp = t.samples_per_pixel
     acc, _ = render.render_filtered(sc, cam, spp, render.sensor(), samples_per_wave=8, seed=3,
@@ -42,7 +44,8 @@ def test_take_renders_as_the_oracle(golden_ratio):
     f = O.Filter(O.FILTER_GAUSSIAN, 2.0)
     s = O.render(pos, idx, tmat, mats, t.width, t.height, spp, t.camera,
                  light_tree(pos, idx, tmat, mats), seed=3, min_bounces=t.min_bounces,
-                 max_bounces=t.max_bounces, border=2, samples=True, golden_ratio=golden_ratio)
+                 max_bounces=t.max_bounces, border=2, samples=True, golden_ratio=golden_ratio,
+                 normals=t.normals)
     ref = O.filtered_sums(s, t.width, t.height, f)
     img = acc[..., 0:3] / acc[..., 3:4]
     rimg = ref[..., 0:3] / ref[..., 3:4]
