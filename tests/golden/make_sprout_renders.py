@@ -15,7 +15,13 @@ its own pixel, as the oracle's and the GPU's plain renders count it).  The
 image is the resolved sensor before any postprocessor, in linear ACEScg.
 
 It also writes take_smooth/sphere.json, the mesh of the second take: a UV
-sphere with explicit unit normals in two parts (upper and lower half).
+sphere with explicit unit normals in two parts (upper and lower half), and
+take_room/, the box room of the path-tracer benchmark (sprout_amd.scenes.room)
+in sprout's formats: its walls as one prop with a part per material (the
+benchmark's Lambert walls, a material sprout lacks, as Substitutes of
+roughness 1), its ceiling light as a Light entity, the benchmark's camera as a Perspective
+camera (looking down -z: rotation y 180 degrees; the vertical fov of 40
+degrees is also the horizontal one of a square image), 64 x 64.
 """
 
 import json
@@ -32,7 +38,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 BIN = os.path.join(ROOT, "oracle", "_ref", "sprout_render")
 SPP = 65536
 TAKES = {"small": os.path.join(HERE, "take_fixture", "small.take"),
-         "smooth": os.path.join(HERE, "take_smooth", "smooth.take")}
+         "smooth": os.path.join(HERE, "take_smooth", "smooth.take"),
+         "room": os.path.join(HERE, "take_room", "room.take")}
 
 
 def sphere_mesh(segments=12, rings=6, radius=0.7):
@@ -73,6 +80,67 @@ def write_sphere():
         json.dump(doc, f)
 
 
+def write_room():
+    sys.path.insert(0, ROOT)
+    from sprout_amd import scenes
+
+    pos, idx, tmat, mats = scenes.room()
+    out = os.path.join(HERE, "take_room")
+    os.makedirs(out, exist_ok=True)
+    light = mats["kind"][tmat] == scenes.LIGHT
+
+    def mesh(tris, parts_of):
+        used = np.unique(idx[tris])
+        remap = np.full(len(pos), -1, np.int64)
+        remap[used] = np.arange(len(used))
+        order = np.argsort(parts_of, kind="stable")
+        parts = []
+        for m in np.unique(parts_of):
+            first = int(np.searchsorted(parts_of[order], m))
+            count = int((parts_of == m).sum())
+            parts.append({"material_index": int(m), "start_index": 3 * first,
+                          "num_indices": 3 * count})
+        return {"geometry": {"parts": parts, "primitive_topology": "triangle_list",
+                             "vertices": {"positions": [round(float(c), 6) for c in
+                                                        pos[used].reshape(-1)]},
+                             "indices": remap[idx[tris][order]].reshape(-1).tolist()}}
+
+    walls = np.nonzero(~light)[0]
+    kinds = tmat[walls]
+    slots = sorted(set(kinds.tolist()))
+    with open(os.path.join(out, "walls.json"), "w") as f:
+        json.dump(mesh(walls, np.array([slots.index(k) for k in kinds])), f)
+    with open(os.path.join(out, "light.json"), "w") as f:
+        json.dump(mesh(np.nonzero(light)[0], np.zeros(int(light.sum()), np.int64)), f)
+    materials = []
+    for k, m in enumerate(mats):
+        if m["kind"] == scenes.LIGHT:
+            materials.append({"name": f"m{k}", "rendering": {"Light": {
+                "emission": [float(c) for c in m["emission"]]}}})
+        else:
+            materials.append({"name": f"m{k}", "rendering": {"Substitute": {
+                "color": [float(c) for c in m["color"]], "roughness": float(m["roughness"]),
+                "metallic": float(m["metallic"]), "ior": float(m["ior"])}}})
+    lamp = int(np.unique(tmat[light])[0])
+    scene = {"materials": materials,
+             "entities": [{"type": "Prop", "shape": {"file": "walls.json"},
+                           "materials": [f"m{k}" for k in slots]},
+                          {"type": "Light", "shape": {"file": "light.json"},
+                           "materials": [f"m{lamp}"]}]}
+    with open(os.path.join(out, "room.scene"), "w") as f:
+        json.dump(scene, f, indent=1)
+    cam = scenes.ROOM_CAMERA
+    take = {"scene": "room.scene",
+            "camera": {"Perspective": {
+                "parameters": {"fov": cam["fov_deg"]},
+                "transformation": {"position": list(cam["eye"]), "rotation": [0, 180, 0]},
+                "sensor": {"resolution": [64, 64]}}},
+            "sampler": {"Random": {"samples_per_pixel": 16}},
+            "integrator": {"surface": {"PTMIS": {"min_bounces": 4, "max_bounces": 8}}}}
+    with open(os.path.join(out, "room.take"), "w") as f:
+        json.dump(take, f, indent=1)
+
+
 def render(name, path):
     with open(path) as f:
         take = json.load(f)
@@ -94,6 +162,7 @@ def main() -> int:
     subprocess.run(["make", "-f", os.path.join(ROOT, "oracle", "ref_render.mk"), "-j8"], cwd=ROOT,
                    check=True)
     write_sphere()
+    write_room()
     images = {name: render(name, path) for name, path in TAKES.items()}
     np.savez_compressed(os.path.join(HERE, "sprout_renders.npz"), spp=np.int64(SPP), **images)
     return 0

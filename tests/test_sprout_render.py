@@ -13,13 +13,16 @@ with other random numbers, so the images agree in expectation:
   geometric normal written to each vertex (triangle_mesh_provider.cpp:
   103-118), which darkens the floor along the shared edge;
 * smooth: a 12 x 6 UV sphere with explicit vertex normals in two parts
-  (copper metal, blue dielectric) on a floor.
+  (copper metal, blue dielectric) on a floor;
+* room: the path-tracer benchmark's box room (its walls as Substitutes of
+  roughness 1, its ceiling light) from the benchmark's camera, 64 x 64.
 
-Each check is also run with the geometric normal instead of the shading
-normal, which it must fail: the shading normal is what the images pin.
+Each check is also run with a planted mistake, which it must fail: the
+geometric normal instead of the shading normal (small, smooth), the colours
+without sprout's sRGB -> ACEScg mapping (room).
 Measured at 4096 spp: image means within 0.1 %, 16 x 16-pixel blocks within
-0.4 % (small) and 1.3 % (smooth).  The smooth take's residual sits on the
-sphere's silhouette in shadow, where sprout's one-bounce indirect light is
+0.4 % (small), 1.3 % (smooth) and 0.2 % (room).  The smooth take's residual
+sits on the sphere's silhouette in shadow, where sprout's one-bounce indirect light is
 lower than the oracle's (whose MIS, BSDF-only and light-only estimators agree
 there); with flat normals the same pixels agree.
 """
@@ -35,10 +38,11 @@ from scene_util import light_tree
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
 TAKES = {"small": os.path.join(GOLDEN, "take_fixture", "small.take"),
-         "smooth": os.path.join(GOLDEN, "take_smooth", "smooth.take")}
+         "smooth": os.path.join(GOLDEN, "take_smooth", "smooth.take"),
+         "room": os.path.join(GOLDEN, "take_room", "room.take")}
 SPP = 4096
 # (image mean, 16 x 16 block) relative tolerances per take
-TOLERANCE = {"small": (0.005, 0.01), "smooth": (0.005, 0.025)}
+TOLERANCE = {"small": (0.005, 0.01), "smooth": (0.005, 0.025), "room": (0.005, 0.01)}
 
 
 @pytest.fixture(scope="module")
@@ -59,11 +63,22 @@ def errors(img, ref):
     return mean, block
 
 
-def oracle_image(name, normals=True):
+def oracle_image(name, normals=True, srgb=False):
+    """srgb: the scene's colours as written, not mapped to ACEScg"""
+    import json
+
     from sprout_amd import take
 
     t = take.Take(TAKES[name])
     pos, idx, tmat, mats = t.scene
+    if srgb:
+        mats = mats.copy()
+        with open(os.path.join(os.path.dirname(TAKES[name]), "room.scene")) as f:
+            doc = json.load(f)
+        for k, m in enumerate(doc["materials"]):
+            kind, node = next(iter(m["rendering"].items()))
+            key = "emission" if kind == "Light" else "color"
+            mats[key][k] = node[key]
     acc = O.render(pos, idx, tmat, mats, t.width, t.height, SPP, t.camera,
                    light_tree(pos, idx, tmat, mats), seed=7, min_bounces=t.min_bounces,
                    max_bounces=t.max_bounces, normals=t.normals if normals else None)
@@ -73,7 +88,7 @@ def oracle_image(name, normals=True):
 @pytest.mark.parametrize("name", sorted(TAKES))
 def test_oracle_converges_to_sprouts_render(sprout_images, name):
     ref = sprout_images[name]
-    assert ref.shape == (48, 64, 3) and int(sprout_images["spp"]) == 65536
+    assert ref.shape[2] == 3 and int(sprout_images["spp"]) == 65536
     mean, block = errors(oracle_image(name), ref)
     tol_mean, tol_block = TOLERANCE[name]
     assert mean < tol_mean, mean
@@ -81,8 +96,10 @@ def test_oracle_converges_to_sprouts_render(sprout_images, name):
 
 
 @pytest.mark.parametrize("name", sorted(TAKES))
-def test_geometric_normals_fail_the_check(sprout_images, name):
-    """the planted mistake: shading with the geometric normal"""
-    mean, block = errors(oracle_image(name, normals=False), sprout_images[name])
+def test_planted_mistakes_fail_the_check(sprout_images, name):
+    """shading with the geometric normal (small, smooth), colours without
+    the ACEScg mapping (room, whose flat walls need no shading normals)"""
+    img = oracle_image(name, srgb=True) if name == "room" else oracle_image(name, normals=False)
+    mean, block = errors(img, sprout_images[name])
     tol_mean, tol_block = TOLERANCE[name]
     assert mean > 4 * tol_mean or block > 4 * tol_block, (mean, block)
