@@ -285,6 +285,70 @@ def cloud_render(frames: int, cpu: bool):
     return out
 
 
+def room_take_render(frames: int, cpu: bool, threads: int):
+    """The box room as a sprout take (tests/golden/take_room: the walls as
+    Substitutes of roughness 1, the ceiling light, the benchmark camera) at
+    256 x 256, 16 spp per frame, Random sampler, no sensor filter, min / max
+    bounces 4 / 8: the GPU against the reference's own renderer (oracle/_ref/
+    sprout_render, compiled from its sources) on the host's threads, on a
+    bounded sample of the same frames (128 spp of 256 x 256)."""
+    import json
+    import subprocess
+    import tempfile
+
+    import torch
+
+    from sprout_amd import render, take
+
+    here = os.path.join(ROOT, "tests", "golden", "take_room")
+    t = take.Take(os.path.join(here, "room.take"))
+    pos, idx, tmat, mats = t.scene
+    sc = render.Scene(pos, idx, tmat, mats, normals=t.normals)
+    cam = render.perspective(256, 256, **t.camera)
+    acc = torch.zeros((256, 256, 3), dtype=torch.float64, device="cuda")
+    kw = dict(samples_per_wave=16, seed=1, min_bounces=t.min_bounces, max_bounces=t.max_bounces)
+    render.render(sc, cam, 16, accum=acc, **kw)
+    torch.cuda.synchronize()
+    paths = 0.0
+    t0 = time.perf_counter()
+    for f in range(frames):
+        _, st = render.render(sc, cam, 16, first_sample=16 * (f + 1), accum=acc, stats=True, **kw)
+        paths += st.paths
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sc.release()
+    out = {"workload": "box room take 256x256 (Substitute walls), 16 spp per frame, "
+                       "min/max bounces 4/8, MIS, shading normals",
+           "ms_per_frame": round(wall / frames * 1e3, 3),
+           "samples_per_s": round(paths / wall, 1)}
+    if not cpu:
+        return out
+    for arch in ("x86_64_v3", None):
+        exe = os.path.join(ROOT, "oracle", "_ref", "sprout_render" + (f"_{arch}" if arch else ""))
+        if os.path.exists(exe):
+            break
+    else:
+        out["cpu_reference"] = {"skipped": "oracle/_ref/sprout_render not built here"}
+        return out
+    with open(os.path.join(here, "room.take")) as f:
+        doc = json.load(f)
+    spp = 128
+    doc["sampler"] = {"Random": {"samples_per_pixel": spp}}
+    doc["camera"]["Perspective"]["sensor"] = {"resolution": [256, 256]}
+    with tempfile.TemporaryDirectory() as tmp:
+        t0 = time.perf_counter()
+        subprocess.run([exe, here, json.dumps(doc), str(threads), os.path.join(tmp, "img.f32")],
+                       check=True, capture_output=True)
+        secs = time.perf_counter() - t0
+    out["cpu_reference"] = {
+        "samples_per_s": round(256 * 256 * spp / secs, 1), "cores": threads, "kind": "reference",
+        "sample": f"{spp} spp of 256x256 by sprout's own renderer ({os.path.basename(exe)}: "
+                  f"take and scene loading, Pathtracer_MIS, sensor) on {threads} host threads, "
+                  f"{secs:.1f} s wall including loading"}
+    out["gpu_over_cpu_reference"] = round(out["samples_per_s"] / out["cpu_reference"]["samples_per_s"], 1)
+    return out
+
+
 def cloud_guided_equal_time(seconds: float = 1.5, ref_spp: int = 65536):
     """The fBm cloud as configured (flat majorants, hero-wavelength
     tracking), at 256 x 256: plain spectral MIS against guided spectral MIS
@@ -643,7 +707,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--guide-seconds", type=float, default=6.0,
                     help="time budget of each side of the guided vs plain comparison")
-    ap.add_argument("--part", choices=["all", "room", "volume", "volume_guiding", "guiding",
+    ap.add_argument("--part", choices=["all", "room", "room_take", "volume", "volume_guiding",
+                                       "guiding",
                                        "instances", "batch",
                                        "batch_instanced", "guide_kernels"],
                     default="all",
@@ -681,6 +746,7 @@ def main() -> int:
     if args.part != "all":
         parts = {"room": lambda: room_render(5, False), "volume": lambda: cloud_render(3, False),
                  "volume_guiding": cloud_guided_equal_time,
+                 "room_take": lambda: room_take_render(5, not args.no_cpu_baseline, host_threads()),
                  "batch": lambda: batch_render(2, False),
                  "batch_instanced": lambda: batch_render(2, True),
                  "guide_kernels": guide_kernels,
@@ -814,6 +880,8 @@ def main() -> int:
         return 0
     log("box room path tracing")
     line["extra"]["room_render"] = room_render(5, not args.no_cpu_baseline)
+    log("box room take: GPU against sprout's own renderer")
+    line["extra"]["room_take"] = room_take_render(5, not args.no_cpu_baseline, host_threads())
     log("batched path tracing")
     line["extra"]["batch_render"] = batch_render(2, False)
     line["extra"]["batch_render_instanced"] = batch_render(2, True)

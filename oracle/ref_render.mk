@@ -7,19 +7,27 @@
 # Flags as oracle/ref.mk (the reference's Release
 # configuration on Linux).
 #
-#   make -f oracle/ref_render.mk -j8
+#   make -f oracle/ref_render.mk -j8                  # native build (golden renders)
+#   make -f oracle/ref_render.mk -j8 ARCH=x86-64-v3   # portable build (CPU baseline on other hosts)
 #
 # Only this container has the reference; the render fixtures it makes are
 # committed under tests/golden/ (tests/golden/make_sprout_renders.py).
 
 REF   ?= /root/reference
+ARCH  ?= native
 HERE  := $(dir $(lastword $(MAKEFILE_LIST)))
 OUT   := $(HERE)_ref
-OBJ   := $(OUT)/render_obj
 SRC   := $(REF)/source
-BIN   := $(OUT)/sprout_render
 
-CXXFLAGS := -std=c++20 -O3 -DNDEBUG -march=native -fno-rtti -fno-exceptions \
+ifeq ($(ARCH),native)
+OBJ   := $(OUT)/render_obj
+BIN   := $(OUT)/sprout_render
+else
+OBJ   := $(OUT)/render_obj_$(subst -,_,$(ARCH))
+BIN   := $(OUT)/sprout_render_$(subst -,_,$(ARCH))
+endif
+
+CXXFLAGS := -std=c++20 -O3 -DNDEBUG -march=$(ARCH) -fno-rtti -fno-exceptions \
             -DSU_ACESCG -DSU_IBL_MIS_COMPENSATION -DSU_ZSTD -w \
             -I$(SRC)/core -I$(SRC) -I$(SRC)/base -I$(REF)/thirdparty/include
 
