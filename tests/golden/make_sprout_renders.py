@@ -11,8 +11,9 @@ repository root, in the container that holds the reference):
 
 Each take is rendered as its file says, except: the sampler is Random with
 SPP samples per pixel, and the sensor has no filter (each sample counts in
-its own pixel, as the oracle's and the GPU's plain renders count it).  The
-image is the resolved sensor before any postprocessor, in linear ACEScg.
+its own pixel, as the oracle's and the GPU's plain renders count it);
+"small_filtered" keeps the take's Gaussian filter (radius 2, alpha 1.8).
+The image is the resolved sensor before any postprocessor, in linear ACEScg.
 
 It also writes take_smooth/sphere.json, the mesh of the second take: a UV
 sphere with explicit unit normals in two parts (upper and lower half), and
@@ -39,7 +40,8 @@ BIN = os.path.join(ROOT, "oracle", "_ref", "sprout_render")
 SPP = 65536
 TAKES = {"small": os.path.join(HERE, "take_fixture", "small.take"),
          "smooth": os.path.join(HERE, "take_smooth", "smooth.take"),
-         "room": os.path.join(HERE, "take_room", "room.take")}
+         "room": os.path.join(HERE, "take_room", "room.take"),
+         "small_filtered": os.path.join(HERE, "take_fixture", "small.take")}
 
 
 def sphere_mesh(segments=12, rings=6, radius=0.7):
@@ -146,7 +148,8 @@ def render(name, path):
         take = json.load(f)
     take["sampler"] = {"Random": {"samples_per_pixel": SPP}}
     sensor = take["camera"]["Perspective"].setdefault("sensor", {})
-    sensor.pop("filter", None)
+    if not name.endswith("_filtered"):
+        sensor.pop("filter", None)
     with tempfile.TemporaryDirectory() as tmp:
         out = os.path.join(tmp, "image.f32")
         subprocess.run([BIN, os.path.dirname(path), json.dumps(take), str(os.cpu_count() or 1),

@@ -82,6 +82,35 @@ def test_gpu_converges_to_sprouts_render(gpu, name):
         assert (np.abs(fb - rb) / np.maximum(rb, 2e-3)).max() > 4 * tol_block
 
 
+def test_filtered_sensor_converges_to_sprouts_render(gpu):
+    """the fixture take through its Gaussian sensor at 16 384 spp against the
+    reference's Filtered_2p0 sensor (see tests/test_sprout_render.py for the
+    top-edge rows beside the lamp)"""
+    import torch
+
+    from sprout_amd import take
+
+    with np.load(os.path.join(GOLDEN, "sprout_renders.npz"), allow_pickle=False) as z:
+        ref = z["small_filtered"]
+    t = take.Take(TAKES["small"])
+    pos, idx, tmat, mats = t.scene
+    sc = gpu.Scene(pos, idx, tmat, mats, normals=t.normals)
+    cam = gpu.perspective(t.width, t.height, **t.camera)
+    acc, _ = gpu.render_filtered(sc, cam, 16384, gpu.sensor(), samples_per_wave=256, seed=23,
+                                 min_bounces=t.min_bounces, max_bounces=t.max_bounces)
+    torch.cuda.synchronize()
+    sc.release()
+    acc = acc.cpu().numpy()
+    img = acc[..., :3] / acc[..., 3:4]
+    mean = np.abs(img.mean(axis=(0, 1)) / ref.mean(axis=(0, 1)) - 1).max()
+    rb, ib = blocks(ref, 16), blocks(img, 16)
+    block = (np.abs(ib - rb) / np.maximum(rb, 2e-3)).max()
+    assert mean < 0.006, mean
+    assert block < 0.015, block
+    inner = img[4:-4, 4:-4].mean() / ref[4:-4, 4:-4].mean() - 1
+    assert abs(inner) < 0.002, inner
+
+
 def test_smooth_take_equals_the_oracle(gpu):
     """same seeds: the interpolated normals and the frames built from them
     agree with the oracle's up to the device's float rounding"""
