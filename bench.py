@@ -243,8 +243,15 @@ def cloud_render(frames: int, cpu: bool):
         "ms_per_frame": round(wall / frames * 1e3, 3),
         "samples_per_s": round(paths / wall, 1),
         "rays_per_sample": round(rays / paths, 3),
-        "seconds_for_256_spp": round(w * w * 256 / (paths / wall), 2),
     }
+    # the configuration's whole render, timed: 256 spp of the image in
+    # 4-spp waves
+    acc.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    render.render(sc, cam, 256, samples_per_wave=spp, seed=2, accum=acc, spectral=True)
+    torch.cuda.synchronize()
+    out["seconds_for_256_spp"] = round(time.perf_counter() - t0, 3)
     out["rgb_flat_majorants_ms_per_frame"] = round(timed(False)[0] / frames * 1e3, 3)
     set_grid("gridtree")
     out["rgb_gridtree_ms_per_frame"] = round(timed(False)[0] / frames * 1e3, 3)
@@ -445,6 +452,15 @@ def batch_render(frames: int, instanced: bool):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     extra = {}
+    # the configuration's whole render, timed: 64 spp of 1080p, frame by frame
+    acc.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(16):
+        render.render(sc, cam, 4, samples_per_wave=1, min_bounces=3, max_bounces=16, seed=3,
+                      first_sample=4 * f, accum=acc)
+    torch.cuda.synchronize()
+    extra["seconds_for_64_spp"] = round(time.perf_counter() - t0, 3)
     # the same frames with regenerating waves (a 1-spp pool of path slots
     # refilled from the frame's 4 spp)
     render.render(sc, cam, 4, samples_per_wave=1, min_bounces=3, max_bounces=16, seed=1, accum=acc,
@@ -503,7 +519,6 @@ def batch_render(frames: int, instanced: bool):
         "samples_per_s": round(paths / wall, 1),
         "grays_all_rays": round(rays / wall / 1e9, 4),
         "rays_per_sample": round(rays / paths, 3),
-        "seconds_for_64_spp": round(1920 * 1080 * 64 / (paths / wall), 2),
     }
 
 
