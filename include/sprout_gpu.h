@@ -559,8 +559,18 @@ int32_t mi_scene_light_tree(uint32_t scene, mi_light_node* nodes, uint32_t* midd
  * normal written to it, triangle_mesh_provider.cpp:103-118); without this
  * call a scene shades with the geometric normal.  normals NULL goes back to
  * that.  Moving the vertices (mi_scene_update_positions) keeps the normals.
- * Mesh instances shade with their geometric normal. */
+ * In an instanced scene it covers the world triangles; meshes take theirs
+ * through mi_scene_set_mesh_normals. */
 int32_t mi_scene_set_normals(uint32_t scene, float const* normals, uint32_t num_vertices);
+
+/* The same for mesh `mesh` of an instanced scene (mi_scene_create_instanced):
+ * its vertex normals in object space (HOST, 3 per vertex), interpolated at a
+ * hit in object space and moved to world space by the instance's
+ * object-to-world matrix, as the geometric normal is (sprout turns both by
+ * the prop's rotation, triangle_mesh.cpp:388-421: the same under uniform
+ * scale).  normals NULL goes back to the geometric normal. */
+int32_t mi_scene_set_mesh_normals(uint32_t scene, uint32_t mesh, float const* normals,
+                                  uint32_t num_vertices);
 
 /* ---- participating medium ---- */
 

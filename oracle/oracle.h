@@ -362,6 +362,8 @@ typedef struct {
     float const*            o2w;      /* 16 per instance */
     uint32_t const*         material; /* per instance */
     uint32_t                world;
+    float const* const*     normals;  /* per mesh: shading normals per vertex, or NULL
+                                         (the array itself may be NULL) */
 } oracle_pt_instances;
 
 typedef struct {

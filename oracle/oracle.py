@@ -919,17 +919,19 @@ def offset_f(t) -> float:
 
 class _OraclePtInstances(ctypes.Structure):
     _fields_ = [("tlas", ctypes.POINTER(_OracleInstances)), ("o2w", ctypes.c_void_p),
-                ("material", ctypes.c_void_p), ("world", ctypes.c_uint32)]
+                ("material", ctypes.c_void_p), ("world", ctypes.c_uint32),
+                ("normals", ctypes.c_void_p)]
 
 
 class PtInstances:
     """An instanced scene for render(): meshes [(positions, indices)] placed
     by mesh_of / world_to_object / object_to_world with one material each,
     plus the render's own triangles (positions, indices) as the last instance
-    with identity matrices; every tree is the oracle's SBVH."""
+    with identity matrices; every tree is the oracle's SBVH.  mesh_normals:
+    per mesh its vertex shading normals or None (None: none at all)."""
 
     def __init__(self, meshes, mesh_of, world_to_object, object_to_world, materials, positions,
-                 indices):
+                 indices, mesh_normals=None):
         trees = [(p, i, build(p, i)) for p, i in meshes]
         trees.append((positions, indices, build(positions, indices)))
         n = len(mesh_of)
@@ -939,8 +941,18 @@ class PtInstances:
                               np.vstack([np.asarray(object_to_world, np.float32).reshape(-1, 16), eye]))
         self.material = np.append(np.asarray(materials, np.uint32), np.uint32(0xFFFFFFFF))
         self.world = n
+        self._normals = None
+        nptr = None
+        if mesh_normals is not None:
+            self._normals = [None if m is None else np.ascontiguousarray(m, np.float32)
+                             for m in mesh_normals]
+            arr = (ctypes.c_void_p * (len(meshes) + 1))()
+            for k, m in enumerate(self._normals):
+                arr[k] = None if m is None else m.ctypes.data
+            self._normal_ptrs = arr
+            nptr = ctypes.cast(arr, ctypes.c_void_p)
         self.c = _OraclePtInstances(ctypes.pointer(self.inst.c), _ptr(self.inst.o2w),
-                                    _ptr(self.material), n)
+                                    _ptr(self.material), n, nptr)
 
 
 def similarity_scale(g, depth) -> float:

@@ -182,8 +182,9 @@ static int32_t occluded(scene_t const* s, v3 o, v3 d, float max_t, uint32_t* occ
  * triangle_bvh_indexed_data.inl:282-293; a normalized transform of the normal
  * in place of the rotation, equal for similarity transforms). */
 static void instance_point(oracle_pt_instances const* pi, uint32_t inst, uint32_t tri, float hu,
-                           float hv, v3* p, v3* n) {
-    oracle_mesh const* mesh = &pi->tlas->meshes[pi->tlas->mesh_of[inst]];
+                           float hv, v3* p, v3* n, v3* ns) {
+    uint32_t const     mi   = pi->tlas->mesh_of[inst];
+    oracle_mesh const* mesh = &pi->tlas->meshes[mi];
     float const*       pa   = mesh->positions + 3 * (size_t)mesh->indices[3 * (size_t)tri + 0];
     float const*       pb   = mesh->positions + 3 * (size_t)mesh->indices[3 * (size_t)tri + 1];
     float const*       pc   = mesh->positions + 3 * (size_t)mesh->indices[3 * (size_t)tri + 2];
@@ -199,8 +200,23 @@ static void instance_point(oracle_pt_instances const* pi, uint32_t inst, uint32_
     float              pw[3], nw[3];
     oracle_transform_point(m, pi3, pw);
     oracle_transform_vector(m, ni3, nw);
-    *p = mk(pw[0], pw[1], pw[2]);
-    *n = normalize(mk(nw[0], nw[1], nw[2]));
+    *p  = mk(pw[0], pw[1], pw[2]);
+    *n  = normalize(mk(nw[0], nw[1], nw[2]));
+    *ns = *n;
+    float const* nn = pi->normals ? pi->normals[mi] : NULL;
+    if (nn) {
+        /* the mesh's vertex normals interpolated in object space, normalized,
+         * moved to world space as the geometric normal is, normalized again */
+        uint32_t const* ix = mesh->indices + 3 * (size_t)tri;
+        v3 const na = mk(nn[3 * (size_t)ix[0]], nn[3 * (size_t)ix[0] + 1], nn[3 * (size_t)ix[0] + 2]);
+        v3 const nb = mk(nn[3 * (size_t)ix[1]], nn[3 * (size_t)ix[1] + 1], nn[3 * (size_t)ix[1] + 2]);
+        v3 const nc = mk(nn[3 * (size_t)ix[2]], nn[3 * (size_t)ix[2] + 1], nn[3 * (size_t)ix[2] + 2]);
+        v3 const so = normalize(add(add(scale(bw, na), scale(hu, nb)), scale(hv, nc)));
+        float const si3[3] = {so.x, so.y, so.z};
+        float       sw[3];
+        oracle_transform_vector(m, si3, sw);
+        *ns = normalize(mk(sw[0], sw[1], sw[2]));
+    }
 }
 
 #define KIND_PHASE 0x100 /* scattering vertex in the medium */
@@ -445,7 +461,7 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
             oracle_material const*     m;
             uint32_t                   light = SENTINEL;
             if (pi && inst != pi->world) {
-                instance_point(pi, inst, tri, hu, hv, &hit_p, &geo_n);
+                instance_point(pi, inst, tri, hu, hv, &hit_p, &geo_n, &shading_n);
                 m = &s->materials[pi->material[inst]];
             } else {
                 v3 a, b, c;
@@ -456,7 +472,9 @@ static int32_t path(scene_t const* s, uint32_t gx, uint32_t gy, uint32_t sample,
                 m              = &s->materials[s->tri_material[tri]];
                 light          = s->tri_light[tri];
             }
-            shading_n = geo_n;
+            if (!(pi && inst != pi->world)) {
+                shading_n = geo_n;
+            }
             if (set->normals && !(pi && inst != pi->world)) {
                 /* the vertex normals interpolated and normalized
                  * (triangle::interpolate_data, triangle_primitive_mt.inl:274-287) */

@@ -178,6 +178,7 @@ SIGNATURES = {
     "mi_trace_instances_any": (_i32, [_u32, _vp, _u32, _vp, _vp]),
     "mi_instances_download": (_i32, [_u32, _vp, _vp, _vp, _pu32, _pu32]),
     "mi_scene_set_normals": (_i32, [_u32, _vp, _u32]),
+    "mi_scene_set_mesh_normals": (_i32, [_u32, _u32, _vp, _u32]),
     "mi_scene_set_medium": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _vp, _f32]),
     "mi_medium_gridtree": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _pu32, _vp, _pu32]),
     "mi_medium_majorant_tree": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _pu32, _vp, _pu32]),

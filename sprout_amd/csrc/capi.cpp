@@ -1180,6 +1180,18 @@ int32_t mi_scene_set_normals(uint32_t scene, float const* normals, uint32_t num_
     return set_normals(*sc, normals, num_vertices);
 }
 
+int32_t mi_scene_set_mesh_normals(uint32_t scene, uint32_t mesh, float const* normals,
+                                  uint32_t num_vertices) {
+    clear_error();
+    RenderScene* sc = lookup_scene(scene);
+    if (!sc) {
+        return fail(context ? MI_ERR_HANDLE : MI_ERR_NO_CONTEXT,
+                    "mi_scene_set_mesh_normals: bad handle %u", scene);
+    }
+    (void)hipDeviceSynchronize();  // renders in flight may read the old normals
+    return set_mesh_normals(*sc, mesh, normals, num_vertices);
+}
+
 int32_t mi_scene_set_medium(uint32_t scene, float const* density, uint32_t const dims[3],
                             float const box_lo[3], float const box_hi[3], float const cc_a[3],
                             float const cc_s[3], float g) {

@@ -374,6 +374,20 @@ __device__ Surface_point surface_point(Wave const& w, Hit const& h) {
     sp.p             = v3(pw.x, pw.y, pw.z);
     sp.n             = dev::normalize(v3(nw.x, nw.y, nw.z));
     sp.ns            = sp.n;
+    if (g.normals) {
+        // the mesh's vertex normals interpolated in object space, normalized,
+        // then moved to world space as the geometric normal is
+        auto const nv = [&g](uint32_t k) {
+            float const* q = g.normals + 3 * size_t(k);
+            return v3(q[0], q[1], q[2]);
+        };
+        V3 const      na = nv(g.indices[3 * size_t(h.triangle) + 0]);
+        V3 const      nb = nv(g.indices[3 * size_t(h.triangle) + 1]);
+        V3 const      nc = nv(g.indices[3 * size_t(h.triangle) + 2]);
+        V3 const      so = dev::normalize((bw * na + h.u * nb) + h.v * nc);
+        dev::F3 const sw = dev::xform_vector(m, {so.x, so.y, so.z});
+        sp.ns            = dev::normalize(v3(sw.x, sw.y, sw.z));
+    }
     sp.material      = w.inst_material[h.instance];
     sp.light         = dev::Null;
     return sp;
@@ -2077,6 +2091,10 @@ void RenderScene::release() {
     for (DeviceBvh& t : mesh_trees) {
         t.release();
     }
+    for (float* p : mesh_normals) {
+        (void)hipFree(p);
+    }
+    mesh_normals.clear();
     for (void* p : mesh_buffers) {
         (void)hipFree(p);
     }
@@ -2098,6 +2116,69 @@ void RenderScene::release_medium() {
 
 constexpr double Max_optical_depth = 1.0e4;
 
+namespace {
+
+// Unit copies of per-vertex normals on the device (reusing *d when set).
+int32_t upload_unit_normals(float const* normals, uint32_t num_vertices, float** d,
+                            char const* who) {
+    std::vector<float> unit(size_t(num_vertices) * 3);
+    for (uint32_t v = 0; v < num_vertices; ++v) {
+        float const* n  = normals + 3 * size_t(v);
+        float const  l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        if (!(l2 > 0.f) || !std::isfinite(l2)) {
+            return fail(MI_ERR_ARGS, "%s: vertex %u has no direction", who, v);
+        }
+        float const il = 1.f / std::sqrt(l2);
+        for (int k = 0; k < 3; ++k) {
+            unit[3 * size_t(v) + k] = n[k] * il;
+        }
+    }
+    float*     p = *d;
+    hipError_t e = hipSuccess;
+    if (!p) {
+        e = hipMalloc(&p, unit.size() * sizeof(float));
+    }
+    if (!e) e = hipMemcpy(p, unit.data(), unit.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e) {
+        if (p != *d) {
+            (void)hipFree(p);
+        }
+        return fail(MI_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
+    }
+    *d = p;
+    return MI_OK;
+}
+
+}  // namespace
+
+int32_t set_mesh_normals(RenderScene& s, uint32_t mesh, float const* normals,
+                         uint32_t num_vertices) {
+    if (!s.instanced || mesh >= s.mesh_geo_host.size()) {
+        return fail(MI_ERR_ARGS, "mi_scene_set_mesh_normals: mesh %u of %zu", mesh,
+                    s.mesh_geo_host.size());
+    }
+    if (normals && num_vertices != s.mesh_vertices[mesh]) {
+        return fail(MI_ERR_ARGS, "mi_scene_set_mesh_normals: %u normals for %u vertices",
+                    num_vertices, s.mesh_vertices[mesh]);
+    }
+    float* d = s.mesh_normals[mesh];
+    if (normals) {
+        if (int32_t const st = upload_unit_normals(normals, num_vertices, &d,
+                                                   "mi_scene_set_mesh_normals");
+            st) {
+            return st;
+        }
+    } else {
+        (void)hipFree(d);
+        d = nullptr;
+    }
+    s.mesh_normals[mesh]          = d;
+    s.mesh_geo_host[mesh].normals = d;
+    hipError_t const e = hipMemcpy(s.mesh_geo + mesh, &s.mesh_geo_host[mesh], sizeof(Mesh_geometry),
+                                   hipMemcpyHostToDevice);
+    return e ? fail(MI_ERR_HIP, "mi_scene_set_mesh_normals: %s", hipGetErrorString(e)) : MI_OK;
+}
+
 int32_t set_normals(RenderScene& s, float const* normals, uint32_t num_vertices) {
     if (!normals) {
         (void)hipFree(s.normals);
@@ -2110,32 +2191,7 @@ int32_t set_normals(RenderScene& s, float const* normals, uint32_t num_vertices)
     }
     // sprout keeps a vertex's tangent frame as a unit quaternion
     // (triangle_bvh_indexed_data.inl:136-152), so its normals are unit length
-    std::vector<float> unit(size_t(num_vertices) * 3);
-    for (uint32_t v = 0; v < num_vertices; ++v) {
-        float const* n  = normals + 3 * size_t(v);
-        float const  l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-        if (!(l2 > 0.f) || !std::isfinite(l2)) {
-            return fail(MI_ERR_ARGS, "mi_scene_set_normals: vertex %u has no direction", v);
-        }
-        float const il = 1.f / std::sqrt(l2);
-        for (int k = 0; k < 3; ++k) {
-            unit[3 * size_t(v) + k] = n[k] * il;
-        }
-    }
-    float*     d = s.normals;
-    hipError_t e = hipSuccess;
-    if (!d) {
-        e = hipMalloc(&d, unit.size() * sizeof(float));
-    }
-    if (!e) e = hipMemcpy(d, unit.data(), unit.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e) {
-        if (d != s.normals) {
-            (void)hipFree(d);
-        }
-        return fail(MI_ERR_HIP, "mi_scene_set_normals: %s", hipGetErrorString(e));
-    }
-    s.normals = d;
-    return MI_OK;
+    return upload_unit_normals(normals, num_vertices, &s.normals, "mi_scene_set_normals");
 }
 
 int32_t set_medium(RenderScene& scene, float const* density, uint32_t const dims[3],
@@ -2900,7 +2956,7 @@ int32_t create_scene_instanced(mi_mesh const* meshes, uint32_t num_meshes,
         if (e) {
             return failed(fail(MI_ERR_HIP, "mi_scene_create_instanced: mesh %u: %s", k, hipGetErrorString(e)));
         }
-        geo[k] = {pos, idx};
+        geo[k] = {pos, idx, nullptr};
         if (int32_t const st = build_bvh(pos, m.num_vertices, 3, idx, m.num_triangles, settings,
                                          nullptr, s.mesh_trees[k]);
             st) {
@@ -2936,6 +2992,12 @@ int32_t create_scene_instanced(mi_mesh const* meshes, uint32_t num_meshes,
     }
     std::vector<uint32_t> mats(instance_materials, instance_materials + num_instances);
     mats.push_back(dev::Null);
+    s.mesh_geo_host = geo;
+    s.mesh_vertices.resize(num_meshes);
+    s.mesh_normals.assign(num_meshes, nullptr);
+    for (uint32_t k = 0; k < num_meshes; ++k) {
+        s.mesh_vertices[k] = meshes[k].num_vertices;
+    }
     hipError_t e = hipMalloc(&s.mesh_geo, std::max<size_t>(1, num_meshes) * sizeof(Mesh_geometry));
     if (!e) e = hipMalloc(&s.inst_o2w, o2w.size() * sizeof(float));
     if (!e) e = hipMalloc(&s.inst_material, mats.size() * sizeof(uint32_t));

@@ -47,6 +47,7 @@ int32_t make_sensor_filter(mi_sensor const& sensor, Sensor_filter& out);
 struct Mesh_geometry {
     float const*    positions;  // 3 floats per vertex
     uint32_t const* indices;    // 3 per triangle
+    float const*    normals;    // 3 per vertex: shading normals, or null (geometric)
 };
 
 // Device buffers of update_positions: the builders' light inputs.
@@ -112,6 +113,9 @@ struct RenderScene {
     std::vector<DeviceBvh> mesh_trees;
     std::vector<void*>     mesh_buffers;             // device positions / indices per mesh
     Mesh_geometry*         mesh_geo         = nullptr;  // per mesh
+    std::vector<Mesh_geometry> mesh_geo_host;             // its host copy
+    std::vector<uint32_t>  mesh_vertices;                 // vertices per mesh
+    std::vector<float*>    mesh_normals;                  // device shading normals per mesh
     float*                 inst_o2w         = nullptr;  // 16 per instance
     uint32_t*              inst_material    = nullptr;  // per instance; Null for the world triangles
     uint32_t               emitter_instance = 0xFFFFFFFFu;
@@ -172,6 +176,11 @@ int32_t update_instances(RenderScene& scene, float const* world_to_object,
 // interpolated at the hit and normalized.  normals == nullptr goes back to
 // the geometric normal.  Mesh instances keep the geometric normal.
 int32_t set_normals(RenderScene& scene, float const* normals, uint32_t num_vertices);
+
+// The same for mesh `mesh` of an instanced scene: its vertex normals in
+// object space, moved to world space at a hit as the geometric normal is.
+int32_t set_mesh_normals(RenderScene& scene, uint32_t mesh, float const* normals,
+                         uint32_t num_vertices);
 
 // Density grid (nx * ny * nz floats, x fastest, host memory) filling the box
 // [box_lo, box_hi]; cc_a / cc_s: absorption and scattering coefficients per

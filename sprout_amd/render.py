@@ -122,6 +122,21 @@ class InstancedScene(Scene):
         self.num_instances = len(mesh_of)
         self.num_triangles = len(idx) + sum(len(np.asarray(m[1]).reshape(-1, 3))
                                             for m in (meshes[j] for j in mesh_of))
+        self.num_vertices = len(np.asarray(positions).reshape(-1, 3))
+        self.mesh_vertices = [len(np.asarray(m[0]).reshape(-1, 3)) for m in meshes]
+
+    def set_mesh_normals(self, mesh: int, normals) -> None:
+        """mi_scene_set_mesh_normals: mesh `mesh`'s vertex shading normals in
+        object space ((n, 3)); None goes back to the geometric normal."""
+        if normals is None:
+            check(_lib.load().mi_scene_set_mesh_normals(self.handle, mesh, None, 0),
+                  "mi_scene_set_mesh_normals")
+            return
+        n = np.ascontiguousarray(normals, np.float32)
+        if n.shape != (self.mesh_vertices[mesh], 3):
+            raise ValueError(f"normals must be ({self.mesh_vertices[mesh]}, 3)")
+        check(_lib.load().mi_scene_set_mesh_normals(self.handle, mesh, n.ctypes.data, len(n)),
+              "mi_scene_set_mesh_normals")
 
     def update_instances(self, world_to_object, object_to_world) -> None:
         """mi_scene_update_instances: new matrices for every instance, in

@@ -224,3 +224,48 @@ def test_batch_scene_equals_the_baked_soup(gpu, batch):
     log(f"  pixels within 1e-3: {close:.4f}, relMSE {e:.3g}")
     assert close > 0.9
     assert e < 1e-3
+
+
+def smooth_normals(p, i):
+    """area-weighted vertex normals of a mesh (shading normals for the test)"""
+    p = np.asarray(p, np.float64).reshape(-1, 3)
+    i = np.asarray(i).reshape(-1, 3)
+    f = np.cross(p[i[:, 1]] - p[i[:, 0]], p[i[:, 2]] - p[i[:, 0]])
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, i[:, k], f)
+    return (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
+
+
+def test_mesh_normals_same_seeds_as_oracle(gpu):
+    """mi_scene_set_mesh_normals: each mesh's vertex normals interpolated in
+    object space and moved to world space; the same image as the oracle's
+    instanced path tracer with the same normals; None goes back to the
+    geometric normal, bit for bit"""
+    from sprout_amd import scenes
+
+    sc = instanced_room()
+    s = make(gpu, sc)
+    w, h, spp = 64, 48, 16
+    plain = image(gpu, s, w, h, spp, samples_per_wave=4, seed=5)
+    normals = [smooth_normals(p, i) for p, i in sc["meshes"]]
+    for k, n in enumerate(normals):
+        s.set_mesh_normals(k, n)
+    log(f"instanced room with mesh shading normals {w}x{h} {spp} spp")
+    img = image(gpu, s, w, h, spp, samples_per_wave=4, seed=5)
+    tree = light_tree(sc["positions"], sc["indices"], sc["triangle_materials"], sc["materials"])
+    inst = O.PtInstances(sc["meshes"], sc["mesh_of"], sc["world_to_object"], sc["object_to_world"],
+                         sc["instance_materials"], sc["positions"], sc["indices"],
+                         mesh_normals=normals)
+    ref = O.render(sc["positions"], sc["indices"], sc["triangle_materials"], sc["materials"], w, h,
+                   spp, scenes.ROOM_CAMERA, tree, seed=5, instances=inst) / spp
+    assert not np.array_equal(img, plain)
+    diff = np.abs(img - ref) / np.maximum(ref, 1e-3)
+    assert np.mean(diff < 1e-4) > 0.95, np.mean(diff < 1e-4)
+    assert rel_mse(img, ref) < 1e-5
+    with pytest.raises(ValueError):
+        s.set_mesh_normals(0, normals[0][:-1])
+    for k in range(len(normals)):
+        s.set_mesh_normals(k, None)
+    assert np.array_equal(image(gpu, s, w, h, spp, samples_per_wave=4, seed=5), plain)
+    s.release()
