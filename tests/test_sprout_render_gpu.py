@@ -129,6 +129,34 @@ def test_smooth_take_equals_the_oracle(gpu):
     assert np.mean((img - ref) ** 2 / (ref * ref + 1e-2)) < 1e-5
 
 
+def test_normals_in_regenerating_and_spectral_waves(gpu):
+    """the smooth take with shading normals: regenerating waves give the
+    wave-by-wave image bit for bit, and hero-wavelength transport matches
+    the oracle's spectral mode under the same seeds"""
+    import torch
+
+    from sprout_amd import take
+
+    t = take.Take(TAKES["smooth"])
+    pos, idx, tmat, mats = t.scene
+    sc = gpu.Scene(pos, idx, tmat, mats, normals=t.normals)
+    cam = gpu.perspective(t.width, t.height, **t.camera)
+    kw = dict(samples_per_wave=4, seed=3, min_bounces=t.min_bounces, max_bounces=t.max_bounces)
+    a, _ = gpu.render(sc, cam, 16, **kw)
+    b, _ = gpu.render(sc, cam, 16, regenerate=True, **kw)
+    c, _ = gpu.render(sc, cam, 16, spectral=True, **kw)
+    torch.cuda.synchronize()
+    sc.release()
+    a, b, c = (x.cpu().numpy() / 16 for x in (a, b, c))
+    assert a.mean() > 1e-3 and np.array_equal(a, b)
+    ref = O.render(pos, idx, tmat, mats, t.width, t.height, 16, t.camera,
+                   light_tree(pos, idx, tmat, mats), seed=3, min_bounces=t.min_bounces,
+                   max_bounces=t.max_bounces, normals=t.normals, spectral=True) / 16
+    diff = np.abs(c - ref) / np.maximum(np.abs(ref), 1e-3)
+    assert np.mean(diff < 1e-4) > 0.95, np.mean(diff < 1e-4)
+    assert np.mean((c - ref) ** 2 / (ref * ref + 1e-2)) < 1e-5
+
+
 def test_set_normals_arguments(gpu):
     from sprout_amd import _lib, take
 
