@@ -157,6 +157,37 @@ def test_normals_in_regenerating_and_spectral_waves(gpu):
     assert np.mean((c - ref) ** 2 / (ref * ref + 1e-2)) < 1e-5
 
 
+def test_guided_render_with_normals_converges_to_sprouts_render(gpu):
+    """the guided integrator on the smooth take (the guide trained online at
+    the first two vertices, mixture fraction 0.5) stays unbiased with shading
+    normals: it converges to the reference's render like the plain one"""
+    import torch
+
+    from sprout_amd import guide, take
+
+    with np.load(os.path.join(GOLDEN, "sprout_renders.npz"), allow_pickle=False) as z:
+        ref = z["smooth"]
+    t = take.Take(TAKES["smooth"])
+    pos, idx, tmat, mats = t.scene
+    sc = gpu.Scene(pos, idx, tmat, mats, normals=t.normals)
+    cam = gpu.perspective(t.width, t.height, **t.camera)
+    g = guide.Guide(8)
+    spp = 8192
+    acc, _ = gpu.render_guided(sc, cam, spp, g, fraction=0.5, train=True, train_steps=2,
+                               batch=16384, learning_rate=3e-3, record_depth=2, guide_depth=2,
+                               samples_per_wave=256, seed=29, min_bounces=t.min_bounces,
+                               max_bounces=t.max_bounces)
+    torch.cuda.synchronize()
+    g.release()
+    sc.release()
+    img = acc.cpu().numpy() / spp
+    mean = np.abs(img.mean(axis=(0, 1)) / ref.mean(axis=(0, 1)) - 1).max()
+    rb, ib = blocks(ref, 16), blocks(img, 16)
+    block = (np.abs(ib - rb) / np.maximum(rb, 2e-3)).max()
+    assert mean < 0.01, mean
+    assert block < 0.04, block
+
+
 def test_set_normals_arguments(gpu):
     from sprout_amd import _lib, take
 
